@@ -1,0 +1,267 @@
+// contact_sphere.hip — sphere contact force pass (contact_sphere / contact_singlemat_sphere_hooke_none,
+// src/sphere/contact_force_sphere.cu.in:44-333 and src/sphere/include/exaDEM/sphere/contact_sphere.hpp:121-297).
+//
+// One thread per particle walks its full (both-direction) neighbour list. An entry the particle owns
+// is evaluated with the particle as i; an entry owned by the partner is evaluated in the partner's
+// orientation (partner as i) and the particle takes -f and the partner-side moment, exactly as the
+// reference's symmetric functor applies them to j. Both copies of the friction history are computed
+// from identical operands and stay identical, so no thread ever writes another particle's force:
+// there are no atomics and the summation order is fixed (stream order, then walls).
+//
+// The fused variant also performs reset_force_moment + gravity_force before the contacts and
+// force_to_accel + the angular-acceleration part of combined_compute_epilog after them.
+#include "contact_law.h"
+
+namespace xdem {
+
+struct Q4f {
+  double w, x, y, z;
+};
+__device__ inline V3 qrot_d(Q4f q, V3 v) {
+  const V3 qv{q.x, q.y, q.z};
+  V3 uv = cross(qv, v);
+  V3 uuv = cross(qv, uv);
+  uv = uv * (2.0 * q.w);
+  uuv = uuv * 2.0;
+  return v + uv + uuv;
+}
+__device__ inline V3 angular_accel_d(Q4f Q, V3 mom, V3 vrot, V3 I) {
+  const Q4f Qi{Q.w, -Q.x, -Q.y, -Q.z};
+  const V3 om = qrot_d(Qi, vrot);
+  const V3 M = qrot_d(Qi, mom);
+  const V3 dom{(M.x - (I.z - I.y) * om.y * om.z) / I.x, (M.y - (I.x - I.z) * om.z * om.x) / I.y,
+               (M.z - (I.y - I.x) * om.x * om.y) / I.z};
+  return qrot_d(Q, dom);
+}
+
+struct ForceArgs {
+  int64_t n;
+  PState s;
+  NbrList nl;
+  WallList wl;
+  const Surface* walls;
+  int nw;
+  int64_t wstride;
+  Grid g;
+  ContactParams cp, cpd;
+  double dt;
+  V3 grav;
+  int gravity_on;
+  int32_t* flags;
+  int32_t step_tag;
+};
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of particles so
+// that neighbour gathers of consecutive blocks share that XCD's L2.
+__device__ inline int64_t xcd_block(int64_t b, int64_t nb) {
+  const int64_t q = nb >> 3, r = nb & 7;
+  const int64_t x = b & 7, k = b >> 3;
+  return x * q + (x < r ? x : r) + k;
+}
+
+template <bool FUSED, bool KR, bool WALLS, bool GUARD>
+__global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
+  if (GUARD) {
+    if (a.flags[1]) return;
+    if (a.flags[0]) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.flags[1] = 1;
+        a.flags[2] = a.step_tag;
+      }
+      return;
+    }
+  }
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t x = blk * blockDim.x + threadIdx.x;
+  if (x >= a.n) return;
+  const PState& s = a.s;
+  const d4 px = s.xr[x];
+  const d4 pv = s.vm[x];
+  const d4 pw = s.w[x];
+  const V3 rx{px.x, px.y, px.z};
+  const V3 vx{pv.x, pv.y, pv.z};
+  const V3 wx{pw.x, pw.y, pw.z};
+  const double mx = pv.w, Rx = px.w;
+  V3 F, M;
+  if (FUSED) {
+    F = {0.0, 0.0, 0.0};
+    M = {0.0, 0.0, 0.0};
+    if (a.gravity_on) F = {F.x + a.grav.x * mx, F.y + a.grav.y * mx, F.z + a.grav.z * mx};
+  } else {
+    F = {s.ax[x], s.ay[x], s.az[x]};
+    M = {s.mx[x], s.my[x], s.mz[x]};
+  }
+  const uint32_t cnt = a.nl.cnt[x];
+  const uint64_t b = a.nl.base[x >> 6] + (uint64_t)(x & 63);
+  for (uint32_t k = 0; k < cnt; k++) {
+    const uint64_t sl = b + (uint64_t)k * kWave;
+    const uint32_t wd = a.nl.word[sl];
+    const uint32_t y = wd & kIdxMask;
+    const bool owner = (wd & kOwnerBit) != 0;
+    const d4 py = s.xr[y];
+    V3 ri, rj;
+    double Ri, Rj;
+    if (owner) {
+      const V3 sh = shift_vec(a.g, (int)((wd >> kShiftShift) & 31u));
+      ri = rx;
+      Ri = Rx;
+      rj = {py.x + sh.x, py.y + sh.y, py.z + sh.z};
+      Rj = py.w;
+    } else {
+      ri = {py.x, py.y, py.z};
+      Ri = py.w;
+      rj = rx;
+      Rj = Rx;
+    }
+    // detection_vertex_vertex_core
+    const V3 dist = ri - rj;
+    const double dnorm = norm(dist);
+    const double dn = dnorm - (Ri + Rj);
+    if (dn < 0) {
+      const double inv = 1.0 / dnorm;
+      const V3 nrm = dist * inv;
+      const V3 cpos = ri - nrm * (Ri + 0.5 * dn);
+      const d4 qv = s.vm[y];
+      const d4 qw = s.w[y];
+      const V3 vy{qv.x, qv.y, qv.z}, wy{qw.x, qw.y, qw.z};
+      const double my = qv.w;
+      const double mi = owner ? mx : my, mj = owner ? my : mx;
+      const double meff = (mi * mj) / (mi + mj);
+      V3 ft{a.nl.hx[sl], a.nl.hy[sl], a.nl.hz[sl]};
+      V3 mint{0.0, 0.0, 0.0};
+      if (KR) mint = {a.nl.mhx[sl], a.nl.mhy[sl], a.nl.mhz[sl]};
+      const V3 f = owner ? hooke_force<KR>(dn, nrm, a.dt, a.cp, meff, ft, mint, cpos, ri, vx, wx, rj, vy, wy)
+                         : hooke_force<KR>(dn, nrm, a.dt, a.cp, meff, ft, mint, cpos, ri, vy, wy, rj, vx, wx);
+      a.nl.hx[sl] = ft.x;
+      a.nl.hy[sl] = ft.y;
+      a.nl.hz[sl] = ft.z;
+      if (KR) {
+        a.nl.mhx[sl] = mint.x;
+        a.nl.mhy[sl] = mint.y;
+        a.nl.mhz[sl] = mint.z;
+      }
+      const bool act = !(ft.x == 0 && ft.y == 0 && ft.z == 0 && mint.x == 0 && mint.y == 0 && mint.z == 0);
+      const uint32_t nwd = act ? (wd | kActiveBit) : (wd & ~kActiveBit);
+      if (nwd != wd) a.nl.word[sl] = nwd;
+      if (owner) {
+        F = F + f;
+        M = M + (cross(cpos - rx, f) + mint);
+      } else {
+        const V3 nf = -f;
+        M = M + (cross(cpos - rx, nf) + (-mint));
+        F = F + nf;
+      }
+    } else if (wd & kActiveBit) {
+      a.nl.hx[sl] = 0.0;
+      a.nl.hy[sl] = 0.0;
+      a.nl.hz[sl] = 0.0;
+      if (KR) {
+        a.nl.mhx[sl] = 0.0;
+        a.nl.mhy[sl] = 0.0;
+        a.nl.mhz[sl] = 0.0;
+      }
+      a.nl.word[sl] = wd & ~kActiveBit;
+    }
+  }
+  if (WALLS) {
+    // ContactLawDriverFunc<Surface> (contact_sphere.hpp:216-297) with Surface::detector (surface.hpp:337-350)
+    const uint32_t mask = a.wl.mask[x];
+    for (int w = 0; w < a.nw; w++) {
+      if (!((mask >> w) & 1u)) continue;
+      const Surface& sf = a.walls[w];
+      const int64_t j = (int64_t)w * a.wstride + x;
+      const V3 proj = dot(rx, sf.normal) * sf.normal;
+      const V3 s2p = -(sf.center_proj - proj);
+      const double d = norm(s2p);
+      const double dn = d - Rx;
+      if (dn > 0) {
+        a.wl.hx[j] = 0.0; a.wl.hy[j] = 0.0; a.wl.hz[j] = 0.0;
+        a.wl.mhx[j] = 0.0; a.wl.mhy[j] = 0.0; a.wl.mhz[j] = 0.0;
+        continue;
+      }
+      const V3 nrm = s2p / d;
+      const V3 cpos = rx - nrm * (Rx + 0.5 * dn);
+      V3 ft{a.wl.hx[j], a.wl.hy[j], a.wl.hz[j]};
+      V3 mint{0.0, 0.0, 0.0};
+      if (KR) mint = {a.wl.mhx[j], a.wl.mhy[j], a.wl.mhz[j]};
+      const V3 f = hooke_force<KR>(dn, nrm, a.dt, a.cpd, mx, ft, mint, cpos, rx, vx, wx, sf.center, sf.vel, sf.vrot);
+      a.wl.hx[j] = ft.x; a.wl.hy[j] = ft.y; a.wl.hz[j] = ft.z;
+      if (KR) {
+        a.wl.mhx[j] = mint.x; a.wl.mhy[j] = mint.y; a.wl.mhz[j] = mint.z;
+      }
+      F = F + f;
+      M = M + (cross(cpos - rx, f) + mint);
+    }
+  }
+  if (FUSED) {
+    s.ax[x] = F.x / mx;
+    s.ay[x] = F.y / mx;
+    s.az[x] = F.z / mx;
+    s.mx[x] = M.x;
+    s.my[x] = M.y;
+    s.mz[x] = M.z;
+    const d4 q = s.q[x];
+    const V3 al = angular_accel_d(Q4f{q.x, q.y, q.z, q.w}, M, wx, V3{s.ix[x], s.iy[x], s.iz[x]});
+    s.alx[x] = al.x;
+    s.aly[x] = al.y;
+    s.alz[x] = al.z;
+  } else {
+    s.ax[x] = F.x;
+    s.ay[x] = F.y;
+    s.az[x] = F.z;
+    s.mx[x] = M.x;
+    s.my[x] = M.y;
+    s.mz[x] = M.z;
+  }
+}
+
+template <bool FUSED, bool KR, bool WALLS>
+static void launch_g(xdem_ctx* c, const ForceArgs& a, bool guarded) {
+  const dim3 grid(grid_blocks(a.n, 256)), block(256);
+  if (guarded) hipLaunchKernelGGL((k_force_fp64<FUSED, KR, WALLS, true>), grid, block, 0, c->stream, a);
+  else hipLaunchKernelGGL((k_force_fp64<FUSED, KR, WALLS, false>), grid, block, 0, c->stream, a);
+}
+
+template <bool FUSED>
+static void launch_f(xdem_ctx* c, const ForceArgs& a, bool kr, bool walls, bool guarded) {
+  if (kr) {
+    if (walls) launch_g<FUSED, true, true>(c, a, guarded);
+    else launch_g<FUSED, true, false>(c, a, guarded);
+  } else {
+    if (walls) launch_g<FUSED, false, true>(c, a, guarded);
+    else launch_g<FUSED, false, false>(c, a, guarded);
+  }
+}
+
+int launch_force_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool fused,
+                      bool gravity_on, V3 g, bool guarded, int32_t step_tag);
+
+int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, int form, bool fused,
+                 bool gravity_on, V3 g, bool guarded, int32_t step_tag) {
+  if (!c->hist_ready) return set_err(c, XDEM_ERR_STATE, "contact: run nbh_sphere after chunk_neighbors_contact");
+  if (form == XDEM_FORM_FP32) return launch_force_fp32(c, cp, cpd, dt, fused, gravity_on, g, guarded, step_tag);
+  if (cp.kr != 0.0 || cpd.kr != 0.0) c->kr_used = true;
+  const bool walls = c->nwalls > 0 && c->wl_valid;
+  ForceArgs a{};
+  a.n = c->n;
+  a.s = live(c);
+  a.nl = c->nl[c->nl_cur];
+  if (walls) a.wl = c->wl[c->wl_cur];
+  a.walls = c->d_walls;
+  a.nw = c->nwalls;
+  a.wstride = c->cap;
+  a.g = c->grid;
+  a.cp = cp;
+  a.cpd = cpd;
+  a.dt = dt;
+  a.grav = g;
+  a.gravity_on = gravity_on ? 1 : 0;
+  a.flags = c->d_flags;
+  a.step_tag = step_tag;
+  if (fused) launch_f<true>(c, a, c->kr_used, walls, guarded);
+  else launch_f<false>(c, a, c->kr_used, walls, guarded);
+  XDEM_HIP(c, hipGetLastError());
+  return XDEM_OK;
+}
+
+}  // namespace xdem

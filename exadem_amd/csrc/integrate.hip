@@ -1,0 +1,217 @@
+// integrate.hip — per-particle velocity-Verlet pieces (HBM-streaming kernels).
+//
+//   prolog  combined_compute_prolog (src/numericalscheme/combined_compute_prolog.cu:35-47,94-116):
+//           r += v dt + a dt^2/2 ; v += a dt/2 ; Q <- normalize(Q + dot(Q,w) dt) ; w += alpha dt/2
+//   epilog  combined_compute_epilog (src/numericalscheme/combined_compute_epilog.cu:35-46,94-113):
+//           alpha = Q (I^-1 (Q* mom - w_b x (I w_b))) ; w += alpha dt/2 ; v += a dt/2
+//   kick    the two epilog updates alone, when alpha was already produced by the fused force pass
+// The fused engine runs "kick(n-1) + prolog(n) + displacement test" as one kernel: the
+// per-particle operation sequence is the same as running the operators one after the other.
+#include "xdem_internal.h"
+
+namespace xdem {
+
+struct Q4 {
+  double w, x, y, z;
+};
+
+// quat::dot (src/shape/include/exaDEM/type/quat.hpp:92-96): 0.5*(0,w)*q
+__device__ inline Q4 qdot(Q4 q, V3 w) {
+  return {-0.5 * (w.x * q.x + w.y * q.y + w.z * q.z), 0.5 * (w.y * q.z - w.z * q.y + w.x * q.w),
+          0.5 * (w.z * q.x - w.x * q.z + w.y * q.w), 0.5 * (w.x * q.y - w.y * q.x + w.z * q.w)};
+}
+__device__ inline Q4 qnormalize(Q4 q) {
+  const double n = sqrt(q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z);
+  return {q.w / n, q.x / n, q.y / n, q.z / n};
+}
+// quat * vec3 (src/shape/include/exaDEM/type/quat.hpp:84-91)
+__device__ inline V3 qrot(Q4 q, V3 v) {
+  const V3 qv{q.x, q.y, q.z};
+  V3 uv = cross(qv, v);
+  V3 uuv = cross(qv, uv);
+  uv = uv * (2.0 * q.w);
+  uuv = uuv * 2.0;
+  return v + uv + uuv;
+}
+
+// PushToAngularAccelerationFunctor (src/numericalscheme/include/exaDEM/angular_acceleration.hpp:28-42)
+__device__ inline V3 angular_accel(Q4 Q, V3 mom, V3 vrot, V3 I) {
+  const Q4 Qi{Q.w, -Q.x, -Q.y, -Q.z};
+  const V3 om = qrot(Qi, vrot);
+  const V3 M = qrot(Qi, mom);
+  const V3 dom{(M.x - (I.z - I.y) * om.y * om.z) / I.x, (M.y - (I.x - I.z) * om.z * om.x) / I.y,
+               (M.z - (I.y - I.x) * om.x * om.y) / I.z};
+  return qrot(Q, dom);
+}
+
+template <bool KICK, bool DISPL, bool GUARD>
+__global__ __launch_bounds__(256) void k_prolog(int64_t n, PState s, double dt, double kdt_2, double thr2,
+                                                int32_t* __restrict__ flags) {
+  if (GUARD && flags[1]) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double dt_2 = dt * 0.5;
+  const double dt2_2 = dt * dt * 0.5;
+  d4 xr = s.xr[i];
+  d4 vm = s.vm[i];
+  d4 w = s.w[i];
+  const d4 qq = s.q[i];
+  const double ax = s.ax[i], ay = s.ay[i], az = s.az[i];
+  const double lx = s.alx[i], ly = s.aly[i], lz = s.alz[i];
+  if (KICK) {
+    // epilog of the previous step: angular velocity, then push_f_v
+    w.x = w.x + lx * kdt_2;
+    w.y = w.y + ly * kdt_2;
+    w.z = w.z + lz * kdt_2;
+    vm.x = vm.x + ax * kdt_2;
+    vm.y = vm.y + ay * kdt_2;
+    vm.z = vm.z + az * kdt_2;
+  }
+  // push_f_v_r, push_f_v
+  xr.x = xr.x + (vm.x * dt + ax * dt2_2);
+  xr.y = xr.y + (vm.y * dt + ay * dt2_2);
+  xr.z = xr.z + (vm.z * dt + az * dt2_2);
+  vm.x = vm.x + ax * dt_2;
+  vm.y = vm.y + ay * dt_2;
+  vm.z = vm.z + az * dt_2;
+  // push_to_quaternion
+  Q4 q{qq.x, qq.y, qq.z, qq.w};
+  const Q4 d = qdot(q, V3{w.x, w.y, w.z});
+  q = {q.w + d.w * dt, q.x + d.x * dt, q.y + d.y * dt, q.z + d.z * dt};
+  q = qnormalize(q);
+  w.x = w.x + dt_2 * lx;
+  w.y = w.y + dt_2 * ly;
+  w.z = w.z + dt_2 * lz;
+  s.xr[i] = xr;
+  s.vm[i] = vm;
+  s.w[i] = w;
+  s.q[i] = {q.w, q.x, q.y, q.z};
+  if (DISPL) {
+    const double dx = xr.x - s.bx[i], dy = xr.y - s.by[i], dz = xr.z - s.bz[i];
+    if (dx * dx + dy * dy + dz * dz >= thr2) flags[0] = 1;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_epilog(int64_t n, PState s, double dt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double dt_2 = dt * 0.5;
+  const d4 qq = s.q[i];
+  d4 w = s.w[i];
+  d4 vm = s.vm[i];
+  const V3 al = angular_accel(Q4{qq.x, qq.y, qq.z, qq.w}, V3{s.mx[i], s.my[i], s.mz[i]}, V3{w.x, w.y, w.z},
+                              V3{s.ix[i], s.iy[i], s.iz[i]});
+  s.alx[i] = al.x;
+  s.aly[i] = al.y;
+  s.alz[i] = al.z;
+  w.x = w.x + al.x * dt_2;
+  w.y = w.y + al.y * dt_2;
+  w.z = w.z + al.z * dt_2;
+  vm.x = vm.x + s.ax[i] * dt_2;
+  vm.y = vm.y + s.ay[i] * dt_2;
+  vm.z = vm.z + s.az[i] * dt_2;
+  s.w[i] = w;
+  s.vm[i] = vm;
+}
+
+__global__ __launch_bounds__(256) void k_kick(int64_t n, PState s, double dt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double dt_2 = dt * 0.5;
+  d4 w = s.w[i];
+  d4 vm = s.vm[i];
+  w.x = w.x + s.alx[i] * dt_2;
+  w.y = w.y + s.aly[i] * dt_2;
+  w.z = w.z + s.alz[i] * dt_2;
+  vm.x = vm.x + s.ax[i] * dt_2;
+  vm.y = vm.y + s.ay[i] * dt_2;
+  vm.z = vm.z + s.az[i] * dt_2;
+  s.w[i] = w;
+  s.vm[i] = vm;
+}
+
+__global__ __launch_bounds__(256) void k_reset(int64_t n, PState s) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  s.ax[i] = 0.0; s.ay[i] = 0.0; s.az[i] = 0.0;
+  s.mx[i] = 0.0; s.my[i] = 0.0; s.mz[i] = 0.0;
+}
+
+// GravityForceFunctor (src/forcefield/include/exaDEM/forcefield/gravity_force.hpp:38-46)
+__global__ __launch_bounds__(256) void k_gravity(int64_t n, PState s, double gx, double gy, double gz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double m = s.vm[i].w;
+  s.ax[i] = s.ax[i] + gx * m;
+  s.ay[i] = s.ay[i] + gy * m;
+  s.az[i] = s.az[i] + gz * m;
+}
+
+// ForceToAccelFunctor (src/numericalscheme/include/exaDEM/force_to_accel.hpp:26-33)
+__global__ __launch_bounds__(256) void k_to_accel(int64_t n, PState s) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double m = s.vm[i].w;
+  s.ax[i] = s.ax[i] / m;
+  s.ay[i] = s.ay[i] / m;
+  s.az[i] = s.az[i] / m;
+}
+
+__global__ __launch_bounds__(256) void k_displ(int64_t n, PState s, double thr2, int32_t* __restrict__ flags) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const d4 xr = s.xr[i];
+  const double dx = xr.x - s.bx[i], dy = xr.y - s.by[i], dz = xr.z - s.bz[i];
+  if (dx * dx + dy * dy + dz * dz >= thr2) flags[0] = 1;
+}
+
+#define LAUNCH1(kern, ...)                                                                          \
+  do {                                                                                              \
+    hipLaunchKernelGGL(kern, dim3(grid_blocks(c->n, 256)), dim3(256), 0, c->stream, __VA_ARGS__); \
+    XDEM_HIP(c, hipGetLastError());                                                                 \
+  } while (0)
+
+int launch_prolog(xdem_ctx* c, double dt, bool with_kick, double kick_dt, bool displ, double thr2, bool guarded) {
+  PState s = live(c);
+  const double kdt_2 = kick_dt * 0.5;
+  const int64_t n = c->n;
+  if (with_kick) {
+    if (displ && guarded) LAUNCH1((k_prolog<true, true, true>), n, s, dt, kdt_2, thr2, c->d_flags);
+    else if (displ) LAUNCH1((k_prolog<true, true, false>), n, s, dt, kdt_2, thr2, c->d_flags);
+    else LAUNCH1((k_prolog<true, false, false>), n, s, dt, kdt_2, thr2, c->d_flags);
+  } else {
+    if (displ && guarded) LAUNCH1((k_prolog<false, true, true>), n, s, dt, kdt_2, thr2, c->d_flags);
+    else if (displ) LAUNCH1((k_prolog<false, true, false>), n, s, dt, kdt_2, thr2, c->d_flags);
+    else LAUNCH1((k_prolog<false, false, false>), n, s, dt, kdt_2, thr2, c->d_flags);
+  }
+  c->cells_valid = false;  // positions moved: the binning no longer matches them
+  return XDEM_OK;
+}
+
+int launch_epilog(xdem_ctx* c, double dt) {
+  LAUNCH1(k_epilog, c->n, live(c), dt);
+  return XDEM_OK;
+}
+int launch_kick(xdem_ctx* c, double dt) {
+  if (c->n == 0) return XDEM_OK;
+  LAUNCH1(k_kick, c->n, live(c), dt);
+  return XDEM_OK;
+}
+int launch_reset(xdem_ctx* c) {
+  LAUNCH1(k_reset, c->n, live(c));
+  return XDEM_OK;
+}
+int launch_gravity(xdem_ctx* c, V3 g) {
+  LAUNCH1(k_gravity, c->n, live(c), g.x, g.y, g.z);
+  return XDEM_OK;
+}
+int launch_to_accel(xdem_ctx* c) {
+  LAUNCH1(k_to_accel, c->n, live(c));
+  return XDEM_OK;
+}
+int launch_displ(xdem_ctx* c, double thr2) {
+  LAUNCH1(k_displ, c->n, live(c), thr2, c->d_flags);
+  return XDEM_OK;
+}
+
+}  // namespace xdem

@@ -1,0 +1,220 @@
+// xdem_internal.h — device data layout, math helpers and the context of libxdem_hip.so.
+//
+// Layout in HBM (all arrays in cell-sorted order, rebuilt at every neighbour rebuild):
+//   xr[i]    = (rx, ry, rz, radius)        gathered by neighbours  -> packed 32 B
+//   vm[i]    = (vx, vy, vz, mass)          gathered by neighbours  -> packed 32 B
+//   w[i]     = (vrotx, vroty, vrotz, 0)    gathered by neighbours  -> packed 32 B
+//   q[i]     = (qw, qx, qy, qz)            own particle only
+//   a?/mom?/alpha?/in?/bk?  3 separate double arrays each (own particle only, coalesced)
+// Neighbour list: per wavefront of 64 particles, an ELL block of K_w x 64 words
+// (slot = base[w] + k*64 + lane), one 32-bit word per entry:
+//   bits 0..24 partner index, 25..29 periodic image code (13 = none), 30 history-nonzero, 31 owner.
+// Friction history lives in the same slots (3 double arrays), one copy per direction; both copies
+// are computed in the owner's orientation and stay bit-identical.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/xdem.h"
+
+namespace xdem {
+
+constexpr uint32_t kIdxBits = 25;
+constexpr uint32_t kIdxMask = (1u << kIdxBits) - 1u;
+constexpr uint32_t kShiftShift = 25;
+constexpr uint32_t kActiveBit = 1u << 30;
+constexpr uint32_t kOwnerBit = 1u << 31;
+constexpr int kNoShift = 13;
+constexpr int kWave = 64;
+
+struct alignas(32) d4 {
+  double x, y, z, w;
+};
+
+struct V3 {
+  double x, y, z;
+};
+
+__host__ __device__ inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__host__ __device__ inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ inline V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+__host__ __device__ inline V3 operator*(V3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+__host__ __device__ inline V3 operator*(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__host__ __device__ inline V3 operator/(V3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+__host__ __device__ inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline V3 cross(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__host__ __device__ inline double norm(V3 a) { return sqrt(dot(a, a)); }
+
+struct ContactParams {
+  double kn, kt, kr, mu, damp_rate, fc, dncut, gamma;
+};
+
+struct Surface {
+  V3 normal, center, vel, vrot, center_proj;
+  double offset;
+};
+
+// Domain and grid as seen by kernels.
+struct Grid {
+  double lo[3];
+  double L[3];
+  double h[3];
+  double inv_h[3];
+  int dims[3];
+  int per[3];
+  int64_t ncells;
+};
+
+__host__ __device__ inline V3 shift_vec(const Grid& g, int code) {
+  const int sx = code % 3 - 1, sy = (code / 3) % 3 - 1, sz = code / 9 - 1;
+  return {sx * g.L[0], sy * g.L[1], sz * g.L[2]};
+}
+
+// Device particle state (one buffer set; two sets are kept and swapped at each reorder).
+struct PState {
+  d4* xr;
+  d4* vm;
+  d4* w;
+  d4* q;
+  double *ax, *ay, *az;
+  double *mx, *my, *mz;
+  double *alx, *aly, *alz;
+  double *ix, *iy, *iz;
+  double *bx, *by, *bz;
+  double* hom;
+  uint64_t* id;
+  uint32_t* type;
+  uint32_t* group;
+};
+
+// Neighbour list + per-slot history.
+struct NbrList {
+  uint32_t* word = nullptr;   // [slots]
+  double* hx = nullptr;       // friction history [slots]
+  double* hy = nullptr;
+  double* hz = nullptr;
+  double* mhx = nullptr;      // rolling-moment history [slots] (allocated when kr != 0 is used)
+  double* mhy = nullptr;
+  double* mhz = nullptr;
+  uint32_t* cnt = nullptr;    // entries per particle [n]
+  uint32_t* kmax = nullptr;   // per wave max count [nwaves]
+  uint64_t* base = nullptr;   // per wave first slot [nwaves+1]
+  int64_t slots = 0;
+  int64_t cap_slots = 0;
+  int64_t pairs = 0;          // half-list pairs (owner entries)
+};
+
+// Wall (Surface driver) interactions: candidate mask per particle, history per (wall, particle).
+struct WallList {
+  uint8_t* mask = nullptr;  // [n] bit w set if wall w is a candidate
+  double* hx = nullptr;     // [nw*n]
+  double* hy = nullptr;
+  double* hz = nullptr;
+  double* mhx = nullptr;
+  double* mhy = nullptr;
+  double* mhz = nullptr;
+};
+
+struct KernelTimer {
+  std::vector<hipEvent_t> ev;  // pairs (start, stop)
+  int64_t count = 0;
+  double total_ms = 0;
+};
+
+}  // namespace xdem
+
+struct xdem_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  int64_t n = 0;
+  int64_t cap = 0;
+  xdem::PState ps[2] = {};
+  int cur = 0;  // index of the live particle buffer set
+  bool have_domain = false;
+  xdem::Grid grid{};
+  int nwalls = 0;
+  xdem::Surface walls[XDEM_MAX_SURFACES];
+  xdem::Surface* d_walls = nullptr;
+
+  // binning scratch
+  uint32_t* cell_of = nullptr;      // [cap]
+  uint32_t* cell_count = nullptr;   // [ncells]
+  uint32_t* cell_start = nullptr;   // [ncells+1]
+  uint32_t* perm = nullptr;         // [cap] new index -> old index (last reorder)
+  uint64_t* sort_key = nullptr;     // [cap]
+  int64_t cap_cells = 0;
+  void* scan_tmp = nullptr;
+  int64_t scan_tmp_bytes = 0;
+
+  // neighbour lists (two: current and previous for history carry-over)
+  xdem::NbrList nl[2];
+  int nl_cur = 0;
+  bool nl_valid = false;       // a list exists for the current particle order
+  bool nl_prev_valid = false;  // previous list is aligned with perm
+  bool perm_valid = false;
+  bool cells_valid = false;     // cell_start matches the current positions and order
+  bool hist_ready = false;      // history of the current list has been carried over (nbh_sphere)
+  bool wl_valid = false;        // wall list exists (for the order it was built in)
+  bool kr_used = false;         // a rolling stiffness was used: moment histories are live
+  double rcut_inc = 0;
+  double nbh_dist = 0;
+  xdem::WallList wl[2];
+  int wl_cur = 0;
+  int wl_nw = 0;  // wall count used to size wl
+
+  // rebuild trigger / stall flags on device: [0] trigger, [1] stall, [2] stall step (low), [3] pad
+  int32_t* d_flags = nullptr;
+  int32_t* h_flags = nullptr;  // pinned
+  uint32_t* d_u32tmp = nullptr;
+  uint32_t* h_u32tmp = nullptr;  // pinned
+
+  // fused-engine state: whether the epilog kicks of the previous force pass are pending
+  bool pending_kick = false;
+  double pending_dt = 0;
+
+  bool timing = false;
+  xdem::KernelTimer timers[2];
+};
+
+// helpers shared between translation units
+namespace xdem {
+int set_err(xdem_ctx* c, int code, const std::string& msg);
+int hip_check(xdem_ctx* c, hipError_t e, const char* what);
+#define XDEM_HIP(c, call)                               \
+  do {                                                  \
+    hipError_t e_ = (call);                             \
+    if (e_ != hipSuccess) return xdem::hip_check(c, e_, #call); \
+  } while (0)
+
+inline PState& live(xdem_ctx* c) { return c->ps[c->cur]; }
+int ensure_capacity(xdem_ctx* c, int64_t n);
+int ensure_cells(xdem_ctx* c, int64_t ncells);
+int scan_u32(xdem_ctx* c, const uint32_t* in, uint32_t* out, int64_t n);  // exclusive, out[n] = total
+int scan_u64_from_u32(xdem_ctx* c, const uint32_t* in, uint64_t* out, int64_t n, uint32_t mul);
+int read_u64(xdem_ctx* c, const uint64_t* d, uint64_t* h);
+
+inline int grid_blocks(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// kernels launched from api code
+int launch_prolog(xdem_ctx* c, double dt, bool with_kick, double kick_dt, bool displ, double thr2, bool guarded);
+int launch_epilog(xdem_ctx* c, double dt);
+int launch_kick(xdem_ctx* c, double dt);
+int launch_reset(xdem_ctx* c);
+int launch_gravity(xdem_ctx* c, V3 g);
+int launch_to_accel(xdem_ctx* c);
+int launch_displ(xdem_ctx* c, double thr2);
+int do_move_particles(xdem_ctx* c);
+int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist);
+int do_nbh_sphere(xdem_ctx* c, double rcut_inc);
+int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, int form, bool fused,
+                 bool gravity_on, V3 g, bool guarded, int32_t step_tag);
+int alloc_walls(xdem_ctx* c, int nw);
+}  // namespace xdem

@@ -1,0 +1,157 @@
+"""Seeded synthetic inputs for the sphere workloads.
+
+Particle fields follow set_fields (src/field/set_fields.cu:323-327): m = rho * 4/3 pi r^3 and an
+isotropic inertia 0.4 m r^2. Random draws come from the counter-based SplitMix64 streams of
+libxdem_host.so, so a workload is a pure function of its seeds.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+from .native import LIBDIR, domain
+
+_host = None
+
+
+def host():
+    global _host
+    if _host is None:
+        path = os.path.join(LIBDIR, "libxdem_host.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: build it with exadem_amd/build.py")
+        lib = C.CDLL(path)
+        dp = C.POINTER(C.c_double)
+        lib.xdem_gen_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, C.c_double, C.c_double, dp]
+        lib.xdem_gen_normal.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, C.c_double, C.c_double, dp]
+        lib.xdem_gen_pack_periodic.argtypes = [C.c_uint64, C.c_int64, dp, dp, dp, C.c_double, C.c_int64,
+                                               C.c_double, dp, dp, dp, dp]
+        lib.xdem_gen_pack_periodic.restype = C.c_int64
+        _host = lib
+    return _host
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def uniform(seed, tag, n, a, b):
+    out = np.empty(n, np.float64)
+    host().xdem_gen_uniform(seed, tag, n, a, b, _p(out))
+    return out
+
+
+def normal(seed, tag, n, mean, sigma):
+    out = np.empty(n, np.float64)
+    host().xdem_gen_normal(seed, tag, n, mean, sigma, _p(out))
+    return out
+
+
+def sphere_set(rx, ry, rz, radius, density, vx=None, vy=None, vz=None):
+    """Full particle field dictionary (input order, ids 0..n-1) for spheres."""
+    n = rx.shape[0]
+    pi = 4 * math.atan(1)
+    r = radius
+    V = ((4.0) / (3.0)) * pi * r * r * r
+    m = V * density
+    inertia = 0.4 * m * r * r
+    z = np.zeros(n)
+    return {
+        "id": np.arange(n, dtype=np.uint64), "type": np.zeros(n, np.uint32), "group": np.zeros(n, np.uint32),
+        "rx": np.ascontiguousarray(rx, np.float64), "ry": np.ascontiguousarray(ry, np.float64),
+        "rz": np.ascontiguousarray(rz, np.float64),
+        "vx": z.copy() if vx is None else np.ascontiguousarray(vx), "vy": z.copy() if vy is None else np.ascontiguousarray(vy),
+        "vz": z.copy() if vz is None else np.ascontiguousarray(vz),
+        "fx": z.copy(), "fy": z.copy(), "fz": z.copy(), "mass": m, "radius": np.ascontiguousarray(r, np.float64),
+        "homothety": np.ones(n), "qw": np.ones(n), "qx": z.copy(), "qy": z.copy(), "qz": z.copy(),
+        "momx": z.copy(), "momy": z.copy(), "momz": z.copy(), "vrotx": z.copy(), "vroty": z.copy(),
+        "vrotz": z.copy(), "arotx": z.copy(), "aroty": z.copy(), "arotz": z.copy(),
+        "inx": inertia.copy(), "iny": inertia.copy(), "inz": inertia.copy(),
+    }
+
+
+class Workload:
+    """Particles + domain + walls + contact law + time step of one configuration."""
+
+    def __init__(self, name, particles, dom, walls, cp, cpd, dt, steps, rcut_inc, gravity, meta=None):
+        self.name = name
+        self.particles = particles
+        self.domain = dom
+        self.walls = walls
+        self.cp = cp
+        self.cpd = cpd
+        self.dt = dt
+        self.steps = steps
+        self.rcut_inc = rcut_inc
+        self.gravity = gravity
+        self.meta = meta or {}
+
+    @property
+    def n(self):
+        return int(self.particles["rx"].shape[0])
+
+
+def box_walls(hi):
+    """Six stationary Surface walls bounding [0,hi] (register_surface with normal/offset)."""
+    walls = []
+    for a in range(3):
+        nrm = [0.0, 0.0, 0.0]
+        nrm[a] = 1.0
+        walls.append({"offset": 0.0, "normal": tuple(nrm)})
+        walls.append({"offset": float(hi[a]), "normal": tuple(nrm)})
+    return walls
+
+
+def deposition(n=10000, spacing=1.2, jitter=0.05, seed=1, radius=0.5, density=2500.0, box=(40.0, 40.0, 80.0),
+               start=1.5, dt=1e-5, steps=1000, rcut_inc=0.1):
+    """Plain FP64 sphere deposition: jittered simple-cubic lattice in a walled box under gravity.
+
+    Sites at start + k*spacing on x and y while <= box - start, layers along z; the first n sites in
+    x-fastest order are kept. spacing < 2*radius gives a contact-rich variant (initial overlaps)."""
+    from .native import contact_params
+    nx = int(math.floor((box[0] - 2 * start) / spacing + 1e-9)) + 1
+    ny = int(math.floor((box[1] - 2 * start) / spacing + 1e-9)) + 1
+    nz = int(math.ceil(n / (nx * ny)))
+    k = np.arange(nx * ny * nz)[:n]
+    ix, iy, iz = k % nx, (k // nx) % ny, k // (nx * ny)
+    jit = uniform(seed, 1, 3 * n, -jitter, jitter).reshape(3, n)
+    rx = start + ix * spacing + jit[0]
+    ry = start + iy * spacing + jit[1]
+    rz = start + iz * spacing + jit[2]
+    p = sphere_set(rx, ry, rz, np.full(n, radius), density)
+    dom = domain((0.0, 0.0, 0.0), box, (False, False, False), 2 * radius + rcut_inc)
+    cp = contact_params(kn=1e5, mu=0.3, damp_rate=0.9)
+    return Workload("deposition", p, dom, box_walls(box), cp, cp, dt, steps, rcut_inc, (0.0, 0.0, -9.81),
+                    {"spacing": spacing, "jitter": jitter, "seed": seed})
+
+
+def polydisperse(n=1_000_000, phi=0.55, rmin=0.5, rmax=1.0, seed_radius=2, seed_vel=3, seed_pos=2, sigma_v=0.1,
+                 box_xy=200.0, density=2500.0, dt=5e-6, steps=2000, rcut_inc=0.1, gravity=(0.0, 0.0, -9.81),
+                 relax_sweeps=400, tol=None):
+    """Polydisperse spheres in a periodic box at solid fraction phi.
+
+    Radii U[rmin, rmax] (seed_radius). At n = 1e6 the box is 200 x 200 x Lz with Lz fixed by phi;
+    smaller n scale all three edges by (n/1e6)^(1/3). Positions: RSA at radius scale 0.8, then
+    full radii and Jacobi overlap relaxation (max_overlap reported in meta). Velocities N(0, sigma_v)."""
+    from .native import contact_params
+    r = uniform(seed_radius, 1, n, rmin, rmax)
+    vol = float(np.sum((4.0 / 3.0) * math.pi * r ** 3))
+    s = (n / 1e6) ** (1.0 / 3.0)
+    bx = by = box_xy * s
+    bz = vol / (phi * bx * by)
+    lo = np.zeros(3)
+    hi = np.array([bx, by, bz])
+    x = np.empty(n)
+    y = np.empty(n)
+    z = np.empty(n)
+    mo = C.c_double()
+    tol = 1e-3 * rmin if tol is None else tol
+    sweeps = host().xdem_gen_pack_periodic(seed_pos, n, _p(r), _p(lo), _p(hi), 0.8, relax_sweeps, tol, _p(x), _p(y),
+                                           _p(z), C.byref(mo))
+    v = normal(seed_vel, 2, 3 * n, 0.0, sigma_v).reshape(3, n)
+    p = sphere_set(x, y, z, r, density, v[0], v[1], v[2])
+    dom = domain((0.0, 0.0, 0.0), tuple(hi), (True, True, True), 2 * rmax + rcut_inc)
+    cp = contact_params(kn=1e5, mu=0.3, damp_rate=0.9)
+    return Workload("polydisperse", p, dom, [], cp, cp, dt, steps, rcut_inc, gravity,
+                    {"phi": phi, "box": tuple(hi), "relax_sweeps": int(sweeps), "max_overlap": mo.value})

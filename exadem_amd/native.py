@@ -1,0 +1,326 @@
+"""ctypes bindings of the C ABI declared in include/xdem.h.
+
+The HIP library (exadem_amd/lib/libxdem_hip.so) is the only compute path: when it is missing
+or cannot be loaded, every entry point raises instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+
+XDEM_OK = 0
+XDEM_FORM_FP64 = 0
+XDEM_FORM_FP32 = 1
+ERR_NAMES = {-1: "XDEM_ERR_ARG", -2: "XDEM_ERR_HIP", -3: "XDEM_ERR_STATE", -4: "XDEM_ERR_CAPACITY",
+             -5: "XDEM_ERR_UNSUPPORTED"}
+
+_dp = C.POINTER(C.c_double)
+_u64p = C.POINTER(C.c_uint64)
+_u32p = C.POINTER(C.c_uint32)
+
+
+class Particles(C.Structure):
+    _fields_ = [("n", C.c_int64), ("id", _u64p), ("type", _u32p), ("group", _u32p)] + [
+        (name, _dp) for name in (
+            "rx", "ry", "rz", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "radius", "homothety",
+            "qw", "qx", "qy", "qz", "momx", "momy", "momz", "vrotx", "vroty", "vrotz",
+            "arotx", "aroty", "arotz", "inx", "iny", "inz")]
+
+
+class Domain(C.Structure):
+    _fields_ = [("lo", C.c_double * 3), ("hi", C.c_double * 3), ("periodic", C.c_int32 * 3),
+                ("cell_size", C.c_double)]
+
+
+class ContactParams(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("kn", "kt", "kr", "mu", "damp_rate", "fc", "dncut", "gamma")]
+
+
+class Surface(C.Structure):
+    _fields_ = [("offset", C.c_double), ("normal", C.c_double * 3), ("center", C.c_double * 3),
+                ("vel", C.c_double * 3), ("vrot", C.c_double * 3)]
+
+
+class Scheme(C.Structure):
+    _fields_ = [("dt", C.c_double), ("gravity", C.c_double * 3), ("gravity_on", C.c_int32),
+                ("form", C.c_int32), ("cp", ContactParams), ("cpd", ContactParams),
+                ("rcut_inc", C.c_double), ("poll", C.c_int32)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("rebuilds", C.c_int64), ("polls", C.c_int64)]
+
+
+# field name -> (numpy dtype) for the xdem_particles members
+FIELDS = ["id", "type", "group", "rx", "ry", "rz", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "radius",
+          "homothety", "qw", "qx", "qy", "qz", "momx", "momy", "momz", "vrotx", "vroty", "vrotz",
+          "arotx", "aroty", "arotz", "inx", "iny", "inz"]
+DTYPES = {"id": np.uint64, "type": np.uint32, "group": np.uint32}
+
+# exported symbols of libxdem_hip.so, one per declaration in include/xdem.h
+HIP_SYMBOLS = [
+    "xdem_ctx_create", "xdem_ctx_destroy", "xdem_last_error", "xdem_synchronize", "xdem_set_domain",
+    "xdem_set_surfaces", "xdem_particles_upload", "xdem_particles_download", "xdem_num_particles",
+    "xdem_combined_compute_prolog", "xdem_combined_compute_epilog", "xdem_reset_force_moment",
+    "xdem_gravity_force", "xdem_force_to_accel", "xdem_particle_displ_over", "xdem_move_particles",
+    "xdem_chunk_neighbors_contact", "xdem_nbh_sphere", "xdem_classify_interactions", "xdem_contact_sphere",
+    "xdem_run_steps", "xdem_interactions_count", "xdem_interactions_dump", "xdem_nbh_stats",
+    "xdem_kernel_timing", "xdem_set_timing", "xdem_streams_dump",
+]
+
+
+class XdemError(RuntimeError):
+    pass
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def particles_struct(arrays, n):
+    """Builds an xdem_particles view over a dict of contiguous numpy arrays (missing -> NULL)."""
+    p = Particles()
+    p.n = n
+    keep = []
+    for name in FIELDS:
+        a = arrays.get(name)
+        if a is None:
+            continue
+        dt = DTYPES.get(name, np.float64)
+        assert a.dtype == dt and a.flags.c_contiguous and a.shape[0] >= n, name
+        keep.append(a)
+        ct = {np.uint64: C.c_uint64, np.uint32: C.c_uint32}.get(dt, C.c_double)
+        setattr(p, name, _ptr(a, ct))
+    return p, keep
+
+
+_lib = None
+
+
+def lib_path():
+    return os.path.join(LIBDIR, "libxdem_hip.so")
+
+
+def load():
+    """Loads libxdem_hip.so (raises XdemError when it is absent: there is no other compute path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = lib_path()
+    if not os.path.exists(path):
+        raise XdemError(f"{path} is missing: build it with exadem_amd/build.py (hipcc, gfx950)")
+    lib = C.CDLL(path, mode=C.RTLD_GLOBAL)
+    vp = C.c_void_p
+    sig = {
+        "xdem_ctx_create": (C.c_int, [C.c_int32, C.POINTER(vp)]),
+        "xdem_ctx_destroy": (None, [vp]),
+        "xdem_last_error": (C.c_int, [vp, C.c_char_p, C.c_int64]),
+        "xdem_synchronize": (C.c_int, [vp]),
+        "xdem_set_domain": (C.c_int, [vp, C.POINTER(Domain)]),
+        "xdem_set_surfaces": (C.c_int, [vp, C.POINTER(Surface), C.c_int32]),
+        "xdem_particles_upload": (C.c_int, [vp, C.POINTER(Particles)]),
+        "xdem_particles_download": (C.c_int, [vp, C.POINTER(Particles)]),
+        "xdem_num_particles": (C.c_int64, [vp]),
+        "xdem_combined_compute_prolog": (C.c_int, [vp, C.c_double]),
+        "xdem_combined_compute_epilog": (C.c_int, [vp, C.c_double]),
+        "xdem_reset_force_moment": (C.c_int, [vp]),
+        "xdem_gravity_force": (C.c_int, [vp, C.POINTER(C.c_double)]),
+        "xdem_force_to_accel": (C.c_int, [vp]),
+        "xdem_particle_displ_over": (C.c_int, [vp, C.c_double, C.POINTER(C.c_int32)]),
+        "xdem_move_particles": (C.c_int, [vp]),
+        "xdem_chunk_neighbors_contact": (C.c_int, [vp, C.c_double, C.c_double]),
+        "xdem_nbh_sphere": (C.c_int, [vp, C.c_double, C.c_int32]),
+        "xdem_classify_interactions": (C.c_int, [vp]),
+        "xdem_contact_sphere": (C.c_int, [vp, C.POINTER(ContactParams), C.POINTER(ContactParams), C.c_double,
+                                          C.c_int32]),
+        "xdem_run_steps": (C.c_int, [vp, C.POINTER(Scheme), C.c_int64, C.POINTER(RunStats)]),
+        "xdem_interactions_count": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "xdem_interactions_dump": (C.c_int, [vp, C.c_int64, _u64p, _u64p, C.POINTER(C.c_uint16), _dp,
+                                             C.POINTER(C.c_int64)]),
+        "xdem_streams_dump": (C.c_int, [vp, C.c_int64, _u64p, _u64p, C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_uint8), C.POINTER(C.c_int64)]),
+        "xdem_nbh_stats": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+        "xdem_kernel_timing": (C.c_int, [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+        "xdem_set_timing": (C.c_int, [vp, C.c_int32]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def contact_params(kn=1e5, kt=None, kr=0.0, mu=0.3, damp_rate=0.9, fc=0.0, dncut=0.0, gamma=0.0):
+    return ContactParams(kn, 0.8 * kn if kt is None else kt, kr, mu, damp_rate, fc, dncut, gamma)
+
+
+def domain(lo, hi, periodic, cell_size):
+    d = Domain()
+    for a in range(3):
+        d.lo[a] = lo[a]
+        d.hi[a] = hi[a]
+        d.periodic[a] = 1 if periodic[a] else 0
+    d.cell_size = cell_size
+    return d
+
+
+def surfaces(walls):
+    """walls: list of dicts {offset, normal[, center, vel, vrot]} -> ctypes array."""
+    arr = (Surface * max(1, len(walls)))()
+    for k, w in enumerate(walls):
+        arr[k].offset = w["offset"]
+        for a in range(3):
+            arr[k].normal[a] = w["normal"][a]
+            arr[k].center[a] = w.get("center", (0, 0, 0))[a]
+            arr[k].vel[a] = w.get("vel", (0, 0, 0))[a]
+            arr[k].vrot[a] = w.get("vrot", (0, 0, 0))[a]
+    return arr
+
+
+class Context:
+    """One xdem_ctx on one GPU. Methods map 1:1 to the operator entry points of include/xdem.h."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.xdem_ctx_create(device, C.byref(h))
+        if rc != XDEM_OK:
+            raise XdemError(f"xdem_ctx_create failed: {ERR_NAMES.get(rc, rc)}")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.xdem_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != XDEM_OK:
+            buf = C.create_string_buffer(512)
+            self.lib.xdem_last_error(self.h, buf, 512)
+            raise XdemError(f"{what}: {ERR_NAMES.get(rc, rc)}: {buf.value.decode()}")
+
+    def call(self, name, *args):
+        rc = getattr(self.lib, name)(self.h, *args)
+        self._check(rc, name)
+
+    # ---- data
+    def set_domain(self, dom):
+        self.call("xdem_set_domain", C.byref(dom))
+
+    def set_surfaces(self, walls):
+        arr = surfaces(walls)
+        self.call("xdem_set_surfaces", arr, len(walls))
+
+    def upload(self, arrays):
+        n = int(arrays["rx"].shape[0])
+        p, keep = particles_struct(arrays, n)
+        self.call("xdem_particles_upload", C.byref(p))
+
+    def download(self):
+        n = int(self.lib.xdem_num_particles(self.h))
+        out = {name: np.zeros(n, dtype=DTYPES.get(name, np.float64)) for name in FIELDS}
+        p, keep = particles_struct(out, n)
+        self.call("xdem_particles_download", C.byref(p))
+        return out
+
+    def num_particles(self):
+        return int(self.lib.xdem_num_particles(self.h))
+
+    def synchronize(self):
+        self.call("xdem_synchronize")
+
+    # ---- operators
+    def combined_compute_prolog(self, dt):
+        self.call("xdem_combined_compute_prolog", dt)
+
+    def combined_compute_epilog(self, dt):
+        self.call("xdem_combined_compute_epilog", dt)
+
+    def reset_force_moment(self):
+        self.call("xdem_reset_force_moment")
+
+    def gravity_force(self, g):
+        self.call("xdem_gravity_force", (C.c_double * 3)(*g))
+
+    def force_to_accel(self):
+        self.call("xdem_force_to_accel")
+
+    def particle_displ_over(self, threshold):
+        r = C.c_int32()
+        self.call("xdem_particle_displ_over", threshold, C.byref(r))
+        return bool(r.value)
+
+    def move_particles(self):
+        self.call("xdem_move_particles")
+
+    def chunk_neighbors_contact(self, rcut_inc, nbh_dist=-1.0):
+        self.call("xdem_chunk_neighbors_contact", rcut_inc, nbh_dist)
+
+    def nbh_sphere(self, rcut_inc, symetric=True):
+        self.call("xdem_nbh_sphere", rcut_inc, 1 if symetric else 0)
+
+    def classify_interactions(self):
+        self.call("xdem_classify_interactions")
+
+    def contact_sphere(self, cp, cpd=None, dt=1e-5, form=XDEM_FORM_FP64):
+        self.call("xdem_contact_sphere", C.byref(cp), C.byref(cpd) if cpd is not None else None, dt, form)
+
+    def run_steps(self, scheme, nsteps):
+        st = RunStats()
+        self.call("xdem_run_steps", C.byref(scheme), nsteps, C.byref(st))
+        return {"steps": st.steps, "rebuilds": st.rebuilds, "polls": st.polls}
+
+    def set_timing(self, on):
+        self.call("xdem_set_timing", 1 if on else 0)
+
+    def kernel_timing(self, which, reset=True):
+        ms = C.c_double()
+        cnt = C.c_int64()
+        self.call("xdem_kernel_timing", which, 1 if reset else 0, C.byref(ms), C.byref(cnt))
+        return ms.value, cnt.value
+
+    # ---- inspection
+    def interactions(self):
+        npp, npw = C.c_int64(), C.c_int64()
+        self.call("xdem_interactions_count", C.byref(npp), C.byref(npw))
+        cap = npp.value + npw.value
+        ii = np.zeros(cap, np.uint64)
+        jj = np.zeros(cap, np.uint64)
+        tt = np.zeros(cap, np.uint16)
+        fr = np.zeros(3 * cap, np.float64)
+        wr = C.c_int64()
+        self.call("xdem_interactions_dump", cap, _ptr(ii, C.c_uint64), _ptr(jj, C.c_uint64),
+                  _ptr(tt, C.c_uint16), _ptr(fr, C.c_double), C.byref(wr))
+        k = wr.value
+        return ii[:k], jj[:k], tt[:k], fr[:3 * k].reshape(-1, 3)
+
+    def streams(self):
+        pairs, slots = C.c_int64(), C.c_int64()
+        dims = (C.c_int32 * 3)()
+        self.call("xdem_nbh_stats", C.byref(pairs), C.byref(slots), dims)
+        cap = max(1, slots.value)
+        a = np.zeros(cap, np.uint64)
+        b = np.zeros(cap, np.uint64)
+        code = np.zeros(cap, np.int32)
+        own = np.zeros(cap, np.uint8)
+        wr = C.c_int64()
+        self.call("xdem_streams_dump", cap, _ptr(a, C.c_uint64), _ptr(b, C.c_uint64), _ptr(code, C.c_int32),
+                  _ptr(own, C.c_uint8), C.byref(wr))
+        k = wr.value
+        return a[:k], b[:k], code[:k], own[:k]
+
+    def nbh_stats(self):
+        pairs, slots = C.c_int64(), C.c_int64()
+        dims = (C.c_int32 * 3)()
+        self.call("xdem_nbh_stats", C.byref(pairs), C.byref(slots), dims)
+        return {"pairs": pairs.value, "slots": slots.value, "dims": tuple(dims)}

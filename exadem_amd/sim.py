@@ -1,0 +1,71 @@
+"""Operator-order driver of the sphere path on one GPU.
+
+`first_iteration` and `step` call the operators one by one in the order of the reference decks
+(data/config/main-config.msp:55-65 and data/config/config_numerical_schemes.msp:6-28, with
+compute_force = [gravity_force, contact_sphere] and the rebuild branch of
+config_move_particles.msp:141-195). `run` hands whole blocks of steps to the fused engine
+(xdem_run_steps), which produces the same state with two kernels per step.
+"""
+from . import native
+
+
+def scheme(workload, form=native.XDEM_FORM_FP64, poll=16):
+    s = native.Scheme()
+    s.dt = workload.dt
+    for a in range(3):
+        s.gravity[a] = workload.gravity[a]
+    s.gravity_on = 1 if any(g != 0 for g in workload.gravity) else 0
+    s.form = form
+    s.cp = workload.cp
+    s.cpd = workload.cpd
+    s.rcut_inc = workload.rcut_inc
+    s.poll = poll
+    return s
+
+
+class SphereSim:
+    def __init__(self, workload, device=0, form=native.XDEM_FORM_FP64, ctx=None):
+        self.w = workload
+        self.form = form
+        self.ctx = ctx or native.Context(device)
+        self.ctx.set_domain(workload.domain)
+        self.ctx.set_surfaces(workload.walls)
+        self.ctx.upload(workload.particles)
+        self.rebuilds = 0
+
+    def rebuild(self):
+        """update_particles_full: move_particles, chunk_neighbors_contact, nbh_sphere, classify."""
+        c = self.ctx
+        c.move_particles()
+        c.chunk_neighbors_contact(self.w.rcut_inc)
+        c.nbh_sphere(self.w.rcut_inc)
+        c.classify_interactions()
+
+    def compute_force(self):
+        c = self.ctx
+        if any(g != 0 for g in self.w.gravity):
+            c.gravity_force(self.w.gravity)
+        c.contact_sphere(self.w.cp, self.w.cpd, self.w.dt, self.form)
+
+    def first_iteration(self):
+        """init_particles (rebuild), reset_force_moment, compute_force; no force_to_accel."""
+        self.rebuild()
+        self.ctx.reset_force_moment()
+        self.compute_force()
+
+    def step(self):
+        """One numerical_scheme_verlet_macro iteration, operator by operator."""
+        c, w = self.ctx, self.w
+        c.combined_compute_prolog(w.dt)
+        if c.particle_displ_over(0.5 * w.rcut_inc):
+            self.rebuild()
+            self.rebuilds += 1
+        c.reset_force_moment()
+        self.compute_force()
+        c.force_to_accel()
+        c.combined_compute_epilog(w.dt)
+
+    def run(self, nsteps, poll=16):
+        st = self.ctx.run_steps(scheme(self.w, self.form, poll), nsteps)
+        self.rebuilds += st["rebuilds"]
+        return st

@@ -1,0 +1,212 @@
+/*
+ * xdem.h — C ABI of libxdem_hip.so, the MI355X (gfx950) implementation of
+ * exaDEM's per-timestep hot path: cell-grid neighbour build, interaction
+ * update with friction-history carry-over, Hooke contact with Coulomb
+ * friction (spheres and walls), and velocity-Verlet integration.
+ *
+ * Rules of the boundary
+ *   - plain C types only: POD structs, raw host pointers, sizes;
+ *   - every call takes an opaque xdem_ctx that owns the device buffers and one
+ *     HIP stream (default priority) on one GPU;
+ *   - every call returns 0 (XDEM_OK) or a negative XDEM_ERR_* code; the text
+ *     of the last error is available through xdem_last_error();
+ *   - device work is queued on the context stream; calls that return host
+ *     scalars or fill host arrays synchronise that stream first;
+ *   - one host thread per context.
+ *
+ * Each operator entry point names the reference operator it replaces
+ * (paths relative to the exaDEM source tree).
+ */
+#ifndef XDEM_H
+#define XDEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ codes */
+#define XDEM_OK 0
+#define XDEM_ERR_ARG (-1)         /* invalid argument or slot value           */
+#define XDEM_ERR_HIP (-2)         /* HIP runtime error                        */
+#define XDEM_ERR_STATE (-3)       /* call made in the wrong state             */
+#define XDEM_ERR_CAPACITY (-4)    /* a documented size limit was exceeded     */
+#define XDEM_ERR_UNSUPPORTED (-5) /* feature not provided by this build      */
+
+/* Largest particle count a context accepts (neighbour words hold 25 bits). */
+#define XDEM_MAX_PARTICLES 33554431
+/* Largest number of planar walls (Surface drivers). */
+#define XDEM_MAX_SURFACES 8
+
+/* Contact-force forms accepted by xdem_contact_sphere / xdem_run_steps. */
+#define XDEM_FORM_FP64 0 /* plain form: FP64 end to end                   */
+#define XDEM_FORM_FP32 1 /* FP32 geometry/force on cell-relative offsets, */
+                         /* FP64 overlap sign test, compensated FP32 sums */
+
+typedef struct xdem_ctx xdem_ctx;
+
+/* Particle fields, structure of arrays, one entry per particle.
+ * Mirrors DEMFieldSet (include/exaDEM/fields.h:34-54): rx..rz implicit,
+ * vx..vz, fx..fz, mass, homothety, radius, orient (w,x,y,z), mom, vrot,
+ * arot, inertia, id, type, group. The stress tensor is not carried.
+ * Pointers may be NULL in a download for fields the caller does not want. */
+typedef struct xdem_particles {
+  int64_t n;
+  uint64_t* id;
+  uint32_t* type;
+  uint32_t* group;
+  double *rx, *ry, *rz;
+  double *vx, *vy, *vz;
+  double *fx, *fy, *fz;
+  double *mass, *radius, *homothety;
+  double *qw, *qx, *qy, *qz;
+  double *momx, *momy, *momz;
+  double *vrotx, *vroty, *vrotz;
+  double *arotx, *aroty, *arotz;
+  double *inx, *iny, *inz;
+} xdem_particles;
+
+/* Simulation box. Axes flagged periodic wrap particles at each rebuild and
+ * see periodic images of the particles near the opposite face (the role of
+ * exaNBody's ghost layer on one rank). cell_size is the minimum cell edge;
+ * the grid uses floor(L/cell_size) cells per axis (at least 3 on periodic
+ * axes) and an edge of L/dims. */
+typedef struct xdem_domain {
+  double lo[3];
+  double hi[3];
+  int32_t periodic[3];
+  double cell_size;
+} xdem_domain;
+
+/* ContactParams (src/forcefield/include/exaDEM/forcefield/contact_parameters.hpp:32-41),
+ * same members in the same order. */
+typedef struct xdem_contact_params {
+  double kn, kt, kr, mu, damp_rate, fc, dncut, gamma;
+} xdem_contact_params;
+
+/* Stationary planar wall: SurfaceFields (src/driver/include/exaDEM/surface.hpp:29-43).
+ * center defaults to offset*normal when all three components are zero. */
+typedef struct xdem_surface {
+  double offset;
+  double normal[3];
+  double center[3];
+  double vel[3];
+  double vrot[3];
+} xdem_surface;
+
+/* One time step of numerical_scheme_verlet_macro
+ * (data/config/config_numerical_schemes.msp:6-28) with
+ * compute_force = [gravity_force, contact_sphere]. */
+typedef struct xdem_scheme {
+  double dt;
+  double gravity[3];
+  int32_t gravity_on;
+  int32_t form;            /* XDEM_FORM_* */
+  xdem_contact_params cp;  /* particle-particle law (slot config)          */
+  xdem_contact_params cpd; /* particle-wall law (slot config_driver)       */
+  double rcut_inc;         /* Verlet skin; rebuild when a particle moved   */
+                           /* at least rcut_inc/2 since the last rebuild   */
+  int32_t poll;            /* steps between host polls of the rebuild flag */
+} xdem_scheme;
+
+/* Per-call statistics of xdem_run_steps. */
+typedef struct xdem_run_stats {
+  int64_t steps;    /* steps executed                                   */
+  int64_t rebuilds; /* neighbour rebuilds triggered inside the call     */
+  int64_t polls;    /* host synchronisations made to read the flag      */
+} xdem_run_stats;
+
+/* --------------------------------------------------------- context / data */
+int xdem_ctx_create(int32_t device, xdem_ctx** out);
+void xdem_ctx_destroy(xdem_ctx* ctx);
+/* Copies the last error text (NUL terminated, truncated to n bytes). */
+int xdem_last_error(const xdem_ctx* ctx, char* buf, int64_t n);
+int xdem_synchronize(xdem_ctx* ctx);
+
+/* domain operator (data/config/main-config.msp:160-163) + grid sizing. */
+int xdem_set_domain(xdem_ctx* ctx, const xdem_domain* dom);
+/* register_surface (src/driver/register_surface.cpp:27-58), whole list at once. */
+int xdem_set_surfaces(xdem_ctx* ctx, const xdem_surface* walls, int32_t n);
+/* Replaces the particle set (input_data); ids must be unique. */
+int xdem_particles_upload(xdem_ctx* ctx, const xdem_particles* p);
+/* Fills the caller's arrays in the current device (cell-sorted) order. */
+int xdem_particles_download(xdem_ctx* ctx, xdem_particles* p);
+int64_t xdem_num_particles(const xdem_ctx* ctx);
+
+/* ------------------------------------------------------------- operators */
+/* combined_compute_prolog (src/numericalscheme/combined_compute_prolog.cu:94-116). */
+int xdem_combined_compute_prolog(xdem_ctx* ctx, double dt);
+/* combined_compute_epilog (src/numericalscheme/combined_compute_epilog.cu:94-113). */
+int xdem_combined_compute_epilog(xdem_ctx* ctx, double dt);
+/* reset_force_moment (src/numericalscheme/reset_force_moment.cu:55-72). */
+int xdem_reset_force_moment(xdem_ctx* ctx);
+/* gravity_force (src/forcefield/gravity_force.cu:38-87). */
+int xdem_gravity_force(xdem_ctx* ctx, const double g[3]);
+/* force_to_accel (src/numericalscheme/include/exaDEM/force_to_accel.hpp:26-33). */
+int xdem_force_to_accel(xdem_ctx* ctx);
+/* particle_displ_over (exaNBody; config_move_particles.msp:12-15): *result = 1
+ * when a particle moved by at least `threshold` since the last rebuild. */
+int xdem_particle_displ_over(xdem_ctx* ctx, double threshold, int32_t* result);
+
+/* migrate_cell_particles_interaction + backup_r (config_spheres.msp:14-17):
+ * wraps periodic coordinates, bins particles into cells and reorders every
+ * field cell by cell (particles of a cell by ascending id). */
+int xdem_move_particles(xdem_ctx* ctx);
+/* chunk_neighbors_contact (src/grid/chunk_neighbors_contact.cpp:45-114):
+ * pair (a,b) is a neighbour iff d2 > 0, d2 < (ra+rb+rcut_inc)^2 and
+ * d2 < nbh_dist^2. nbh_dist <= 0 selects 2*max(radius)+rcut_inc. */
+int xdem_chunk_neighbors_contact(xdem_ctx* ctx, double rcut_inc, double nbh_dist);
+/* nbh_sphere (src/sphere/nbh_sphere.cu:64-309) with symetric=true: owner
+ * choice, wall candidates, friction-history carry-over by key. */
+int xdem_nbh_sphere(xdem_ctx* ctx, double rcut_inc, int32_t symetric);
+/* classify_interactions (src/interaction/classify_interactions.cpp:52-136):
+ * interactions already live in per-type device arrays; validates state. */
+int xdem_classify_interactions(xdem_ctx* ctx);
+/* contact_sphere / contact_singlemat_sphere[_hooke[_none]]
+ * (src/sphere/contact_force_sphere.cu.in:44-333): adds contact forces and
+ * moments of every particle-particle and particle-wall interaction to f and
+ * mom, and updates friction history. Deterministic, no atomics. */
+int xdem_contact_sphere(xdem_ctx* ctx, const xdem_contact_params* cp, const xdem_contact_params* cp_driver,
+                        double dt, int32_t form);
+
+/* Fused engine: nsteps iterations of numerical_scheme_verlet_macro with
+ * compute_force = [gravity_force, contact_sphere], rebuilding
+ * (move_particles, chunk_neighbors_contact, nbh_sphere) whenever
+ * particle_displ_over fires. Produces the same state as the operator-by-
+ * operator sequence. Requires a prior neighbour build and force pass
+ * (first_iteration). stats may be NULL. */
+int xdem_run_steps(xdem_ctx* ctx, const xdem_scheme* scheme, int64_t nsteps, xdem_run_stats* stats);
+
+/* ------------------------------------------------------------ inspection */
+/* Number of particle-particle interactions held (owner side, half list) and
+ * of particle-wall interactions. */
+int xdem_interactions_count(xdem_ctx* ctx, int64_t* n_pp, int64_t* n_pw);
+/* Writes the interactions in device order: owner id, partner id (wall
+ * index for walls), type (0 = VertexVertex, 5 = VertexSurface) and the
+ * friction history (3 doubles per interaction). cap is the array length. */
+int xdem_interactions_dump(xdem_ctx* ctx, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint16_t* type,
+                           double* friction, int64_t* written);
+/* Neighbour-list statistics of the last build: half-list pairs P, list slots
+ * allocated (ELL with per-wave padding) and the grid dimensions. */
+int xdem_nbh_stats(xdem_ctx* ctx, int64_t* pairs, int64_t* slots, int32_t dims[3]);
+
+/* Full (both-direction) neighbour streams of the last build, in device order: particle id, partner
+ * id, periodic image code (13 = none, else (sx+1)+3(sy+1)+9(sz+1)) and owner flag. */
+int xdem_streams_dump(xdem_ctx* ctx, int64_t cap, uint64_t* id_a, uint64_t* id_b, int32_t* code, uint8_t* owner,
+                      int64_t* written);
+
+/* Enables (on != 0) HIP-event timing of every integrate and force launch of xdem_run_steps. */
+int xdem_set_timing(xdem_ctx* ctx, int32_t on);
+
+/* Device time of the last kernel launches of the fused engine, in
+ * milliseconds per launch, measured with HIP events on the context stream:
+ * which = 0 integrate, 1 force. Returns the average over `count` launches
+ * recorded since the last reset (reset when reset != 0). */
+int xdem_kernel_timing(xdem_ctx* ctx, int32_t which, int32_t reset, double* avg_ms, int64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XDEM_H */

@@ -1,0 +1,196 @@
+"""ctypes wrapper of the CPU oracle (oracle/libxdem_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by the cpu_baseline
+leg of bench.py, as the checker. Nothing in the exadem_amd package imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libxdem_oracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    from exadem_amd.native import Particles, Domain, Surface, ContactParams, Scheme  # POD layouts only
+    lib = C.CDLL(LIB)
+    vp = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    u64p = C.POINTER(C.c_uint64)
+    lib.xo_create.restype = vp
+    lib.xo_create.argtypes = [C.POINTER(Particles), C.POINTER(Domain)]
+    lib.xo_destroy.argtypes = [vp]
+    lib.xo_set_mutation.argtypes = [C.c_int]
+    lib.xo_set_surfaces.argtypes = [vp, C.POINTER(Surface), C.c_int]
+    lib.xo_rebuild.argtypes = [vp, C.c_double]
+    lib.xo_move_particles.argtypes = [vp]
+    lib.xo_chunk_neighbors.argtypes = [vp, C.c_double, C.c_double]
+    lib.xo_nbh_sphere.argtypes = [vp, C.c_double]
+    lib.xo_reset_force_moment.argtypes = [vp]
+    lib.xo_gravity.argtypes = [vp, dp]
+    lib.xo_contact.argtypes = [vp, C.POINTER(ContactParams), C.POINTER(ContactParams), C.c_double]
+    lib.xo_force_to_accel.argtypes = [vp]
+    lib.xo_prolog.argtypes = [vp, C.c_double]
+    lib.xo_epilog.argtypes = [vp, C.c_double]
+    lib.xo_displ_over.argtypes = [vp, C.c_double]
+    lib.xo_displ_over.restype = C.c_int
+    lib.xo_first_iteration.argtypes = [vp, C.POINTER(Scheme)]
+    lib.xo_run_steps.argtypes = [vp, C.POINTER(Scheme), C.c_int64]
+    lib.xo_run_steps.restype = C.c_int64
+    lib.xo_num_particles.argtypes = [vp]
+    lib.xo_num_particles.restype = C.c_int64
+    lib.xo_get_particles.argtypes = [vp, C.POINTER(Particles)]
+    lib.xo_interactions_count.argtypes = [vp]
+    lib.xo_interactions_count.restype = C.c_int64
+    lib.xo_interactions_dump.argtypes = [vp, C.c_int64, u64p, u64p, C.POINTER(C.c_uint16), dp, dp]
+    lib.xo_interactions_dump.restype = C.c_int64
+    lib.xo_streams_count.argtypes = [vp]
+    lib.xo_streams_count.restype = C.c_int64
+    lib.xo_streams_dump.argtypes = [vp, C.c_int64, u64p, u64p, C.POINTER(C.c_int32)]
+    lib.xo_streams_dump.restype = C.c_int64
+    lib.xo_pair_force.argtypes = [dp, dp, dp, C.c_double, C.c_double, dp, dp, dp, C.c_double, C.c_double,
+                                  C.POINTER(ContactParams), C.c_double, dp, dp]
+    lib.xo_angular_accel.argtypes = [dp, dp, dp, dp, dp]
+    lib.xo_quat_rotate.argtypes = [dp, dp, dp]
+    _lib = lib
+    return lib
+
+
+def _d(a):
+    return np.ascontiguousarray(a, np.float64).ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Oracle:
+    """The CPU restatement, holding one particle system (same entry-point order as the operators)."""
+
+    def __init__(self, particles, dom, walls=()):
+        from exadem_amd.native import particles_struct, surfaces
+        self.lib = load()
+        n = int(particles["rx"].shape[0])
+        self._keep = {k: np.ascontiguousarray(v) for k, v in particles.items()}
+        p, _ = particles_struct(self._keep, n)
+        self.h = self.lib.xo_create(C.byref(p), C.byref(dom))
+        if walls:
+            self.lib.xo_set_surfaces(self.h, surfaces(list(walls)), len(walls))
+
+    def close(self):
+        if self.h:
+            self.lib.xo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def rebuild(self, rcut_inc):
+        self.lib.xo_rebuild(self.h, rcut_inc)
+
+    def move_particles(self):
+        self.lib.xo_move_particles(self.h)
+
+    def chunk_neighbors(self, rcut_inc, nbh_dist):
+        self.lib.xo_chunk_neighbors(self.h, rcut_inc, nbh_dist)
+
+    def nbh_sphere(self, rcut_inc):
+        self.lib.xo_nbh_sphere(self.h, rcut_inc)
+
+    def reset_force_moment(self):
+        self.lib.xo_reset_force_moment(self.h)
+
+    def gravity(self, g):
+        self.lib.xo_gravity(self.h, (C.c_double * 3)(*g))
+
+    def contact(self, cp, cpd, dt):
+        self.lib.xo_contact(self.h, C.byref(cp), C.byref(cpd), dt)
+
+    def force_to_accel(self):
+        self.lib.xo_force_to_accel(self.h)
+
+    def prolog(self, dt):
+        self.lib.xo_prolog(self.h, dt)
+
+    def epilog(self, dt):
+        self.lib.xo_epilog(self.h, dt)
+
+    def displ_over(self, thr):
+        return bool(self.lib.xo_displ_over(self.h, thr))
+
+    def first_iteration(self, scheme):
+        self.lib.xo_first_iteration(self.h, C.byref(scheme))
+
+    def run_steps(self, scheme, nsteps):
+        return int(self.lib.xo_run_steps(self.h, C.byref(scheme), nsteps))
+
+    def particles(self):
+        from exadem_amd.native import FIELDS, DTYPES, particles_struct
+        n = int(self.lib.xo_num_particles(self.h))
+        out = {name: np.zeros(n, dtype=DTYPES.get(name, np.float64)) for name in FIELDS}
+        p, _ = particles_struct(out, n)
+        self.lib.xo_get_particles(self.h, C.byref(p))
+        return out
+
+    def interactions(self):
+        cap = int(self.lib.xo_interactions_count(self.h))
+        ii = np.zeros(cap, np.uint64)
+        jj = np.zeros(cap, np.uint64)
+        tt = np.zeros(cap, np.uint16)
+        fr = np.zeros(3 * cap)
+        mo = np.zeros(3 * cap)
+        k = self.lib.xo_interactions_dump(self.h, cap, ii.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                          jj.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                          tt.ctypes.data_as(C.POINTER(C.c_uint16)),
+                                          fr.ctypes.data_as(C.POINTER(C.c_double)),
+                                          mo.ctypes.data_as(C.POINTER(C.c_double)))
+        return ii[:k], jj[:k], tt[:k], fr[:3 * k].reshape(-1, 3), mo[:3 * k].reshape(-1, 3)
+
+    def streams(self):
+        cap = int(self.lib.xo_streams_count(self.h))
+        a = np.zeros(cap, np.uint64)
+        b = np.zeros(cap, np.uint64)
+        code = np.zeros(cap, np.int32)
+        k = self.lib.xo_streams_dump(self.h, cap, a.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                     b.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                     code.ctypes.data_as(C.POINTER(C.c_int32)))
+        return a[:k], b[:k], code[:k]
+
+
+def set_mutation(m):
+    load().xo_set_mutation(m)
+
+
+def pair_force(pi, vi, wi, ri, mi, pj, vj, wj, rj, mj, cp, dt, friction=(0.0, 0.0, 0.0)):
+    """One owner-side sphere-sphere contact evaluation. Returns dict(f, friction, dn, n, cp, contact)."""
+    lib = load()
+    fr = np.array(friction, np.float64)
+    out = np.zeros(14)
+    lib.xo_pair_force(_d(pi), _d(vi), _d(wi), ri, mi, _d(pj), _d(vj), _d(wj), rj, mj, C.byref(cp), dt,
+                      fr.ctypes.data_as(C.POINTER(C.c_double)), out.ctypes.data_as(C.POINTER(C.c_double)))
+    return {"f": out[0:3], "friction": out[3:6], "dn": out[6], "n": out[7:10], "cp": out[10:13],
+            "contact": bool(out[13])}
+
+
+def angular_accel(q, mom, vrot, inertia):
+    out = np.zeros(3)
+    load().xo_angular_accel(_d(q), _d(mom), _d(vrot), _d(inertia), out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def quat_rotate(q, v):
+    out = np.zeros(3)
+    load().xo_quat_rotate(_d(q), _d(v), out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out

@@ -1,0 +1,792 @@
+// xdem_oracle.cpp — CPU oracle for exaDEM's per-timestep sphere path.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker the HIP path is compared
+// against; it is loaded by tests/, by __graft_entry__.smoke() and by the
+// cpu_baseline leg of bench.py, and by nothing in the product.
+//
+// It restates, in plain FP64 C++ (OpenMP over particles), the reference
+// algorithm for one time step (paths relative to the exaDEM source tree):
+//   integration      src/numericalscheme/combined_compute_prolog.cu:35-47,94-116
+//                    src/numericalscheme/combined_compute_epilog.cu:35-46,94-113
+//                    src/numericalscheme/include/exaDEM/{push_to_quaternion,angular_acceleration,
+//                      angular_velocity,force_to_accel}.hpp
+//                    quaternion helpers: src/shape/include/exaDEM/type/quat.hpp:76-106
+//   gravity / reset  src/forcefield/include/exaDEM/forcefield/gravity_force.hpp:38-46,
+//                    src/numericalscheme/reset_force_moment.cu:69-70
+//   neighbours       src/grid/chunk_neighbors_contact.cpp:45-57 (filter), periodic images as
+//                    exaNBody ghost cells (one layer, positions r + shift)
+//   interactions     src/sphere/nbh_sphere.cu:64-309 (sym mode), interaction_manager.hpp:88-112,205-217,
+//                    placeholder_interaction.hpp:398-408, interaction_pair.hpp:186-249
+//   contact law      src/forcefield/include/exaDEM/forcefield/common_kernels.hpp:27-145,
+//                    contact_force.hpp:146-235, src/sphere/include/exaDEM/sphere/contact_sphere.hpp:121-297,
+//                    src/shape/include/exaDEM/shape_detection_core.hpp:57-81,
+//                    src/driver/include/exaDEM/surface.hpp:320-350
+//   operator order   data/config/main-config.msp:55-65 (first_iteration),
+//                    data/config/config_numerical_schemes.msp:6-28, config_move_particles.msp:141-195
+//
+// Choices where the reference leaves the result open (exaNBody/onika are not in the tree):
+//   * order of particles inside a cell: ascending id;
+//   * neighbour stream order: neighbour cells dz,dy,dx in -1..1 (dx fastest), particles in cell order;
+//   * force/moment summation order (the reference uses atomics, so any order is a valid result):
+//     per particle, gravity, then particle-particle contributions in its stream order, then walls
+//     in driver index order;
+//   * PushVec3SecondOrder: r += v*dt + f*dt^2/2; PushVec3FirstOrder: v += f*dt/2;
+//   * periodic wrap at rebuild: one shift by +-L per axis; displacement test |r - r_backup|^2 >= max_displ^2.
+// Build: see Makefile in this directory (g++ -O2 -ffp-contract=off -fopenmp).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../include/xdem.h"
+
+namespace orc {
+
+struct V3 {
+  double x, y, z;
+};
+static inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+static inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+static inline V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+static inline V3 operator*(V3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+static inline V3 operator*(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+static inline V3 operator/(V3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+static inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+static inline double norm(V3 a) { return std::sqrt(dot(a, a)); }
+
+struct Q4 {
+  double w, x, y, z;
+};
+
+// Mutation switch used by the oracle's own self-tests: a non-zero value plants one known
+// mistake so that the closed-form checks can show they catch it. Always 0 in normal use.
+static int g_mutation = 0;
+
+// ---------------------------------------------------------------- quaternion helpers
+// quat::dot(omega): time derivative 0.5*(0,omega)*q   (type/quat.hpp:92-96)
+static inline Q4 qdot(Q4 q, V3 w) {
+  return {-0.5 * (w.x * q.x + w.y * q.y + w.z * q.z), 0.5 * (w.y * q.z - w.z * q.y + w.x * q.w),
+          0.5 * (w.z * q.x - w.x * q.z + w.y * q.w), 0.5 * (w.x * q.y - w.y * q.x + w.z * q.w)};
+}
+static inline Q4 qnormalize(Q4 q) {
+  const double n = std::sqrt(q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z);
+  return {q.w / n, q.x / n, q.y / n, q.z / n};
+}
+// quat * vec3 rotation (type/quat.hpp:84-91)
+static inline V3 qrot(Q4 q, V3 v) {
+  V3 qv{q.x, q.y, q.z};
+  V3 uv = cross(qv, v);
+  V3 uuv = cross(qv, uv);
+  uv = uv * (2.0 * q.w);
+  uuv = uuv * 2.0;
+  return v + uv + uuv;
+}
+static inline Q4 qconj(Q4 q) { return {q.w, -q.x, -q.y, -q.z}; }
+
+// ---------------------------------------------------------------- contact law
+struct CP {
+  double kn, kt, kr, mu, damp_rate, fc, dncut, gamma;
+};
+
+// detection_vertex_vertex_core (shape_detection_core.hpp:57-81)
+struct Contact {
+  bool contact;
+  double dn;
+  V3 n, cp;
+};
+static inline Contact detect_vv(V3 pi, double ri, V3 pj, double rj) {
+  const double R = ri + rj;
+  const V3 dist = pi - pj;
+  const double dist_norm = norm(dist);
+  const double inv = 1.0 / dist_norm;
+  const double dn = dist_norm - R;
+  const V3 n = dist * inv;
+  const V3 cp = pi - n * (ri + 0.5 * dn);
+  return {dn < 0, dn, n, cp};
+}
+
+// contact_force_core<Hooke, None> (contact_force.hpp:146-235) with common_kernels.hpp:27-145
+static inline void contact_force_core(double dn, V3 n, double dt, const CP& p, double meff, V3& ft, V3 cp,
+                                      V3 pos_i, V3 vel_i, V3& f_i, V3& mom_i, V3 vrot_i, V3 pos_j, V3 vel_j,
+                                      V3 vrot_j) {
+  const double damp = p.damp_rate * 2.0 * std::sqrt(p.kn * meff);
+  const V3 ci = vel_i - cross(cp - pos_i, vrot_i);
+  const V3 cj = vel_j - cross(cp - pos_j, vrot_j);
+  const V3 vel = cj - ci;
+  const double vn = dot(vel, n);
+  // normal_force
+  const double fne = -p.kn * dn;
+  const double fnv = (g_mutation == 1 ? -damp : damp) * vn;
+  const double fnval = (fnv + fne > 0) ? fnv + fne : 0;
+  const V3 fn = fnval * n;
+  // tangential force: kt * (dt * (vel - vn*n))
+  const V3 vt = vel - vn * n;
+  const V3 inc = (g_mutation == 2 ? -p.kt : p.kt) * (dt * vt);
+  ft = ft + inc;
+  // compute_threshold_ft + fit_tangential_force
+  const double thr = std::fabs(p.mu * (p.kn * dn));
+  const double ft2 = dot(ft, ft);
+  if (g_mutation != 3 && ft2 > 0.0 && ft2 > thr * thr) ft = ft * (thr / std::sqrt(ft2));
+  f_i = fn + ft;
+  // rolling moment history and its clip (mur = 0)
+  mom_i = mom_i + (p.kr * (vrot_j - vrot_i)) * dt;
+  V3 branch = cp - pos_i;
+  const double r = dot(branch, vrot_i) / dot(vrot_i, vrot_i);
+  branch = branch - r * vrot_i;
+  const double mur = 0;
+  const double thr_m = std::fabs(mur * norm(branch) * norm(fn));
+  const double m2 = dot(mom_i, mom_i);
+  if (m2 > 0.0 && m2 > thr_m * thr_m) mom_i = mom_i * (thr_m / std::sqrt(m2));
+}
+
+// compute_moments (common_kernels.hpp:134-142)
+static inline V3 compute_moments(V3 cp, V3 p, V3 f, V3 m) { return cross(cp - p, f) + m; }
+
+// Surface (surface.hpp:320-350)
+struct Surf {
+  double offset;
+  V3 normal, center, vel, vrot, center_proj;
+};
+static inline bool surf_filter(const Surf& s, double rcut, V3 p) {
+  const V3 proj = dot(p, s.normal) * s.normal;
+  const double d = norm(proj - s.center_proj);
+  return d <= rcut;
+}
+static inline Contact surf_detect(const Surf& s, double rcut, V3 p) {
+  const V3 proj = dot(p, s.normal) * s.normal;
+  const V3 s2p = -(s.center_proj - proj);
+  const double d = norm(s2p);
+  const double dn = d - rcut;
+  if (dn > 0) return {false, dn, {0, 0, 0}, {0, 0, 0}};
+  const V3 n = s2p / d;
+  const V3 cp = p - n * (rcut + 0.5 * dn);
+  return {true, dn, n, cp};
+}
+
+// ---------------------------------------------------------------- interactions
+// InteractionPair key (interaction_pair.hpp:53-62,186-249) plus payload (interaction.hpp:33-35)
+struct Item {
+  uint64_t id_i, id_j;  // owner id, partner id (driver index for walls)
+  uint32_t sub_i, sub_j;
+  uint16_t type;  // 0 VertexVertex, 5 VertexSurface
+  uint8_t swap, ghost;
+  int32_t partner;  // sorted index of the partner particle (-1 for walls)
+  int32_t shift;    // periodic image code of the partner, 13 = none
+  V3 friction, moment;
+  // results of the last contact evaluation (for the canonical accumulation)
+  bool in_contact;
+  V3 f, cp, mint;
+};
+static inline bool key_less(const Item& a, const Item& b) {  // operator< (swap = 0 throughout)
+  if (a.id_i != b.id_i) return a.id_i < b.id_i;
+  if (a.id_j != b.id_j) return a.id_j < b.id_j;
+  if (a.type != b.type) return a.type < b.type;
+  if (a.swap != b.swap) return a.swap < b.swap;
+  if (a.sub_i != b.sub_i) return a.sub_i < b.sub_i;
+  return a.sub_j < b.sub_j;
+}
+static inline bool key_eq(const Item& a, const Item& b) {
+  return a.id_i == b.id_i && a.id_j == b.id_j && a.sub_i == b.sub_i && a.sub_j == b.sub_j && a.swap == b.swap &&
+         a.type == b.type;
+}
+static inline bool item_active(const Item& it) {
+  const bool zf = it.friction.x == 0 && it.friction.y == 0 && it.friction.z == 0;
+  const bool zm = it.moment.x == 0 && it.moment.y == 0 && it.moment.z == 0;
+  return !(zf && zm);
+}
+
+struct Stream {  // one neighbour of a particle in its stream
+  int32_t j;
+  int32_t shift;
+};
+
+struct Oracle {
+  int64_t n = 0;
+  // particle arrays (current, cell-sorted order)
+  std::vector<uint64_t> id;
+  std::vector<uint32_t> type, group;
+  std::vector<V3> r, v, f, mom, vrot, arot, inertia, backup;
+  std::vector<double> mass, radius, homothety;
+  std::vector<Q4> q;
+  // domain / grid
+  double lo[3] = {0, 0, 0}, L[3] = {1, 1, 1}, h[3] = {1, 1, 1};
+  int per[3] = {0, 0, 0};
+  int dims[3] = {1, 1, 1};
+  std::vector<int64_t> cell_start;  // size ncells+1
+  std::vector<Surf> walls;
+  // neighbour streams and interaction lists per particle
+  std::vector<std::vector<Stream>> streams;
+  std::vector<std::vector<Item>> items;  // per owner particle: walls first, then particle pairs
+  bool built = false;
+  int threads = 0;
+  std::string err;
+};
+
+static inline V3 shift_vec(const Oracle& o, int code) {
+  const int sx = code % 3 - 1, sy = (code / 3) % 3 - 1, sz = code / 9 - 1;
+  return {sx * o.L[0], sy * o.L[1], sz * o.L[2]};
+}
+
+static void set_domain(Oracle& o, const xdem_domain* d) {
+  for (int a = 0; a < 3; a++) {
+    o.lo[a] = d->lo[a];
+    o.L[a] = d->hi[a] - d->lo[a];
+    o.per[a] = d->periodic[a] ? 1 : 0;
+    int nd = (int)std::floor(o.L[a] / d->cell_size);
+    if (nd < 1) nd = 1;
+    o.dims[a] = nd;
+    o.h[a] = o.L[a] / nd;
+  }
+}
+
+// migrate + backup_r: wrap, bin, reorder (cells in index order, particles by id)
+static void move_particles(Oracle& o) {
+  const int64_t n = o.n;
+  std::vector<int64_t> cell(n);
+  for (int64_t i = 0; i < n; i++) {
+    double c[3] = {o.r[i].x, o.r[i].y, o.r[i].z};
+    int ci[3];
+    for (int a = 0; a < 3; a++) {
+      if (o.per[a]) {
+        if (c[a] < o.lo[a]) c[a] += o.L[a];
+        else if (c[a] >= o.lo[a] + o.L[a]) c[a] -= o.L[a];
+      }
+      int k = (int)std::floor((c[a] - o.lo[a]) / o.h[a]);
+      if (k < 0) k = 0;
+      if (k >= o.dims[a]) k = o.dims[a] - 1;
+      ci[a] = k;
+    }
+    o.r[i] = {c[0], c[1], c[2]};
+    cell[i] = ((int64_t)ci[2] * o.dims[1] + ci[1]) * o.dims[0] + ci[0];
+  }
+  std::vector<int64_t> perm(n);
+  for (int64_t i = 0; i < n; i++) perm[i] = i;
+  std::sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) {
+    if (cell[a] != cell[b]) return cell[a] < cell[b];
+    return o.id[a] < o.id[b];
+  });
+  auto apply = [&](auto& vec) {
+    auto tmp = vec;
+    for (int64_t i = 0; i < n; i++) vec[i] = tmp[perm[i]];
+  };
+  apply(o.id); apply(o.type); apply(o.group);
+  apply(o.r); apply(o.v); apply(o.f); apply(o.mom); apply(o.vrot); apply(o.arot); apply(o.inertia);
+  apply(o.mass); apply(o.radius); apply(o.homothety); apply(o.q);
+  // interaction storage travels with its particle
+  {
+    auto tmp = std::move(o.items);
+    o.items.assign(n, {});
+    if ((int64_t)tmp.size() == n)
+      for (int64_t i = 0; i < n; i++) o.items[i] = std::move(tmp[perm[i]]);
+  }
+  const int64_t nc = (int64_t)o.dims[0] * o.dims[1] * o.dims[2];
+  o.cell_start.assign(nc + 1, 0);
+  for (int64_t i = 0; i < n; i++) o.cell_start[cell[perm[i]] + 1]++;
+  for (int64_t c = 0; c < nc; c++) o.cell_start[c + 1] += o.cell_start[c];
+  o.backup = o.r;
+}
+
+// chunk_neighbors_contact: full (both direction) streams with the DEM filter.
+static void chunk_neighbors(Oracle& o, double rcut_inc, double nbh_dist) {
+  const int64_t n = o.n;
+  o.streams.assign(n, {});
+  const double rcut2 = nbh_dist * nbh_dist;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t x = 0; x < n; x++) {
+    const V3 rx = o.r[x];
+    int ci[3];
+    // the cell of x is recomputed from its (wrapped) position at rebuild
+    double c[3] = {rx.x, rx.y, rx.z};
+    for (int a = 0; a < 3; a++) {
+      int k = (int)std::floor((c[a] - o.lo[a]) / o.h[a]);
+      if (k < 0) k = 0;
+      if (k >= o.dims[a]) k = o.dims[a] - 1;
+      ci[a] = k;
+    }
+    auto& st = o.streams[x];
+    for (int dz = -1; dz <= 1; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          int nb[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz};
+          int s[3] = {0, 0, 0};
+          bool ok = true;
+          for (int a = 0; a < 3; a++) {
+            if (nb[a] < 0) {
+              if (!o.per[a]) { ok = false; break; }
+              nb[a] += o.dims[a];
+              s[a] = -1;
+            } else if (nb[a] >= o.dims[a]) {
+              if (!o.per[a]) { ok = false; break; }
+              nb[a] -= o.dims[a];
+              s[a] = 1;
+            }
+          }
+          if (!ok) continue;
+          const int code = (s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1);
+          const V3 sh = shift_vec(o, code);
+          const int64_t cb = ((int64_t)nb[2] * o.dims[1] + nb[1]) * o.dims[0] + nb[0];
+          for (int64_t y = o.cell_start[cb]; y < o.cell_start[cb + 1]; y++) {
+            if (y == x && code == 13) continue;
+            const V3 ry = o.r[y] + sh;  // ghost copy position
+            const V3 d = rx - ry;
+            const double d2 = dot(d, d);
+            const double rmax = o.radius[x] + o.radius[y] + rcut_inc;
+            if (d2 > 0.0 && d2 < rmax * rmax && d2 < rcut2) st.push_back({(int32_t)y, code});
+          }
+        }
+  }
+}
+
+// nbh_sphere, symetric mode: rebuild per-particle item lists carrying history by key.
+static void nbh_sphere(Oracle& o, double rcut_inc) {
+  const int64_t n = o.n;
+  if ((int64_t)o.items.size() != n) o.items.assign(n, {});
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t x = 0; x < n; x++) {
+    // extract_history + stable_sort (interaction_manager.hpp:205-217; nbh_sphere.cu:122-124)
+    std::vector<Item> hist;
+    for (auto& it : o.items[x])
+      if (item_active(it)) hist.push_back(it);
+    std::stable_sort(hist.begin(), hist.end(), key_less);
+    std::vector<Item> lst;
+    Item base{};
+    base.id_i = o.id[x];
+    base.sub_i = base.sub_j = 0;
+    base.swap = 0;
+    base.ghost = 0;
+    base.friction = {0, 0, 0};
+    base.moment = {0, 0, 0};
+    // drivers first (nbh_sphere.cu:137-233): Surface filter with rad + rcut_inc
+    for (size_t w = 0; w < o.walls.size(); w++) {
+      if (surf_filter(o.walls[w], o.radius[x] + rcut_inc, o.r[x])) {
+        Item it = base;
+        it.type = 5;
+        it.id_j = w;
+        it.partner = -1;
+        it.shift = 13;
+        lst.push_back(it);
+      }
+    }
+    // particle pairs in stream order, owner = lower id unless the partner is a periodic image
+    for (auto& s : o.streams[x]) {
+      const uint64_t idn = o.id[s.j];
+      if (o.id[x] >= idn && s.shift == 13) continue;
+      Item it = base;
+      it.type = 0;
+      it.id_j = idn;
+      it.partner = s.j;
+      it.shift = s.shift;
+      lst.push_back(it);
+    }
+    // update(): lower_bound in history, copy payload on key equality (placeholder_interaction.hpp:398-408)
+    if (g_mutation != 4) {
+      for (auto& it : lst) {
+        auto lb = std::lower_bound(hist.begin(), hist.end(), it, key_less);
+        if (lb != hist.end() && key_eq(it, *lb)) {
+          it.friction = lb->friction;
+          it.moment = lb->moment;
+        }
+      }
+    }
+    o.items[x] = std::move(lst);
+  }
+  o.built = true;
+}
+
+static void rebuild(Oracle& o, double rcut_inc) {
+  move_particles(o);
+  double rmax = 0;
+  for (int64_t i = 0; i < o.n; i++) rmax = std::max(rmax, 2 * o.radius[i]);
+  chunk_neighbors(o, rcut_inc, rmax + rcut_inc);
+  nbh_sphere(o, rcut_inc);
+}
+
+// contact_sphere: evaluate every interaction (owner side), then accumulate per particle.
+static void contact(Oracle& o, const CP& cp, const CP& cpd, double dt) {
+  const int64_t n = o.n;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t x = 0; x < n; x++) {
+    for (auto& it : o.items[x]) {
+      it.in_contact = false;
+      if (it.type == 0) {
+        const int64_t y = it.partner;
+        const V3 ri = o.r[x];
+        const V3 rj = o.r[y] + shift_vec(o, it.shift);
+        const Contact c = detect_vv(ri, o.radius[x], rj, o.radius[y]);
+        if (c.contact) {
+          const double mi = o.mass[x], mj = o.mass[y];
+          const double meff = (mi * mj) / (mi + mj);
+          V3 f{0, 0, 0};
+          contact_force_core(c.dn, c.n, dt, cp, meff, it.friction, c.cp, ri, o.v[x], f, it.moment, o.vrot[x], rj,
+                             o.v[y], o.vrot[y]);
+          it.in_contact = true;
+          it.f = f;
+          it.cp = c.cp;
+          it.mint = it.moment;
+        } else {
+          it.friction = {0, 0, 0};
+          it.moment = {0, 0, 0};
+        }
+      } else {  // wall (ContactLawDriverFunc<Surface>, contact_sphere.hpp:216-297)
+        const Surf& s = o.walls[it.id_j];
+        const V3 ri = o.r[x];
+        const Contact c = surf_detect(s, o.radius[x], ri);
+        if (c.contact) {
+          V3 f{0, 0, 0};
+          contact_force_core(c.dn, c.n, dt, cpd, o.mass[x], it.friction, c.cp, ri, o.v[x], f, it.moment, o.vrot[x],
+                             s.center, s.vel, s.vrot);
+          it.in_contact = true;
+          it.f = f;
+          it.cp = c.cp;
+          it.mint = it.moment;
+        } else {
+          it.friction = {0, 0, 0};
+          it.moment = {0, 0, 0};
+        }
+      }
+    }
+  }
+  // canonical accumulation: stream order, then walls
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t x = 0; x < n; x++) {
+    V3 fx = o.f[x], mx = o.mom[x];
+    const auto& own = o.items[x];
+    size_t k = 0;
+    while (k < own.size() && own[k].type != 0) k++;  // skip wall items (listed first)
+    for (auto& s : o.streams[x]) {
+      const uint64_t idn = o.id[s.j];
+      const bool owner = !(o.id[x] >= idn && s.shift == 13);
+      if (owner) {
+        const Item& it = own[k++];
+        if (it.in_contact) {
+          fx = fx + it.f;
+          mx = mx + compute_moments(it.cp, o.r[x], it.f, it.mint);
+        }
+      } else {
+        // the partner owns the interaction (interior pair): find its item for x
+        const auto& ol = o.items[s.j];
+        for (const auto& it : ol) {
+          if (it.type == 0 && it.partner == x && it.shift == 13) {
+            if (it.in_contact) {
+              const V3 nf = -it.f;
+              fx = fx + nf;
+              mx = mx + compute_moments(it.cp, o.r[x], nf, -it.mint);
+            }
+            break;
+          }
+        }
+      }
+    }
+    for (auto& it : own) {
+      if (it.type != 5 || !it.in_contact) continue;
+      fx = fx + it.f;
+      mx = mx + compute_moments(it.cp, o.r[x], it.f, it.mint);
+    }
+    o.f[x] = fx;
+    o.mom[x] = mx;
+  }
+}
+
+static void reset_force_moment(Oracle& o) {
+  for (int64_t i = 0; i < o.n; i++) {
+    o.f[i] = {0, 0, 0};
+    o.mom[i] = {0, 0, 0};
+  }
+}
+static void gravity(Oracle& o, V3 g) {
+  for (int64_t i = 0; i < o.n; i++) {
+    const double m = o.mass[i];
+    o.f[i] = {o.f[i].x + g.x * m, o.f[i].y + g.y * m, o.f[i].z + g.z * m};
+  }
+}
+static void force_to_accel(Oracle& o) {
+  for (int64_t i = 0; i < o.n; i++) {
+    const double m = o.mass[i];
+    o.f[i] = {o.f[i].x / m, o.f[i].y / m, o.f[i].z / m};
+  }
+}
+static void prolog(Oracle& o, double dt) {
+  const double dt_2 = dt * 0.5, dt2_2 = dt * dt * 0.5;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < o.n; i++) {
+    V3& r = o.r[i];
+    V3& v = o.v[i];
+    const V3 a = o.f[i];
+    r.x = r.x + (v.x * dt + a.x * dt2_2);
+    r.y = r.y + (v.y * dt + a.y * dt2_2);
+    r.z = r.z + (v.z * dt + a.z * dt2_2);
+    v.x = v.x + a.x * dt_2;
+    v.y = v.y + a.y * dt_2;
+    v.z = v.z + a.z * dt_2;
+    // push_to_quaternion (push_to_quaternion.hpp:31-37)
+    Q4 q = o.q[i];
+    const Q4 d = qdot(q, o.vrot[i]);
+    q = {q.w + d.w * dt, q.x + d.x * dt, q.y + d.y * dt, q.z + d.z * dt};
+    o.q[i] = qnormalize(q);
+    o.vrot[i] = o.vrot[i] + dt_2 * o.arot[i];
+  }
+}
+static inline V3 angular_accel(Q4 Q, V3 mom, V3 vrot, V3 I) {  // angular_acceleration.hpp:28-42
+  const Q4 Qi = qconj(Q);
+  const V3 om = qrot(Qi, vrot);
+  const V3 M = qrot(Qi, mom);
+  const V3 dom{(M.x - (I.z - I.y) * om.y * om.z) / I.x, (M.y - (I.x - I.z) * om.z * om.x) / I.y,
+               (M.z - (I.y - I.x) * om.x * om.y) / I.z};
+  return qrot(Q, dom);
+}
+static void epilog(Oracle& o, double dt) {
+  const double dt_2 = dt * 0.5;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < o.n; i++) {
+    o.arot[i] = angular_accel(o.q[i], o.mom[i], o.vrot[i], o.inertia[i]);
+    o.vrot[i] = o.vrot[i] + o.arot[i] * dt_2;
+    o.v[i] = {o.v[i].x + o.f[i].x * dt_2, o.v[i].y + o.f[i].y * dt_2, o.v[i].z + o.f[i].z * dt_2};
+  }
+}
+static bool displ_over(const Oracle& o, double thr) {
+  const double t2 = thr * thr;
+  for (int64_t i = 0; i < o.n; i++) {
+    const V3 d = o.r[i] - o.backup[i];
+    if (dot(d, d) >= t2) return true;
+  }
+  return false;
+}
+
+static CP to_cp(const xdem_contact_params* p) {
+  CP c{};
+  if (p) c = {p->kn, p->kt, p->kr, p->mu, p->damp_rate, p->fc, p->dncut, p->gamma};
+  return c;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+// ------------------------------------------------------------------ C interface (ctypes)
+extern "C" {
+
+void* xo_create(const xdem_particles* p, const xdem_domain* dom) {
+  Oracle* o = new Oracle();
+  const int64_t n = p->n;
+  o->n = n;
+  o->id.assign(p->id, p->id + n);
+  o->type.assign(n, 0);
+  o->group.assign(n, 0);
+  if (p->type) o->type.assign(p->type, p->type + n);
+  if (p->group) o->group.assign(p->group, p->group + n);
+  auto v3 = [&](double* a, double* b, double* c) {
+    std::vector<V3> out(n);
+    for (int64_t i = 0; i < n; i++) out[i] = {a ? a[i] : 0.0, b ? b[i] : 0.0, c ? c[i] : 0.0};
+    return out;
+  };
+  o->r = v3(p->rx, p->ry, p->rz);
+  o->v = v3(p->vx, p->vy, p->vz);
+  o->f = v3(p->fx, p->fy, p->fz);
+  o->mom = v3(p->momx, p->momy, p->momz);
+  o->vrot = v3(p->vrotx, p->vroty, p->vrotz);
+  o->arot = v3(p->arotx, p->aroty, p->arotz);
+  o->inertia = v3(p->inx, p->iny, p->inz);
+  o->mass.assign(p->mass, p->mass + n);
+  o->radius.assign(p->radius, p->radius + n);
+  o->homothety.assign(n, 1.0);
+  if (p->homothety) o->homothety.assign(p->homothety, p->homothety + n);
+  o->q.resize(n);
+  for (int64_t i = 0; i < n; i++) o->q[i] = {p->qw ? p->qw[i] : 1.0, p->qx ? p->qx[i] : 0.0, p->qy ? p->qy[i] : 0.0,
+                                              p->qz ? p->qz[i] : 0.0};
+  o->backup = o->r;
+  set_domain(*o, dom);
+  return o;
+}
+
+void xo_destroy(void* h) { delete (Oracle*)h; }
+
+void xo_set_mutation(int m) { g_mutation = m; }
+
+void xo_set_surfaces(void* h, const xdem_surface* w, int nw) {
+  Oracle* o = (Oracle*)h;
+  o->walls.clear();
+  for (int k = 0; k < nw; k++) {
+    Surf s{};
+    s.offset = w[k].offset;
+    s.normal = {w[k].normal[0], w[k].normal[1], w[k].normal[2]};
+    s.center = {w[k].center[0], w[k].center[1], w[k].center[2]};
+    if (s.center.x == 0 && s.center.y == 0 && s.center.z == 0) s.center = s.offset * s.normal;
+    s.vel = {w[k].vel[0], w[k].vel[1], w[k].vel[2]};
+    s.vrot = {w[k].vrot[0], w[k].vrot[1], w[k].vrot[2]};
+    s.center_proj = s.offset * s.normal;  // Surface::initialize (surface.hpp:145-150)
+    o->walls.push_back(s);
+  }
+}
+
+void xo_rebuild(void* h, double rcut_inc) { rebuild(*(Oracle*)h, rcut_inc); }
+void xo_move_particles(void* h) { move_particles(*(Oracle*)h); }
+void xo_chunk_neighbors(void* h, double rcut_inc, double nbh_dist) { chunk_neighbors(*(Oracle*)h, rcut_inc, nbh_dist); }
+void xo_nbh_sphere(void* h, double rcut_inc) { nbh_sphere(*(Oracle*)h, rcut_inc); }
+void xo_reset_force_moment(void* h) { reset_force_moment(*(Oracle*)h); }
+void xo_gravity(void* h, const double* g) { gravity(*(Oracle*)h, {g[0], g[1], g[2]}); }
+void xo_contact(void* h, const xdem_contact_params* cp, const xdem_contact_params* cpd, double dt) {
+  contact(*(Oracle*)h, to_cp(cp), to_cp(cpd), dt);
+}
+void xo_force_to_accel(void* h) { force_to_accel(*(Oracle*)h); }
+void xo_prolog(void* h, double dt) { prolog(*(Oracle*)h, dt); }
+void xo_epilog(void* h, double dt) { epilog(*(Oracle*)h, dt); }
+int xo_displ_over(void* h, double thr) { return displ_over(*(Oracle*)h, thr) ? 1 : 0; }
+
+// first_iteration (main-config.msp:55-65): init_particles (rebuild), reset_force_moment,
+// compute_force (gravity + contact); force_to_accel is not part of it.
+void xo_first_iteration(void* h, const xdem_scheme* s) {
+  Oracle& o = *(Oracle*)h;
+  rebuild(o, s->rcut_inc);
+  reset_force_moment(o);
+  if (s->gravity_on) gravity(o, {s->gravity[0], s->gravity[1], s->gravity[2]});
+  contact(o, to_cp(&s->cp), to_cp(&s->cpd), s->dt);
+}
+
+// numerical_scheme_verlet_macro, nsteps times. Returns the number of rebuilds.
+int64_t xo_run_steps(void* h, const xdem_scheme* s, int64_t nsteps) {
+  Oracle& o = *(Oracle*)h;
+  int64_t rebuilds = 0;
+  for (int64_t k = 0; k < nsteps; k++) {
+    prolog(o, s->dt);
+    if (displ_over(o, 0.5 * s->rcut_inc)) {
+      rebuild(o, s->rcut_inc);
+      rebuilds++;
+    }
+    reset_force_moment(o);
+    if (s->gravity_on) gravity(o, {s->gravity[0], s->gravity[1], s->gravity[2]});
+    contact(o, to_cp(&s->cp), to_cp(&s->cpd), s->dt);
+    force_to_accel(o);
+    epilog(o, s->dt);
+  }
+  return rebuilds;
+}
+
+int64_t xo_num_particles(void* h) { return ((Oracle*)h)->n; }
+
+void xo_get_particles(void* h, xdem_particles* p) {
+  Oracle& o = *(Oracle*)h;
+  const int64_t n = o.n;
+  for (int64_t i = 0; i < n; i++) {
+    if (p->id) p->id[i] = o.id[i];
+    if (p->type) p->type[i] = o.type[i];
+    if (p->group) p->group[i] = o.group[i];
+#define PUT3(A, B, C, F) \
+  if (p->A) p->A[i] = o.F[i].x; \
+  if (p->B) p->B[i] = o.F[i].y; \
+  if (p->C) p->C[i] = o.F[i].z;
+    PUT3(rx, ry, rz, r)
+    PUT3(vx, vy, vz, v)
+    PUT3(fx, fy, fz, f)
+    PUT3(momx, momy, momz, mom)
+    PUT3(vrotx, vroty, vrotz, vrot)
+    PUT3(arotx, aroty, arotz, arot)
+    PUT3(inx, iny, inz, inertia)
+#undef PUT3
+    if (p->mass) p->mass[i] = o.mass[i];
+    if (p->radius) p->radius[i] = o.radius[i];
+    if (p->homothety) p->homothety[i] = o.homothety[i];
+    if (p->qw) p->qw[i] = o.q[i].w;
+    if (p->qx) p->qx[i] = o.q[i].x;
+    if (p->qy) p->qy[i] = o.q[i].y;
+    if (p->qz) p->qz[i] = o.q[i].z;
+  }
+}
+
+int64_t xo_interactions_count(void* h) {
+  Oracle& o = *(Oracle*)h;
+  int64_t c = 0;
+  for (auto& l : o.items) c += (int64_t)l.size();
+  return c;
+}
+
+// owner id, partner id (wall index), type, friction (3 per item), moment (3 per item)
+int64_t xo_interactions_dump(void* h, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint16_t* type, double* fr,
+                             double* mo) {
+  Oracle& o = *(Oracle*)h;
+  int64_t k = 0;
+  for (auto& l : o.items)
+    for (auto& it : l) {
+      if (k >= cap) return k;
+      id_i[k] = it.id_i;
+      id_j[k] = it.id_j;
+      type[k] = it.type;
+      fr[3 * k] = it.friction.x;
+      fr[3 * k + 1] = it.friction.y;
+      fr[3 * k + 2] = it.friction.z;
+      if (mo) {
+        mo[3 * k] = it.moment.x;
+        mo[3 * k + 1] = it.moment.y;
+        mo[3 * k + 2] = it.moment.z;
+      }
+      k++;
+    }
+  return k;
+}
+
+// Stream (neighbour list) dump: particle id, neighbour id, image code; full both-direction streams.
+int64_t xo_streams_count(void* h) {
+  Oracle& o = *(Oracle*)h;
+  int64_t c = 0;
+  for (auto& s : o.streams) c += (int64_t)s.size();
+  return c;
+}
+int64_t xo_streams_dump(void* h, int64_t cap, uint64_t* a, uint64_t* b, int32_t* code) {
+  Oracle& o = *(Oracle*)h;
+  int64_t k = 0;
+  for (int64_t x = 0; x < o.n; x++)
+    for (auto& s : o.streams[x]) {
+      if (k >= cap) return k;
+      a[k] = o.id[x];
+      b[k] = o.id[s.j];
+      code[k] = s.shift;
+      k++;
+    }
+  return k;
+}
+
+// Contact-law unit entry: one sphere-sphere evaluation (owner side) for closed-form checks.
+// out: [0..2] f_i, [3..5] friction after, [6] dn, [7..9] n, [10..12] cp, [13] contact flag
+void xo_pair_force(const double* pi, const double* vi, const double* wi, double ri, double mi, const double* pj,
+                   const double* vj, const double* wj, double rj, double mj, const xdem_contact_params* cp, double dt,
+                   double* friction_io, double* out) {
+  const CP p = to_cp(cp);
+  const V3 Pi{pi[0], pi[1], pi[2]}, Pj{pj[0], pj[1], pj[2]};
+  const Contact c = detect_vv(Pi, ri, Pj, rj);
+  V3 ft{friction_io[0], friction_io[1], friction_io[2]};
+  V3 mom{0, 0, 0}, f{0, 0, 0};
+  if (c.contact) {
+    const double meff = (mi * mj) / (mi + mj);
+    contact_force_core(c.dn, c.n, dt, p, meff, ft, c.cp, Pi, {vi[0], vi[1], vi[2]}, f, mom, {wi[0], wi[1], wi[2]}, Pj,
+                       {vj[0], vj[1], vj[2]}, {wj[0], wj[1], wj[2]});
+  } else {
+    ft = {0, 0, 0};
+  }
+  friction_io[0] = ft.x;
+  friction_io[1] = ft.y;
+  friction_io[2] = ft.z;
+  const double o[14] = {f.x, f.y, f.z, ft.x, ft.y, ft.z, c.dn, c.n.x, c.n.y, c.n.z, c.cp.x, c.cp.y, c.cp.z,
+                        c.contact ? 1.0 : 0.0};
+  std::memcpy(out, o, sizeof(o));
+}
+
+// Rigid-body helpers for closed-form checks of the integrator pieces.
+void xo_angular_accel(const double* q, const double* mom, const double* vrot, const double* I, double* out) {
+  const V3 a = angular_accel({q[0], q[1], q[2], q[3]}, {mom[0], mom[1], mom[2]}, {vrot[0], vrot[1], vrot[2]},
+                             {I[0], I[1], I[2]});
+  out[0] = a.x;
+  out[1] = a.y;
+  out[2] = a.z;
+}
+void xo_quat_rotate(const double* q, const double* v, double* out) {
+  const V3 a = qrot({q[0], q[1], q[2], q[3]}, {v[0], v[1], v[2]});
+  out[0] = a.x;
+  out[1] = a.y;
+  out[2] = a.z;
+}
+
+}  // extern "C"

@@ -1,0 +1,198 @@
+"""Parity of the HIP sphere path (through the C ABI) with the CPU oracle, on seeded inputs.
+
+Bars: neighbour streams and interaction lists bit-exact (same order, same keys); friction history
+bit-exact; per-particle forces and moments within 1e-12 relative (floor: m|g|, the smallest force
+scale of the case); 50-100 step trajectories within 1e-10 relative. The fused multi-step engine
+must reproduce the operator-by-operator path bit for bit.
+"""
+import numpy as np
+import pytest
+
+from exadem_amd import native
+from exadem_amd.sim import SphereSim, scheme
+import pyoracle as po
+
+from helpers import (by_id, dense_deposition, force, interaction_table, periodic_pack, pos, rel_err, vec, vel,
+                     wall_deposition)
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 1e-12
+TRAJ_TOL = 1e-10
+
+
+def oracle_for(w):
+    return po.Oracle(w.particles, w.domain, w.walls)
+
+
+def gpu_for(w):
+    return SphereSim(w)
+
+
+def compare_state(pg, po_, force_floor, what, ftol=FORCE_TOL, ptol=TRAJ_TOL):
+    g, o = by_id(pg), by_id(po_)
+    assert np.array_equal(g["id"], o["id"]), what
+    L = max(1.0, float(np.max(np.abs(pos(o)))))
+    dpos = float(np.max(np.abs(pos(g) - pos(o)))) / L
+    vscale = max(1e-3, float(np.max(np.abs(vel(o)))))
+    dvel = float(np.max(np.abs(vel(g) - vel(o)))) / vscale
+    ef = rel_err(force(g), force(o), force_floor)
+    mom_scale = np.maximum(np.linalg.norm(vec(o, "mom"), axis=1), 1e-6 * float(np.max(force_floor)))
+    em = float(np.max(np.linalg.norm(vec(g, "mom") - vec(o, "mom"), axis=1) / mom_scale))
+    assert dpos <= ptol, f"{what}: positions differ by {dpos:.3e} (relative to box)"
+    assert dvel <= ptol, f"{what}: velocities differ by {dvel:.3e}"
+    assert ef <= ftol, f"{what}: forces differ by {ef:.3e} relative"
+    assert em <= ftol, f"{what}: moments differ by {em:.3e} relative"
+    return dpos, dvel, ef, em
+
+
+def test_upload_download_roundtrip(gpu_ok):
+    w = periodic_pack(n=1000)
+    ctx = native.Context()
+    ctx.set_domain(w.domain)
+    ctx.upload(w.particles)
+    out = ctx.download()
+    for k in ("rx", "ry", "rz", "vx", "radius", "mass", "inx", "qw"):
+        assert np.array_equal(out[k], w.particles[k]), k
+    assert np.array_equal(out["id"], w.particles["id"])
+
+
+@pytest.mark.parametrize("case", ["dense", "walls", "periodic"])
+def test_streams_bit_exact(gpu_ok, case):
+    w = {"dense": dense_deposition, "walls": wall_deposition, "periodic": periodic_pack}[case]()
+    o = oracle_for(w)
+    o.rebuild(w.rcut_inc)
+    s = gpu_for(w)
+    s.rebuild()
+    a, b, code, own = s.ctx.streams()
+    oa, ob, ocode = o.streams()
+    assert len(a) == len(oa) and len(a) > 0
+    assert np.array_equal(a, oa) and np.array_equal(b, ob) and np.array_equal(code, ocode)
+    # owner rule of nbh_sphere (symetric): lower id owns, periodic images are owned by both sides
+    exp_own = (a < b) | (code != 13)
+    assert np.array_equal(own.astype(bool), exp_own)
+    # positions after the wrap + cell sort are identical, in the same order
+    pg, pq = s.ctx.download(), o.particles()
+    assert np.array_equal(pg["id"], pq["id"])
+    assert np.array_equal(pg["rx"], pq["rx"]) and np.array_equal(pg["rz"], pq["rz"])
+
+
+@pytest.mark.parametrize("case", ["dense", "walls", "periodic"])
+def test_first_iteration_forces(gpu_ok, case):
+    w = {"dense": dense_deposition, "walls": wall_deposition, "periodic": periodic_pack}[case]()
+    o = oracle_for(w)
+    o.first_iteration(scheme(w))
+    s = gpu_for(w)
+    s.first_iteration()
+    pg, pq = s.ctx.download(), o.particles()
+    floor = by_id(pq)["mass"] * 9.81
+    compare_state(pg, pq, floor, f"{case} first_iteration", ptol=0.0)
+    # interaction lists: same keys, same (zero or carried) history
+    gi, gj, gt, gf = s.ctx.interactions()
+    oi, oj, ot, of, om = o.interactions()
+    kg, fg = interaction_table(gi, gj, gt, gf)
+    ko, fo = interaction_table(oi, oj, ot, of)
+    assert np.array_equal(kg, ko)
+    assert np.array_equal(fg, fo)
+    if case == "walls":
+        assert np.sum(kg[:, 2] == 5) > 0
+
+
+@pytest.mark.parametrize("case", ["dense", "walls", "periodic"])
+def test_operator_steps_match_oracle(gpu_ok, case):
+    w = {"dense": dense_deposition, "walls": wall_deposition, "periodic": periodic_pack}[case]()
+    sch = scheme(w)
+    o = oracle_for(w)
+    o.first_iteration(sch)
+    s = gpu_for(w)
+    s.first_iteration()
+    nsteps = 60
+    orb = o.run_steps(sch, nsteps)
+    for _ in range(nsteps):
+        s.step()
+    pg, pq = s.ctx.download(), o.particles()
+    floor = by_id(pq)["mass"] * 9.81 * 1e-3
+    compare_state(pg, pq, floor / by_id(pq)["mass"], f"{case} {nsteps} steps")
+    assert s.rebuilds == orb
+    gi, gj, gt, gf = s.ctx.interactions()
+    oi, oj, ot, of, om = o.interactions()
+    kg, fg = interaction_table(gi, gj, gt, gf)
+    ko, fo = interaction_table(oi, oj, ot, of)
+    assert np.array_equal(kg, ko)
+    scale = max(1e-30, float(np.max(np.abs(fo))))
+    assert float(np.max(np.abs(fg - fo))) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("case", ["dense", "periodic"])
+def test_fused_engine_equals_operator_path(gpu_ok, case):
+    w = {"dense": dense_deposition, "periodic": periodic_pack}[case]()
+    a = gpu_for(w)
+    a.first_iteration()
+    b = gpu_for(w)
+    b.first_iteration()
+    for _ in range(40):
+        a.step()
+    b.run(40, poll=7)
+    pa, pb = a.ctx.download(), b.ctx.download()
+    for k in ("rx", "ry", "rz", "vx", "vy", "vz", "fx", "fy", "fz", "momx", "vrotx", "vroty", "qw", "qx", "arotx"):
+        assert np.array_equal(pa[k], pb[k]), k
+    assert a.rebuilds == b.rebuilds
+
+
+def test_rebuild_history_carry_over(gpu_ok):
+    """Small skin: rebuilds fire every few steps; history must survive them bit for bit."""
+    w = periodic_pack(n=2000, sigma_v=3.0)
+    w.rcut_inc = 0.004
+    w.dt = 2e-4
+    sch = scheme(w)
+    o = oracle_for(w)
+    o.first_iteration(sch)
+    s = gpu_for(w)
+    s.first_iteration()
+    orb = o.run_steps(sch, 60)
+    st = s.run(60, poll=5)
+    assert orb >= 3, "the case must trigger several rebuilds"
+    assert st["rebuilds"] == orb
+    pg, pq = s.ctx.download(), o.particles()
+    compare_state(pg, pq, by_id(pq)["mass"] * 1e-2, "rebuild carry-over", ptol=1e-9, ftol=1e-9)
+    gi, gj, gt, gf = s.ctx.interactions()
+    oi, oj, ot, of, om = o.interactions()
+    kg, fg = interaction_table(gi, gj, gt, gf)
+    ko, fo = interaction_table(oi, oj, ot, of)
+    assert np.array_equal(kg, ko)
+    assert np.sum(np.any(fo != 0, axis=1)) > 0, "some interactions must carry history"
+    scale = max(1e-30, float(np.max(np.abs(fo))))
+    assert float(np.max(np.abs(fg - fo))) <= 1e-9 * scale
+
+
+def test_deterministic(gpu_ok):
+    w = periodic_pack(n=3000)
+    runs = []
+    for _ in range(2):
+        s = gpu_for(w)
+        s.first_iteration()
+        s.run(30)
+        runs.append(s.ctx.download())
+    for k in ("rx", "vx", "fx", "momz"):
+        assert np.array_equal(runs[0][k], runs[1][k]), k
+
+
+def test_error_paths(gpu_ok):
+    w = periodic_pack(n=500)
+    ctx = native.Context()
+    ctx.set_domain(w.domain)
+    ctx.upload(w.particles)
+    with pytest.raises(native.XdemError, match="STATE"):
+        ctx.chunk_neighbors_contact(w.rcut_inc)  # before move_particles
+    ctx.move_particles()
+    ctx.chunk_neighbors_contact(w.rcut_inc)
+    with pytest.raises(native.XdemError, match="UNSUPPORTED"):
+        ctx.nbh_sphere(w.rcut_inc, symetric=False)
+    ctx.nbh_sphere(w.rcut_inc)
+    bad = native.contact_params(kn=1e5, fc=0.05, dncut=0.1)
+    with pytest.raises(native.XdemError, match="UNSUPPORTED"):
+        ctx.contact_sphere(bad, bad, 1e-5)
+    dup = dict(w.particles)
+    dup["id"] = np.zeros_like(dup["id"])
+    with pytest.raises(native.XdemError, match="unique"):
+        ctx.upload(dup)
