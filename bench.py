@@ -35,6 +35,10 @@ def bytes_model(form, n, pairs):
     return 412 * n + 52 * pairs, 140 * n + 52 * pairs
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -53,7 +57,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--poll", type=int, default=64)
-    ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps timed for the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU-baseline sample length (oracle steps)")
+    ap.add_argument("--relax-sweeps", type=int, default=150)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a device copy of 1 GiB (counter calibration for --pmc runs) and exit")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", choices=["fp64", "fp32"], default="fp64")
     args = ap.parse_args()
@@ -62,12 +69,26 @@ def main():
                                    "(the sharding design is described in DESIGN.md)"}))
         sys.exit(2)
 
+    if args.copy_probe:
+        import torch  # plumbing: a plain copy kernel of known size
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": a.numel() * 8, "copy_probe_bytes_written": b.numel() * 8,
+                          "launches": 5}))
+        return
+
     from exadem_amd import inputs, native
     from exadem_amd.sim import SphereSim, scheme
 
     t0 = time.time()
-    w = inputs.polydisperse(n=args.n)
+    w = inputs.polydisperse(n=args.n, relax_sweeps=args.relax_sweeps)
     t_gen = time.time() - t0
+    log(f"generated {w.n} spheres in {t_gen:.1f} s (max overlap {w.meta['max_overlap']:.3g})")
     form = native.XDEM_FORM_FP32 if args.form == "fp32" else native.XDEM_FORM_FP64
     sim = SphereSim(w, form=form)
     sim.first_iteration()
@@ -84,6 +105,7 @@ def main():
     t2 = time.perf_counter()
     sim.ctx.set_timing(False)
     elapsed = t2 - t1
+    log(f"timed {args.steps} steps in {elapsed:.3f} s")
     f_ms, f_cnt = sim.ctx.kernel_timing(1)
     i_ms, i_cnt = sim.ctx.kernel_timing(0)
     st_nb2 = sim.ctx.nbh_stats()
@@ -101,13 +123,19 @@ def main():
         o = po.Oracle(w.particles, w.domain, w.walls)
         sch = scheme(w)
         o.first_iteration(sch)
+        log("cpu baseline: oracle first_iteration done")
+        steps = 0
         tc = time.perf_counter()
-        o.run_steps(sch, args.cpu_steps)
-        tcpu = time.perf_counter() - tc
+        while True:
+            o.run_steps(sch, 2)
+            steps += 2
+            tcpu = time.perf_counter() - tc
+            if tcpu >= args.cpu_seconds:
+                break
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        cpu = {"value": n * args.cpu_steps / tcpu, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+        cpu = {"value": n * steps / tcpu, "unit": "particle-steps/s", "cores": threads, "kind": "port",
                "sample": f"CPU oracle (FP64 restatement of the reference sphere path, OpenMP) on the same "
-                         f"{n}-sphere input: {args.cpu_steps} steps after first_iteration, {tcpu:.1f} s; "
+                         f"{n}-sphere input: {steps} steps after first_iteration, {tcpu:.1f} s; "
                          f"{cpu_model()}"}
 
     out = {

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <unordered_set>
 
@@ -284,6 +285,8 @@ int xdem_ctx_create(int32_t device, xdem_ctx** out) {
   }
   (void)hipMemsetAsync(c->d_flags, 0, 4 * sizeof(int32_t), c->stream);
   (void)hipStreamSynchronize(c->stream);
+  c->force_per_lane = true;
+  if (const char* e = std::getenv("XDEM_FORCE_KERNEL")) c->force_per_lane = std::string(e) != "tile";
   *out = c;
   return XDEM_OK;
 }

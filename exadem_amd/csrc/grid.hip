@@ -208,7 +208,8 @@ __device__ inline void for_each_neighbour(const NbhArgs& a, int64_t x, F&& fn) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_nbh_count(NbhArgs a, uint32_t* __restrict__ cnt, uint32_t* __restrict__ kmax) {
+__global__ __launch_bounds__(256) void k_nbh_count(NbhArgs a, uint32_t* __restrict__ cnt, uint32_t* __restrict__ kmax,
+                                                   uint32_t* __restrict__ max_rows) {
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t k = 0;
   if (x < a.n) {
@@ -222,7 +223,10 @@ __global__ __launch_bounds__(256) void k_nbh_count(NbhArgs a, uint32_t* __restri
     m = t > m ? t : m;
   }
   const int64_t w = x >> 6;
-  if ((threadIdx.x & 63) == 0 && (w << 6) < a.n) kmax[w] = m;
+  if ((threadIdx.x & 63) == 0 && (w << 6) < a.n) {
+    kmax[w] = m;
+    atomicMax(max_rows, m);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_nbh_fill(NbhArgs a, const uint64_t* __restrict__ base,
@@ -287,11 +291,15 @@ int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist) {
   }
   NbhArgs a{g, rcut_inc, nbh_dist * nbh_dist, c->cell_start, live(c).xr, live(c).id, n};
   const int64_t nw = (n + kWave - 1) / kWave;
-  hipLaunchKernelGGL(k_nbh_count, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a, L.cnt, L.kmax);
+  uint32_t* max_rows = c->d_u32tmp + 2;
+  XDEM_HIP(c, hipMemsetAsync(max_rows, 0, sizeof(uint32_t), c->stream));
+  hipLaunchKernelGGL(k_nbh_count, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a, L.cnt, L.kmax, max_rows);
   int rc = scan_u64_from_u32(c, L.kmax, L.base, nw, (uint32_t)kWave);
   if (rc) return rc;
+  XDEM_HIP(c, hipMemcpyAsync(c->h_u32tmp + 2, max_rows, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
   uint64_t slots = 0;
   if ((rc = read_u64(c, L.base + nw, &slots))) return rc;
+  c->nl_max_rows = (int)c->h_u32tmp[2];
   if ((int64_t)slots > L.cap_slots) {
     const int64_t cap = (int64_t)(slots * 1.25) + 4096;
     if ((rc = grow(c, &L.word, cap)) || (rc = grow(c, &L.hx, cap)) || (rc = grow(c, &L.hy, cap)) ||

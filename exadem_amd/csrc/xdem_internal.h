@@ -164,6 +164,8 @@ struct xdem_ctx {
   bool hist_ready = false;      // history of the current list has been carried over (nbh_sphere)
   bool wl_valid = false;        // wall list exists (for the order it was built in)
   bool kr_used = false;         // a rolling stiffness was used: moment histories are live
+  int nl_max_rows = 0;          // largest per-wavefront row count of the current list
+  bool force_per_lane = false;  // select the per-lane force kernel (XDEM_FORCE_KERNEL=lane)
   double rcut_inc = 0;
   double nbh_dist = 0;
   xdem::WallList wl[2];

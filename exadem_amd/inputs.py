@@ -128,7 +128,7 @@ def deposition(n=10000, spacing=1.2, jitter=0.05, seed=1, radius=0.5, density=25
 
 def polydisperse(n=1_000_000, phi=0.55, rmin=0.5, rmax=1.0, seed_radius=2, seed_vel=3, seed_pos=2, sigma_v=0.1,
                  box_xy=200.0, density=2500.0, dt=5e-6, steps=2000, rcut_inc=0.1, gravity=(0.0, 0.0, -9.81),
-                 relax_sweeps=400, tol=None):
+                 relax_sweeps=150, tol=None):
     """Polydisperse spheres in a periodic box at solid fraction phi.
 
     Radii U[rmin, rmax] (seed_radius). At n = 1e6 the box is 200 x 200 x Lz with Lz fixed by phi;
