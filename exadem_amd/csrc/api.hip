@@ -48,18 +48,18 @@ static int alloc_pstate(xdem_ctx* c, PState& s, int64_t n) {
 #define A(ptr)                                 \
   if ((rc = dalloc(c, &s.ptr, n)) != XDEM_OK) \
     return rc;
-  A(xr) A(vm) A(w) A(q)
-  A(ax) A(ay) A(az) A(mx) A(my) A(mz) A(alx) A(aly) A(alz) A(ix) A(iy) A(iz) A(bx) A(by) A(bz) A(hom)
+  A(core)
+  A(ax) A(ay) A(az) A(mx) A(my) A(mz) A(alx) A(aly) A(alz) A(ix) A(iy) A(iz) A(bx) A(by) A(bz)
   A(id) A(type) A(group)
 #undef A
   return XDEM_OK;
 }
 
 static void free_pstate(PState& s) {
-  dfree(&s.xr); dfree(&s.vm); dfree(&s.w); dfree(&s.q);
+  dfree(&s.core);
   dfree(&s.ax); dfree(&s.ay); dfree(&s.az); dfree(&s.mx); dfree(&s.my); dfree(&s.mz);
   dfree(&s.alx); dfree(&s.aly); dfree(&s.alz); dfree(&s.ix); dfree(&s.iy); dfree(&s.iz);
-  dfree(&s.bx); dfree(&s.by); dfree(&s.bz); dfree(&s.hom);
+  dfree(&s.bx); dfree(&s.by); dfree(&s.bz);
   dfree(&s.id); dfree(&s.type); dfree(&s.group);
 }
 
@@ -97,11 +97,33 @@ int ensure_capacity(xdem_ctx* c, int64_t n) {
 }
 
 int ensure_cells(xdem_ctx* c, int64_t ncells) {
-  if (ncells <= c->cap_cells) return XDEM_OK;
-  int rc;
-  if ((rc = dalloc(c, &c->cell_count, ncells + 1))) return rc;
-  if ((rc = dalloc(c, &c->cell_start, ncells + 1))) return rc;
-  c->cap_cells = ncells;
+  if (ncells > c->cap_cells) {
+    int rc;
+    if ((rc = dalloc(c, &c->cell_count, ncells + 1))) return rc;
+    if ((rc = dalloc(c, &c->cell_start, ncells + 1))) return rc;
+    if ((rc = dalloc(c, &c->cell_rank, ncells))) return rc;
+    c->cap_cells = ncells;
+  }
+  // rank of every cell along a Morton (Z-order) curve over the grid: consecutive particles then
+  // cover compact 3-D regions, which keeps the 27-cell stencils of a block inside one L2
+  const Grid& g = c->grid;
+  auto spread = [](uint64_t v) {
+    uint64_t r = 0;
+    for (int b = 0; b < 21; b++) r |= ((v >> b) & 1ull) << (3 * b);
+    return r;
+  };
+  std::vector<std::pair<uint64_t, uint32_t>> key(ncells);
+  for (int z = 0; z < g.dims[2]; z++)
+    for (int y = 0; y < g.dims[1]; y++)
+      for (int x = 0; x < g.dims[0]; x++) {
+        const uint32_t cid = (uint32_t)(((int64_t)z * g.dims[1] + y) * g.dims[0] + x);
+        key[cid] = {spread(x) | (spread(y) << 1) | (spread(z) << 2), cid};
+      }
+  std::sort(key.begin(), key.end());
+  std::vector<uint32_t> rank(ncells);
+  for (int64_t r = 0; r < ncells; r++) rank[key[r].second] = (uint32_t)r;
+  XDEM_HIP(c, hipMemcpyAsync(c->cell_rank, rank.data(), sizeof(uint32_t) * ncells, hipMemcpyHostToDevice, c->stream));
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
   return XDEM_OK;
 }
 
@@ -240,24 +262,27 @@ int read_u64(xdem_ctx* c, const uint64_t* d, uint64_t* h) {
 __global__ void k_pack_in(int64_t n, const double* rx, const double* ry, const double* rz, const double* rad,
                           const double* vx, const double* vy, const double* vz, const double* m, const double* wx,
                           const double* wy, const double* wz, const double* qw, const double* qx, const double* qy,
-                          const double* qz, PState s) {
+                          const double* qz, const double* hom, PState s) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  s.xr[i] = {rx[i], ry[i], rz[i], rad[i]};
-  s.vm[i] = {vx[i], vy[i], vz[i], m[i]};
-  s.w[i] = {wx[i], wy[i], wz[i], 0.0};
-  s.q[i] = {qw[i], qx[i], qy[i], qz[i]};
+  Core c;
+  c.xr = {rx[i], ry[i], rz[i], rad[i]};
+  c.vm = {vx[i], vy[i], vz[i], m[i]};
+  c.w = {wx[i], wy[i], wz[i], hom[i]};
+  c.q = {qw[i], qx[i], qy[i], qz[i]};
+  s.core[i] = c;
 }
 
 __global__ void k_unpack_out(int64_t n, PState s, double* rx, double* ry, double* rz, double* rad, double* vx,
                              double* vy, double* vz, double* m, double* wx, double* wy, double* wz, double* qw,
-                             double* qx, double* qy, double* qz) {
+                             double* qx, double* qy, double* qz, double* hom) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const d4 a = s.xr[i], b = s.vm[i], w = s.w[i], q = s.q[i];
+  const Core c = s.core[i];
+  const d4 a = c.xr, b = c.vm, w = c.w, q = c.q;
   rx[i] = a.x; ry[i] = a.y; rz[i] = a.z; rad[i] = a.w;
   vx[i] = b.x; vy[i] = b.y; vz[i] = b.z; m[i] = b.w;
-  wx[i] = w.x; wy[i] = w.y; wz[i] = w.z;
+  wx[i] = w.x; wy[i] = w.y; wz[i] = w.z; hom[i] = w.w;
   qw[i] = q.x; qx[i] = q.y; qy[i] = q.z; qz[i] = q.w;
 }
 
@@ -301,7 +326,8 @@ void xdem_ctx_destroy(xdem_ctx* c) {
     free_wl(c->wl[k]);
     for (auto& e : c->timers[k].ev) (void)hipEventDestroy(e);
   }
-  dfree(&c->cell_of); dfree(&c->cell_count); dfree(&c->cell_start); dfree(&c->perm); dfree(&c->sort_key);
+  dfree(&c->cell_of); dfree(&c->cell_count); dfree(&c->cell_start); dfree(&c->cell_rank); dfree(&c->perm);
+  dfree(&c->sort_key);
   dfree(&c->d_flags); dfree(&c->d_u32tmp); dfree(&c->d_walls);
   if (c->scan_tmp) (void)hipFree(c->scan_tmp);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
@@ -410,9 +436,9 @@ int xdem_particles_upload(xdem_ctx* c, const xdem_particles* p) {
     dtmp.push_back(d);
     return d;
   };
-  double* t[15] = {up(p->rx), up(p->ry), up(p->rz), up(p->radius), up(or0(p->vx)), up(or0(p->vy)), up(or0(p->vz)),
+  double* t[16] = {up(p->rx), up(p->ry), up(p->rz), up(p->radius), up(or0(p->vx)), up(or0(p->vy)), up(or0(p->vz)),
                    up(p->mass), up(or0(p->vrotx)), up(or0(p->vroty)), up(or0(p->vrotz)), up(or1(p->qw)),
-                   up(or0(p->qx)), up(or0(p->qy)), up(or0(p->qz))};
+                   up(or0(p->qx)), up(or0(p->qy)), up(or0(p->qz)), up(or1(p->homothety))};
   for (auto* d : t)
     if (!d) {
       for (auto* x : dtmp) (void)hipFree(x);
@@ -420,7 +446,7 @@ int xdem_particles_upload(xdem_ctx* c, const xdem_particles* p) {
     }
   if (n > 0)
     hipLaunchKernelGGL(k_pack_in, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, t[0], t[1], t[2], t[3], t[4],
-                       t[5], t[6], t[7], t[8], t[9], t[10], t[11], t[12], t[13], t[14], s);
+                       t[5], t[6], t[7], t[8], t[9], t[10], t[11], t[12], t[13], t[14], t[15], s);
   auto cp = [&](double* d, const double* h) {
     return hipMemcpyAsync(d, h, sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
   };
@@ -439,7 +465,6 @@ int xdem_particles_upload(xdem_ctx* c, const xdem_particles* p) {
   XDEM_HIP(c, cp(s.bx, p->rx));
   XDEM_HIP(c, cp(s.by, p->ry));
   XDEM_HIP(c, cp(s.bz, p->rz));
-  XDEM_HIP(c, cp(s.hom, or1(p->homothety)));
   XDEM_HIP(c, hipMemcpyAsync(s.id, p->id, sizeof(uint64_t) * n, hipMemcpyHostToDevice, c->stream));
   std::vector<uint32_t> z32(n, 0);
   XDEM_HIP(c, hipMemcpyAsync(s.type, p->type ? p->type : z32.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice,
@@ -474,8 +499,8 @@ int xdem_particles_download(xdem_ctx* c, xdem_particles* p) {
     dtmp.push_back(d);
     return d;
   };
-  double* t[15];
-  for (int k = 0; k < 15; k++) {
+  double* t[16];
+  for (int k = 0; k < 16; k++) {
     t[k] = dt();
     if (!t[k]) {
       for (auto* x : dtmp) (void)hipFree(x);
@@ -483,10 +508,10 @@ int xdem_particles_download(xdem_ctx* c, xdem_particles* p) {
     }
   }
   hipLaunchKernelGGL(k_unpack_out, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, s, t[0], t[1], t[2], t[3],
-                     t[4], t[5], t[6], t[7], t[8], t[9], t[10], t[11], t[12], t[13], t[14]);
-  double* hosts[15] = {p->rx, p->ry, p->rz, p->radius, p->vx, p->vy, p->vz, p->mass,
-                       p->vrotx, p->vroty, p->vrotz, p->qw, p->qx, p->qy, p->qz};
-  for (int k = 0; k < 15; k++)
+                     t[4], t[5], t[6], t[7], t[8], t[9], t[10], t[11], t[12], t[13], t[14], t[15]);
+  double* hosts[16] = {p->rx, p->ry, p->rz, p->radius, p->vx, p->vy, p->vz, p->mass,
+                       p->vrotx, p->vroty, p->vrotz, p->qw, p->qx, p->qy, p->qz, p->homothety};
+  for (int k = 0; k < 16; k++)
     if (hosts[k]) XDEM_HIP(c, hipMemcpyAsync(hosts[k], t[k], sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   auto cp = [&](double* h, const double* d) -> hipError_t {
     if (!h) return hipSuccess;
@@ -504,7 +529,6 @@ int xdem_particles_download(xdem_ctx* c, xdem_particles* p) {
   XDEM_HIP(c, cp(p->inx, s.ix));
   XDEM_HIP(c, cp(p->iny, s.iy));
   XDEM_HIP(c, cp(p->inz, s.iz));
-  XDEM_HIP(c, cp(p->homothety, s.hom));
   if (p->id) XDEM_HIP(c, hipMemcpyAsync(p->id, s.id, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, c->stream));
   if (p->type) XDEM_HIP(c, hipMemcpyAsync(p->type, s.type, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
   if (p->group)

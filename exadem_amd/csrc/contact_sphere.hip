@@ -59,6 +59,34 @@ __device__ inline int64_t xcd_block(int64_t b, int64_t nb) {
   return x * q + (x < r ? x : r) + k;
 }
 
+// One entry of a particle's list with the partner data it needs, loaded ahead of use.
+struct Entry {
+  uint32_t wd;
+  d4 p;     // partner xr
+  d4 v;     // partner vm
+  d4 w;     // partner w
+  V3 ft;    // friction history (zero when the active bit is clear)
+  V3 mint;  // rolling-moment history
+};
+
+template <bool KR>
+__device__ inline void fetch_entry(const ForceArgs& a, uint64_t sl, uint32_t wd, Entry& e) {
+  const Core* __restrict__ core = a.s.core;
+  const uint32_t y = wd & kIdxMask;
+  e.wd = wd;
+  e.p = core[y].xr;
+  e.v = core[y].vm;
+  e.w = core[y].w;
+  if (wd & kActiveBit) {
+    e.ft = {a.nl.hx[sl], a.nl.hy[sl], a.nl.hz[sl]};
+    if (KR) e.mint = {a.nl.mhx[sl], a.nl.mhy[sl], a.nl.mhz[sl]};
+    else e.mint = {0.0, 0.0, 0.0};
+  } else {
+    e.ft = {0.0, 0.0, 0.0};
+    e.mint = {0.0, 0.0, 0.0};
+  }
+}
+
 template <bool FUSED, bool KR, bool WALLS, bool GUARD>
 __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
   if (GUARD) {
@@ -75,13 +103,19 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
   const int64_t x = blk * blockDim.x + threadIdx.x;
   if (x >= a.n) return;
   const PState& s = a.s;
-  const d4 px = s.xr[x];
-  const d4 pv = s.vm[x];
-  const d4 pw = s.w[x];
-  const V3 rx{px.x, px.y, px.z};
-  const V3 vx{pv.x, pv.y, pv.z};
-  const V3 wx{pw.x, pw.y, pw.z};
-  const double mx = pv.w, Rx = px.w;
+  uint32_t* __restrict__ word = a.nl.word;
+  const uint32_t cnt = a.nl.cnt[x];
+  const uint64_t b = a.nl.base[x >> 6] + (uint64_t)(x & 63);
+  // software pipeline: entry k+1 (record + history) and word k+2 are in flight while k is computed
+  Entry nxt;
+  uint32_t wd2 = 0;
+  if (cnt > 0) fetch_entry<KR>(a, b, word[b], nxt);
+  if (cnt > 1) wd2 = word[b + kWave];
+  const Core me = s.core[x];
+  const V3 rx{me.xr.x, me.xr.y, me.xr.z};
+  const V3 vx{me.vm.x, me.vm.y, me.vm.z};
+  const V3 wx{me.w.x, me.w.y, me.w.z};
+  const double mx = me.vm.w, Rx = me.xr.w;
   V3 F, M;
   if (FUSED) {
     F = {0.0, 0.0, 0.0};
@@ -91,25 +125,26 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
     F = {s.ax[x], s.ay[x], s.az[x]};
     M = {s.mx[x], s.my[x], s.mz[x]};
   }
-  const uint32_t cnt = a.nl.cnt[x];
-  const uint64_t b = a.nl.base[x >> 6] + (uint64_t)(x & 63);
   for (uint32_t k = 0; k < cnt; k++) {
     const uint64_t sl = b + (uint64_t)k * kWave;
-    const uint32_t wd = a.nl.word[sl];
-    const uint32_t y = wd & kIdxMask;
+    Entry e = nxt;
+    if (k + 1 < cnt) {
+      fetch_entry<KR>(a, sl + kWave, wd2, nxt);
+      if (k + 2 < cnt) wd2 = word[sl + 2 * kWave];
+    }
+    const uint32_t wd = e.wd;
     const bool owner = (wd & kOwnerBit) != 0;
-    const d4 py = s.xr[y];
     V3 ri, rj;
     double Ri, Rj;
     if (owner) {
       const V3 sh = shift_vec(a.g, (int)((wd >> kShiftShift) & 31u));
       ri = rx;
       Ri = Rx;
-      rj = {py.x + sh.x, py.y + sh.y, py.z + sh.z};
-      Rj = py.w;
+      rj = {e.p.x + sh.x, e.p.y + sh.y, e.p.z + sh.z};
+      Rj = e.p.w;
     } else {
-      ri = {py.x, py.y, py.z};
-      Ri = py.w;
+      ri = {e.p.x, e.p.y, e.p.z};
+      Ri = e.p.w;
       rj = rx;
       Rj = Rx;
     }
@@ -121,28 +156,27 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
       const double inv = 1.0 / dnorm;
       const V3 nrm = dist * inv;
       const V3 cpos = ri - nrm * (Ri + 0.5 * dn);
-      const d4 qv = s.vm[y];
-      const d4 qw = s.w[y];
-      const V3 vy{qv.x, qv.y, qv.z}, wy{qw.x, qw.y, qw.z};
-      const double my = qv.w;
+      const V3 vy{e.v.x, e.v.y, e.v.z}, wy{e.w.x, e.w.y, e.w.z};
+      const double my = e.v.w;
       const double mi = owner ? mx : my, mj = owner ? my : mx;
       const double meff = (mi * mj) / (mi + mj);
-      V3 ft{a.nl.hx[sl], a.nl.hy[sl], a.nl.hz[sl]};
-      V3 mint{0.0, 0.0, 0.0};
-      if (KR) mint = {a.nl.mhx[sl], a.nl.mhy[sl], a.nl.mhz[sl]};
+      V3 ft = e.ft;
+      V3 mint = e.mint;
       const V3 f = owner ? hooke_force<KR>(dn, nrm, a.dt, a.cp, meff, ft, mint, cpos, ri, vx, wx, rj, vy, wy)
                          : hooke_force<KR>(dn, nrm, a.dt, a.cp, meff, ft, mint, cpos, ri, vy, wy, rj, vx, wx);
-      a.nl.hx[sl] = ft.x;
-      a.nl.hy[sl] = ft.y;
-      a.nl.hz[sl] = ft.z;
-      if (KR) {
-        a.nl.mhx[sl] = mint.x;
-        a.nl.mhy[sl] = mint.y;
-        a.nl.mhz[sl] = mint.z;
-      }
       const bool act = !(ft.x == 0 && ft.y == 0 && ft.z == 0 && mint.x == 0 && mint.y == 0 && mint.z == 0);
+      if (act || (wd & kActiveBit)) {
+        a.nl.hx[sl] = ft.x;
+        a.nl.hy[sl] = ft.y;
+        a.nl.hz[sl] = ft.z;
+        if (KR) {
+          a.nl.mhx[sl] = mint.x;
+          a.nl.mhy[sl] = mint.y;
+          a.nl.mhz[sl] = mint.z;
+        }
+      }
       const uint32_t nwd = act ? (wd | kActiveBit) : (wd & ~kActiveBit);
-      if (nwd != wd) a.nl.word[sl] = nwd;
+      if (nwd != wd) word[sl] = nwd;
       if (owner) {
         F = F + f;
         M = M + (cross(cpos - rx, f) + mint);
@@ -160,7 +194,7 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
         a.nl.mhy[sl] = 0.0;
         a.nl.mhz[sl] = 0.0;
       }
-      a.nl.word[sl] = wd & ~kActiveBit;
+      word[sl] = wd & ~kActiveBit;
     }
   }
   if (WALLS) {
@@ -200,7 +234,7 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
     s.mx[x] = M.x;
     s.my[x] = M.y;
     s.mz[x] = M.z;
-    const d4 q = s.q[x];
+    const d4 q = s.core[x].q;
     const V3 al = angular_accel_d(Q4f{q.x, q.y, q.z, q.w}, M, wx, V3{s.ix[x], s.iy[x], s.iz[x]});
     s.alx[x] = al.x;
     s.aly[x] = al.y;
@@ -268,12 +302,12 @@ __global__ __launch_bounds__(64) void k_force_fp64_tile(ForceArgs a) {
   const bool live = x < a.n;
   const PState& s = a.s;
   uint32_t* __restrict__ word = a.nl.word;
-  const d4* __restrict__ xr = s.xr;
+  const Core* __restrict__ core = s.core;
   const int K = (int)a.nl.kmax[tile];
   const uint64_t tb = a.nl.base[tile];
   const uint32_t cnt = live ? a.nl.cnt[x] : 0u;
   d4 px = {0, 0, 0, 0};
-  if (live) px = xr[x];
+  if (live) px = core[x].xr;
   const V3 rx{px.x, px.y, px.z};
   const double Rx = px.w;
   // ---- pass 1: overlap test of every entry
@@ -286,7 +320,7 @@ __global__ __launch_bounds__(64) void k_force_fp64_tile(ForceArgs a) {
       const uint32_t wd = word[sl];
       const uint32_t y = wd & kIdxMask;
       const bool owner = (wd & kOwnerBit) != 0;
-      const d4 py = xr[y];
+      const d4 py = core[y].xr;
       V3 ri, rj;
       double Rs;
       if (owner) {
@@ -330,8 +364,8 @@ __global__ __launch_bounds__(64) void k_force_fp64_tile(ForceArgs a) {
   d4 pv = {0, 0, 0, 1};
   d4 pw = {0, 0, 0, 0};
   if (live) {
-    pv = s.vm[x];
-    pw = s.w[x];
+    pv = s.core[x].vm;
+    pw = s.core[x].w;
   }
   const V3 vx{pv.x, pv.y, pv.z};
   const V3 wx{pw.x, pw.y, pw.z};
@@ -362,12 +396,12 @@ __global__ __launch_bounds__(64) void k_force_fp64_tile(ForceArgs a) {
       const uint32_t wd = word[sl];
       const uint32_t y = wd & kIdxMask;
       const bool owner = (wd & kOwnerBit) != 0;
-      const d4 qa = xr[xo];
-      const d4 qb = xr[y];
-      const d4 va = s.vm[xo];
-      const d4 vb = s.vm[y];
-      const d4 wa = s.w[xo];
-      const d4 wb = s.w[y];
+      const d4 qa = core[xo].xr;
+      const d4 qb = core[y].xr;
+      const d4 va = s.core[xo].vm;
+      const d4 vb = s.core[y].vm;
+      const d4 wa = s.core[xo].w;
+      const d4 wb = s.core[y].w;
       V3 ri, rj, vi, vj, wi, wj;
       double Ri, Rj, mi, mj;
       if (owner) {
@@ -478,7 +512,7 @@ __global__ __launch_bounds__(64) void k_force_fp64_tile(ForceArgs a) {
     s.mx[x] = M.x;
     s.my[x] = M.y;
     s.mz[x] = M.z;
-    const d4 q = s.q[x];
+    const d4 q = s.core[x].q;
     const V3 al = angular_accel_d(Q4f{q.x, q.y, q.z, q.w}, M, wx, V3{s.ix[x], s.iy[x], s.iz[x]});
     s.alx[x] = al.x;
     s.aly[x] = al.y;

@@ -5,8 +5,8 @@
 //   chunk_neighbors_contact src/grid/chunk_neighbors_contact.cpp:45-57 (DEM filter)
 //   nbh_sphere              src/sphere/nbh_sphere.cu:64-309 (sym mode owner choice, history by key)
 //
-// Binning is a counting sort (cell histogram, exclusive scan, scatter), followed by a per-cell
-// insertion sort on particle id so that the order is deterministic. Neighbour lists are built in
+// Binning is a counting sort over the cells' Morton ranks (histogram, exclusive scan, scatter),
+// followed by a per-cell insertion sort on particle id so that the order is deterministic. Neighbour lists are built in
 // two passes (count, fill) over the 27-cell stencil into a per-wavefront ELL block; the history
 // carry-over looks each new entry up among the owner's previous entries (owner id, partner id key).
 #include <algorithm>
@@ -23,11 +23,11 @@ __device__ inline int cell_coord(double x, double lo, double h, int dims) {
   return k;
 }
 
-__global__ __launch_bounds__(256) void k_wrap_bin(int64_t n, PState s, Grid g, uint32_t* __restrict__ cell_of,
-                                                  uint32_t* __restrict__ count) {
+__global__ __launch_bounds__(256) void k_wrap_bin(int64_t n, PState s, Grid g, const uint32_t* __restrict__ rank,
+                                                  uint32_t* __restrict__ cell_of, uint32_t* __restrict__ count) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  d4 xr = s.xr[i];
+  d4 xr = s.core[i].xr;
   double c[3] = {xr.x, xr.y, xr.z};
   int ci[3];
 #pragma unroll
@@ -41,8 +41,8 @@ __global__ __launch_bounds__(256) void k_wrap_bin(int64_t n, PState s, Grid g, u
   xr.x = c[0];
   xr.y = c[1];
   xr.z = c[2];
-  s.xr[i] = xr;
-  const uint32_t cell = (uint32_t)(((int64_t)ci[2] * g.dims[1] + ci[1]) * g.dims[0] + ci[0]);
+  s.core[i].xr = xr;
+  const uint32_t cell = rank[((int64_t)ci[2] * g.dims[1] + ci[1]) * g.dims[0] + ci[0]];
   cell_of[i] = cell;
   atomicAdd(&count[cell], 1u);
 }
@@ -79,16 +79,13 @@ __global__ __launch_bounds__(256) void k_gather(int64_t n, const uint32_t* __res
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t o = perm[i];
-  const d4 xr = src.xr[o];
-  dst.xr[i] = xr;
-  dst.vm[i] = src.vm[o];
-  dst.w[i] = src.w[o];
-  dst.q[i] = src.q[o];
+  const Core c = src.core[o];
+  const d4 xr = c.xr;
+  dst.core[i] = c;
   dst.ax[i] = src.ax[o]; dst.ay[i] = src.ay[o]; dst.az[i] = src.az[o];
   dst.mx[i] = src.mx[o]; dst.my[i] = src.my[o]; dst.mz[i] = src.mz[o];
   dst.alx[i] = src.alx[o]; dst.aly[i] = src.aly[o]; dst.alz[i] = src.alz[o];
   dst.ix[i] = src.ix[o]; dst.iy[i] = src.iy[o]; dst.iz[i] = src.iz[o];
-  dst.hom[i] = src.hom[o];
   dst.id[i] = src.id[o];
   dst.type[i] = src.type[o];
   dst.group[i] = src.group[o];
@@ -98,7 +95,7 @@ __global__ __launch_bounds__(256) void k_gather(int64_t n, const uint32_t* __res
 __global__ __launch_bounds__(256) void k_max_radius(int64_t n, PState s, uint64_t* __restrict__ out_bits) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double r = 0.0;
-  if (i < n) r = s.xr[i].w;
+  if (i < n) r = s.core[i].xr.w;
   // radii are positive: the IEEE bit pattern orders like the value
   unsigned long long bits = (unsigned long long)__double_as_longlong(r);
   for (int o = 32; o > 0; o >>= 1) {
@@ -117,8 +114,8 @@ int do_move_particles(xdem_ctx* c) {
   }
   XDEM_HIP(c, hipMemsetAsync(c->cell_count, 0, sizeof(uint32_t) * (g.ncells + 1), c->stream));
   PState src = live(c);
-  hipLaunchKernelGGL(k_wrap_bin, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, src, g, c->cell_of,
-                     c->cell_count);
+  hipLaunchKernelGGL(k_wrap_bin, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, src, g, c->cell_rank,
+                     c->cell_of, c->cell_count);
   int rc = scan_u32(c, c->cell_count, c->cell_start, g.ncells);
   if (rc) return rc;
   XDEM_HIP(c, hipMemsetAsync(c->cell_count, 0, sizeof(uint32_t) * (g.ncells + 1), c->stream));
@@ -143,8 +140,9 @@ struct NbhArgs {
   Grid g;
   double rcut_inc;
   double rcut2;
-  const uint32_t* cell_start;
-  const d4* xr;
+  const uint32_t* cell_start;  // indexed by Morton rank
+  const uint32_t* rank;
+  const Core* core;
   const uint64_t* id;
   int64_t n;
 };
@@ -153,7 +151,7 @@ struct NbhArgs {
 // particles of each neighbour cell in cell order. fn(y, code) is called for each accepted partner.
 template <typename F>
 __device__ inline void for_each_neighbour(const NbhArgs& a, int64_t x, F&& fn) {
-  const d4 px = a.xr[x];
+  const d4 px = a.core[x].xr;
   const int cx = cell_coord(px.x, a.g.lo[0], a.g.h[0], a.g.dims[0]);
   const int cy = cell_coord(px.y, a.g.lo[1], a.g.h[1], a.g.dims[1]);
   const int cz = cell_coord(px.z, a.g.lo[2], a.g.h[2], a.g.dims[2]);
@@ -192,11 +190,11 @@ __device__ inline void for_each_neighbour(const NbhArgs& a, int64_t x, F&& fn) {
         }
         const int code = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
         const V3 sh = shift_vec(a.g, code);
-        const int64_t cb = ((int64_t)nz * a.g.dims[1] + ny) * a.g.dims[0] + nx;
+        const uint32_t cb = a.rank[((int64_t)nz * a.g.dims[1] + ny) * a.g.dims[0] + nx];
         const uint32_t b = a.cell_start[cb], e = a.cell_start[cb + 1];
         for (uint32_t y = b; y < e; y++) {
           if ((int64_t)y == x && code == kNoShift) continue;
-          const d4 py = a.xr[y];
+          const d4 py = a.core[y].xr;
           const V3 img{py.x + sh.x, py.y + sh.y, py.z + sh.z};
           const V3 d{px.x - img.x, px.y - img.y, px.z - img.z};
           const double d2 = dot(d, d);
@@ -289,7 +287,7 @@ int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist) {
     c->nl_valid = true;
     return XDEM_OK;
   }
-  NbhArgs a{g, rcut_inc, nbh_dist * nbh_dist, c->cell_start, live(c).xr, live(c).id, n};
+  NbhArgs a{g, rcut_inc, nbh_dist * nbh_dist, c->cell_start, c->cell_rank, live(c).core, live(c).id, n};
   const int64_t nw = (n + kWave - 1) / kWave;
   uint32_t* max_rows = c->d_u32tmp + 2;
   XDEM_HIP(c, hipMemsetAsync(max_rows, 0, sizeof(uint32_t), c->stream));
@@ -375,7 +373,7 @@ __global__ __launch_bounds__(256) void k_wall_carry(int64_t n, int nw, const Sur
                                                     const uint32_t* __restrict__ perm, int have_old, int64_t stride) {
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= n) return;
-  const d4 xr = s.xr[x];
+  const d4 xr = s.core[x].xr;
   const V3 p{xr.x, xr.y, xr.z};
   const double rc = xr.w + rcut_inc;
   uint8_t mask = 0;

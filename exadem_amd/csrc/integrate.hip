@@ -44,61 +44,94 @@ __device__ inline V3 angular_accel(Q4 Q, V3 mom, V3 vrot, V3 I) {
   return qrot(Q, dom);
 }
 
+// The 128-byte records of a block's 256 particles are moved between HBM and LDS with fully
+// coalesced 16-byte accesses (a record-per-lane access pattern would touch 64 lines per
+// instruction). LDS rows are padded to 144 bytes to spread the per-lane reads over the banks.
+constexpr int kStageBS = 256;
+constexpr int kStageRow = 9;  // uint4 per record row in LDS (8 used + 1 pad)
+
+__device__ inline void stage_in(const Core* __restrict__ core, int64_t first, int cnt, uint4* lds) {
+  const uint4* src = reinterpret_cast<const uint4*>(core + first);
+  for (int j = threadIdx.x; j < cnt * 8; j += kStageBS) lds[(j >> 3) * kStageRow + (j & 7)] = src[j];
+}
+__device__ inline void stage_out(Core* __restrict__ core, int64_t first, int cnt, const uint4* lds) {
+  uint4* dst = reinterpret_cast<uint4*>(core + first);
+  for (int j = threadIdx.x; j < cnt * 8; j += kStageBS) dst[j] = lds[(j >> 3) * kStageRow + (j & 7)];
+}
+
+// LDS rows are only 16-byte aligned: move the 32-byte fields with memcpy
+__device__ inline d4 lds_get(const uint4* p) {
+  d4 v;
+  __builtin_memcpy(&v, p, sizeof(d4));
+  return v;
+}
+__device__ inline void lds_put(uint4* p, const d4& v) { __builtin_memcpy(p, &v, sizeof(d4)); }
+
 template <bool KICK, bool DISPL, bool GUARD>
-__global__ __launch_bounds__(256) void k_prolog(int64_t n, PState s, double dt, double kdt_2, double thr2,
-                                                int32_t* __restrict__ flags) {
+__global__ __launch_bounds__(kStageBS) void k_prolog(int64_t n, PState s, double dt, double kdt_2, double thr2,
+                                                     int32_t* __restrict__ flags) {
   if (GUARD && flags[1]) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double dt_2 = dt * 0.5;
-  const double dt2_2 = dt * dt * 0.5;
-  d4 xr = s.xr[i];
-  d4 vm = s.vm[i];
-  d4 w = s.w[i];
-  const d4 qq = s.q[i];
-  const double ax = s.ax[i], ay = s.ay[i], az = s.az[i];
-  const double lx = s.alx[i], ly = s.aly[i], lz = s.alz[i];
-  if (KICK) {
-    // epilog of the previous step: angular velocity, then push_f_v
-    w.x = w.x + lx * kdt_2;
-    w.y = w.y + ly * kdt_2;
-    w.z = w.z + lz * kdt_2;
-    vm.x = vm.x + ax * kdt_2;
-    vm.y = vm.y + ay * kdt_2;
-    vm.z = vm.z + az * kdt_2;
+  __shared__ uint4 lds[kStageBS * kStageRow];
+  const int64_t first = (int64_t)blockIdx.x * kStageBS;
+  const int cnt = (int)((n - first) < kStageBS ? (n - first) : kStageBS);
+  stage_in(s.core, first, cnt, lds);
+  __syncthreads();
+  const int t = threadIdx.x;
+  const int64_t i = first + t;
+  if (t < cnt) {
+    const double dt_2 = dt * 0.5;
+    const double dt2_2 = dt * dt * 0.5;
+    uint4* row = lds + t * kStageRow;
+    d4 xr = lds_get(row);
+    d4 vm = lds_get(row + 2);
+    d4 w = lds_get(row + 4);
+    const d4 qq = lds_get(row + 6);
+    const double ax = s.ax[i], ay = s.ay[i], az = s.az[i];
+    const double lx = s.alx[i], ly = s.aly[i], lz = s.alz[i];
+    if (KICK) {
+      // epilog of the previous step: angular velocity, then push_f_v
+      w.x = w.x + lx * kdt_2;
+      w.y = w.y + ly * kdt_2;
+      w.z = w.z + lz * kdt_2;
+      vm.x = vm.x + ax * kdt_2;
+      vm.y = vm.y + ay * kdt_2;
+      vm.z = vm.z + az * kdt_2;
+    }
+    // push_f_v_r, push_f_v
+    xr.x = xr.x + (vm.x * dt + ax * dt2_2);
+    xr.y = xr.y + (vm.y * dt + ay * dt2_2);
+    xr.z = xr.z + (vm.z * dt + az * dt2_2);
+    vm.x = vm.x + ax * dt_2;
+    vm.y = vm.y + ay * dt_2;
+    vm.z = vm.z + az * dt_2;
+    // push_to_quaternion
+    Q4 q{qq.x, qq.y, qq.z, qq.w};
+    const Q4 d = qdot(q, V3{w.x, w.y, w.z});
+    q = {q.w + d.w * dt, q.x + d.x * dt, q.y + d.y * dt, q.z + d.z * dt};
+    q = qnormalize(q);
+    w.x = w.x + dt_2 * lx;
+    w.y = w.y + dt_2 * ly;
+    w.z = w.z + dt_2 * lz;
+    lds_put(row, xr);
+    lds_put(row + 2, vm);
+    lds_put(row + 4, w);
+    lds_put(row + 6, d4{q.w, q.x, q.y, q.z});
+    if (DISPL) {
+      const double dx = xr.x - s.bx[i], dy = xr.y - s.by[i], dz = xr.z - s.bz[i];
+      if (dx * dx + dy * dy + dz * dz >= thr2) flags[0] = 1;
+    }
   }
-  // push_f_v_r, push_f_v
-  xr.x = xr.x + (vm.x * dt + ax * dt2_2);
-  xr.y = xr.y + (vm.y * dt + ay * dt2_2);
-  xr.z = xr.z + (vm.z * dt + az * dt2_2);
-  vm.x = vm.x + ax * dt_2;
-  vm.y = vm.y + ay * dt_2;
-  vm.z = vm.z + az * dt_2;
-  // push_to_quaternion
-  Q4 q{qq.x, qq.y, qq.z, qq.w};
-  const Q4 d = qdot(q, V3{w.x, w.y, w.z});
-  q = {q.w + d.w * dt, q.x + d.x * dt, q.y + d.y * dt, q.z + d.z * dt};
-  q = qnormalize(q);
-  w.x = w.x + dt_2 * lx;
-  w.y = w.y + dt_2 * ly;
-  w.z = w.z + dt_2 * lz;
-  s.xr[i] = xr;
-  s.vm[i] = vm;
-  s.w[i] = w;
-  s.q[i] = {q.w, q.x, q.y, q.z};
-  if (DISPL) {
-    const double dx = xr.x - s.bx[i], dy = xr.y - s.by[i], dz = xr.z - s.bz[i];
-    if (dx * dx + dy * dy + dz * dz >= thr2) flags[0] = 1;
-  }
+  __syncthreads();
+  stage_out(s.core, first, cnt, lds);
 }
 
 __global__ __launch_bounds__(256) void k_epilog(int64_t n, PState s, double dt) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double dt_2 = dt * 0.5;
-  const d4 qq = s.q[i];
-  d4 w = s.w[i];
-  d4 vm = s.vm[i];
+  const d4 qq = s.core[i].q;
+  d4 w = s.core[i].w;
+  d4 vm = s.core[i].vm;
   const V3 al = angular_accel(Q4{qq.x, qq.y, qq.z, qq.w}, V3{s.mx[i], s.my[i], s.mz[i]}, V3{w.x, w.y, w.z},
                               V3{s.ix[i], s.iy[i], s.iz[i]});
   s.alx[i] = al.x;
@@ -110,24 +143,24 @@ __global__ __launch_bounds__(256) void k_epilog(int64_t n, PState s, double dt) 
   vm.x = vm.x + s.ax[i] * dt_2;
   vm.y = vm.y + s.ay[i] * dt_2;
   vm.z = vm.z + s.az[i] * dt_2;
-  s.w[i] = w;
-  s.vm[i] = vm;
+  s.core[i].w = w;
+  s.core[i].vm = vm;
 }
 
 __global__ __launch_bounds__(256) void k_kick(int64_t n, PState s, double dt) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double dt_2 = dt * 0.5;
-  d4 w = s.w[i];
-  d4 vm = s.vm[i];
+  d4 w = s.core[i].w;
+  d4 vm = s.core[i].vm;
   w.x = w.x + s.alx[i] * dt_2;
   w.y = w.y + s.aly[i] * dt_2;
   w.z = w.z + s.alz[i] * dt_2;
   vm.x = vm.x + s.ax[i] * dt_2;
   vm.y = vm.y + s.ay[i] * dt_2;
   vm.z = vm.z + s.az[i] * dt_2;
-  s.w[i] = w;
-  s.vm[i] = vm;
+  s.core[i].w = w;
+  s.core[i].vm = vm;
 }
 
 __global__ __launch_bounds__(256) void k_reset(int64_t n, PState s) {
@@ -141,7 +174,7 @@ __global__ __launch_bounds__(256) void k_reset(int64_t n, PState s) {
 __global__ __launch_bounds__(256) void k_gravity(int64_t n, PState s, double gx, double gy, double gz) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double m = s.vm[i].w;
+  const double m = s.core[i].vm.w;
   s.ax[i] = s.ax[i] + gx * m;
   s.ay[i] = s.ay[i] + gy * m;
   s.az[i] = s.az[i] + gz * m;
@@ -151,7 +184,7 @@ __global__ __launch_bounds__(256) void k_gravity(int64_t n, PState s, double gx,
 __global__ __launch_bounds__(256) void k_to_accel(int64_t n, PState s) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double m = s.vm[i].w;
+  const double m = s.core[i].vm.w;
   s.ax[i] = s.ax[i] / m;
   s.ay[i] = s.ay[i] / m;
   s.az[i] = s.az[i] / m;
@@ -160,7 +193,7 @@ __global__ __launch_bounds__(256) void k_to_accel(int64_t n, PState s) {
 __global__ __launch_bounds__(256) void k_displ(int64_t n, PState s, double thr2, int32_t* __restrict__ flags) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const d4 xr = s.xr[i];
+  const d4 xr = s.core[i].xr;
   const double dx = xr.x - s.bx[i], dy = xr.y - s.by[i], dz = xr.z - s.bz[i];
   if (dx * dx + dy * dy + dz * dz >= thr2) flags[0] = 1;
 }
@@ -170,6 +203,7 @@ __global__ __launch_bounds__(256) void k_displ(int64_t n, PState s, double thr2,
     hipLaunchKernelGGL(kern, dim3(grid_blocks(c->n, 256)), dim3(256), 0, c->stream, __VA_ARGS__); \
     XDEM_HIP(c, hipGetLastError());                                                                 \
   } while (0)
+static_assert(kStageBS == 256, "prolog launches use 256-thread blocks");
 
 int launch_prolog(xdem_ctx* c, double dt, bool with_kick, double kick_dt, bool displ, double thr2, bool guarded) {
   PState s = live(c);

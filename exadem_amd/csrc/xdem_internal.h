@@ -1,10 +1,9 @@
 // xdem_internal.h — device data layout, math helpers and the context of libxdem_hip.so.
 //
-// Layout in HBM (all arrays in cell-sorted order, rebuilt at every neighbour rebuild):
-//   xr[i]    = (rx, ry, rz, radius)        gathered by neighbours  -> packed 32 B
-//   vm[i]    = (vx, vy, vz, mass)          gathered by neighbours  -> packed 32 B
-//   w[i]     = (vrotx, vroty, vrotz, 0)    gathered by neighbours  -> packed 32 B
-//   q[i]     = (qw, qx, qy, qz)            own particle only
+// Layout in HBM (all arrays in cell order along a Morton curve, rebuilt at every neighbour rebuild):
+//   core[i]  one 128-byte record = one L2 line, everything a neighbour or the integrator touches:
+//            xr = (rx, ry, rz, radius), vm = (vx, vy, vz, mass), w = (vrotx, vroty, vrotz, homothety),
+//            q = (qw, qx, qy, qz)
 //   a?/mom?/alpha?/in?/bk?  3 separate double arrays each (own particle only, coalesced)
 // Neighbour list: per wavefront of 64 particles, an ELL block of K_w x 64 words
 // (slot = base[w] + k*64 + lane), one 32-bit word per entry:
@@ -76,18 +75,22 @@ __host__ __device__ inline V3 shift_vec(const Grid& g, int code) {
   return {sx * g.L[0], sy * g.L[1], sz * g.L[2]};
 }
 
+struct alignas(128) Core {
+  d4 xr;
+  d4 vm;
+  d4 w;
+  d4 q;
+};
+static_assert(sizeof(Core) == 128, "one particle record per 128-byte line");
+
 // Device particle state (one buffer set; two sets are kept and swapped at each reorder).
 struct PState {
-  d4* xr;
-  d4* vm;
-  d4* w;
-  d4* q;
+  Core* core;
   double *ax, *ay, *az;
   double *mx, *my, *mz;
   double *alx, *aly, *alz;
   double *ix, *iy, *iz;
   double *bx, *by, *bz;
-  double* hom;
   uint64_t* id;
   uint32_t* type;
   uint32_t* group;
@@ -147,7 +150,8 @@ struct xdem_ctx {
   // binning scratch
   uint32_t* cell_of = nullptr;      // [cap]
   uint32_t* cell_count = nullptr;   // [ncells]
-  uint32_t* cell_start = nullptr;   // [ncells+1]
+  uint32_t* cell_start = nullptr;   // [ncells+1], indexed by Morton rank of the cell
+  uint32_t* cell_rank = nullptr;    // [ncells] rank of each cell along the Morton curve
   uint32_t* perm = nullptr;         // [cap] new index -> old index (last reorder)
   uint64_t* sort_key = nullptr;     // [cap]
   int64_t cap_cells = 0;

@@ -67,12 +67,17 @@ def test_streams_bit_exact(gpu_ok, case):
     a, b, code, own = s.ctx.streams()
     oa, ob, ocode = o.streams()
     assert len(a) == len(oa) and len(a) > 0
-    assert np.array_equal(a, oa) and np.array_equal(b, ob) and np.array_equal(code, ocode)
+    # the device stores particles along a Morton curve of cells, the oracle in cell-index order:
+    # compare each particle's stream, whose order (stencil dz,dy,dx then id) must be identical
+    g_ord = np.argsort(a, kind="stable")
+    o_ord = np.argsort(oa, kind="stable")
+    assert np.array_equal(a[g_ord], oa[o_ord])
+    assert np.array_equal(b[g_ord], ob[o_ord]) and np.array_equal(code[g_ord], ocode[o_ord])
     # owner rule of nbh_sphere (symetric): lower id owns, periodic images are owned by both sides
     exp_own = (a < b) | (code != 13)
     assert np.array_equal(own.astype(bool), exp_own)
-    # positions after the wrap + cell sort are identical, in the same order
-    pg, pq = s.ctx.download(), o.particles()
+    # positions after the periodic wrap are identical
+    pg, pq = by_id(s.ctx.download()), by_id(o.particles())
     assert np.array_equal(pg["id"], pq["id"])
     assert np.array_equal(pg["rx"], pq["rx"]) and np.array_equal(pg["rz"], pq["rz"])
 
