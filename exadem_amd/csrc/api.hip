@@ -65,6 +65,7 @@ static void free_pstate(PState& s) {
 
 static void free_nl(NbrList& l) {
   dfree(&l.word); dfree(&l.hx); dfree(&l.hy); dfree(&l.hz); dfree(&l.mhx); dfree(&l.mhy); dfree(&l.mhz);
+  dfree(&l.fx32); dfree(&l.fy32); dfree(&l.fz32); dfree(&l.mfx32); dfree(&l.mfy32); dfree(&l.mfz32);
   dfree(&l.cnt); dfree(&l.kmax); dfree(&l.base);
   l.slots = l.cap_slots = l.pairs = 0;
 }

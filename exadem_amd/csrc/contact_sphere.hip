@@ -558,6 +558,8 @@ int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd,
                  bool gravity_on, V3 g, bool guarded, int32_t step_tag) {
   if (!c->hist_ready) return set_err(c, XDEM_ERR_STATE, "contact: run nbh_sphere after chunk_neighbors_contact");
   if (form == XDEM_FORM_FP32) return launch_force_fp32(c, cp, cpd, dt, fused, gravity_on, g, guarded, step_tag);
+  int rc = set_history_form(c, false);
+  if (rc) return rc;
   if (cp.kr != 0.0 || cpd.kr != 0.0) c->kr_used = true;
   const bool walls = c->nwalls > 0 && c->wl_valid;
   ForceArgs a{};

@@ -1,12 +1,401 @@
-// contact_sphere_fp32.hip — FP32-force form of the sphere contact pass (placeholder until the
-// cell-relative FP32 kernel lands).
+// contact_sphere_fp32.hip — FP32-force form of the sphere contact pass.
+//
+// What stays FP64: absolute positions, the overlap sign test dn = |ri - rj| - (Ri + Rj) < 0 (so the
+// contact set is exactly the FP64 one), walls, gravity, force_to_accel, the angular acceleration and
+// the integrator. What is FP32: the contact geometry on relative vectors (the FP64 separation rounded
+// once; contact arms cp - ri = -n (Ri + dn/2) and cp - rj = (cp - ri) + (ri - rj), never absolute
+// coordinates), velocities, the Hooke + Coulomb law (same expression order as
+// src/forcefield/include/exaDEM/forcefield/contact_force.hpp:146-235) and the friction history (12 bytes
+// per slot instead of 24). Per-particle sums of the contact forces and moments are accumulated in FP32
+// with TwoSum compensation and added to the FP64 gravity term at the end.
 #include "contact_law.h"
 
 namespace xdem {
 
-int launch_force_fp32(xdem_ctx* c, const ContactParams&, const ContactParams&, double, bool, bool, V3, bool,
-                      int32_t) {
-  return set_err(c, XDEM_ERR_UNSUPPORTED, "FP32-force form not built yet");
+struct F3 {
+  float x, y, z;
+};
+__device__ inline F3 operator+(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ inline F3 operator-(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline F3 operator-(F3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ inline F3 operator*(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ inline F3 operator*(float s, F3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ inline float dotf(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline F3 crossf(F3 a, F3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+__device__ inline F3 to_f3(double x, double y, double z) { return {(float)x, (float)y, (float)z}; }
+
+// TwoSum-compensated accumulator (Knuth): s + c carries the sum with about twice FP32 precision.
+struct CSum {
+  float s, c;
+};
+__device__ inline void cadd(CSum& a, float x) {
+  const float t = a.s + x;
+  const float bp = t - a.s;
+  a.c += (a.s - (t - bp)) + (x - bp);
+  a.s = t;
+}
+__device__ inline double cval(const CSum& a) { return (double)a.s + (double)a.c; }
+
+struct ParamsF {
+  float kn, kt, kr, mu, damp_rate;
+};
+
+// contact_force_core<Hooke,None> in FP32 on relative vectors: bi = cp - pos_i, bj = cp - pos_j.
+template <bool KR>
+__device__ inline F3 hooke_force_f32(float dn, F3 n, float dt, const ParamsF& p, float meff, F3& ft, F3& mom,
+                                     F3 bi, F3 bj, F3 vel_i, F3 vrot_i, F3 vel_j, F3 vrot_j) {
+  const float damp = p.damp_rate * 2.0f * sqrtf(p.kn * meff);
+  const F3 ci = vel_i - crossf(bi, vrot_i);
+  const F3 cj = vel_j - crossf(bj, vrot_j);
+  const F3 vel = cj - ci;
+  const float vn = dotf(vel, n);
+  const float fne = -p.kn * dn;
+  const float fnv = damp * vn;
+  const float fnval = (fnv + fne > 0.0f) ? fnv + fne : 0.0f;
+  const F3 fn = fnval * n;
+  const F3 vt = vel - vn * n;
+  ft = ft + p.kt * (dt * vt);
+  const float thr = fabsf(p.mu * (p.kn * dn));
+  const float ft2 = dotf(ft, ft);
+  if (ft2 > 0.0f && ft2 > thr * thr) ft = ft * (thr / sqrtf(ft2));
+  const F3 f = fn + ft;
+  if (KR) {
+    mom = mom + (p.kr * (vrot_j - vrot_i)) * dt;
+    F3 branch = bi;
+    const float r = dotf(branch, vrot_i) / dotf(vrot_i, vrot_i);
+    branch = branch - r * vrot_i;
+    const float thr_m = fabsf(0.0f * sqrtf(dotf(branch, branch)) * sqrtf(dotf(fn, fn)));
+    const float m2 = dotf(mom, mom);
+    if (m2 > 0.0f && m2 > thr_m * thr_m) mom = mom * (thr_m / sqrtf(m2));
+  }
+  return f;
+}
+
+struct ForceArgs32 {
+  int64_t n;
+  PState s;
+  NbrList nl;
+  WallList wl;
+  const Surface* walls;
+  int nw;
+  int64_t wstride;
+  Grid g;
+  ContactParams cpd;  // walls stay FP64
+  ContactParams cp64;
+  ParamsF cp;
+  double dt;
+  float dtf;
+  V3 grav;
+  int gravity_on;
+  int32_t* flags;
+  int32_t step_tag;
+};
+
+__device__ inline int64_t xcd_block32(int64_t b, int64_t nb) {
+  const int64_t q = nb >> 3, r = nb & 7;
+  const int64_t x = b & 7, k = b >> 3;
+  return x * q + (x < r ? x : r) + k;
+}
+
+struct Q4g {
+  double w, x, y, z;
+};
+__device__ inline V3 qrot_g(Q4g q, V3 v) {
+  const V3 qv{q.x, q.y, q.z};
+  V3 uv = cross(qv, v);
+  V3 uuv = cross(qv, uv);
+  uv = uv * (2.0 * q.w);
+  uuv = uuv * 2.0;
+  return v + uv + uuv;
+}
+__device__ inline V3 angular_accel_g(Q4g Q, V3 mom, V3 vrot, V3 I) {
+  const Q4g Qi{Q.w, -Q.x, -Q.y, -Q.z};
+  const V3 om = qrot_g(Qi, vrot);
+  const V3 M = qrot_g(Qi, mom);
+  const V3 dom{(M.x - (I.z - I.y) * om.y * om.z) / I.x, (M.y - (I.x - I.z) * om.z * om.x) / I.y,
+               (M.z - (I.y - I.x) * om.x * om.y) / I.z};
+  return qrot_g(Q, dom);
+}
+
+struct Entry32 {
+  uint32_t wd;
+  d4 p;
+  d4 v;
+  d4 w;
+  F3 ft;
+  F3 mint;
+};
+
+template <bool KR>
+__device__ inline void fetch32(const ForceArgs32& a, uint64_t sl, uint32_t wd, Entry32& e) {
+  const Core* __restrict__ core = a.s.core;
+  const uint32_t y = wd & kIdxMask;
+  e.wd = wd;
+  e.p = core[y].xr;
+  e.v = core[y].vm;
+  e.w = core[y].w;
+  if (wd & kActiveBit) {
+    e.ft = {a.nl.fx32[sl], a.nl.fy32[sl], a.nl.fz32[sl]};
+    if (KR) e.mint = {a.nl.mfx32[sl], a.nl.mfy32[sl], a.nl.mfz32[sl]};
+    else e.mint = {0.0f, 0.0f, 0.0f};
+  } else {
+    e.ft = {0.0f, 0.0f, 0.0f};
+    e.mint = {0.0f, 0.0f, 0.0f};
+  }
+}
+
+template <bool FUSED, bool KR, bool WALLS, bool GUARD>
+__global__ __launch_bounds__(256) void k_force_fp32(ForceArgs32 a) {
+  if (GUARD) {
+    if (a.flags[1]) return;
+    if (a.flags[0]) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.flags[1] = 1;
+        a.flags[2] = a.step_tag;
+      }
+      return;
+    }
+  }
+  const int64_t blk = xcd_block32(blockIdx.x, gridDim.x);
+  const int64_t x = blk * blockDim.x + threadIdx.x;
+  if (x >= a.n) return;
+  const PState& s = a.s;
+  uint32_t* __restrict__ word = a.nl.word;
+  const uint32_t cnt = a.nl.cnt[x];
+  const uint64_t b = a.nl.base[x >> 6] + (uint64_t)(x & 63);
+  Entry32 nxt;
+  uint32_t wd2 = 0;
+  if (cnt > 0) fetch32<KR>(a, b, word[b], nxt);
+  if (cnt > 1) wd2 = word[b + kWave];
+  const Core me = s.core[x];
+  const V3 rx{me.xr.x, me.xr.y, me.xr.z};
+  const double Rx = me.xr.w, mx = me.vm.w;
+  const F3 vx = to_f3(me.vm.x, me.vm.y, me.vm.z);
+  const F3 wx = to_f3(me.w.x, me.w.y, me.w.z);
+  const float mxf = (float)mx;
+  CSum Fx{0, 0}, Fy{0, 0}, Fz{0, 0}, Mx{0, 0}, My{0, 0}, Mz{0, 0};
+  for (uint32_t k = 0; k < cnt; k++) {
+    const uint64_t sl = b + (uint64_t)k * kWave;
+    Entry32 e = nxt;
+    if (k + 1 < cnt) {
+      fetch32<KR>(a, sl + kWave, wd2, nxt);
+      if (k + 2 < cnt) wd2 = word[sl + 2 * kWave];
+    }
+    const uint32_t wd = e.wd;
+    const bool owner = (wd & kOwnerBit) != 0;
+    V3 ri, rj;
+    double Ri, Rj;
+    if (owner) {
+      const V3 sh = shift_vec(a.g, (int)((wd >> kShiftShift) & 31u));
+      ri = rx;
+      Ri = Rx;
+      rj = {e.p.x + sh.x, e.p.y + sh.y, e.p.z + sh.z};
+      Rj = e.p.w;
+    } else {
+      ri = {e.p.x, e.p.y, e.p.z};
+      Ri = e.p.w;
+      rj = rx;
+      Rj = Rx;
+    }
+    // FP64 overlap sign test: the contact set is the plain form's
+    const V3 dist = ri - rj;
+    const double dnorm = sqrt(dot(dist, dist));
+    const double dn = dnorm - (Ri + Rj);
+    if (dn < 0) {
+      const F3 d = to_f3(dist.x, dist.y, dist.z);
+      const float dnf = (float)dn;
+      const float inv = 1.0f / (float)dnorm;
+      const F3 nrm = d * inv;
+      const F3 bi = -(nrm * ((float)Ri + 0.5f * dnf));  // cp - ri
+      const F3 bj = bi + d;                            // cp - rj
+      const F3 vy = to_f3(e.v.x, e.v.y, e.v.z);
+      const F3 wy = to_f3(e.w.x, e.w.y, e.w.z);
+      const float my = (float)e.v.w;
+      const float mi = owner ? mxf : my, mj = owner ? my : mxf;
+      const float meff = (mi * mj) / (mi + mj);
+      F3 ft = e.ft;
+      F3 mint = e.mint;
+      const F3 f = owner ? hooke_force_f32<KR>(dnf, nrm, a.dtf, a.cp, meff, ft, mint, bi, bj, vx, wx, vy, wy)
+                         : hooke_force_f32<KR>(dnf, nrm, a.dtf, a.cp, meff, ft, mint, bi, bj, vy, wy, vx, wx);
+      const bool act = !(ft.x == 0 && ft.y == 0 && ft.z == 0 && mint.x == 0 && mint.y == 0 && mint.z == 0);
+      if (act || (wd & kActiveBit)) {
+        a.nl.fx32[sl] = ft.x;
+        a.nl.fy32[sl] = ft.y;
+        a.nl.fz32[sl] = ft.z;
+        if (KR) {
+          a.nl.mfx32[sl] = mint.x;
+          a.nl.mfy32[sl] = mint.y;
+          a.nl.mfz32[sl] = mint.z;
+        }
+      }
+      const uint32_t nwd = act ? (wd | kActiveBit) : (wd & ~kActiveBit);
+      if (nwd != wd) word[sl] = nwd;
+      F3 fc, tc;
+      if (owner) {
+        fc = f;
+        tc = crossf(bi, f) + mint;
+      } else {
+        fc = -f;
+        tc = crossf(bj, fc) + (-mint);
+      }
+      cadd(Fx, fc.x);
+      cadd(Fy, fc.y);
+      cadd(Fz, fc.z);
+      cadd(Mx, tc.x);
+      cadd(My, tc.y);
+      cadd(Mz, tc.z);
+    } else if (wd & kActiveBit) {
+      a.nl.fx32[sl] = 0.0f;
+      a.nl.fy32[sl] = 0.0f;
+      a.nl.fz32[sl] = 0.0f;
+      if (KR) {
+        a.nl.mfx32[sl] = 0.0f;
+        a.nl.mfy32[sl] = 0.0f;
+        a.nl.mfz32[sl] = 0.0f;
+      }
+      word[sl] = wd & ~kActiveBit;
+    }
+  }
+  V3 F, M;
+  if (FUSED) {
+    F = {0.0, 0.0, 0.0};
+    M = {0.0, 0.0, 0.0};
+    if (a.gravity_on) F = {F.x + a.grav.x * mx, F.y + a.grav.y * mx, F.z + a.grav.z * mx};
+  } else {
+    F = {s.ax[x], s.ay[x], s.az[x]};
+    M = {s.mx[x], s.my[x], s.mz[x]};
+  }
+  F = F + V3{cval(Fx), cval(Fy), cval(Fz)};
+  M = M + V3{cval(Mx), cval(My), cval(Mz)};
+  if (WALLS) {
+    const V3 vx64{me.vm.x, me.vm.y, me.vm.z}, wx64{me.w.x, me.w.y, me.w.z};
+    const uint32_t mask = a.wl.mask[x];
+    for (int w = 0; w < a.nw; w++) {
+      if (!((mask >> w) & 1u)) continue;
+      const Surface& sf = a.walls[w];
+      const int64_t j = (int64_t)w * a.wstride + x;
+      const V3 proj = dot(rx, sf.normal) * sf.normal;
+      const V3 s2p = -(sf.center_proj - proj);
+      const double d = norm(s2p);
+      const double dnw = d - Rx;
+      if (dnw > 0) {
+        a.wl.hx[j] = 0.0; a.wl.hy[j] = 0.0; a.wl.hz[j] = 0.0;
+        a.wl.mhx[j] = 0.0; a.wl.mhy[j] = 0.0; a.wl.mhz[j] = 0.0;
+        continue;
+      }
+      const V3 nrm = s2p / d;
+      const V3 cpos = rx - nrm * (Rx + 0.5 * dnw);
+      V3 ft{a.wl.hx[j], a.wl.hy[j], a.wl.hz[j]};
+      V3 mint{0.0, 0.0, 0.0};
+      if (KR) mint = {a.wl.mhx[j], a.wl.mhy[j], a.wl.mhz[j]};
+      const V3 f =
+          hooke_force<KR>(dnw, nrm, a.dt, a.cpd, mx, ft, mint, cpos, rx, vx64, wx64, sf.center, sf.vel, sf.vrot);
+      a.wl.hx[j] = ft.x; a.wl.hy[j] = ft.y; a.wl.hz[j] = ft.z;
+      if (KR) {
+        a.wl.mhx[j] = mint.x; a.wl.mhy[j] = mint.y; a.wl.mhz[j] = mint.z;
+      }
+      F = F + f;
+      M = M + (cross(cpos - rx, f) + mint);
+    }
+  }
+  if (FUSED) {
+    s.ax[x] = F.x / mx;
+    s.ay[x] = F.y / mx;
+    s.az[x] = F.z / mx;
+    s.mx[x] = M.x;
+    s.my[x] = M.y;
+    s.mz[x] = M.z;
+    const d4 q = me.q;
+    const V3 al =
+        angular_accel_g(Q4g{q.x, q.y, q.z, q.w}, M, V3{me.w.x, me.w.y, me.w.z}, V3{s.ix[x], s.iy[x], s.iz[x]});
+    s.alx[x] = al.x;
+    s.aly[x] = al.y;
+    s.alz[x] = al.z;
+  } else {
+    s.ax[x] = F.x;
+    s.ay[x] = F.y;
+    s.az[x] = F.z;
+    s.mx[x] = M.x;
+    s.my[x] = M.y;
+    s.mz[x] = M.z;
+  }
+}
+
+// history representation switch (FP64 <-> FP32) over every slot of the current list
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void k_hist_convert(int64_t slots, const TI* __restrict__ ix, const TI* __restrict__ iy,
+                                                      const TI* __restrict__ iz, TO* __restrict__ ox, TO* __restrict__ oy,
+                                                      TO* __restrict__ oz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= slots) return;
+  ox[i] = (TO)ix[i];
+  oy[i] = (TO)iy[i];
+  oz[i] = (TO)iz[i];
+}
+
+int set_history_form(xdem_ctx* c, bool fp32) {
+  if (c->hist_fp32 == fp32) return XDEM_OK;
+  NbrList& L = c->nl[c->nl_cur];
+  if (c->nl_valid && L.slots > 0) {
+    const dim3 grid(grid_blocks(L.slots, 256)), block(256);
+    if (fp32) {
+      hipLaunchKernelGGL((k_hist_convert<double, float>), grid, block, 0, c->stream, L.slots, L.hx, L.hy, L.hz,
+                         L.fx32, L.fy32, L.fz32);
+      hipLaunchKernelGGL((k_hist_convert<double, float>), grid, block, 0, c->stream, L.slots, L.mhx, L.mhy, L.mhz,
+                         L.mfx32, L.mfy32, L.mfz32);
+    } else {
+      hipLaunchKernelGGL((k_hist_convert<float, double>), grid, block, 0, c->stream, L.slots, L.fx32, L.fy32,
+                         L.fz32, L.hx, L.hy, L.hz);
+      hipLaunchKernelGGL((k_hist_convert<float, double>), grid, block, 0, c->stream, L.slots, L.mfx32, L.mfy32,
+                         L.mfz32, L.mhx, L.mhy, L.mhz);
+    }
+    XDEM_HIP(c, hipGetLastError());
+  }
+  c->hist_fp32 = fp32;
+  return XDEM_OK;
+}
+
+template <bool FUSED, bool KR, bool WALLS>
+static void launch32(xdem_ctx* c, const ForceArgs32& a, bool guarded) {
+  const dim3 grid(grid_blocks(a.n, 256)), block(256);
+  if (guarded) hipLaunchKernelGGL((k_force_fp32<FUSED, KR, WALLS, true>), grid, block, 0, c->stream, a);
+  else hipLaunchKernelGGL((k_force_fp32<FUSED, KR, WALLS, false>), grid, block, 0, c->stream, a);
+}
+
+int launch_force_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool fused,
+                      bool gravity_on, V3 g, bool guarded, int32_t step_tag) {
+  int rc = set_history_form(c, true);
+  if (rc) return rc;
+  if (cp.kr != 0.0 || cpd.kr != 0.0) c->kr_used = true;
+  const bool walls = c->nwalls > 0 && c->wl_valid;
+  ForceArgs32 a{};
+  a.n = c->n;
+  a.s = live(c);
+  a.nl = c->nl[c->nl_cur];
+  if (walls) a.wl = c->wl[c->wl_cur];
+  a.walls = c->d_walls;
+  a.nw = c->nwalls;
+  a.wstride = c->cap;
+  a.g = c->grid;
+  a.cpd = cpd;
+  a.cp64 = cp;
+  a.cp = {(float)cp.kn, (float)cp.kt, (float)cp.kr, (float)cp.mu, (float)cp.damp_rate};
+  a.dt = dt;
+  a.dtf = (float)dt;
+  a.grav = g;
+  a.gravity_on = gravity_on ? 1 : 0;
+  a.flags = c->d_flags;
+  a.step_tag = step_tag;
+  const bool kr = c->kr_used;
+  if (fused) {
+    if (kr) walls ? launch32<true, true, true>(c, a, guarded) : launch32<true, true, false>(c, a, guarded);
+    else walls ? launch32<true, false, true>(c, a, guarded) : launch32<true, false, false>(c, a, guarded);
+  } else {
+    if (kr) walls ? launch32<false, true, true>(c, a, guarded) : launch32<false, true, false>(c, a, guarded);
+    else walls ? launch32<false, false, true>(c, a, guarded) : launch32<false, false, false>(c, a, guarded);
+  }
+  XDEM_HIP(c, hipGetLastError());
+  return XDEM_OK;
 }
 
 }  // namespace xdem

@@ -302,7 +302,9 @@ int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist) {
     const int64_t cap = (int64_t)(slots * 1.25) + 4096;
     if ((rc = grow(c, &L.word, cap)) || (rc = grow(c, &L.hx, cap)) || (rc = grow(c, &L.hy, cap)) ||
         (rc = grow(c, &L.hz, cap)) || (rc = grow(c, &L.mhx, cap)) || (rc = grow(c, &L.mhy, cap)) ||
-        (rc = grow(c, &L.mhz, cap)))
+        (rc = grow(c, &L.mhz, cap)) || (rc = grow(c, &L.fx32, cap)) || (rc = grow(c, &L.fy32, cap)) ||
+        (rc = grow(c, &L.fz32, cap)) || (rc = grow(c, &L.mfx32, cap)) || (rc = grow(c, &L.mfy32, cap)) ||
+        (rc = grow(c, &L.mfz32, cap)))
       return rc;
     L.cap_slots = cap;
   }
@@ -329,18 +331,31 @@ struct CarryArgs {
   int have_old;
 };
 
+template <typename T>
+struct HistView {
+  T *x, *y, *z, *mx, *my, *mz;
+};
+__host__ __device__ inline HistView<double> hist_view(const NbrList& l, double*) {
+  return {l.hx, l.hy, l.hz, l.mhx, l.mhy, l.mhz};
+}
+__host__ __device__ inline HistView<float> hist_view(const NbrList& l, float*) {
+  return {l.fx32, l.fy32, l.fz32, l.mfx32, l.mfy32, l.mfz32};
+}
+
 // One thread per (particle, entry). New entries start from zero history; an entry whose key
 // (owner id, partner id) matches an active owner entry of the previous list inherits its friction
 // and moment (interaction_manager.hpp:88-112 with placeholder_interaction.hpp:398-408).
+template <typename T>
 __global__ __launch_bounds__(256) void k_hist_carry(CarryArgs a) {
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= a.n) return;
+  const HistView<T> nh = hist_view(a.nl, (T*)nullptr), oh = hist_view(a.ol, (T*)nullptr);
   const uint32_t cnt = a.nl.cnt[x];
   const uint64_t b = a.nl.base[x >> 6] + (uint64_t)(x & 63);
   for (uint32_t k = 0; k < cnt; k++) {
     const uint64_t s = b + (uint64_t)k * kWave;
     uint32_t wd = a.nl.word[s] & ~kActiveBit;
-    double hx = 0, hy = 0, hz = 0, mx = 0, my = 0, mz = 0;
+    T hx = 0, hy = 0, hz = 0, mx = 0, my = 0, mz = 0;
     bool act = false;
     if (a.have_old) {
       const uint32_t y = wd & kIdxMask;
@@ -354,15 +369,15 @@ __global__ __launch_bounds__(256) void k_hist_carry(CarryArgs a) {
         const uint64_t s2 = ob + (uint64_t)k2 * kWave;
         const uint32_t w2 = a.ol.word[s2];
         if ((w2 & kOwnerBit) && (w2 & kActiveBit) && (w2 & kIdxMask) == op) {
-          hx = a.ol.hx[s2]; hy = a.ol.hy[s2]; hz = a.ol.hz[s2];
-          mx = a.ol.mhx[s2]; my = a.ol.mhy[s2]; mz = a.ol.mhz[s2];
+          hx = oh.x[s2]; hy = oh.y[s2]; hz = oh.z[s2];
+          mx = oh.mx[s2]; my = oh.my[s2]; mz = oh.mz[s2];
           act = true;
           break;
         }
       }
     }
-    a.nl.hx[s] = hx; a.nl.hy[s] = hy; a.nl.hz[s] = hz;
-    a.nl.mhx[s] = mx; a.nl.mhy[s] = my; a.nl.mhz[s] = mz;
+    nh.x[s] = hx; nh.y[s] = hy; nh.z[s] = hz;
+    nh.mx[s] = mx; nh.my[s] = my; nh.mz[s] = mz;
     a.nl.word[s] = wd | (act ? kActiveBit : 0u);
   }
 }
@@ -408,7 +423,10 @@ int do_nbh_sphere(xdem_ctx* c, double rcut_inc) {
   }
   const bool have_old = c->nl_prev_valid && c->perm_valid;
   CarryArgs a{n, c->nl[c->nl_cur], c->nl[1 - c->nl_cur], c->perm, have_old ? 1 : 0};
-  hipLaunchKernelGGL(k_hist_carry, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a);
+  if (c->hist_fp32)
+    hipLaunchKernelGGL(k_hist_carry<float>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a);
+  else
+    hipLaunchKernelGGL(k_hist_carry<double>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a);
   XDEM_HIP(c, hipGetLastError());
   // walls
   int rc = alloc_walls(c, c->nwalls);
@@ -478,11 +496,18 @@ extern "C" int xdem_interactions_dump(xdem_ctx* c, int64_t cap, uint64_t* id_i, 
   XDEM_HIP(c, hipMemcpyAsync(cnt.data(), L.cnt, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
   XDEM_HIP(c, hipMemcpyAsync(base.data(), L.base, sizeof(uint64_t) * (nw + 1), hipMemcpyDeviceToHost, c->stream));
   XDEM_HIP(c, hipMemcpyAsync(ids.data(), live(c).id, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, c->stream));
+  std::vector<float> fx(c->hist_fp32 ? L.slots : 0), fy(fx.size()), fz(fx.size());
   if (L.slots > 0) {
     XDEM_HIP(c, hipMemcpyAsync(word.data(), L.word, sizeof(uint32_t) * L.slots, hipMemcpyDeviceToHost, c->stream));
-    XDEM_HIP(c, hipMemcpyAsync(hx.data(), L.hx, sizeof(double) * L.slots, hipMemcpyDeviceToHost, c->stream));
-    XDEM_HIP(c, hipMemcpyAsync(hy.data(), L.hy, sizeof(double) * L.slots, hipMemcpyDeviceToHost, c->stream));
-    XDEM_HIP(c, hipMemcpyAsync(hz.data(), L.hz, sizeof(double) * L.slots, hipMemcpyDeviceToHost, c->stream));
+    if (c->hist_fp32) {
+      XDEM_HIP(c, hipMemcpyAsync(fx.data(), L.fx32, sizeof(float) * L.slots, hipMemcpyDeviceToHost, c->stream));
+      XDEM_HIP(c, hipMemcpyAsync(fy.data(), L.fy32, sizeof(float) * L.slots, hipMemcpyDeviceToHost, c->stream));
+      XDEM_HIP(c, hipMemcpyAsync(fz.data(), L.fz32, sizeof(float) * L.slots, hipMemcpyDeviceToHost, c->stream));
+    } else {
+      XDEM_HIP(c, hipMemcpyAsync(hx.data(), L.hx, sizeof(double) * L.slots, hipMemcpyDeviceToHost, c->stream));
+      XDEM_HIP(c, hipMemcpyAsync(hy.data(), L.hy, sizeof(double) * L.slots, hipMemcpyDeviceToHost, c->stream));
+      XDEM_HIP(c, hipMemcpyAsync(hz.data(), L.hz, sizeof(double) * L.slots, hipMemcpyDeviceToHost, c->stream));
+    }
   }
   std::vector<uint8_t> mask;
   std::vector<double> whx, why, whz;
@@ -497,6 +522,12 @@ extern "C" int xdem_interactions_dump(xdem_ctx* c, int64_t cap, uint64_t* id_i, 
     XDEM_HIP(c, hipMemcpyAsync(whz.data(), W.hz, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
   }
   XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  if (c->hist_fp32)
+    for (int64_t i = 0; i < L.slots; i++) {
+      hx[i] = fx[i];
+      hy[i] = fy[i];
+      hz[i] = fz[i];
+    }
   int64_t k = 0;
   for (int64_t x = 0; x < n; x++) {
     if (!mask.empty())

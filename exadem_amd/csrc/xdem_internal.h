@@ -105,6 +105,12 @@ struct NbrList {
   double* mhx = nullptr;      // rolling-moment history [slots] (allocated when kr != 0 is used)
   double* mhy = nullptr;
   double* mhz = nullptr;
+  float* fx32 = nullptr;      // friction history of the FP32-force form [slots]
+  float* fy32 = nullptr;
+  float* fz32 = nullptr;
+  float* mfx32 = nullptr;
+  float* mfy32 = nullptr;
+  float* mfz32 = nullptr;
   uint32_t* cnt = nullptr;    // entries per particle [n]
   uint32_t* kmax = nullptr;   // per wave max count [nwaves]
   uint64_t* base = nullptr;   // per wave first slot [nwaves+1]
@@ -168,6 +174,7 @@ struct xdem_ctx {
   bool hist_ready = false;      // history of the current list has been carried over (nbh_sphere)
   bool wl_valid = false;        // wall list exists (for the order it was built in)
   bool kr_used = false;         // a rolling stiffness was used: moment histories are live
+  bool hist_fp32 = false;       // the live history of the list is the FP32 copy
   int nl_max_rows = 0;          // largest per-wavefront row count of the current list
   bool force_per_lane = false;  // select the per-lane force kernel (XDEM_FORCE_KERNEL=lane)
   double rcut_inc = 0;
@@ -223,4 +230,5 @@ int do_nbh_sphere(xdem_ctx* c, double rcut_inc);
 int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, int form, bool fused,
                  bool gravity_on, V3 g, bool guarded, int32_t step_tag);
 int alloc_walls(xdem_ctx* c, int nw);
+int set_history_form(xdem_ctx* c, bool fp32);
 }  // namespace xdem

@@ -201,3 +201,25 @@ def test_error_paths(gpu_ok):
     dup["id"] = np.zeros_like(dup["id"])
     with pytest.raises(native.XdemError, match="unique"):
         ctx.upload(dup)
+
+
+def test_fp32_form_contact_set_and_force_error(gpu_ok):
+    """FP32-force form: same contact set as FP64 (FP64 sign test), forces within 1e-5 relative."""
+    w = periodic_pack(n=4000)
+    a = SphereSim(w, form=native.XDEM_FORM_FP64)
+    b = SphereSim(w, form=native.XDEM_FORM_FP32)
+    a.first_iteration()
+    b.first_iteration()
+    pa, pb = by_id(a.ctx.download()), by_id(b.ctx.download())
+    floor = pa["mass"] * 9.81
+    err = rel_err(force(pb), force(pa), floor)
+    assert err <= 1e-5, err
+    a.run(50)
+    b.run(50)
+    ka, _ = interaction_table(*a.ctx.interactions())
+    kb, _ = interaction_table(*b.ctx.interactions())
+    assert np.array_equal(ka, kb)
+    pa, pb = by_id(a.ctx.download()), by_id(b.ctx.download())
+    err = rel_err(force(pb), force(pa), 9.81 * np.ones(len(pa["id"])))
+    assert err <= 1e-5, err
+    assert float(np.max(np.abs(pos(pb) - pos(pa)))) <= 1e-9
