@@ -84,6 +84,7 @@ int ensure_capacity(xdem_ctx* c, int64_t n) {
   int rc;
   if ((rc = dalloc(c, &c->cell_of, n))) return rc;
   if ((rc = dalloc(c, &c->perm, n))) return rc;
+  if ((rc = dalloc(c, &c->perm_tmp, n))) return rc;
   if ((rc = dalloc(c, &c->sort_key, n))) return rc;
   const int64_t nw = (n + kWave - 1) / kWave;
   for (int k = 0; k < 2; k++) {
@@ -328,6 +329,7 @@ void xdem_ctx_destroy(xdem_ctx* c) {
     for (auto& e : c->timers[k].ev) (void)hipEventDestroy(e);
   }
   dfree(&c->cell_of); dfree(&c->cell_count); dfree(&c->cell_start); dfree(&c->cell_rank); dfree(&c->perm);
+  dfree(&c->perm_tmp);
   dfree(&c->sort_key);
   dfree(&c->d_flags); dfree(&c->d_u32tmp); dfree(&c->d_walls);
   if (c->scan_tmp) (void)hipFree(c->scan_tmp);

@@ -206,38 +206,84 @@ __device__ inline void for_each_neighbour(const NbhArgs& a, int64_t x, F&& fn) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_nbh_count(NbhArgs a, uint32_t* __restrict__ cnt, uint32_t* __restrict__ kmax,
-                                                   uint32_t* __restrict__ max_rows) {
+__global__ __launch_bounds__(256) void k_nbh_count(NbhArgs a, uint32_t* __restrict__ cnt) {
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= a.n) return;
   uint32_t k = 0;
-  if (x < a.n) {
-    for_each_neighbour(a, x, [&](uint32_t, int) { k++; });
-    cnt[x] = k;
+  for_each_neighbour(a, x, [&](uint32_t, int) { k++; });
+  cnt[x] = k;
+}
+
+// Work balancing: inside each group of kBalance consecutive particles (a compact Morton region),
+// a stable counting sort by decreasing list length, so that the 64 lanes of a wavefront walk lists of
+// nearly equal length. perm2[new] = morton index, inv2[morton] = new index.
+constexpr int kBalance = 2048;
+constexpr int kBalanceBins = 64;
+__global__ __launch_bounds__(64) void k_balance(int64_t n, const uint32_t* __restrict__ cnt,
+                                                uint32_t* __restrict__ perm2, uint32_t* __restrict__ inv2) {
+  __shared__ uint32_t start[kBalanceBins];
+  __shared__ uint8_t bin[kBalance];
+  const int64_t g0 = (int64_t)blockIdx.x * kBalance;
+  const int m = (int)(((g0 + kBalance) < n ? (g0 + kBalance) : n) - g0);
+  for (int i = threadIdx.x; i < m; i += 64) {
+    const uint32_t c = cnt[g0 + i];
+    bin[i] = (uint8_t)(kBalanceBins - 1 - (c < kBalanceBins ? c : kBalanceBins - 1));  // longest lists first
   }
-  // per-wavefront maximum (blockDim is a multiple of 64)
-  uint32_t m = k;
-  for (int o = 32; o > 0; o >>= 1) {
-    const uint32_t t = __shfl_down(m, o, 64);
-    m = t > m ? t : m;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int b = 0; b < kBalanceBins; b++) start[b] = 0;
+  for (int i = 0; i < m; i++) start[bin[i]]++;
+  uint32_t run = (uint32_t)g0;
+  for (int b = 0; b < kBalanceBins; b++) {
+    const uint32_t t = start[b];
+    start[b] = run;
+    run += t;
   }
-  const int64_t w = x >> 6;
-  if ((threadIdx.x & 63) == 0 && (w << 6) < a.n) {
-    kmax[w] = m;
-    atomicMax(max_rows, m);
+  for (int i = 0; i < m; i++) {
+    const uint32_t pos = start[bin[i]]++;
+    perm2[pos] = (uint32_t)(g0 + i);
+    inv2[g0 + i] = pos;
   }
 }
 
-__global__ __launch_bounds__(256) void k_nbh_fill(NbhArgs a, const uint64_t* __restrict__ base,
-                                                  uint32_t* __restrict__ word, unsigned long long* __restrict__ pairs) {
+__global__ __launch_bounds__(256) void k_balance_apply(int64_t n, const uint32_t* __restrict__ perm2,
+                                                       const uint32_t* __restrict__ cnt_m, uint32_t* __restrict__ cnt,
+                                                       uint32_t* __restrict__ kmax, uint32_t* __restrict__ max_rows,
+                                                       const uint32_t* __restrict__ perm_old,
+                                                       uint32_t* __restrict__ perm_new) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t k = 0;
+  if (x < n) {
+    const uint32_t m = perm2[x];
+    k = cnt_m[m];
+    cnt[x] = k;
+    if (perm_old) perm_new[x] = perm_old[m];
+  }
+  uint32_t mx = k;
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t t = __shfl_down(mx, o, 64);
+    mx = t > mx ? t : mx;
+  }
+  const int64_t w = x >> 6;
+  if ((threadIdx.x & 63) == 0 && (w << 6) < n) {
+    kmax[w] = mx;
+    atomicMax(max_rows, mx);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_nbh_fill(NbhArgs a, const uint32_t* __restrict__ inv2,
+                                                  const uint64_t* __restrict__ base, uint32_t* __restrict__ word,
+                                                  unsigned long long* __restrict__ pairs) {
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t owned = 0;
   if (x < a.n) {
-    const uint64_t b = base[x >> 6] + (uint64_t)(x & 63);
+    const uint32_t xn = inv2[x];
+    const uint64_t b = base[xn >> 6] + (uint64_t)(xn & 63);
     const uint64_t idx = a.id[x];
     uint32_t k = 0;
     for_each_neighbour(a, x, [&](uint32_t y, int code) {
       const bool owner = !(idx >= a.id[y] && code == kNoShift);
-      word[b + (uint64_t)k * kWave] = y | ((uint32_t)code << kShiftShift) | (owner ? kOwnerBit : 0u);
+      word[b + (uint64_t)k * kWave] = inv2[y] | ((uint32_t)code << kShiftShift) | (owner ? kOwnerBit : 0u);
       owned += owner ? 1u : 0u;
       k++;
     });
@@ -290,8 +336,15 @@ int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist) {
   NbhArgs a{g, rcut_inc, nbh_dist * nbh_dist, c->cell_start, c->cell_rank, live(c).core, live(c).id, n};
   const int64_t nw = (n + kWave - 1) / kWave;
   uint32_t* max_rows = c->d_u32tmp + 2;
+  uint32_t* cnt_m = c->cell_of;  // list lengths in Morton order (cell_of is free after the binning)
+  uint32_t* perm2 = reinterpret_cast<uint32_t*>(c->sort_key);
+  uint32_t* inv2 = perm2 + c->cap;
+  uint32_t* perm_tmp = c->perm_tmp;
   XDEM_HIP(c, hipMemsetAsync(max_rows, 0, sizeof(uint32_t), c->stream));
-  hipLaunchKernelGGL(k_nbh_count, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a, L.cnt, L.kmax, max_rows);
+  hipLaunchKernelGGL(k_nbh_count, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a, cnt_m);
+  hipLaunchKernelGGL(k_balance, dim3(grid_blocks(n, kBalance)), dim3(64), 0, c->stream, n, cnt_m, perm2, inv2);
+  hipLaunchKernelGGL(k_balance_apply, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, perm2, cnt_m, L.cnt,
+                     L.kmax, max_rows, c->perm_valid ? c->perm : nullptr, perm_tmp);
   int rc = scan_u64_from_u32(c, L.kmax, L.base, nw, (uint32_t)kWave);
   if (rc) return rc;
   XDEM_HIP(c, hipMemcpyAsync(c->h_u32tmp + 2, max_rows, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
@@ -311,8 +364,16 @@ int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist) {
   L.slots = (int64_t)slots;
   unsigned long long* pairs = (unsigned long long*)c->d_u32tmp;
   XDEM_HIP(c, hipMemsetAsync(pairs, 0, sizeof(uint64_t), c->stream));
-  hipLaunchKernelGGL(k_nbh_fill, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a, L.base, L.word, pairs);
+  hipLaunchKernelGGL(k_nbh_fill, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a, inv2, L.base, L.word,
+                     pairs);
+  // move every field into the balanced order; compose the history permutation (new -> before binning)
+  PState dst = c->ps[1 - c->cur];
+  hipLaunchKernelGGL(k_gather, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, perm2, live(c), dst);
+  if (c->perm_valid)
+    XDEM_HIP(c, hipMemcpyAsync(c->perm, perm_tmp, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, c->stream));
   XDEM_HIP(c, hipGetLastError());
+  c->cur = 1 - c->cur;
+  c->cells_valid = false;  // cell ranges refer to the Morton order
   uint64_t hp = 0;
   if ((rc = read_u64(c, (const uint64_t*)pairs, &hp))) return rc;
   L.pairs = (int64_t)hp;

@@ -159,6 +159,7 @@ struct xdem_ctx {
   uint32_t* cell_start = nullptr;   // [ncells+1], indexed by Morton rank of the cell
   uint32_t* cell_rank = nullptr;    // [ncells] rank of each cell along the Morton curve
   uint32_t* perm = nullptr;         // [cap] new index -> old index (last reorder)
+  uint32_t* perm_tmp = nullptr;     // [cap] scratch for composing permutations
   uint64_t* sort_key = nullptr;     // [cap]
   int64_t cap_cells = 0;
   void* scan_tmp = nullptr;
