@@ -18,6 +18,7 @@ HIP_SOURCES = [
     "grid.hip",
     "contact_sphere.hip",
     "contact_sphere_fp32.hip",
+    "poly.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -49,7 +49,7 @@ static int alloc_pstate(xdem_ctx* c, PState& s, int64_t n) {
   if ((rc = dalloc(c, &s.ptr, n)) != XDEM_OK) \
     return rc;
   A(core)
-  A(ax) A(ay) A(az) A(mx) A(my) A(mz) A(alx) A(aly) A(alz) A(ix) A(iy) A(iz) A(bx) A(by) A(bz)
+  A(ax) A(ay) A(az) A(mx) A(my) A(mz) A(alx) A(aly) A(alz) A(ix) A(iy) A(iz) A(bx) A(by) A(bz) A(bq)
   A(id) A(type) A(group)
 #undef A
   return XDEM_OK;
@@ -59,7 +59,7 @@ static void free_pstate(PState& s) {
   dfree(&s.core);
   dfree(&s.ax); dfree(&s.ay); dfree(&s.az); dfree(&s.mx); dfree(&s.my); dfree(&s.mz);
   dfree(&s.alx); dfree(&s.aly); dfree(&s.alz); dfree(&s.ix); dfree(&s.iy); dfree(&s.iz);
-  dfree(&s.bx); dfree(&s.by); dfree(&s.bz);
+  dfree(&s.bx); dfree(&s.by); dfree(&s.bz); dfree(&s.bq);
   dfree(&s.id); dfree(&s.type); dfree(&s.group);
 }
 
@@ -273,6 +273,7 @@ __global__ void k_pack_in(int64_t n, const double* rx, const double* ry, const d
   c.w = {wx[i], wy[i], wz[i], hom[i]};
   c.q = {qw[i], qx[i], qy[i], qz[i]};
   s.core[i] = c;
+  s.bq[i] = c.q;
 }
 
 __global__ void k_unpack_out(int64_t n, PState s, double* rx, double* ry, double* rz, double* rad, double* vx,
@@ -332,6 +333,14 @@ void xdem_ctx_destroy(xdem_ctx* c) {
   dfree(&c->perm_tmp);
   dfree(&c->sort_key);
   dfree(&c->d_flags); dfree(&c->d_u32tmp); dfree(&c->d_walls);
+  dfree(&c->d_shapes); dfree(&c->vtx); dfree(&c->mirror); dfree(&c->pair_buf);
+  for (int k = 0; k < 2; k++) {
+    PolyList& P = c->pl[k];
+    dfree(&P.start); dfree(&P.feat); dfree(&P.hx); dfree(&P.hy); dfree(&P.hz);
+    dfree(&P.mhx); dfree(&P.mhy); dfree(&P.mhz); dfree(&P.fcount);
+    PolyWall& W = c->pw[k];
+    dfree(&W.mask); dfree(&W.hx); dfree(&W.hy); dfree(&W.hz); dfree(&W.mhx); dfree(&W.mhy); dfree(&W.mhz);
+  }
   if (c->scan_tmp) (void)hipFree(c->scan_tmp);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
   if (c->h_u32tmp) (void)hipHostFree(c->h_u32tmp);
@@ -470,6 +479,9 @@ int xdem_particles_upload(xdem_ctx* c, const xdem_particles* p) {
   XDEM_HIP(c, cp(s.bz, p->rz));
   XDEM_HIP(c, hipMemcpyAsync(s.id, p->id, sizeof(uint64_t) * n, hipMemcpyHostToDevice, c->stream));
   std::vector<uint32_t> z32(n, 0);
+  c->max_type = 0;
+  if (p->type)
+    for (int64_t i = 0; i < n; i++) c->max_type = std::max(c->max_type, p->type[i]);
   XDEM_HIP(c, hipMemcpyAsync(s.type, p->type ? p->type : z32.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice,
                              c->stream));
   XDEM_HIP(c, hipMemcpyAsync(s.group, p->group ? p->group : z32.data(), sizeof(uint32_t) * n,
@@ -668,7 +680,15 @@ static int rebuild(xdem_ctx* c, double rcut_inc) {
   int rc;
   if ((rc = do_move_particles(c))) return rc;
   if ((rc = do_chunk_neighbors(c, rcut_inc, -1.0))) return rc;
+  if (c->nshapes > 0) return poly_nbh(c, rcut_inc);
   return do_nbh_sphere(c, rcut_inc);
+}
+
+// one force pass of the engine: spheres (one kernel) or polyhedra (pair + gather kernels)
+static int engine_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, const xdem_scheme* s, V3 g,
+                        bool guarded, int32_t tag) {
+  if (c->nshapes > 0) return poly_force(c, cp, cpd, s->dt, true, s->gravity_on != 0, g, guarded, tag);
+  return launch_force(c, cp, cpd, s->dt, s->form, true, s->gravity_on != 0, g, guarded, tag);
 }
 
 static int record_timer(xdem_ctx* c, int which, bool start) {
@@ -704,6 +724,10 @@ int xdem_run_steps(xdem_ctx* c, const xdem_scheme* s, int64_t nsteps, xdem_run_s
   if (c->nwalls > 0 && (rc = check_law(c, &s->cpd, "scheme cpd"))) return rc;
   if (s->form != XDEM_FORM_FP64 && s->form != XDEM_FORM_FP32) return set_err(c, XDEM_ERR_ARG, "run_steps: bad form");
   if (!(s->dt > 0)) return set_err(c, XDEM_ERR_ARG, "run_steps: dt must be > 0");
+  if (c->nshapes > 0 && s->form != XDEM_FORM_FP64)
+    return set_err(c, XDEM_ERR_UNSUPPORTED, "run_steps: polyhedra run in the FP64 form only");
+  if (c->nshapes > 0 && c->n > 0 && c->max_type >= (uint32_t)c->nshapes)
+    return set_err(c, XDEM_ERR_ARG, "run_steps: a particle type has no registered shape");
   if (stats) *stats = {0, 0, 0};
   if (c->n == 0 || nsteps == 0) return XDEM_OK;
   if (!c->nl_valid) return set_err(c, XDEM_ERR_STATE, "run_steps: run first_iteration (neighbour build) first");
@@ -722,10 +746,13 @@ int xdem_run_steps(xdem_ctx* c, const xdem_scheme* s, int64_t nsteps, xdem_run_s
       const bool kick = c->pending_kick;
       const double kdt = c->pending_dt;
       if (c->timing && (rc = record_timer(c, 0, true))) return rc;
-      if ((rc = launch_prolog(c, s->dt, kick, kdt, true, thr * thr, true))) return rc;
+      const bool poly = c->nshapes > 0;
+      // spheres test the centre displacement in the prolog; polyhedra test vertices (vertex_displ_over)
+      if ((rc = launch_prolog(c, s->dt, kick, kdt, !poly, thr * thr, true))) return rc;
+      if (poly && (rc = poly_compute_vertices(c, true, thr * thr, true))) return rc;
       if (c->timing && (rc = record_timer(c, 0, false))) return rc;
       if (c->timing && (rc = record_timer(c, 1, true))) return rc;
-      if ((rc = launch_force(c, cp, cpd, s->dt, s->form, true, s->gravity_on != 0, g, true, (int32_t)step))) return rc;
+      if ((rc = engine_force(c, cp, cpd, s, g, true, (int32_t)step))) return rc;
       if (c->timing && (rc = record_timer(c, 1, false))) return rc;
       c->pending_kick = true;
       c->pending_dt = s->dt;
@@ -739,7 +766,7 @@ int xdem_run_steps(xdem_ctx* c, const xdem_scheme* s, int64_t nsteps, xdem_run_s
       XDEM_HIP(c, hipMemsetAsync(c->d_flags, 0, 4 * sizeof(int32_t), c->stream));
       if ((rc = rebuild(c, s->rcut_inc))) return rc;
       rebuilds++;
-      if ((rc = launch_force(c, cp, cpd, s->dt, s->form, true, s->gravity_on != 0, g, false, 0))) return rc;
+      if ((rc = engine_force(c, cp, cpd, s, g, false, 0))) return rc;
       c->pending_kick = true;
       c->pending_dt = s->dt;
       done = stalled + 1;

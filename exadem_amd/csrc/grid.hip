@@ -90,6 +90,7 @@ __global__ __launch_bounds__(256) void k_gather(int64_t n, const uint32_t* __res
   dst.type[i] = src.type[o];
   dst.group[i] = src.group[o];
   dst.bx[i] = xr.x; dst.by[i] = xr.y; dst.bz[i] = xr.z;  // backup_r
+  dst.bq[i] = c.q;
 }
 
 __global__ __launch_bounds__(256) void k_max_radius(int64_t n, PState s, uint64_t* __restrict__ out_bits) {
@@ -145,6 +146,7 @@ struct NbhArgs {
   const Core* core;
   const uint64_t* id;
   int64_t n;
+  int poly;  // polyhedra: owner = lower id always (nbh_polyhedron.cpp:234-236)
 };
 
 // Visits the stencil of particle x in stream order: dz, dy, dx in -1..1 (dx fastest), then the
@@ -282,7 +284,7 @@ __global__ __launch_bounds__(256) void k_nbh_fill(NbhArgs a, const uint32_t* __r
     const uint64_t idx = a.id[x];
     uint32_t k = 0;
     for_each_neighbour(a, x, [&](uint32_t y, int code) {
-      const bool owner = !(idx >= a.id[y] && code == kNoShift);
+      const bool owner = a.poly ? (idx < a.id[y]) : !(idx >= a.id[y] && code == kNoShift);
       word[b + (uint64_t)k * kWave] = inv2[y] | ((uint32_t)code << kShiftShift) | (owner ? kOwnerBit : 0u);
       owned += owner ? 1u : 0u;
       k++;
@@ -333,7 +335,8 @@ int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist) {
     c->nl_valid = true;
     return XDEM_OK;
   }
-  NbhArgs a{g, rcut_inc, nbh_dist * nbh_dist, c->cell_start, c->cell_rank, live(c).core, live(c).id, n};
+  NbhArgs a{g, rcut_inc, nbh_dist * nbh_dist, c->cell_start, c->cell_rank, live(c).core, live(c).id, n,
+            c->nshapes > 0 ? 1 : 0};
   const int64_t nw = (n + kWave - 1) / kWave;
   uint32_t* max_rows = c->d_u32tmp + 2;
   uint32_t* cnt_m = c->cell_of;  // list lengths in Morton order (cell_of is free after the binning)
@@ -380,6 +383,7 @@ int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist) {
   c->nl_cur = nxt;
   c->nl_valid = true;
   c->hist_ready = false;
+  c->nl_epoch++;
   return XDEM_OK;
 }
 

@@ -209,13 +209,17 @@ int launch_prolog(xdem_ctx* c, double dt, bool with_kick, double kick_dt, bool d
   PState s = live(c);
   const double kdt_2 = kick_dt * 0.5;
   const int64_t n = c->n;
+  // guarded launches (fused engine) become no-ops once a stall is raised, with or without the
+  // centre-displacement test (polyhedra test vertices in a kernel of their own)
   if (with_kick) {
     if (displ && guarded) LAUNCH1((k_prolog<true, true, true>), n, s, dt, kdt_2, thr2, c->d_flags);
     else if (displ) LAUNCH1((k_prolog<true, true, false>), n, s, dt, kdt_2, thr2, c->d_flags);
+    else if (guarded) LAUNCH1((k_prolog<true, false, true>), n, s, dt, kdt_2, thr2, c->d_flags);
     else LAUNCH1((k_prolog<true, false, false>), n, s, dt, kdt_2, thr2, c->d_flags);
   } else {
     if (displ && guarded) LAUNCH1((k_prolog<false, true, true>), n, s, dt, kdt_2, thr2, c->d_flags);
     else if (displ) LAUNCH1((k_prolog<false, true, false>), n, s, dt, kdt_2, thr2, c->d_flags);
+    else if (guarded) LAUNCH1((k_prolog<false, false, true>), n, s, dt, kdt_2, thr2, c->d_flags);
     else LAUNCH1((k_prolog<false, false, false>), n, s, dt, kdt_2, thr2, c->d_flags);
   }
   c->cells_valid = false;  // positions moved: the binning no longer matches them

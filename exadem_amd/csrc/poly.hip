@@ -1,0 +1,1081 @@
+// poly.hip — convex sphero-polyhedra: world vertices, feature-interaction lists with history carry-over,
+// and the atomic-free contact pass.
+//
+//   vertices   shape::get_vertex p + Q*(h v) (src/shape/include/exaDEM/shape.hpp:244-262),
+//              compute_vertices.hpp:50-67; vertex_displ_over (src/polyhedron/vertex_displ_over.cu:74-133)
+//   lists      nbh_polyhedron (src/polyhedron/nbh_polyhedron.cpp:223-376): owner = lower id, OBB test with both
+//              boxes enlarged by rVerlet/2 (then again), per-vertex OBB gate, VV, VE, VF, then EE, then the
+//              swap pass VE/VF from the partner's vertices; vertex-wall candidates
+//              (nbh_polyhedron_driver.hpp:216-236)
+//   filters    src/shape/include/exaDEM/shape_detection_core.hpp:43-53, 83-101, 245-309, 468-504
+//   detection  shape_detection_core.hpp:57-81, 122-148, 397-466, 520-566
+//   contact    src/polyhedron/include/exaDEM/polyhedron/contact.hpp:124-211 (symmetric), 219-297 (walls)
+//
+// Contact pass without atomics: one thread per particle evaluates every feature of the pairs it owns,
+// in list order, and writes per-pair subtotals (force on the owner, moment on the owner, moment on the
+// partner; the partner's force is the exact negation). A second kernel sums, per particle and in
+// stream order, the subtotals of its pairs (own slots, or the owner's slot through `mirror`), then the
+// vertex-wall contacts, then finishes force_to_accel and the angular acceleration.
+#include <algorithm>
+
+#include "contact_law.h"
+
+namespace xdem {
+
+constexpr uint32_t kFeatTypeMask = 3u;
+constexpr uint32_t kFeatSwap = 4u;
+constexpr uint32_t kFeatKeyMask = (1u << 11) - 1u;  // type, swap, owner sub, partner sub
+constexpr uint32_t kFeatActive = 1u << 30;
+
+__device__ inline uint32_t feat_word(int type, int swap, int osub, int psub) {
+  return (uint32_t)type | (swap ? kFeatSwap : 0u) | ((uint32_t)osub << 3) | ((uint32_t)psub << 7);
+}
+
+struct OBBd {
+  V3 c, e1, e2, e3, ext;
+};
+__device__ inline void obb_enlarge(OBBd& b, double more) {
+  b.ext.x += more;
+  b.ext.y += more;
+  b.ext.z += more;
+}
+__device__ inline double vcomp(const V3& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// OBB::intersect (src/shape/include/exaDEM/type/OBB.hpp:90-224), tolerance FLT_EPSILON
+__device__ inline bool obb_intersect(const OBBd& a, const OBBd& b) {
+  const double tol = 1.1920928955078125e-07;
+  const V3 ea[3] = {a.e1, a.e2, a.e3}, eb[3] = {b.e1, b.e2, b.e3};
+  const double ae[3] = {a.ext.x, a.ext.y, a.ext.z}, be[3] = {b.ext.x, b.ext.y, b.ext.z};
+  const V3 tt = a.c - b.c;
+  const double t[3] = {dot(tt, a.e1), dot(tt, a.e2), dot(tt, a.e3)};
+  double R[3][3], A[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      R[i][j] = dot(ea[i], eb[j]);
+      A[i][j] = fabs(R[i][j]) + tol;
+    }
+  double ra, rb;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    ra = ae[i];
+    rb = be[0] * A[i][0] + be[1] * A[i][1] + be[2] * A[i][2];
+    if (fabs(t[i]) > ra + rb) return false;
+  }
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    ra = ae[0] * A[0][j] + ae[1] * A[1][j] + ae[2] * A[2][j];
+    rb = be[j];
+    if (fabs(t[0] * R[0][j] + t[1] * R[1][j] + t[2] * R[2][j]) > ra + rb) return false;
+  }
+  ra = ae[1] * A[2][0] + ae[2] * A[1][0];
+  rb = be[1] * A[0][2] + be[2] * A[0][1];
+  if (fabs(t[2] * R[1][0] - t[1] * R[2][0]) > ra + rb) return false;
+  ra = ae[1] * A[2][1] + ae[2] * A[1][1];
+  rb = be[0] * A[0][2] + be[2] * A[0][0];
+  if (fabs(t[2] * R[1][1] - t[1] * R[2][1]) > ra + rb) return false;
+  ra = ae[1] * A[2][2] + ae[2] * A[1][2];
+  rb = be[0] * A[0][1] + be[1] * A[0][0];
+  if (fabs(t[2] * R[1][2] - t[1] * R[2][2]) > ra + rb) return false;
+  ra = ae[0] * A[2][0] + ae[2] * A[0][0];
+  rb = be[1] * A[1][2] + be[2] * A[1][1];
+  if (fabs(t[0] * R[2][0] - t[2] * R[0][0]) > ra + rb) return false;
+  ra = ae[0] * A[2][1] + ae[2] * A[0][1];
+  rb = be[0] * A[1][2] + be[2] * A[1][0];
+  if (fabs(t[0] * R[2][1] - t[2] * R[0][1]) > ra + rb) return false;
+  ra = ae[0] * A[2][2] + ae[2] * A[0][2];
+  rb = be[0] * A[1][1] + be[1] * A[1][0];
+  if (fabs(t[0] * R[2][2] - t[2] * R[0][2]) > ra + rb) return false;
+  ra = ae[0] * A[1][0] + ae[1] * A[0][0];
+  rb = be[1] * A[2][2] + be[2] * A[2][1];
+  if (fabs(t[1] * R[0][0] - t[0] * R[1][0]) > ra + rb) return false;
+  ra = ae[0] * A[1][1] + ae[1] * A[0][1];
+  rb = be[0] * A[2][2] + be[2] * A[2][0];
+  if (fabs(t[1] * R[0][1] - t[0] * R[1][1]) > ra + rb) return false;
+  ra = ae[0] * A[1][2] + ae[1] * A[0][2];
+  rb = be[0] * A[2][1] + be[1] * A[2][0];
+  if (fabs(t[1] * R[0][2] - t[0] * R[1][2]) > ra + rb) return false;
+  return true;
+}
+
+// compute_obb (src/shape/include/exaDEM/basic_types.hpp:37-49)
+__device__ inline OBBd compute_obb(const xdem_shape& sh, V3 p, const d4& q, double h) {
+  OBBd b;
+  b.c = {0, 0, 0};
+  b.e1 = qrot4(q, V3{1, 0, 0});
+  b.e2 = qrot4(q, V3{0, 1, 0});
+  b.e3 = qrot4(q, V3{0, 0, 1});
+  b.c = qrot4(q, b.c);
+  b.ext = {sh.obb_extent[0], sh.obb_extent[1], sh.obb_extent[2]};
+  if (h != 1.0) {
+    b.ext = b.ext * h;
+    b.c = b.c * h;
+  }
+  b.c = b.c + p;
+  return b;
+}
+
+__device__ inline V3 get_vertex(const xdem_shape& sh, int k, V3 p, double h, const d4& q) {
+  const V3 v{sh.vertices[k][0], sh.vertices[k][1], sh.vertices[k][2]};
+  return p + qrot4(q, h * v);
+}
+__device__ inline V3 vnormalize(V3 v) { return v / norm(v); }
+
+struct Verts {
+  V3 v[XDEM_MAX_SHAPE_VERTICES];
+};
+
+__device__ inline void load_verts(const d4* __restrict__ vtx, int64_t p, int nv, V3 sh, Verts& out) {
+  for (int k = 0; k < nv; k++) {
+    const d4 a = vtx[p * XDEM_MAX_SHAPE_VERTICES + k];
+    out.v[k] = {a.x + sh.x, a.y + sh.y, a.z + sh.z};
+  }
+}
+
+__device__ inline bool filter_vv(double rv, V3 vi, double ri, V3 vj, double rj) {
+  const double R = ri + rj + rv;
+  const V3 d = vi - vj;
+  return dot(d, d) <= R * R;
+}
+__device__ inline bool filter_ve(double rv, V3 vi, double ri, V3 vf, V3 vs, double rj) {
+  const V3 fs = vs - vf;
+  const V3 fi = vi - vf;
+  const double r = dot(fs, fi) / dot(fs, fs);
+  if (r < -rv || r > 1.0 + rv) return false;
+  const V3 d = fi - fs * r;
+  const double Rmax = ri + rj + rv;
+  return dot(d, d) < Rmax * Rmax;
+}
+// vertex-face core shared by the filter (limit = ri + rj + rVerlet) and the detection (limit = ri + rj)
+__device__ inline bool vf_core(V3 vi, const Verts& vj, const xdem_shape& shj, int face, double limit, double& dist_out,
+                               V3& n_out, bool& early) {
+  const int nf = shj.face_size[face];
+  const int* f = shj.faces[face];
+  const V3 va = vj.v[f[0]];
+  const V3 vb = vj.v[f[1]];
+  const V3 vc = vj.v[f[nf - 1]];
+  const V3 v = vi - va;
+  V3 v1 = vb - va;
+  V3 v2 = vc - va;
+  v1 = vnormalize(v1);
+  V3 n = cross(v1, v2);
+  n = vnormalize(n);
+  double dist = dot(n, v);
+  if (dist < 0) {
+    n = n * (-1);
+    dist = -dist;
+  }
+  early = false;
+  if (dist > limit) {
+    early = true;
+    return false;
+  }
+  const V3 P = vi - n * dist;
+  int odd = 0;
+  v2 = cross(n, v1);
+  const double ori1 = dot(P, v1), ori2 = dot(P, v2);
+  for (int iva = 0; iva < nf; ++iva) {
+    int ivb = iva + 1;
+    if (ivb == nf) ivb = 0;
+    const V3 A = vj.v[f[iva]], B = vj.v[f[ivb]];
+    const double pa1 = dot(A, v1), pb1 = dot(B, v1), pa2 = dot(A, v2), pb2 = dot(B, v2);
+    if ((pa2 < ori2 && pb2 >= ori2) || (pb2 < ori2 && pa2 >= ori2)) {
+      if (pa1 + (ori2 - pa2) / (pb2 - pa2) * (pb1 - pa1) < ori1) odd = 1 - odd;
+    }
+  }
+  dist_out = dist;
+  n_out = n;
+  return odd == 1;
+}
+__device__ inline bool filter_ee(double rv, V3 vfi, V3 vsi, double ri, V3 vfj, V3 vsj, double rj) {
+  const double R = ri + rj + rv;
+  const V3 Ei = vsi - vfi, Ej = vsj - vfj, v = vfi - vfj;
+  const double c = dot(Ei, Ei), d = dot(Ej, Ej), e = dot(Ei, Ej);
+  double f = (c * d) - (e * e);
+  if (fabs(f) > 1.0e-12) {
+    f = 1.0 / f;
+    const double a = dot(Ei, v), b = dot(Ej, v);
+    const double s = (e * b - a * d) * f, t = (c * b - e * a) * f;
+    if (s <= 0.0 || s >= 1.0 || t <= 0.0 || t >= 1.0) return false;
+    const V3 pi = vfi + Ei * s, pj = vfj + Ej * t;
+    const V3 n = pi - pj;
+    return dot(n, n) <= R * R;
+  }
+  return false;
+}
+
+struct Det {
+  bool contact;
+  double dn;
+  V3 n, cp;
+};
+__device__ inline Det det_none() { return {false, 0, {0, 0, 0}, {0, 0, 0}}; }
+__device__ inline Det detect_vv(V3 pi, double ri, V3 pj, double rj) {
+  const double R = ri + rj;
+  const V3 dist = pi - pj;
+  const double dnorm = norm(dist);
+  const double inv = 1.0 / dnorm;
+  const double dn = dnorm - R;
+  const V3 n = dist * inv;
+  return {dn < 0, dn, n, pi - n * (ri + 0.5 * dn)};
+}
+__device__ inline Det detect_ve(V3 vi, double ri, V3 vf, V3 vs, double rj) {
+  const V3 fs = vs - vf, fi = vi - vf;
+  const double r = dot(fs, fi) / dot(fs, fs);
+  if (r <= 0 || r >= 1.0) return det_none();
+  V3 n = fi - fs * r;
+  const double dn = norm(n) - (ri + rj);
+  if (dn > 0) return det_none();
+  n = vnormalize(n);
+  return {true, dn, n, vi - n * (ri + 0.5 * dn)};
+}
+__device__ inline Det detect_vf(V3 vi, double ri, const Verts& vj, const xdem_shape& shj, int face, double rj) {
+  double dist;
+  V3 n;
+  bool early;
+  const bool in = vf_core(vi, vj, shj, face, ri + rj, dist, n, early);
+  if (early) return det_none();
+  const double dn = dist - (ri + rj);
+  return {in, dn, n, vi - n * (ri + 0.5 * dn)};
+}
+__device__ inline Det detect_ee(V3 vfi, V3 vsi, double ri, V3 vfj, V3 vsj, double rj) {
+  const double R = ri + rj;
+  const V3 Ei = vsi - vfi, Ej = vsj - vfj, v = vfi - vfj;
+  const double c = dot(Ei, Ei), d = dot(Ej, Ej), e = dot(Ei, Ej);
+  double f = (c * d) - (e * e);
+  if (fabs(f) > 1.0e-12) {
+    f = 1.0 / f;
+    const double a = dot(Ei, v), b = dot(Ej, v);
+    const double s = (e * b - a * d) * f, t = (c * b - e * a) * f;
+    if (s <= 0.0 || s >= 1.0 || t <= 0.0 || t >= 1.0) return det_none();
+    const V3 pi = vfi + Ei * s, pj = vfj + Ej * t;
+    V3 n = pi - pj;
+    const double dn = norm(n) - R;
+    if (dn > 0) return det_none();
+    n = vnormalize(n);
+    return {dn <= 0, dn, n, pi - n * (ri + 0.5 * dn)};
+  }
+  return det_none();
+}
+
+// ------------------------------------------------------------------------------------------ vertices
+__global__ __launch_bounds__(256) void k_poly_vertices(int64_t n, PState s, const xdem_shape* __restrict__ shapes,
+                                                       d4* __restrict__ vtx, int displ, double thr2,
+                                                       int32_t* __restrict__ flags, int guarded) {
+  if (guarded && flags[1]) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Core c = s.core[i];
+  const xdem_shape& sh = shapes[s.type[i]];
+  const V3 p{c.xr.x, c.xr.y, c.xr.z};
+  const double h = c.w.w;
+  bool moved = false;
+  const V3 pb{s.bx[i], s.by[i], s.bz[i]};
+  const d4 qb = s.bq[i];
+  for (int k = 0; k < sh.nv; k++) {
+    const V3 v = get_vertex(sh, k, p, h, c.q);
+    vtx[i * XDEM_MAX_SHAPE_VERTICES + k] = {v.x, v.y, v.z, 0.0};
+    if (displ) {
+      const V3 a = get_vertex(sh, k, pb, h, qb);
+      const V3 d = v - a;
+      if (dot(d, d) >= thr2) moved = true;
+    }
+  }
+  if (moved) flags[0] = 1;
+}
+
+int poly_compute_vertices(xdem_ctx* c, bool displ, double thr2, bool guarded) {
+  if (c->n == 0) return XDEM_OK;
+  if (c->cap_vtx < c->cap) {
+    if (c->vtx) (void)hipFree(c->vtx);
+    c->vtx = nullptr;
+    XDEM_HIP(c, hipMalloc((void**)&c->vtx, sizeof(d4) * XDEM_MAX_SHAPE_VERTICES * (size_t)c->cap));
+    c->cap_vtx = c->cap;
+  }
+  hipLaunchKernelGGL(k_poly_vertices, dim3(grid_blocks(c->n, 256)), dim3(256), 0, c->stream, c->n, live(c),
+                     c->d_shapes, c->vtx, displ ? 1 : 0, thr2, c->d_flags, guarded ? 1 : 0);
+  XDEM_HIP(c, hipGetLastError());
+  return XDEM_OK;
+}
+
+// ------------------------------------------------------------------------------------------ feature lists
+struct PolyNbhArgs {
+  int64_t n;
+  PState s;
+  Grid g;
+  const xdem_shape* shapes;
+  const d4* vtx;
+  NbrList nl;
+  double rv;
+};
+
+// Enumerates the feature interactions of the pair (x owner, y partner seen through image `code`) in the
+// order of nbh_polyhedron; fn(type, swap, owner_sub, partner_sub).
+template <typename F>
+__device__ inline void for_each_feature(const PolyNbhArgs& a, int64_t x, uint32_t y, int code, F&& fn) {
+  const Core cx = a.s.core[x], cy = a.s.core[y];
+  const xdem_shape& sa = a.shapes[a.s.type[x]];
+  const xdem_shape& sb = a.shapes[a.s.type[y]];
+  const double ha = cx.w.w, hb = cy.w.w;
+  const double ra = sa.minkowski * ha, rb = sb.minkowski * hb;
+  const V3 sh = shift_vec(a.g, code);
+  OBBd obb_i = compute_obb(sa, V3{cx.xr.x, cx.xr.y, cx.xr.z}, cx.q, ha);
+  OBBd obb_j = compute_obb(sb, V3{cy.xr.x, cy.xr.y, cy.xr.z} + sh, cy.q, hb);
+  obb_enlarge(obb_i, 0.5 * a.rv);
+  obb_enlarge(obb_j, 0.5 * a.rv);
+  if (!obb_intersect(obb_i, obb_j)) return;
+  obb_enlarge(obb_i, 0.5 * a.rv);
+  obb_enlarge(obb_j, 0.5 * a.rv);
+  Verts va, vb;
+  load_verts(a.vtx, x, sa.nv, V3{0, 0, 0}, va);
+  load_verts(a.vtx, y, sb.nv, sh, vb);
+  const double rv = a.rv;
+  for (int i = 0; i < sa.nv; i++) {
+    OBBd ov;
+    ov.c = va.v[i];
+    ov.e1 = {1, 0, 0};
+    ov.e2 = {0, 1, 0};
+    ov.e3 = {0, 0, 1};
+    ov.ext = {0, 0, 0};
+    obb_enlarge(ov, ra);
+    if (!obb_intersect(obb_j, ov)) continue;
+    for (int j = 0; j < sb.nv; j++)
+      if (filter_vv(rv, va.v[i], ra, vb.v[j], rb)) fn(0, 0, i, j);
+    for (int j = 0; j < sb.ne; j++)
+      if (filter_ve(rv, va.v[i], ra, vb.v[sb.edges[j][0]], vb.v[sb.edges[j][1]], rb)) fn(1, 0, i, j);
+    for (int j = 0; j < sb.nf; j++) {
+      double dist;
+      V3 nn;
+      bool early;
+      if (vf_core(va.v[i], vb, sb, j, ra + rb + rv, dist, nn, early)) fn(2, 0, i, j);
+    }
+  }
+  for (int i = 0; i < sa.ne; i++)
+    for (int j = 0; j < sb.ne; j++)
+      if (filter_ee(rv, va.v[sa.edges[i][0]], va.v[sa.edges[i][1]], ra, vb.v[sb.edges[j][0]], vb.v[sb.edges[j][1]],
+                    rb))
+        fn(3, 0, i, j);
+  for (int j = 0; j < sb.nv; j++) {
+    OBBd ov;
+    ov.c = vb.v[j];
+    ov.e1 = {1, 0, 0};
+    ov.e2 = {0, 1, 0};
+    ov.e3 = {0, 0, 1};
+    ov.ext = {0, 0, 0};
+    obb_enlarge(ov, rb);
+    if (!obb_intersect(obb_i, ov)) continue;
+    for (int i = 0; i < sa.ne; i++)
+      if (filter_ve(rv, vb.v[j], rb, va.v[sa.edges[i][0]], va.v[sa.edges[i][1]], ra)) fn(1, 1, i, j);
+    for (int i = 0; i < sa.nf; i++) {
+      double dist;
+      V3 nn;
+      bool early;
+      if (vf_core(vb.v[j], va, sa, i, rb + ra + rv, dist, nn, early)) fn(2, 1, i, j);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_feat_count(PolyNbhArgs a, uint32_t* __restrict__ fcount) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= a.n) return;
+  const uint32_t cnt = a.nl.cnt[x];
+  const uint64_t b = a.nl.base[x >> 6] + (uint64_t)(x & 63);
+  for (uint32_t k = 0; k < cnt; k++) {
+    const uint64_t sl = b + (uint64_t)k * kWave;
+    const uint32_t wd = a.nl.word[sl];
+    uint32_t nf = 0;
+    if (wd & kOwnerBit)
+      for_each_feature(a, x, wd & kIdxMask, (int)((wd >> kShiftShift) & 31u), [&](int, int, int, int) { nf++; });
+    fcount[sl] = nf;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_feat_fill(PolyNbhArgs a, const uint64_t* __restrict__ start,
+                                                   uint32_t* __restrict__ feat) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= a.n) return;
+  const uint32_t cnt = a.nl.cnt[x];
+  const uint64_t b = a.nl.base[x >> 6] + (uint64_t)(x & 63);
+  for (uint32_t k = 0; k < cnt; k++) {
+    const uint64_t sl = b + (uint64_t)k * kWave;
+    const uint32_t wd = a.nl.word[sl];
+    if (!(wd & kOwnerBit)) continue;
+    uint64_t o = start[sl];
+    for_each_feature(a, x, wd & kIdxMask, (int)((wd >> kShiftShift) & 31u), [&](int t, int sw, int si, int sj) {
+      // keys use owner().sub_, partner().sub_: the swap pass stores the vertex of the partner as partner sub
+      feat[o++] = sw ? feat_word(t, 1, si, sj) : feat_word(t, 0, si, sj);
+    });
+  }
+}
+
+// Owner slot of every partner entry: the owner's entry pointing back with the mirrored image code.
+__global__ __launch_bounds__(256) void k_mirror(int64_t n, NbrList nl, uint32_t* __restrict__ mirror) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  const uint32_t cnt = nl.cnt[x];
+  const uint64_t b = nl.base[x >> 6] + (uint64_t)(x & 63);
+  for (uint32_t k = 0; k < cnt; k++) {
+    const uint64_t sl = b + (uint64_t)k * kWave;
+    const uint32_t wd = nl.word[sl];
+    if (wd & kOwnerBit) {
+      mirror[sl] = (uint32_t)sl;
+      continue;
+    }
+    const uint32_t y = wd & kIdxMask;
+    const uint32_t code = (wd >> kShiftShift) & 31u;
+    const uint32_t want = 26u - code;
+    const uint32_t cy = nl.cnt[y];
+    const uint64_t by = nl.base[y >> 6] + (uint64_t)(y & 63);
+    uint32_t found = 0xffffffffu;
+    for (uint32_t k2 = 0; k2 < cy; k2++) {
+      const uint64_t s2 = by + (uint64_t)k2 * kWave;
+      const uint32_t w2 = nl.word[s2];
+      if ((w2 & kOwnerBit) && (w2 & kIdxMask) == (uint32_t)x && ((w2 >> kShiftShift) & 31u) == want) {
+        found = (uint32_t)s2;
+        break;
+      }
+    }
+    mirror[sl] = found;
+  }
+}
+
+struct FeatCarryArgs {
+  int64_t n;
+  NbrList nl, ol;
+  PolyList pl, opl;
+  const uint32_t* perm;
+  int have_old;
+};
+
+// history of each new feature: the old owner pair (old owner, old partner) and the same feature key
+__global__ __launch_bounds__(256) void k_feat_carry(FeatCarryArgs a) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= a.n) return;
+  const uint32_t cnt = a.nl.cnt[x];
+  const uint64_t b = a.nl.base[x >> 6] + (uint64_t)(x & 63);
+  for (uint32_t k = 0; k < cnt; k++) {
+    const uint64_t sl = b + (uint64_t)k * kWave;
+    const uint32_t wd = a.nl.word[sl];
+    if (!(wd & kOwnerBit)) continue;
+    const uint64_t f0 = a.pl.start[sl], f1 = a.pl.start[sl + 1];
+    uint64_t o0 = 0, o1 = 0;
+    if (a.have_old) {
+      const uint32_t ox = a.perm[x], oy = a.perm[wd & kIdxMask];
+      const uint32_t ocnt = a.ol.cnt[ox];
+      const uint64_t ob = a.ol.base[ox >> 6] + (uint64_t)(ox & 63);
+      for (uint32_t k2 = 0; k2 < ocnt; k2++) {
+        const uint64_t s2 = ob + (uint64_t)k2 * kWave;
+        const uint32_t w2 = a.ol.word[s2];
+        if ((w2 & kOwnerBit) && (w2 & kIdxMask) == oy) {
+          o0 = a.opl.start[s2];
+          o1 = a.opl.start[s2 + 1];
+          break;
+        }
+      }
+    }
+    for (uint64_t f = f0; f < f1; f++) {
+      const uint32_t key = a.pl.feat[f] & kFeatKeyMask;
+      double hx = 0, hy = 0, hz = 0, mx = 0, my = 0, mz = 0;
+      bool act = false;
+      for (uint64_t g = o0; g < o1; g++) {
+        const uint32_t w2 = a.opl.feat[g];
+        if ((w2 & kFeatKeyMask) == key) {
+          if (w2 & kFeatActive) {
+            hx = a.opl.hx[g]; hy = a.opl.hy[g]; hz = a.opl.hz[g];
+            mx = a.opl.mhx[g]; my = a.opl.mhy[g]; mz = a.opl.mhz[g];
+            act = true;
+          }
+          break;
+        }
+      }
+      a.pl.hx[f] = hx; a.pl.hy[f] = hy; a.pl.hz[f] = hz;
+      a.pl.mhx[f] = mx; a.pl.mhy[f] = my; a.pl.mhz[f] = mz;
+      a.pl.feat[f] = key | (act ? kFeatActive : 0u);
+    }
+  }
+}
+
+// vertex-wall candidates (Surface::filter with rVerlet + Minkowski radius) and their history
+__global__ __launch_bounds__(256) void k_poly_wall(int64_t n, PState s, const xdem_shape* __restrict__ shapes,
+                                                   const d4* __restrict__ vtx, int nw, const Surface* __restrict__ walls,
+                                                   double rv, PolyWall nwl, PolyWall owl, const uint32_t* __restrict__ perm,
+                                                   int have_old, int64_t stride) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  const xdem_shape& sh = shapes[s.type[x]];
+  const double r = sh.minkowski * s.core[x].w.w;
+  const uint32_t o = have_old ? perm[x] : 0u;
+  const uint64_t omask = have_old ? owl.mask[o] : 0ull;
+  uint64_t mask = 0;
+  for (int w = 0; w < nw; w++)
+    for (int k = 0; k < sh.nv; k++) {
+      const d4 v4 = vtx[x * XDEM_MAX_SHAPE_VERTICES + k];
+      const V3 p{v4.x, v4.y, v4.z};
+      const Surface& sf = walls[w];
+      const V3 proj = dot(p, sf.normal) * sf.normal;
+      const double d = norm(proj - sf.center_proj);
+      const int bit = w * XDEM_MAX_SHAPE_VERTICES + k;
+      const int64_t j = (int64_t)bit * stride + x;
+      double hx = 0, hy = 0, hz = 0, mx = 0, my = 0, mz = 0;
+      if (d <= rv + r) {
+        mask |= 1ull << bit;
+        if ((omask >> bit) & 1ull) {
+          const int64_t jo = (int64_t)bit * stride + o;
+          hx = owl.hx[jo]; hy = owl.hy[jo]; hz = owl.hz[jo];
+          mx = owl.mhx[jo]; my = owl.mhy[jo]; mz = owl.mhz[jo];
+        }
+      }
+      nwl.hx[j] = hx; nwl.hy[j] = hy; nwl.hz[j] = hz;
+      nwl.mhx[j] = mx; nwl.mhy[j] = my; nwl.mhz[j] = mz;
+    }
+  nwl.mask[x] = mask;
+}
+
+template <typename T>
+static int galloc(xdem_ctx* c, T** p, int64_t count) {
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  XDEM_HIP(c, hipMalloc((void**)p, sizeof(T) * (size_t)std::max<int64_t>(count, 1)));
+  return XDEM_OK;
+}
+
+int poly_nbh(xdem_ctx* c, double rcut_inc) {
+  if (!c->nl_valid) return set_err(c, XDEM_ERR_STATE, "nbh_polyhedron: run chunk_neighbors_contact first");
+  const int64_t n = c->n;
+  int rc;
+  if ((rc = poly_compute_vertices(c, false, 0.0, false))) return rc;
+  const NbrList& L = c->nl[c->nl_cur];
+  const int nxt = 1 - c->pl_cur;
+  PolyList& P = c->pl[nxt];
+  if (L.slots + 1 > P.cap_slots) {
+    const int64_t cap = (int64_t)(L.slots * 1.25) + 4096;
+    if ((rc = galloc(c, &P.start, cap + 1)) || (rc = galloc(c, &P.fcount, cap))) return rc;
+    P.cap_slots = cap;
+  }
+  if (L.slots > c->cap_mirror) {
+    const int64_t cap = (int64_t)(L.slots * 1.25) + 4096;
+    if ((rc = galloc(c, &c->mirror, cap)) || (rc = galloc(c, &c->pair_buf, cap * 9))) return rc;
+    c->cap_mirror = cap;
+    c->cap_pair = cap;
+  }
+  XDEM_HIP(c, hipMemsetAsync(P.fcount, 0, sizeof(uint32_t) * std::max<int64_t>(L.slots, 1), c->stream));
+  PolyNbhArgs a{n, live(c), c->grid, c->d_shapes, c->vtx, L, rcut_inc};
+  if (n > 0) {
+    hipLaunchKernelGGL(k_feat_count, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a, P.fcount);
+    if ((rc = scan_u64_from_u32(c, P.fcount, P.start, L.slots, 1u))) return rc;
+  }
+  uint64_t nf = 0;
+  if (n > 0 && (rc = read_u64(c, P.start + L.slots, &nf))) return rc;
+  if ((int64_t)nf > P.cap_feat) {
+    const int64_t cap = (int64_t)(nf * 1.25) + 4096;
+    if ((rc = galloc(c, &P.feat, cap)) || (rc = galloc(c, &P.hx, cap)) || (rc = galloc(c, &P.hy, cap)) ||
+        (rc = galloc(c, &P.hz, cap)) || (rc = galloc(c, &P.mhx, cap)) || (rc = galloc(c, &P.mhy, cap)) ||
+        (rc = galloc(c, &P.mhz, cap)))
+      return rc;
+    P.cap_feat = cap;
+  }
+  P.nfeat = (int64_t)nf;
+  if (n > 0) {
+    hipLaunchKernelGGL(k_feat_fill, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a, P.start, P.feat);
+    hipLaunchKernelGGL(k_mirror, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, L, c->mirror);
+    const bool have_old = c->nl_prev_valid && c->perm_valid && c->pl_valid;
+    FeatCarryArgs fa{n, L, c->nl[1 - c->nl_cur], P, c->pl[c->pl_cur], c->perm, have_old ? 1 : 0};
+    hipLaunchKernelGGL(k_feat_carry, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, fa);
+    XDEM_HIP(c, hipGetLastError());
+    // vertex-wall candidates
+    if (c->nwalls > 0) {
+      const int pn = 1 - c->pw_cur;
+      const int64_t m = (int64_t)c->nwalls * XDEM_MAX_SHAPE_VERTICES * c->cap;
+      for (int k = 0; k < 2; k++) {
+        PolyWall& W = c->pw[k];
+        if (W.cap < m) {
+          if ((rc = galloc(c, &W.mask, c->cap)) || (rc = galloc(c, &W.hx, m)) || (rc = galloc(c, &W.hy, m)) ||
+              (rc = galloc(c, &W.hz, m)) || (rc = galloc(c, &W.mhx, m)) || (rc = galloc(c, &W.mhy, m)) ||
+              (rc = galloc(c, &W.mhz, m)))
+            return rc;
+          W.cap = m;
+          if (k == c->pw_cur) c->pw_valid = false;
+        }
+      }
+      const bool wold = have_old && c->pw_valid;
+      hipLaunchKernelGGL(k_poly_wall, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, live(c), c->d_shapes,
+                         c->vtx, c->nwalls, c->d_walls, rcut_inc, c->pw[pn], c->pw[c->pw_cur], c->perm,
+                         wold ? 1 : 0, c->cap);
+      XDEM_HIP(c, hipGetLastError());
+      c->pw_cur = pn;
+      c->pw_valid = true;
+    }
+  }
+  c->pl_cur = nxt;
+  c->pl_valid = true;
+  c->pl_epoch = c->nl_epoch;
+  c->nl_prev_valid = false;
+  c->perm_valid = false;
+  c->hist_ready = true;
+  return XDEM_OK;
+}
+
+// ------------------------------------------------------------------------------------------ forces
+struct PolyForceArgs {
+  int64_t n;
+  PState s;
+  Grid g;
+  const xdem_shape* shapes;
+  const d4* vtx;
+  NbrList nl;
+  PolyList pl;
+  PolyWall pw;
+  const uint32_t* mirror;
+  double* pair;
+  const Surface* walls;
+  int nw;
+  int64_t wstride;
+  ContactParams cp, cpd;
+  double dt;
+  V3 grav;
+  int gravity_on;
+  int32_t* flags;
+  int32_t step_tag;
+};
+
+__device__ inline bool poly_guard(const PolyForceArgs& a) {
+  if (a.flags[1]) return true;
+  if (a.flags[0]) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      a.flags[1] = 1;
+      a.flags[2] = a.step_tag;
+    }
+    return true;
+  }
+  return false;
+}
+
+template <bool KR, bool GUARD>
+__global__ __launch_bounds__(128) void k_poly_pairs(PolyForceArgs a) {
+  if (GUARD && poly_guard(a)) return;
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= a.n) return;
+  const PState& s = a.s;
+  const uint32_t cnt = a.nl.cnt[x];
+  const uint64_t b = a.nl.base[x >> 6] + (uint64_t)(x & 63);
+  const Core cx = s.core[x];
+  const xdem_shape& sa = a.shapes[s.type[x]];
+  const double ra = sa.minkowski * cx.w.w;
+  Verts va;
+  load_verts(a.vtx, x, sa.nv, V3{0, 0, 0}, va);
+  const V3 rx{cx.xr.x, cx.xr.y, cx.xr.z}, vx{cx.vm.x, cx.vm.y, cx.vm.z}, wx{cx.w.x, cx.w.y, cx.w.z};
+  for (uint32_t k = 0; k < cnt; k++) {
+    const uint64_t sl = b + (uint64_t)k * kWave;
+    const uint32_t wd = a.nl.word[sl];
+    if (!(wd & kOwnerBit)) continue;
+    const uint32_t y = wd & kIdxMask;
+    const V3 sh = shift_vec(a.g, (int)((wd >> kShiftShift) & 31u));
+    const Core cy = s.core[y];
+    const xdem_shape& sb = a.shapes[s.type[y]];
+    const double rb = sb.minkowski * cy.w.w;
+    const V3 ry{cy.xr.x + sh.x, cy.xr.y + sh.y, cy.xr.z + sh.z}, vy{cy.vm.x, cy.vm.y, cy.vm.z},
+        wy{cy.w.x, cy.w.y, cy.w.z};
+    const double mx = cx.vm.w, my = cy.vm.w;
+    V3 Fo{0, 0, 0}, Mo{0, 0, 0}, Mp{0, 0, 0};
+    const uint64_t f0 = a.pl.start[sl], f1 = a.pl.start[sl + 1];
+    if (f1 > f0) {
+      Verts vb;
+      load_verts(a.vtx, y, sb.nv, sh, vb);
+      for (uint64_t f = f0; f < f1; f++) {
+        const uint32_t fw = a.pl.feat[f];
+        const int type = (int)(fw & kFeatTypeMask);
+        const bool swap = (fw & kFeatSwap) != 0;
+        const int osub = (int)((fw >> 3) & 15u), psub = (int)((fw >> 7) & 15u);
+        // evaluation particle i is the vertex side: the owner unless swap
+        const Verts& vi = swap ? vb : va;
+        const Verts& vj = swap ? va : vb;
+        const xdem_shape& si = swap ? sb : sa;
+        const xdem_shape& sj = swap ? sa : sb;
+        const double ri = swap ? rb : ra, rj = swap ? ra : rb;
+        const int fi = swap ? psub : osub, fj = swap ? osub : psub;
+        Det d;
+        if (type == 0) d = detect_vv(vi.v[fi], ri, vj.v[fj], rj);
+        else if (type == 1) d = detect_ve(vi.v[fi], ri, vj.v[sj.edges[fj][0]], vj.v[sj.edges[fj][1]], rj);
+        else if (type == 2) d = detect_vf(vi.v[fi], ri, vj, sj, fj, rj);
+        else d = detect_ee(vi.v[si.edges[fi][0]], vi.v[si.edges[fi][1]], ri, vj.v[sj.edges[fj][0]],
+                           vj.v[sj.edges[fj][1]], rj);
+        if (d.contact) {
+          const V3 r_i = swap ? ry : rx, r_j = swap ? rx : ry;
+          const V3 v_i = swap ? vy : vx, v_j = swap ? vx : vy;
+          const V3 w_i = swap ? wy : wx, w_j = swap ? wx : wy;
+          const double mi = swap ? my : mx, mj = swap ? mx : my;
+          const double meff = (mi * mj) / (mi + mj);
+          V3 ft{0, 0, 0}, mint{0, 0, 0};
+          if (fw & kFeatActive) {
+            ft = {a.pl.hx[f], a.pl.hy[f], a.pl.hz[f]};
+            if (KR) mint = {a.pl.mhx[f], a.pl.mhy[f], a.pl.mhz[f]};
+          }
+          const V3 fc = hooke_force<KR>(d.dn, d.n, a.dt, a.cp, meff, ft, mint, d.cp, r_i, v_i, w_i, r_j, v_j, w_j);
+          const bool act = !(ft.x == 0 && ft.y == 0 && ft.z == 0 && mint.x == 0 && mint.y == 0 && mint.z == 0);
+          if (act || (fw & kFeatActive)) {
+            a.pl.hx[f] = ft.x; a.pl.hy[f] = ft.y; a.pl.hz[f] = ft.z;
+            if (KR) {
+              a.pl.mhx[f] = mint.x; a.pl.mhy[f] = mint.y; a.pl.mhz[f] = mint.z;
+            }
+          }
+          const uint32_t nfw = act ? (fw | kFeatActive) : (fw & ~kFeatActive);
+          if (nfw != fw) a.pl.feat[f] = nfw;
+          const V3 nf = -fc;
+          if (!swap) {  // owner is i
+            Fo = Fo + fc;
+            Mo = Mo + (cross(d.cp - r_i, fc) + mint);
+            Mp = Mp + (cross(d.cp - r_j, nf) + (-mint));
+          } else {  // owner is j
+            Fo = Fo + nf;
+            Mo = Mo + (cross(d.cp - r_j, nf) + (-mint));
+            Mp = Mp + (cross(d.cp - r_i, fc) + mint);
+          }
+        } else if (fw & kFeatActive) {
+          a.pl.hx[f] = 0.0; a.pl.hy[f] = 0.0; a.pl.hz[f] = 0.0;
+          a.pl.mhx[f] = 0.0; a.pl.mhy[f] = 0.0; a.pl.mhz[f] = 0.0;
+          a.pl.feat[f] = fw & ~kFeatActive;
+        }
+      }
+    }
+    double* pr = a.pair + 9 * sl;
+    pr[0] = Fo.x; pr[1] = Fo.y; pr[2] = Fo.z;
+    pr[3] = Mo.x; pr[4] = Mo.y; pr[5] = Mo.z;
+    pr[6] = Mp.x; pr[7] = Mp.y; pr[8] = Mp.z;
+  }
+}
+
+struct Q4p {
+  double w, x, y, z;
+};
+__device__ inline V3 angular_accel_p(const d4& q, V3 mom, V3 vrot, V3 I) {
+  const d4 qi{q.x, -q.y, -q.z, -q.w};
+  const V3 om = qrot4(qi, vrot);
+  const V3 M = qrot4(qi, mom);
+  const V3 dom{(M.x - (I.z - I.y) * om.y * om.z) / I.x, (M.y - (I.x - I.z) * om.z * om.x) / I.y,
+               (M.z - (I.y - I.x) * om.x * om.y) / I.z};
+  return qrot4(q, dom);
+}
+
+template <bool FUSED, bool KR, bool GUARD>
+__global__ __launch_bounds__(256) void k_poly_gather(PolyForceArgs a) {
+  // the stall decision was taken by k_poly_pairs of the same step
+  if (GUARD && a.flags[1]) return;
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= a.n) return;
+  const PState& s = a.s;
+  const Core cx = s.core[x];
+  const double mx = cx.vm.w;
+  V3 F, M;
+  if (FUSED) {
+    F = {0.0, 0.0, 0.0};
+    M = {0.0, 0.0, 0.0};
+    if (a.gravity_on) F = {F.x + a.grav.x * mx, F.y + a.grav.y * mx, F.z + a.grav.z * mx};
+  } else {
+    F = {s.ax[x], s.ay[x], s.az[x]};
+    M = {s.mx[x], s.my[x], s.mz[x]};
+  }
+  const uint32_t cnt = a.nl.cnt[x];
+  const uint64_t b = a.nl.base[x >> 6] + (uint64_t)(x & 63);
+  for (uint32_t k = 0; k < cnt; k++) {
+    const uint64_t sl = b + (uint64_t)k * kWave;
+    const uint32_t wd = a.nl.word[sl];
+    if (wd & kOwnerBit) {
+      const double* pr = a.pair + 9 * sl;
+      F = F + V3{pr[0], pr[1], pr[2]};
+      M = M + V3{pr[3], pr[4], pr[5]};
+    } else {
+      const uint32_t m = a.mirror[sl];
+      if (m == 0xffffffffu) continue;
+      const double* pr = a.pair + 9 * (uint64_t)m;
+      F = F + V3{-pr[0], -pr[1], -pr[2]};
+      M = M + V3{pr[6], pr[7], pr[8]};
+    }
+  }
+  if (a.nw > 0) {
+    const xdem_shape& sh = a.shapes[s.type[x]];
+    const double r = sh.minkowski * cx.w.w;
+    const uint64_t mask = a.pw.mask[x];
+    const V3 rx{cx.xr.x, cx.xr.y, cx.xr.z}, vx{cx.vm.x, cx.vm.y, cx.vm.z}, wx{cx.w.x, cx.w.y, cx.w.z};
+    for (int w = 0; w < a.nw; w++)
+      for (int k = 0; k < sh.nv; k++) {
+        const int bit = w * XDEM_MAX_SHAPE_VERTICES + k;
+        if (!((mask >> bit) & 1ull)) continue;
+        const int64_t j = (int64_t)bit * a.wstride + x;
+        const d4 v4 = a.vtx[x * XDEM_MAX_SHAPE_VERTICES + k];
+        const V3 p{v4.x, v4.y, v4.z};
+        const Surface& sf = a.walls[w];
+        const V3 proj = dot(p, sf.normal) * sf.normal;
+        const V3 s2p = -(sf.center_proj - proj);
+        const double d = norm(s2p);
+        const double dn = d - r;
+        if (dn > 0) {
+          a.pw.hx[j] = 0.0; a.pw.hy[j] = 0.0; a.pw.hz[j] = 0.0;
+          a.pw.mhx[j] = 0.0; a.pw.mhy[j] = 0.0; a.pw.mhz[j] = 0.0;
+          continue;
+        }
+        const V3 nrm = s2p / d;
+        const V3 cpos = p - nrm * (r + 0.5 * dn);
+        V3 ft{a.pw.hx[j], a.pw.hy[j], a.pw.hz[j]};
+        V3 mint{0.0, 0.0, 0.0};
+        if (KR) mint = {a.pw.mhx[j], a.pw.mhy[j], a.pw.mhz[j]};
+        const V3 f = hooke_force<KR>(dn, nrm, a.dt, a.cpd, mx, ft, mint, cpos, rx, vx, wx, sf.center, sf.vel, sf.vrot);
+        a.pw.hx[j] = ft.x; a.pw.hy[j] = ft.y; a.pw.hz[j] = ft.z;
+        if (KR) {
+          a.pw.mhx[j] = mint.x; a.pw.mhy[j] = mint.y; a.pw.mhz[j] = mint.z;
+        }
+        F = F + f;
+        M = M + (cross(cpos - rx, f) + mint);
+      }
+  }
+  if (FUSED) {
+    s.ax[x] = F.x / mx;
+    s.ay[x] = F.y / mx;
+    s.az[x] = F.z / mx;
+    s.mx[x] = M.x;
+    s.my[x] = M.y;
+    s.mz[x] = M.z;
+    const V3 al = angular_accel_p(cx.q, M, V3{cx.w.x, cx.w.y, cx.w.z}, V3{s.ix[x], s.iy[x], s.iz[x]});
+    s.alx[x] = al.x;
+    s.aly[x] = al.y;
+    s.alz[x] = al.z;
+  } else {
+    s.ax[x] = F.x;
+    s.ay[x] = F.y;
+    s.az[x] = F.z;
+    s.mx[x] = M.x;
+    s.my[x] = M.y;
+    s.mz[x] = M.z;
+  }
+}
+
+int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool fused, bool gravity_on,
+               V3 g, bool guarded, int32_t step_tag) {
+  if (!c->pl_valid || !c->hist_ready || c->pl_epoch != c->nl_epoch)
+    return set_err(c, XDEM_ERR_STATE, "contact_polyhedron: run nbh_polyhedron after chunk_neighbors_contact");
+  if (cp.kr != 0.0 || cpd.kr != 0.0) c->kr_used = true;
+  if (c->n == 0) return XDEM_OK;
+  PolyForceArgs a{};
+  a.n = c->n;
+  a.s = live(c);
+  a.g = c->grid;
+  a.shapes = c->d_shapes;
+  a.vtx = c->vtx;
+  a.nl = c->nl[c->nl_cur];
+  a.pl = c->pl[c->pl_cur];
+  a.mirror = c->mirror;
+  a.pair = c->pair_buf;
+  a.nw = (c->nwalls > 0 && c->pw_valid) ? c->nwalls : 0;
+  if (a.nw) a.pw = c->pw[c->pw_cur];
+  a.walls = c->d_walls;
+  a.wstride = c->cap;
+  a.cp = cp;
+  a.cpd = cpd;
+  a.dt = dt;
+  a.grav = g;
+  a.gravity_on = gravity_on ? 1 : 0;
+  a.flags = c->d_flags;
+  a.step_tag = step_tag;
+  const dim3 gp(grid_blocks(a.n, 128)), bp(128), gg(grid_blocks(a.n, 256)), bg(256);
+  const bool kr = c->kr_used;
+#define PAIRS(KRV, G) hipLaunchKernelGGL((k_poly_pairs<KRV, G>), gp, bp, 0, c->stream, a)
+#define GATHER(F, KRV, G) hipLaunchKernelGGL((k_poly_gather<F, KRV, G>), gg, bg, 0, c->stream, a)
+  if (kr) {
+    if (guarded) PAIRS(true, true); else PAIRS(true, false);
+  } else {
+    if (guarded) PAIRS(false, true); else PAIRS(false, false);
+  }
+  if (fused) {
+    if (kr) { if (guarded) GATHER(true, true, true); else GATHER(true, true, false); }
+    else { if (guarded) GATHER(true, false, true); else GATHER(true, false, false); }
+  } else {
+    if (kr) { if (guarded) GATHER(false, true, true); else GATHER(false, true, false); }
+    else { if (guarded) GATHER(false, false, true); else GATHER(false, false, false); }
+  }
+#undef PAIRS
+#undef GATHER
+  XDEM_HIP(c, hipGetLastError());
+  return XDEM_OK;
+}
+
+}  // namespace xdem
+
+using namespace xdem;
+
+// ------------------------------------------------------------------------------------------ C ABI
+extern "C" int xdem_set_shapes(xdem_ctx* c, const xdem_shape* shapes, int32_t n) {
+  if (!c || (n > 0 && !shapes)) return XDEM_ERR_ARG;
+  if (n < 0 || n > XDEM_MAX_SHAPES) return set_err(c, XDEM_ERR_CAPACITY, "at most 8 shapes");
+  for (int k = 0; k < n; k++) {
+    const xdem_shape& s = shapes[k];
+    if (s.nv < 1 || s.nv > XDEM_MAX_SHAPE_VERTICES || s.ne < 0 || s.ne > XDEM_MAX_SHAPE_EDGES || s.nf < 0 ||
+        s.nf > XDEM_MAX_SHAPE_FACES)
+      return set_err(c, XDEM_ERR_CAPACITY, "shape: too many vertices, edges or faces");
+    for (int f = 0; f < s.nf; f++)
+      if (s.face_size[f] < 3 || s.face_size[f] > XDEM_MAX_FACE_VERTICES)
+        return set_err(c, XDEM_ERR_CAPACITY, "shape: faces need 3 or 4 vertices");
+    for (int e = 0; e < s.ne; e++)
+      if (s.edges[e][0] < 0 || s.edges[e][0] >= s.nv || s.edges[e][1] < 0 || s.edges[e][1] >= s.nv)
+        return set_err(c, XDEM_ERR_ARG, "shape: edge vertex out of range");
+    for (int f = 0; f < s.nf; f++)
+      for (int v = 0; v < s.face_size[f]; v++)
+        if (s.faces[f][v] < 0 || s.faces[f][v] >= s.nv) return set_err(c, XDEM_ERR_ARG, "shape: face vertex out of range");
+  }
+  c->nshapes = n;
+  for (int k = 0; k < n; k++) c->shapes[k] = shapes[k];
+  if (!c->d_shapes) XDEM_HIP(c, hipMalloc((void**)&c->d_shapes, sizeof(xdem_shape) * XDEM_MAX_SHAPES));
+  XDEM_HIP(c, hipMemcpyAsync(c->d_shapes, c->shapes, sizeof(xdem_shape) * XDEM_MAX_SHAPES, hipMemcpyHostToDevice,
+                             c->stream));
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  c->nl_valid = false;
+  c->pl_valid = false;
+  return XDEM_OK;
+}
+
+static int poly_ready(xdem_ctx* c) {
+  if (c->nshapes == 0) return set_err(c, XDEM_ERR_STATE, "no shapes registered (xdem_set_shapes)");
+  if (c->n > 0 && c->max_type >= (uint32_t)c->nshapes)
+    return set_err(c, XDEM_ERR_ARG, "a particle type has no registered shape");
+  return XDEM_OK;
+}
+
+extern "C" int xdem_compute_vertices(xdem_ctx* c) {
+  if (!c) return XDEM_ERR_ARG;
+  int rc = poly_ready(c);
+  if (rc) return rc;
+  return poly_compute_vertices(c, false, 0.0, false);
+}
+
+extern "C" int xdem_vertex_displ_over(xdem_ctx* c, double threshold, int32_t* result) {
+  if (!c || !result) return XDEM_ERR_ARG;
+  int rc = poly_ready(c);
+  if (rc) return rc;
+  *result = 0;
+  XDEM_HIP(c, hipMemsetAsync(c->d_flags, 0, sizeof(int32_t), c->stream));
+  if ((rc = poly_compute_vertices(c, true, threshold * threshold, false))) return rc;
+  XDEM_HIP(c, hipMemcpyAsync(c->h_flags, c->d_flags, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  *result = c->h_flags[0] ? 1 : 0;
+  XDEM_HIP(c, hipMemsetAsync(c->d_flags, 0, sizeof(int32_t), c->stream));
+  return XDEM_OK;
+}
+
+extern "C" int xdem_nbh_polyhedron(xdem_ctx* c, double rcut_inc) {
+  if (!c) return XDEM_ERR_ARG;
+  int rc = poly_ready(c);
+  if (rc) return rc;
+  return poly_nbh(c, rcut_inc);
+}
+
+extern "C" int xdem_contact_polyhedron(xdem_ctx* c, const xdem_contact_params* cp, const xdem_contact_params* cpd,
+                                       double dt) {
+  if (!c || !cp) return XDEM_ERR_ARG;
+  int rc = poly_ready(c);
+  if (rc) return rc;
+  if (cp->dncut > 0 || cp->fc != 0 || cp->gamma > 0)
+    return set_err(c, XDEM_ERR_UNSUPPORTED, "contact_polyhedron: cohesive and DMT laws are not provided");
+  if (c->nwalls > 0 && !cpd)
+    return set_err(c, XDEM_ERR_ARG, "contact_polyhedron: config_driver is required when surfaces are registered");
+  ContactParams a{cp->kn, cp->kt, cp->kr, cp->mu, cp->damp_rate, cp->fc, cp->dncut, cp->gamma};
+  ContactParams b{};
+  if (cpd) b = {cpd->kn, cpd->kt, cpd->kr, cpd->mu, cpd->damp_rate, cpd->fc, cpd->dncut, cpd->gamma};
+  return poly_force(c, a, b, dt, false, false, V3{0, 0, 0}, false, 0);
+}
+
+extern "C" int xdem_poly_interactions_dump(xdem_ctx* c, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint16_t* type,
+                                           uint8_t* swap, uint32_t* sub_i, uint32_t* sub_j, double* friction,
+                                           int64_t* written) {
+  if (!c || !id_i || !id_j || !type || !swap || !sub_i || !sub_j || !friction) return XDEM_ERR_ARG;
+  if (written) *written = 0;
+  if (!c->pl_valid || c->n == 0) return XDEM_OK;
+  if (c->pl_epoch != c->nl_epoch) return set_err(c, XDEM_ERR_STATE, "poly dump: run nbh_polyhedron first");
+  const int64_t n = c->n;
+  const NbrList& L = c->nl[c->nl_cur];
+  const PolyList& P = c->pl[c->pl_cur];
+  const int64_t nw = (n + kWave - 1) / kWave;
+  std::vector<uint32_t> cnt(n), word(L.slots), feat(P.nfeat);
+  std::vector<uint64_t> base(nw + 1), ids(n), start(L.slots + 1);
+  std::vector<double> hx(P.nfeat), hy(P.nfeat), hz(P.nfeat);
+  std::vector<uint32_t> types(n);
+  XDEM_HIP(c, hipMemcpyAsync(cnt.data(), L.cnt, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipMemcpyAsync(base.data(), L.base, sizeof(uint64_t) * (nw + 1), hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipMemcpyAsync(ids.data(), live(c).id, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipMemcpyAsync(types.data(), live(c).type, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+  if (L.slots > 0) {
+    XDEM_HIP(c, hipMemcpyAsync(word.data(), L.word, sizeof(uint32_t) * L.slots, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(start.data(), P.start, sizeof(uint64_t) * (L.slots + 1), hipMemcpyDeviceToHost,
+                               c->stream));
+  }
+  if (P.nfeat > 0) {
+    XDEM_HIP(c, hipMemcpyAsync(feat.data(), P.feat, sizeof(uint32_t) * P.nfeat, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(hx.data(), P.hx, sizeof(double) * P.nfeat, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(hy.data(), P.hy, sizeof(double) * P.nfeat, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(hz.data(), P.hz, sizeof(double) * P.nfeat, hipMemcpyDeviceToHost, c->stream));
+  }
+  std::vector<uint64_t> wmask;
+  std::vector<double> whx, why, whz;
+  const bool walls = c->nwalls > 0 && c->pw_valid;
+  if (walls) {
+    const PolyWall& W = c->pw[c->pw_cur];
+    const int64_t m = (int64_t)c->nwalls * XDEM_MAX_SHAPE_VERTICES * c->cap;
+    wmask.resize(n);
+    whx.resize(m); why.resize(m); whz.resize(m);
+    XDEM_HIP(c, hipMemcpyAsync(wmask.data(), W.mask, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(whx.data(), W.hx, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(why.data(), W.hy, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(whz.data(), W.hz, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+  }
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  int64_t k = 0;
+  for (int64_t x = 0; x < n; x++) {
+    if (walls)
+      for (int w = 0; w < c->nwalls; w++)
+        for (int v = 0; v < c->shapes[types[x]].nv; v++) {
+          const int bit = w * XDEM_MAX_SHAPE_VERTICES + v;
+          if (!((wmask[x] >> bit) & 1ull)) continue;
+          if (k >= cap) return set_err(c, XDEM_ERR_CAPACITY, "poly dump: cap too small");
+          const int64_t j = (int64_t)bit * c->cap + x;
+          id_i[k] = ids[x]; id_j[k] = (uint64_t)w; type[k] = 5; swap[k] = 0;
+          sub_i[k] = (uint32_t)v; sub_j[k] = 0xffffffffu;
+          friction[3 * k] = whx[j]; friction[3 * k + 1] = why[j]; friction[3 * k + 2] = whz[j];
+          k++;
+        }
+    const uint64_t b = base[x >> 6] + (uint64_t)(x & 63);
+    for (uint32_t e = 0; e < cnt[x]; e++) {
+      const uint64_t s = b + (uint64_t)e * kWave;
+      const uint32_t wd = word[s];
+      if (!(wd & kOwnerBit)) continue;
+      for (uint64_t f = start[s]; f < start[s + 1]; f++) {
+        if (k >= cap) return set_err(c, XDEM_ERR_CAPACITY, "poly dump: cap too small");
+        const uint32_t fw = feat[f];
+        id_i[k] = ids[x];
+        id_j[k] = ids[wd & kIdxMask];
+        type[k] = (uint16_t)(fw & kFeatTypeMask);
+        swap[k] = (fw & kFeatSwap) ? 1 : 0;
+        sub_i[k] = (fw >> 3) & 15u;
+        sub_j[k] = (fw >> 7) & 15u;
+        friction[3 * k] = hx[f]; friction[3 * k + 1] = hy[f]; friction[3 * k + 2] = hz[f];
+        k++;
+      }
+    }
+  }
+  if (written) *written = k;
+  return XDEM_OK;
+}
+
+extern "C" int xdem_poly_interactions_count(xdem_ctx* c, int64_t* n) {
+  if (!c || !n) return XDEM_ERR_ARG;
+  *n = 0;
+  if (!c->pl_valid) return XDEM_OK;
+  if (c->pl_epoch != c->nl_epoch) return set_err(c, XDEM_ERR_STATE, "poly dump: run nbh_polyhedron first");
+  int64_t walls = 0;
+  if (c->nwalls > 0 && c->pw_valid) {
+    std::vector<uint64_t> m(c->n);
+    XDEM_HIP(c, hipMemcpyAsync(m.data(), c->pw[c->pw_cur].mask, sizeof(uint64_t) * c->n, hipMemcpyDeviceToHost,
+                               c->stream));
+    XDEM_HIP(c, hipStreamSynchronize(c->stream));
+    for (auto v : m) walls += __builtin_popcountll(v);
+  }
+  *n = c->pl[c->pl_cur].nfeat + walls;
+  return XDEM_OK;
+}

@@ -70,6 +70,16 @@ struct Grid {
   int64_t ncells;
 };
 
+// quat * vec3 (src/shape/include/exaDEM/type/quat.hpp:84-91); q stored as d4 {w, x, y, z} in {x, y, z, w}
+__host__ __device__ inline V3 qrot4(const d4& q, V3 v) {
+  const V3 qv{q.y, q.z, q.w};
+  V3 uv = cross(qv, v);
+  V3 uuv = cross(qv, uv);
+  uv = uv * (2.0 * q.x);
+  uuv = uuv * 2.0;
+  return v + uv + uuv;
+}
+
 __host__ __device__ inline V3 shift_vec(const Grid& g, int code) {
   const int sx = code % 3 - 1, sy = (code / 3) % 3 - 1, sz = code / 9 - 1;
   return {sx * g.L[0], sy * g.L[1], sz * g.L[2]};
@@ -91,6 +101,7 @@ struct PState {
   double *alx, *aly, *alz;
   double *ix, *iy, *iz;
   double *bx, *by, *bz;
+  d4* bq;  // orientation at the last rebuild (vertex displacement trigger of polyhedra)
   uint64_t* id;
   uint32_t* type;
   uint32_t* group;
@@ -128,6 +139,28 @@ struct WallList {
   double* mhx = nullptr;
   double* mhy = nullptr;
   double* mhz = nullptr;
+};
+
+// Polyhedron feature interactions, grouped by owner list slot (CSR over the ELL slots):
+// features of slot s are [start[s], start[s+1]); one 32-bit word each:
+//   bits 0..1 type (0 VV, 1 VE, 2 VF, 3 EE), bit 2 swap, bits 3..6 owner sub, bits 7..10 partner sub,
+//   bit 30 history non-zero. Friction history in three FP64 arrays (moment history alongside).
+struct PolyList {
+  uint64_t* start = nullptr;  // [cap_slots + 1]
+  uint32_t* feat = nullptr;   // [cap_feat]
+  double *hx = nullptr, *hy = nullptr, *hz = nullptr;
+  double *mhx = nullptr, *mhy = nullptr, *mhz = nullptr;
+  uint32_t* fcount = nullptr;  // [cap_slots] scratch: features per slot
+  int64_t cap_slots = 0;
+  int64_t cap_feat = 0;
+  int64_t nfeat = 0;
+};
+// Vertex-wall candidates: per particle a 64-bit mask (bit w*8 + vertex), history per (wall, vertex, particle).
+struct PolyWall {
+  uint64_t* mask = nullptr;
+  double *hx = nullptr, *hy = nullptr, *hz = nullptr;
+  double *mhx = nullptr, *mhy = nullptr, *mhz = nullptr;
+  int64_t cap = 0;
 };
 
 struct KernelTimer {
@@ -173,6 +206,9 @@ struct xdem_ctx {
   bool perm_valid = false;
   bool cells_valid = false;     // cell_start matches the current positions and order
   bool hist_ready = false;      // history of the current list has been carried over (nbh_sphere)
+  uint32_t max_type = 0;        // largest particle type of the last upload
+  uint64_t nl_epoch = 0;        // number of neighbour builds so far
+  uint64_t pl_epoch = ~0ull;    // neighbour build the feature lists were made from
   bool wl_valid = false;        // wall list exists (for the order it was built in)
   bool kr_used = false;         // a rolling stiffness was used: moment histories are live
   bool hist_fp32 = false;       // the live history of the list is the FP32 copy
@@ -196,6 +232,23 @@ struct xdem_ctx {
 
   bool timing = false;
   xdem::KernelTimer timers[2];
+
+  // polyhedra (set when shapes are registered)
+  int nshapes = 0;
+  xdem_shape shapes[XDEM_MAX_SHAPES];
+  xdem_shape* d_shapes = nullptr;
+  xdem::d4* vtx = nullptr;          // [cap * 8] world vertices (x, y, z, pad)
+  int64_t cap_vtx = 0;
+  xdem::PolyList pl[2];             // feature lists, current and previous
+  int pl_cur = 0;
+  bool pl_valid = false;
+  double* pair_buf = nullptr;       // [cap_pair * 9] per owner slot: F_owner, M_owner, M_partner
+  int64_t cap_pair = 0;
+  uint32_t* mirror = nullptr;       // [cap_mirror] owner slot of each partner entry
+  int64_t cap_mirror = 0;
+  xdem::PolyWall pw[2];             // vertex-wall candidates and history, current and previous
+  int pw_cur = 0;
+  bool pw_valid = false;
 };
 
 // helpers shared between translation units
@@ -232,4 +285,9 @@ int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd,
                  bool gravity_on, V3 g, bool guarded, int32_t step_tag);
 int alloc_walls(xdem_ctx* c, int nw);
 int set_history_form(xdem_ctx* c, bool fp32);
+// polyhedra
+int poly_compute_vertices(xdem_ctx* c, bool displ, double thr2, bool guarded);
+int poly_nbh(xdem_ctx* c, double rcut_inc);
+int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool fused, bool gravity_on,
+               V3 g, bool guarded, int32_t step_tag);
 }  // namespace xdem

@@ -155,3 +155,57 @@ def polydisperse(n=1_000_000, phi=0.55, rmin=0.5, rmax=1.0, seed_radius=2, seed_
     cp = contact_params(kn=1e5, mu=0.3, damp_rate=0.9)
     return Workload("polydisperse", p, dom, [], cp, cp, dt, steps, rcut_inc, gravity,
                     {"phi": phi, "box": tuple(hi), "relax_sweeps": int(sweeps), "max_overlap": mo.value})
+
+
+# ---------------------------------------------------------------------------------------------- polyhedra
+def random_quaternions(seed, n):
+    """Uniform random unit quaternions (Shoemake), (w, x, y, z)."""
+    u = uniform(seed, 4, 3 * n, 0.0, 1.0).reshape(3, n)
+    u1, u2, u3 = u
+    a, b = np.sqrt(1.0 - u1), np.sqrt(u1)
+    return np.stack([b * np.cos(2 * np.pi * u3), a * np.sin(2 * np.pi * u2), a * np.cos(2 * np.pi * u2),
+                     b * np.sin(2 * np.pi * u3)])
+
+
+def polyhedra(n=200_000, spacing=1.7, jitter=0.05, seed=4, density=2500.0, minkowski=0.02, dt=2e-6, steps=1000,
+              rcut_inc=0.1, nx=None, gravity=(0.0, 0.0, -9.81), sigma_v=0.0, z0=None, periodic=(True, True, False)):
+    """Cubes (edge 1.0) and octahedra (circumradius 0.8), 50/50 by id parity, random orientations, on a
+    jittered simple-cubic lattice; periodic in x and y, floor wall at z = 0 and lid above."""
+    from .native import contact_params
+    from . import shapes as shp
+    defs = [shp.cube(1.0, minkowski), shp.octahedron(0.8, minkowski)]
+    shapes = [d.to_record() for d in defs]
+    rb = [shp.bounding_radius(d) for d in defs]
+    if nx is None:
+        nx = max(3, int(round((n / 2.0) ** (1.0 / 3.0))))
+    ny = nx
+    nz = int(math.ceil(n / (nx * ny)))
+    k = np.arange(n)
+    ix, iy, iz = k % nx, (k // nx) % ny, k // (nx * ny)
+    jit = uniform(seed, 1, 3 * n, -jitter, jitter).reshape(3, n)
+    L = nx * spacing
+    rx = (ix + 0.5) * spacing + jit[0]
+    ry = (iy + 0.5) * spacing + jit[1]
+    rz = (max(rb) + 0.05 if z0 is None else z0) + iz * spacing + jit[2]
+    types = (k % 2).astype(np.uint32)
+    p = sphere_set(rx, ry, rz, np.array([rb[t] for t in types]), density)
+    vol = np.array([shapes[t].volume for t in types])
+    im = np.array([shapes[t].inertia_on_mass[0] for t in types])
+    p["mass"] = density * vol
+    for a in ("inx", "iny", "inz"):
+        p[a] = p["mass"] * im
+    q = random_quaternions(seed, n)
+    p["qw"], p["qx"], p["qy"], p["qz"] = (np.ascontiguousarray(q[i]) for i in range(4))
+    p["type"] = types
+    if sigma_v > 0:
+        v = normal(seed + 1, 2, 3 * n, 0.0, sigma_v).reshape(3, n)
+        p["vx"], p["vy"], p["vz"] = (np.ascontiguousarray(v[i]) for i in range(3))
+    top = float(rz.max() + max(rb) + 2.0)
+    cell = 2 * max(rb) + rcut_inc
+    dom = domain((0.0, 0.0, 0.0), (L, L, top), tuple(periodic), cell)
+    walls = [{"offset": 0.0, "normal": (0.0, 0.0, 1.0)}, {"offset": top, "normal": (0.0, 0.0, 1.0)}]
+    cp = contact_params(kn=1e5, mu=0.3, damp_rate=0.9)
+    w = Workload("polyhedra", p, dom, walls, cp, cp, dt, steps, rcut_inc, gravity,
+                 {"spacing": spacing, "box": (L, L, top), "nx": nx})
+    w.shapes = shapes
+    return w

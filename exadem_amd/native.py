@@ -44,6 +44,13 @@ class Surface(C.Structure):
                 ("vel", C.c_double * 3), ("vrot", C.c_double * 3)]
 
 
+class Shape(C.Structure):
+    _fields_ = [("nv", C.c_int32), ("ne", C.c_int32), ("nf", C.c_int32), ("pad", C.c_int32),
+                ("minkowski", C.c_double), ("volume", C.c_double), ("inertia_on_mass", C.c_double * 3),
+                ("obb_extent", C.c_double * 3), ("vertices", (C.c_double * 3) * 8), ("edges", (C.c_int32 * 2) * 12),
+                ("face_size", C.c_int32 * 8), ("faces", (C.c_int32 * 4) * 8)]
+
+
 class Scheme(C.Structure):
     _fields_ = [("dt", C.c_double), ("gravity", C.c_double * 3), ("gravity_on", C.c_int32),
                 ("form", C.c_int32), ("cp", ContactParams), ("cpd", ContactParams),
@@ -69,6 +76,8 @@ HIP_SYMBOLS = [
     "xdem_chunk_neighbors_contact", "xdem_nbh_sphere", "xdem_classify_interactions", "xdem_contact_sphere",
     "xdem_run_steps", "xdem_interactions_count", "xdem_interactions_dump", "xdem_nbh_stats",
     "xdem_kernel_timing", "xdem_set_timing", "xdem_streams_dump",
+    "xdem_set_shapes", "xdem_compute_vertices", "xdem_vertex_displ_over", "xdem_nbh_polyhedron",
+    "xdem_contact_polyhedron", "xdem_poly_interactions_count", "xdem_poly_interactions_dump",
 ]
 
 
@@ -145,6 +154,14 @@ def load():
         "xdem_nbh_stats": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
         "xdem_kernel_timing": (C.c_int, [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
         "xdem_set_timing": (C.c_int, [vp, C.c_int32]),
+        "xdem_set_shapes": (C.c_int, [vp, C.POINTER(Shape), C.c_int32]),
+        "xdem_compute_vertices": (C.c_int, [vp]),
+        "xdem_vertex_displ_over": (C.c_int, [vp, C.c_double, C.POINTER(C.c_int32)]),
+        "xdem_nbh_polyhedron": (C.c_int, [vp, C.c_double]),
+        "xdem_contact_polyhedron": (C.c_int, [vp, C.POINTER(ContactParams), C.POINTER(ContactParams), C.c_double]),
+        "xdem_poly_interactions_count": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "xdem_poly_interactions_dump": (C.c_int, [vp, C.c_int64, _u64p, _u64p, C.POINTER(C.c_uint16),
+                                                  C.POINTER(C.c_uint8), _u32p, _u32p, _dp, C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -318,6 +335,40 @@ class Context:
                   _ptr(own, C.c_uint8), C.byref(wr))
         k = wr.value
         return a[:k], b[:k], code[:k], own[:k]
+
+    # ---- polyhedra
+    def set_shapes(self, shapes):
+        arr = (Shape * len(shapes))(*shapes)
+        self.call("xdem_set_shapes", arr, len(shapes))
+
+    def compute_vertices(self):
+        self.call("xdem_compute_vertices")
+
+    def vertex_displ_over(self, threshold):
+        r = C.c_int32()
+        self.call("xdem_vertex_displ_over", threshold, C.byref(r))
+        return bool(r.value)
+
+    def nbh_polyhedron(self, rcut_inc):
+        self.call("xdem_nbh_polyhedron", rcut_inc)
+
+    def contact_polyhedron(self, cp, cpd=None, dt=1e-5):
+        self.call("xdem_contact_polyhedron", C.byref(cp), C.byref(cpd) if cpd is not None else None, dt)
+
+    def poly_interactions(self):
+        n = C.c_int64()
+        self.call("xdem_poly_interactions_count", C.byref(n))
+        cap = max(1, n.value)
+        ii, jj = np.zeros(cap, np.uint64), np.zeros(cap, np.uint64)
+        tt, sw = np.zeros(cap, np.uint16), np.zeros(cap, np.uint8)
+        si, sj = np.zeros(cap, np.uint32), np.zeros(cap, np.uint32)
+        fr = np.zeros(3 * cap)
+        wr = C.c_int64()
+        self.call("xdem_poly_interactions_dump", cap, _ptr(ii, C.c_uint64), _ptr(jj, C.c_uint64),
+                  _ptr(tt, C.c_uint16), _ptr(sw, C.c_uint8), _ptr(si, C.c_uint32), _ptr(sj, C.c_uint32),
+                  _ptr(fr, C.c_double), C.byref(wr))
+        k = wr.value
+        return ii[:k], jj[:k], tt[:k], sw[:k], si[:k], sj[:k], fr[:3 * k].reshape(-1, 3)
 
     def nbh_stats(self):
         pairs, slots = C.c_int64(), C.c_int64()

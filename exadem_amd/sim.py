@@ -69,3 +69,37 @@ class SphereSim:
         st = self.ctx.run_steps(scheme(self.w, self.form, poll), nsteps)
         self.rebuilds += st["rebuilds"]
         return st
+
+
+class PolySim(SphereSim):
+    """The polyhedra scheme (data/config/config_polyhedra.msp): compute_vertices on the fast path,
+    vertex_displ_over as the rebuild trigger, nbh_polyhedron and contact_polyhedron."""
+
+    def __init__(self, workload, device=0, ctx=None):
+        super().__init__(workload, device, native.XDEM_FORM_FP64, ctx)
+        self.ctx.set_shapes(workload.shapes)
+
+    def rebuild(self):
+        c = self.ctx
+        c.move_particles()
+        c.chunk_neighbors_contact(self.w.rcut_inc)
+        c.nbh_polyhedron(self.w.rcut_inc)
+
+    def compute_force(self):
+        c = self.ctx
+        if any(g != 0 for g in self.w.gravity):
+            c.gravity_force(self.w.gravity)
+        c.contact_polyhedron(self.w.cp, self.w.cpd, self.w.dt)
+
+    def step(self):
+        c, w = self.ctx, self.w
+        c.combined_compute_prolog(w.dt)
+        if c.vertex_displ_over(0.5 * w.rcut_inc):
+            self.rebuild()
+            self.rebuilds += 1
+        else:
+            c.compute_vertices()
+        c.reset_force_moment()
+        self.compute_force()
+        c.force_to_accel()
+        c.combined_compute_epilog(w.dt)

@@ -95,6 +95,27 @@ typedef struct xdem_surface {
   double vrot[3];
 } xdem_surface;
 
+/* Convex sphero-polyhedron shape (src/shape/include/exaDEM/shape.hpp:50-466): vertices, edges and
+ * faces of the core polyhedron at homothety 1, its Minkowski radius, volume and inertia over mass,
+ * and its oriented bounding box in the body frame (axes x, y, z, centre 0, extents including the
+ * Minkowski radius). Faces list their vertex indices in order around the face. */
+#define XDEM_MAX_SHAPES 8
+#define XDEM_MAX_SHAPE_VERTICES 8
+#define XDEM_MAX_SHAPE_EDGES 12
+#define XDEM_MAX_SHAPE_FACES 8
+#define XDEM_MAX_FACE_VERTICES 4
+typedef struct xdem_shape {
+  int32_t nv, ne, nf, pad;
+  double minkowski;
+  double volume;
+  double inertia_on_mass[3];
+  double obb_extent[3];
+  double vertices[XDEM_MAX_SHAPE_VERTICES][3];
+  int32_t edges[XDEM_MAX_SHAPE_EDGES][2];
+  int32_t face_size[XDEM_MAX_SHAPE_FACES];
+  int32_t faces[XDEM_MAX_SHAPE_FACES][XDEM_MAX_FACE_VERTICES];
+} xdem_shape;
+
 /* One time step of numerical_scheme_verlet_macro
  * (data/config/config_numerical_schemes.msp:6-28) with
  * compute_force = [gravity_force, contact_sphere]. */
@@ -171,12 +192,37 @@ int xdem_contact_sphere(xdem_ctx* ctx, const xdem_contact_params* cp, const xdem
                         double dt, int32_t form);
 
 /* Fused engine: nsteps iterations of numerical_scheme_verlet_macro with
- * compute_force = [gravity_force, contact_sphere], rebuilding
+ * compute_force = [gravity_force, contact_sphere] (or contact_polyhedron once shapes are registered,
+ * with compute_vertices and vertex_displ_over as in config_polyhedra.msp), rebuilding
  * (move_particles, chunk_neighbors_contact, nbh_sphere) whenever
  * particle_displ_over fires. Produces the same state as the operator-by-
  * operator sequence. Requires a prior neighbour build and force pass
  * (first_iteration). stats may be NULL. */
 int xdem_run_steps(xdem_ctx* ctx, const xdem_scheme* scheme, int64_t nsteps, xdem_run_stats* stats);
+
+/* ------------------------------------------------------------- polyhedra */
+/* Shape table (ParticleTypeInt -> shape, as loaded by read_shape_file); particle `type` indexes it,
+ * `radius` must hold the bounding radius h * (max |vertex| + minkowski) (set_fields.cu:296-310). */
+int xdem_set_shapes(xdem_ctx* ctx, const xdem_shape* shapes, int32_t n);
+/* compute_vertices (src/polyhedron/compute_vertices.cu:68-108): world vertices p + Q*(h v). */
+int xdem_compute_vertices(xdem_ctx* ctx);
+/* vertex_displ_over (src/polyhedron/vertex_displ_over.cu:43-143): *result = 1 when a vertex moved by
+ * at least `threshold` since the last rebuild. */
+int xdem_vertex_displ_over(xdem_ctx* ctx, double threshold, int32_t* result);
+/* nbh_polyhedron (src/polyhedron/nbh_polyhedron.cpp:75-384): per owner pair (lower id) the OBB prefilter
+ * and the VV/VE/VF/EE feature filters, vertex-wall candidates, friction history carried by key
+ * (owner id, partner id, type, swap, owner sub, partner sub). Requires chunk_neighbors_contact. */
+int xdem_nbh_polyhedron(xdem_ctx* ctx, double rcut_inc);
+/* contact_polyhedron / contact_singlemat_polyhedron[_hooke[_none]]
+ * (src/polyhedron/contact_force_polyhedron.cu.in:44-319): adds forces and moments of every feature
+ * interaction (both particles) and vertex-wall interaction to f and mom. Deterministic, no atomics. */
+int xdem_contact_polyhedron(xdem_ctx* ctx, const xdem_contact_params* cp, const xdem_contact_params* cp_driver,
+                            double dt);
+/* Feature interactions held (walls included) and their dump: owner id, partner id (wall index), type
+ * (0 VV, 1 VE, 2 VF, 3 EE, 5 vertex-wall), swap, owner sub, partner sub, friction (3 per entry). */
+int xdem_poly_interactions_count(xdem_ctx* ctx, int64_t* n);
+int xdem_poly_interactions_dump(xdem_ctx* ctx, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint16_t* type,
+                                uint8_t* swap, uint32_t* sub_i, uint32_t* sub_j, double* friction, int64_t* written);
 
 /* ------------------------------------------------------------ inspection */
 /* Number of particle-particle interactions held (owner side, half list) and

@@ -35,6 +35,8 @@
 // Build: see Makefile in this directory (g++ -O2 -ffp-contract=off -fopenmp).
 
 #include <algorithm>
+#include <array>
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -225,6 +227,11 @@ struct Oracle {
   bool built = false;
   int threads = 0;
   std::string err;
+  // polyhedra
+  bool poly = false;
+  std::vector<xdem_shape> shapes;
+  std::vector<std::array<V3, 8>> verts;  // world vertices per particle
+  std::vector<Q4> backup_q;              // orientation at the last rebuild (vertex_displ_over)
 };
 
 static inline V3 shift_vec(const Oracle& o, int code) {
@@ -277,6 +284,7 @@ static void move_particles(Oracle& o) {
   apply(o.id); apply(o.type); apply(o.group);
   apply(o.r); apply(o.v); apply(o.f); apply(o.mom); apply(o.vrot); apply(o.arot); apply(o.inertia);
   apply(o.mass); apply(o.radius); apply(o.homothety); apply(o.q);
+  if (o.poly) apply(o.backup_q);
   // interaction storage travels with its particle
   {
     auto tmp = std::move(o.items);
@@ -289,6 +297,7 @@ static void move_particles(Oracle& o) {
   for (int64_t i = 0; i < n; i++) o.cell_start[cell[perm[i]] + 1]++;
   for (int64_t c = 0; c < nc; c++) o.cell_start[c + 1] += o.cell_start[c];
   o.backup = o.r;
+  if (o.poly) o.backup_q = o.q;
 }
 
 // chunk_neighbors_contact: full (both direction) streams with the DEM filter.
@@ -563,6 +572,535 @@ static CP to_cp(const xdem_contact_params* p) {
   return c;
 }
 
+
+// ============================================================================ polyhedra
+// Restates, for convex sphero-polyhedra (paths in the exaDEM source tree):
+//   vertices        shape::get_vertex p + Q*(h v) (src/shape/include/exaDEM/shape.hpp:244-262),
+//                   compute_vertices.hpp:50-67
+//   OBB             type/OBB.hpp:62-224 (enlarge, rotate, 15-axis separating test with FLT_EPSILON),
+//                   compute_obb (basic_types.hpp:37-49)
+//   filters         shape_detection_core.hpp:43-53 (VV), 83-101 (VE), 245-309 (VF), 468-504 (EE)
+//   detection       shape_detection_core.hpp:57-81, 122-148, 397-466, 520-566
+//   interactions    src/polyhedron/nbh_polyhedron.cpp:223-376 (owner = lower id, OBB prefilter enlarged twice
+//                   by rVerlet/2, per-vertex OBB gate, VV/VE/VF then EE, then the swap pass VE/VF B->A),
+//                   nbh_polyhedron_driver.hpp:216-236 (vertex-wall candidates)
+//   contact         src/polyhedron/include/exaDEM/polyhedron/contact.hpp:124-211 (symmetric) and
+//                   contact.hpp:219-297 (vertex-wall)
+//   trigger         vertex_displ_over (src/polyhedron/vertex_displ_over.cu:74-133, backup_dem.hpp:65-110)
+// Accumulation order (free in the reference, which uses atomics): per particle, gravity, then for every
+// neighbour of its stream one subtotal over that pair's feature interactions in list order, then walls.
+
+struct OBBo {
+  V3 c, e1, e2, e3, ext;
+};
+
+static inline void obb_enlarge(OBBo& b, double more) {
+  b.ext.x += more;
+  b.ext.y += more;
+  b.ext.z += more;
+}
+
+// OBB::intersect (type/OBB.hpp:90-224)
+static bool obb_intersect(const OBBo& a, const OBBo& b) {
+  const double tol = FLT_EPSILON;
+  double ra, rb;
+  double R[3][3], A[3][3];
+  const V3 ea[3] = {a.e1, a.e2, a.e3}, eb[3] = {b.e1, b.e2, b.e3};
+  const double ae[3] = {a.ext.x, a.ext.y, a.ext.z}, be[3] = {b.ext.x, b.ext.y, b.ext.z};
+  const V3 tt = a.c - b.c;
+  const double t[3] = {dot(tt, a.e1), dot(tt, a.e2), dot(tt, a.e3)};
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      R[i][j] = dot(ea[i], eb[j]);
+      A[i][j] = std::fabs(R[i][j]) + tol;
+    }
+  for (int i = 0; i < 3; i++) {
+    ra = ae[i];
+    rb = be[0] * A[i][0] + be[1] * A[i][1] + be[2] * A[i][2];
+    if (std::fabs(t[i]) > ra + rb) return false;
+  }
+  for (int j = 0; j < 3; j++) {
+    ra = ae[0] * A[0][j] + ae[1] * A[1][j] + ae[2] * A[2][j];
+    rb = be[j];
+    if (std::fabs(t[0] * R[0][j] + t[1] * R[1][j] + t[2] * R[2][j]) > ra + rb) return false;
+  }
+  // cross axes a_i x b_j
+  ra = ae[1] * A[2][0] + ae[2] * A[1][0];
+  rb = be[1] * A[0][2] + be[2] * A[0][1];
+  if (std::fabs(t[2] * R[1][0] - t[1] * R[2][0]) > ra + rb) return false;
+  ra = ae[1] * A[2][1] + ae[2] * A[1][1];
+  rb = be[0] * A[0][2] + be[2] * A[0][0];
+  if (std::fabs(t[2] * R[1][1] - t[1] * R[2][1]) > ra + rb) return false;
+  ra = ae[1] * A[2][2] + ae[2] * A[1][2];
+  rb = be[0] * A[0][1] + be[1] * A[0][0];
+  if (std::fabs(t[2] * R[1][2] - t[1] * R[2][2]) > ra + rb) return false;
+  ra = ae[0] * A[2][0] + ae[2] * A[0][0];
+  rb = be[1] * A[1][2] + be[2] * A[1][1];
+  if (std::fabs(t[0] * R[2][0] - t[2] * R[0][0]) > ra + rb) return false;
+  ra = ae[0] * A[2][1] + ae[2] * A[0][1];
+  rb = be[0] * A[1][2] + be[2] * A[1][0];
+  if (std::fabs(t[0] * R[2][1] - t[2] * R[0][1]) > ra + rb) return false;
+  ra = ae[0] * A[2][2] + ae[2] * A[0][2];
+  rb = be[0] * A[1][1] + be[1] * A[1][0];
+  if (std::fabs(t[0] * R[2][2] - t[2] * R[0][2]) > ra + rb) return false;
+  ra = ae[0] * A[1][0] + ae[1] * A[0][0];
+  rb = be[1] * A[2][2] + be[2] * A[2][1];
+  if (std::fabs(t[1] * R[0][0] - t[0] * R[1][0]) > ra + rb) return false;
+  ra = ae[0] * A[1][1] + ae[1] * A[0][1];
+  rb = be[0] * A[2][2] + be[2] * A[2][0];
+  if (std::fabs(t[1] * R[0][1] - t[0] * R[1][1]) > ra + rb) return false;
+  ra = ae[0] * A[1][2] + ae[1] * A[0][2];
+  rb = be[0] * A[2][1] + be[1] * A[2][0];
+  if (std::fabs(t[1] * R[0][2] - t[0] * R[1][2]) > ra + rb) return false;
+  return true;
+}
+
+// compute_obb: body-frame box rotated by Q, scaled by h, centred on the particle
+static OBBo compute_obb(const xdem_shape& sh, V3 p, Q4 q, double h) {
+  OBBo b;
+  b.c = {0, 0, 0};
+  b.e1 = qrot(q, V3{1, 0, 0});
+  b.e2 = qrot(q, V3{0, 1, 0});
+  b.e3 = qrot(q, V3{0, 0, 1});
+  b.c = qrot(q, b.c);
+  b.ext = {sh.obb_extent[0], sh.obb_extent[1], sh.obb_extent[2]};
+  if (h != 1.0) {
+    b.ext = b.ext * h;
+    b.c = b.c * h;
+  }
+  b.c = b.c + p;
+  return b;
+}
+
+static inline V3 shape_vertex(const xdem_shape& sh, int k) {
+  return {sh.vertices[k][0], sh.vertices[k][1], sh.vertices[k][2]};
+}
+static inline V3 get_vertex(const xdem_shape& sh, int k, V3 p, double h, Q4 q) {
+  return p + qrot(q, h * shape_vertex(sh, k));
+}
+static inline V3 vnormalize(V3 v) { return v / norm(v); }
+
+static inline bool filter_vv(double rv, V3 vi, double ri, V3 vj, double rj) {
+  const double R = ri + rj + rv;
+  const V3 d = vi - vj;
+  return dot(d, d) <= R * R;
+}
+static inline bool filter_ve(double rv, V3 vi, double ri, V3 vf, V3 vs, double rj) {
+  const V3 fs = vs - vf;
+  const V3 fi = vi - vf;
+  const double r = dot(fs, fi) / dot(fs, fs);
+  if (r < -rv || r > 1.0 + rv) return false;
+  const V3 d = fi - fs * r;
+  const double Rmax = ri + rj + rv;
+  return dot(d, d) < Rmax * Rmax;
+}
+
+// shared body of the vertex-face filter and detection; returns (inside polygon, dist, n)
+static inline bool vf_core(V3 vi, const std::array<V3, 8>& vj, const xdem_shape& shj, int face, double limit,
+                           double& dist_out, V3& n_out, bool& early_out) {
+  const int nf = shj.face_size[face];
+  const int* f = shj.faces[face];
+  V3 va = vj[f[0]];
+  V3 vb = vj[f[1]];
+  const V3 vc = vj[f[nf - 1]];
+  const V3 v = vi - va;
+  V3 v1 = vb - va;
+  V3 v2 = vc - va;
+  v1 = vnormalize(v1);
+  V3 n = cross(v1, v2);
+  n = vnormalize(n);
+  double dist = dot(n, v);
+  if (dist < 0) {
+    n = n * (-1);
+    dist = -dist;
+  }
+  early_out = false;
+  if (dist > limit) {
+    early_out = true;
+    return false;
+  }
+  const V3 P = vi - n * dist;
+  int odd = 0;
+  v2 = cross(n, v1);
+  const double ori1 = dot(P, v1), ori2 = dot(P, v2);
+  for (int iva = 0; iva < nf; ++iva) {
+    int ivb = iva + 1;
+    if (ivb == nf) ivb = 0;
+    const V3 A = vj[f[iva]], B = vj[f[ivb]];
+    const double pa1 = dot(A, v1), pb1 = dot(B, v1), pa2 = dot(A, v2), pb2 = dot(B, v2);
+    if ((pa2 < ori2 && pb2 >= ori2) || (pb2 < ori2 && pa2 >= ori2)) {
+      if (pa1 + (ori2 - pa2) / (pb2 - pa2) * (pb1 - pa1) < ori1) odd = 1 - odd;
+    }
+  }
+  dist_out = dist;
+  n_out = n;
+  return odd == 1;
+}
+
+static inline bool filter_ee(double rv, V3 vfi, V3 vsi, double ri, V3 vfj, V3 vsj, double rj) {
+  const double R = ri + rj + rv;
+  const V3 Ei = vsi - vfi, Ej = vsj - vfj, v = vfi - vfj;
+  const double c = dot(Ei, Ei), d = dot(Ej, Ej), e = dot(Ei, Ej);
+  double f = (c * d) - (e * e);
+  if (std::fabs(f) > 1.0e-12) {
+    f = 1.0 / f;
+    const double a = dot(Ei, v), b = dot(Ej, v);
+    const double s = (e * b - a * d) * f, t = (c * b - e * a) * f;
+    if (s <= 0.0 || s >= 1.0 || t <= 0.0 || t >= 1.0) return false;
+    const V3 pi = vfi + Ei * s, pj = vfj + Ej * t;
+    const V3 n = pi - pj;
+    return dot(n, n) <= R * R;
+  }
+  return false;
+}
+
+static inline Contact detect_ve(V3 vi, double ri, V3 vf, V3 vs, double rj) {
+  const V3 fs = vs - vf, fi = vi - vf;
+  const double r = dot(fs, fi) / dot(fs, fs);
+  if (r <= 0 || r >= 1.0) return {false, 0, {0, 0, 0}, {0, 0, 0}};
+  V3 n = fi - fs * r;
+  const double dn = norm(n) - (ri + rj);
+  if (dn > 0) return {false, 0, {0, 0, 0}, {0, 0, 0}};
+  n = vnormalize(n);
+  return {true, dn, n, vi - n * (ri + 0.5 * dn)};
+}
+
+static inline Contact detect_vf(V3 vi, double ri, const std::array<V3, 8>& vj, const xdem_shape& shj, int face,
+                                double rj) {
+  double dist;
+  V3 n;
+  bool early;
+  const bool in = vf_core(vi, vj, shj, face, ri + rj, dist, n, early);
+  if (early) return {false, 0, {0, 0, 0}, {0, 0, 0}};
+  const double dn = dist - (ri + rj);
+  return {in, dn, n, vi - n * (ri + 0.5 * dn)};
+}
+
+static inline Contact detect_ee(V3 vfi, V3 vsi, double ri, V3 vfj, V3 vsj, double rj) {
+  const double R = ri + rj;
+  const V3 Ei = vsi - vfi, Ej = vsj - vfj, v = vfi - vfj;
+  const double c = dot(Ei, Ei), d = dot(Ej, Ej), e = dot(Ei, Ej);
+  double f = (c * d) - (e * e);
+  if (std::fabs(f) > 1.0e-12) {
+    f = 1.0 / f;
+    const double a = dot(Ei, v), b = dot(Ej, v);
+    const double s = (e * b - a * d) * f, t = (c * b - e * a) * f;
+    if (s <= 0.0 || s >= 1.0 || t <= 0.0 || t >= 1.0) return {false, 0, {0, 0, 0}, {0, 0, 0}};
+    const V3 pi = vfi + Ei * s, pj = vfj + Ej * t;
+    V3 n = pi - pj;
+    const double dn = norm(n) - R;
+    if (dn > 0) return {false, 0, {0, 0, 0}, {0, 0, 0}};
+    n = vnormalize(n);
+    return {dn <= 0, dn, n, pi - n * (ri + 0.5 * dn)};
+  }
+  return {false, 0, {0, 0, 0}, {0, 0, 0}};
+}
+
+static void compute_vertices(Oracle& o) {
+  o.verts.resize(o.n);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < o.n; i++) {
+    const xdem_shape& sh = o.shapes[o.type[i]];
+    for (int k = 0; k < sh.nv; k++) o.verts[i][k] = get_vertex(sh, k, o.r[i], o.homothety[i], o.q[i]);
+  }
+}
+
+// feature evaluation of item `it` owned by x (returns contact + force on the item's particle i)
+struct PolyEval {
+  bool in_contact;
+  V3 f, cp, mint;
+  bool x_is_i;  // the list holder is the evaluation's particle i (swap = 0)
+  V3 ri, rj;    // centre of i and j in the owner's frame
+};
+
+static std::array<V3, 8> shifted(const std::array<V3, 8>& v, int nv, V3 s) {
+  std::array<V3, 8> out{};
+  for (int k = 0; k < nv; k++) out[k] = v[k] + s;
+  return out;
+}
+
+// nbh_polyhedron: rebuild the per-particle item lists (owner = lower id), carrying history by key.
+static void nbh_polyhedron(Oracle& o, double rv) {
+  const int64_t n = o.n;
+  if ((int64_t)o.items.size() != n) o.items.assign(n, {});
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t x = 0; x < n; x++) {
+    std::vector<Item> hist;
+    for (auto& it : o.items[x])
+      if (item_active(it)) hist.push_back(it);
+    std::stable_sort(hist.begin(), hist.end(), key_less);
+    std::vector<Item> lst;
+    Item base{};
+    base.id_i = o.id[x];
+    const xdem_shape& sa = o.shapes[o.type[x]];
+    const double ha = o.homothety[x];
+    const double ra = sa.minkowski * ha;
+    // drivers first: one vertex-wall candidate per vertex passing Surface::filter(rVerlet + r, vertex)
+    for (size_t w = 0; w < o.walls.size(); w++)
+      for (int k = 0; k < sa.nv; k++)
+        if (surf_filter(o.walls[w], rv + ra, o.verts[x][k])) {
+          Item it = base;
+          it.type = 5;
+          it.id_j = w;
+          it.partner = -1;
+          it.shift = 13;
+          it.sub_i = (uint32_t)k;
+          it.sub_j = (uint32_t)-1;
+          lst.push_back(it);
+        }
+    OBBo obb_i0 = compute_obb(sa, o.r[x], o.q[x], ha);
+    for (auto& s : o.streams[x]) {
+      const int64_t y = s.j;
+      if (o.id[x] >= o.id[y]) continue;
+      const V3 sh = shift_vec(o, s.shift);
+      const xdem_shape& sb = o.shapes[o.type[y]];
+      const double hb = o.homothety[y];
+      const double rb = sb.minkowski * hb;
+      const std::array<V3, 8> vb = shifted(o.verts[y], sb.nv, sh);
+      const std::array<V3, 8>& va = o.verts[x];
+      OBBo obb_i = obb_i0;
+      OBBo obb_j = compute_obb(sb, o.r[y] + sh, o.q[y], hb);
+      obb_enlarge(obb_i, 0.5 * rv);
+      obb_enlarge(obb_j, 0.5 * rv);
+      if (!obb_intersect(obb_i, obb_j)) continue;
+      obb_enlarge(obb_i, 0.5 * rv);
+      obb_enlarge(obb_j, 0.5 * rv);
+      Item it = base;
+      it.id_j = o.id[y];
+      it.partner = (int32_t)y;
+      it.shift = s.shift;
+      auto add = [&](int type, int swap, int sub_i, int sub_j) {
+        it.type = (uint16_t)type;
+        it.swap = (uint8_t)swap;
+        // key subs are owner().sub_, partner().sub_
+        it.sub_i = (uint32_t)(swap ? sub_j : sub_i);
+        it.sub_j = (uint32_t)(swap ? sub_i : sub_j);
+        lst.push_back(it);
+      };
+      for (int i = 0; i < sa.nv; i++) {
+        OBBo ov;
+        ov.c = va[i];
+        ov.e1 = {1, 0, 0};
+        ov.e2 = {0, 1, 0};
+        ov.e3 = {0, 0, 1};
+        ov.ext = {0, 0, 0};
+        obb_enlarge(ov, ra);
+        if (!obb_intersect(obb_j, ov)) continue;
+        for (int j = 0; j < sb.nv; j++)
+          if (filter_vv(rv, va[i], ra, vb[j], rb)) add(0, 0, i, j);
+        for (int j = 0; j < sb.ne; j++)
+          if (filter_ve(rv, va[i], ra, vb[sb.edges[j][0]], vb[sb.edges[j][1]], rb)) add(1, 0, i, j);
+        for (int j = 0; j < sb.nf; j++) {
+          double dist;
+          V3 nn;
+          bool early;
+          if (vf_core(va[i], vb, sb, j, ra + rb + rv, dist, nn, early)) add(2, 0, i, j);
+        }
+      }
+      for (int i = 0; i < sa.ne; i++)
+        for (int j = 0; j < sb.ne; j++)
+          if (filter_ee(rv, va[sa.edges[i][0]], va[sa.edges[i][1]], ra, vb[sb.edges[j][0]], vb[sb.edges[j][1]], rb))
+            add(3, 0, i, j);
+      for (int j = 0; j < sb.nv; j++) {
+        OBBo ov;
+        ov.c = vb[j];
+        ov.e1 = {1, 0, 0};
+        ov.e2 = {0, 1, 0};
+        ov.e3 = {0, 0, 1};
+        ov.ext = {0, 0, 0};
+        obb_enlarge(ov, rb);
+        if (!obb_intersect(obb_i, ov)) continue;
+        for (int i = 0; i < sa.ne; i++)
+          if (filter_ve(rv, vb[j], rb, va[sa.edges[i][0]], va[sa.edges[i][1]], ra)) add(1, 1, j, i);
+        for (int i = 0; i < sa.nf; i++) {
+          double dist;
+          V3 nn;
+          bool early;
+          if (vf_core(vb[j], va, sa, i, rb + ra + rv, dist, nn, early)) add(2, 1, j, i);
+        }
+      }
+    }
+    if (g_mutation != 4) {
+      for (auto& it : lst) {
+        auto lb = std::lower_bound(hist.begin(), hist.end(), it, key_less);
+        if (lb != hist.end() && key_eq(it, *lb)) {
+          it.friction = lb->friction;
+          it.moment = lb->moment;
+        }
+      }
+    }
+    o.items[x] = std::move(lst);
+  }
+  o.built = true;
+}
+
+// evaluation of one particle-particle feature item held by x
+static PolyEval eval_pp(Oracle& o, int64_t x, Item& it, const CP& cp, double dt) {
+  PolyEval e{};
+  const int64_t y = it.partner;
+  const V3 sh = shift_vec(o, it.shift);
+  const xdem_shape& sa = o.shapes[o.type[x]];
+  const xdem_shape& sb = o.shapes[o.type[y]];
+  const double ha = o.homothety[x], hb = o.homothety[y];
+  const std::array<V3, 8> vb = shifted(o.verts[y], sb.nv, sh);
+  const std::array<V3, 8>& va = o.verts[x];
+  const bool swap = it.swap != 0;
+  // evaluation particle i: the vertex side; j: the other one
+  const xdem_shape& si = swap ? sb : sa;
+  const xdem_shape& sj = swap ? sa : sb;
+  const std::array<V3, 8>& vi = swap ? vb : va;
+  const std::array<V3, 8>& vj = swap ? va : vb;
+  const double ri = si.minkowski * (swap ? hb : ha), rj = sj.minkowski * (swap ? ha : hb);
+  const int fi = (int)(swap ? it.sub_j : it.sub_i), fj = (int)(swap ? it.sub_i : it.sub_j);
+  Contact c{false, 0, {0, 0, 0}, {0, 0, 0}};
+  switch (it.type) {
+    case 0: {
+      const Contact d = detect_vv(vi[fi], ri, vj[fj], rj);
+      c = d;
+      break;
+    }
+    case 1:
+      c = detect_ve(vi[fi], ri, vj[sj.edges[fj][0]], vj[sj.edges[fj][1]], rj);
+      break;
+    case 2:
+      c = detect_vf(vi[fi], ri, vj, sj, fj, rj);
+      break;
+    case 3:
+      c = detect_ee(vi[si.edges[fi][0]], vi[si.edges[fi][1]], ri, vj[sj.edges[fj][0]], vj[sj.edges[fj][1]], rj);
+      break;
+  }
+  const int64_t pi = swap ? y : x, pj = swap ? x : y;
+  const V3 r_i = swap ? o.r[y] + sh : o.r[x];
+  const V3 r_j = swap ? o.r[x] : o.r[y] + sh;
+  e.x_is_i = !swap;
+  e.ri = r_i;
+  e.rj = r_j;
+  if (c.contact) {
+    const double mi = o.mass[pi], mj = o.mass[pj];
+    const double meff = (mi * mj) / (mi + mj);
+    V3 f{0, 0, 0};
+    contact_force_core(c.dn, c.n, dt, cp, meff, it.friction, c.cp, r_i, o.v[pi], f, it.moment, o.vrot[pi], r_j,
+                       o.v[pj], o.vrot[pj]);
+    e.in_contact = true;
+    e.f = f;
+    e.cp = c.cp;
+    e.mint = it.moment;
+  } else {
+    it.friction = {0, 0, 0};
+    it.moment = {0, 0, 0};
+  }
+  return e;
+}
+
+static void contact_poly(Oracle& o, const CP& cp, const CP& cpd, double dt) {
+  const int64_t n = o.n;
+  std::vector<std::vector<PolyEval>> ev(n);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t x = 0; x < n; x++) {
+    ev[x].resize(o.items[x].size());
+    for (size_t k = 0; k < o.items[x].size(); k++) {
+      Item& it = o.items[x][k];
+      if (it.type == 5) {  // vertex - wall
+        const Surf& s = o.walls[it.id_j];
+        const xdem_shape& sa = o.shapes[o.type[x]];
+        const double r = sa.minkowski * o.homothety[x];
+        const Contact c = surf_detect(s, r, o.verts[x][it.sub_i]);
+        PolyEval e{};
+        if (c.contact) {
+          V3 f{0, 0, 0};
+          contact_force_core(c.dn, c.n, dt, cpd, o.mass[x], it.friction, c.cp, o.r[x], o.v[x], f, it.moment,
+                             o.vrot[x], s.center, s.vel, s.vrot);
+          e.in_contact = true;
+          e.f = f;
+          e.cp = c.cp;
+          e.mint = it.moment;
+        } else {
+          it.friction = {0, 0, 0};
+          it.moment = {0, 0, 0};
+        }
+        ev[x][k] = e;
+      } else {
+        ev[x][k] = eval_pp(o, x, it, cp, dt);
+      }
+    }
+  }
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t x = 0; x < n; x++) {
+    V3 F = o.f[x], M = o.mom[x];
+    for (auto& s : o.streams[x]) {
+      const int64_t y = s.j;
+      V3 Fp{0, 0, 0}, Mp{0, 0, 0};
+      if (o.id[x] < o.id[y]) {
+        for (size_t k = 0; k < o.items[x].size(); k++) {
+          const Item& it = o.items[x][k];
+          if (it.type == 5 || it.partner != y || it.shift != s.shift) continue;
+          const PolyEval& e = ev[x][k];
+          if (!e.in_contact) continue;
+          if (e.x_is_i) {
+            Fp = Fp + e.f;
+            Mp = Mp + compute_moments(e.cp, e.ri, e.f, e.mint);
+          } else {
+            const V3 nf = -e.f;
+            Fp = Fp + nf;
+            Mp = Mp + compute_moments(e.cp, e.rj, nf, -e.mint);
+          }
+        }
+      } else {
+        const int mirror = 26 - s.shift;
+        for (size_t k = 0; k < o.items[y].size(); k++) {
+          const Item& it = o.items[y][k];
+          if (it.type == 5 || it.partner != x || it.shift != mirror) continue;
+          const PolyEval& e = ev[y][k];
+          if (!e.in_contact) continue;
+          if (e.x_is_i) {  // y is i, x is j
+            const V3 nf = -e.f;
+            Fp = Fp + nf;
+            Mp = Mp + compute_moments(e.cp, e.rj, nf, -e.mint);
+          } else {  // x is i (vertex side of a swap item held by y)
+            Fp = Fp + e.f;
+            Mp = Mp + compute_moments(e.cp, e.ri, e.f, e.mint);
+          }
+        }
+      }
+      F = F + Fp;
+      M = M + Mp;
+    }
+    for (size_t k = 0; k < o.items[x].size(); k++) {
+      const Item& it = o.items[x][k];
+      if (it.type != 5 || !ev[x][k].in_contact) continue;
+      const PolyEval& e = ev[x][k];
+      F = F + e.f;
+      M = M + compute_moments(e.cp, o.r[x], e.f, e.mint);
+    }
+    o.f[x] = F;
+    o.mom[x] = M;
+  }
+}
+
+static void rebuild_poly(Oracle& o, double rcut_inc) {
+  move_particles(o);
+  compute_vertices(o);
+  double rmax = 0;
+  for (int64_t i = 0; i < o.n; i++) rmax = std::max(rmax, 2 * o.radius[i]);
+  chunk_neighbors(o, rcut_inc, rmax + rcut_inc);
+  nbh_polyhedron(o, rcut_inc);
+}
+
+// vertex_displ_over: any vertex moved by at least thr since the last rebuild
+static bool vertex_displ_over(const Oracle& o, double thr) {
+  const double t2 = thr * thr;
+  for (int64_t i = 0; i < o.n; i++) {
+    const xdem_shape& sh = o.shapes[o.type[i]];
+    for (int k = 0; k < sh.nv; k++) {
+      const V3 a = get_vertex(sh, k, o.backup[i], o.homothety[i], o.backup_q[i]);
+      const V3 b = get_vertex(sh, k, o.r[i], o.homothety[i], o.q[i]);
+      const V3 d = b - a;
+      if (dot(d, d) >= t2) return true;
+    }
+  }
+  return false;
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -787,6 +1325,101 @@ void xo_quat_rotate(const double* q, const double* v, double* out) {
   out[0] = a.x;
   out[1] = a.y;
   out[2] = a.z;
+}
+
+// ---------------------------------------------------------------- polyhedra entry points
+void xo_set_shapes(void* h, const xdem_shape* shapes, int n) {
+  Oracle& o = *(Oracle*)h;
+  o.shapes.assign(shapes, shapes + n);
+  o.poly = true;
+  o.backup_q = o.q;
+}
+void xo_compute_vertices(void* h) { compute_vertices(*(Oracle*)h); }
+void xo_rebuild_poly(void* h, double rcut_inc) { rebuild_poly(*(Oracle*)h, rcut_inc); }
+void xo_contact_poly(void* h, const xdem_contact_params* cp, const xdem_contact_params* cpd, double dt) {
+  contact_poly(*(Oracle*)h, to_cp(cp), to_cp(cpd), dt);
+}
+int xo_vertex_displ_over(void* h, double thr) { return vertex_displ_over(*(Oracle*)h, thr) ? 1 : 0; }
+
+// first_iteration + nsteps of the polyhedra scheme (config_polyhedra.msp: compute_vertices on the fast
+// path, vertex_displ_over as the trigger, contact_polyhedron in compute_force).
+void xo_first_iteration_poly(void* h, const xdem_scheme* s) {
+  Oracle& o = *(Oracle*)h;
+  rebuild_poly(o, s->rcut_inc);
+  reset_force_moment(o);
+  if (s->gravity_on) gravity(o, {s->gravity[0], s->gravity[1], s->gravity[2]});
+  contact_poly(o, to_cp(&s->cp), to_cp(&s->cpd), s->dt);
+}
+int64_t xo_run_steps_poly(void* h, const xdem_scheme* s, int64_t nsteps) {
+  Oracle& o = *(Oracle*)h;
+  int64_t rebuilds = 0;
+  for (int64_t k = 0; k < nsteps; k++) {
+    prolog(o, s->dt);
+    if (vertex_displ_over(o, 0.5 * s->rcut_inc)) {
+      rebuild_poly(o, s->rcut_inc);
+      rebuilds++;
+    } else {
+      compute_vertices(o);
+    }
+    reset_force_moment(o);
+    if (s->gravity_on) gravity(o, {s->gravity[0], s->gravity[1], s->gravity[2]});
+    contact_poly(o, to_cp(&s->cp), to_cp(&s->cpd), s->dt);
+    force_to_accel(o);
+    epilog(o, s->dt);
+  }
+  return rebuilds;
+}
+
+// interaction dump with feature keys: owner id, partner id, type, swap, owner sub, partner sub, friction
+int64_t xo_poly_interactions_dump(void* h, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint16_t* type, uint8_t* swap,
+                                  uint32_t* sub_i, uint32_t* sub_j, double* fr) {
+  Oracle& o = *(Oracle*)h;
+  int64_t k = 0;
+  for (auto& l : o.items)
+    for (auto& it : l) {
+      if (k >= cap) return k;
+      id_i[k] = it.id_i;
+      id_j[k] = it.id_j;
+      type[k] = it.type;
+      swap[k] = it.swap;
+      sub_i[k] = it.sub_i;
+      sub_j[k] = it.sub_j;
+      fr[3 * k] = it.friction.x;
+      fr[3 * k + 1] = it.friction.y;
+      fr[3 * k + 2] = it.friction.z;
+      k++;
+    }
+  return k;
+}
+
+// One feature evaluation for closed-form checks: two polyhedra (shape a at pa/qa, shape b at pb/qb),
+// feature type/subs with i = a; returns contact flag, dn, n, cp.
+int xo_poly_detect(const xdem_shape* sa, const double* pa, const double* qa, const xdem_shape* sb, const double* pb,
+                   const double* qb, int type, int sub_i, int sub_j, double* out) {
+  std::array<V3, 8> va{}, vb{};
+  const Q4 QA{qa[0], qa[1], qa[2], qa[3]}, QB{qb[0], qb[1], qb[2], qb[3]};
+  for (int k = 0; k < sa->nv; k++) va[k] = get_vertex(*sa, k, {pa[0], pa[1], pa[2]}, 1.0, QA);
+  for (int k = 0; k < sb->nv; k++) vb[k] = get_vertex(*sb, k, {pb[0], pb[1], pb[2]}, 1.0, QB);
+  const double ri = sa->minkowski, rj = sb->minkowski;
+  Contact c{false, 0, {0, 0, 0}, {0, 0, 0}};
+  if (type == 0) c = detect_vv(va[sub_i], ri, vb[sub_j], rj);
+  if (type == 1) c = detect_ve(va[sub_i], ri, vb[sb->edges[sub_j][0]], vb[sb->edges[sub_j][1]], rj);
+  if (type == 2) c = detect_vf(va[sub_i], ri, vb, *sb, sub_j, rj);
+  if (type == 3)
+    c = detect_ee(va[sa->edges[sub_i][0]], va[sa->edges[sub_i][1]], ri, vb[sb->edges[sub_j][0]],
+                  vb[sb->edges[sub_j][1]], rj);
+  const double o8[8] = {c.contact ? 1.0 : 0.0, c.dn, c.n.x, c.n.y, c.n.z, c.cp.x, c.cp.y, c.cp.z};
+  std::memcpy(out, o8, sizeof(o8));
+  return c.contact ? 1 : 0;
+}
+
+int xo_obb_intersect(const xdem_shape* sa, const double* pa, const double* qa, const xdem_shape* sb, const double* pb,
+                     const double* qb, double enlarge) {
+  OBBo a = compute_obb(*sa, {pa[0], pa[1], pa[2]}, {qa[0], qa[1], qa[2], qa[3]}, 1.0);
+  OBBo b = compute_obb(*sb, {pb[0], pb[1], pb[2]}, {qb[0], qb[1], qb[2], qb[3]}, 1.0);
+  obb_enlarge(a, enlarge);
+  obb_enlarge(b, enlarge);
+  return obb_intersect(a, b) ? 1 : 0;
 }
 
 }  // extern "C"

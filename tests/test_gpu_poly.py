@@ -1,0 +1,157 @@
+"""Parity of the HIP polyhedra path (through the C ABI) with the CPU oracle, on seeded inputs.
+
+Cubes (edge 1) and octahedra (circumradius 0.8), Minkowski radius 0.02, random orientations.
+Bars: the feature interaction lists (owner, partner, type, swap, sub-features, order per owner)
+bit-exact; friction history bit-exact; forces and moments within 1e-12 relative (floor m|g|);
+trajectories within 1e-10 relative. The fused engine must equal the operator-by-operator path.
+"""
+import numpy as np
+import pytest
+
+from exadem_amd import inputs
+from exadem_amd.sim import PolySim, scheme
+import pyoracle as po
+
+from helpers import by_id, force, pos, rel_err, vec, vel
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 1e-12
+TRAJ_TOL = 1e-10
+
+
+def poly_case(name):
+    if name == "periodic":
+        return inputs.polyhedra(n=2000, spacing=1.45, jitter=0.08, seed=21, dt=2e-4, sigma_v=1.0, z0=0.75)
+    if name == "walls":
+        return inputs.polyhedra(n=1500, spacing=1.5, jitter=0.05, seed=5, dt=2e-4, sigma_v=0.5, z0=0.6,
+                                periodic=(False, False, False))
+    raise KeyError(name)
+
+
+def oracle_for(w):
+    o = po.Oracle(w.particles, w.domain, w.walls)
+    o.set_shapes(w.shapes)
+    return o
+
+
+def feature_table(ii, jj, tt, sw, si, sj, fr):
+    """Stable sort by owner id: keeps each owner's list in storage order."""
+    order = np.argsort(ii, kind="stable")
+    keys = np.stack([ii.astype(np.int64), jj.astype(np.int64), tt.astype(np.int64), sw.astype(np.int64),
+                     si.astype(np.int64), sj.astype(np.int64)], axis=1)
+    return keys[order], fr[order]
+
+
+def force_floor(w):
+    p = by_id(w.particles)
+    return p["mass"] * 9.81
+
+
+def compare_state(g, o, floor, what):
+    g, o = by_id(g), by_id(o)
+    assert np.array_equal(g["id"], o["id"]), what
+    L = max(1.0, float(np.max(np.abs(pos(o)))))
+    dpos = float(np.max(np.abs(pos(g) - pos(o)))) / L
+    vs = max(1e-3, float(np.max(np.abs(vel(o)))))
+    dvel = float(np.max(np.abs(vel(g) - vel(o)))) / vs
+    dq = float(np.max(np.abs(np.stack([g[k] - o[k] for k in ("qw", "qx", "qy", "qz")]))))
+    ef = rel_err(force(g), force(o), floor)
+    mscale = np.maximum(np.linalg.norm(vec(o, "mom"), axis=1), 1e-6 * float(np.max(floor)))
+    em = float(np.max(np.linalg.norm(vec(g, "mom") - vec(o, "mom"), axis=1) / mscale))
+    assert dpos <= TRAJ_TOL, f"{what}: positions differ by {dpos:.3e}"
+    assert dvel <= TRAJ_TOL, f"{what}: velocities differ by {dvel:.3e}"
+    assert dq <= TRAJ_TOL, f"{what}: orientations differ by {dq:.3e}"
+    assert ef <= FORCE_TOL, f"{what}: forces differ by {ef:.3e} relative"
+    assert em <= FORCE_TOL, f"{what}: moments differ by {em:.3e} relative"
+    return dpos, dvel, ef, em
+
+
+@pytest.mark.parametrize("case", ["periodic", "walls"])
+def test_feature_lists_bit_exact(gpu_ok, case):
+    w = poly_case(case)
+    o = oracle_for(w)
+    o.rebuild_poly(w.rcut_inc)
+    s = PolySim(w)
+    s.rebuild()
+    gk, gf = feature_table(*s.ctx.poly_interactions())
+    ok, of = feature_table(*o.poly_interactions())
+    assert len(gk) == len(ok) and len(gk) > 100
+    assert np.array_equal(gk, ok)
+    assert np.array_equal(gf, of)
+    types = np.bincount(gk[:, 2], minlength=6)
+    assert types[0] + types[1] > 0 and types[2] > 0 and types[3] > 0, types
+    if case == "walls":
+        assert types[5] > 0
+
+
+@pytest.mark.parametrize("case", ["periodic", "walls"])
+def test_first_force_pass(gpu_ok, case):
+    w = poly_case(case)
+    sch = scheme(w)
+    o = oracle_for(w)
+    o.first_iteration_poly(sch)
+    s = PolySim(w)
+    s.first_iteration()
+    g, q = by_id(s.ctx.download()), by_id(o.particles())
+    floor = force_floor(w)
+    assert rel_err(force(g), force(q), floor) <= FORCE_TOL
+    mscale = np.maximum(np.linalg.norm(vec(q, "mom"), axis=1), 1e-6 * float(np.max(floor)))
+    assert float(np.max(np.linalg.norm(vec(g, "mom") - vec(q, "mom"), axis=1) / mscale)) <= FORCE_TOL
+    # some particles are in contact: the force differs from gravity alone
+    fg = np.abs(force(q)[:, 2] + q["mass"] * 9.81)
+    assert np.count_nonzero(fg > 1e-6 * q["mass"]) > 20
+    gk, gf = feature_table(*s.ctx.poly_interactions())
+    ok, of = feature_table(*o.poly_interactions())
+    assert np.array_equal(gk, ok) and np.array_equal(gf, of)
+    assert np.count_nonzero(np.abs(of).sum(axis=1)) > 10
+
+
+@pytest.mark.parametrize("case", ["periodic", "walls"])
+def test_run_steps_match_oracle(gpu_ok, case):
+    """120 steps of the fused engine against the oracle's scheme, with vertex-displacement rebuilds."""
+    w = poly_case(case)
+    sch = scheme(w, poll=8)
+    o = oracle_for(w)
+    o.first_iteration_poly(sch)
+    orb = o.run_steps_poly(sch, 120)
+    s = PolySim(w)
+    s.first_iteration()
+    st = s.run(120, poll=8)
+    assert st["rebuilds"] == orb and orb >= 1
+    compare_state(s.ctx.download(), o.particles(), force_floor(w), case)
+    gk, gf = feature_table(*s.ctx.poly_interactions())
+    ok, of = feature_table(*o.poly_interactions())
+    assert np.array_equal(gk, ok)
+    assert np.array_equal(gf, of)
+
+
+def test_engine_equals_operator_path(gpu_ok):
+    w = poly_case("periodic")
+    a = PolySim(w)
+    a.first_iteration()
+    for _ in range(40):
+        a.step()
+    b = PolySim(w)
+    b.first_iteration()
+    b.run(40, poll=4)
+    assert a.rebuilds == b.rebuilds
+    pa, pb = by_id(a.ctx.download()), by_id(b.ctx.download())
+    for k in ("rx", "ry", "rz", "vx", "vy", "vz", "qw", "qx", "qy", "qz", "fx", "fy", "fz", "momx", "vrotx"):
+        assert np.array_equal(pa[k], pb[k]), k
+    ka, fa = feature_table(*a.ctx.poly_interactions())
+    kb, fb = feature_table(*b.ctx.poly_interactions())
+    assert np.array_equal(ka, kb) and np.array_equal(fa, fb)
+
+
+def test_vertex_displ_over(gpu_ok):
+    w = poly_case("periodic")
+    s = PolySim(w)
+    s.first_iteration()
+    assert not s.ctx.vertex_displ_over(0.05)
+    o = oracle_for(w)
+    o.first_iteration_poly(scheme(w))
+    for k in range(6):
+        s.ctx.combined_compute_prolog(0.02)
+        o.prolog(0.02)
+        assert s.ctx.vertex_displ_over(0.05) == o.vertex_displ_over(0.05), k
