@@ -35,6 +35,20 @@ def bytes_model(form, n, pairs):
     return 412 * n + 52 * pairs, 140 * n + 52 * pairs
 
 
+def profiled_traffic(form):
+    """HBM bytes per force launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE, scaled by a copy
+    probe), read from the committed summary for this form; None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"k_force_{form}_1M_spheres.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -115,6 +129,7 @@ def main():
     value = n * args.steps / elapsed
     ms_step = 1e3 * elapsed / args.steps
     achieved = force_bytes / (f_ms * 1e-3) / 1e9 if f_ms > 0 else 0.0
+    traffic, traffic_src = profiled_traffic(args.form)
 
     cpu = None
     if not args.no_cpu:
@@ -156,7 +171,7 @@ def main():
                    "n": n, "pairs": pairs, "rebuilds": st["rebuilds"], "polls": st["polls"],
                    "list_slots": st_nb2["slots"], "max_initial_overlap": w.meta["max_overlap"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_force_fp64 (fused force pass)", "avg_ms": f_ms, "launches": f_cnt,
                      "bytes_per_launch": force_bytes, "integrate_avg_ms": i_ms,
                      "step_bytes": step_bytes, "step_frac": step_bytes / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS},
