@@ -19,6 +19,7 @@ HIP_SOURCES = [
     "contact_sphere.hip",
     "contact_sphere_fp32.hip",
     "poly.hip",
+    "gnn.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

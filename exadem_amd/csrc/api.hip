@@ -479,6 +479,14 @@ int xdem_particles_upload(xdem_ctx* c, const xdem_particles* p) {
   XDEM_HIP(c, cp(s.bz, p->rz));
   XDEM_HIP(c, hipMemcpyAsync(s.id, p->id, sizeof(uint64_t) * n, hipMemcpyHostToDevice, c->stream));
   std::vector<uint32_t> z32(n, 0);
+  c->ids_dense = true;
+  {
+    std::vector<uint8_t> seen(n, 0);
+    for (int64_t i = 0; i < n && c->ids_dense; i++) {
+      if (p->id[i] >= (uint64_t)n || seen[p->id[i]]) c->ids_dense = false;
+      else seen[p->id[i]] = 1;
+    }
+  }
   c->max_type = 0;
   if (p->type)
     for (int64_t i = 0; i < n; i++) c->max_type = std::max(c->max_type, p->type[i]);
@@ -815,3 +823,16 @@ int xdem_set_timing(xdem_ctx* c, int32_t on) {
 }
 
 }  // extern "C"
+
+// engine pieces shared with the surrogate driver
+namespace xdem {
+int engine_rebuild(xdem_ctx* c, double rcut_inc) { return rebuild(c, rcut_inc); }
+int engine_force_pass(xdem_ctx* c, const xdem_scheme* s) {
+  int rc;
+  if ((rc = check_law(c, &s->cp, "scheme cp"))) return rc;
+  if (c->nwalls > 0 && (rc = check_law(c, &s->cpd, "scheme cpd"))) return rc;
+  const V3 g{s->gravity[0], s->gravity[1], s->gravity[2]};
+  return engine_force(c, to_cp(&s->cp), to_cp(&s->cpd), s, g, false, 0);
+}
+int engine_flush_kick(xdem_ctx* c) { return flush_kick(c); }
+}  // namespace xdem

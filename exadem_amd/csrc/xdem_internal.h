@@ -207,6 +207,7 @@ struct xdem_ctx {
   bool cells_valid = false;     // cell_start matches the current positions and order
   bool hist_ready = false;      // history of the current list has been carried over (nbh_sphere)
   uint32_t max_type = 0;        // largest particle type of the last upload
+  bool ids_dense = false;       // the uploaded ids are a permutation of 0..n-1
   uint64_t nl_epoch = 0;        // number of neighbour builds so far
   uint64_t pl_epoch = ~0ull;    // neighbour build the feature lists were made from
   bool wl_valid = false;        // wall list exists (for the order it was built in)
@@ -285,6 +286,9 @@ int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd,
                  bool gravity_on, V3 g, bool guarded, int32_t step_tag);
 int alloc_walls(xdem_ctx* c, int nw);
 int set_history_form(xdem_ctx* c, bool fp32);
+int engine_rebuild(xdem_ctx* c, double rcut_inc);
+int engine_force_pass(xdem_ctx* c, const xdem_scheme* s);  // fused force pass at the current state
+int engine_flush_kick(xdem_ctx* c);
 // polyhedra
 int poly_compute_vertices(xdem_ctx* c, bool displ, double thr2, bool guarded);
 int poly_nbh(xdem_ctx* c, double rcut_inc);

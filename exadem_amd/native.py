@@ -20,6 +20,7 @@ ERR_NAMES = {-1: "XDEM_ERR_ARG", -2: "XDEM_ERR_HIP", -3: "XDEM_ERR_STATE", -4: "
 _dp = C.POINTER(C.c_double)
 _u64p = C.POINTER(C.c_uint64)
 _u32p = C.POINTER(C.c_uint32)
+_fp = C.POINTER(C.c_float)
 
 
 class Particles(C.Structure):
@@ -61,6 +62,20 @@ class RunStats(C.Structure):
     _fields_ = [("steps", C.c_int64), ("rebuilds", C.c_int64), ("polls", C.c_int64)]
 
 
+class GnnConfig(C.Structure):
+    _fields_ = [("r0", C.c_double), ("v0", C.c_double), ("a0", C.c_double), ("m0", C.c_double),
+                ("ov_scale", C.c_double), ("substeps", C.c_int32), ("train", C.c_int32)]
+
+
+class SurrogateStats(C.Structure):
+    _fields_ = [("surrogate_steps", C.c_int64), ("dem_steps", C.c_int64), ("checks", C.c_int64),
+                ("fallbacks", C.c_int64), ("rebuilds", C.c_int64), ("max_err", C.c_double),
+                ("max_err_free", C.c_double)]
+
+
+GNN_NPARAMS = 205446
+
+
 # field name -> (numpy dtype) for the xdem_particles members
 FIELDS = ["id", "type", "group", "rx", "ry", "rz", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "radius",
           "homothety", "qw", "qx", "qy", "qz", "momx", "momy", "momz", "vrotx", "vroty", "vrotz",
@@ -78,6 +93,9 @@ HIP_SYMBOLS = [
     "xdem_kernel_timing", "xdem_set_timing", "xdem_streams_dump",
     "xdem_set_shapes", "xdem_compute_vertices", "xdem_vertex_displ_over", "xdem_nbh_polyhedron",
     "xdem_contact_polyhedron", "xdem_poly_interactions_count", "xdem_poly_interactions_dump",
+    "xdem_gnn_create", "xdem_gnn_destroy", "xdem_gnn_set_params", "xdem_gnn_get_params", "xdem_gnn_get_grads",
+    "xdem_gnn_forward", "xdem_gnn_set_target", "xdem_gnn_backward", "xdem_gnn_adam", "xdem_gnn_train_window",
+    "xdem_gnn_dump_graph", "xdem_surrogate_run",
 ]
 
 
@@ -162,6 +180,21 @@ def load():
         "xdem_poly_interactions_count": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "xdem_poly_interactions_dump": (C.c_int, [vp, C.c_int64, _u64p, _u64p, C.POINTER(C.c_uint16),
                                                   C.POINTER(C.c_uint8), _u32p, _u32p, _dp, C.POINTER(C.c_int64)]),
+        "xdem_gnn_create": (C.c_int, [vp, C.POINTER(GnnConfig), C.POINTER(vp)]),
+        "xdem_gnn_destroy": (C.c_int, [vp]),
+        "xdem_gnn_set_params": (C.c_int, [vp, _fp, C.c_int64]),
+        "xdem_gnn_get_params": (C.c_int, [vp, _fp, C.c_int64]),
+        "xdem_gnn_get_grads": (C.c_int, [vp, _fp, C.c_int64]),
+        "xdem_gnn_forward": (C.c_int, [vp, _fp]),
+        "xdem_gnn_set_target": (C.c_int, [vp, _fp]),
+        "xdem_gnn_backward": (C.c_int, [vp, C.POINTER(C.c_double)]),
+        "xdem_gnn_adam": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_double]),
+        "xdem_gnn_train_window": (C.c_int, [vp, C.POINTER(Scheme), C.c_double, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_int32)]),
+        "xdem_gnn_dump_graph": (C.c_int, [vp, C.c_int64, _u32p, _u32p, C.POINTER(C.c_uint16),
+                                          C.POINTER(C.c_uint16), C.POINTER(C.c_int64)]),
+        "xdem_surrogate_run": (C.c_int, [vp, C.POINTER(Scheme), C.c_int64, C.c_int32, C.c_double,
+                                         C.POINTER(SurrogateStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -375,3 +408,94 @@ class Context:
         dims = (C.c_int32 * 3)()
         self.call("xdem_nbh_stats", C.byref(pairs), C.byref(slots), dims)
         return {"pairs": pairs.value, "slots": slots.value, "dims": tuple(dims)}
+
+
+def bf16_to_f32(bits):
+    """bfloat16 bit patterns (uint16) -> float32."""
+    return (np.asarray(bits, np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+class Gnn:
+    """The learned surrogate's network bound to a Context (xdem_gnn_* of include/xdem.h)."""
+
+    def __init__(self, ctx, r0, v0, a0, m0, ov_scale=50.0, substeps=10, train=False):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        cfg = GnnConfig(r0, v0, a0, m0, ov_scale, substeps, 1 if train else 0)
+        self.cfg = cfg
+        h = C.c_void_p()
+        ctx._check(self.lib.xdem_gnn_create(ctx.h, C.byref(cfg), C.byref(h)), "xdem_gnn_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.xdem_gnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def call(self, name, *args):
+        self.ctx._check(getattr(self.lib, name)(self.h, *args), name)
+
+    def set_params(self, flat):
+        a = np.ascontiguousarray(flat, np.float32)
+        assert a.shape == (GNN_NPARAMS,)
+        self.call("xdem_gnn_set_params", _ptr(a, C.c_float), GNN_NPARAMS)
+
+    def get_params(self):
+        a = np.zeros(GNN_NPARAMS, np.float32)
+        self.call("xdem_gnn_get_params", _ptr(a, C.c_float), GNN_NPARAMS)
+        return a
+
+    def get_grads(self):
+        a = np.zeros(GNN_NPARAMS, np.float32)
+        self.call("xdem_gnn_get_grads", _ptr(a, C.c_float), GNN_NPARAMS)
+        return a
+
+    def forward(self):
+        n = self.ctx.num_particles()
+        out = np.zeros((max(n, 1), 6), np.float32)
+        self.call("xdem_gnn_forward", _ptr(out, C.c_float))
+        return out[:n]
+
+    def set_target(self, t):
+        a = np.ascontiguousarray(t, np.float32)
+        self.call("xdem_gnn_set_target", _ptr(a, C.c_float))
+
+    def backward(self):
+        loss = C.c_double()
+        self.call("xdem_gnn_backward", C.byref(loss))
+        return loss.value
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.call("xdem_gnn_adam", lr, beta1, beta2, eps)
+
+    def train_window(self, scheme, lr):
+        loss = C.c_double()
+        used = C.c_int32()
+        self.call("xdem_gnn_train_window", C.byref(scheme), lr, C.byref(loss), C.byref(used))
+        return loss.value, bool(used.value)
+
+    def dump_graph(self):
+        """(src, dst, edge inputs [E][16] float32, node inputs [n][8] float32), device order."""
+        n = self.ctx.num_particles()
+        st = self.ctx.nbh_stats()
+        cap = int(st["slots"]) + 1
+        src = np.zeros(cap, np.uint32)
+        dst = np.zeros(cap, np.uint32)
+        e = np.zeros((cap, 16), np.uint16)
+        x = np.zeros((max(n, 1), 8), np.uint16)
+        k = C.c_int64()
+        self.call("xdem_gnn_dump_graph", cap, _ptr(src, C.c_uint32), _ptr(dst, C.c_uint32), _ptr(e, C.c_uint16),
+                  _ptr(x, C.c_uint16), C.byref(k))
+        k = k.value
+        return src[:k], dst[:k], bf16_to_f32(e[:k]), bf16_to_f32(x[:n])
+
+    def surrogate_run(self, scheme, nsteps, check_every=20, tol=0.01):
+        st = SurrogateStats()
+        self.call("xdem_surrogate_run", C.byref(scheme), nsteps, check_every, tol, C.byref(st))
+        return {f: getattr(st, f) for f, _ in SurrogateStats._fields_}

@@ -251,6 +251,79 @@ int xdem_set_timing(xdem_ctx* ctx, int32_t on);
  * recorded since the last reset (reset when reset != 0). */
 int xdem_kernel_timing(xdem_ctx* ctx, int32_t which, int32_t reset, double* avg_ms, int64_t* count);
 
+
+/* ------------------------------------------------------------------------------------------------
+ * Learned surrogate of K DEM steps. No counterpart in the reference (exaDEM has no learned part).
+ *
+ * An edge-message graph network on the contact neighbour list of the context: 3 message-passing
+ * layers of width 128, BF16 weights and activations, FP32 accumulation. Node inputs (8): radius and
+ * angular velocity; edge inputs (16, one per directed list entry i -> j): contact normal, relative
+ * overlap, relative surface velocity at the contact point, radius asymmetry, and the contact-gated
+ * overlap and normal-velocity vectors. Outputs per particle: the mean contact force and torque over
+ * the next K DEM steps in units of m0 a0 and m0 a0 r0; the surrogate step turns them into linear and
+ * angular accelerations with the particle's mass and inertia.
+ * The flat parameter vector (XDEM_GNN_NPARAMS floats) is, in order: W_in[128][8], b_in[128], then per
+ * layer W_AB[256][128], W_E[128][16], b_E[128], W_U[128][256], b_U[128], then W_out[6][128], b_out[6]
+ * (matrices row-major, out x in). Spheres only (no registered shapes).
+ * ---------------------------------------------------------------------------------------------- */
+#define XDEM_GNN_HIDDEN 128
+#define XDEM_GNN_LAYERS 3
+#define XDEM_GNN_NPARAMS 205446
+
+typedef struct xdem_gnn xdem_gnn;
+
+typedef struct xdem_gnn_config {
+  double r0;        /* radius scale of the inputs                            */
+  double v0;        /* velocity scale of the inputs                          */
+  double a0;        /* acceleration scale of the outputs                     */
+  double m0;        /* mass scale of the outputs                             */
+  double ov_scale;  /* factor of the relative-overlap input                  */
+  int32_t substeps; /* K: DEM steps replaced by one surrogate step           */
+  int32_t train;    /* non-zero: keep the activations of training             */
+} xdem_gnn_config;
+
+typedef struct xdem_surrogate_stats {
+  int64_t surrogate_steps; /* surrogate steps accepted                              */
+  int64_t dem_steps;       /* DEM steps run (drift checks and fallback)               */
+  int64_t checks;          /* drift checks made                                      */
+  int64_t fallbacks;       /* checks that failed (the DEM result was kept, DEM ran on) */
+  int64_t rebuilds;        /* neighbour rebuilds                                     */
+  double max_err;          /* largest checked position error / radius                */
+  double max_err_free;     /* same check for free flight under gravity (no network)   */
+} xdem_surrogate_stats;
+
+/* Creates a network bound to a context (parameters zero until set). */
+int xdem_gnn_create(xdem_ctx* ctx, const xdem_gnn_config* cfg, xdem_gnn** out);
+int xdem_gnn_destroy(xdem_gnn* gnn);
+/* Parameters and gradients as the flat FP32 vector described above (n = XDEM_GNN_NPARAMS). */
+int xdem_gnn_set_params(xdem_gnn* gnn, const float* params, int64_t n);
+int xdem_gnn_get_params(xdem_gnn* gnn, float* params, int64_t n);
+int xdem_gnn_get_grads(xdem_gnn* gnn, float* grads, int64_t n);
+/* Inputs and forward pass at the current state of the context and its neighbour list. out (host,
+ * may be NULL): [n][6] in device order. */
+int xdem_gnn_forward(xdem_gnn* gnn, float* out);
+/* Training targets, host [n][6] in device order (in the units of the outputs). */
+int xdem_gnn_set_target(xdem_gnn* gnn, const float* target);
+/* Backward pass of the mean squared error of the last forward pass (train mode); loss may be NULL. */
+int xdem_gnn_backward(xdem_gnn* gnn, double* loss);
+/* One Adam update with the gradients of the last backward pass. */
+int xdem_gnn_adam(xdem_gnn* gnn, double lr, double beta1, double beta2, double eps);
+/* One training sample from the DEM engine: forward at the current state, K fused DEM steps, targets
+ * m ((v(t+K dt) - v(t)) / (K dt) - g) and I (w(t+K dt) - w(t)) / (K dt), backward, Adam. *used = 0 when
+ * a neighbour rebuild fell inside the window (the DEM steps stand, no update is made). */
+int xdem_gnn_train_window(xdem_gnn* gnn, const xdem_scheme* scheme, double lr, double* loss, int32_t* used);
+/* Graph of the last forward pass in device order: per entry source and target index and the 8
+ * edge inputs (16 bfloat16 bit patterns); per particle the 8 node inputs. */
+int xdem_gnn_dump_graph(xdem_gnn* gnn, int64_t cap, uint32_t* src, uint32_t* dst, uint16_t* edge_in,
+                        uint16_t* node_in, int64_t* n_edges);
+/* Surrogate-accelerated run of nsteps surrogate steps (nsteps * K DEM steps of simulated time). Every
+ * check_every-th step is checked: the surrogate step and K DEM steps start from the same state, the
+ * largest position difference over radius is compared with tol, and the DEM state is kept. A
+ * failed check makes the next check_every steps plain DEM. Needs ids that are a permutation of
+ * 0..n-1. */
+int xdem_surrogate_run(xdem_gnn* gnn, const xdem_scheme* scheme, int64_t nsteps, int32_t check_every, double tol,
+                       xdem_surrogate_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif
