@@ -53,6 +53,8 @@ def build_hip(verbose=True):
 
 HOST_SOURCES = [
     "host/generators.cpp",
+    "host/msp.cpp",
+    "host/deck.cpp",
 ]
 CXX = os.environ.get("CXX", "g++")
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wno-unused-function"]
@@ -64,6 +66,9 @@ def build_host():
     srcs = [os.path.join(CSRC, s) for s in HOST_SOURCES]
     _run([CXX, *HOST_FLAGS, "-shared", "-o", out, *srcs,
           "-L" + LIBDIR, "-lxdem_hip", "-Wl,-rpath,$ORIGIN"])
+    exe = os.path.join(LIBDIR, "xdem-run")
+    _run([CXX, "-O2", "-std=c++17", "-Wall", "-o", exe, os.path.join(CSRC, "host", "xdem_run.cpp"),
+          "-L" + LIBDIR, "-lxdem_host", "-lxdem_hip", "-Wl,-rpath,$ORIGIN"])
     return out
 
 

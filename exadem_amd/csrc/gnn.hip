@@ -189,25 +189,6 @@ struct EncArgs {
   const float* b;
   uint16_t* h;
 };
-// one wavefront per particle, two channels per lane
-__global__ __launch_bounds__(256) void k_encode(EncArgs a) {
-  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (i >= a.n) return;
-  const int c = 2 * (threadIdx.x & 63);
-  float xv[FN];
-  for (int k = 0; k < FN; k++) xv[k] = bf2f(a.x[i * FN + k]);
-  float y0 = a.b[c], y1 = a.b[c + 1];
-  float s0 = 0.f, s1 = 0.f;
-  for (int k = 0; k < FN; k++) {
-    s0 = fmaf(a.w[c * FN + k], xv[k], s0);
-    s1 = fmaf(a.w[(c + 1) * FN + k], xv[k], s1);
-  }
-  y0 = fmaxf(s0 + y0, 0.f);
-  y1 = fmaxf(s1 + y1, 0.f);
-  const uint32_t pk = (uint32_t)f2bf(y0) | ((uint32_t)f2bf(y1) << 16);
-  *(uint32_t*)(a.h + i * H + c) = pk;
-}
-
 // ------------------------------------------------------------------------------------- MFMA GEMM
 enum { EPI_BF16 = 0, EPI_UPDATE = 1, EPI_BWD_U = 2, EPI_ADD = 3 };
 
@@ -227,64 +208,78 @@ struct GemmArgs {
   float* dm;        // EPI_BWD_U: [n][H] (=, times 1/8)
 };
 
-// Y[r][c] = sum_k A[r][k] B[c][k] with A = [a1 | a2]. Block: 4 wavefronts, 64 rows x 128 columns;
-// wavefront w: rows 32*(w&1), columns 64*(w>>1), two 32x32 MFMA tiles. Operand fragments: lane l
-// holds row (or column) l%32 and the 8 consecutive k values 8*(l/32)..+7 of each 16-wide k step;
-// accumulator element v of lane l is D[8*(v/4) + 4*(l/32) + v%4][l%32].
-template <int EPI>
-__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+// Y[r][c] = sum_k A[r][k] B[c][k] with A = [a1 | a2], for the fixed shapes of the network. MFMA
+// fragments: lane l holds row (or column) l%32 and the 8 consecutive k values 8*(l/32)..+7 of each
+// 16-wide k step; accumulator element v of lane l is D[8*(v/4) + 4*(l/32) + v%4][l%32].
+// Persistent blocks: each block stages B
+// (M x K bf16, rows padded by 8 elements so that the 32 rows read by one MFMA fragment load fall in
+// different LDS banks) once, then walks row tiles of 128 rows (one 32-row strip per wavefront, all M
+// columns: M/32 MFMA tiles per wavefront). A fragments come from global memory, row l%32, 16 bytes.
+template <int EPI, int M, int K1, int K2>
+__global__ __launch_bounds__(256) void k_gemm_p(GemmArgs g) {
+  constexpr int K = K1 + K2, KP = K + 8, NT = M / 32;
+  extern __shared__ uint16_t w_lds[];
+  for (int idx = threadIdx.x; idx < M * K / 8; idx += 256) {
+    const int r = idx / (K / 8), c8 = (idx % (K / 8)) * 8;
+    *(uint4*)(w_lds + r * KP + c8) = *(const uint4*)(g.b + (int64_t)r * K + c8);
+  }
+  __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int64_t r0 = (int64_t)blockIdx.x * 64 + (w & 1) * 32;
-  const int c0 = blockIdx.y * 128 + (w >> 1) * 64;
-  const int K = g.k1 + g.k2;
-  int64_t ra = r0 + (lane & 31);
-  if (ra >= g.n) ra = g.n - 1;
   const int kk = (lane >> 5) * 8;
-  floatx16 acc0, acc1;
-  for (int v = 0; v < 16; v++) {
-    acc0[v] = 0.f;
-    acc1[v] = 0.f;
-  }
-  const uint16_t* bp0 = g.b + (int64_t)(c0 + (lane & 31)) * K + kk;
-  const uint16_t* bp1 = bp0 + (int64_t)32 * K;
-  const uint16_t* ap1 = g.a1 + ra * g.k1 + kk;
-  for (int k = 0; k < g.k1; k += 16) {
-    const bf16x8 av = *(const bf16x8*)(ap1 + k);
-    const bf16x8 b0 = *(const bf16x8*)(bp0 + k);
-    const bf16x8 b1 = *(const bf16x8*)(bp1 + k);
-    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b0, acc0, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b1, acc1, 0, 0, 0);
-  }
-  if (g.k2 > 0) {
-    const uint16_t* ap2 = g.a2 + ra * g.k2 + kk;
-    for (int k = 0; k < g.k2; k += 16) {
-      const bf16x8 av = *(const bf16x8*)(ap2 + k);
-      const bf16x8 b0 = *(const bf16x8*)(bp0 + g.k1 + k);
-      const bf16x8 b1 = *(const bf16x8*)(bp1 + g.k1 + k);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b1, acc1, 0, 0, 0);
+  const int64_t ntiles = (g.n + 127) / 128;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t r0 = tile * 128 + w * 32;
+    if (r0 >= g.n) continue;
+    int64_t ra = r0 + (lane & 31);
+    if (ra >= g.n) ra = g.n - 1;
+    floatx16 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) acc[t][v] = 0.f;
+    const uint16_t* ap1 = g.a1 + ra * K1 + kk;
+#pragma unroll 2
+    for (int k = 0; k < K1; k += 16) {
+      const bf16x8 av = *(const bf16x8*)(ap1 + k);
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        const bf16x8 bv = *(const bf16x8*)(w_lds + (t * 32 + (lane & 31)) * KP + kk + k);
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[t], 0, 0, 0);
+      }
     }
-  }
+    if (K2 > 0) {
+      const uint16_t* ap2 = g.a2 + ra * K2 + kk;
+#pragma unroll 2
+      for (int k = 0; k < K2; k += 16) {
+        const bf16x8 av = *(const bf16x8*)(ap2 + k);
 #pragma unroll
-  for (int t = 0; t < 2; t++) {
-    const int c = c0 + t * 32 + (lane & 31);
+        for (int t = 0; t < NT; t++) {
+          const bf16x8 bv = *(const bf16x8*)(w_lds + (t * 32 + (lane & 31)) * KP + kk + K1 + k);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[t], 0, 0, 0);
+        }
+      }
+    }
 #pragma unroll
-    for (int v = 0; v < 16; v++) {
-      const int64_t r = r0 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-      if (r >= g.n) continue;
-      const float y = t ? acc1[v] : acc0[v];
-      if (EPI == EPI_BF16) {
-        g.ybf[r * g.m + c] = f2bf(y);
-      } else if (EPI == EPI_UPDATE) {
-        const uint16_t ub = f2bf(y + g.bias[c]);
-        if (g.u_out) g.u_out[r * H + c] = ub;
-        const float hv = bf2f(g.a1[r * H + c]);
-        g.h_out[r * H + c] = f2bf(hv + fmaxf(bf2f(ub), 0.f));
-      } else if (EPI == EPI_BWD_U) {
-        if (c < H) g.dh[r * H + c] += y;
-        else g.dm[r * H + (c - H)] = y * kMsgScale;
-      } else {
-        g.dh[r * H + c] += y;
+    for (int t = 0; t < NT; t++) {
+      const int c = t * 32 + (lane & 31);
+#pragma unroll
+      for (int v = 0; v < 16; v++) {
+        const int64_t r = r0 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        if (r >= g.n) continue;
+        const float y = acc[t][v];
+        if (EPI == EPI_BF16) {
+          g.ybf[r * M + c] = f2bf(y);
+        } else if (EPI == EPI_UPDATE) {
+          const uint16_t ub = f2bf(y + g.bias[c]);
+          if (g.u_out) g.u_out[r * H + c] = ub;
+          const float hv = bf2f(g.a1[r * H + c]);
+          g.h_out[r * H + c] = f2bf(hv + fmaxf(bf2f(ub), 0.f));
+        } else if (EPI == EPI_BWD_U) {
+          if (c < H) g.dh[r * H + c] += y;
+          else g.dm[r * H + (c - H)] = y * kMsgScale;
+        } else {
+          g.dh[r * H + c] += y;
+        }
       }
     }
   }
@@ -424,36 +419,83 @@ __device__ inline void ld2(const uint16_t* p, float& a, float& b) {
   b = bf2f((uint16_t)(v >> 16));
 }
 
-// one wavefront per particle (grid-stride), lanes own channels 2l, 2l+1
-__global__ __launch_bounds__(256) void k_msg_fwd(MsgArgs a) {
+// Message pass on MFMA: one wavefront per 32 consecutive particles (half of a 64-particle list
+// group). Step k takes neighbour rank k of each of the 32 particles, i.e. 32 consecutive list slots,
+// and computes D[channel][edge] = sum_f W_E[channel][f] e[edge][f] + (P_i + b_E)[channel] for the 128
+// channels with four 32x32x16 MFMAs (A = W_E rows, B = the edges' 16 inputs, C = P_i + b_E); then
+// z = D + Q_j, and relu(z) is summed over k in the accumulator layout: lane l holds edge column l%32
+// and, per 32-channel tile, the 4-channel groups 8g + 4*(l/32) .. +3.
+__global__ __launch_bounds__(256) void k_msg_fwd_mfma(MsgArgs a, const uint16_t* __restrict__ wEb) {
   const int lane = threadIdx.x & 63;
-  const int c = 2 * lane;
-  float w0[FE], w1[FE];
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t i0 = wave * 32;
+  if (i0 >= a.n) return;
+  const int col = lane & 31, half = lane >> 5;
+  const int64_t i = i0 + col;
+  const bool vi = i < a.n;
+  const int64_t ic = vi ? i : a.n - 1;
+  const int64_t grp = i0 >> 6;
+  const uint64_t base = a.nl.base[grp] + (uint64_t)(ic & 63);
+  const uint32_t cnt = vi ? a.nl.cnt[ic] : 0u;
+  const uint32_t kmax = a.nl.kmax[grp];
+  bf16x8 wa[4];
 #pragma unroll
-  for (int k = 0; k < FE; k++) {
-    w0[k] = a.wE[c * FE + k];
-    w1[k] = a.wE[(c + 1) * FE + k];
-  }
-  const float b0 = a.bE[c], b1 = a.bE[c + 1];
-  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
-  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < a.n; i += nw) {
-    float P0, P1;
-    ld2(a.pq + i * H2 + c, P0, P1);
-    const uint32_t cnt = a.nl.cnt[i];
-    const uint64_t base = a.nl.base[i >> 6] + (uint64_t)(i & 63);
-    float s0 = 0.f, s1 = 0.f;
-    for (uint32_t k = 0; k < cnt; k++) {
-      const uint64_t sl = base + (uint64_t)k * kWave;
-      const uint32_t j = a.nl.word[sl] & kIdxMask;
-      float ev[FE];
-      load_e(a.e + sl * FE, ev);
-      float Q0, Q1;
-      ld2(a.pq + (int64_t)j * H2 + H + c, Q0, Q1);
-      s0 += fmaxf(edge_z(P0, Q0, ev, w0, b0), 0.f);
-      s1 += fmaxf(edge_z(P1, Q1, ev, w1, b1), 0.f);
+  for (int t = 0; t < 4; t++) wa[t] = *(const bf16x8*)(wEb + (32 * t + col) * FE + 8 * half);
+  floatx16 cinit[4], msum[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int gq = 0; gq < 4; gq++) {
+      const int c0 = 32 * t + 8 * gq + 4 * half;
+      const uint2 pv = *(const uint2*)(a.pq + ic * H2 + c0);
+      cinit[t][4 * gq + 0] = bf2f((uint16_t)(pv.x & 0xffffu)) + a.bE[c0 + 0];
+      cinit[t][4 * gq + 1] = bf2f((uint16_t)(pv.x >> 16)) + a.bE[c0 + 1];
+      cinit[t][4 * gq + 2] = bf2f((uint16_t)(pv.y & 0xffffu)) + a.bE[c0 + 2];
+      cinit[t][4 * gq + 3] = bf2f((uint16_t)(pv.y >> 16)) + a.bE[c0 + 3];
+#pragma unroll
+      for (int q = 0; q < 4; q++) msum[t][4 * gq + q] = 0.f;
     }
-    *(uint32_t*)(a.m + i * H + c) = (uint32_t)f2bf(s0 * kMsgScale) | ((uint32_t)f2bf(s1 * kMsgScale) << 16);
+  for (uint32_t k = 0; k < kmax; k++) {
+    const uint64_t sl = base + (uint64_t)k * kWave;
+    const bool ok = k < cnt;
+    uint32_t j = (uint32_t)ic;
+    bf16x8 eb;
+    if (ok) {
+      j = a.nl.word[sl] & kIdxMask;
+      eb = *(const bf16x8*)(a.e + sl * FE + 8 * half);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; q++) eb[q] = (__bf16)0.0f;
+    }
+    const uint16_t* qrow = a.pq + (int64_t)j * H2 + H;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const floatx16 d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[t], eb, cinit[t], 0, 0, 0);
+#pragma unroll
+      for (int gq = 0; gq < 4; gq++) {
+        const uint2 qv = *(const uint2*)(qrow + 32 * t + 8 * gq + 4 * half);
+        const float q0 = bf2f((uint16_t)(qv.x & 0xffffu)), q1 = bf2f((uint16_t)(qv.x >> 16));
+        const float q2 = bf2f((uint16_t)(qv.y & 0xffffu)), q3 = bf2f((uint16_t)(qv.y >> 16));
+        if (ok) {
+          msum[t][4 * gq + 0] += fmaxf(d[4 * gq + 0] + q0, 0.f);
+          msum[t][4 * gq + 1] += fmaxf(d[4 * gq + 1] + q1, 0.f);
+          msum[t][4 * gq + 2] += fmaxf(d[4 * gq + 2] + q2, 0.f);
+          msum[t][4 * gq + 3] += fmaxf(d[4 * gq + 3] + q3, 0.f);
+        }
+      }
+    }
   }
+  if (!vi) return;
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int gq = 0; gq < 4; gq++) {
+      const int c0 = 32 * t + 8 * gq + 4 * half;
+      uint2 o;
+      o.x = (uint32_t)f2bf(msum[t][4 * gq + 0] * kMsgScale) | ((uint32_t)f2bf(msum[t][4 * gq + 1] * kMsgScale) << 16);
+      o.y = (uint32_t)f2bf(msum[t][4 * gq + 2] * kMsgScale) | ((uint32_t)f2bf(msum[t][4 * gq + 3] * kMsgScale) << 16);
+      *(uint2*)(a.m + i * H + c0) = o;
+    }
 }
 
 // backward of the messages: per particle j, dP_j = sum over own entries j->i of dm_j (z_ji > 0),
@@ -608,6 +650,56 @@ __global__ __launch_bounds__(256) void k_decode(DecArgs a) {
     for (int idx = threadIdx.x; idx < len; idx += blockDim.x)
       a.part[(int64_t)blockIdx.x * len + idx] = ((red[0][idx] + red[1][idx]) + red[2][idx]) + red[3][idx];
   }
+}
+
+// encoder, one thread per (particle, 8 channels)
+__global__ __launch_bounds__(256) void k_encode8(EncArgs a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t >> 4;
+  if (i >= a.n) return;
+  const int c0 = (int)(t & 15) * 8;
+  const uint4 xp = *(const uint4*)(a.x + i * FN);
+  uint16_t xb[FN];
+  memcpy(xb, &xp, 16);
+  float xv[FN];
+#pragma unroll
+  for (int k = 0; k < FN; k++) xv[k] = bf2f(xb[k]);
+  uint16_t o[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < FN; k++) s = fmaf(a.w[(c0 + q) * FN + k], xv[k], s);
+    o[q] = f2bf(fmaxf(s + a.b[c0 + q], 0.f));
+  }
+  uint4 op;
+  memcpy(&op, o, 16);
+  *(uint4*)(a.h + i * H + c0) = op;
+}
+
+// decoder (forward only), one thread per particle, W_out staged in LDS
+__global__ __launch_bounds__(256) void k_decode1(DecArgs a) {
+  __shared__ float wo[FO * H];
+  for (int k = threadIdx.x; k < FO * H; k += blockDim.x) wo[k] = a.w[k];
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  float y[FO];
+#pragma unroll
+  for (int o = 0; o < FO; o++) y[o] = 0.f;
+  for (int c8 = 0; c8 < H; c8 += 8) {
+    const uint4 hp = *(const uint4*)(a.h + i * H + c8);
+    uint16_t hb[8];
+    memcpy(hb, &hp, 16);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const float hv = bf2f(hb[q]);
+#pragma unroll
+      for (int o = 0; o < FO; o++) y[o] = fmaf(wo[o * H + c8 + q], hv, y[o]);
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < FO; o++) a.out[i * FO + o] = y[o] + a.b[o];
 }
 
 // ------------------------------------------------------------------------------------- elementwise
@@ -913,16 +1005,17 @@ int refresh_weights(xdem_gnn* G) {
   return XDEM_OK;
 }
 
-int gemm(xdem_gnn* G, int epi, GemmArgs a) {
+template <int EPI, int M, int K1, int K2>
+int gemm_p(xdem_gnn* G, const GemmArgs& a) {
   xdem_ctx* c = G->c;
   if (a.n <= 0) return XDEM_OK;
-  const dim3 grid(nb(a.n, 64), a.m / 128);
-  switch (epi) {
-    case EPI_BF16: hipLaunchKernelGGL(k_gemm<EPI_BF16>, grid, dim3(256), 0, c->stream, a); break;
-    case EPI_UPDATE: hipLaunchKernelGGL(k_gemm<EPI_UPDATE>, grid, dim3(256), 0, c->stream, a); break;
-    case EPI_BWD_U: hipLaunchKernelGGL(k_gemm<EPI_BWD_U>, grid, dim3(256), 0, c->stream, a); break;
-    default: hipLaunchKernelGGL(k_gemm<EPI_ADD>, grid, dim3(256), 0, c->stream, a); break;
-  }
+  constexpr int K = K1 + K2;
+  const int lds = M * (K + 8) * (int)sizeof(uint16_t);
+  XDEM_HIP(c, hipFuncSetAttribute((const void*)k_gemm_p<EPI, M, K1, K2>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  const int64_t ntiles = (a.n + 127) / 128;
+  const int grid = (int)std::min<int64_t>(ntiles, 512);
+  hipLaunchKernelGGL((k_gemm_p<EPI, M, K1, K2>), dim3(grid), dim3(256), lds, c->stream, a);
   XDEM_HIP(c, hipGetLastError());
   return XDEM_OK;
 }
@@ -992,9 +1085,8 @@ int forward(xdem_gnn* G) {
   hipLaunchKernelGGL(k_features, dim3(nb(n, 256)), dim3(256), 0, c->stream, fa);
   if (tr) hipLaunchKernelGGL(k_mirror_all, dim3(nb(n, 256)), dim3(256), 0, c->stream, n, nl, G->mirror);
   EncArgs ea{n, G->x, G->wr + G->off.W_in, G->wr + G->off.b_in, G->h[0]};
-  hipLaunchKernelGGL(k_encode, dim3(nb(n * 64, 256)), dim3(256), 0, c->stream, ea);
+  hipLaunchKernelGGL(k_encode8, dim3(nb(n * 16, 256)), dim3(256), 0, c->stream, ea);
   XDEM_HIP(c, hipGetLastError());
-  const int wgrid = (int)std::min<int64_t>(nb(n * 64, 256), 8192);
   for (int l = 0; l < L; l++) {
     uint16_t* hin = tr ? G->h[l] : G->h[l & 1];
     uint16_t* hout = tr ? G->h[l + 1] : G->h[(l + 1) & 1];
@@ -1003,20 +1095,21 @@ int forward(xdem_gnn* G) {
     GemmArgs g1{};
     g1.n = n; g1.a1 = hin; g1.k1 = H; g1.a2 = nullptr; g1.k2 = 0;
     g1.b = G->wbf + G->off.W_AB[l]; g1.m = H2; g1.ybf = pq;
-    if ((rc = gemm(G, EPI_BF16, g1))) return rc;
+    if ((rc = gemm_p<EPI_BF16, H2, H, 0>(G, g1))) return rc;
     MsgArgs ma{};
     ma.n = n; ma.nl = nl; ma.e = G->e; ma.pq = pq;
     ma.wE = G->wr + G->off.W_E[l]; ma.bE = G->wr + G->off.b_E[l]; ma.m = mm;
-    hipLaunchKernelGGL(k_msg_fwd, dim3(wgrid), dim3(256), 0, c->stream, ma);
+    hipLaunchKernelGGL(k_msg_fwd_mfma, dim3(nb((n + 31) / 32 * 64, 256)), dim3(256), 0, c->stream, ma,
+                       G->wbf + G->off.W_E[l]);
     GemmArgs g2{};
     g2.n = n; g2.a1 = hin; g2.k1 = H; g2.a2 = mm; g2.k2 = H;
     g2.b = G->wbf + G->off.W_U[l]; g2.m = H; g2.bias = G->wr + G->off.b_U[l];
     g2.u_out = tr ? G->u[l] : nullptr; g2.h_out = hout;
-    if ((rc = gemm(G, EPI_UPDATE, g2))) return rc;
+    if ((rc = gemm_p<EPI_UPDATE, H, H, H>(G, g2))) return rc;
   }
   DecArgs da{n, tr ? G->h[L] : G->h[L & 1], G->wr + G->off.W_out, G->wr + G->off.b_out, G->out, nullptr, nullptr,
              nullptr};
-  hipLaunchKernelGGL(k_decode<false>, dim3(wgrid), dim3(256), 0, c->stream, da);
+  hipLaunchKernelGGL(k_decode1, dim3(nb(n, 256)), dim3(256), 0, c->stream, da);
   XDEM_HIP(c, hipGetLastError());
   G->fwd_valid = true;
   G->fwd_n = n;
@@ -1053,7 +1146,7 @@ int backward(xdem_gnn* G, double* loss) {
     g1.n = n; g1.a1 = G->du; g1.k1 = H; g1.k2 = 0;
     g1.b = tab + (int64_t)H * H2;  // W_U^T [2H][H]
     g1.m = H2; g1.dh = G->dh; g1.dm = G->dm;
-    if ((rc = gemm(G, EPI_BWD_U, g1))) return rc;
+    if ((rc = gemm_p<EPI_BWD_U, H2, H, 0>(G, g1))) return rc;
     // messages: dP|dQ, dW_E, db_E
     const int mb = wgrid;
     if ((rc = ensure_partial(G, (int64_t)mb * H * (FE + 1)))) return rc;
@@ -1074,7 +1167,7 @@ int backward(xdem_gnn* G, double* loss) {
     g2.n = n; g2.a1 = G->dpq; g2.k1 = H2; g2.k2 = 0;
     g2.b = tab;  // W_AB^T [H][2H]
     g2.m = H; g2.dh = G->dh;
-    if ((rc = gemm(G, EPI_ADD, g2))) return rc;
+    if ((rc = gemm_p<EPI_ADD, H, H2, 0>(G, g2))) return rc;
   }
   // encoder
   {

@@ -499,3 +499,83 @@ class Gnn:
         st = SurrogateStats()
         self.call("xdem_surrogate_run", C.byref(scheme), nsteps, check_every, tol, C.byref(st))
         return {f: getattr(st, f) for f, _ in SurrogateStats._fields_}
+
+
+# ------------------------------------------------------------------------------ host library
+class DeckStats(C.Structure):
+    _fields_ = [("particles", C.c_int64), ("iterations", C.c_int64), ("rebuilds", C.c_int64),
+                ("operator_calls", C.c_int64), ("seconds", C.c_double)]
+
+
+HOST_SYMBOLS = ["xdem_gen_uniform", "xdem_gen_normal", "xdem_gen_pack_periodic", "xdem_deck_load",
+                "xdem_deck_set_global", "xdem_deck_run", "xdem_deck_last_error", "xdem_deck_destroy"]
+CONFIG_DIR = os.path.join(HERE, "config")
+_host = None
+
+
+def load_host():
+    """libxdem_host.so (deck runner, generators); needs libxdem_hip.so loaded first."""
+    global _host
+    if _host is not None:
+        return _host
+    load()
+    path = os.path.join(LIBDIR, "libxdem_host.so")
+    if not os.path.exists(path):
+        raise XdemError(f"{path} is missing: build it with exadem_amd/build.py")
+    h = C.CDLL(path)
+    vp = C.c_void_p
+    h.xdem_deck_load.restype = C.c_int
+    h.xdem_deck_load.argtypes = [C.c_char_p, C.POINTER(C.c_char_p), C.c_int32, C.POINTER(vp)]
+    h.xdem_deck_set_global.restype = C.c_int
+    h.xdem_deck_set_global.argtypes = [vp, C.c_char_p, C.c_char_p]
+    h.xdem_deck_run.restype = C.c_int
+    h.xdem_deck_run.argtypes = [vp, vp, C.POINTER(DeckStats)]
+    h.xdem_deck_last_error.restype = C.c_int
+    h.xdem_deck_last_error.argtypes = [vp, C.c_char_p, C.c_int64]
+    h.xdem_deck_destroy.restype = None
+    h.xdem_deck_destroy.argtypes = [vp]
+    _host = h
+    return h
+
+
+class Deck:
+    """An exaDEM-style deck (.msp) loaded by the C++ runner of libxdem_host.so."""
+
+    def __init__(self, path, search_dirs=()):
+        self.lib = load_host()
+        dirs = [CONFIG_DIR, *search_dirs]
+        arr = (C.c_char_p * len(dirs))(*[d.encode() for d in dirs])
+        h = C.c_void_p()
+        rc = self.lib.xdem_deck_load(str(path).encode(), arr, len(dirs), C.byref(h))
+        self.h = h
+        if rc != XDEM_OK:
+            msg = self.last_error()
+            self.close()
+            raise XdemError(f"xdem_deck_load: {ERR_NAMES.get(rc, rc)}: {msg}")
+
+    def last_error(self):
+        buf = C.create_string_buffer(1024)
+        if self.h:
+            self.lib.xdem_deck_last_error(self.h, buf, 1024)
+        return buf.value.decode()
+
+    def set_global(self, name, value):
+        self.lib.xdem_deck_set_global(self.h, name.encode(), str(value).encode())
+
+    def run(self, ctx):
+        st = DeckStats()
+        rc = self.lib.xdem_deck_run(self.h, ctx.h, C.byref(st))
+        if rc != XDEM_OK:
+            raise XdemError(f"xdem_deck_run: {ERR_NAMES.get(rc, rc)}: {self.last_error()}")
+        return {f: getattr(st, f) for f, _ in DeckStats._fields_}
+
+    def close(self):
+        if self.h:
+            self.lib.xdem_deck_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

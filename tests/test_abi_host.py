@@ -25,6 +25,15 @@ def test_hip_library_exports_every_declared_symbol():
     assert set(names) <= set(native.HIP_SYMBOLS) | set(names)
 
 
+def test_host_library_exports_every_declared_symbol():
+    lib = native.load_host()
+    names = declared("xdem_host.h")
+    assert len(names) >= 8
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(native.HOST_SYMBOLS)
+
+
 def test_context_creation_fails_cleanly_without_gpu():
     import torch
     if torch.cuda.is_available():
