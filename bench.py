@@ -17,6 +17,8 @@ import platform
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -77,6 +79,8 @@ def main():
                     help="run a device copy of 1 GiB (counter calibration for --pmc runs) and exit")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--no-forms", action="store_true", help="skip the FP32-force and surrogate lines")
+    ap.add_argument("--sur-steps", type=int, default=100, help="timed surrogate steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not provided "
@@ -103,33 +107,99 @@ def main():
     w = inputs.polydisperse(n=args.n, relax_sweeps=args.relax_sweeps)
     t_gen = time.time() - t0
     log(f"generated {w.n} spheres in {t_gen:.1f} s (max overlap {w.meta['max_overlap']:.3g})")
-    form = native.XDEM_FORM_FP32 if args.form == "fp32" else native.XDEM_FORM_FP64
-    sim = SphereSim(w, form=form)
-    sim.first_iteration()
-    sim.run(args.warmup, poll=args.poll)
-    sim.ctx.synchronize()
-    st_nb = sim.ctx.nbh_stats()
-    sim.ctx.set_timing(True)
-    sim.ctx.kernel_timing(0, reset=True)
-    sim.ctx.kernel_timing(1, reset=True)
-    sim.ctx.synchronize()
-    t1 = time.perf_counter()
-    st = sim.run(args.steps, poll=args.poll)
-    sim.ctx.synchronize()
-    t2 = time.perf_counter()
-    sim.ctx.set_timing(False)
-    elapsed = t2 - t1
-    log(f"timed {args.steps} steps in {elapsed:.3f} s")
-    f_ms, f_cnt = sim.ctx.kernel_timing(1)
-    i_ms, i_cnt = sim.ctx.kernel_timing(0)
-    st_nb2 = sim.ctx.nbh_stats()
-    pairs = st_nb2["pairs"]
     n = w.n
+
+    def timed(form_name):
+        form = native.XDEM_FORM_FP32 if form_name == "fp32" else native.XDEM_FORM_FP64
+        sim = SphereSim(w, form=form)
+        sim.first_iteration()
+        sim.run(args.warmup, poll=args.poll)
+        sim.ctx.synchronize()
+        sim.ctx.set_timing(True)
+        sim.ctx.kernel_timing(0, reset=True)
+        sim.ctx.kernel_timing(1, reset=True)
+        sim.ctx.synchronize()
+        t1 = time.perf_counter()
+        st = sim.run(args.steps, poll=args.poll)
+        sim.ctx.synchronize()
+        elapsed = time.perf_counter() - t1
+        sim.ctx.set_timing(False)
+        f_ms, f_cnt = sim.ctx.kernel_timing(1)
+        i_ms, _ = sim.ctx.kernel_timing(0)
+        nb = sim.ctx.nbh_stats()
+        log(f"{form_name}: timed {args.steps} steps in {elapsed:.3f} s (force {f_ms * 1e3:.1f} us, "
+            f"integrate {i_ms * 1e3:.1f} us)")
+        return sim, {"elapsed": elapsed, "f_ms": f_ms, "f_cnt": f_cnt, "i_ms": i_ms, "stats": st, "nbh": nb}
+
+    # headline: the plain FP64 form
+    sim64, r64 = timed(args.form)
+    elapsed = r64["elapsed"]
+    f_ms, f_cnt, i_ms, st, st_nb2 = r64["f_ms"], r64["f_cnt"], r64["i_ms"], r64["stats"], r64["nbh"]
+    pairs = st_nb2["pairs"]
     step_bytes, force_bytes = bytes_model(args.form, n, pairs)
     value = n * args.steps / elapsed
     ms_step = 1e3 * elapsed / args.steps
     achieved = force_bytes / (f_ms * 1e-3) / 1e9 if f_ms > 0 else 0.0
     traffic, traffic_src = profiled_traffic(args.form)
+    del sim64
+
+    forms = {}
+    if args.form == "fp64" and not args.no_forms:
+        # FP32-force form: throughput, and the error of its first force pass against the FP64 pass
+        # from the same state (same contacts: the sign test is FP64)
+        a = SphereSim(w, form=native.XDEM_FORM_FP64)
+        a.first_iteration()
+        b = SphereSim(w, form=native.XDEM_FORM_FP32)
+        b.first_iteration()
+        pa, pb = a.ctx.download(), b.ctx.download()
+        oa, ob = np.argsort(pa["id"]), np.argsort(pb["id"])
+        fa = np.stack([pa[k][oa] for k in ("fx", "fy", "fz")], 1)
+        fb = np.stack([pb[k][ob] for k in ("fx", "fy", "fz")], 1)
+        floor = pa["mass"][oa] * 9.81
+        ferr = float(np.max(np.linalg.norm(fa - fb, axis=1) / np.maximum(np.linalg.norm(fa, axis=1), floor)))
+        ia, ib = a.ctx.interactions(), b.ctx.interactions()
+        # contact set: the interactions whose friction history is non-zero after the first pass
+        act_a, act_b = np.any(ia[3] != 0, axis=1), np.any(ib[3] != 0, axis=1)
+        same_set = (len(ia[0]) == len(ib[0]) and all(np.array_equal(x, y) for x, y in zip(ia[:3], ib[:3]))
+                    and np.array_equal(act_a, act_b))
+        del a, b
+        _, r32 = timed("fp32")
+        v32 = n * args.steps / r32["elapsed"]
+        forms["fp32"] = {"value": v32, "ms_per_step": 1e3 * r32["elapsed"] / args.steps,
+                         "force_avg_ms": r32["f_ms"], "speedup_vs_fp64": v32 / value,
+                         "max_force_rel_err": ferr, "contact_set_identical": bool(same_set),
+                         "contacts": int(np.count_nonzero(act_a)),
+                         "error_stated_as": "max over particles of |F32 - F64| / max(|F64|, m g), first force pass"}
+        # surrogate-accelerated form (trained weights of the network, K = 10 DEM steps per step)
+        from exadem_amd import surrogate
+        if os.path.exists(surrogate.WEIGHTS):
+            params, scales = surrogate.load()
+            s = SphereSim(w)
+            s.first_iteration()
+            surrogate.settle(s, scales["substeps"])
+            g = surrogate.attach(s.ctx, params, scales)
+            sch = scheme(w)
+            g.surrogate_run(sch, 20, check_every=20, tol=0.01)
+            s.ctx.synchronize()
+            t1 = time.perf_counter()
+            sst = g.surrogate_run(sch, args.sur_steps, check_every=20, tol=0.01)
+            s.ctx.synchronize()
+            el = time.perf_counter() - t1
+            K = scales["substeps"]
+            eff = n * args.sur_steps * K / el
+            forms["surrogate"] = {"value": eff, "unit": "effective particle-steps/s (simulated time x n / dt)",
+                                  "ms_per_surrogate_step": 1e3 * el / args.sur_steps, "substeps": K,
+                                  "speedup_vs_fp64": eff / value,
+                                  "simulated_time_per_s": args.sur_steps * K * w.dt / el,
+                                  "checks": sst["checks"], "fallbacks": sst["fallbacks"],
+                                  "max_pos_err_over_radius": sst["max_err"],
+                                  "max_pos_err_over_radius_free_flight": sst["max_err_free"],
+                                  "error_stated_as": "largest |x_surrogate - x_DEM| / R over the drift checks "
+                                                     "(one surrogate step against K DEM steps from the same state)"}
+            log(f"surrogate: {args.sur_steps} steps in {el:.3f} s, checks {sst['checks']}, "
+                f"max err {sst['max_err']:.3g} R")
+        else:
+            forms["surrogate"] = {"value": None, "status": "no trained weights at " + surrogate.WEIGHTS}
 
     cpu = None
     if not args.no_cpu:
@@ -176,6 +246,7 @@ def main():
                      "bytes_per_launch": force_bytes, "integrate_avg_ms": i_ms,
                      "step_bytes": step_bytes, "step_frac": step_bytes / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS},
         "cpu_baseline": cpu,
+        "forms": forms,
         "setup_s": {"generate": t_gen},
     }
     print(json.dumps(out), flush=True)
