@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--form", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--no-forms", action="store_true", help="skip the FP32-force and surrogate lines")
     ap.add_argument("--sur-steps", type=int, default=100, help="timed surrogate steps")
+    ap.add_argument("--poly-n", type=int, default=200_000)
+    ap.add_argument("--poly-steps", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not provided "
@@ -200,6 +202,35 @@ def main():
                 f"max err {sst['max_err']:.3g} R")
         else:
             forms["surrogate"] = {"value": None, "status": "no trained weights at " + surrogate.WEIGHTS}
+        # polyhedra: cubes (edge 1) and octahedra (circumradius 0.8), Minkowski radius 0.02, random
+        # orientations, jittered lattice (spacing 1.55, solid fraction 0.25) periodic in x and y with
+        # a floor and a lid, gravity, dt 2e-6
+        from exadem_amd.sim import PolySim
+        wp = inputs.polyhedra(n=args.poly_n, spacing=1.55, jitter=0.05, seed=4, dt=2e-6)
+        ps = PolySim(wp)
+        ps.first_iteration()
+        ps.run(20, poll=args.poll)
+        ps.ctx.synchronize()
+        ps.ctx.set_timing(True)
+        ps.ctx.kernel_timing(0, reset=True)
+        ps.ctx.kernel_timing(1, reset=True)
+        t1 = time.perf_counter()
+        pst = ps.run(args.poly_steps, poll=args.poll)
+        ps.ctx.synchronize()
+        el = time.perf_counter() - t1
+        ps.ctx.set_timing(False)
+        pf_ms, _ = ps.ctx.kernel_timing(1)
+        pi_ms, _ = ps.ctx.kernel_timing(0)
+        nfeat = ps.ctx.poly_interactions_count()
+        vol = float(np.sum(wp.particles["mass"] / 2500.0))
+        forms["polyhedra"] = {"value": wp.n * args.poly_steps / el, "unit": "particle-steps/s",
+                              "n": wp.n, "steps": args.poly_steps, "ms_per_step": 1e3 * el / args.poly_steps,
+                              "force_avg_ms": pf_ms, "integrate_vertices_avg_ms": pi_ms,
+                              "feature_interactions": nfeat, "rebuilds": pst["rebuilds"],
+                              "solid_fraction": vol / float(np.prod(wp.meta["box"])),
+                              "workload": "cubes + octahedra 50/50, random orientations, jittered lattice "
+                                          "spacing 1.55, periodic x/y, floor and lid, dt 2e-6"}
+        log(f"polyhedra: {args.poly_steps} steps of {wp.n} in {el:.3f} s (force {pf_ms * 1e3:.1f} us)")
 
     cpu = None
     if not args.no_cpu:

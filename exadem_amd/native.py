@@ -388,6 +388,11 @@ class Context:
     def contact_polyhedron(self, cp, cpd=None, dt=1e-5):
         self.call("xdem_contact_polyhedron", C.byref(cp), C.byref(cpd) if cpd is not None else None, dt)
 
+    def poly_interactions_count(self):
+        n = C.c_int64()
+        self.call("xdem_poly_interactions_count", C.byref(n))
+        return n.value
+
     def poly_interactions(self):
         n = C.c_int64()
         self.call("xdem_poly_interactions_count", C.byref(n))
