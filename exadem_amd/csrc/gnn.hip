@@ -217,11 +217,13 @@ struct GemmArgs {
 // columns: M/32 MFMA tiles per wavefront). A fragments come from global memory, row l%32, 16 bytes.
 template <int EPI, int M, int K1, int K2>
 __global__ __launch_bounds__(256) void k_gemm_p(GemmArgs g) {
-  constexpr int K = K1 + K2, KP = K + 8, NT = M / 32;
+  // each block owns 128 of the M columns (blockIdx.y) and stages those rows of B
+  constexpr int K = K1 + K2, KP = K + 8, NT = 4;
   extern __shared__ uint16_t w_lds[];
-  for (int idx = threadIdx.x; idx < M * K / 8; idx += 256) {
+  const int cb = (int)blockIdx.y * 128;
+  for (int idx = threadIdx.x; idx < 128 * K / 8; idx += 256) {
     const int r = idx / (K / 8), c8 = (idx % (K / 8)) * 8;
-    *(uint4*)(w_lds + r * KP + c8) = *(const uint4*)(g.b + (int64_t)r * K + c8);
+    *(uint4*)(w_lds + r * KP + c8) = *(const uint4*)(g.b + (int64_t)(cb + r) * K + c8);
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -237,31 +239,27 @@ __global__ __launch_bounds__(256) void k_gemm_p(GemmArgs g) {
     for (int t = 0; t < NT; t++)
 #pragma unroll
       for (int v = 0; v < 16; v++) acc[t][v] = 0.f;
+    // every A fragment of the 32-row strip is requested before the first MFMA
+    bf16x8 af[K / 16];
     const uint16_t* ap1 = g.a1 + ra * K1 + kk;
-#pragma unroll 2
-    for (int k = 0; k < K1; k += 16) {
-      const bf16x8 av = *(const bf16x8*)(ap1 + k);
 #pragma unroll
-      for (int t = 0; t < NT; t++) {
-        const bf16x8 bv = *(const bf16x8*)(w_lds + (t * 32 + (lane & 31)) * KP + kk + k);
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[t], 0, 0, 0);
-      }
-    }
+    for (int ks = 0; ks < K1 / 16; ks++) af[ks] = *(const bf16x8*)(ap1 + 16 * ks);
     if (K2 > 0) {
       const uint16_t* ap2 = g.a2 + ra * K2 + kk;
-#pragma unroll 2
-      for (int k = 0; k < K2; k += 16) {
-        const bf16x8 av = *(const bf16x8*)(ap2 + k);
 #pragma unroll
-        for (int t = 0; t < NT; t++) {
-          const bf16x8 bv = *(const bf16x8*)(w_lds + (t * 32 + (lane & 31)) * KP + kk + K1 + k);
-          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[t], 0, 0, 0);
-        }
+      for (int ks = 0; ks < K2 / 16; ks++) af[K1 / 16 + ks] = *(const bf16x8*)(ap2 + 16 * ks);
+    }
+#pragma unroll
+    for (int ks = 0; ks < K / 16; ks++) {
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        const bf16x8 bv = *(const bf16x8*)(w_lds + (t * 32 + (lane & 31)) * KP + kk + 16 * ks);
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[ks], bv, acc[t], 0, 0, 0);
       }
     }
 #pragma unroll
     for (int t = 0; t < NT; t++) {
-      const int c = t * 32 + (lane & 31);
+      const int c = cb + t * 32 + (lane & 31);
 #pragma unroll
       for (int v = 0; v < 16; v++) {
         const int64_t r = r0 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
@@ -419,16 +417,40 @@ __device__ inline void ld2(const uint16_t* p, float& a, float& b) {
   b = bf2f((uint16_t)(v >> 16));
 }
 
-// Message pass on MFMA: one wavefront per 32 consecutive particles (half of a 64-particle list
-// group). Step k takes neighbour rank k of each of the 32 particles, i.e. 32 consecutive list slots,
-// and computes D[channel][edge] = sum_f W_E[channel][f] e[edge][f] + (P_i + b_E)[channel] for the 128
-// channels with four 32x32x16 MFMAs (A = W_E rows, B = the edges' 16 inputs, C = P_i + b_E); then
-// z = D + Q_j, and relu(z) is summed over k in the accumulator layout: lane l holds edge column l%32
-// and, per 32-channel tile, the 4-channel groups 8g + 4*(l/32) .. +3.
+// Message pass on MFMA: two wavefronts per 32 consecutive particles (half of a 64-particle list
+// group), each for 64 of the 128 channels. Step k takes neighbour rank k of each of the 32 particles,
+// i.e. 32 consecutive list slots, and computes D[channel][edge] = sum_f W_E[channel][f] e[edge][f] +
+// (P_i + b_E)[channel] with two 32x32x16 MFMAs (A = W_E rows, B = the edges' 16 inputs, C = P_i +
+// b_E); then z = D + Q_j, and relu(z) is summed over k in the accumulator layout: lane l holds edge
+// column l%32 and, per 32-channel tile, the 4-channel groups 8g + 4*(l/32) .. +3. The list word, edge
+// inputs and Q rows of step k+1 are requested before step k is computed.
+struct MsgStep {
+  bool ok;
+  bf16x8 eb;
+  uint2 q[8];
+};
+__device__ inline void msg_load(const MsgArgs& a, uint64_t sl, bool ok, uint32_t wd, int half, int ch0, uint32_t self,
+                                MsgStep& st) {
+  st.ok = ok;
+  const uint32_t j = ok ? (wd & kIdxMask) : self;
+  if (ok) {
+    st.eb = *(const bf16x8*)(a.e + sl * FE + 8 * half);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; q++) st.eb[q] = (__bf16)0.0f;
+  }
+  const uint16_t* qrow = a.pq + (int64_t)j * H2 + H + ch0;
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int gq = 0; gq < 4; gq++) st.q[4 * t + gq] = *(const uint2*)(qrow + 32 * t + 8 * gq + 4 * half);
+}
+
 __global__ __launch_bounds__(256) void k_msg_fwd_mfma(MsgArgs a, const uint16_t* __restrict__ wEb) {
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t i0 = wave * 32;
+  const int64_t i0 = (wave >> 1) * 32;
+  const int ch0 = (int)(wave & 1) * 64;
   if (i0 >= a.n) return;
   const int col = lane & 31, half = lane >> 5;
   const int64_t i = i0 + col;
@@ -438,15 +460,15 @@ __global__ __launch_bounds__(256) void k_msg_fwd_mfma(MsgArgs a, const uint16_t*
   const uint64_t base = a.nl.base[grp] + (uint64_t)(ic & 63);
   const uint32_t cnt = vi ? a.nl.cnt[ic] : 0u;
   const uint32_t kmax = a.nl.kmax[grp];
-  bf16x8 wa[4];
+  bf16x8 wa[2];
 #pragma unroll
-  for (int t = 0; t < 4; t++) wa[t] = *(const bf16x8*)(wEb + (32 * t + col) * FE + 8 * half);
-  floatx16 cinit[4], msum[4];
+  for (int t = 0; t < 2; t++) wa[t] = *(const bf16x8*)(wEb + (ch0 + 32 * t + col) * FE + 8 * half);
+  floatx16 cinit[2], msum[2];
 #pragma unroll
-  for (int t = 0; t < 4; t++)
+  for (int t = 0; t < 2; t++)
 #pragma unroll
     for (int gq = 0; gq < 4; gq++) {
-      const int c0 = 32 * t + 8 * gq + 4 * half;
+      const int c0 = ch0 + 32 * t + 8 * gq + 4 * half;
       const uint2 pv = *(const uint2*)(a.pq + ic * H2 + c0);
       cinit[t][4 * gq + 0] = bf2f((uint16_t)(pv.x & 0xffffu)) + a.bE[c0 + 0];
       cinit[t][4 * gq + 1] = bf2f((uint16_t)(pv.x >> 16)) + a.bE[c0 + 1];
@@ -455,42 +477,41 @@ __global__ __launch_bounds__(256) void k_msg_fwd_mfma(MsgArgs a, const uint16_t*
 #pragma unroll
       for (int q = 0; q < 4; q++) msum[t][4 * gq + q] = 0.f;
     }
+  MsgStep cur, nxt;
+  uint32_t wnext = 0;
+  if (kmax > 0) {
+    const uint32_t w0 = cnt > 0 ? a.nl.word[base] : 0u;
+    msg_load(a, base, cnt > 0, w0, half, ch0, (uint32_t)ic, cur);
+    if (kmax > 1 && cnt > 1) wnext = a.nl.word[base + kWave];
+  }
   for (uint32_t k = 0; k < kmax; k++) {
-    const uint64_t sl = base + (uint64_t)k * kWave;
-    const bool ok = k < cnt;
-    uint32_t j = (uint32_t)ic;
-    bf16x8 eb;
-    if (ok) {
-      j = a.nl.word[sl] & kIdxMask;
-      eb = *(const bf16x8*)(a.e + sl * FE + 8 * half);
-    } else {
-#pragma unroll
-      for (int q = 0; q < 8; q++) eb[q] = (__bf16)0.0f;
+    if (k + 1 < kmax) {
+      const uint64_t sl1 = base + (uint64_t)(k + 1) * kWave;
+      msg_load(a, sl1, k + 1 < cnt, wnext, half, ch0, (uint32_t)ic, nxt);
+      wnext = (k + 2 < kmax && k + 2 < cnt) ? a.nl.word[sl1 + kWave] : 0u;
     }
-    const uint16_t* qrow = a.pq + (int64_t)j * H2 + H;
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-      const floatx16 d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[t], eb, cinit[t], 0, 0, 0);
+    for (int t = 0; t < 2; t++) {
+      const floatx16 d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[t], cur.eb, cinit[t], 0, 0, 0);
+      if (cur.ok) {
 #pragma unroll
-      for (int gq = 0; gq < 4; gq++) {
-        const uint2 qv = *(const uint2*)(qrow + 32 * t + 8 * gq + 4 * half);
-        const float q0 = bf2f((uint16_t)(qv.x & 0xffffu)), q1 = bf2f((uint16_t)(qv.x >> 16));
-        const float q2 = bf2f((uint16_t)(qv.y & 0xffffu)), q3 = bf2f((uint16_t)(qv.y >> 16));
-        if (ok) {
-          msum[t][4 * gq + 0] += fmaxf(d[4 * gq + 0] + q0, 0.f);
-          msum[t][4 * gq + 1] += fmaxf(d[4 * gq + 1] + q1, 0.f);
-          msum[t][4 * gq + 2] += fmaxf(d[4 * gq + 2] + q2, 0.f);
-          msum[t][4 * gq + 3] += fmaxf(d[4 * gq + 3] + q3, 0.f);
+        for (int gq = 0; gq < 4; gq++) {
+          const uint2 qv = cur.q[4 * t + gq];
+          msum[t][4 * gq + 0] += fmaxf(d[4 * gq + 0] + bf2f((uint16_t)(qv.x & 0xffffu)), 0.f);
+          msum[t][4 * gq + 1] += fmaxf(d[4 * gq + 1] + bf2f((uint16_t)(qv.x >> 16)), 0.f);
+          msum[t][4 * gq + 2] += fmaxf(d[4 * gq + 2] + bf2f((uint16_t)(qv.y & 0xffffu)), 0.f);
+          msum[t][4 * gq + 3] += fmaxf(d[4 * gq + 3] + bf2f((uint16_t)(qv.y >> 16)), 0.f);
         }
       }
     }
+    cur = nxt;
   }
   if (!vi) return;
 #pragma unroll
-  for (int t = 0; t < 4; t++)
+  for (int t = 0; t < 2; t++)
 #pragma unroll
     for (int gq = 0; gq < 4; gq++) {
-      const int c0 = 32 * t + 8 * gq + 4 * half;
+      const int c0 = ch0 + 32 * t + 8 * gq + 4 * half;
       uint2 o;
       o.x = (uint32_t)f2bf(msum[t][4 * gq + 0] * kMsgScale) | ((uint32_t)f2bf(msum[t][4 * gq + 1] * kMsgScale) << 16);
       o.y = (uint32_t)f2bf(msum[t][4 * gq + 2] * kMsgScale) | ((uint32_t)f2bf(msum[t][4 * gq + 3] * kMsgScale) << 16);
@@ -652,12 +673,12 @@ __global__ __launch_bounds__(256) void k_decode(DecArgs a) {
   }
 }
 
-// encoder, one thread per (particle, 8 channels)
+// encoder: one thread per particle for the 8 channels of chunk blockIdx.y, so that the weights of a
+// wavefront are uniform (scalar loads) and the node inputs of consecutive particles are contiguous
 __global__ __launch_bounds__(256) void k_encode8(EncArgs a) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t i = t >> 4;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
-  const int c0 = (int)(t & 15) * 8;
+  const int c0 = (int)blockIdx.y * 8;
   const uint4 xp = *(const uint4*)(a.x + i * FN);
   uint16_t xb[FN];
   memcpy(xb, &xp, 16);
@@ -677,29 +698,35 @@ __global__ __launch_bounds__(256) void k_encode8(EncArgs a) {
   *(uint4*)(a.h + i * H + c0) = op;
 }
 
-// decoder (forward only), one thread per particle, W_out staged in LDS
+// decoder (forward only): 16 lanes per particle, each with 8 channels (one coalesced 256-byte row per
+// 16 lanes), partial dot products reduced over the 16 lanes
 __global__ __launch_bounds__(256) void k_decode1(DecArgs a) {
   __shared__ float wo[FO * H];
   for (int k = threadIdx.x; k < FO * H; k += blockDim.x) wo[k] = a.w[k];
   __syncthreads();
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t >> 4;
+  const int c0 = (int)(t & 15) * 8;
   float y[FO];
 #pragma unroll
   for (int o = 0; o < FO; o++) y[o] = 0.f;
-  for (int c8 = 0; c8 < H; c8 += 8) {
-    const uint4 hp = *(const uint4*)(a.h + i * H + c8);
+  if (i < a.n) {
+    const uint4 hp = *(const uint4*)(a.h + i * H + c0);
     uint16_t hb[8];
     memcpy(hb, &hp, 16);
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       const float hv = bf2f(hb[q]);
 #pragma unroll
-      for (int o = 0; o < FO; o++) y[o] = fmaf(wo[o * H + c8 + q], hv, y[o]);
+      for (int o = 0; o < FO; o++) y[o] = fmaf(wo[o * H + c0 + q], hv, y[o]);
     }
   }
 #pragma unroll
-  for (int o = 0; o < FO; o++) a.out[i * FO + o] = y[o] + a.b[o];
+  for (int o = 0; o < FO; o++)
+    for (int m = 8; m > 0; m >>= 1) y[o] += __shfl_xor(y[o], m, 16);
+  if (i < a.n && (t & 15) == 0)
+#pragma unroll
+    for (int o = 0; o < FO; o++) a.out[i * FO + o] = y[o] + a.b[o];
 }
 
 // ------------------------------------------------------------------------------------- elementwise
@@ -1010,12 +1037,13 @@ int gemm_p(xdem_gnn* G, const GemmArgs& a) {
   xdem_ctx* c = G->c;
   if (a.n <= 0) return XDEM_OK;
   constexpr int K = K1 + K2;
-  const int lds = M * (K + 8) * (int)sizeof(uint16_t);
+  static_assert(M % 128 == 0, "column groups of 128");
+  const int lds = 128 * (K + 8) * (int)sizeof(uint16_t);
   XDEM_HIP(c, hipFuncSetAttribute((const void*)k_gemm_p<EPI, M, K1, K2>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   const int64_t ntiles = (a.n + 127) / 128;
-  const int grid = (int)std::min<int64_t>(ntiles, 512);
-  hipLaunchKernelGGL((k_gemm_p<EPI, M, K1, K2>), dim3(grid), dim3(256), lds, c->stream, a);
+  const int grid = (int)std::min<int64_t>(ntiles, 1024 / (M / 128));
+  hipLaunchKernelGGL((k_gemm_p<EPI, M, K1, K2>), dim3(grid, M / 128), dim3(256), lds, c->stream, a);
   XDEM_HIP(c, hipGetLastError());
   return XDEM_OK;
 }
@@ -1085,7 +1113,7 @@ int forward(xdem_gnn* G) {
   hipLaunchKernelGGL(k_features, dim3(nb(n, 256)), dim3(256), 0, c->stream, fa);
   if (tr) hipLaunchKernelGGL(k_mirror_all, dim3(nb(n, 256)), dim3(256), 0, c->stream, n, nl, G->mirror);
   EncArgs ea{n, G->x, G->wr + G->off.W_in, G->wr + G->off.b_in, G->h[0]};
-  hipLaunchKernelGGL(k_encode8, dim3(nb(n * 16, 256)), dim3(256), 0, c->stream, ea);
+  hipLaunchKernelGGL(k_encode8, dim3(nb(n, 256), H / 8), dim3(256), 0, c->stream, ea);
   XDEM_HIP(c, hipGetLastError());
   for (int l = 0; l < L; l++) {
     uint16_t* hin = tr ? G->h[l] : G->h[l & 1];
@@ -1099,7 +1127,7 @@ int forward(xdem_gnn* G) {
     MsgArgs ma{};
     ma.n = n; ma.nl = nl; ma.e = G->e; ma.pq = pq;
     ma.wE = G->wr + G->off.W_E[l]; ma.bE = G->wr + G->off.b_E[l]; ma.m = mm;
-    hipLaunchKernelGGL(k_msg_fwd_mfma, dim3(nb((n + 31) / 32 * 64, 256)), dim3(256), 0, c->stream, ma,
+    hipLaunchKernelGGL(k_msg_fwd_mfma, dim3(nb((n + 31) / 32 * 128, 256)), dim3(256), 0, c->stream, ma,
                        G->wbf + G->off.W_E[l]);
     GemmArgs g2{};
     g2.n = n; g2.a1 = hin; g2.k1 = H; g2.a2 = mm; g2.k2 = H;
@@ -1109,7 +1137,7 @@ int forward(xdem_gnn* G) {
   }
   DecArgs da{n, tr ? G->h[L] : G->h[L & 1], G->wr + G->off.W_out, G->wr + G->off.b_out, G->out, nullptr, nullptr,
              nullptr};
-  hipLaunchKernelGGL(k_decode1, dim3(nb(n, 256)), dim3(256), 0, c->stream, da);
+  hipLaunchKernelGGL(k_decode1, dim3(nb(n * 16, 256)), dim3(256), 0, c->stream, da);
   XDEM_HIP(c, hipGetLastError());
   G->fwd_valid = true;
   G->fwd_n = n;

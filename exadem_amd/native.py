@@ -127,8 +127,13 @@ def particles_struct(arrays, n):
 _lib = None
 
 
+def lib_dir():
+    """Directory of the built libraries (XDEM_LIB_DIR overrides the in-tree exadem_amd/lib)."""
+    return os.environ.get("XDEM_LIB_DIR", LIBDIR)
+
+
 def lib_path():
-    return os.path.join(LIBDIR, "libxdem_hip.so")
+    return os.path.join(lib_dir(), "libxdem_hip.so")
 
 
 def load():
@@ -524,7 +529,7 @@ def load_host():
     if _host is not None:
         return _host
     load()
-    path = os.path.join(LIBDIR, "libxdem_host.so")
+    path = os.path.join(lib_dir(), "libxdem_host.so")
     if not os.path.exists(path):
         raise XdemError(f"{path} is missing: build it with exadem_amd/build.py")
     h = C.CDLL(path)

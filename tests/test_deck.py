@@ -68,7 +68,7 @@ def wall_deck(tmp_path, w, steps):
     with open(tmp_path / "particles.xyz", "w") as f:
         f.write(f"{w.n}\nwalled deposition\n")
         for k in range(w.n):
-            f.write(f"Sphere1 {p['rx'][k]!r} {p['ry'][k]!r} {p['rz'][k]!r}\n")
+            f.write(f"Sphere1 {float(p['rx'][k])!r} {float(p['ry'][k])!r} {float(p['rz'][k])!r}\n")
     lo, hi = list(w.domain.lo), list(w.domain.hi)
     walls = "\n".join(
         f"  - register_surface: {{ id: {k}, state: {{ normal: [{wl['normal'][0]}, {wl['normal'][1]}, "
