@@ -273,7 +273,7 @@ def main():
                    "list_slots": st_nb2["slots"], "max_initial_overlap": w.meta["max_overlap"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": "k_force_fp64 (fused force pass)", "avg_ms": f_ms, "launches": f_cnt,
+                     "kernel": f"k_force_{args.form} (fused force pass)", "avg_ms": f_ms, "launches": f_cnt,
                      "bytes_per_launch": force_bytes, "integrate_avg_ms": i_ms,
                      "step_bytes": step_bytes, "step_frac": step_bytes / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS},
         "cpu_baseline": cpu,

@@ -70,13 +70,21 @@ struct Entry {
 };
 
 template <bool KR>
-__device__ inline void fetch_entry(const ForceArgs& a, uint64_t sl, uint32_t wd, Entry& e) {
+__device__ inline void fetch_entry(const ForceArgs& a, uint64_t sl, uint32_t wd, Entry& e, const Core* tile,
+                                   int64_t b0) {
   const Core* __restrict__ core = a.s.core;
   const uint32_t y = wd & kIdxMask;
   e.wd = wd;
-  e.p = core[y].xr;
-  e.v = core[y].vm;
-  e.w = core[y].w;
+  const uint64_t rel = (uint64_t)((int64_t)y - b0);
+  if (rel < 256) {
+    e.p = tile[rel].xr;
+    e.v = tile[rel].vm;
+    e.w = tile[rel].w;
+  } else {
+    e.p = core[y].xr;
+    e.v = core[y].vm;
+    e.w = core[y].w;
+  }
   if (wd & kActiveBit) {
     e.ft = {a.nl.hx[sl], a.nl.hy[sl], a.nl.hz[sl]};
     if (KR) e.mint = {a.nl.mhx[sl], a.nl.mhy[sl], a.nl.mhz[sl]};
@@ -100,7 +108,16 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
     }
   }
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t x = blk * blockDim.x + threadIdx.x;
+  const int64_t b0 = blk * blockDim.x;
+  // the block's own 256 records (32 KB, coalesced 16-byte copies): partners inside the block are
+  // read from LDS, the others from global memory
+  __shared__ Core tile[256];
+  for (int k = threadIdx.x; k < 256 * 8; k += 256) {
+    const int rec = k >> 3, part = k & 7;
+    if (b0 + rec < a.n) ((uint4*)&tile[rec])[part] = ((const uint4*)&a.s.core[b0 + rec])[part];
+  }
+  __syncthreads();
+  const int64_t x = b0 + threadIdx.x;
   if (x >= a.n) return;
   const PState& s = a.s;
   uint32_t* __restrict__ word = a.nl.word;
@@ -109,9 +126,9 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
   // software pipeline: entry k+1 (record + history) and word k+2 are in flight while k is computed
   Entry nxt;
   uint32_t wd2 = 0;
-  if (cnt > 0) fetch_entry<KR>(a, b, word[b], nxt);
+  if (cnt > 0) fetch_entry<KR>(a, b, word[b], nxt, tile, b0);
   if (cnt > 1) wd2 = word[b + kWave];
-  const Core me = s.core[x];
+  const Core me = tile[threadIdx.x];
   const V3 rx{me.xr.x, me.xr.y, me.xr.z};
   const V3 vx{me.vm.x, me.vm.y, me.vm.z};
   const V3 wx{me.w.x, me.w.y, me.w.z};
@@ -129,7 +146,7 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
     const uint64_t sl = b + (uint64_t)k * kWave;
     Entry e = nxt;
     if (k + 1 < cnt) {
-      fetch_entry<KR>(a, sl + kWave, wd2, nxt);
+      fetch_entry<KR>(a, sl + kWave, wd2, nxt, tile, b0);
       if (k + 2 < cnt) wd2 = word[sl + 2 * kWave];
     }
     const uint32_t wd = e.wd;

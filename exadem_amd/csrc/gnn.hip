@@ -673,26 +673,34 @@ __global__ __launch_bounds__(256) void k_decode(DecArgs a) {
   }
 }
 
-// encoder: one thread per particle for the 8 channels of chunk blockIdx.y, so that the weights of a
-// wavefront are uniform (scalar loads) and the node inputs of consecutive particles are contiguous
+// encoder: 16 lanes per particle, 8 channels each, so that a wavefront stores four whole 256-byte
+// rows; the weights sit transposed in LDS ([input][channel]) so that the 16 lanes of a particle read
+// 128 consecutive floats
 __global__ __launch_bounds__(256) void k_encode8(EncArgs a) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ float wt[FN * H];
+  __shared__ float bb[H];
+  for (int k = threadIdx.x; k < FN * H; k += blockDim.x) wt[(k % FN) * H + k / FN] = a.w[k];
+  for (int k = threadIdx.x; k < H; k += blockDim.x) bb[k] = a.b[k];
+  __syncthreads();
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t >> 4;
   if (i >= a.n) return;
-  const int c0 = (int)blockIdx.y * 8;
+  const int c0 = (int)(t & 15) * 8;
   const uint4 xp = *(const uint4*)(a.x + i * FN);
   uint16_t xb[FN];
   memcpy(xb, &xp, 16);
-  float xv[FN];
+  float s[8];
 #pragma unroll
-  for (int k = 0; k < FN; k++) xv[k] = bf2f(xb[k]);
+  for (int q = 0; q < 8; q++) s[q] = 0.f;
+#pragma unroll
+  for (int k = 0; k < FN; k++) {
+    const float xv = bf2f(xb[k]);
+#pragma unroll
+    for (int q = 0; q < 8; q++) s[q] = fmaf(wt[k * H + c0 + q], xv, s[q]);
+  }
   uint16_t o[8];
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
-    float s = 0.f;
-#pragma unroll
-    for (int k = 0; k < FN; k++) s = fmaf(a.w[(c0 + q) * FN + k], xv[k], s);
-    o[q] = f2bf(fmaxf(s + a.b[c0 + q], 0.f));
-  }
+  for (int q = 0; q < 8; q++) o[q] = f2bf(fmaxf(s[q] + bb[c0 + q], 0.f));
   uint4 op;
   memcpy(&op, o, 16);
   *(uint4*)(a.h + i * H + c0) = op;
@@ -1113,7 +1121,7 @@ int forward(xdem_gnn* G) {
   hipLaunchKernelGGL(k_features, dim3(nb(n, 256)), dim3(256), 0, c->stream, fa);
   if (tr) hipLaunchKernelGGL(k_mirror_all, dim3(nb(n, 256)), dim3(256), 0, c->stream, n, nl, G->mirror);
   EncArgs ea{n, G->x, G->wr + G->off.W_in, G->wr + G->off.b_in, G->h[0]};
-  hipLaunchKernelGGL(k_encode8, dim3(nb(n, 256), H / 8), dim3(256), 0, c->stream, ea);
+  hipLaunchKernelGGL(k_encode8, dim3(nb(n * 16, 256)), dim3(256), 0, c->stream, ea);
   XDEM_HIP(c, hipGetLastError());
   for (int l = 0; l < L; l++) {
     uint16_t* hin = tr ? G->h[l] : G->h[l & 1];

@@ -122,15 +122,19 @@ __device__ inline V3 get_vertex(const xdem_shape& sh, int k, V3 p, double h, con
 }
 __device__ inline V3 vnormalize(V3 v) { return v / norm(v); }
 
+// Vertices of one particle read in place from the world-vertex array (plus a periodic image shift),
+// so that no per-lane vertex array is built (dynamically indexed arrays live in scratch memory).
 struct Verts {
-  V3 v[XDEM_MAX_SHAPE_VERTICES];
+  const d4* __restrict__ p;
+  V3 sh;
+  __device__ V3 operator[](int k) const {
+    const d4 a = p[k];
+    return {a.x + sh.x, a.y + sh.y, a.z + sh.z};
+  }
 };
 
-__device__ inline void load_verts(const d4* __restrict__ vtx, int64_t p, int nv, V3 sh, Verts& out) {
-  for (int k = 0; k < nv; k++) {
-    const d4 a = vtx[p * XDEM_MAX_SHAPE_VERTICES + k];
-    out.v[k] = {a.x + sh.x, a.y + sh.y, a.z + sh.z};
-  }
+__device__ inline Verts verts_of(const d4* __restrict__ vtx, int64_t p, V3 sh) {
+  return Verts{vtx + p * XDEM_MAX_SHAPE_VERTICES, sh};
 }
 
 __device__ inline bool filter_vv(double rv, V3 vi, double ri, V3 vj, double rj) {
@@ -152,9 +156,9 @@ __device__ inline bool vf_core(V3 vi, const Verts& vj, const xdem_shape& shj, in
                                V3& n_out, bool& early) {
   const int nf = shj.face_size[face];
   const int* f = shj.faces[face];
-  const V3 va = vj.v[f[0]];
-  const V3 vb = vj.v[f[1]];
-  const V3 vc = vj.v[f[nf - 1]];
+  const V3 va = vj[f[0]];
+  const V3 vb = vj[f[1]];
+  const V3 vc = vj[f[nf - 1]];
   const V3 v = vi - va;
   V3 v1 = vb - va;
   V3 v2 = vc - va;
@@ -178,7 +182,7 @@ __device__ inline bool vf_core(V3 vi, const Verts& vj, const xdem_shape& shj, in
   for (int iva = 0; iva < nf; ++iva) {
     int ivb = iva + 1;
     if (ivb == nf) ivb = 0;
-    const V3 A = vj.v[f[iva]], B = vj.v[f[ivb]];
+    const V3 A = vj[f[iva]], B = vj[f[ivb]];
     const double pa1 = dot(A, v1), pb1 = dot(B, v1), pa2 = dot(A, v2), pb2 = dot(B, v2);
     if ((pa2 < ori2 && pb2 >= ori2) || (pb2 < ori2 && pa2 >= ori2)) {
       if (pa1 + (ori2 - pa2) / (pb2 - pa2) * (pb1 - pa1) < ori1) odd = 1 - odd;
@@ -327,13 +331,12 @@ __device__ inline void for_each_feature(const PolyNbhArgs& a, int64_t x, uint32_
   if (!obb_intersect(obb_i, obb_j)) return;
   obb_enlarge(obb_i, 0.5 * a.rv);
   obb_enlarge(obb_j, 0.5 * a.rv);
-  Verts va, vb;
-  load_verts(a.vtx, x, sa.nv, V3{0, 0, 0}, va);
-  load_verts(a.vtx, y, sb.nv, sh, vb);
+  const Verts va = verts_of(a.vtx, x, V3{0, 0, 0});
+  const Verts vb = verts_of(a.vtx, y, sh);
   const double rv = a.rv;
   for (int i = 0; i < sa.nv; i++) {
     OBBd ov;
-    ov.c = va.v[i];
+    ov.c = va[i];
     ov.e1 = {1, 0, 0};
     ov.e2 = {0, 1, 0};
     ov.e3 = {0, 0, 1};
@@ -341,24 +344,24 @@ __device__ inline void for_each_feature(const PolyNbhArgs& a, int64_t x, uint32_
     obb_enlarge(ov, ra);
     if (!obb_intersect(obb_j, ov)) continue;
     for (int j = 0; j < sb.nv; j++)
-      if (filter_vv(rv, va.v[i], ra, vb.v[j], rb)) fn(0, 0, i, j);
+      if (filter_vv(rv, va[i], ra, vb[j], rb)) fn(0, 0, i, j);
     for (int j = 0; j < sb.ne; j++)
-      if (filter_ve(rv, va.v[i], ra, vb.v[sb.edges[j][0]], vb.v[sb.edges[j][1]], rb)) fn(1, 0, i, j);
+      if (filter_ve(rv, va[i], ra, vb[sb.edges[j][0]], vb[sb.edges[j][1]], rb)) fn(1, 0, i, j);
     for (int j = 0; j < sb.nf; j++) {
       double dist;
       V3 nn;
       bool early;
-      if (vf_core(va.v[i], vb, sb, j, ra + rb + rv, dist, nn, early)) fn(2, 0, i, j);
+      if (vf_core(va[i], vb, sb, j, ra + rb + rv, dist, nn, early)) fn(2, 0, i, j);
     }
   }
   for (int i = 0; i < sa.ne; i++)
     for (int j = 0; j < sb.ne; j++)
-      if (filter_ee(rv, va.v[sa.edges[i][0]], va.v[sa.edges[i][1]], ra, vb.v[sb.edges[j][0]], vb.v[sb.edges[j][1]],
+      if (filter_ee(rv, va[sa.edges[i][0]], va[sa.edges[i][1]], ra, vb[sb.edges[j][0]], vb[sb.edges[j][1]],
                     rb))
         fn(3, 0, i, j);
   for (int j = 0; j < sb.nv; j++) {
     OBBd ov;
-    ov.c = vb.v[j];
+    ov.c = vb[j];
     ov.e1 = {1, 0, 0};
     ov.e2 = {0, 1, 0};
     ov.e3 = {0, 0, 1};
@@ -366,12 +369,12 @@ __device__ inline void for_each_feature(const PolyNbhArgs& a, int64_t x, uint32_
     obb_enlarge(ov, rb);
     if (!obb_intersect(obb_i, ov)) continue;
     for (int i = 0; i < sa.ne; i++)
-      if (filter_ve(rv, vb.v[j], rb, va.v[sa.edges[i][0]], va.v[sa.edges[i][1]], ra)) fn(1, 1, i, j);
+      if (filter_ve(rv, vb[j], rb, va[sa.edges[i][0]], va[sa.edges[i][1]], ra)) fn(1, 1, i, j);
     for (int i = 0; i < sa.nf; i++) {
       double dist;
       V3 nn;
       bool early;
-      if (vf_core(vb.v[j], va, sa, i, rb + ra + rv, dist, nn, early)) fn(2, 1, i, j);
+      if (vf_core(vb[j], va, sa, i, rb + ra + rv, dist, nn, early)) fn(2, 1, i, j);
     }
   }
 }
@@ -662,8 +665,7 @@ __global__ __launch_bounds__(128) void k_poly_pairs(PolyForceArgs a) {
   const Core cx = s.core[x];
   const xdem_shape& sa = a.shapes[s.type[x]];
   const double ra = sa.minkowski * cx.w.w;
-  Verts va;
-  load_verts(a.vtx, x, sa.nv, V3{0, 0, 0}, va);
+  const Verts va = verts_of(a.vtx, x, V3{0, 0, 0});
   const V3 rx{cx.xr.x, cx.xr.y, cx.xr.z}, vx{cx.vm.x, cx.vm.y, cx.vm.z}, wx{cx.w.x, cx.w.y, cx.w.z};
   for (uint32_t k = 0; k < cnt; k++) {
     const uint64_t sl = b + (uint64_t)k * kWave;
@@ -680,8 +682,7 @@ __global__ __launch_bounds__(128) void k_poly_pairs(PolyForceArgs a) {
     V3 Fo{0, 0, 0}, Mo{0, 0, 0}, Mp{0, 0, 0};
     const uint64_t f0 = a.pl.start[sl], f1 = a.pl.start[sl + 1];
     if (f1 > f0) {
-      Verts vb;
-      load_verts(a.vtx, y, sb.nv, sh, vb);
+      const Verts vb = verts_of(a.vtx, y, sh);
       for (uint64_t f = f0; f < f1; f++) {
         const uint32_t fw = a.pl.feat[f];
         const int type = (int)(fw & kFeatTypeMask);
@@ -695,11 +696,11 @@ __global__ __launch_bounds__(128) void k_poly_pairs(PolyForceArgs a) {
         const double ri = swap ? rb : ra, rj = swap ? ra : rb;
         const int fi = swap ? psub : osub, fj = swap ? osub : psub;
         Det d;
-        if (type == 0) d = detect_vv(vi.v[fi], ri, vj.v[fj], rj);
-        else if (type == 1) d = detect_ve(vi.v[fi], ri, vj.v[sj.edges[fj][0]], vj.v[sj.edges[fj][1]], rj);
-        else if (type == 2) d = detect_vf(vi.v[fi], ri, vj, sj, fj, rj);
-        else d = detect_ee(vi.v[si.edges[fi][0]], vi.v[si.edges[fi][1]], ri, vj.v[sj.edges[fj][0]],
-                           vj.v[sj.edges[fj][1]], rj);
+        if (type == 0) d = detect_vv(vi[fi], ri, vj[fj], rj);
+        else if (type == 1) d = detect_ve(vi[fi], ri, vj[sj.edges[fj][0]], vj[sj.edges[fj][1]], rj);
+        else if (type == 2) d = detect_vf(vi[fi], ri, vj, sj, fj, rj);
+        else d = detect_ee(vi[si.edges[fi][0]], vi[si.edges[fi][1]], ri, vj[sj.edges[fj][0]],
+                           vj[sj.edges[fj][1]], rj);
         if (d.contact) {
           const V3 r_i = swap ? ry : rx, r_j = swap ? rx : ry;
           const V3 v_i = swap ? vy : vx, v_j = swap ? vx : vy;
