@@ -127,13 +127,21 @@ struct Entry32 {
 };
 
 template <bool KR>
-__device__ inline void fetch32(const ForceArgs32& a, uint64_t sl, uint32_t wd, Entry32& e) {
+__device__ inline void fetch32(const ForceArgs32& a, uint64_t sl, uint32_t wd, Entry32& e, const Core* tile,
+                               int64_t b0) {
   const Core* __restrict__ core = a.s.core;
   const uint32_t y = wd & kIdxMask;
   e.wd = wd;
-  e.p = core[y].xr;
-  e.v = core[y].vm;
-  e.w = core[y].w;
+  const uint64_t rel = (uint64_t)((int64_t)y - b0);
+  if (rel < 256) {
+    e.p = tile[rel].xr;
+    e.v = tile[rel].vm;
+    e.w = tile[rel].w;
+  } else {
+    e.p = core[y].xr;
+    e.v = core[y].vm;
+    e.w = core[y].w;
+  }
   if (wd & kActiveBit) {
     e.ft = {a.nl.fx32[sl], a.nl.fy32[sl], a.nl.fz32[sl]};
     if (KR) e.mint = {a.nl.mfx32[sl], a.nl.mfy32[sl], a.nl.mfz32[sl]};
@@ -157,7 +165,15 @@ __global__ __launch_bounds__(256) void k_force_fp32(ForceArgs32 a) {
     }
   }
   const int64_t blk = xcd_block32(blockIdx.x, gridDim.x);
-  const int64_t x = blk * blockDim.x + threadIdx.x;
+  const int64_t b0 = blk * blockDim.x;
+  // the block's own 256 records in LDS: partners inside the block are read from there
+  __shared__ Core tile[256];
+  for (int k = threadIdx.x; k < 256 * 8; k += 256) {
+    const int rec = k >> 3, part = k & 7;
+    if (b0 + rec < a.n) ((uint4*)&tile[rec])[part] = ((const uint4*)&a.s.core[b0 + rec])[part];
+  }
+  __syncthreads();
+  const int64_t x = b0 + threadIdx.x;
   if (x >= a.n) return;
   const PState& s = a.s;
   uint32_t* __restrict__ word = a.nl.word;
@@ -165,9 +181,9 @@ __global__ __launch_bounds__(256) void k_force_fp32(ForceArgs32 a) {
   const uint64_t b = a.nl.base[x >> 6] + (uint64_t)(x & 63);
   Entry32 nxt;
   uint32_t wd2 = 0;
-  if (cnt > 0) fetch32<KR>(a, b, word[b], nxt);
+  if (cnt > 0) fetch32<KR>(a, b, word[b], nxt, tile, b0);
   if (cnt > 1) wd2 = word[b + kWave];
-  const Core me = s.core[x];
+  const Core me = tile[threadIdx.x];
   const V3 rx{me.xr.x, me.xr.y, me.xr.z};
   const double Rx = me.xr.w, mx = me.vm.w;
   const F3 vx = to_f3(me.vm.x, me.vm.y, me.vm.z);
@@ -178,7 +194,7 @@ __global__ __launch_bounds__(256) void k_force_fp32(ForceArgs32 a) {
     const uint64_t sl = b + (uint64_t)k * kWave;
     Entry32 e = nxt;
     if (k + 1 < cnt) {
-      fetch32<KR>(a, sl + kWave, wd2, nxt);
+      fetch32<KR>(a, sl + kWave, wd2, nxt, tile, b0);
       if (k + 2 < cnt) wd2 = word[sl + 2 * kWave];
     }
     const uint32_t wd = e.wd;

@@ -682,28 +682,28 @@ __global__ __launch_bounds__(256) void k_encode8(EncArgs a) {
   for (int k = threadIdx.x; k < FN * H; k += blockDim.x) wt[(k % FN) * H + k / FN] = a.w[k];
   for (int k = threadIdx.x; k < H; k += blockDim.x) bb[k] = a.b[k];
   __syncthreads();
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t i = t >> 4;
-  if (i >= a.n) return;
-  const int c0 = (int)(t & 15) * 8;
-  const uint4 xp = *(const uint4*)(a.x + i * FN);
-  uint16_t xb[FN];
-  memcpy(xb, &xp, 16);
-  float s[8];
+  const int c0 = (int)(threadIdx.x & 15) * 8;
+  const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 4);
+  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; i < a.n; i += stride) {
+    const uint4 xp = *(const uint4*)(a.x + i * FN);
+    uint16_t xb[FN];
+    memcpy(xb, &xp, 16);
+    float s[8];
 #pragma unroll
-  for (int q = 0; q < 8; q++) s[q] = 0.f;
+    for (int q = 0; q < 8; q++) s[q] = 0.f;
 #pragma unroll
-  for (int k = 0; k < FN; k++) {
-    const float xv = bf2f(xb[k]);
+    for (int k = 0; k < FN; k++) {
+      const float xv = bf2f(xb[k]);
 #pragma unroll
-    for (int q = 0; q < 8; q++) s[q] = fmaf(wt[k * H + c0 + q], xv, s[q]);
+      for (int q = 0; q < 8; q++) s[q] = fmaf(wt[k * H + c0 + q], xv, s[q]);
+    }
+    uint16_t o[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) o[q] = f2bf(fmaxf(s[q] + bb[c0 + q], 0.f));
+    uint4 op;
+    memcpy(&op, o, 16);
+    *(uint4*)(a.h + i * H + c0) = op;
   }
-  uint16_t o[8];
-#pragma unroll
-  for (int q = 0; q < 8; q++) o[q] = f2bf(fmaxf(s[q] + bb[c0 + q], 0.f));
-  uint4 op;
-  memcpy(&op, o, 16);
-  *(uint4*)(a.h + i * H + c0) = op;
 }
 
 // decoder (forward only): 16 lanes per particle, each with 8 channels (one coalesced 256-byte row per
@@ -712,29 +712,31 @@ __global__ __launch_bounds__(256) void k_decode1(DecArgs a) {
   __shared__ float wo[FO * H];
   for (int k = threadIdx.x; k < FO * H; k += blockDim.x) wo[k] = a.w[k];
   __syncthreads();
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t i = t >> 4;
-  const int c0 = (int)(t & 15) * 8;
-  float y[FO];
+  const int c0 = (int)(threadIdx.x & 15) * 8;
+  const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 4);
+  const int64_t nr = (a.n + stride - 1) / stride * stride;  // every lane runs the same trip count
+  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; i < nr; i += stride) {
+    float y[FO];
 #pragma unroll
-  for (int o = 0; o < FO; o++) y[o] = 0.f;
-  if (i < a.n) {
-    const uint4 hp = *(const uint4*)(a.h + i * H + c0);
-    uint16_t hb[8];
-    memcpy(hb, &hp, 16);
+    for (int o = 0; o < FO; o++) y[o] = 0.f;
+    if (i < a.n) {
+      const uint4 hp = *(const uint4*)(a.h + i * H + c0);
+      uint16_t hb[8];
+      memcpy(hb, &hp, 16);
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const float hv = bf2f(hb[q]);
+      for (int q = 0; q < 8; q++) {
+        const float hv = bf2f(hb[q]);
 #pragma unroll
-      for (int o = 0; o < FO; o++) y[o] = fmaf(wo[o * H + c0 + q], hv, y[o]);
+        for (int o = 0; o < FO; o++) y[o] = fmaf(wo[o * H + c0 + q], hv, y[o]);
+      }
     }
+#pragma unroll
+    for (int o = 0; o < FO; o++)
+      for (int m = 8; m > 0; m >>= 1) y[o] += __shfl_xor(y[o], m, 16);
+    if (i < a.n && (threadIdx.x & 15) == 0)
+#pragma unroll
+      for (int o = 0; o < FO; o++) a.out[i * FO + o] = y[o] + a.b[o];
   }
-#pragma unroll
-  for (int o = 0; o < FO; o++)
-    for (int m = 8; m > 0; m >>= 1) y[o] += __shfl_xor(y[o], m, 16);
-  if (i < a.n && (t & 15) == 0)
-#pragma unroll
-    for (int o = 0; o < FO; o++) a.out[i * FO + o] = y[o] + a.b[o];
 }
 
 // ------------------------------------------------------------------------------------- elementwise
@@ -1121,7 +1123,7 @@ int forward(xdem_gnn* G) {
   hipLaunchKernelGGL(k_features, dim3(nb(n, 256)), dim3(256), 0, c->stream, fa);
   if (tr) hipLaunchKernelGGL(k_mirror_all, dim3(nb(n, 256)), dim3(256), 0, c->stream, n, nl, G->mirror);
   EncArgs ea{n, G->x, G->wr + G->off.W_in, G->wr + G->off.b_in, G->h[0]};
-  hipLaunchKernelGGL(k_encode8, dim3(nb(n * 16, 256)), dim3(256), 0, c->stream, ea);
+  hipLaunchKernelGGL(k_encode8, dim3((int)std::min<int64_t>(nb(n * 16, 256), 2048)), dim3(256), 0, c->stream, ea);
   XDEM_HIP(c, hipGetLastError());
   for (int l = 0; l < L; l++) {
     uint16_t* hin = tr ? G->h[l] : G->h[l & 1];
@@ -1145,7 +1147,7 @@ int forward(xdem_gnn* G) {
   }
   DecArgs da{n, tr ? G->h[L] : G->h[L & 1], G->wr + G->off.W_out, G->wr + G->off.b_out, G->out, nullptr, nullptr,
              nullptr};
-  hipLaunchKernelGGL(k_decode1, dim3(nb(n * 16, 256)), dim3(256), 0, c->stream, da);
+  hipLaunchKernelGGL(k_decode1, dim3((int)std::min<int64_t>(nb(n * 16, 256), 2048)), dim3(256), 0, c->stream, da);
   XDEM_HIP(c, hipGetLastError());
   G->fwd_valid = true;
   G->fwd_n = n;
