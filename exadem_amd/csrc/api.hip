@@ -333,11 +333,11 @@ void xdem_ctx_destroy(xdem_ctx* c) {
   dfree(&c->perm_tmp);
   dfree(&c->sort_key);
   dfree(&c->d_flags); dfree(&c->d_u32tmp); dfree(&c->d_walls);
-  dfree(&c->d_shapes); dfree(&c->vtx); dfree(&c->mirror); dfree(&c->pair_buf);
+  dfree(&c->d_shapes); dfree(&c->vtx); dfree(&c->mirror); dfree(&c->fcontrib); dfree(&c->fcontact);
   for (int k = 0; k < 2; k++) {
     PolyList& P = c->pl[k];
     dfree(&P.start); dfree(&P.feat); dfree(&P.hx); dfree(&P.hy); dfree(&P.hz);
-    dfree(&P.mhx); dfree(&P.mhy); dfree(&P.mhz); dfree(&P.fcount);
+    dfree(&P.mhx); dfree(&P.mhy); dfree(&P.mhz); dfree(&P.fcount); dfree(&P.fslot); dfree(&P.fown);
     PolyWall& W = c->pw[k];
     dfree(&W.mask); dfree(&W.hx); dfree(&W.hy); dfree(&W.hz); dfree(&W.mhx); dfree(&W.mhy); dfree(&W.mhz);
   }

@@ -11,11 +11,11 @@
 //   detection  shape_detection_core.hpp:57-81, 122-148, 397-466, 520-566
 //   contact    src/polyhedron/include/exaDEM/polyhedron/contact.hpp:124-211 (symmetric), 219-297 (walls)
 //
-// Contact pass without atomics: one thread per particle evaluates every feature of the pairs it owns,
-// in list order, and writes per-pair subtotals (force on the owner, moment on the owner, moment on the
-// partner; the partner's force is the exact negation). A second kernel sums, per particle and in
-// stream order, the subtotals of its pairs (own slots, or the owner's slot through `mirror`), then the
-// vertex-wall contacts, then finishes force_to_accel and the angular acceleration.
+// Contact pass without atomics: one thread per feature evaluates it and writes its terms of the owner
+// force, owner moment and partner moment (the partner's force term is the exact negation). A second
+// kernel walks, per particle and in stream order, its entries (own slots, or the owner's slot through
+// `mirror`), forms each pair's subtotal over the features in list order, adds it, then the vertex-wall
+// contacts, then finishes force_to_accel and the angular acceleration.
 #include <algorithm>
 
 #include "contact_law.h"
@@ -395,7 +395,8 @@ __global__ __launch_bounds__(256) void k_feat_count(PolyNbhArgs a, uint32_t* __r
 }
 
 __global__ __launch_bounds__(256) void k_feat_fill(PolyNbhArgs a, const uint64_t* __restrict__ start,
-                                                   uint32_t* __restrict__ feat) {
+                                                   uint32_t* __restrict__ feat, uint32_t* __restrict__ fslot,
+                                                   uint32_t* __restrict__ fown) {
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= a.n) return;
   const uint32_t cnt = a.nl.cnt[x];
@@ -407,7 +408,10 @@ __global__ __launch_bounds__(256) void k_feat_fill(PolyNbhArgs a, const uint64_t
     uint64_t o = start[sl];
     for_each_feature(a, x, wd & kIdxMask, (int)((wd >> kShiftShift) & 31u), [&](int t, int sw, int si, int sj) {
       // keys use owner().sub_, partner().sub_: the swap pass stores the vertex of the partner as partner sub
-      feat[o++] = sw ? feat_word(t, 1, si, sj) : feat_word(t, 0, si, sj);
+      feat[o] = sw ? feat_word(t, 1, si, sj) : feat_word(t, 0, si, sj);
+      fslot[o] = (uint32_t)sl;
+      fown[o] = (uint32_t)x;
+      o++;
     });
   }
 }
@@ -558,9 +562,8 @@ int poly_nbh(xdem_ctx* c, double rcut_inc) {
   }
   if (L.slots > c->cap_mirror) {
     const int64_t cap = (int64_t)(L.slots * 1.25) + 4096;
-    if ((rc = galloc(c, &c->mirror, cap)) || (rc = galloc(c, &c->pair_buf, cap * 9))) return rc;
+    if ((rc = galloc(c, &c->mirror, cap))) return rc;
     c->cap_mirror = cap;
-    c->cap_pair = cap;
   }
   XDEM_HIP(c, hipMemsetAsync(P.fcount, 0, sizeof(uint32_t) * std::max<int64_t>(L.slots, 1), c->stream));
   PolyNbhArgs a{n, live(c), c->grid, c->d_shapes, c->vtx, L, rcut_inc};
@@ -574,13 +577,19 @@ int poly_nbh(xdem_ctx* c, double rcut_inc) {
     const int64_t cap = (int64_t)(nf * 1.25) + 4096;
     if ((rc = galloc(c, &P.feat, cap)) || (rc = galloc(c, &P.hx, cap)) || (rc = galloc(c, &P.hy, cap)) ||
         (rc = galloc(c, &P.hz, cap)) || (rc = galloc(c, &P.mhx, cap)) || (rc = galloc(c, &P.mhy, cap)) ||
-        (rc = galloc(c, &P.mhz, cap)))
+        (rc = galloc(c, &P.mhz, cap)) || (rc = galloc(c, &P.fslot, cap)) || (rc = galloc(c, &P.fown, cap)))
       return rc;
     P.cap_feat = cap;
   }
+  if ((int64_t)nf > c->cap_contrib) {
+    const int64_t cap = (int64_t)(nf * 1.25) + 4096;
+    if ((rc = galloc(c, &c->fcontrib, cap * 9)) || (rc = galloc(c, &c->fcontact, cap))) return rc;
+    c->cap_contrib = cap;
+  }
   P.nfeat = (int64_t)nf;
   if (n > 0) {
-    hipLaunchKernelGGL(k_feat_fill, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a, P.start, P.feat);
+    hipLaunchKernelGGL(k_feat_fill, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a, P.start, P.feat, P.fslot,
+                       P.fown);
     hipLaunchKernelGGL(k_mirror, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, L, c->mirror);
     const bool have_old = c->nl_prev_valid && c->perm_valid && c->pl_valid;
     FeatCarryArgs fa{n, L, c->nl[1 - c->nl_cur], P, c->pl[c->pl_cur], c->perm, have_old ? 1 : 0};
@@ -630,7 +639,8 @@ struct PolyForceArgs {
   PolyList pl;
   PolyWall pw;
   const uint32_t* mirror;
-  double* pair;
+  double* contrib;    // [nfeat * 9]
+  uint8_t* contact;   // [nfeat]
   const Surface* walls;
   int nw;
   int64_t wstride;
@@ -654,95 +664,90 @@ __device__ inline bool poly_guard(const PolyForceArgs& a) {
   return false;
 }
 
+// One thread per feature: detection, Hooke law and history of that feature; its terms of the owner
+// force, owner moment and partner moment go to contrib (9 doubles) when it is in contact. The pair
+// subtotals are formed by k_poly_gather in feature order.
 template <bool KR, bool GUARD>
-__global__ __launch_bounds__(128) void k_poly_pairs(PolyForceArgs a) {
+__global__ __launch_bounds__(128) void k_poly_feat(PolyForceArgs a) {
   if (GUARD && poly_guard(a)) return;
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= a.n) return;
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= a.pl.nfeat) return;
   const PState& s = a.s;
-  const uint32_t cnt = a.nl.cnt[x];
-  const uint64_t b = a.nl.base[x >> 6] + (uint64_t)(x & 63);
-  const Core cx = s.core[x];
+  const uint32_t sl = a.pl.fslot[f];
+  const uint32_t x = a.pl.fown[f];
+  const uint32_t wd = a.nl.word[sl];
+  const uint32_t y = wd & kIdxMask;
+  const V3 sh = shift_vec(a.g, (int)((wd >> kShiftShift) & 31u));
+  const Core cx = s.core[x], cy = s.core[y];
   const xdem_shape& sa = a.shapes[s.type[x]];
-  const double ra = sa.minkowski * cx.w.w;
+  const xdem_shape& sb = a.shapes[s.type[y]];
+  const double ra = sa.minkowski * cx.w.w, rb = sb.minkowski * cy.w.w;
   const Verts va = verts_of(a.vtx, x, V3{0, 0, 0});
-  const V3 rx{cx.xr.x, cx.xr.y, cx.xr.z}, vx{cx.vm.x, cx.vm.y, cx.vm.z}, wx{cx.w.x, cx.w.y, cx.w.z};
-  for (uint32_t k = 0; k < cnt; k++) {
-    const uint64_t sl = b + (uint64_t)k * kWave;
-    const uint32_t wd = a.nl.word[sl];
-    if (!(wd & kOwnerBit)) continue;
-    const uint32_t y = wd & kIdxMask;
-    const V3 sh = shift_vec(a.g, (int)((wd >> kShiftShift) & 31u));
-    const Core cy = s.core[y];
-    const xdem_shape& sb = a.shapes[s.type[y]];
-    const double rb = sb.minkowski * cy.w.w;
+  const Verts vb = verts_of(a.vtx, y, sh);
+  const uint32_t fw = a.pl.feat[f];
+  const int type = (int)(fw & kFeatTypeMask);
+  const bool swap = (fw & kFeatSwap) != 0;
+  const int osub = (int)((fw >> 3) & 15u), psub = (int)((fw >> 7) & 15u);
+  // evaluation particle i is the vertex side: the owner unless swap
+  const Verts& vi = swap ? vb : va;
+  const Verts& vj = swap ? va : vb;
+  const xdem_shape& si = swap ? sb : sa;
+  const xdem_shape& sj = swap ? sa : sb;
+  const double ri = swap ? rb : ra, rj = swap ? ra : rb;
+  const int fi = swap ? psub : osub, fj = swap ? osub : psub;
+  Det d;
+  if (type == 0) d = detect_vv(vi[fi], ri, vj[fj], rj);
+  else if (type == 1) d = detect_ve(vi[fi], ri, vj[sj.edges[fj][0]], vj[sj.edges[fj][1]], rj);
+  else if (type == 2) d = detect_vf(vi[fi], ri, vj, sj, fj, rj);
+  else d = detect_ee(vi[si.edges[fi][0]], vi[si.edges[fi][1]], ri, vj[sj.edges[fj][0]], vj[sj.edges[fj][1]], rj);
+  if (d.contact) {
+    const V3 rx{cx.xr.x, cx.xr.y, cx.xr.z}, vx{cx.vm.x, cx.vm.y, cx.vm.z}, wx{cx.w.x, cx.w.y, cx.w.z};
     const V3 ry{cy.xr.x + sh.x, cy.xr.y + sh.y, cy.xr.z + sh.z}, vy{cy.vm.x, cy.vm.y, cy.vm.z},
         wy{cy.w.x, cy.w.y, cy.w.z};
     const double mx = cx.vm.w, my = cy.vm.w;
-    V3 Fo{0, 0, 0}, Mo{0, 0, 0}, Mp{0, 0, 0};
-    const uint64_t f0 = a.pl.start[sl], f1 = a.pl.start[sl + 1];
-    if (f1 > f0) {
-      const Verts vb = verts_of(a.vtx, y, sh);
-      for (uint64_t f = f0; f < f1; f++) {
-        const uint32_t fw = a.pl.feat[f];
-        const int type = (int)(fw & kFeatTypeMask);
-        const bool swap = (fw & kFeatSwap) != 0;
-        const int osub = (int)((fw >> 3) & 15u), psub = (int)((fw >> 7) & 15u);
-        // evaluation particle i is the vertex side: the owner unless swap
-        const Verts& vi = swap ? vb : va;
-        const Verts& vj = swap ? va : vb;
-        const xdem_shape& si = swap ? sb : sa;
-        const xdem_shape& sj = swap ? sa : sb;
-        const double ri = swap ? rb : ra, rj = swap ? ra : rb;
-        const int fi = swap ? psub : osub, fj = swap ? osub : psub;
-        Det d;
-        if (type == 0) d = detect_vv(vi[fi], ri, vj[fj], rj);
-        else if (type == 1) d = detect_ve(vi[fi], ri, vj[sj.edges[fj][0]], vj[sj.edges[fj][1]], rj);
-        else if (type == 2) d = detect_vf(vi[fi], ri, vj, sj, fj, rj);
-        else d = detect_ee(vi[si.edges[fi][0]], vi[si.edges[fi][1]], ri, vj[sj.edges[fj][0]],
-                           vj[sj.edges[fj][1]], rj);
-        if (d.contact) {
-          const V3 r_i = swap ? ry : rx, r_j = swap ? rx : ry;
-          const V3 v_i = swap ? vy : vx, v_j = swap ? vx : vy;
-          const V3 w_i = swap ? wy : wx, w_j = swap ? wx : wy;
-          const double mi = swap ? my : mx, mj = swap ? mx : my;
-          const double meff = (mi * mj) / (mi + mj);
-          V3 ft{0, 0, 0}, mint{0, 0, 0};
-          if (fw & kFeatActive) {
-            ft = {a.pl.hx[f], a.pl.hy[f], a.pl.hz[f]};
-            if (KR) mint = {a.pl.mhx[f], a.pl.mhy[f], a.pl.mhz[f]};
-          }
-          const V3 fc = hooke_force<KR>(d.dn, d.n, a.dt, a.cp, meff, ft, mint, d.cp, r_i, v_i, w_i, r_j, v_j, w_j);
-          const bool act = !(ft.x == 0 && ft.y == 0 && ft.z == 0 && mint.x == 0 && mint.y == 0 && mint.z == 0);
-          if (act || (fw & kFeatActive)) {
-            a.pl.hx[f] = ft.x; a.pl.hy[f] = ft.y; a.pl.hz[f] = ft.z;
-            if (KR) {
-              a.pl.mhx[f] = mint.x; a.pl.mhy[f] = mint.y; a.pl.mhz[f] = mint.z;
-            }
-          }
-          const uint32_t nfw = act ? (fw | kFeatActive) : (fw & ~kFeatActive);
-          if (nfw != fw) a.pl.feat[f] = nfw;
-          const V3 nf = -fc;
-          if (!swap) {  // owner is i
-            Fo = Fo + fc;
-            Mo = Mo + (cross(d.cp - r_i, fc) + mint);
-            Mp = Mp + (cross(d.cp - r_j, nf) + (-mint));
-          } else {  // owner is j
-            Fo = Fo + nf;
-            Mo = Mo + (cross(d.cp - r_j, nf) + (-mint));
-            Mp = Mp + (cross(d.cp - r_i, fc) + mint);
-          }
-        } else if (fw & kFeatActive) {
-          a.pl.hx[f] = 0.0; a.pl.hy[f] = 0.0; a.pl.hz[f] = 0.0;
-          a.pl.mhx[f] = 0.0; a.pl.mhy[f] = 0.0; a.pl.mhz[f] = 0.0;
-          a.pl.feat[f] = fw & ~kFeatActive;
-        }
+    const V3 r_i = swap ? ry : rx, r_j = swap ? rx : ry;
+    const V3 v_i = swap ? vy : vx, v_j = swap ? vx : vy;
+    const V3 w_i = swap ? wy : wx, w_j = swap ? wx : wy;
+    const double mi = swap ? my : mx, mj = swap ? mx : my;
+    const double meff = (mi * mj) / (mi + mj);
+    V3 ft{0, 0, 0}, mint{0, 0, 0};
+    if (fw & kFeatActive) {
+      ft = {a.pl.hx[f], a.pl.hy[f], a.pl.hz[f]};
+      if (KR) mint = {a.pl.mhx[f], a.pl.mhy[f], a.pl.mhz[f]};
+    }
+    const V3 fc = hooke_force<KR>(d.dn, d.n, a.dt, a.cp, meff, ft, mint, d.cp, r_i, v_i, w_i, r_j, v_j, w_j);
+    const bool act = !(ft.x == 0 && ft.y == 0 && ft.z == 0 && mint.x == 0 && mint.y == 0 && mint.z == 0);
+    if (act || (fw & kFeatActive)) {
+      a.pl.hx[f] = ft.x; a.pl.hy[f] = ft.y; a.pl.hz[f] = ft.z;
+      if (KR) {
+        a.pl.mhx[f] = mint.x; a.pl.mhy[f] = mint.y; a.pl.mhz[f] = mint.z;
       }
     }
-    double* pr = a.pair + 9 * sl;
+    const uint32_t nfw = act ? (fw | kFeatActive) : (fw & ~kFeatActive);
+    if (nfw != fw) a.pl.feat[f] = nfw;
+    const V3 nf = -fc;
+    V3 Fo, Mo, Mp;
+    if (!swap) {  // owner is i
+      Fo = fc;
+      Mo = cross(d.cp - r_i, fc) + mint;
+      Mp = cross(d.cp - r_j, nf) + (-mint);
+    } else {  // owner is j
+      Fo = nf;
+      Mo = cross(d.cp - r_j, nf) + (-mint);
+      Mp = cross(d.cp - r_i, fc) + mint;
+    }
+    double* pr = a.contrib + 9 * f;
     pr[0] = Fo.x; pr[1] = Fo.y; pr[2] = Fo.z;
     pr[3] = Mo.x; pr[4] = Mo.y; pr[5] = Mo.z;
     pr[6] = Mp.x; pr[7] = Mp.y; pr[8] = Mp.z;
+    a.contact[f] = 1;
+  } else {
+    a.contact[f] = 0;
+    if (fw & kFeatActive) {
+      a.pl.hx[f] = 0.0; a.pl.hy[f] = 0.0; a.pl.hz[f] = 0.0;
+      a.pl.mhx[f] = 0.0; a.pl.mhy[f] = 0.0; a.pl.mhz[f] = 0.0;
+      a.pl.feat[f] = fw & ~kFeatActive;
+    }
   }
 }
 
@@ -760,7 +765,7 @@ __device__ inline V3 angular_accel_p(const d4& q, V3 mom, V3 vrot, V3 I) {
 
 template <bool FUSED, bool KR, bool GUARD>
 __global__ __launch_bounds__(256) void k_poly_gather(PolyForceArgs a) {
-  // the stall decision was taken by k_poly_pairs of the same step
+  // the stall decision was taken by k_poly_feat of the same step
   if (GUARD && a.flags[1]) return;
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= a.n) return;
@@ -778,20 +783,32 @@ __global__ __launch_bounds__(256) void k_poly_gather(PolyForceArgs a) {
   }
   const uint32_t cnt = a.nl.cnt[x];
   const uint64_t b = a.nl.base[x >> 6] + (uint64_t)(x & 63);
+  // per entry, the pair subtotal in feature order (as the owner's pass forms it), then the sum
   for (uint32_t k = 0; k < cnt; k++) {
     const uint64_t sl = b + (uint64_t)k * kWave;
     const uint32_t wd = a.nl.word[sl];
-    if (wd & kOwnerBit) {
-      const double* pr = a.pair + 9 * sl;
-      F = F + V3{pr[0], pr[1], pr[2]};
-      M = M + V3{pr[3], pr[4], pr[5]};
-    } else {
+    const bool own = (wd & kOwnerBit) != 0;
+    uint64_t ps = sl;
+    if (!own) {
       const uint32_t m = a.mirror[sl];
       if (m == 0xffffffffu) continue;
-      const double* pr = a.pair + 9 * (uint64_t)m;
-      F = F + V3{-pr[0], -pr[1], -pr[2]};
-      M = M + V3{pr[6], pr[7], pr[8]};
+      ps = m;
     }
+    V3 Fp{0, 0, 0}, Mp{0, 0, 0};
+    const uint64_t f0 = a.pl.start[ps], f1 = a.pl.start[ps + 1];
+    for (uint64_t f = f0; f < f1; f++) {
+      if (!a.contact[f]) continue;
+      const double* pr = a.contrib + 9 * f;
+      if (own) {
+        Fp = Fp + V3{pr[0], pr[1], pr[2]};
+        Mp = Mp + V3{pr[3], pr[4], pr[5]};
+      } else {
+        Fp = Fp + V3{-pr[0], -pr[1], -pr[2]};
+        Mp = Mp + V3{pr[6], pr[7], pr[8]};
+      }
+    }
+    F = F + Fp;
+    M = M + Mp;
   }
   if (a.nw > 0) {
     const xdem_shape& sh = a.shapes[s.type[x]];
@@ -865,7 +882,8 @@ int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, d
   a.nl = c->nl[c->nl_cur];
   a.pl = c->pl[c->pl_cur];
   a.mirror = c->mirror;
-  a.pair = c->pair_buf;
+  a.contrib = c->fcontrib;
+  a.contact = c->fcontact;
   a.nw = (c->nwalls > 0 && c->pw_valid) ? c->nwalls : 0;
   if (a.nw) a.pw = c->pw[c->pw_cur];
   a.walls = c->d_walls;
@@ -877,9 +895,9 @@ int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, d
   a.gravity_on = gravity_on ? 1 : 0;
   a.flags = c->d_flags;
   a.step_tag = step_tag;
-  const dim3 gp(grid_blocks(a.n, 128)), bp(128), gg(grid_blocks(a.n, 256)), bg(256);
+  const dim3 gp(grid_blocks(std::max<int64_t>(a.pl.nfeat, 1), 128)), bp(128), gg(grid_blocks(a.n, 256)), bg(256);
   const bool kr = c->kr_used;
-#define PAIRS(KRV, G) hipLaunchKernelGGL((k_poly_pairs<KRV, G>), gp, bp, 0, c->stream, a)
+#define PAIRS(KRV, G) hipLaunchKernelGGL((k_poly_feat<KRV, G>), gp, bp, 0, c->stream, a)
 #define GATHER(F, KRV, G) hipLaunchKernelGGL((k_poly_gather<F, KRV, G>), gg, bg, 0, c->stream, a)
   if (kr) {
     if (guarded) PAIRS(true, true); else PAIRS(true, false);

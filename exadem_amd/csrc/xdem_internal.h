@@ -151,6 +151,8 @@ struct PolyList {
   double *hx = nullptr, *hy = nullptr, *hz = nullptr;
   double *mhx = nullptr, *mhy = nullptr, *mhz = nullptr;
   uint32_t* fcount = nullptr;  // [cap_slots] scratch: features per slot
+  uint32_t* fslot = nullptr;   // [cap_feat] list slot of each feature (owner entry)
+  uint32_t* fown = nullptr;    // [cap_feat] owner particle of each feature
   int64_t cap_slots = 0;
   int64_t cap_feat = 0;
   int64_t nfeat = 0;
@@ -243,8 +245,9 @@ struct xdem_ctx {
   xdem::PolyList pl[2];             // feature lists, current and previous
   int pl_cur = 0;
   bool pl_valid = false;
-  double* pair_buf = nullptr;       // [cap_pair * 9] per owner slot: F_owner, M_owner, M_partner
-  int64_t cap_pair = 0;
+  double* fcontrib = nullptr;       // [cap_contrib * 9] per feature: F_owner, M_owner, M_partner terms
+  uint8_t* fcontact = nullptr;      // [cap_contrib] 1 when the feature is in contact
+  int64_t cap_contrib = 0;
   uint32_t* mirror = nullptr;       // [cap_mirror] owner slot of each partner entry
   int64_t cap_mirror = 0;
   xdem::PolyWall pw[2];             // vertex-wall candidates and history, current and previous
