@@ -208,16 +208,18 @@ struct GemmArgs {
   float* dm;        // EPI_BWD_U: [n][H] (=, times 1/8)
 };
 
-// Y[r][c] = sum_k A[r][k] B[c][k] with A = [a1 | a2], for the fixed shapes of the network. MFMA
-// fragments: lane l holds row (or column) l%32 and the 8 consecutive k values 8*(l/32)..+7 of each
-// 16-wide k step; accumulator element v of lane l is D[8*(v/4) + 4*(l/32) + v%4][l%32].
-// Persistent blocks: each block stages B
-// (M x K bf16, rows padded by 8 elements so that the 32 rows read by one MFMA fragment load fall in
-// different LDS banks) once, then walks row tiles of 128 rows (one 32-row strip per wavefront, all M
-// columns: M/32 MFMA tiles per wavefront). A fragments come from global memory, row l%32, 16 bytes.
+// Y[r][c] = sum_k A[r][k] B[c][k] with A = [a1 | a2] (rows = particles), for the fixed shapes of the
+// network, computed transposed on MFMA: D[row][particle] with the MFMA A operand = rows of B (the
+// weights, staged in LDS) and the MFMA B operand = the particle's row of A (16-byte fragments: lane l
+// holds particle l%32 and the 8 consecutive k values 8*(l/32)..+7 of each 16-wide k step).
+// Accumulator element v of lane l is MFMA row 8*(v/4) + 4*(l/32) + v%4 of particle column l%32, and
+// MFMA rows map to output columns by c(8g + 4h + q) = 16h + 4g + q inside each 32-column tile, so a
+// lane holds 16 consecutive output columns of one particle: the epilogue reads and writes 16 bytes.
+// Persistent blocks: each block owns 128 of the M output columns (blockIdx.y), stages those rows of
+// B once (rows padded by 8 elements against LDS bank conflicts) and walks strips of 128 particles
+// (32 per wavefront).
 template <int EPI, int M, int K1, int K2>
 __global__ __launch_bounds__(256) void k_gemm_p(GemmArgs g) {
-  // each block owns 128 of the M columns (blockIdx.y) and stages those rows of B
   constexpr int K = K1 + K2, KP = K + 8, NT = 4;
   extern __shared__ uint16_t w_lds[];
   const int cb = (int)blockIdx.y * 128;
@@ -227,56 +229,87 @@ __global__ __launch_bounds__(256) void k_gemm_p(GemmArgs g) {
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int kk = (lane >> 5) * 8;
+  const int col = lane & 31, half = lane >> 5;
+  const int kk = half * 8;
+  const int crow = 16 * ((col >> 2) & 1) + 4 * (col >> 3) + (col & 3);  // MFMA row `col` -> column
   const int64_t ntiles = (g.n + 127) / 128;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t r0 = tile * 128 + w * 32;
     if (r0 >= g.n) continue;
-    int64_t ra = r0 + (lane & 31);
-    if (ra >= g.n) ra = g.n - 1;
+    const int64_t r = r0 + col;  // this lane's particle
+    const int64_t ra = r < g.n ? r : g.n - 1;
     floatx16 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++)
 #pragma unroll
       for (int v = 0; v < 16; v++) acc[t][v] = 0.f;
-    // every A fragment of the 32-row strip is requested before the first MFMA
-    bf16x8 af[K / 16];
+    // every fragment of the particle rows is requested before the first MFMA
+    bf16x8 xf[K / 16];
     const uint16_t* ap1 = g.a1 + ra * K1 + kk;
 #pragma unroll
-    for (int ks = 0; ks < K1 / 16; ks++) af[ks] = *(const bf16x8*)(ap1 + 16 * ks);
+    for (int ks = 0; ks < K1 / 16; ks++) xf[ks] = *(const bf16x8*)(ap1 + 16 * ks);
     if (K2 > 0) {
       const uint16_t* ap2 = g.a2 + ra * K2 + kk;
 #pragma unroll
-      for (int ks = 0; ks < K2 / 16; ks++) af[K1 / 16 + ks] = *(const bf16x8*)(ap2 + 16 * ks);
+      for (int ks = 0; ks < K2 / 16; ks++) xf[K1 / 16 + ks] = *(const bf16x8*)(ap2 + 16 * ks);
     }
 #pragma unroll
     for (int ks = 0; ks < K / 16; ks++) {
 #pragma unroll
       for (int t = 0; t < NT; t++) {
-        const bf16x8 bv = *(const bf16x8*)(w_lds + (t * 32 + (lane & 31)) * KP + kk + 16 * ks);
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[ks], bv, acc[t], 0, 0, 0);
+        const bf16x8 wv = *(const bf16x8*)(w_lds + (t * 32 + crow) * KP + kk + 16 * ks);
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, xf[ks], acc[t], 0, 0, 0);
       }
     }
+    if (r >= g.n) continue;
 #pragma unroll
     for (int t = 0; t < NT; t++) {
-      const int c = cb + t * 32 + (lane & 31);
+      const int c0 = cb + t * 32 + 16 * half;  // columns c0 .. c0+15
+      if (EPI == EPI_BF16) {
+        uint32_t o[8];
 #pragma unroll
-      for (int v = 0; v < 16; v++) {
-        const int64_t r = r0 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        if (r >= g.n) continue;
-        const float y = acc[t][v];
-        if (EPI == EPI_BF16) {
-          g.ybf[r * M + c] = f2bf(y);
-        } else if (EPI == EPI_UPDATE) {
-          const uint16_t ub = f2bf(y + g.bias[c]);
-          if (g.u_out) g.u_out[r * H + c] = ub;
-          const float hv = bf2f(g.a1[r * H + c]);
-          g.h_out[r * H + c] = f2bf(hv + fmaxf(bf2f(ub), 0.f));
-        } else if (EPI == EPI_BWD_U) {
-          if (c < H) g.dh[r * H + c] += y;
-          else g.dm[r * H + (c - H)] = y * kMsgScale;
-        } else {
-          g.dh[r * H + c] += y;
+        for (int v = 0; v < 8; v++) o[v] = (uint32_t)f2bf(acc[t][2 * v]) | ((uint32_t)f2bf(acc[t][2 * v + 1]) << 16);
+        uint4* dst = (uint4*)(g.ybf + r * M + c0);
+        dst[0] = uint4{o[0], o[1], o[2], o[3]};
+        dst[1] = uint4{o[4], o[5], o[6], o[7]};
+      } else if (EPI == EPI_UPDATE) {
+        const uint4* hp = (const uint4*)(g.a1 + r * H + c0);
+        const uint4 h0 = hp[0], h1 = hp[1];
+        const uint32_t hw[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
+        uint32_t uo[8], ho[8];
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+          const uint16_t u0 = f2bf(acc[t][2 * v] + g.bias[c0 + 2 * v]);
+          const uint16_t u1 = f2bf(acc[t][2 * v + 1] + g.bias[c0 + 2 * v + 1]);
+          uo[v] = (uint32_t)u0 | ((uint32_t)u1 << 16);
+          const uint16_t n0 = f2bf(bf2f((uint16_t)(hw[v] & 0xffffu)) + fmaxf(bf2f(u0), 0.f));
+          const uint16_t n1 = f2bf(bf2f((uint16_t)(hw[v] >> 16)) + fmaxf(bf2f(u1), 0.f));
+          ho[v] = (uint32_t)n0 | ((uint32_t)n1 << 16);
+        }
+        if (g.u_out) {
+          uint4* ud = (uint4*)(g.u_out + r * H + c0);
+          ud[0] = uint4{uo[0], uo[1], uo[2], uo[3]};
+          ud[1] = uint4{uo[4], uo[5], uo[6], uo[7]};
+        }
+        uint4* hd = (uint4*)(g.h_out + r * H + c0);
+        hd[0] = uint4{ho[0], ho[1], ho[2], ho[3]};
+        hd[1] = uint4{ho[4], ho[5], ho[6], ho[7]};
+      } else if (EPI == EPI_BWD_U && c0 >= H) {
+        float4* dst = (float4*)(g.dm + r * H + (c0 - H));
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          dst[q] = float4{acc[t][4 * q] * kMsgScale, acc[t][4 * q + 1] * kMsgScale, acc[t][4 * q + 2] * kMsgScale,
+                          acc[t][4 * q + 3] * kMsgScale};
+      } else {  // EPI_ADD, and the dh half of EPI_BWD_U
+        float4* dst = (float4*)(g.dh + r * H + c0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          float4 cur = dst[q];
+          cur.x += acc[t][4 * q];
+          cur.y += acc[t][4 * q + 1];
+          cur.z += acc[t][4 * q + 2];
+          cur.w += acc[t][4 * q + 3];
+          dst[q] = cur;
         }
       }
     }
@@ -419,15 +452,17 @@ __device__ inline void ld2(const uint16_t* p, float& a, float& b) {
 
 // Message pass on MFMA: two wavefronts per 32 consecutive particles (half of a 64-particle list
 // group), each for 64 of the 128 channels. Step k takes neighbour rank k of each of the 32 particles,
-// i.e. 32 consecutive list slots, and computes D[channel][edge] = sum_f W_E[channel][f] e[edge][f] +
-// (P_i + b_E)[channel] with two 32x32x16 MFMAs (A = W_E rows, B = the edges' 16 inputs, C = P_i +
-// b_E); then z = D + Q_j, and relu(z) is summed over k in the accumulator layout: lane l holds edge
-// column l%32 and, per 32-channel tile, the 4-channel groups 8g + 4*(l/32) .. +3. The list word, edge
-// inputs and Q rows of step k+1 are requested before step k is computed.
+// i.e. 32 consecutive list slots, and computes D[row][edge] = sum_f W_E[c(row)][f] e[edge][f] +
+// (P_i + b_E)[c(row)] with two 32x32x16 MFMAs (A = W_E rows, B = the edges' 16 inputs, C = P_i +
+// b_E); then z = D + Q_j, and relu(z) is summed over k in the accumulator layout. Accumulator element
+// v of lane l is row 8*(v/4) + 4*(l/32) + v%4 of edge column l%32; rows are mapped to channels by
+// c(8g + 4h + q) = 16h + 4g + q inside each 32-channel tile, so that lane half h holds the 16
+// consecutive channels 16h..16h+15 (16-byte loads of P and Q, 16-byte stores of m). The list word,
+// edge inputs and Q rows of step k+1 are requested before step k is computed.
 struct MsgStep {
   bool ok;
   bf16x8 eb;
-  uint2 q[8];
+  uint4 q[4];
 };
 __device__ inline void msg_load(const MsgArgs& a, uint64_t sl, bool ok, uint32_t wd, int half, int ch0, uint32_t self,
                                 MsgStep& st) {
@@ -439,12 +474,15 @@ __device__ inline void msg_load(const MsgArgs& a, uint64_t sl, bool ok, uint32_t
 #pragma unroll
     for (int q = 0; q < 8; q++) st.eb[q] = (__bf16)0.0f;
   }
-  const uint16_t* qrow = a.pq + (int64_t)j * H2 + H + ch0;
+  const uint16_t* qrow = a.pq + (int64_t)j * H2 + H + ch0 + 16 * half;
 #pragma unroll
-  for (int t = 0; t < 2; t++)
-#pragma unroll
-    for (int gq = 0; gq < 4; gq++) st.q[4 * t + gq] = *(const uint2*)(qrow + 32 * t + 8 * gq + 4 * half);
+  for (int t = 0; t < 2; t++) {
+    st.q[2 * t] = *(const uint4*)(qrow + 32 * t);
+    st.q[2 * t + 1] = *(const uint4*)(qrow + 32 * t + 8);
+  }
 }
+__device__ inline float bf_lo(uint32_t v) { return bf2f((uint16_t)(v & 0xffffu)); }
+__device__ inline float bf_hi(uint32_t v) { return bf2f((uint16_t)(v >> 16)); }
 
 __global__ __launch_bounds__(256) void k_msg_fwd_mfma(MsgArgs a, const uint16_t* __restrict__ wEb) {
   const int lane = threadIdx.x & 63;
@@ -460,23 +498,26 @@ __global__ __launch_bounds__(256) void k_msg_fwd_mfma(MsgArgs a, const uint16_t*
   const uint64_t base = a.nl.base[grp] + (uint64_t)(ic & 63);
   const uint32_t cnt = vi ? a.nl.cnt[ic] : 0u;
   const uint32_t kmax = a.nl.kmax[grp];
+  // A rows: MFMA row `col` of each tile is channel c(col)
+  const int crow = 16 * ((col >> 2) & 1) + 4 * (col >> 3) + (col & 3);
   bf16x8 wa[2];
 #pragma unroll
-  for (int t = 0; t < 2; t++) wa[t] = *(const bf16x8*)(wEb + (ch0 + 32 * t + col) * FE + 8 * half);
+  for (int t = 0; t < 2; t++) wa[t] = *(const bf16x8*)(wEb + (ch0 + 32 * t + crow) * FE + 8 * half);
   floatx16 cinit[2], msum[2];
 #pragma unroll
-  for (int t = 0; t < 2; t++)
+  for (int t = 0; t < 2; t++) {
+    const int c0 = ch0 + 32 * t + 16 * half;
+    const uint4 p0 = *(const uint4*)(a.pq + ic * H2 + c0);
+    const uint4 p1 = *(const uint4*)(a.pq + ic * H2 + c0 + 8);
+    const uint32_t pw[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
 #pragma unroll
-    for (int gq = 0; gq < 4; gq++) {
-      const int c0 = ch0 + 32 * t + 8 * gq + 4 * half;
-      const uint2 pv = *(const uint2*)(a.pq + ic * H2 + c0);
-      cinit[t][4 * gq + 0] = bf2f((uint16_t)(pv.x & 0xffffu)) + a.bE[c0 + 0];
-      cinit[t][4 * gq + 1] = bf2f((uint16_t)(pv.x >> 16)) + a.bE[c0 + 1];
-      cinit[t][4 * gq + 2] = bf2f((uint16_t)(pv.y & 0xffffu)) + a.bE[c0 + 2];
-      cinit[t][4 * gq + 3] = bf2f((uint16_t)(pv.y >> 16)) + a.bE[c0 + 3];
-#pragma unroll
-      for (int q = 0; q < 4; q++) msum[t][4 * gq + q] = 0.f;
+    for (int v = 0; v < 8; v++) {
+      cinit[t][2 * v] = bf_lo(pw[v]) + a.bE[c0 + 2 * v];
+      cinit[t][2 * v + 1] = bf_hi(pw[v]) + a.bE[c0 + 2 * v + 1];
     }
+#pragma unroll
+    for (int v = 0; v < 16; v++) msum[t][v] = 0.f;
+  }
   MsgStep cur, nxt;
   uint32_t wnext = 0;
   if (kmax > 0) {
@@ -494,13 +535,12 @@ __global__ __launch_bounds__(256) void k_msg_fwd_mfma(MsgArgs a, const uint16_t*
     for (int t = 0; t < 2; t++) {
       const floatx16 d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[t], cur.eb, cinit[t], 0, 0, 0);
       if (cur.ok) {
+        const uint4 q0 = cur.q[2 * t], q1 = cur.q[2 * t + 1];
+        const uint32_t qw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
 #pragma unroll
-        for (int gq = 0; gq < 4; gq++) {
-          const uint2 qv = cur.q[4 * t + gq];
-          msum[t][4 * gq + 0] += fmaxf(d[4 * gq + 0] + bf2f((uint16_t)(qv.x & 0xffffu)), 0.f);
-          msum[t][4 * gq + 1] += fmaxf(d[4 * gq + 1] + bf2f((uint16_t)(qv.x >> 16)), 0.f);
-          msum[t][4 * gq + 2] += fmaxf(d[4 * gq + 2] + bf2f((uint16_t)(qv.y & 0xffffu)), 0.f);
-          msum[t][4 * gq + 3] += fmaxf(d[4 * gq + 3] + bf2f((uint16_t)(qv.y >> 16)), 0.f);
+        for (int v = 0; v < 8; v++) {
+          msum[t][2 * v] += fmaxf(d[2 * v] + bf_lo(qw[v]), 0.f);
+          msum[t][2 * v + 1] += fmaxf(d[2 * v + 1] + bf_hi(qw[v]), 0.f);
         }
       }
     }
@@ -508,15 +548,15 @@ __global__ __launch_bounds__(256) void k_msg_fwd_mfma(MsgArgs a, const uint16_t*
   }
   if (!vi) return;
 #pragma unroll
-  for (int t = 0; t < 2; t++)
+  for (int t = 0; t < 2; t++) {
+    const int c0 = ch0 + 32 * t + 16 * half;
+    uint32_t o[8];
 #pragma unroll
-    for (int gq = 0; gq < 4; gq++) {
-      const int c0 = ch0 + 32 * t + 8 * gq + 4 * half;
-      uint2 o;
-      o.x = (uint32_t)f2bf(msum[t][4 * gq + 0] * kMsgScale) | ((uint32_t)f2bf(msum[t][4 * gq + 1] * kMsgScale) << 16);
-      o.y = (uint32_t)f2bf(msum[t][4 * gq + 2] * kMsgScale) | ((uint32_t)f2bf(msum[t][4 * gq + 3] * kMsgScale) << 16);
-      *(uint2*)(a.m + i * H + c0) = o;
-    }
+    for (int v = 0; v < 8; v++)
+      o[v] = (uint32_t)f2bf(msum[t][2 * v] * kMsgScale) | ((uint32_t)f2bf(msum[t][2 * v + 1] * kMsgScale) << 16);
+    *(uint4*)(a.m + i * H + c0) = uint4{o[0], o[1], o[2], o[3]};
+    *(uint4*)(a.m + i * H + c0 + 8) = uint4{o[4], o[5], o[6], o[7]};
+  }
 }
 
 // backward of the messages: per particle j, dP_j = sum over own entries j->i of dm_j (z_ji > 0),
