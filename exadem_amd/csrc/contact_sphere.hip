@@ -75,16 +75,7 @@ __device__ inline void fetch_entry(const ForceArgs& a, uint64_t sl, uint32_t wd,
   const Core* __restrict__ core = a.s.core;
   const uint32_t y = wd & kIdxMask;
   e.wd = wd;
-  const uint64_t rel = (uint64_t)((int64_t)y - b0);
-  if (rel < 256) {
-    e.p = tile[rel].xr;
-    e.v = tile[rel].vm;
-    e.w = tile[rel].w;
-  } else {
-    e.p = core[y].xr;
-    e.v = core[y].vm;
-    e.w = core[y].w;
-  }
+  partner_global(core, b0, y, e.p, e.v, e.w);
   if (wd & kActiveBit) {
     e.ft = {a.nl.hx[sl], a.nl.hy[sl], a.nl.hz[sl]};
     if (KR) e.mint = {a.nl.mhx[sl], a.nl.mhy[sl], a.nl.mhz[sl]};
@@ -112,10 +103,7 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
   // the block's own 256 records (32 KB, coalesced 16-byte copies): partners inside the block are
   // read from LDS, the others from global memory
   __shared__ Core tile[256];
-  for (int k = threadIdx.x; k < 256 * 8; k += 256) {
-    const int rec = k >> 3, part = k & 7;
-    if (b0 + rec < a.n) ((uint4*)&tile[rec])[part] = ((const uint4*)&a.s.core[b0 + rec])[part];
-  }
+  stage_tile(tile, a.s.core, b0, a.n);
   __syncthreads();
   const int64_t x = b0 + threadIdx.x;
   if (x >= a.n) return;
@@ -150,6 +138,7 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
       if (k + 2 < cnt) wd2 = word[sl + 2 * kWave];
     }
     const uint32_t wd = e.wd;
+    partner_tile(tile, b0, wd & kIdxMask, e.p, e.v, e.w);
     const bool owner = (wd & kOwnerBit) != 0;
     V3 ri, rj;
     double Ri, Rj;

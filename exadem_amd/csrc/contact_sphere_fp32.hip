@@ -132,16 +132,7 @@ __device__ inline void fetch32(const ForceArgs32& a, uint64_t sl, uint32_t wd, E
   const Core* __restrict__ core = a.s.core;
   const uint32_t y = wd & kIdxMask;
   e.wd = wd;
-  const uint64_t rel = (uint64_t)((int64_t)y - b0);
-  if (rel < 256) {
-    e.p = tile[rel].xr;
-    e.v = tile[rel].vm;
-    e.w = tile[rel].w;
-  } else {
-    e.p = core[y].xr;
-    e.v = core[y].vm;
-    e.w = core[y].w;
-  }
+  partner_global(core, b0, y, e.p, e.v, e.w);
   if (wd & kActiveBit) {
     e.ft = {a.nl.fx32[sl], a.nl.fy32[sl], a.nl.fz32[sl]};
     if (KR) e.mint = {a.nl.mfx32[sl], a.nl.mfy32[sl], a.nl.mfz32[sl]};
@@ -168,10 +159,7 @@ __global__ __launch_bounds__(256) void k_force_fp32(ForceArgs32 a) {
   const int64_t b0 = blk * blockDim.x;
   // the block's own 256 records in LDS: partners inside the block are read from there
   __shared__ Core tile[256];
-  for (int k = threadIdx.x; k < 256 * 8; k += 256) {
-    const int rec = k >> 3, part = k & 7;
-    if (b0 + rec < a.n) ((uint4*)&tile[rec])[part] = ((const uint4*)&a.s.core[b0 + rec])[part];
-  }
+  stage_tile(tile, a.s.core, b0, a.n);
   __syncthreads();
   const int64_t x = b0 + threadIdx.x;
   if (x >= a.n) return;
@@ -198,6 +186,7 @@ __global__ __launch_bounds__(256) void k_force_fp32(ForceArgs32 a) {
       if (k + 2 < cnt) wd2 = word[sl + 2 * kWave];
     }
     const uint32_t wd = e.wd;
+    partner_tile(tile, b0, wd & kIdxMask, e.p, e.v, e.w);
     const bool owner = (wd & kOwnerBit) != 0;
     V3 ri, rj;
     double Ri, Rj;

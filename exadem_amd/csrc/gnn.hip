@@ -215,26 +215,31 @@ struct GemmArgs {
 // Accumulator element v of lane l is MFMA row 8*(v/4) + 4*(l/32) + v%4 of particle column l%32, and
 // MFMA rows map to output columns by c(8g + 4h + q) = 16h + 4g + q inside each 32-column tile, so a
 // lane holds 16 consecutive output columns of one particle: the epilogue reads and writes 16 bytes.
-// Persistent blocks: each block owns 128 of the M output columns (blockIdx.y), stages those rows of
-// B once (rows padded by 8 elements against LDS bank conflicts) and walks strips of 128 particles
-// (32 per wavefront).
+// The k order is permuted the same way on both operands: k step s = 2t + u takes, in lane half h,
+// the 8 inputs 32t + 16h + 8u .. +7, so the fragments a lane loads for k steps 2t and 2t+1 are the
+// 16 channels it writes for output tile t (the residual of the update comes from registers).
+// Persistent blocks of 8 wavefronts: each block owns 128 of the M output columns (blockIdx.y),
+// stages those rows of B once (rows padded by 8 elements against LDS bank conflicts; 68 KB for
+// K = 256, so two blocks, 16 wavefronts, fit a CU) and walks strips of 256 particles (32 per
+// wavefront).
 template <int EPI, int M, int K1, int K2>
-__global__ __launch_bounds__(256) void k_gemm_p(GemmArgs g) {
+__global__ __launch_bounds__(512) void k_gemm_p(GemmArgs g) {
   constexpr int K = K1 + K2, KP = K + 8, NT = 4;
   extern __shared__ uint16_t w_lds[];
   const int cb = (int)blockIdx.y * 128;
-  for (int idx = threadIdx.x; idx < 128 * K / 8; idx += 256) {
+  for (int idx = threadIdx.x; idx < 128 * K / 8; idx += 512) {
     const int r = idx / (K / 8), c8 = (idx % (K / 8)) * 8;
     *(uint4*)(w_lds + r * KP + c8) = *(const uint4*)(g.b + (int64_t)(cb + r) * K + c8);
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int col = lane & 31, half = lane >> 5;
-  const int kk = half * 8;
+  static_assert(K1 % 32 == 0 && K2 % 32 == 0, "k steps come in pairs");
+  static_assert(EPI != EPI_UPDATE || (M == 128 && K1 == 128), "the residual is the first 128 inputs");
   const int crow = 16 * ((col >> 2) & 1) + 4 * (col >> 3) + (col & 3);  // MFMA row `col` -> column
-  const int64_t ntiles = (g.n + 127) / 128;
+  const int64_t ntiles = (g.n + 255) / 256;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int64_t r0 = tile * 128 + w * 32;
+    const int64_t r0 = tile * 256 + w * 32;
     if (r0 >= g.n) continue;
     const int64_t r = r0 + col;  // this lane's particle
     const int64_t ra = r < g.n ? r : g.n - 1;
@@ -245,19 +250,21 @@ __global__ __launch_bounds__(256) void k_gemm_p(GemmArgs g) {
       for (int v = 0; v < 16; v++) acc[t][v] = 0.f;
     // every fragment of the particle rows is requested before the first MFMA
     bf16x8 xf[K / 16];
-    const uint16_t* ap1 = g.a1 + ra * K1 + kk;
+    const uint16_t* ap1 = g.a1 + ra * K1 + 16 * half;
 #pragma unroll
-    for (int ks = 0; ks < K1 / 16; ks++) xf[ks] = *(const bf16x8*)(ap1 + 16 * ks);
+    for (int ks = 0; ks < K1 / 16; ks++) xf[ks] = *(const bf16x8*)(ap1 + 32 * (ks >> 1) + 8 * (ks & 1));
     if (K2 > 0) {
-      const uint16_t* ap2 = g.a2 + ra * K2 + kk;
+      const uint16_t* ap2 = g.a2 + ra * K2 + 16 * half;
 #pragma unroll
-      for (int ks = 0; ks < K2 / 16; ks++) xf[K1 / 16 + ks] = *(const bf16x8*)(ap2 + 16 * ks);
+      for (int ks = 0; ks < K2 / 16; ks++)
+        xf[K1 / 16 + ks] = *(const bf16x8*)(ap2 + 32 * (ks >> 1) + 8 * (ks & 1));
     }
 #pragma unroll
     for (int ks = 0; ks < K / 16; ks++) {
+      const int kw = 32 * (ks >> 1) + 16 * half + 8 * (ks & 1);
 #pragma unroll
       for (int t = 0; t < NT; t++) {
-        const bf16x8 wv = *(const bf16x8*)(w_lds + (t * 32 + crow) * KP + kk + 16 * ks);
+        const bf16x8 wv = *(const bf16x8*)(w_lds + (t * 32 + crow) * KP + kw);
         acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, xf[ks], acc[t], 0, 0, 0);
       }
     }
@@ -273,8 +280,7 @@ __global__ __launch_bounds__(256) void k_gemm_p(GemmArgs g) {
         dst[0] = uint4{o[0], o[1], o[2], o[3]};
         dst[1] = uint4{o[4], o[5], o[6], o[7]};
       } else if (EPI == EPI_UPDATE) {
-        const uint4* hp = (const uint4*)(g.a1 + r * H + c0);
-        const uint4 h0 = hp[0], h1 = hp[1];
+        const uint4 h0 = __builtin_bit_cast(uint4, xf[2 * t]), h1 = __builtin_bit_cast(uint4, xf[2 * t + 1]);
         const uint32_t hw[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
         uint32_t uo[8], ho[8];
 #pragma unroll
@@ -450,19 +456,21 @@ __device__ inline void ld2(const uint16_t* p, float& a, float& b) {
   b = bf2f((uint16_t)(v >> 16));
 }
 
-// Message pass on MFMA: two wavefronts per 32 consecutive particles (half of a 64-particle list
-// group), each for 64 of the 128 channels. Step k takes neighbour rank k of each of the 32 particles,
-// i.e. 32 consecutive list slots, and computes D[row][edge] = sum_f W_E[c(row)][f] e[edge][f] +
-// (P_i + b_E)[c(row)] with two 32x32x16 MFMAs (A = W_E rows, B = the edges' 16 inputs, C = P_i +
-// b_E); then z = D + Q_j, and relu(z) is summed over k in the accumulator layout. Accumulator element
-// v of lane l is row 8*(v/4) + 4*(l/32) + v%4 of edge column l%32; rows are mapped to channels by
-// c(8g + 4h + q) = 16h + 4g + q inside each 32-channel tile, so that lane half h holds the 16
-// consecutive channels 16h..16h+15 (16-byte loads of P and Q, 16-byte stores of m). The list word,
-// edge inputs and Q rows of step k+1 are requested before step k is computed.
+// Message pass on MFMA: one block of four wavefronts per 32 consecutive particles (half of a
+// 64-particle list group), each wavefront for 32 of the 128 channels. Step k takes neighbour rank k
+// of each of the 32 particles, i.e. 32 consecutive list slots, and computes D[row][edge] =
+// sum_f W_E[c(row)][f] e[edge][f] + (P_i + b_E)[c(row)] with one 32x32x16 MFMA (A = W_E rows,
+// B = the edges' 16 inputs, C = P_i + b_E); then z = D + Q_j, and relu(z) is summed over k in the
+// accumulator layout. Accumulator element v of lane l is row 8*(v/4) + 4*(l/32) + v%4 of edge column
+// l%32; rows are mapped to channels by c(8g + 4h + q) = 16h + 4g + q inside the 32-channel tile, so
+// that lane half h holds the 16 consecutive channels 16h..16h+15 (16-byte loads of P and Q, 16-byte
+// stores of m). The list words and edge inputs of the block's 32 slots are shared by its four
+// wavefronts through L1. The list word, edge inputs and Q rows of step k+1 are requested before
+// step k is computed, the list word of step k+2 before that.
 struct MsgStep {
   bool ok;
   bf16x8 eb;
-  uint4 q[4];
+  uint4 q[2];
 };
 __device__ inline void msg_load(const MsgArgs& a, uint64_t sl, bool ok, uint32_t wd, int half, int ch0, uint32_t self,
                                 MsgStep& st) {
@@ -475,20 +483,16 @@ __device__ inline void msg_load(const MsgArgs& a, uint64_t sl, bool ok, uint32_t
     for (int q = 0; q < 8; q++) st.eb[q] = (__bf16)0.0f;
   }
   const uint16_t* qrow = a.pq + (int64_t)j * H2 + H + ch0 + 16 * half;
-#pragma unroll
-  for (int t = 0; t < 2; t++) {
-    st.q[2 * t] = *(const uint4*)(qrow + 32 * t);
-    st.q[2 * t + 1] = *(const uint4*)(qrow + 32 * t + 8);
-  }
+  st.q[0] = *(const uint4*)(qrow);
+  st.q[1] = *(const uint4*)(qrow + 8);
 }
 __device__ inline float bf_lo(uint32_t v) { return bf2f((uint16_t)(v & 0xffffu)); }
 __device__ inline float bf_hi(uint32_t v) { return bf2f((uint16_t)(v >> 16)); }
 
 __global__ __launch_bounds__(256) void k_msg_fwd_mfma(MsgArgs a, const uint16_t* __restrict__ wEb) {
   const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t i0 = (wave >> 1) * 32;
-  const int ch0 = (int)(wave & 1) * 64;
+  const int64_t i0 = (int64_t)blockIdx.x * 32;
+  const int ch0 = (int)(threadIdx.x >> 6) * 32;
   if (i0 >= a.n) return;
   const int col = lane & 31, half = lane >> 5;
   const int64_t i = i0 + col;
@@ -498,25 +502,22 @@ __global__ __launch_bounds__(256) void k_msg_fwd_mfma(MsgArgs a, const uint16_t*
   const uint64_t base = a.nl.base[grp] + (uint64_t)(ic & 63);
   const uint32_t cnt = vi ? a.nl.cnt[ic] : 0u;
   const uint32_t kmax = a.nl.kmax[grp];
-  // A rows: MFMA row `col` of each tile is channel c(col)
+  // A rows: MFMA row `col` of the tile is channel c(col)
   const int crow = 16 * ((col >> 2) & 1) + 4 * (col >> 3) + (col & 3);
-  bf16x8 wa[2];
-#pragma unroll
-  for (int t = 0; t < 2; t++) wa[t] = *(const bf16x8*)(wEb + (ch0 + 32 * t + crow) * FE + 8 * half);
-  floatx16 cinit[2], msum[2];
-#pragma unroll
-  for (int t = 0; t < 2; t++) {
-    const int c0 = ch0 + 32 * t + 16 * half;
+  const bf16x8 wa = *(const bf16x8*)(wEb + (ch0 + crow) * FE + 8 * half);
+  const int c0 = ch0 + 16 * half;
+  floatx16 cinit, msum;
+  {
     const uint4 p0 = *(const uint4*)(a.pq + ic * H2 + c0);
     const uint4 p1 = *(const uint4*)(a.pq + ic * H2 + c0 + 8);
     const uint32_t pw[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
 #pragma unroll
     for (int v = 0; v < 8; v++) {
-      cinit[t][2 * v] = bf_lo(pw[v]) + a.bE[c0 + 2 * v];
-      cinit[t][2 * v + 1] = bf_hi(pw[v]) + a.bE[c0 + 2 * v + 1];
+      cinit[2 * v] = bf_lo(pw[v]) + a.bE[c0 + 2 * v];
+      cinit[2 * v + 1] = bf_hi(pw[v]) + a.bE[c0 + 2 * v + 1];
     }
 #pragma unroll
-    for (int v = 0; v < 16; v++) msum[t][v] = 0.f;
+    for (int v = 0; v < 16; v++) msum[v] = 0.f;
   }
   MsgStep cur, nxt;
   uint32_t wnext = 0;
@@ -531,32 +532,25 @@ __global__ __launch_bounds__(256) void k_msg_fwd_mfma(MsgArgs a, const uint16_t*
       msg_load(a, sl1, k + 1 < cnt, wnext, half, ch0, (uint32_t)ic, nxt);
       wnext = (k + 2 < kmax && k + 2 < cnt) ? a.nl.word[sl1 + kWave] : 0u;
     }
+    const floatx16 d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa, cur.eb, cinit, 0, 0, 0);
+    if (cur.ok) {
+      const uint32_t qw[8] = {cur.q[0].x, cur.q[0].y, cur.q[0].z, cur.q[0].w,
+                              cur.q[1].x, cur.q[1].y, cur.q[1].z, cur.q[1].w};
 #pragma unroll
-    for (int t = 0; t < 2; t++) {
-      const floatx16 d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[t], cur.eb, cinit[t], 0, 0, 0);
-      if (cur.ok) {
-        const uint4 q0 = cur.q[2 * t], q1 = cur.q[2 * t + 1];
-        const uint32_t qw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-#pragma unroll
-        for (int v = 0; v < 8; v++) {
-          msum[t][2 * v] += fmaxf(d[2 * v] + bf_lo(qw[v]), 0.f);
-          msum[t][2 * v + 1] += fmaxf(d[2 * v + 1] + bf_hi(qw[v]), 0.f);
-        }
+      for (int v = 0; v < 8; v++) {
+        msum[2 * v] += fmaxf(d[2 * v] + bf_lo(qw[v]), 0.f);
+        msum[2 * v + 1] += fmaxf(d[2 * v + 1] + bf_hi(qw[v]), 0.f);
       }
     }
     cur = nxt;
   }
   if (!vi) return;
+  uint32_t o[8];
 #pragma unroll
-  for (int t = 0; t < 2; t++) {
-    const int c0 = ch0 + 32 * t + 16 * half;
-    uint32_t o[8];
-#pragma unroll
-    for (int v = 0; v < 8; v++)
-      o[v] = (uint32_t)f2bf(msum[t][2 * v] * kMsgScale) | ((uint32_t)f2bf(msum[t][2 * v + 1] * kMsgScale) << 16);
-    *(uint4*)(a.m + i * H + c0) = uint4{o[0], o[1], o[2], o[3]};
-    *(uint4*)(a.m + i * H + c0 + 8) = uint4{o[4], o[5], o[6], o[7]};
-  }
+  for (int v = 0; v < 8; v++)
+    o[v] = (uint32_t)f2bf(msum[2 * v] * kMsgScale) | ((uint32_t)f2bf(msum[2 * v + 1] * kMsgScale) << 16);
+  *(uint4*)(a.m + i * H + c0) = uint4{o[0], o[1], o[2], o[3]};
+  *(uint4*)(a.m + i * H + c0 + 8) = uint4{o[4], o[5], o[6], o[7]};
 }
 
 // backward of the messages: per particle j, dP_j = sum over own entries j->i of dm_j (z_ji > 0),
@@ -1091,9 +1085,9 @@ int gemm_p(xdem_gnn* G, const GemmArgs& a) {
   const int lds = 128 * (K + 8) * (int)sizeof(uint16_t);
   XDEM_HIP(c, hipFuncSetAttribute((const void*)k_gemm_p<EPI, M, K1, K2>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  const int64_t ntiles = (a.n + 127) / 128;
-  const int grid = (int)std::min<int64_t>(ntiles, 1024 / (M / 128));
-  hipLaunchKernelGGL((k_gemm_p<EPI, M, K1, K2>), dim3(grid, M / 128), dim3(256), lds, c->stream, a);
+  const int64_t ntiles = (a.n + 255) / 256;
+  const int grid = (int)std::min<int64_t>(ntiles, 512 / (M / 128));
+  hipLaunchKernelGGL((k_gemm_p<EPI, M, K1, K2>), dim3(grid, M / 128), dim3(512), lds, c->stream, a);
   XDEM_HIP(c, hipGetLastError());
   return XDEM_OK;
 }
@@ -1177,7 +1171,7 @@ int forward(xdem_gnn* G) {
     MsgArgs ma{};
     ma.n = n; ma.nl = nl; ma.e = G->e; ma.pq = pq;
     ma.wE = G->wr + G->off.W_E[l]; ma.bE = G->wr + G->off.b_E[l]; ma.m = mm;
-    hipLaunchKernelGGL(k_msg_fwd_mfma, dim3(nb((n + 31) / 32 * 128, 256)), dim3(256), 0, c->stream, ma,
+    hipLaunchKernelGGL(k_msg_fwd_mfma, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, c->stream, ma,
                        G->wbf + G->off.W_E[l]);
     GemmArgs g2{};
     g2.n = n; g2.a1 = hin; g2.k1 = H; g2.a2 = mm; g2.k2 = H;

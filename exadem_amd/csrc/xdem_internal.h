@@ -93,6 +93,53 @@ struct alignas(128) Core {
 };
 static_assert(sizeof(Core) == 128, "one particle record per 128-byte line");
 
+// Partner records are read either from a block's LDS copy or from global memory. The two reads go
+// through pointers typed with their address space: through plain (generic) pointers the compiler
+// may fold the two branches into one load through a selected pointer, i.e. a flat load, which also
+// counts on the LDS counter; every later wait for an LDS read then waits for the gathers in flight
+// as well, which undoes the software pipeline of the contact loops.
+typedef const double __attribute__((address_space(3)))* LdsF64;
+typedef const double __attribute__((address_space(1)))* GlbF64;
+__device__ inline d4 ld4(LdsF64 p) { return d4{p[0], p[1], p[2], p[3]}; }
+__device__ inline d4 ld4(GlbF64 p) { return d4{p[0], p[1], p[2], p[3]}; }
+__device__ inline void rec_fields(LdsF64 r, d4& p, d4& v, d4& w) {
+  p = ld4(r);
+  v = ld4(r + 4);
+  w = ld4(r + 8);
+}
+__device__ inline void rec_fields(GlbF64 r, d4& p, d4& v, d4& w) {
+  p = ld4(r);
+  v = ld4(r + 4);
+  w = ld4(r + 8);
+}
+// Partner y's position, velocity and spin in two halves: the prefetch reads partners outside the
+// tile of records b0 .. b0+255 from `core` (global memory, issued one entry ahead), and the use reads
+// partners inside the tile from LDS. Were both reads in the prefetch, the LDS read into the registers
+// of a pending global load would wait for that load at once.
+__device__ inline void partner_global(const Core* core, int64_t b0, uint32_t y, d4& p, d4& v, d4& w) {
+  if ((uint64_t)((int64_t)y - b0) >= 256) rec_fields((GlbF64)(const double*)(core + y), p, v, w);
+}
+__device__ inline void partner_tile(const Core* tile, int64_t b0, uint32_t y, d4& p, d4& v, d4& w) {
+  const uint64_t rel = (uint64_t)((int64_t)y - b0);
+  if (rel < 256) rec_fields((LdsF64)(const double*)(tile + rel), p, v, w);
+}
+// Copies records b0 .. b0+255 into the tile with coalesced 16-byte loads, all eight issued before
+// the first LDS write; indexes past n read record n-1 (never used).
+__device__ inline void stage_tile(Core* tile, const Core* core, int64_t b0, int64_t n) {
+  uint4 v[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int k = (int)threadIdx.x + 256 * q;
+    const int64_t src = b0 + (k >> 3) < n ? b0 + (k >> 3) : n - 1;
+    v[q] = ((const uint4*)(core + src))[k & 7];
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int k = (int)threadIdx.x + 256 * q;
+    ((uint4*)(tile + (k >> 3)))[k & 7] = v[q];
+  }
+}
+
 // Device particle state (one buffer set; two sets are kept and swapped at each reorder).
 struct PState {
   Core* core;
