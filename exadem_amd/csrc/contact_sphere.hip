@@ -51,13 +51,6 @@ struct ForceArgs {
   int32_t step_tag;
 };
 
-// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of particles so
-// that neighbour gathers of consecutive blocks share that XCD's L2.
-__device__ inline int64_t xcd_block(int64_t b, int64_t nb) {
-  const int64_t q = nb >> 3, r = nb & 7;
-  const int64_t x = b & 7, k = b >> 3;
-  return x * q + (x < r ? x : r) + k;
-}
 
 // One entry of a particle's list with the partner data it needs, loaded ahead of use.
 struct Entry {

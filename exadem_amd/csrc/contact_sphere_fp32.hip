@@ -91,11 +91,6 @@ struct ForceArgs32 {
   int32_t step_tag;
 };
 
-__device__ inline int64_t xcd_block32(int64_t b, int64_t nb) {
-  const int64_t q = nb >> 3, r = nb & 7;
-  const int64_t x = b & 7, k = b >> 3;
-  return x * q + (x < r ? x : r) + k;
-}
 
 struct Q4g {
   double w, x, y, z;
@@ -155,7 +150,7 @@ __global__ __launch_bounds__(256) void k_force_fp32(ForceArgs32 a) {
       return;
     }
   }
-  const int64_t blk = xcd_block32(blockIdx.x, gridDim.x);
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
   const int64_t b0 = blk * blockDim.x;
   // the block's own 256 records in LDS: partners inside the block are read from there
   __shared__ Core tile[256];

@@ -61,12 +61,25 @@ __host__ __device__ inline float bf2f(uint16_t v) {
   return f;
 }
 // round to nearest even; NaN stays NaN
-__host__ __device__ inline uint16_t f2bf(float f) {
+// float -> bfloat16, round to nearest even, on the hardware conversion (v_cvt_pk_bf16_f32)
+typedef float float2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+__device__ inline uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+// the same rounding in integer arithmetic: cheaper than the hardware conversion inside kernels that
+// keep the FP64 divide unit busy (k_features: 198 us against 265 us at 1M particles)
+__device__ inline uint16_t f2bf_int(float f) {
   uint32_t u;
   memcpy(&u, &f, 4);
   if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
   u += 0x7fffu + ((u >> 16) & 1u);
   return (uint16_t)(u >> 16);
+}
+__device__ inline uint32_t pack_bf16_int(float lo, float hi) {
+  return (uint32_t)f2bf_int(lo) | ((uint32_t)f2bf_int(hi) << 16);
+}
+// two values in one 32-bit word, `lo` in the low half
+__device__ inline uint32_t pack_bf16(float lo, float hi) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(float2v{lo, hi}, bf16x2v));
 }
 __device__ inline float rbf(float f) { return bf2f(f2bf(f)); }
 
@@ -88,7 +101,8 @@ struct FeatArgs {
 //   e0..2 = n, e3 = o, e4..6 = v, e7 = (R_j - R_i)/Rs * 4,
 //   e8..10 = max(o, 0) n, e11..13 = c vn n, e14 = max(o, 0), e15 = c
 __global__ __launch_bounds__(256) void k_features(FeatArgs a) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // XCD-aware block order (consecutive particle ranges share an XCD's L2 for the partner gathers)
+  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const Core ci = a.s.core[i];
   const V3 ri{ci.xr.x, ci.xr.y, ci.xr.z}, vi{ci.vm.x, ci.vm.y, ci.vm.z}, wi{ci.w.x, ci.w.y, ci.w.z};
@@ -97,11 +111,11 @@ __global__ __launch_bounds__(256) void k_features(FeatArgs a) {
   {
     uint16_t xv[8];
     const float rr = (float)(Ri / a.r0);
-    xv[0] = f2bf((float)((Ri - a.r0) / a.r0) * 4.0f);
-    xv[1] = f2bf((float)(wi.x * Ri) * inv_v0);
-    xv[2] = f2bf((float)(wi.y * Ri) * inv_v0);
-    xv[3] = f2bf((float)(wi.z * Ri) * inv_v0);
-    xv[4] = f2bf(rr * rr * rr);
+    xv[0] = f2bf_int((float)((Ri - a.r0) / a.r0) * 4.0f);
+    xv[1] = f2bf_int((float)(wi.x * Ri) * inv_v0);
+    xv[2] = f2bf_int((float)(wi.y * Ri) * inv_v0);
+    xv[3] = f2bf_int((float)(wi.z * Ri) * inv_v0);
+    xv[4] = f2bf_int(rr * rr * rr);
     xv[5] = xv[6] = xv[7] = 0;
     uint4 pk;
     memcpy(&pk, xv, 16);
@@ -129,25 +143,16 @@ __global__ __launch_bounds__(256) void k_features(FeatArgs a) {
     const float op = fmaxf(o, 0.f), cf = o > 0.f ? 1.f : 0.f;
     const float v0 = (float)vr.x * inv_v0, v1 = (float)vr.y * inv_v0, v2 = (float)vr.z * inv_v0;
     const float vn = cf * ((v0 * n0 + v1 * n1) + v2 * n2);
-    uint16_t ev[FE];
-    ev[0] = f2bf(n0);
-    ev[1] = f2bf(n1);
-    ev[2] = f2bf(n2);
-    ev[3] = f2bf(o);
-    ev[4] = f2bf(v0);
-    ev[5] = f2bf(v1);
-    ev[6] = f2bf(v2);
-    ev[7] = f2bf((float)((Rj - Ri) / Rs) * 4.0f);
-    ev[8] = f2bf(op * n0);
-    ev[9] = f2bf(op * n1);
-    ev[10] = f2bf(op * n2);
-    ev[11] = f2bf(vn * n0);
-    ev[12] = f2bf(vn * n1);
-    ev[13] = f2bf(vn * n2);
-    ev[14] = f2bf(op);
-    ev[15] = f2bf(cf);
-    uint4 pk[2];
-    memcpy(pk, ev, 32);
+    uint32_t ev[FE / 2];
+    ev[0] = pack_bf16_int(n0, n1);
+    ev[1] = pack_bf16_int(n2, o);
+    ev[2] = pack_bf16_int(v0, v1);
+    ev[3] = pack_bf16_int(v2, (float)((Rj - Ri) / Rs) * 4.0f);
+    ev[4] = pack_bf16_int(op * n0, op * n1);
+    ev[5] = pack_bf16_int(op * n2, vn * n0);
+    ev[6] = pack_bf16_int(vn * n1, vn * n2);
+    ev[7] = pack_bf16_int(op, cf);
+    uint4 pk[2] = {uint4{ev[0], ev[1], ev[2], ev[3]}, uint4{ev[4], ev[5], ev[6], ev[7]}};
     uint4* dst = (uint4*)(a.e + sl * FE);
     dst[0] = pk[0];
     dst[1] = pk[1];
@@ -185,8 +190,8 @@ __global__ __launch_bounds__(256) void k_mirror_all(int64_t n, NbrList nl, uint3
 struct EncArgs {
   int64_t n;
   const uint16_t* x;
-  const float* w;  // W_in rounded [H][FN]
-  const float* b;
+  const uint16_t* w;  // W_in, bfloat16 [H][FN]
+  const uint16_t* b;  // b_in, bfloat16 [H]
   uint16_t* h;
 };
 // ------------------------------------------------------------------------------------- MFMA GEMM
@@ -227,16 +232,18 @@ __global__ __launch_bounds__(512) void k_gemm_p(GemmArgs g) {
   constexpr int K = K1 + K2, KP = K + 8, NT = 4;
   extern __shared__ uint16_t w_lds[];
   const int cb = (int)blockIdx.y * 128;
+  // LDS row t*32 + m holds the weight row of MFMA row m of tile t, i.e. column 32t + c(m): lanes then
+  // read consecutive LDS rows, which the 8-element padding spreads over distinct banks
   for (int idx = threadIdx.x; idx < 128 * K / 8; idx += 512) {
     const int r = idx / (K / 8), c8 = (idx % (K / 8)) * 8;
-    *(uint4*)(w_lds + r * KP + c8) = *(const uint4*)(g.b + (int64_t)(cb + r) * K + c8);
+    const int c = r & 31, m = 8 * ((c >> 2) & 3) + 4 * (c >> 4) + (c & 3);
+    *(uint4*)(w_lds + ((r & ~31) + m) * KP + c8) = *(const uint4*)(g.b + (int64_t)(cb + r) * K + c8);
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int col = lane & 31, half = lane >> 5;
   static_assert(K1 % 32 == 0 && K2 % 32 == 0, "k steps come in pairs");
   static_assert(EPI != EPI_UPDATE || (M == 128 && K1 == 128), "the residual is the first 128 inputs");
-  const int crow = 16 * ((col >> 2) & 1) + 4 * (col >> 3) + (col & 3);  // MFMA row `col` -> column
   const int64_t ntiles = (g.n + 255) / 256;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t r0 = tile * 256 + w * 32;
@@ -259,12 +266,16 @@ __global__ __launch_bounds__(512) void k_gemm_p(GemmArgs g) {
       for (int ks = 0; ks < K2 / 16; ks++)
         xf[K1 / 16 + ks] = *(const bf16x8*)(ap2 + 32 * (ks >> 1) + 8 * (ks & 1));
     }
+    // keep every fragment load ahead of the first MFMA: left to itself the scheduler sinks each load
+    // to its use, and the k steps then wait for memory one after the other (the update product, with
+    // twice the fragments, is left to the scheduler: holding all of them costs it occupancy)
+    if (EPI != EPI_UPDATE) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int ks = 0; ks < K / 16; ks++) {
       const int kw = 32 * (ks >> 1) + 16 * half + 8 * (ks & 1);
 #pragma unroll
       for (int t = 0; t < NT; t++) {
-        const bf16x8 wv = *(const bf16x8*)(w_lds + (t * 32 + crow) * KP + kw);
+        const bf16x8 wv = *(const bf16x8*)(w_lds + (t * 32 + col) * KP + kw);
         acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, xf[ks], acc[t], 0, 0, 0);
       }
     }
@@ -275,7 +286,7 @@ __global__ __launch_bounds__(512) void k_gemm_p(GemmArgs g) {
       if (EPI == EPI_BF16) {
         uint32_t o[8];
 #pragma unroll
-        for (int v = 0; v < 8; v++) o[v] = (uint32_t)f2bf(acc[t][2 * v]) | ((uint32_t)f2bf(acc[t][2 * v + 1]) << 16);
+        for (int v = 0; v < 8; v++) o[v] = pack_bf16(acc[t][2 * v], acc[t][2 * v + 1]);
         uint4* dst = (uint4*)(g.ybf + r * M + c0);
         dst[0] = uint4{o[0], o[1], o[2], o[3]};
         dst[1] = uint4{o[4], o[5], o[6], o[7]};
@@ -285,12 +296,10 @@ __global__ __launch_bounds__(512) void k_gemm_p(GemmArgs g) {
         uint32_t uo[8], ho[8];
 #pragma unroll
         for (int v = 0; v < 8; v++) {
-          const uint16_t u0 = f2bf(acc[t][2 * v] + g.bias[c0 + 2 * v]);
-          const uint16_t u1 = f2bf(acc[t][2 * v + 1] + g.bias[c0 + 2 * v + 1]);
-          uo[v] = (uint32_t)u0 | ((uint32_t)u1 << 16);
-          const uint16_t n0 = f2bf(bf2f((uint16_t)(hw[v] & 0xffffu)) + fmaxf(bf2f(u0), 0.f));
-          const uint16_t n1 = f2bf(bf2f((uint16_t)(hw[v] >> 16)) + fmaxf(bf2f(u1), 0.f));
-          ho[v] = (uint32_t)n0 | ((uint32_t)n1 << 16);
+          uo[v] = pack_bf16(acc[t][2 * v] + g.bias[c0 + 2 * v], acc[t][2 * v + 1] + g.bias[c0 + 2 * v + 1]);
+          const float u0 = bf2f((uint16_t)(uo[v] & 0xffffu)), u1 = bf2f((uint16_t)(uo[v] >> 16));
+          ho[v] = pack_bf16(bf2f((uint16_t)(hw[v] & 0xffffu)) + fmaxf(u0, 0.f),
+                            bf2f((uint16_t)(hw[v] >> 16)) + fmaxf(u1, 0.f));
         }
         if (g.u_out) {
           uint4* ud = (uint4*)(g.u_out + r * H + c0);
@@ -459,12 +468,13 @@ __device__ inline void ld2(const uint16_t* p, float& a, float& b) {
 // Message pass on MFMA: one block of four wavefronts per 32 consecutive particles (half of a
 // 64-particle list group), each wavefront for 32 of the 128 channels. Step k takes neighbour rank k
 // of each of the 32 particles, i.e. 32 consecutive list slots, and computes D[row][edge] =
-// sum_f W_E[c(row)][f] e[edge][f] + (P_i + b_E)[c(row)] with one 32x32x16 MFMA (A = W_E rows,
-// B = the edges' 16 inputs, C = P_i + b_E); then z = D + Q_j, and relu(z) is summed over k in the
-// accumulator layout. Accumulator element v of lane l is row 8*(v/4) + 4*(l/32) + v%4 of edge column
-// l%32; rows are mapped to channels by c(8g + 4h + q) = 16h + 4g + q inside the 32-channel tile, so
-// that lane half h holds the 16 consecutive channels 16h..16h+15 (16-byte loads of P and Q, 16-byte
-// stores of m). The list words and edge inputs of the block's 32 slots are shared by its four
+// sum_f W_E[c(row)][f] e[edge][f] + (P_i + b_E)[c(row)] + Q_j[c(row)] with three 32x32x16 MFMAs:
+// A = W_E rows and B = the edges' 16 inputs, C = P_i + b_E; then A = identity rows and B = the 32
+// channels of Q_j, 16 per MFMA (lane half h holds channels 8h..8h+7 of each half), so that Q_j is
+// added on the matrix unit; relu(z) is summed over k in the accumulator layout. Accumulator element
+// v of lane l is row 8*(v/4) + 4*(l/32) + v%4 of edge column l%32; rows are mapped to channels by
+// c(8g + 4h + q) = 16h + 4g + q inside the 32-channel tile, so that lane half h holds the 16
+// consecutive channels 16h..16h+15 (16-byte loads of P, 16-byte stores of m). The list words and edge inputs of the block's 32 slots are shared by its four
 // wavefronts through L1. The list word, edge inputs and Q rows of step k+1 are requested before
 // step k is computed, the list word of step k+2 before that.
 struct MsgStep {
@@ -482,16 +492,17 @@ __device__ inline void msg_load(const MsgArgs& a, uint64_t sl, bool ok, uint32_t
 #pragma unroll
     for (int q = 0; q < 8; q++) st.eb[q] = (__bf16)0.0f;
   }
-  const uint16_t* qrow = a.pq + (int64_t)j * H2 + H + ch0 + 16 * half;
+  const uint16_t* qrow = a.pq + (int64_t)j * H2 + H + ch0 + 8 * half;
   st.q[0] = *(const uint4*)(qrow);
-  st.q[1] = *(const uint4*)(qrow + 8);
+  st.q[1] = *(const uint4*)(qrow + 16);
 }
 __device__ inline float bf_lo(uint32_t v) { return bf2f((uint16_t)(v & 0xffffu)); }
 __device__ inline float bf_hi(uint32_t v) { return bf2f((uint16_t)(v >> 16)); }
 
 __global__ __launch_bounds__(256) void k_msg_fwd_mfma(MsgArgs a, const uint16_t* __restrict__ wEb) {
   const int lane = threadIdx.x & 63;
-  const int64_t i0 = (int64_t)blockIdx.x * 32;
+  // XCD-aware block order: consecutive particle groups on one XCD share its L2 for the Q gathers
+  const int64_t i0 = xcd_block(blockIdx.x, gridDim.x) * 32;
   const int ch0 = (int)(threadIdx.x >> 6) * 32;
   if (i0 >= a.n) return;
   const int col = lane & 31, half = lane >> 5;
@@ -505,6 +516,16 @@ __global__ __launch_bounds__(256) void k_msg_fwd_mfma(MsgArgs a, const uint16_t*
   // A rows: MFMA row `col` of the tile is channel c(col)
   const int crow = 16 * ((col >> 2) & 1) + 4 * (col >> 3) + (col & 3);
   const bf16x8 wa = *(const bf16x8*)(wEb + (ch0 + crow) * FE + 8 * half);
+  // identity rows: MFMA row `col` (channel crow) picks input channel crow from the Q fragments
+  uint32_t ia[4], ib[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int k0 = 8 * half + 2 * q;
+    ia[q] = (crow == k0 ? 0x3f80u : 0u) | (crow == k0 + 1 ? 0x3f800000u : 0u);
+    ib[q] = (crow == 16 + k0 ? 0x3f80u : 0u) | (crow == 16 + k0 + 1 ? 0x3f800000u : 0u);
+  }
+  const bf16x8 wia = __builtin_bit_cast(bf16x8, uint4{ia[0], ia[1], ia[2], ia[3]});
+  const bf16x8 wib = __builtin_bit_cast(bf16x8, uint4{ib[0], ib[1], ib[2], ib[3]});
   const int c0 = ch0 + 16 * half;
   floatx16 cinit, msum;
   {
@@ -532,15 +553,12 @@ __global__ __launch_bounds__(256) void k_msg_fwd_mfma(MsgArgs a, const uint16_t*
       msg_load(a, sl1, k + 1 < cnt, wnext, half, ch0, (uint32_t)ic, nxt);
       wnext = (k + 2 < kmax && k + 2 < cnt) ? a.nl.word[sl1 + kWave] : 0u;
     }
-    const floatx16 d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa, cur.eb, cinit, 0, 0, 0);
+    floatx16 d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa, cur.eb, cinit, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wia, __builtin_bit_cast(bf16x8, cur.q[0]), d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wib, __builtin_bit_cast(bf16x8, cur.q[1]), d, 0, 0, 0);
     if (cur.ok) {
-      const uint32_t qw[8] = {cur.q[0].x, cur.q[0].y, cur.q[0].z, cur.q[0].w,
-                              cur.q[1].x, cur.q[1].y, cur.q[1].z, cur.q[1].w};
 #pragma unroll
-      for (int v = 0; v < 8; v++) {
-        msum[2 * v] += fmaxf(d[2 * v] + bf_lo(qw[v]), 0.f);
-        msum[2 * v + 1] += fmaxf(d[2 * v + 1] + bf_hi(qw[v]), 0.f);
-      }
+      for (int v = 0; v < 16; v++) msum[v] += fmaxf(d[v], 0.f);
     }
     cur = nxt;
   }
@@ -548,7 +566,7 @@ __global__ __launch_bounds__(256) void k_msg_fwd_mfma(MsgArgs a, const uint16_t*
   uint32_t o[8];
 #pragma unroll
   for (int v = 0; v < 8; v++)
-    o[v] = (uint32_t)f2bf(msum[2 * v] * kMsgScale) | ((uint32_t)f2bf(msum[2 * v + 1] * kMsgScale) << 16);
+    o[v] = pack_bf16(msum[2 * v] * kMsgScale, msum[2 * v + 1] * kMsgScale);
   *(uint4*)(a.m + i * H + c0) = uint4{o[0], o[1], o[2], o[3]};
   *(uint4*)(a.m + i * H + c0 + 8) = uint4{o[4], o[5], o[6], o[7]};
 }
@@ -606,8 +624,8 @@ __global__ __launch_bounds__(256) void k_msg_bwd(MsgArgs a) {
       dQ0 += edge_z(Pi0, Qj0, er, w0, b0) > 0.f ? dmi0 : 0.f;
       dQ1 += edge_z(Pi1, Qj1, er, w1, b1) > 0.f ? dmi1 : 0.f;
     }
-    *(uint32_t*)(a.dpq + j * H2 + c) = (uint32_t)f2bf(dP0) | ((uint32_t)f2bf(dP1) << 16);
-    *(uint32_t*)(a.dpq + j * H2 + H + c) = (uint32_t)f2bf(dQ0) | ((uint32_t)f2bf(dQ1) << 16);
+    *(uint32_t*)(a.dpq + j * H2 + c) = pack_bf16(dP0, dP1);
+    *(uint32_t*)(a.dpq + j * H2 + H + c) = pack_bf16(dQ0, dQ1);
   }
   // block reduction of the W_E / b_E gradients in wavefront order
 #pragma unroll
@@ -631,6 +649,7 @@ struct DecArgs {
   int64_t n;
   const uint16_t* h;
   const float* w;  // W_out rounded [FO][H]
+  const uint16_t* wb;  // W_out, bfloat16 [FO][H] (forward on MFMA)
   const float* b;
   float* out;           // [n][FO]
   const float* target;  // training: [n][FO]
@@ -707,69 +726,77 @@ __global__ __launch_bounds__(256) void k_decode(DecArgs a) {
   }
 }
 
-// encoder: 16 lanes per particle, 8 channels each, so that a wavefront stores four whole 256-byte
-// rows; the weights sit transposed in LDS ([input][channel]) so that the 16 lanes of a particle read
-// 128 consecutive floats
-__global__ __launch_bounds__(256) void k_encode8(EncArgs a) {
-  __shared__ float wt[FN * H];
-  __shared__ float bb[H];
-  for (int k = threadIdx.x; k < FN * H; k += blockDim.x) wt[(k % FN) * H + k / FN] = a.w[k];
-  for (int k = threadIdx.x; k < H; k += blockDim.x) bb[k] = a.b[k];
-  __syncthreads();
-  const int c0 = (int)(threadIdx.x & 15) * 8;
-  const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 4);
-  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; i < a.n; i += stride) {
-    const uint4 xp = *(const uint4*)(a.x + i * FN);
-    uint16_t xb[FN];
-    memcpy(xb, &xp, 16);
-    float s[8];
+
+// Encoder on MFMA: h0 = relu(x W_in^T + b_in), one wavefront per 32 particles. The B operand is a
+// particle's 8 inputs (lane half 0) and a constant 1 at k = 8 (lane half 1), whose weight is the
+// bias, so the bias is part of the product. MFMA rows map to channels as in the dense products:
+// lane half h holds channels 32t + 16h .. +15 of its particle, stored with 16-byte writes.
+__global__ __launch_bounds__(256) void k_encode_mfma(EncArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t p0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
+  if (p0 >= a.n) return;
+  const int col = lane & 31, half = lane >> 5;
+  const int crow = 16 * ((col >> 2) & 1) + 4 * (col >> 3) + (col & 3);
+  const int64_t p = p0 + col;
+  const int64_t pc = p < a.n ? p : a.n - 1;
+  const uint4 xin = *(const uint4*)(a.x + pc * FN);
+  const bf16x8 xb = __builtin_bit_cast(bf16x8, half ? uint4{0x3f80u, 0u, 0u, 0u} : xin);
+  bf16x8 wa[4];
 #pragma unroll
-    for (int q = 0; q < 8; q++) s[q] = 0.f;
+  for (int t = 0; t < 4; t++) {
+    const int ch = 32 * t + crow;
+    const uint4 wrow = *(const uint4*)(a.w + ch * FN);
+    wa[t] = __builtin_bit_cast(bf16x8, half ? uint4{(uint32_t)a.b[ch], 0u, 0u, 0u} : wrow);
+  }
+  floatx16 zero;
 #pragma unroll
-    for (int k = 0; k < FN; k++) {
-      const float xv = bf2f(xb[k]);
+  for (int v = 0; v < 16; v++) zero[v] = 0.f;
 #pragma unroll
-      for (int q = 0; q < 8; q++) s[q] = fmaf(wt[k * H + c0 + q], xv, s[q]);
+  for (int t = 0; t < 4; t++) {
+    const floatx16 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[t], xb, zero, 0, 0, 0);
+    uint32_t o[8];
+#pragma unroll
+    for (int v = 0; v < 8; v++)
+      o[v] = pack_bf16(fmaxf(acc[2 * v], 0.f), fmaxf(acc[2 * v + 1], 0.f));
+    if (p < a.n) {
+      uint4* dst = (uint4*)(a.h + p * H + 32 * t + 16 * half);
+      dst[0] = uint4{o[0], o[1], o[2], o[3]};
+      dst[1] = uint4{o[4], o[5], o[6], o[7]};
     }
-    uint16_t o[8];
-#pragma unroll
-    for (int q = 0; q < 8; q++) o[q] = f2bf(fmaxf(s[q] + bb[c0 + q], 0.f));
-    uint4 op;
-    memcpy(&op, o, 16);
-    *(uint4*)(a.h + i * H + c0) = op;
   }
 }
 
-// decoder (forward only): 16 lanes per particle, each with 8 channels (one coalesced 256-byte row per
-// 16 lanes), partial dot products reduced over the 16 lanes
-__global__ __launch_bounds__(256) void k_decode1(DecArgs a) {
-  __shared__ float wo[FO * H];
-  for (int k = threadIdx.x; k < FO * H; k += blockDim.x) wo[k] = a.w[k];
-  __syncthreads();
-  const int c0 = (int)(threadIdx.x & 15) * 8;
-  const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 4);
-  const int64_t nr = (a.n + stride - 1) / stride * stride;  // every lane runs the same trip count
-  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; i < nr; i += stride) {
-    float y[FO];
+// Decoder on MFMA (forward): out = h W_out^T + b_out, one wavefront per 32 particles. The FO = 6
+// outputs are rows 0..5 of one 32-row tile (rows 6..31 have zero weights); k step s takes channels
+// 16s + 8h .. +7 in lane half h. Accumulator element v of lane l is output row 8(v/4) + 4(l/32) + v%4.
+__global__ __launch_bounds__(256) void k_decode_mfma(DecArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t p0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
+  if (p0 >= a.n) return;
+  const int col = lane & 31, half = lane >> 5;
+  const int64_t p = p0 + col;
+  const int64_t pc = p < a.n ? p : a.n - 1;
+  bf16x8 hf[H / 16], wf[H / 16];
+  const int wr = col < FO ? col : 0;  // rows past FO read row 0 and are zeroed
 #pragma unroll
-    for (int o = 0; o < FO; o++) y[o] = 0.f;
-    if (i < a.n) {
-      const uint4 hp = *(const uint4*)(a.h + i * H + c0);
-      uint16_t hb[8];
-      memcpy(hb, &hp, 16);
+  for (int s = 0; s < H / 16; s++) {
+    hf[s] = *(const bf16x8*)(a.h + pc * H + 16 * s + 8 * half);
+    const uint4 w = *(const uint4*)(a.wb + wr * H + 16 * s + 8 * half);
+    wf[s] = __builtin_bit_cast(bf16x8, col < FO ? w : uint4{0u, 0u, 0u, 0u});
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  floatx16 acc;
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
-        const float hv = bf2f(hb[q]);
+  for (int v = 0; v < 16; v++) acc[v] = 0.f;
 #pragma unroll
-        for (int o = 0; o < FO; o++) y[o] = fmaf(wo[o * H + c0 + q], hv, y[o]);
-      }
-    }
+  for (int s = 0; s < H / 16; s++) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[s], hf[s], acc, 0, 0, 0);
+  if (p >= a.n) return;
+  if (half == 0) {
 #pragma unroll
-    for (int o = 0; o < FO; o++)
-      for (int m = 8; m > 0; m >>= 1) y[o] += __shfl_xor(y[o], m, 16);
-    if (i < a.n && (threadIdx.x & 15) == 0)
+    for (int r = 0; r < 4; r++) a.out[p * FO + r] = acc[r] + a.b[r];
+  } else {
 #pragma unroll
-      for (int o = 0; o < FO; o++) a.out[i * FO + o] = y[o] + a.b[o];
+    for (int r = 0; r < 2; r++) a.out[p * FO + 4 + r] = acc[r] + a.b[4 + r];
   }
 }
 
@@ -1156,8 +1183,8 @@ int forward(xdem_gnn* G) {
   FeatArgs fa{n, live(c), c->grid, nl, (float)G->cfg.r0, (float)G->cfg.v0, (float)G->cfg.ov_scale, G->x, G->e};
   hipLaunchKernelGGL(k_features, dim3(nb(n, 256)), dim3(256), 0, c->stream, fa);
   if (tr) hipLaunchKernelGGL(k_mirror_all, dim3(nb(n, 256)), dim3(256), 0, c->stream, n, nl, G->mirror);
-  EncArgs ea{n, G->x, G->wr + G->off.W_in, G->wr + G->off.b_in, G->h[0]};
-  hipLaunchKernelGGL(k_encode8, dim3((int)std::min<int64_t>(nb(n * 16, 256), 2048)), dim3(256), 0, c->stream, ea);
+  EncArgs ea{n, G->x, G->wbf + G->off.W_in, G->wbf + G->off.b_in, G->h[0]};
+  hipLaunchKernelGGL(k_encode_mfma, dim3(nb(n, 128)), dim3(256), 0, c->stream, ea);
   XDEM_HIP(c, hipGetLastError());
   for (int l = 0; l < L; l++) {
     uint16_t* hin = tr ? G->h[l] : G->h[l & 1];
@@ -1179,9 +1206,9 @@ int forward(xdem_gnn* G) {
     g2.u_out = tr ? G->u[l] : nullptr; g2.h_out = hout;
     if ((rc = gemm_p<EPI_UPDATE, H, H, H>(G, g2))) return rc;
   }
-  DecArgs da{n, tr ? G->h[L] : G->h[L & 1], G->wr + G->off.W_out, G->wr + G->off.b_out, G->out, nullptr, nullptr,
+  DecArgs da{n, tr ? G->h[L] : G->h[L & 1], G->wr + G->off.W_out, G->wbf + G->off.W_out, G->wr + G->off.b_out, G->out, nullptr, nullptr,
              nullptr};
-  hipLaunchKernelGGL(k_decode1, dim3((int)std::min<int64_t>(nb(n * 16, 256), 2048)), dim3(256), 0, c->stream, da);
+  hipLaunchKernelGGL(k_decode_mfma, dim3(nb(n, 128)), dim3(256), 0, c->stream, da);
   XDEM_HIP(c, hipGetLastError());
   G->fwd_valid = true;
   G->fwd_n = n;
@@ -1199,7 +1226,7 @@ int backward(xdem_gnn* G, double* loss) {
   // decoder: loss, dh_L, dW_out, db_out
   const int dlen = FO * H + FO + 1;
   if ((rc = ensure_partial(G, (int64_t)wgrid * dlen))) return rc;
-  DecArgs da{n, G->h[L], G->wr + G->off.W_out, G->wr + G->off.b_out, G->out, G->target, G->dh, G->partial};
+  DecArgs da{n, G->h[L], G->wr + G->off.W_out, G->wbf + G->off.W_out, G->wr + G->off.b_out, G->out, G->target, G->dh, G->partial};
   hipLaunchKernelGGL(k_decode<true>, dim3(wgrid), dim3(256), 0, c->stream, da);
   // reduce into g[W_out..b_out] (contiguous) and the loss (one float after)
   hipLaunchKernelGGL(k_sum_partials, dim3(nb(dlen, 256)), dim3(256), 0, c->stream, G->partial, wgrid, (int64_t)dlen,

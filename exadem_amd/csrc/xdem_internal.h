@@ -93,6 +93,14 @@ struct alignas(128) Core {
 };
 static_assert(sizeof(Core) == 128, "one particle record per 128-byte line");
 
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of particles so
+// that neighbour gathers of consecutive blocks share that XCD's L2.
+__device__ inline int64_t xcd_block(int64_t b, int64_t nb) {
+  const int64_t q = nb >> 3, r = nb & 7;
+  const int64_t x = b & 7, k = b >> 3;
+  return x * q + (x < r ? x : r) + k;
+}
+
 // Partner records are read either from a block's LDS copy or from global memory. The two reads go
 // through pointers typed with their address space: through plain (generic) pointers the compiler
 // may fold the two branches into one load through a selected pointer, i.e. a flat load, which also

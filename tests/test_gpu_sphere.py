@@ -12,6 +12,7 @@ from exadem_amd import native
 from exadem_amd.sim import SphereSim, scheme
 import pyoracle as po
 
+from exadem_amd import inputs
 from helpers import (by_id, dense_deposition, force, interaction_table, periodic_pack, pos, rel_err, vec, vel,
                      wall_deposition)
 
@@ -223,3 +224,75 @@ def test_fp32_form_contact_set_and_force_error(gpu_ok):
     err = rel_err(force(pb), force(pa), 9.81 * np.ones(len(pa["id"])))
     assert err <= 1e-5, err
     assert float(np.max(np.abs(pos(pb) - pos(pa)))) <= 1e-9
+
+
+def shell_cluster(n_shell=220, seed=5):
+    """One sphere of radius 4 with n_shell spheres of radius 0.5 on a Fibonacci shell around it, each
+    overlapping it by 0.05: the large sphere's list holds every shell sphere (longer than one 64-slot
+    tile of the list), small spheres touch their shell neighbours too. No gravity, no walls."""
+    k = np.arange(n_shell) + 0.5
+    th = np.arccos(1.0 - 2.0 * k / n_shell)
+    ph = np.pi * (1.0 + 5.0 ** 0.5) * k
+    R = 4.0 + 0.5 - 0.05
+    c = 12.0
+    rx = np.concatenate([[c], c + R * np.sin(th) * np.cos(ph)])
+    ry = np.concatenate([[c], c + R * np.sin(th) * np.sin(ph)])
+    rz = np.concatenate([[c], c + R * np.cos(th)])
+    rad = np.concatenate([[4.0], np.full(n_shell, 0.5)])
+    v = inputs.normal(seed, 3, 3 * (n_shell + 1), 0.0, 0.05).reshape(3, -1)
+    p = inputs.sphere_set(rx, ry, rz, rad, 2500.0, v[0], v[1], v[2])
+    dom = native.domain((0.0, 0.0, 0.0), (24.0, 24.0, 24.0), (False, False, False), 8.1)
+    cp = native.contact_params(kn=1e5, mu=0.3, damp_rate=0.9)
+    return inputs.Workload("shell", p, dom, [], cp, cp, 1e-5, 40, 0.1, (0.0, 0.0, 0.0))
+
+
+def test_long_list_matches_oracle(gpu_ok):
+    """A particle with ~220 list entries (several 64-slot tiles): streams, history and 40 steps."""
+    w = shell_cluster()
+    o = oracle_for(w)
+    g = gpu_for(w)
+    o.first_iteration(scheme(w))
+    g.first_iteration()
+    gi, gj, gt, gf = g.ctx.interactions()
+    assert int(np.sum(gi == 0)) + int(np.sum(gj == 0)) >= 200  # the large sphere (id 0) takes part in >= 200
+    pg, pq = g.ctx.download(), o.particles()
+    floor = by_id(pq)["mass"] * 1e-3
+    compare_state(pg, pq, floor, "shell first pass")
+    orb = o.run_steps(scheme(w), 40)
+    for _ in range(40):
+        g.step()
+    compare_state(g.ctx.download(), o.particles(), floor, "shell 40 steps")
+    assert g.rebuilds == orb
+    kg, fg = interaction_table(*g.ctx.interactions())
+    oi, oj, ot, of, om = o.interactions()
+    ko, fo = interaction_table(oi, oj, ot, of)
+    assert np.array_equal(kg, ko)
+    assert np.array_equal(fg, fo)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2])
+def test_tiny_inputs(gpu_ok, n):
+    """Empty input, a lone sphere and one touching pair: the operators and the fused engine run, and
+    the state matches the oracle (free fall under gravity for the lone sphere)."""
+    rx = np.array([5.0, 5.9])[:n]
+    p = inputs.sphere_set(rx, np.full(n, 5.0), np.full(n, 5.0), np.full(n, 0.5), 2500.0,
+                          np.array([0.1, -0.1])[:n], np.zeros(n), np.zeros(n))
+    dom = native.domain((0.0, 0.0, 0.0), (10.0, 10.0, 10.0), (False, False, False), 1.1)
+    cp = native.contact_params(kn=1e5, mu=0.3, damp_rate=0.9)
+    w = inputs.Workload("tiny", p, dom, [], cp, cp, 1e-5, 20, 0.1, (0.0, 0.0, -9.81))
+    a, b = gpu_for(w), gpu_for(w)
+    a.first_iteration()
+    b.first_iteration()
+    for _ in range(20):
+        a.step()
+    b.run(20, poll=4)
+    pa, pb = a.ctx.download(), b.ctx.download()
+    assert len(pa["id"]) == n and len(pb["id"]) == n
+    for k in ("rx", "rz", "vx", "vz", "fx", "fz"):
+        assert np.array_equal(pa[k], pb[k]), k
+    if n == 0:
+        return
+    o = oracle_for(w)
+    o.first_iteration(scheme(w))
+    o.run_steps(scheme(w), 20)
+    compare_state(pa, o.particles(), np.full(n, 9.81), f"{n} particles 20 steps")
