@@ -223,21 +223,20 @@ struct GemmArgs {
 // The k order is permuted the same way on both operands: k step s = 2t + u takes, in lane half h,
 // the 8 inputs 32t + 16h + 8u .. +7, so the fragments a lane loads for k steps 2t and 2t+1 are the
 // 16 channels it writes for output tile t (the residual of the update comes from registers).
-// Persistent blocks of 8 wavefronts: each block owns 128 of the M output columns (blockIdx.y),
-// stages those rows of B once (rows padded by 8 elements against LDS bank conflicts; 68 KB for
-// K = 256, so two blocks, 16 wavefronts, fit a CU) and walks strips of 256 particles (32 per
-// wavefront).
+// Persistent blocks of 8 wavefronts: each block stages all M rows of B once (rows padded by 8
+// elements against LDS bank conflicts; 68-70 KB, so two blocks, 16 wavefronts, fit a CU) and walks
+// strips of 256 particles (32 per wavefront); a wavefront loads its particles' rows once and forms
+// the M columns in passes of 128.
 template <int EPI, int M, int K1, int K2>
 __global__ __launch_bounds__(512) void k_gemm_p(GemmArgs g) {
-  constexpr int K = K1 + K2, KP = K + 8, NT = 4;
+  constexpr int K = K1 + K2, KP = K + 8, NT = 4, NPASS = M / 128;
   extern __shared__ uint16_t w_lds[];
-  const int cb = (int)blockIdx.y * 128;
   // LDS row t*32 + m holds the weight row of MFMA row m of tile t, i.e. column 32t + c(m): lanes then
   // read consecutive LDS rows, which the 8-element padding spreads over distinct banks
-  for (int idx = threadIdx.x; idx < 128 * K / 8; idx += 512) {
+  for (int idx = threadIdx.x; idx < M * K / 8; idx += 512) {
     const int r = idx / (K / 8), c8 = (idx % (K / 8)) * 8;
     const int c = r & 31, m = 8 * ((c >> 2) & 3) + 4 * (c >> 4) + (c & 3);
-    *(uint4*)(w_lds + ((r & ~31) + m) * KP + c8) = *(const uint4*)(g.b + (int64_t)(cb + r) * K + c8);
+    *(uint4*)(w_lds + ((r & ~31) + m) * KP + c8) = *(const uint4*)(g.b + (int64_t)r * K + c8);
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -250,11 +249,6 @@ __global__ __launch_bounds__(512) void k_gemm_p(GemmArgs g) {
     if (r0 >= g.n) continue;
     const int64_t r = r0 + col;  // this lane's particle
     const int64_t ra = r < g.n ? r : g.n - 1;
-    floatx16 acc[NT];
-#pragma unroll
-    for (int t = 0; t < NT; t++)
-#pragma unroll
-      for (int v = 0; v < 16; v++) acc[t][v] = 0.f;
     // every fragment of the particle rows is requested before the first MFMA
     bf16x8 xf[K / 16];
     const uint16_t* ap1 = g.a1 + ra * K1 + 16 * half;
@@ -270,61 +264,71 @@ __global__ __launch_bounds__(512) void k_gemm_p(GemmArgs g) {
     // to its use, and the k steps then wait for memory one after the other (the update product, with
     // twice the fragments, is left to the scheduler: holding all of them costs it occupancy)
     if (EPI != EPI_UPDATE) __builtin_amdgcn_sched_barrier(0);
+    // the M output columns in passes of 128 (four 32-column tiles), the fragments held across passes
 #pragma unroll
-    for (int ks = 0; ks < K / 16; ks++) {
-      const int kw = 32 * (ks >> 1) + 16 * half + 8 * (ks & 1);
+    for (int ps = 0; ps < NPASS; ps++) {
+      const int cb = ps * 128;
+      floatx16 acc[NT];
 #pragma unroll
-      for (int t = 0; t < NT; t++) {
-        const bf16x8 wv = *(const bf16x8*)(w_lds + (t * 32 + col) * KP + kw);
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, xf[ks], acc[t], 0, 0, 0);
+      for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) acc[t][v] = 0.f;
+#pragma unroll
+      for (int ks = 0; ks < K / 16; ks++) {
+        const int kw = 32 * (ks >> 1) + 16 * half + 8 * (ks & 1);
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const bf16x8 wv = *(const bf16x8*)(w_lds + ((4 * ps + t) * 32 + col) * KP + kw);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, xf[ks], acc[t], 0, 0, 0);
+        }
       }
-    }
-    if (r >= g.n) continue;
+      // stores only: a lane that left early would run the next pass's MFMAs with part of EXEC
 #pragma unroll
-    for (int t = 0; t < NT; t++) {
-      const int c0 = cb + t * 32 + 16 * half;  // columns c0 .. c0+15
-      if (EPI == EPI_BF16) {
-        uint32_t o[8];
+      for (int t = 0; t < NT && r < g.n; t++) {
+        const int c0 = cb + t * 32 + 16 * half;  // columns c0 .. c0+15
+        if (EPI == EPI_BF16) {
+          uint32_t o[8];
 #pragma unroll
-        for (int v = 0; v < 8; v++) o[v] = pack_bf16(acc[t][2 * v], acc[t][2 * v + 1]);
-        uint4* dst = (uint4*)(g.ybf + r * M + c0);
-        dst[0] = uint4{o[0], o[1], o[2], o[3]};
-        dst[1] = uint4{o[4], o[5], o[6], o[7]};
-      } else if (EPI == EPI_UPDATE) {
-        const uint4 h0 = __builtin_bit_cast(uint4, xf[2 * t]), h1 = __builtin_bit_cast(uint4, xf[2 * t + 1]);
-        const uint32_t hw[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
-        uint32_t uo[8], ho[8];
+          for (int v = 0; v < 8; v++) o[v] = pack_bf16(acc[t][2 * v], acc[t][2 * v + 1]);
+          uint4* dst = (uint4*)(g.ybf + r * M + c0);
+          dst[0] = uint4{o[0], o[1], o[2], o[3]};
+          dst[1] = uint4{o[4], o[5], o[6], o[7]};
+        } else if (EPI == EPI_UPDATE) {
+          const uint4 h0 = __builtin_bit_cast(uint4, xf[2 * t]), h1 = __builtin_bit_cast(uint4, xf[2 * t + 1]);
+          const uint32_t hw[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
+          uint32_t uo[8], ho[8];
 #pragma unroll
-        for (int v = 0; v < 8; v++) {
-          uo[v] = pack_bf16(acc[t][2 * v] + g.bias[c0 + 2 * v], acc[t][2 * v + 1] + g.bias[c0 + 2 * v + 1]);
-          const float u0 = bf2f((uint16_t)(uo[v] & 0xffffu)), u1 = bf2f((uint16_t)(uo[v] >> 16));
-          ho[v] = pack_bf16(bf2f((uint16_t)(hw[v] & 0xffffu)) + fmaxf(u0, 0.f),
-                            bf2f((uint16_t)(hw[v] >> 16)) + fmaxf(u1, 0.f));
-        }
-        if (g.u_out) {
-          uint4* ud = (uint4*)(g.u_out + r * H + c0);
-          ud[0] = uint4{uo[0], uo[1], uo[2], uo[3]};
-          ud[1] = uint4{uo[4], uo[5], uo[6], uo[7]};
-        }
-        uint4* hd = (uint4*)(g.h_out + r * H + c0);
-        hd[0] = uint4{ho[0], ho[1], ho[2], ho[3]};
-        hd[1] = uint4{ho[4], ho[5], ho[6], ho[7]};
-      } else if (EPI == EPI_BWD_U && c0 >= H) {
-        float4* dst = (float4*)(g.dm + r * H + (c0 - H));
+          for (int v = 0; v < 8; v++) {
+            uo[v] = pack_bf16(acc[t][2 * v] + g.bias[c0 + 2 * v], acc[t][2 * v + 1] + g.bias[c0 + 2 * v + 1]);
+            const float u0 = bf2f((uint16_t)(uo[v] & 0xffffu)), u1 = bf2f((uint16_t)(uo[v] >> 16));
+            ho[v] = pack_bf16(bf2f((uint16_t)(hw[v] & 0xffffu)) + fmaxf(u0, 0.f),
+                              bf2f((uint16_t)(hw[v] >> 16)) + fmaxf(u1, 0.f));
+          }
+          if (g.u_out) {
+            uint4* ud = (uint4*)(g.u_out + r * H + c0);
+            ud[0] = uint4{uo[0], uo[1], uo[2], uo[3]};
+            ud[1] = uint4{uo[4], uo[5], uo[6], uo[7]};
+          }
+          uint4* hd = (uint4*)(g.h_out + r * H + c0);
+          hd[0] = uint4{ho[0], ho[1], ho[2], ho[3]};
+          hd[1] = uint4{ho[4], ho[5], ho[6], ho[7]};
+        } else if (EPI == EPI_BWD_U && c0 >= H) {
+          float4* dst = (float4*)(g.dm + r * H + (c0 - H));
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-          dst[q] = float4{acc[t][4 * q] * kMsgScale, acc[t][4 * q + 1] * kMsgScale, acc[t][4 * q + 2] * kMsgScale,
-                          acc[t][4 * q + 3] * kMsgScale};
-      } else {  // EPI_ADD, and the dh half of EPI_BWD_U
-        float4* dst = (float4*)(g.dh + r * H + c0);
+          for (int q = 0; q < 4; q++)
+            dst[q] = float4{acc[t][4 * q] * kMsgScale, acc[t][4 * q + 1] * kMsgScale, acc[t][4 * q + 2] * kMsgScale,
+                            acc[t][4 * q + 3] * kMsgScale};
+        } else {  // EPI_ADD, and the dh half of EPI_BWD_U
+          float4* dst = (float4*)(g.dh + r * H + c0);
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          float4 cur = dst[q];
-          cur.x += acc[t][4 * q];
-          cur.y += acc[t][4 * q + 1];
-          cur.z += acc[t][4 * q + 2];
-          cur.w += acc[t][4 * q + 3];
-          dst[q] = cur;
+          for (int q = 0; q < 4; q++) {
+            float4 cur = dst[q];
+            cur.x += acc[t][4 * q];
+            cur.y += acc[t][4 * q + 1];
+            cur.z += acc[t][4 * q + 2];
+            cur.w += acc[t][4 * q + 3];
+            dst[q] = cur;
+          }
         }
       }
     }
@@ -1109,12 +1113,12 @@ int gemm_p(xdem_gnn* G, const GemmArgs& a) {
   if (a.n <= 0) return XDEM_OK;
   constexpr int K = K1 + K2;
   static_assert(M % 128 == 0, "column groups of 128");
-  const int lds = 128 * (K + 8) * (int)sizeof(uint16_t);
+  const int lds = M * (K + 8) * (int)sizeof(uint16_t);
   XDEM_HIP(c, hipFuncSetAttribute((const void*)k_gemm_p<EPI, M, K1, K2>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   const int64_t ntiles = (a.n + 255) / 256;
-  const int grid = (int)std::min<int64_t>(ntiles, 512 / (M / 128));
-  hipLaunchKernelGGL((k_gemm_p<EPI, M, K1, K2>), dim3(grid, M / 128), dim3(512), lds, c->stream, a);
+  const int grid = (int)std::min<int64_t>(ntiles, 512);
+  hipLaunchKernelGGL((k_gemm_p<EPI, M, K1, K2>), dim3(grid), dim3(512), lds, c->stream, a);
   XDEM_HIP(c, hipGetLastError());
   return XDEM_OK;
 }
