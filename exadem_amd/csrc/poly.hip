@@ -670,7 +670,8 @@ __device__ inline bool poly_guard(const PolyForceArgs& a) {
 template <bool KR, bool GUARD>
 __global__ __launch_bounds__(128) void k_poly_feat(PolyForceArgs a) {
   if (GUARD && poly_guard(a)) return;
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // XCD-aware block order: consecutive features (owners close in the order) share an XCD's L2
+  const int64_t f = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (f >= a.pl.nfeat) return;
   const PState& s = a.s;
   const uint32_t sl = a.pl.fslot[f];
@@ -767,7 +768,8 @@ template <bool FUSED, bool KR, bool GUARD>
 __global__ __launch_bounds__(256) void k_poly_gather(PolyForceArgs a) {
   // the stall decision was taken by k_poly_feat of the same step
   if (GUARD && a.flags[1]) return;
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // XCD-aware block order (partner contributions are read through the mirror slots)
+  const int64_t x = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (x >= a.n) return;
   const PState& s = a.s;
   const Core cx = s.core[x];
