@@ -172,13 +172,17 @@ def main():
                          "max_force_rel_err": ferr, "contact_set_identical": bool(same_set),
                          "contacts": int(np.count_nonzero(act_a)),
                          "error_stated_as": "max over particles of |F32 - F64| / max(|F64|, m g), first force pass"}
-        # surrogate-accelerated form (trained weights of the network, K = 10 DEM steps per step)
+        # surrogate-accelerated forms: the trained network with K = 100 and K = 10 DEM steps per step
         from exadem_amd import surrogate
-        if os.path.exists(surrogate.WEIGHTS):
-            params, scales = surrogate.load()
+        for form_key, wpath in (("surrogate", surrogate.WEIGHTS_K100), ("surrogate_k10", surrogate.WEIGHTS)):
+            if not os.path.exists(wpath):
+                forms[form_key] = {"value": None, "status": "no trained weights at " + wpath}
+                continue
+            params, scales = surrogate.load(wpath)
+            K = scales["substeps"]
             s = SphereSim(w)
             s.first_iteration()
-            surrogate.settle(s, scales["substeps"])
+            surrogate.settle(s, K)
             g = surrogate.attach(s.ctx, params, scales)
             sch = scheme(w)
             g.surrogate_run(sch, 20, check_every=20, tol=0.01)
@@ -187,21 +191,29 @@ def main():
             sst = g.surrogate_run(sch, args.sur_steps, check_every=20, tol=0.01)
             s.ctx.synchronize()
             el = time.perf_counter() - t1
-            K = scales["substeps"]
+            # network against the DEM on two windows of K steps from the state the run left
+            werr = [e for e in (surrogate.window_errors(s, g, K) for _ in range(2)) if e is not None]
             eff = n * args.sur_steps * K / el
-            forms["surrogate"] = {"value": eff, "unit": "effective particle-steps/s (simulated time x n / dt)",
-                                  "ms_per_surrogate_step": 1e3 * el / args.sur_steps, "substeps": K,
-                                  "speedup_vs_fp64": eff / value,
-                                  "simulated_time_per_s": args.sur_steps * K * w.dt / el,
-                                  "checks": sst["checks"], "fallbacks": sst["fallbacks"],
-                                  "max_pos_err_over_radius": sst["max_err"],
-                                  "max_pos_err_over_radius_free_flight": sst["max_err_free"],
-                                  "error_stated_as": "largest |x_surrogate - x_DEM| / R over the drift checks "
-                                                     "(one surrogate step against K DEM steps from the same state)"}
-            log(f"surrogate: {args.sur_steps} steps in {el:.3f} s, checks {sst['checks']}, "
-                f"max err {sst['max_err']:.3g} R")
-        else:
-            forms["surrogate"] = {"value": None, "status": "no trained weights at " + surrogate.WEIGHTS}
+            forms[form_key] = {"value": eff, "unit": "effective particle-steps/s (simulated time x n / dt)",
+                               "ms_per_surrogate_step": 1e3 * el / args.sur_steps, "substeps": K,
+                               "speedup_vs_fp64": eff / value,
+                               "simulated_time_per_s": args.sur_steps * K * w.dt / el,
+                               "checks": sst["checks"], "fallbacks": sst["fallbacks"],
+                               "max_pos_err_over_radius": sst["max_err"],
+                               "max_pos_err_over_radius_free_flight": sst["max_err_free"],
+                               "rms_pos_err_over_radius": sst["rms_err"],
+                               "rms_pos_err_over_radius_free_flight": sst["rms_err_free"],
+                               "rms_contact_force_err": werr[-1][0] if werr else None,
+                               "rms_contact_force_free_flight": werr[-1][1] if werr else None,
+                               "error_stated_as": "largest and RMS |x_surrogate - x_DEM| / R over the drift checks "
+                                                  "(one surrogate step against K DEM steps from the same state); "
+                                                  "RMS error of the predicted mean contact force over a window, "
+                                                  "units m0 a0, against zero contact force (free flight)"}
+            log(f"{form_key}: {args.sur_steps} steps of K={K} in {el:.3f} s, checks {sst['checks']}, "
+                f"max err {sst['max_err']:.3g} R (free flight {sst['max_err_free']:.3g} R), "
+                f"rms {sst['rms_err']:.3g} R (free flight {sst['rms_err_free']:.3g} R), "
+                f"force err {werr[-1][0] if werr else float('nan'):.3g} vs {werr[-1][1] if werr else float('nan'):.3g}")
+            del g, s
         # polyhedra: cubes (edge 1) and octahedra (circumradius 0.8), Minkowski radius 0.02, random
         # orientations, jittered lattice (spacing 1.55, solid fraction 0.25) periodic in x and y with
         # a floor and a lid, gravity, dt 2e-6

@@ -970,7 +970,9 @@ __global__ __launch_bounds__(256) void k_sur_advance(int64_t n, PState s, const 
 // largest |x_dem - x_ref[id]| / R per block (two references)
 __global__ __launch_bounds__(256) void k_drift(int64_t n, PState s, const double* __restrict__ xs,
                                                const double* __restrict__ xf, double* __restrict__ part) {
-  __shared__ double r1[256], r2[256];
+  // per block: largest and summed squared |x - x_ref| / R against the surrogate (xs) and the free
+  // flight (xf) positions
+  __shared__ double r[4][256];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double e1 = 0, e2 = 0;
   if (i < n) {
@@ -982,20 +984,21 @@ __global__ __launch_bounds__(256) void k_drift(int64_t n, PState s, const double
     e1 = norm(d1) / c.xr.w;
     e2 = norm(d2) / c.xr.w;
   }
-  r1[threadIdx.x] = e1;
-  r2[threadIdx.x] = e2;
+  r[0][threadIdx.x] = e1;
+  r[1][threadIdx.x] = e2;
+  r[2][threadIdx.x] = e1 * e1;
+  r[3][threadIdx.x] = e2 * e2;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
     if ((int)threadIdx.x < st) {
-      r1[threadIdx.x] = fmax(r1[threadIdx.x], r1[threadIdx.x + st]);
-      r2[threadIdx.x] = fmax(r2[threadIdx.x], r2[threadIdx.x + st]);
+      r[0][threadIdx.x] = fmax(r[0][threadIdx.x], r[0][threadIdx.x + st]);
+      r[1][threadIdx.x] = fmax(r[1][threadIdx.x], r[1][threadIdx.x + st]);
+      r[2][threadIdx.x] += r[2][threadIdx.x + st];
+      r[3][threadIdx.x] += r[3][threadIdx.x + st];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = r1[0];
-    part[2 * blockIdx.x + 1] = r2[0];
-  }
+  if (threadIdx.x < 4) part[4 * blockIdx.x + threadIdx.x] = r[threadIdx.x][0];
 }
 
 }  // namespace gnn
@@ -1306,7 +1309,7 @@ extern "C" int xdem_gnn_create(xdem_ctx* c, const xdem_gnn_config* cfg, xdem_gnn
   int rc;
   if ((rc = galloc(c, &G->p, np)) || (rc = galloc(c, &G->g, np)) || (rc = galloc(c, &G->m1, np)) ||
       (rc = galloc(c, &G->m2, np)) || (rc = galloc(c, &G->wbf, np)) || (rc = galloc(c, &G->wr, np)) ||
-      (rc = galloc(c, &G->wT, (int64_t)L * 2 * H * H2)) || (rc = galloc(c, &G->dpart, 2 * 65536)) ||
+      (rc = galloc(c, &G->wT, (int64_t)L * 2 * H * H2)) || (rc = galloc(c, &G->dpart, 4 * 65536)) ||
       (rc = galloc(c, &G->dec_sum, FO * H + FO + 1)) || (rc = ensure_partial(G, kPartialFloats))) {
     delete G;
     return rc;
@@ -1317,7 +1320,7 @@ extern "C" int xdem_gnn_create(xdem_ctx* c, const xdem_gnn_config* cfg, xdem_gnn
   XDEM_HIP(c, hipMemsetAsync(G->m2, 0, sizeof(float) * np, c->stream));
   hipLaunchKernelGGL(k_round_params, dim3(nb(np, 256)), dim3(256), 0, c->stream, np, G->p, G->wbf, G->wr);
   if ((rc = refresh_weights(G))) return rc;
-  XDEM_HIP(c, hipHostMalloc((void**)&G->h_dpart, sizeof(double) * 2 * 65536, 0));
+  XDEM_HIP(c, hipHostMalloc((void**)&G->h_dpart, sizeof(double) * 4 * 65536, 0));
   XDEM_HIP(c, hipHostMalloc((void**)&G->h_loss, sizeof(float) * 4, 0));
   XDEM_HIP(c, hipStreamSynchronize(c->stream));
   *out = G;
@@ -1501,21 +1504,25 @@ int maybe_rebuild(xdem_gnn* G, const xdem_scheme* s, int64_t* rebuilds) {
   return XDEM_OK;
 }
 
-int drift(xdem_gnn* G, double* e_sur, double* e_free) {
+int drift(xdem_gnn* G, double* e_sur, double* e_free, double* r_sur, double* r_free) {
   xdem_ctx* c = G->c;
   const int64_t n = c->n;
   const int blocks = nb(n, 256);
   if (blocks > 65536) return set_err(c, XDEM_ERR_CAPACITY, "surrogate: too many particles for the drift check");
   hipLaunchKernelGGL(k_drift, dim3(blocks), dim3(256), 0, c->stream, n, live(c), G->xs, G->xf, G->dpart);
-  XDEM_HIP(c, hipMemcpyAsync(G->h_dpart, G->dpart, sizeof(double) * 2 * blocks, hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipMemcpyAsync(G->h_dpart, G->dpart, sizeof(double) * 4 * blocks, hipMemcpyDeviceToHost, c->stream));
   XDEM_HIP(c, hipStreamSynchronize(c->stream));
-  double a = 0, b = 0;
+  double a = 0, b = 0, sa = 0, sb = 0;
   for (int k = 0; k < blocks; k++) {
-    a = std::max(a, G->h_dpart[2 * k]);
-    b = std::max(b, G->h_dpart[2 * k + 1]);
+    a = std::max(a, G->h_dpart[4 * k]);
+    b = std::max(b, G->h_dpart[4 * k + 1]);
+    sa += G->h_dpart[4 * k + 2];
+    sb += G->h_dpart[4 * k + 3];
   }
   *e_sur = a;
   *e_free = b;
+  *r_sur = n > 0 ? std::sqrt(sa / (double)n) : 0.0;
+  *r_free = n > 0 ? std::sqrt(sb / (double)n) : 0.0;
   return XDEM_OK;
 }
 }  // namespace
@@ -1555,11 +1562,13 @@ extern "C" int xdem_surrogate_run(xdem_gnn* G, const xdem_scheme* s, int64_t nst
       st.dem_steps += K;
       st.rebuilds += rs.rebuilds;
       if (check) {
-        double e1 = 0, e2 = 0;
-        if ((rc = drift(G, &e1, &e2))) return rc;
+        double e1 = 0, e2 = 0, q1 = 0, q2 = 0;
+        if ((rc = drift(G, &e1, &e2, &q1, &q2))) return rc;
         st.checks++;
         st.max_err = std::max(st.max_err, e1);
         st.max_err_free = std::max(st.max_err_free, e2);
+        st.rms_err = std::max(st.rms_err, q1);
+        st.rms_err_free = std::max(st.rms_err_free, q2);
         if (e1 > tol) {
           st.fallbacks++;
           dem_until = k + 1 + check_every;

@@ -70,7 +70,7 @@ class GnnConfig(C.Structure):
 class SurrogateStats(C.Structure):
     _fields_ = [("surrogate_steps", C.c_int64), ("dem_steps", C.c_int64), ("checks", C.c_int64),
                 ("fallbacks", C.c_int64), ("rebuilds", C.c_int64), ("max_err", C.c_double),
-                ("max_err_free", C.c_double)]
+                ("max_err_free", C.c_double), ("rms_err", C.c_double), ("rms_err_free", C.c_double)]
 
 
 GNN_NPARAMS = 205446

@@ -19,6 +19,8 @@ from .sim import SphereSim, scheme
 
 HIDDEN, LAYERS, FN, FE, FO = 128, 3, 8, 16, 6
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "surrogate_polydisperse.npz")
+# the same network trained on windows of K = 100 DEM steps (fine-tuned from the K = 10 weights)
+WEIGHTS_K100 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "surrogate_polydisperse_k100.npz")
 
 
 def param_shapes():
@@ -168,9 +170,11 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--time-budget", type=float, default=None)
     ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--init", default=None, help="start from the parameters of this weights file")
     args = ap.parse_args()
+    init = load(args.init)[0] if args.init else None
     params, scales, hist = train(args.n, args.runs, args.steps, args.substeps, args.lr, seed=args.seed,
-                                 time_budget=args.time_budget)
+                                 time_budget=args.time_budget, params=init)
     save(args.out, params, scales, hist)
     print(f"[surrogate] saved {args.out}", flush=True)
 

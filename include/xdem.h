@@ -290,6 +290,8 @@ typedef struct xdem_surrogate_stats {
   int64_t rebuilds;        /* neighbour rebuilds                                     */
   double max_err;          /* largest checked position error / radius                */
   double max_err_free;     /* same check for free flight under gravity (no network)   */
+  double rms_err;          /* largest (over checks) RMS position error / radius       */
+  double rms_err_free;     /* same for free flight                                    */
 } xdem_surrogate_stats;
 
 /* Creates a network bound to a context (parameters zero until set). */

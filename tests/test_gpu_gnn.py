@@ -150,3 +150,27 @@ def test_surrogate_run_with_drift_checks(gpu_ok):
     assert st2["fallbacks"] >= 1 and st2["dem_steps"] > 40
     p = sim.ctx.download()
     assert np.all(np.isfinite(p["rx"])) and np.all(np.isfinite(p["vx"]))
+
+
+@pytest.mark.parametrize("weights", ["k10", "k100"])
+def test_shipped_weights_beat_free_flight(gpu_ok, weights):
+    """The shipped networks (K = 10 and K = 100 DEM steps per surrogate step) on a packing they were
+    not trained on: drift-checked run without fallback, RMS position error at the checks and RMS
+    contact-force error on a window below the free-flight (zero contact force) values."""
+    path = surrogate.WEIGHTS if weights == "k10" else surrogate.WEIGHTS_K100
+    params, scales = surrogate.load(path)
+    K = scales["substeps"]
+    assert K == (10 if weights == "k10" else 100)
+    w = surrogate.training_workload(20000, seed=4242)
+    sim = SphereSim(w)
+    sim.first_iteration()
+    surrogate.settle(sim, K)
+    g = surrogate.attach(sim.ctx, params, scales)
+    st = g.surrogate_run(scheme(w), 20, check_every=10, tol=0.01)
+    assert st["checks"] == 2 and st["fallbacks"] == 0
+    assert 0 < st["rms_err"] < st["rms_err_free"], st
+    errs = [e for e in (surrogate.window_errors(sim, g, K) for _ in range(2)) if e is not None]
+    assert errs
+    print(f"{weights}: rms position error {st['rms_err']:.3g} R (free flight {st['rms_err_free']:.3g} R), "
+          f"contact-force error {errs[-1][0]:.3g} (free flight {errs[-1][1]:.3g}) m0 a0")
+    assert errs[-1][0] < 0.3 * errs[-1][1], errs
