@@ -78,6 +78,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a device copy of 1 GiB (counter calibration for --pmc runs) and exit")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--timing-every", type=int, default=8,
+                    help="HIP-event timing of the kernels of every k-th timed step (an event pair costs the stream "
+                         "a few microseconds; 1 times every step)")
     ap.add_argument("--form", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--no-forms", action="store_true", help="skip the FP32-force and surrogate lines")
     ap.add_argument("--sur-steps", type=int, default=100, help="timed surrogate steps")
@@ -117,7 +120,7 @@ def main():
         sim.first_iteration()
         sim.run(args.warmup, poll=args.poll)
         sim.ctx.synchronize()
-        sim.ctx.set_timing(True)
+        sim.ctx.set_timing(True, every=args.timing_every)
         sim.ctx.kernel_timing(0, reset=True)
         sim.ctx.kernel_timing(1, reset=True)
         sim.ctx.synchronize()
@@ -223,7 +226,7 @@ def main():
         ps.first_iteration()
         ps.run(20, poll=args.poll)
         ps.ctx.synchronize()
-        ps.ctx.set_timing(True)
+        ps.ctx.set_timing(True, every=args.timing_every)
         ps.ctx.kernel_timing(0, reset=True)
         ps.ctx.kernel_timing(1, reset=True)
         t1 = time.perf_counter()
@@ -286,6 +289,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": f"k_force_{args.form} (fused force pass)", "avg_ms": f_ms, "launches": f_cnt,
+                     "timed_launches": f"HIP events around the launches of every {args.timing_every}-th timed step",
                      "bytes_per_launch": force_bytes, "integrate_avg_ms": i_ms,
                      "step_bytes": step_bytes, "step_frac": step_bytes / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS},
         "cpu_baseline": cpu,

@@ -753,15 +753,16 @@ int xdem_run_steps(xdem_ctx* c, const xdem_scheme* s, int64_t nsteps, xdem_run_s
     for (int64_t step = done; step < upto; step++) {
       const bool kick = c->pending_kick;
       const double kdt = c->pending_dt;
-      if (c->timing && (rc = record_timer(c, 0, true))) return rc;
+      const bool timed = c->timing && step % c->timing_every == 0;
+      if (timed && (rc = record_timer(c, 0, true))) return rc;
       const bool poly = c->nshapes > 0;
       // spheres test the centre displacement in the prolog; polyhedra test vertices (vertex_displ_over)
       if ((rc = launch_prolog(c, s->dt, kick, kdt, !poly, thr * thr, true))) return rc;
       if (poly && (rc = poly_compute_vertices(c, true, thr * thr, true))) return rc;
-      if (c->timing && (rc = record_timer(c, 0, false))) return rc;
-      if (c->timing && (rc = record_timer(c, 1, true))) return rc;
+      if (timed && (rc = record_timer(c, 0, false))) return rc;
+      if (timed && (rc = record_timer(c, 1, true))) return rc;
       if ((rc = engine_force(c, cp, cpd, s, g, true, (int32_t)step))) return rc;
-      if (c->timing && (rc = record_timer(c, 1, false))) return rc;
+      if (timed && (rc = record_timer(c, 1, false))) return rc;
       c->pending_kick = true;
       c->pending_dt = s->dt;
     }
@@ -818,7 +819,9 @@ int xdem_kernel_timing(xdem_ctx* c, int32_t which, int32_t reset, double* avg_ms
 
 int xdem_set_timing(xdem_ctx* c, int32_t on) {
   if (!c) return XDEM_ERR_ARG;
+  if (on < 0) return set_err(c, XDEM_ERR_ARG, "set_timing: on must be >= 0");
   c->timing = on != 0;
+  c->timing_every = on > 0 ? on : 1;
   return XDEM_OK;
 }
 

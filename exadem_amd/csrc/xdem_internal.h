@@ -289,6 +289,7 @@ struct xdem_ctx {
   double pending_dt = 0;
 
   bool timing = false;
+  int32_t timing_every = 1;  // time the launches of every timing_every-th step
   xdem::KernelTimer timers[2];
 
   // polyhedra (set when shapes are registered)

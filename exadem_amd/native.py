@@ -335,8 +335,9 @@ class Context:
         self.call("xdem_run_steps", C.byref(scheme), nsteps, C.byref(st))
         return {"steps": st.steps, "rebuilds": st.rebuilds, "polls": st.polls}
 
-    def set_timing(self, on):
-        self.call("xdem_set_timing", 1 if on else 0)
+    def set_timing(self, on, every=1):
+        """HIP-event timing of the run's integrate and force launches, of every `every`-th step."""
+        self.call("xdem_set_timing", max(1, int(every)) if on else 0)
 
     def kernel_timing(self, which, reset=True):
         ms = C.c_double()

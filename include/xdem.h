@@ -242,7 +242,9 @@ int xdem_nbh_stats(xdem_ctx* ctx, int64_t* pairs, int64_t* slots, int32_t dims[3
 int xdem_streams_dump(xdem_ctx* ctx, int64_t cap, uint64_t* id_a, uint64_t* id_b, int32_t* code, uint8_t* owner,
                       int64_t* written);
 
-/* Enables (on != 0) HIP-event timing of every integrate and force launch of xdem_run_steps. */
+/* HIP-event timing of the integrate and force launches of xdem_run_steps: on = 0 disables it, on = k
+ * >= 1 times the launches of every k-th step (an event pair costs the stream a few microseconds, so a
+ * sampled timing leaves the step time of the run nearly unchanged). */
 int xdem_set_timing(xdem_ctx* ctx, int32_t on);
 
 /* Device time of the last kernel launches of the fused engine, in

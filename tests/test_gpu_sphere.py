@@ -296,3 +296,22 @@ def test_tiny_inputs(gpu_ok, n):
     o.first_iteration(scheme(w))
     o.run_steps(scheme(w), 20)
     compare_state(pa, o.particles(), np.full(n, 9.81), f"{n} particles 20 steps")
+
+
+def test_sampled_kernel_timing(gpu_ok):
+    """set_timing(every=k) times the integrate and force launches of every k-th step of the engine."""
+    w = periodic_pack(n=2000)
+    s = gpu_for(w)
+    s.first_iteration()
+    s.ctx.set_timing(True, every=4)
+    s.ctx.kernel_timing(0, reset=True)
+    s.ctx.kernel_timing(1, reset=True)
+    s.run(16, poll=8)
+    s.ctx.set_timing(False)
+    f_ms, f_cnt = s.ctx.kernel_timing(1)
+    i_ms, i_cnt = s.ctx.kernel_timing(0)
+    # steps 0, 4, 8, 12 (a step re-issued after a rebuild stall is timed again)
+    assert 4 <= f_cnt <= 8 and f_cnt == i_cnt
+    assert f_ms > 0 and i_ms > 0
+    with pytest.raises(native.XdemError):
+        s.ctx.call("xdem_set_timing", -1)
