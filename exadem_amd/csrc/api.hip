@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <unordered_set>
+#include <vector>
 
 #include "xdem_internal.h"
 
@@ -82,6 +83,8 @@ int ensure_capacity(xdem_ctx* c, int64_t n) {
     if (rc) return rc;
   }
   int rc;
+  dfree(&c->core_alt);
+  if ((rc = dalloc(c, &c->core_alt, n))) return rc;
   if ((rc = dalloc(c, &c->cell_of, n))) return rc;
   if ((rc = dalloc(c, &c->perm, n))) return rc;
   if ((rc = dalloc(c, &c->perm_tmp, n))) return rc;
@@ -313,8 +316,6 @@ int xdem_ctx_create(int32_t device, xdem_ctx** out) {
   }
   (void)hipMemsetAsync(c->d_flags, 0, 4 * sizeof(int32_t), c->stream);
   (void)hipStreamSynchronize(c->stream);
-  c->force_per_lane = true;
-  if (const char* e = std::getenv("XDEM_FORCE_KERNEL")) c->force_per_lane = std::string(e) != "tile";
   *out = c;
   return XDEM_OK;
 }
@@ -329,6 +330,7 @@ void xdem_ctx_destroy(xdem_ctx* c) {
     free_wl(c->wl[k]);
     for (auto& e : c->timers[k].ev) (void)hipEventDestroy(e);
   }
+  dfree(&c->core_alt);
   dfree(&c->cell_of); dfree(&c->cell_count); dfree(&c->cell_start); dfree(&c->cell_rank); dfree(&c->perm);
   dfree(&c->perm_tmp);
   dfree(&c->sort_key);
@@ -745,37 +747,62 @@ int xdem_run_steps(xdem_ctx* c, const xdem_scheme* s, int64_t nsteps, xdem_run_s
   const double thr = 0.5 * s->rcut_inc;
   const int poll = s->poll > 0 ? s->poll : 16;
   XDEM_HIP(c, hipMemsetAsync(c->d_flags, 0, 4 * sizeof(int32_t), c->stream));
+  c->trig_slot = 0;
+  // FP64 spheres run a step as one kernel (force pass + kick + next prolog, launch_step) between a
+  // chain-start prolog and the last step of the run, which keeps the separate kernels so that the run
+  // ends in the same state (pending kick flushed) as the operator sequence
+  const bool fuse = c->nshapes == 0 && s->form == XDEM_FORM_FP64;
+  std::vector<Core*> input(poll);  // record buffer read by the integrating pass of each step of a chunk
+  bool chain = false;              // the live records hold the prolog of the next step
   int64_t done = 0, rebuilds = 0, polls = 0;
   while (done < nsteps) {
-    // issue up to `poll` steps: K1 (kicks of the previous pass + prolog + displacement test) and
-    // K2 (force pass). K2 turns a raised trigger into a stall that makes every later launch a no-op.
+    // issue up to `poll` steps. A step is either K1 (kicks of the previous pass + prolog +
+    // displacement test) and K2 (force pass), or, inside a chain, one integrating pass. A force pass
+    // turns a raised trigger into a stall that makes every later launch a no-op.
     const int64_t upto = std::min<int64_t>(nsteps, done + poll);
     for (int64_t step = done; step < upto; step++) {
       const bool kick = c->pending_kick;
       const double kdt = c->pending_dt;
       const bool timed = c->timing && step % c->timing_every == 0;
-      if (timed && (rc = record_timer(c, 0, true))) return rc;
       const bool poly = c->nshapes > 0;
-      // spheres test the centre displacement in the prolog; polyhedra test vertices (vertex_displ_over)
-      if ((rc = launch_prolog(c, s->dt, kick, kdt, !poly, thr * thr, true))) return rc;
-      if (poly && (rc = poly_compute_vertices(c, true, thr * thr, true))) return rc;
-      if (timed && (rc = record_timer(c, 0, false))) return rc;
+      input[step - done] = nullptr;
+      if (!chain) {
+        if (timed && (rc = record_timer(c, 0, true))) return rc;
+        // spheres test the centre displacement in the prolog; polyhedra test vertices (vertex_displ_over)
+        if ((rc = launch_prolog(c, s->dt, kick, kdt, !poly, thr * thr, true))) return rc;
+        if (poly && (rc = poly_compute_vertices(c, true, thr * thr, true))) return rc;
+        if (timed && (rc = record_timer(c, 0, false))) return rc;
+        c->trig_slot = 0;
+      }
       if (timed && (rc = record_timer(c, 1, true))) return rc;
-      if ((rc = engine_force(c, cp, cpd, s, g, true, (int32_t)step))) return rc;
+      if (fuse && step + 1 < nsteps) {
+        if ((rc = launch_step(c, cp, cpd, s->dt, s->gravity_on != 0, g, thr * thr, (int32_t)step, &input[step - done])))
+          return rc;
+        chain = true;
+        c->pending_kick = false;
+      } else {
+        if ((rc = engine_force(c, cp, cpd, s, g, true, (int32_t)step))) return rc;
+        chain = false;
+        c->pending_kick = true;
+        c->pending_dt = s->dt;
+      }
       if (timed && (rc = record_timer(c, 1, false))) return rc;
-      c->pending_kick = true;
-      c->pending_dt = s->dt;
     }
     XDEM_HIP(c, hipMemcpyAsync(c->h_flags, c->d_flags, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     XDEM_HIP(c, hipStreamSynchronize(c->stream));
     polls++;
     if (c->h_flags[1]) {
-      // step `stalled` ran its prolog, then found the trigger raised: rebuild and redo its force pass
+      // step `stalled` has its prolog done and found the trigger raised: put back the record buffer
+      // that holds that state (integrating passes after it were no-ops), rebuild, redo its force pass
       const int64_t stalled = (int64_t)c->h_flags[2];
+      Core* state = stalled >= done && stalled < upto ? input[stalled - done] : nullptr;
+      if (state && live(c).core != state) std::swap(live(c).core, c->core_alt);
       XDEM_HIP(c, hipMemsetAsync(c->d_flags, 0, 4 * sizeof(int32_t), c->stream));
+      c->trig_slot = 0;
       if ((rc = rebuild(c, s->rcut_inc))) return rc;
       rebuilds++;
       if ((rc = engine_force(c, cp, cpd, s, g, false, 0))) return rc;
+      chain = false;
       c->pending_kick = true;
       c->pending_dt = s->dt;
       done = stalled + 1;
@@ -783,6 +810,7 @@ int xdem_run_steps(xdem_ctx* c, const xdem_scheme* s, int64_t nsteps, xdem_run_s
       done = upto;
     }
   }
+  c->trig_slot = 0;
   if ((rc = flush_kick(c))) return rc;
   if (c->timing && (rc = harvest_timers(c))) return rc;
   if (stats) *stats = {nsteps, rebuilds, polls};

@@ -11,6 +11,7 @@
 // The fused variant also performs reset_force_moment + gravity_force before the contacts and
 // force_to_accel + the angular-acceleration part of combined_compute_epilog after them.
 #include "contact_law.h"
+#include "step.h"
 
 namespace xdem {
 
@@ -49,6 +50,16 @@ struct ForceArgs {
   int gravity_on;
   int32_t* flags;
   int32_t step_tag;
+  // flags slot of the rebuild trigger this pass obeys (raised by the pass before it)
+  int32_t trig_in;
+  // integrating pass only: the next state goes to core_out (the records of this pass are still being
+  // read as partners), kick dt/2 of the finished step, squared displacement threshold, and the
+  // trigger slot it raises (another slot than trig_in: blocks of this pass that start late must not
+  // see the trigger of this pass)
+  Core* core_out;
+  double kdt_2;
+  double thr2;
+  int32_t trig_out;
 };
 
 
@@ -79,11 +90,15 @@ __device__ inline void fetch_entry(const ForceArgs& a, uint64_t sl, uint32_t wd,
   }
 }
 
-template <bool FUSED, bool KR, bool WALLS, bool GUARD>
+// INTEG (with FUSED): after the particle's force, its epilog kick and the prolog of the next step
+// are applied and the next record is written to core_out, with the displacement test of the rebuild
+// trigger; accelerations, moments and angular accelerations are then not stored (the next pass
+// recomputes them).
+template <bool FUSED, bool KR, bool WALLS, bool GUARD, bool INTEG = false>
 __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
   if (GUARD) {
     if (a.flags[1]) return;
-    if (a.flags[0]) {
+    if (a.flags[a.trig_in]) {
       if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.flags[1] = 1;
         a.flags[2] = a.step_tag;
@@ -226,285 +241,14 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
       M = M + (cross(cpos - rx, f) + mint);
     }
   }
-  if (FUSED) {
-    s.ax[x] = F.x / mx;
-    s.ay[x] = F.y / mx;
-    s.az[x] = F.z / mx;
-    s.mx[x] = M.x;
-    s.my[x] = M.y;
-    s.mz[x] = M.z;
-    const d4 q = s.core[x].q;
-    const V3 al = angular_accel_d(Q4f{q.x, q.y, q.z, q.w}, M, wx, V3{s.ix[x], s.iy[x], s.iz[x]});
-    s.alx[x] = al.x;
-    s.aly[x] = al.y;
-    s.alz[x] = al.z;
-  } else {
-    s.ax[x] = F.x;
-    s.ay[x] = F.y;
-    s.az[x] = F.z;
-    s.mx[x] = M.x;
-    s.my[x] = M.y;
-    s.mz[x] = M.z;
-  }
-}
-
-// ---------------------------------------------------------------------------------------------
-// Compacted variant: one wavefront per ELL tile (64 particles).
-//   pass 1  every lane tests its entries k < cnt for overlap (loads only, independent iterations);
-//           the contact pattern of row k is the wavefront ballot rows[k]; prefix[k] counts the
-//           contacts of rows < k (tile contacts are numbered row-major: k, then lane);
-//   pass 2  the 64 lanes evaluate the tile's contacts 64 at a time in that numbering, write each
-//           contribution to LDS, and every lane then adds the contributions that belong to its own
-//           particle in increasing k (its stream order) -> same sums, same order as the per-lane loop.
-constexpr int kTileMaxK = 64;
-
-__device__ inline int nth_set_bit(uint64_t m, int r) {
-  // position of the r-th (0-based) set bit of m
-  int pos = 0;
-#pragma unroll
-  for (int w = 32; w >= 1; w >>= 1) {
-    const uint64_t low = m & ((w == 64 ? ~0ull : ((1ull << w) - 1ull)));
-    const int c = __popcll(low);
-    if (r >= c) {
-      r -= c;
-      m >>= w;
-      pos += w;
-    } else {
-      m = low;
-    }
-  }
-  return pos;
-}
-
-struct alignas(16) Contrib {
-  double fx, fy, fz, mx, my, mz;
-};
-
-template <bool FUSED, bool KR, bool WALLS, bool GUARD>
-__global__ __launch_bounds__(64) void k_force_fp64_tile(ForceArgs a) {
-  if (GUARD) {
-    if (a.flags[1]) return;
-    if (a.flags[0]) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.flags[1] = 1;
-        a.flags[2] = a.step_tag;
-      }
-      return;
-    }
-  }
-  __shared__ uint64_t rows[kTileMaxK];
-  __shared__ uint32_t prefix[kTileMaxK + 1];
-  __shared__ Contrib contrib[64];
-  const int64_t tile = xcd_block(blockIdx.x, gridDim.x);
-  const int lane = threadIdx.x;
-  const int64_t x = tile * 64 + lane;
-  const bool live = x < a.n;
-  const PState& s = a.s;
-  uint32_t* __restrict__ word = a.nl.word;
-  const Core* __restrict__ core = s.core;
-  const int K = (int)a.nl.kmax[tile];
-  const uint64_t tb = a.nl.base[tile];
-  const uint32_t cnt = live ? a.nl.cnt[x] : 0u;
-  d4 px = {0, 0, 0, 0};
-  if (live) px = core[x].xr;
-  const V3 rx{px.x, px.y, px.z};
-  const double Rx = px.w;
-  // ---- pass 1: overlap test of every entry
-  uint64_t mycontacts = 0;
-  uint32_t run = 0;
-  for (int k = 0; k < K; k++) {
-    bool contact = false;
-    if ((uint32_t)k < cnt) {
-      const uint64_t sl = tb + (uint64_t)k * kWave + lane;
-      const uint32_t wd = word[sl];
-      const uint32_t y = wd & kIdxMask;
-      const bool owner = (wd & kOwnerBit) != 0;
-      const d4 py = core[y].xr;
-      V3 ri, rj;
-      double Rs;
-      if (owner) {
-        const V3 sh = shift_vec(a.g, (int)((wd >> kShiftShift) & 31u));
-        ri = rx;
-        rj = {py.x + sh.x, py.y + sh.y, py.z + sh.z};
-        Rs = Rx + py.w;
-      } else {
-        ri = {py.x, py.y, py.z};
-        rj = rx;
-        Rs = py.w + Rx;
-      }
-      const V3 dist = ri - rj;
-      const double dn = norm(dist) - Rs;
-      contact = dn < 0;
-      if (!contact && (wd & kActiveBit)) {
-        a.nl.hx[sl] = 0.0;
-        a.nl.hy[sl] = 0.0;
-        a.nl.hz[sl] = 0.0;
-        if (KR) {
-          a.nl.mhx[sl] = 0.0;
-          a.nl.mhy[sl] = 0.0;
-          a.nl.mhz[sl] = 0.0;
-        }
-        word[sl] = wd & ~kActiveBit;
-      }
-    }
-    const uint64_t b = __ballot(contact);
-    if (contact) mycontacts |= 1ull << k;
-    if (lane == 0) {
-      rows[k] = b;
-      prefix[k] = run;
-    }
-    run += (uint32_t)__popcll(b);
-  }
-  if (lane == 0) prefix[K] = run;
-  __syncthreads();
-  const uint32_t total = run;
-  const uint64_t below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  // own accumulators
-  d4 pv = {0, 0, 0, 1};
-  d4 pw = {0, 0, 0, 0};
-  if (live) {
-    pv = s.core[x].vm;
-    pw = s.core[x].w;
-  }
-  const V3 vx{pv.x, pv.y, pv.z};
-  const V3 wx{pw.x, pw.y, pw.z};
-  const double mx = pv.w;
-  V3 F, M;
-  if (FUSED) {
-    F = {0.0, 0.0, 0.0};
-    M = {0.0, 0.0, 0.0};
-    if (a.gravity_on) F = {F.x + a.grav.x * mx, F.y + a.grav.y * mx, F.z + a.grav.z * mx};
-  } else if (live) {
-    F = {s.ax[x], s.ay[x], s.az[x]};
-    M = {s.mx[x], s.my[x], s.mz[x]};
-  } else {
-    F = {0, 0, 0};
-    M = {0, 0, 0};
-  }
-  // my next contact (k) and its tile-wide number
-  uint64_t pending = mycontacts;
-  for (uint32_t c0 = 0; c0 < total; c0 += 64) {
-    // ---- pass 2a: lane evaluates tile contact number g
-    const uint32_t g = c0 + (uint32_t)lane;
-    if (g < total) {
-      int k = 0;
-      while (prefix[k + 1] <= g) k++;
-      const int plane = nth_set_bit(rows[k], (int)(g - prefix[k]));
-      const int64_t xo = tile * 64 + plane;
-      const uint64_t sl = tb + (uint64_t)k * kWave + plane;
-      const uint32_t wd = word[sl];
-      const uint32_t y = wd & kIdxMask;
-      const bool owner = (wd & kOwnerBit) != 0;
-      const d4 qa = core[xo].xr;
-      const d4 qb = core[y].xr;
-      const d4 va = s.core[xo].vm;
-      const d4 vb = s.core[y].vm;
-      const d4 wa = s.core[xo].w;
-      const d4 wb = s.core[y].w;
-      V3 ri, rj, vi, vj, wi, wj;
-      double Ri, Rj, mi, mj;
-      if (owner) {
-        const V3 sh = shift_vec(a.g, (int)((wd >> kShiftShift) & 31u));
-        ri = {qa.x, qa.y, qa.z};
-        Ri = qa.w;
-        rj = {qb.x + sh.x, qb.y + sh.y, qb.z + sh.z};
-        Rj = qb.w;
-        vi = {va.x, va.y, va.z};
-        vj = {vb.x, vb.y, vb.z};
-        wi = {wa.x, wa.y, wa.z};
-        wj = {wb.x, wb.y, wb.z};
-        mi = va.w;
-        mj = vb.w;
-      } else {
-        ri = {qb.x, qb.y, qb.z};
-        Ri = qb.w;
-        rj = {qa.x, qa.y, qa.z};
-        Rj = qa.w;
-        vi = {vb.x, vb.y, vb.z};
-        vj = {va.x, va.y, va.z};
-        wi = {wb.x, wb.y, wb.z};
-        wj = {wa.x, wa.y, wa.z};
-        mi = vb.w;
-        mj = va.w;
-      }
-      const V3 dist = ri - rj;
-      const double dnorm = norm(dist);
-      const double dn = dnorm - (Ri + Rj);
-      const double inv = 1.0 / dnorm;
-      const V3 nrm = dist * inv;
-      const V3 cpos = ri - nrm * (Ri + 0.5 * dn);
-      const double meff = (mi * mj) / (mi + mj);
-      V3 ft{a.nl.hx[sl], a.nl.hy[sl], a.nl.hz[sl]};
-      V3 mint{0.0, 0.0, 0.0};
-      if (KR) mint = {a.nl.mhx[sl], a.nl.mhy[sl], a.nl.mhz[sl]};
-      const V3 f = hooke_force<KR>(dn, nrm, a.dt, a.cp, meff, ft, mint, cpos, ri, vi, wi, rj, vj, wj);
-      a.nl.hx[sl] = ft.x;
-      a.nl.hy[sl] = ft.y;
-      a.nl.hz[sl] = ft.z;
-      if (KR) {
-        a.nl.mhx[sl] = mint.x;
-        a.nl.mhy[sl] = mint.y;
-        a.nl.mhz[sl] = mint.z;
-      }
-      const bool act = !(ft.x == 0 && ft.y == 0 && ft.z == 0 && mint.x == 0 && mint.y == 0 && mint.z == 0);
-      const uint32_t nwd = act ? (wd | kActiveBit) : (wd & ~kActiveBit);
-      if (nwd != wd) word[sl] = nwd;
-      const V3 ra{qa.x, qa.y, qa.z};
-      Contrib cb;
-      if (owner) {
-        const V3 t = cross(cpos - ra, f) + mint;
-        cb = {f.x, f.y, f.z, t.x, t.y, t.z};
-      } else {
-        const V3 nf = -f;
-        const V3 t = cross(cpos - ra, nf) + (-mint);
-        cb = {nf.x, nf.y, nf.z, t.x, t.y, t.z};
-      }
-      contrib[lane] = cb;
-    }
-    __syncthreads();
-    // ---- pass 2b: add my contributions of this chunk, in stream order
-    while (pending) {
-      const int k = __ffsll((unsigned long long)pending) - 1;
-      const uint32_t num = prefix[k] + (uint32_t)__popcll(rows[k] & below);
-      if (num >= c0 + 64) break;
-      const Contrib& cb = contrib[num - c0];
-      F = F + V3{cb.fx, cb.fy, cb.fz};
-      M = M + V3{cb.mx, cb.my, cb.mz};
-      pending &= pending - 1;
-    }
-    __syncthreads();
-  }
-  if (!live) return;
-  if (WALLS) {
-    const uint32_t mask = a.wl.mask[x];
-    for (int w = 0; w < a.nw; w++) {
-      if (!((mask >> w) & 1u)) continue;
-      const Surface& sf = a.walls[w];
-      const int64_t j = (int64_t)w * a.wstride + x;
-      const V3 proj = dot(rx, sf.normal) * sf.normal;
-      const V3 s2p = -(sf.center_proj - proj);
-      const double d = norm(s2p);
-      const double dn = d - Rx;
-      if (dn > 0) {
-        a.wl.hx[j] = 0.0; a.wl.hy[j] = 0.0; a.wl.hz[j] = 0.0;
-        a.wl.mhx[j] = 0.0; a.wl.mhy[j] = 0.0; a.wl.mhz[j] = 0.0;
-        continue;
-      }
-      const V3 nrm = s2p / d;
-      const V3 cpos = rx - nrm * (Rx + 0.5 * dn);
-      V3 ft{a.wl.hx[j], a.wl.hy[j], a.wl.hz[j]};
-      V3 mint{0.0, 0.0, 0.0};
-      if (KR) mint = {a.wl.mhx[j], a.wl.mhy[j], a.wl.mhz[j]};
-      const V3 f = hooke_force<KR>(dn, nrm, a.dt, a.cpd, mx, ft, mint, cpos, rx, vx, wx, sf.center, sf.vel, sf.vrot);
-      a.wl.hx[j] = ft.x; a.wl.hy[j] = ft.y; a.wl.hz[j] = ft.z;
-      if (KR) {
-        a.wl.mhx[j] = mint.x; a.wl.mhy[j] = mint.y; a.wl.mhz[j] = mint.z;
-      }
-      F = F + f;
-      M = M + (cross(cpos - rx, f) + mint);
-    }
-  }
-  if (FUSED) {
+  if (FUSED && INTEG) {
+    const V3 al = angular_accel_d(Q4f{me.q.x, me.q.y, me.q.z, me.q.w}, M, wx, V3{s.ix[x], s.iy[x], s.iz[x]});
+    Core nx = me;
+    kick_prolog<true>(nx.xr, nx.vm, nx.w, nx.q, F.x / mx, F.y / mx, F.z / mx, al.x, al.y, al.z, a.dt, a.kdt_2);
+    a.core_out[x] = nx;
+    const double dx = nx.xr.x - s.bx[x], dy = nx.xr.y - s.by[x], dz = nx.xr.z - s.bz[x];
+    if (dx * dx + dy * dy + dz * dz >= a.thr2) a.flags[a.trig_out] = 1;
+  } else if (FUSED) {
     s.ax[x] = F.x / mx;
     s.ay[x] = F.y / mx;
     s.az[x] = F.z / mx;
@@ -528,12 +272,6 @@ __global__ __launch_bounds__(64) void k_force_fp64_tile(ForceArgs a) {
 
 template <bool FUSED, bool KR, bool WALLS>
 static void launch_g(xdem_ctx* c, const ForceArgs& a, bool guarded) {
-  if (c->nl_max_rows <= kTileMaxK && !c->force_per_lane) {
-    const dim3 grid(grid_blocks(a.n, 64)), block(64);
-    if (guarded) hipLaunchKernelGGL((k_force_fp64_tile<FUSED, KR, WALLS, true>), grid, block, 0, c->stream, a);
-    else hipLaunchKernelGGL((k_force_fp64_tile<FUSED, KR, WALLS, false>), grid, block, 0, c->stream, a);
-    return;
-  }
   const dim3 grid(grid_blocks(a.n, 256)), block(256);
   if (guarded) hipLaunchKernelGGL((k_force_fp64<FUSED, KR, WALLS, true>), grid, block, 0, c->stream, a);
   else hipLaunchKernelGGL((k_force_fp64<FUSED, KR, WALLS, false>), grid, block, 0, c->stream, a);
@@ -552,6 +290,52 @@ static void launch_f(xdem_ctx* c, const ForceArgs& a, bool kr, bool walls, bool 
 
 int launch_force_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool fused,
                       bool gravity_on, V3 g, bool guarded, int32_t step_tag);
+
+// One step of the engine in one kernel: force pass of the current state, then each particle's kick
+// and next prolog into the other record buffer (c->core_alt), which becomes the live one. *input
+// receives the record buffer the pass read (it holds this step's state if the pass stalls).
+int launch_step(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool gravity_on, V3 g,
+                double thr2, int32_t step_tag, Core** input) {
+  if (!c->hist_ready) return set_err(c, XDEM_ERR_STATE, "contact: run nbh_sphere after chunk_neighbors_contact");
+  int rc = set_history_form(c, false);
+  if (rc) return rc;
+  if (cp.kr != 0.0 || cpd.kr != 0.0) c->kr_used = true;
+  const bool walls = c->nwalls > 0 && c->wl_valid;
+  ForceArgs a{};
+  a.n = c->n;
+  a.s = live(c);
+  a.nl = c->nl[c->nl_cur];
+  if (walls) a.wl = c->wl[c->wl_cur];
+  a.walls = c->d_walls;
+  a.nw = c->nwalls;
+  a.wstride = c->cap;
+  a.g = c->grid;
+  a.cp = cp;
+  a.cpd = cpd;
+  a.dt = dt;
+  a.grav = g;
+  a.gravity_on = gravity_on ? 1 : 0;
+  a.flags = c->d_flags;
+  a.step_tag = step_tag;
+  a.trig_in = c->trig_slot;
+  a.trig_out = c->trig_slot == 0 ? 3 : 0;
+  a.core_out = c->core_alt;
+  a.kdt_2 = dt * 0.5;
+  a.thr2 = thr2;
+  const dim3 grid(grid_blocks(a.n, 256)), block(256);
+  if (c->kr_used) {
+    if (walls) hipLaunchKernelGGL((k_force_fp64<true, true, true, true, true>), grid, block, 0, c->stream, a);
+    else hipLaunchKernelGGL((k_force_fp64<true, true, false, true, true>), grid, block, 0, c->stream, a);
+  } else {
+    if (walls) hipLaunchKernelGGL((k_force_fp64<true, false, true, true, true>), grid, block, 0, c->stream, a);
+    else hipLaunchKernelGGL((k_force_fp64<true, false, false, true, true>), grid, block, 0, c->stream, a);
+  }
+  XDEM_HIP(c, hipGetLastError());
+  *input = live(c).core;
+  std::swap(live(c).core, c->core_alt);
+  c->trig_slot = a.trig_out;
+  return XDEM_OK;
+}
 
 int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, int form, bool fused,
                  bool gravity_on, V3 g, bool guarded, int32_t step_tag) {
@@ -577,6 +361,7 @@ int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd,
   a.gravity_on = gravity_on ? 1 : 0;
   a.flags = c->d_flags;
   a.step_tag = step_tag;
+  a.trig_in = c->trig_slot;
   if (fused) launch_f<true>(c, a, c->kr_used, walls, guarded);
   else launch_f<false>(c, a, c->kr_used, walls, guarded);
   XDEM_HIP(c, hipGetLastError());

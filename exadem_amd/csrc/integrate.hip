@@ -7,23 +7,10 @@
 //   kick    the two epilog updates alone, when alpha was already produced by the fused force pass
 // The fused engine runs "kick(n-1) + prolog(n) + displacement test" as one kernel: the
 // per-particle operation sequence is the same as running the operators one after the other.
-#include "xdem_internal.h"
+#include "step.h"
 
 namespace xdem {
 
-struct Q4 {
-  double w, x, y, z;
-};
-
-// quat::dot (src/shape/include/exaDEM/type/quat.hpp:92-96): 0.5*(0,w)*q
-__device__ inline Q4 qdot(Q4 q, V3 w) {
-  return {-0.5 * (w.x * q.x + w.y * q.y + w.z * q.z), 0.5 * (w.y * q.z - w.z * q.y + w.x * q.w),
-          0.5 * (w.z * q.x - w.x * q.z + w.y * q.w), 0.5 * (w.x * q.y - w.y * q.x + w.z * q.w)};
-}
-__device__ inline Q4 qnormalize(Q4 q) {
-  const double n = sqrt(q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z);
-  return {q.w / n, q.x / n, q.y / n, q.z / n};
-}
 // quat * vec3 (src/shape/include/exaDEM/type/quat.hpp:84-91)
 __device__ inline V3 qrot(Q4 q, V3 v) {
   const V3 qv{q.x, q.y, q.z};
@@ -79,43 +66,16 @@ __global__ __launch_bounds__(kStageBS) void k_prolog(int64_t n, PState s, double
   const int t = threadIdx.x;
   const int64_t i = first + t;
   if (t < cnt) {
-    const double dt_2 = dt * 0.5;
-    const double dt2_2 = dt * dt * 0.5;
     uint4* row = lds + t * kStageRow;
     d4 xr = lds_get(row);
     d4 vm = lds_get(row + 2);
     d4 w = lds_get(row + 4);
-    const d4 qq = lds_get(row + 6);
-    const double ax = s.ax[i], ay = s.ay[i], az = s.az[i];
-    const double lx = s.alx[i], ly = s.aly[i], lz = s.alz[i];
-    if (KICK) {
-      // epilog of the previous step: angular velocity, then push_f_v
-      w.x = w.x + lx * kdt_2;
-      w.y = w.y + ly * kdt_2;
-      w.z = w.z + lz * kdt_2;
-      vm.x = vm.x + ax * kdt_2;
-      vm.y = vm.y + ay * kdt_2;
-      vm.z = vm.z + az * kdt_2;
-    }
-    // push_f_v_r, push_f_v
-    xr.x = xr.x + (vm.x * dt + ax * dt2_2);
-    xr.y = xr.y + (vm.y * dt + ay * dt2_2);
-    xr.z = xr.z + (vm.z * dt + az * dt2_2);
-    vm.x = vm.x + ax * dt_2;
-    vm.y = vm.y + ay * dt_2;
-    vm.z = vm.z + az * dt_2;
-    // push_to_quaternion
-    Q4 q{qq.x, qq.y, qq.z, qq.w};
-    const Q4 d = qdot(q, V3{w.x, w.y, w.z});
-    q = {q.w + d.w * dt, q.x + d.x * dt, q.y + d.y * dt, q.z + d.z * dt};
-    q = qnormalize(q);
-    w.x = w.x + dt_2 * lx;
-    w.y = w.y + dt_2 * ly;
-    w.z = w.z + dt_2 * lz;
+    d4 qq = lds_get(row + 6);
+    kick_prolog<KICK>(xr, vm, w, qq, s.ax[i], s.ay[i], s.az[i], s.alx[i], s.aly[i], s.alz[i], dt, kdt_2);
     lds_put(row, xr);
     lds_put(row + 2, vm);
     lds_put(row + 4, w);
-    lds_put(row + 6, d4{q.w, q.x, q.y, q.z});
+    lds_put(row + 6, qq);
     if (DISPL) {
       const double dx = xr.x - s.bx[i], dy = xr.y - s.by[i], dz = xr.z - s.bz[i];
       if (dx * dx + dy * dy + dz * dz >= thr2) flags[0] = 1;

@@ -271,7 +271,8 @@ struct xdem_ctx {
   bool kr_used = false;         // a rolling stiffness was used: moment histories are live
   bool hist_fp32 = false;       // the live history of the list is the FP32 copy
   int nl_max_rows = 0;          // largest per-wavefront row count of the current list
-  bool force_per_lane = false;  // select the per-lane force kernel (XDEM_FORCE_KERNEL=lane)
+  xdem::Core* core_alt = nullptr;  // second record buffer of the integrating force pass (ping-pong)
+  int32_t trig_slot = 0;        // d_flags slot of the pending rebuild trigger (0, or 3 after an integrating pass)
   double rcut_inc = 0;
   double nbh_dist = 0;
   xdem::WallList wl[2];
@@ -343,6 +344,10 @@ int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist);
 int do_nbh_sphere(xdem_ctx* c, double rcut_inc);
 int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, int form, bool fused,
                  bool gravity_on, V3 g, bool guarded, int32_t step_tag);
+// engine step of FP64 spheres in one kernel (force pass + kick + next prolog into c->core_alt, which
+// becomes live); *input = the record buffer the pass read
+int launch_step(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool gravity_on, V3 g,
+                double thr2, int32_t step_tag, Core** input);
 int alloc_walls(xdem_ctx* c, int nw);
 int set_history_form(xdem_ctx* c, bool fp32);
 int engine_rebuild(xdem_ctx* c, double rcut_inc);
