@@ -748,10 +748,10 @@ int xdem_run_steps(xdem_ctx* c, const xdem_scheme* s, int64_t nsteps, xdem_run_s
   const int poll = s->poll > 0 ? s->poll : 16;
   XDEM_HIP(c, hipMemsetAsync(c->d_flags, 0, 4 * sizeof(int32_t), c->stream));
   c->trig_slot = 0;
-  // FP64 spheres run a step as one kernel (force pass + kick + next prolog, launch_step) between a
+  // spheres run a step as one kernel (force pass + kick + next prolog, launch_step) between a
   // chain-start prolog and the last step of the run, which keeps the separate kernels so that the run
   // ends in the same state (pending kick flushed) as the operator sequence
-  const bool fuse = c->nshapes == 0 && s->form == XDEM_FORM_FP64;
+  const bool fuse = c->nshapes == 0;
   std::vector<Core*> input(poll);  // record buffer read by the integrating pass of each step of a chunk
   bool chain = false;              // the live records hold the prolog of the next step
   int64_t done = 0, rebuilds = 0, polls = 0;
@@ -776,7 +776,8 @@ int xdem_run_steps(xdem_ctx* c, const xdem_scheme* s, int64_t nsteps, xdem_run_s
       }
       if (timed && (rc = record_timer(c, 1, true))) return rc;
       if (fuse && step + 1 < nsteps) {
-        if ((rc = launch_step(c, cp, cpd, s->dt, s->gravity_on != 0, g, thr * thr, (int32_t)step, &input[step - done])))
+        if ((rc = launch_step(c, cp, cpd, s->dt, s->form, s->gravity_on != 0, g, thr * thr, (int32_t)step,
+                              &input[step - done])))
           return rc;
         chain = true;
         c->pending_kick = false;

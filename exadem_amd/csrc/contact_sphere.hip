@@ -294,9 +294,13 @@ int launch_force_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams&
 // One step of the engine in one kernel: force pass of the current state, then each particle's kick
 // and next prolog into the other record buffer (c->core_alt), which becomes the live one. *input
 // receives the record buffer the pass read (it holds this step's state if the pass stalls).
-int launch_step(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool gravity_on, V3 g,
-                double thr2, int32_t step_tag, Core** input) {
+int launch_step_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool gravity_on, V3 g,
+                     double thr2, int32_t step_tag, Core** input);
+
+int launch_step(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, int form, bool gravity_on,
+                V3 g, double thr2, int32_t step_tag, Core** input) {
   if (!c->hist_ready) return set_err(c, XDEM_ERR_STATE, "contact: run nbh_sphere after chunk_neighbors_contact");
+  if (form == XDEM_FORM_FP32) return launch_step_fp32(c, cp, cpd, dt, gravity_on, g, thr2, step_tag, input);
   int rc = set_history_form(c, false);
   if (rc) return rc;
   if (cp.kr != 0.0 || cpd.kr != 0.0) c->kr_used = true;

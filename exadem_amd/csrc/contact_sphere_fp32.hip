@@ -9,6 +9,7 @@
 // per slot instead of 24). Per-particle sums of the contact forces and moments are accumulated in FP32
 // with TwoSum compensation and added to the FP64 gravity term at the end.
 #include "contact_law.h"
+#include "step.h"
 
 namespace xdem {
 
@@ -89,6 +90,13 @@ struct ForceArgs32 {
   int gravity_on;
   int32_t* flags;
   int32_t step_tag;
+  int32_t trig_in;  // flags slot of the rebuild trigger this pass obeys
+  // integrating pass only (see ForceArgs in contact_sphere.hip): next records, kick dt/2, squared
+  // displacement threshold, trigger slot raised
+  Core* core_out;
+  double kdt_2;
+  double thr2;
+  int32_t trig_out;
 };
 
 
@@ -138,11 +146,13 @@ __device__ inline void fetch32(const ForceArgs32& a, uint64_t sl, uint32_t wd, E
   }
 }
 
-template <bool FUSED, bool KR, bool WALLS, bool GUARD>
+// INTEG (with FUSED): as k_force_fp64's integrating pass, the kick and next prolog of each particle
+// go to core_out right after its force.
+template <bool FUSED, bool KR, bool WALLS, bool GUARD, bool INTEG = false>
 __global__ __launch_bounds__(256) void k_force_fp32(ForceArgs32 a) {
   if (GUARD) {
     if (a.flags[1]) return;
-    if (a.flags[0]) {
+    if (a.flags[a.trig_in]) {
       if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.flags[1] = 1;
         a.flags[2] = a.step_tag;
@@ -298,7 +308,15 @@ __global__ __launch_bounds__(256) void k_force_fp32(ForceArgs32 a) {
       M = M + (cross(cpos - rx, f) + mint);
     }
   }
-  if (FUSED) {
+  if (FUSED && INTEG) {
+    const V3 al = angular_accel_g(Q4g{me.q.x, me.q.y, me.q.z, me.q.w}, M, V3{me.w.x, me.w.y, me.w.z},
+                                  V3{s.ix[x], s.iy[x], s.iz[x]});
+    Core nx = me;
+    kick_prolog<true>(nx.xr, nx.vm, nx.w, nx.q, F.x / mx, F.y / mx, F.z / mx, al.x, al.y, al.z, a.dt, a.kdt_2);
+    a.core_out[x] = nx;
+    const double dx = nx.xr.x - s.bx[x], dy = nx.xr.y - s.by[x], dz = nx.xr.z - s.bz[x];
+    if (dx * dx + dy * dy + dz * dz >= a.thr2) a.flags[a.trig_out] = 1;
+  } else if (FUSED) {
     s.ax[x] = F.x / mx;
     s.ay[x] = F.y / mx;
     s.az[x] = F.z / mx;
@@ -386,6 +404,7 @@ int launch_force_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams&
   a.gravity_on = gravity_on ? 1 : 0;
   a.flags = c->d_flags;
   a.step_tag = step_tag;
+  a.trig_in = c->trig_slot;
   const bool kr = c->kr_used;
   if (fused) {
     if (kr) walls ? launch32<true, true, true>(c, a, guarded) : launch32<true, true, false>(c, a, guarded);
@@ -395,6 +414,51 @@ int launch_force_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams&
     else walls ? launch32<false, false, true>(c, a, guarded) : launch32<false, false, false>(c, a, guarded);
   }
   XDEM_HIP(c, hipGetLastError());
+  return XDEM_OK;
+}
+
+// FP32-form engine step in one kernel (see launch_step)
+int launch_step_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool gravity_on, V3 g,
+                     double thr2, int32_t step_tag, Core** input) {
+  int rc = set_history_form(c, true);
+  if (rc) return rc;
+  if (cp.kr != 0.0 || cpd.kr != 0.0) c->kr_used = true;
+  const bool walls = c->nwalls > 0 && c->wl_valid;
+  ForceArgs32 a{};
+  a.n = c->n;
+  a.s = live(c);
+  a.nl = c->nl[c->nl_cur];
+  if (walls) a.wl = c->wl[c->wl_cur];
+  a.walls = c->d_walls;
+  a.nw = c->nwalls;
+  a.wstride = c->cap;
+  a.g = c->grid;
+  a.cpd = cpd;
+  a.cp64 = cp;
+  a.cp = {(float)cp.kn, (float)cp.kt, (float)cp.kr, (float)cp.mu, (float)cp.damp_rate};
+  a.dt = dt;
+  a.dtf = (float)dt;
+  a.grav = g;
+  a.gravity_on = gravity_on ? 1 : 0;
+  a.flags = c->d_flags;
+  a.step_tag = step_tag;
+  a.trig_in = c->trig_slot;
+  a.trig_out = c->trig_slot == 0 ? 3 : 0;
+  a.core_out = c->core_alt;
+  a.kdt_2 = dt * 0.5;
+  a.thr2 = thr2;
+  const dim3 grid(grid_blocks(a.n, 256)), block(256);
+  if (c->kr_used) {
+    if (walls) hipLaunchKernelGGL((k_force_fp32<true, true, true, true, true>), grid, block, 0, c->stream, a);
+    else hipLaunchKernelGGL((k_force_fp32<true, true, false, true, true>), grid, block, 0, c->stream, a);
+  } else {
+    if (walls) hipLaunchKernelGGL((k_force_fp32<true, false, true, true, true>), grid, block, 0, c->stream, a);
+    else hipLaunchKernelGGL((k_force_fp32<true, false, false, true, true>), grid, block, 0, c->stream, a);
+  }
+  XDEM_HIP(c, hipGetLastError());
+  *input = live(c).core;
+  std::swap(live(c).core, c->core_alt);
+  c->trig_slot = a.trig_out;
   return XDEM_OK;
 }
 

@@ -344,10 +344,10 @@ int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist);
 int do_nbh_sphere(xdem_ctx* c, double rcut_inc);
 int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, int form, bool fused,
                  bool gravity_on, V3 g, bool guarded, int32_t step_tag);
-// engine step of FP64 spheres in one kernel (force pass + kick + next prolog into c->core_alt, which
-// becomes live); *input = the record buffer the pass read
-int launch_step(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool gravity_on, V3 g,
-                double thr2, int32_t step_tag, Core** input);
+// engine step of spheres in one kernel (force pass + kick + next prolog into c->core_alt, which
+// becomes live), FP64 or FP32 form; *input = the record buffer the pass read
+int launch_step(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, int form, bool gravity_on,
+                V3 g, double thr2, int32_t step_tag, Core** input);
 int alloc_walls(xdem_ctx* c, int nw);
 int set_history_form(xdem_ctx* c, bool fp32);
 int engine_rebuild(xdem_ctx* c, double rcut_inc);
