@@ -26,15 +26,18 @@ PEAK_HBM_GBS = 8000.0
 
 
 def bytes_model(form, n, pairs):
-    """Algorithmic HBM bytes per step and for the force pass alone (kr = 0).
+    """Algorithmic HBM bytes per step and for the engine's integrating force pass alone (kr = 0).
 
-    Per step FP64: 412 B/particle + 52 B/pair (integrate 272 B/particle: reads x, v, a, mom, w, Q, m, I,
-    writes x, v, w, Q; force pass 140 B/particle: reads x, v, w, R, m, writes f, mom, CSR offset; per
-    half-list pair: neighbour index 4 B + friction history read/write 48 B).
-    FP32-force: 356 B/particle + 28 B/pair (force pass 44 B read + 24 B write + 4 B offset)."""
+    Per step FP64: 412 B/particle + 52 B/pair (operator sequence: integrate 272 B/particle reads x, v,
+    a, mom, w, Q, m, I and writes x, v, w, Q; force pass 140 B/particle reads x, v, w, R, m, writes f,
+    mom, CSR offset; per half-list pair: neighbour index 4 B + friction history read/write 48 B).
+    The engine runs a step as one integrating force pass: 300 B/particle + 52 B/pair (the force pass
+    without its f and mom writes, plus Q, I and the rebuild reference read and the 128-byte record
+    written). FP32-force: 356 B/particle + 28 B/pair per step; integrating pass 256 B/particle +
+    28 B/pair (force pass 44 B read + 24 B write + 4 B offset)."""
     if form == "fp32":
-        return 356 * n + 28 * pairs, 72 * n + 28 * pairs
-    return 412 * n + 52 * pairs, 140 * n + 52 * pairs
+        return 356 * n + 28 * pairs, 256 * n + 28 * pairs
+    return 412 * n + 52 * pairs, 300 * n + 52 * pairs
 
 
 def profiled_traffic(form):
@@ -288,7 +291,7 @@ def main():
                    "list_slots": st_nb2["slots"], "max_initial_overlap": w.meta["max_overlap"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": f"k_force_{args.form} (fused force pass)", "avg_ms": f_ms, "launches": f_cnt,
+                     "kernel": f"k_force_{args.form} (integrating force pass: force + kick + next prolog)", "avg_ms": f_ms, "launches": f_cnt,
                      "timed_launches": f"HIP events around the launches of every {args.timing_every}-th timed step",
                      "bytes_per_launch": force_bytes, "integrate_avg_ms": i_ms,
                      "step_bytes": step_bytes, "step_frac": step_bytes / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS},
