@@ -748,9 +748,11 @@ int xdem_run_steps(xdem_ctx* c, const xdem_scheme* s, int64_t nsteps, xdem_run_s
   const int poll = s->poll > 0 ? s->poll : 16;
   XDEM_HIP(c, hipMemsetAsync(c->d_flags, 0, 4 * sizeof(int32_t), c->stream));
   c->trig_slot = 0;
-  // spheres run a step as one kernel (force pass + kick + next prolog, launch_step) between a
-  // chain-start prolog and the last step of the run, which keeps the separate kernels so that the run
-  // ends in the same state (pending kick flushed) as the operator sequence
+  // spheres run a step as one kernel (force pass + kick + next prolog into the other record buffer,
+  // launch_step) between a chain-start prolog and the last step of the run, which keeps the separate
+  // kernels so that the run ends in the same state (pending kick flushed) as the operator sequence.
+  // (Polyhedra keep separate kernels: an integrating gather that also recomputes the vertices measured
+  // no faster than the separate vertex pass.)
   const bool fuse = c->nshapes == 0;
   std::vector<Core*> input(poll);  // record buffer read by the integrating pass of each step of a chunk
   bool chain = false;              // the live records hold the prolog of the next step
