@@ -299,6 +299,8 @@ int launch_step_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams& 
 
 int launch_step(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, int form, bool gravity_on,
                 V3 g, double thr2, int32_t step_tag, Core** input) {
+  *input = nullptr;
+  if (c->n == 0) return XDEM_OK;  // the kernels stage records n-1 and below
   if (!c->hist_ready) return set_err(c, XDEM_ERR_STATE, "contact: run nbh_sphere after chunk_neighbors_contact");
   if (form == XDEM_FORM_FP32) return launch_step_fp32(c, cp, cpd, dt, gravity_on, g, thr2, step_tag, input);
   int rc = set_history_form(c, false);
@@ -343,6 +345,7 @@ int launch_step(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, 
 
 int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, int form, bool fused,
                  bool gravity_on, V3 g, bool guarded, int32_t step_tag) {
+  if (c->n == 0) return XDEM_OK;  // the kernels stage records n-1 and below
   if (!c->hist_ready) return set_err(c, XDEM_ERR_STATE, "contact: run nbh_sphere after chunk_neighbors_contact");
   if (form == XDEM_FORM_FP32) return launch_force_fp32(c, cp, cpd, dt, fused, gravity_on, g, guarded, step_tag);
   int rc = set_history_form(c, false);
