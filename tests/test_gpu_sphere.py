@@ -338,3 +338,26 @@ def test_engine_rebuilds_inside_step_chain(gpu_ok, poll):
     pa, pb = a.ctx.download(), b.ctx.download()
     for k in ("rx", "ry", "rz", "vx", "vy", "vz", "fx", "fy", "fz", "momx", "vrotx", "vroty", "qw", "qx", "arotx"):
         assert np.array_equal(pa[k], pb[k]), k
+
+
+@pytest.mark.parametrize("form,case,kr", [("fp32", "walls", 0.0), ("fp64", "walls", 2e3), ("fp64", "periodic", 2e3),
+                                          ("fp32", "periodic", 2e3)])
+def test_engine_variants_equal_operator_path(gpu_ok, form, case, kr):
+    """The engine's integrating passes with walls, with a rolling-moment law (kr > 0, rolling history
+    slots) and in the FP32-force form reproduce the operator path bit for bit."""
+    w = {"walls": wall_deposition, "periodic": periodic_pack}[case]()
+    if kr:
+        w.cp = native.contact_params(kn=1e5, kt=8e4, kr=kr, mu=0.3, damp_rate=0.9)
+        w.cpd = w.cp
+    f = native.XDEM_FORM_FP32 if form == "fp32" else native.XDEM_FORM_FP64
+    a = SphereSim(w, form=f)
+    a.first_iteration()
+    b = SphereSim(w, form=f)
+    b.first_iteration()
+    for _ in range(30):
+        a.step()
+    b.run(30, poll=8)
+    assert a.rebuilds == b.rebuilds
+    pa, pb = a.ctx.download(), b.ctx.download()
+    for k in ("rx", "ry", "rz", "vx", "vy", "vz", "fx", "fy", "fz", "momx", "vrotx", "vroty", "qw", "qx", "arotx"):
+        assert np.array_equal(pa[k], pb[k]), k
