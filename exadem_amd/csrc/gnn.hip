@@ -195,7 +195,7 @@ struct EncArgs {
   uint16_t* h;
 };
 // ------------------------------------------------------------------------------------- MFMA GEMM
-enum { EPI_BF16 = 0, EPI_UPDATE = 1, EPI_BWD_U = 2, EPI_ADD = 3 };
+enum { EPI_BF16 = 0, EPI_UPDATE = 1, EPI_BWD_U = 2, EPI_ADD = 3, EPI_UPDATE_DEC = 4 };
 
 struct GemmArgs {
   int64_t n;
@@ -211,6 +211,11 @@ struct GemmArgs {
   uint16_t* h_out;  // EPI_UPDATE: h + relu(U)
   float* dh;        // EPI_BWD_U / EPI_ADD: [n][H] (+=)
   float* dm;        // EPI_BWD_U: [n][H] (=, times 1/8)
+  // EPI_UPDATE_DEC: the update of the last layer followed by the decoder on the new hidden state in
+  // registers (the state itself is not stored): W_out bfloat16 [FO][H], b_out, outputs [n][FO]
+  const uint16_t* dec_w;
+  const float* dec_b;
+  float* dec_out;
 };
 
 // Y[r][c] = sum_k A[r][k] B[c][k] with A = [a1 | a2] (rows = particles), for the fixed shapes of the
@@ -238,11 +243,21 @@ __global__ __launch_bounds__(512) void k_gemm_p(GemmArgs g) {
     const int c = r & 31, m = 8 * ((c >> 2) & 3) + 4 * (c >> 4) + (c & 3);
     *(uint4*)(w_lds + ((r & ~31) + m) * KP + c8) = *(const uint4*)(g.b + (int64_t)r * K + c8);
   }
+  // EPI_UPDATE_DEC: the decoder's 32-row tile after the weights (rows FO..31 zero), row stride H + 8
+  constexpr int DP = H + 8;
+  uint16_t* dec_lds = w_lds + M * KP;
+  if (EPI == EPI_UPDATE_DEC) {
+    for (int idx = threadIdx.x; idx < 32 * H / 8; idx += 512) {
+      const int r = idx / (H / 8), c8 = (idx % (H / 8)) * 8;
+      *(uint4*)(dec_lds + r * DP + c8) = r < FO ? *(const uint4*)(g.dec_w + r * H + c8) : uint4{0u, 0u, 0u, 0u};
+    }
+  }
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int col = lane & 31, half = lane >> 5;
   static_assert(K1 % 32 == 0 && K2 % 32 == 0, "k steps come in pairs");
-  static_assert(EPI != EPI_UPDATE || (M == 128 && K1 == 128), "the residual is the first 128 inputs");
+  static_assert((EPI != EPI_UPDATE && EPI != EPI_UPDATE_DEC) || (M == 128 && K1 == 128),
+                "the residual is the first 128 inputs");
   const int64_t ntiles = (g.n + 255) / 256;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t r0 = tile * 256 + w * 32;
@@ -282,9 +297,15 @@ __global__ __launch_bounds__(512) void k_gemm_p(GemmArgs g) {
           acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, xf[ks], acc[t], 0, 0, 0);
         }
       }
-      // stores only: a lane that left early would run the next pass's MFMAs with part of EXEC
+      floatx16 dacc;
+      if (EPI == EPI_UPDATE_DEC) {
 #pragma unroll
-      for (int t = 0; t < NT && r < g.n; t++) {
+        for (int v = 0; v < 16; v++) dacc[v] = 0.f;
+      }
+      // stores only: a lane that left early would run the next pass's (or the decoder's) MFMAs with
+      // part of EXEC; the decoder variant runs every lane through the epilogue and guards the stores
+#pragma unroll
+      for (int t = 0; t < NT && (EPI == EPI_UPDATE_DEC || r < g.n); t++) {
         const int c0 = cb + t * 32 + 16 * half;  // columns c0 .. c0+15
         if (EPI == EPI_BF16) {
           uint32_t o[8];
@@ -293,7 +314,7 @@ __global__ __launch_bounds__(512) void k_gemm_p(GemmArgs g) {
           uint4* dst = (uint4*)(g.ybf + r * M + c0);
           dst[0] = uint4{o[0], o[1], o[2], o[3]};
           dst[1] = uint4{o[4], o[5], o[6], o[7]};
-        } else if (EPI == EPI_UPDATE) {
+        } else if (EPI == EPI_UPDATE || EPI == EPI_UPDATE_DEC) {
           const uint4 h0 = __builtin_bit_cast(uint4, xf[2 * t]), h1 = __builtin_bit_cast(uint4, xf[2 * t + 1]);
           const uint32_t hw[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
           uint32_t uo[8], ho[8];
@@ -304,14 +325,25 @@ __global__ __launch_bounds__(512) void k_gemm_p(GemmArgs g) {
             ho[v] = pack_bf16(bf2f((uint16_t)(hw[v] & 0xffffu)) + fmaxf(u0, 0.f),
                               bf2f((uint16_t)(hw[v] >> 16)) + fmaxf(u1, 0.f));
           }
-          if (g.u_out) {
-            uint4* ud = (uint4*)(g.u_out + r * H + c0);
-            ud[0] = uint4{uo[0], uo[1], uo[2], uo[3]};
-            ud[1] = uint4{uo[4], uo[5], uo[6], uo[7]};
+          if (EPI == EPI_UPDATE_DEC) {
+            // decoder k steps 2t and 2t+1 over the lane's channels 32t + 16h + 0..7 and + 8..15
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+              const bf16x8 wd = *(const bf16x8*)(dec_lds + col * DP + 32 * t + 16 * half + 8 * u);
+              dacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                  wd, __builtin_bit_cast(bf16x8, uint4{ho[4 * u], ho[4 * u + 1], ho[4 * u + 2], ho[4 * u + 3]}), dacc,
+                  0, 0, 0);
+            }
+          } else {
+            if (g.u_out) {
+              uint4* ud = (uint4*)(g.u_out + r * H + c0);
+              ud[0] = uint4{uo[0], uo[1], uo[2], uo[3]};
+              ud[1] = uint4{uo[4], uo[5], uo[6], uo[7]};
+            }
+            uint4* hd = (uint4*)(g.h_out + r * H + c0);
+            hd[0] = uint4{ho[0], ho[1], ho[2], ho[3]};
+            hd[1] = uint4{ho[4], ho[5], ho[6], ho[7]};
           }
-          uint4* hd = (uint4*)(g.h_out + r * H + c0);
-          hd[0] = uint4{ho[0], ho[1], ho[2], ho[3]};
-          hd[1] = uint4{ho[4], ho[5], ho[6], ho[7]};
         } else if (EPI == EPI_BWD_U && c0 >= H) {
           float4* dst = (float4*)(g.dm + r * H + (c0 - H));
 #pragma unroll
@@ -329,6 +361,16 @@ __global__ __launch_bounds__(512) void k_gemm_p(GemmArgs g) {
             cur.w += acc[t][4 * q + 3];
             dst[q] = cur;
           }
+        }
+      }
+      if (EPI == EPI_UPDATE_DEC && r < g.n) {
+        // decoder accumulator element v of lane l is output row 8(v/4) + 4(l/32) + v%4
+        if (half == 0) {
+#pragma unroll
+          for (int q = 0; q < 4; q++) g.dec_out[r * FO + q] = dacc[q] + g.dec_b[q];
+        } else {
+#pragma unroll
+          for (int q = 0; q < 2; q++) g.dec_out[r * FO + 4 + q] = dacc[q] + g.dec_b[4 + q];
         }
       }
     }
@@ -1116,7 +1158,7 @@ int gemm_p(xdem_gnn* G, const GemmArgs& a) {
   if (a.n <= 0) return XDEM_OK;
   constexpr int K = K1 + K2;
   static_assert(M % 128 == 0, "column groups of 128");
-  const int lds = M * (K + 8) * (int)sizeof(uint16_t);
+  const int lds = (M * (K + 8) + (EPI == EPI_UPDATE_DEC ? 32 * (H + 8) : 0)) * (int)sizeof(uint16_t);
   XDEM_HIP(c, hipFuncSetAttribute((const void*)k_gemm_p<EPI, M, K1, K2>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   const int64_t ntiles = (a.n + 255) / 256;
@@ -1211,11 +1253,20 @@ int forward(xdem_gnn* G) {
     g2.n = n; g2.a1 = hin; g2.k1 = H; g2.a2 = mm; g2.k2 = H;
     g2.b = G->wbf + G->off.W_U[l]; g2.m = H; g2.bias = G->wr + G->off.b_U[l];
     g2.u_out = tr ? G->u[l] : nullptr; g2.h_out = hout;
-    if ((rc = gemm_p<EPI_UPDATE, H, H, H>(G, g2))) return rc;
+    if (!tr && l == L - 1) {
+      // inference: the decoder runs on the last hidden state in the update's registers
+      g2.h_out = nullptr;
+      g2.dec_w = G->wbf + G->off.W_out; g2.dec_b = G->wr + G->off.b_out; g2.dec_out = G->out;
+      if ((rc = gemm_p<EPI_UPDATE_DEC, H, H, H>(G, g2))) return rc;
+    } else if ((rc = gemm_p<EPI_UPDATE, H, H, H>(G, g2))) {
+      return rc;
+    }
   }
-  DecArgs da{n, tr ? G->h[L] : G->h[L & 1], G->wr + G->off.W_out, G->wbf + G->off.W_out, G->wr + G->off.b_out, G->out, nullptr, nullptr,
-             nullptr};
-  hipLaunchKernelGGL(k_decode_mfma, dim3(nb(n, 128)), dim3(256), 0, c->stream, da);
+  if (tr) {
+    DecArgs da{n, G->h[L], G->wr + G->off.W_out, G->wbf + G->off.W_out, G->wr + G->off.b_out, G->out, nullptr, nullptr,
+               nullptr};
+    hipLaunchKernelGGL(k_decode_mfma, dim3(nb(n, 128)), dim3(256), 0, c->stream, da);
+  }
   XDEM_HIP(c, hipGetLastError());
   G->fwd_valid = true;
   G->fwd_n = n;

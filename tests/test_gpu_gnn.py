@@ -85,6 +85,24 @@ def test_forward_matches_oracle(net):
     assert float(np.mean(diff)) <= 1e-3 * scale
 
 
+def test_inference_forward_matches_oracle(net):
+    """A network created for inference runs the decoder inside the last update product: same outputs
+    as the oracle and as the training-mode forward pass (up to FP32 summation order)."""
+    sim, gnn, flat = net
+    inf = native.Gnn(sim.ctx, r0=0.75, v0=0.1, a0=1.0, m0=4000.0, ov_scale=50.0, substeps=10, train=False)
+    inf.set_params(flat)
+    out = inf.forward()
+    ref_gpu = gnn.forward()
+    src, dst, e, x = gnn.dump_graph()
+    ref, _ = G.Net(flat.astype(np.float64), exact=False).forward(x, src.astype(np.int64), dst.astype(np.int64), e)
+    scale = float(np.max(np.abs(ref)))
+    diff = np.abs(out - ref)
+    assert float(np.max(diff)) <= 1e-2 * scale, float(np.max(diff)) / scale
+    assert float(np.mean(diff)) <= 1e-3 * scale
+    assert float(np.max(np.abs(out - ref_gpu))) <= 1e-5 * scale
+    inf.close()
+
+
 def test_backward_matches_oracle(net):
     _, gnn, flat = net
     out = gnn.forward()
