@@ -218,8 +218,10 @@ __global__ __launch_bounds__(256) void k_nbh_count(NbhArgs a, uint32_t* __restri
 
 // Work balancing: inside each group of kBalance consecutive particles (a compact Morton region),
 // a stable counting sort by decreasing list length, so that the 64 lanes of a wavefront walk lists of
-// nearly equal length. perm2[new] = morton index, inv2[morton] = new index.
-constexpr int kBalance = 2048;
+// nearly equal length. perm2[new] = morton index, inv2[morton] = new index. A group is one block of
+// the force pass (256 particles): the block's LDS copy of its records then holds a compact region,
+// where most partners are (1M spheres: 181.5 us per step against 212.5 us with groups of 2048).
+constexpr int kBalance = 256;
 constexpr int kBalanceBins = 64;
 __global__ __launch_bounds__(64) void k_balance(int64_t n, const uint32_t* __restrict__ cnt,
                                                 uint32_t* __restrict__ perm2, uint32_t* __restrict__ inv2) {
