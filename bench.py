@@ -178,9 +178,10 @@ def main():
                          "max_force_rel_err": ferr, "contact_set_identical": bool(same_set),
                          "contacts": int(np.count_nonzero(act_a)),
                          "error_stated_as": "max over particles of |F32 - F64| / max(|F64|, m g), first force pass"}
-        # surrogate-accelerated forms: the trained network with K = 100 and K = 10 DEM steps per step
+        # surrogate-accelerated forms: the trained network with K = 200, 100 and 10 DEM steps per step
         from exadem_amd import surrogate
-        for form_key, wpath in (("surrogate", surrogate.WEIGHTS_K100), ("surrogate_k10", surrogate.WEIGHTS)):
+        for form_key, wpath in (("surrogate", surrogate.WEIGHTS_K200), ("surrogate_k100", surrogate.WEIGHTS_K100),
+                                ("surrogate_k10", surrogate.WEIGHTS)):
             if not os.path.exists(wpath):
                 forms[form_key] = {"value": None, "status": "no trained weights at " + wpath}
                 continue

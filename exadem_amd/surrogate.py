@@ -21,6 +21,8 @@ HIDDEN, LAYERS, FN, FE, FO = 128, 3, 8, 16, 6
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "surrogate_polydisperse.npz")
 # the same network trained on windows of K = 100 DEM steps (fine-tuned from the K = 10 weights)
 WEIGHTS_K100 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "surrogate_polydisperse_k100.npz")
+# K = 200 DEM steps per surrogate step (fine-tuned from the K = 100 weights)
+WEIGHTS_K200 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "surrogate_polydisperse_k200.npz")
 
 
 def param_shapes():

@@ -170,15 +170,15 @@ def test_surrogate_run_with_drift_checks(gpu_ok):
     assert np.all(np.isfinite(p["rx"])) and np.all(np.isfinite(p["vx"]))
 
 
-@pytest.mark.parametrize("weights", ["k10", "k100"])
+@pytest.mark.parametrize("weights", ["k10", "k100", "k200"])
 def test_shipped_weights_beat_free_flight(gpu_ok, weights):
-    """The shipped networks (K = 10 and K = 100 DEM steps per surrogate step) on a packing they were
+    """The shipped networks (K = 10, 100 and 200 DEM steps per surrogate step) on a packing they were
     not trained on: drift-checked run without fallback, RMS position error at the checks and RMS
     contact-force error on a window below the free-flight (zero contact force) values."""
-    path = surrogate.WEIGHTS if weights == "k10" else surrogate.WEIGHTS_K100
+    path = {"k10": surrogate.WEIGHTS, "k100": surrogate.WEIGHTS_K100, "k200": surrogate.WEIGHTS_K200}[weights]
     params, scales = surrogate.load(path)
     K = scales["substeps"]
-    assert K == (10 if weights == "k10" else 100)
+    assert K == int(weights[1:])
     w = surrogate.training_workload(20000, seed=4242)
     sim = SphereSim(w)
     sim.first_iteration()
