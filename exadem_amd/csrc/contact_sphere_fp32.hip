@@ -240,13 +240,17 @@ __global__ __launch_bounds__(256) void k_force_fp32(ForceArgs32 a) {
       }
       const uint32_t nwd = act ? (wd | kActiveBit) : (wd & ~kActiveBit);
       if (nwd != wd) word[sl] = nwd;
+      // Both contact arms are parallel to n, so the normal force has no moment about either centre:
+      // (cp - r) x f = (cp - r) x ft exactly. The FP64 form evaluates the full product, whose normal
+      // part there is round-off of ~1e-16 |b| |fn|; in FP32 the same round-off (~1e-7 |b| |fn|) would
+      // swamp the friction moment, so only the tangential part enters.
       F3 fc, tc;
       if (owner) {
         fc = f;
-        tc = crossf(bi, f) + mint;
+        tc = crossf(bi, ft) + mint;
       } else {
         fc = -f;
-        tc = crossf(bj, fc) + (-mint);
+        tc = crossf(bj, -ft) + (-mint);
       }
       cadd(Fx, fc.x);
       cadd(Fy, fc.y);

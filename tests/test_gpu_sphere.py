@@ -215,6 +215,8 @@ def test_fp32_form_contact_set_and_force_error(gpu_ok):
     floor = pa["mass"] * 9.81
     err = rel_err(force(pb), force(pa), floor)
     assert err <= 1e-5, err
+    merr = moment_rel_err(pb, pa)
+    assert merr <= 1e-5, merr
     a.run(50)
     b.run(50)
     ka, _ = interaction_table(*a.ctx.interactions())
@@ -223,7 +225,20 @@ def test_fp32_form_contact_set_and_force_error(gpu_ok):
     pa, pb = by_id(a.ctx.download()), by_id(b.ctx.download())
     err = rel_err(force(pb), force(pa), 9.81 * np.ones(len(pa["id"])))
     assert err <= 1e-5, err
+    merr = moment_rel_err(pb, pa)
+    print(f"fp32 after 50 steps: force error {err:.3g}, moment error {merr:.3g}")
+    assert merr <= 1e-5, merr
     assert float(np.max(np.abs(pos(pb) - pos(pa)))) <= 1e-9
+
+
+def moment_rel_err(pb, pa):
+    """max |M32 - M64| / |M64| over the particles whose moment is at least 1% of the largest."""
+    ma = np.stack([pa["momx"], pa["momy"], pa["momz"]], axis=1)
+    mb = np.stack([pb["momx"], pb["momy"], pb["momz"]], axis=1)
+    mn = np.linalg.norm(ma, axis=1)
+    sel = mn >= 1e-2 * float(np.max(mn))
+    assert np.count_nonzero(sel) > 0
+    return float(np.max(np.linalg.norm(mb - ma, axis=1)[sel] / mn[sel]))
 
 
 def shell_cluster(n_shell=220, seed=5):
