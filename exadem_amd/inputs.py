@@ -157,6 +157,36 @@ def polydisperse(n=1_000_000, phi=0.55, rmin=0.5, rmax=1.0, seed_radius=2, seed_
                     {"phi": phi, "box": tuple(hi), "relax_sweeps": int(sweeps), "max_overlap": mo.value})
 
 
+def precision_spheres(n=4_000_000, phi=0.58, median=0.6, sigma=0.2, rmin=0.3, rmax=1.2, seed_radius=5, seed_vel=6,
+                      seed_pos=5, sigma_v=0.1, density=2500.0, dt=5e-6, steps=500, rcut_inc=0.1,
+                      gravity=(0.0, 0.0, -9.81), relax_sweeps=40):
+    """Input of the precision comparison (FP32-force against FP64): n spheres with lognormal radii
+    R = median exp(sigma Z), Z ~ N(0, 1) (seed_radius), clipped to [rmin, rmax], in a periodic cube
+    whose edge gives solid fraction phi. Positions: RSA at radius scale 0.8, then full radii and
+    relax_sweeps Jacobi overlap-relaxation sweeps (at phi 0.58 the relaxation leaves overlaps of a few
+    percent of a radius; max_overlap in meta). Velocities N(0, sigma_v), the contact law of the 1M
+    benchmark."""
+    from .native import contact_params
+    z = normal(seed_radius, 1, n, 0.0, 1.0)
+    r = np.clip(median * np.exp(sigma * z), rmin, rmax)
+    vol = float(np.sum((4.0 / 3.0) * math.pi * r ** 3))
+    edge = (vol / phi) ** (1.0 / 3.0)
+    lo = np.zeros(3)
+    hi = np.array([edge, edge, edge])
+    x = np.empty(n)
+    y = np.empty(n)
+    zz = np.empty(n)
+    mo = C.c_double()
+    sweeps = host().xdem_gen_pack_periodic(seed_pos, n, _p(r), _p(lo), _p(hi), 0.8, relax_sweeps, 1e-3 * rmin, _p(x),
+                                           _p(y), _p(zz), C.byref(mo))
+    v = normal(seed_vel, 2, 3 * n, 0.0, sigma_v).reshape(3, n)
+    p = sphere_set(x, y, zz, r, density, v[0], v[1], v[2])
+    dom = domain((0.0, 0.0, 0.0), tuple(hi), (True, True, True), 2 * rmax + rcut_inc)
+    cp = contact_params(kn=1e5, mu=0.3, damp_rate=0.9)
+    return Workload("precision", p, dom, [], cp, cp, dt, steps, rcut_inc, gravity,
+                    {"phi": phi, "box": tuple(hi), "relax_sweeps": int(sweeps), "max_overlap": mo.value})
+
+
 # ---------------------------------------------------------------------------------------------- polyhedra
 def random_quaternions(seed, n):
     """Uniform random unit quaternions (Shoemake), (w, x, y, z)."""
