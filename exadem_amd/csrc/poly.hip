@@ -13,9 +13,9 @@
 //
 // Contact pass without atomics: one thread per feature evaluates it and writes its terms of the owner
 // force, owner moment and partner moment (the partner's force term is the exact negation). A second
-// kernel walks, per particle and in stream order, its entries (own slots, or the owner's slot through
-// `mirror`), forms each pair's subtotal over the features in list order, adds it, then the vertex-wall
-// contacts, then finishes force_to_accel and the angular acceleration.
+// kernel walks, per particle and in stream order, its entries (each slot holds its pair's feature range,
+// the owner slot's for a partner entry), forms each pair's subtotal over the features in list order,
+// adds it, then the vertex-wall contacts, then finishes force_to_accel and the angular acceleration.
 #include <algorithm>
 
 #include "contact_law.h"
@@ -416,8 +416,11 @@ __global__ __launch_bounds__(256) void k_feat_fill(PolyNbhArgs a, const uint64_t
   }
 }
 
-// Owner slot of every partner entry: the owner's entry pointing back with the mirrored image code.
-__global__ __launch_bounds__(256) void k_mirror(int64_t n, NbrList nl, uint32_t* __restrict__ mirror) {
+// Feature range of every list slot: an owner entry's own range; a partner entry gets the range of the
+// owner's entry pointing back with the mirrored image code, or x = ~0u when there is none. The force
+// gather then reads one 8-byte record per entry instead of following slot -> owner slot -> range.
+__global__ __launch_bounds__(256) void k_feat_range(int64_t n, NbrList nl, const uint64_t* __restrict__ start,
+                                                    uint2* __restrict__ frange) {
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= n) return;
   const uint32_t cnt = nl.cnt[x];
@@ -426,7 +429,7 @@ __global__ __launch_bounds__(256) void k_mirror(int64_t n, NbrList nl, uint32_t*
     const uint64_t sl = b + (uint64_t)k * kWave;
     const uint32_t wd = nl.word[sl];
     if (wd & kOwnerBit) {
-      mirror[sl] = (uint32_t)sl;
+      frange[sl] = uint2{(uint32_t)start[sl], (uint32_t)(start[sl + 1] - start[sl])};
       continue;
     }
     const uint32_t y = wd & kIdxMask;
@@ -443,7 +446,8 @@ __global__ __launch_bounds__(256) void k_mirror(int64_t n, NbrList nl, uint32_t*
         break;
       }
     }
-    mirror[sl] = found;
+    frange[sl] = found == 0xffffffffu ? uint2{0xffffffffu, 0u}
+                                      : uint2{(uint32_t)start[found], (uint32_t)(start[found + 1] - start[found])};
   }
 }
 
@@ -560,10 +564,10 @@ int poly_nbh(xdem_ctx* c, double rcut_inc) {
     if ((rc = galloc(c, &P.start, cap + 1)) || (rc = galloc(c, &P.fcount, cap))) return rc;
     P.cap_slots = cap;
   }
-  if (L.slots > c->cap_mirror) {
+  if (L.slots > c->cap_frange) {
     const int64_t cap = (int64_t)(L.slots * 1.25) + 4096;
-    if ((rc = galloc(c, &c->mirror, cap))) return rc;
-    c->cap_mirror = cap;
+    if ((rc = galloc(c, &c->frange, cap))) return rc;
+    c->cap_frange = cap;
   }
   XDEM_HIP(c, hipMemsetAsync(P.fcount, 0, sizeof(uint32_t) * std::max<int64_t>(L.slots, 1), c->stream));
   PolyNbhArgs a{n, live(c), c->grid, c->d_shapes, c->vtx, L, rcut_inc};
@@ -586,11 +590,13 @@ int poly_nbh(xdem_ctx* c, double rcut_inc) {
     if ((rc = galloc(c, &c->fcontrib, cap * 9)) || (rc = galloc(c, &c->fcontact, cap))) return rc;
     c->cap_contrib = cap;
   }
+  if (nf >= 0xffffffffull) return set_err(c, XDEM_ERR_CAPACITY, "nbh_polyhedron: more than 2^32 - 2 features");
   P.nfeat = (int64_t)nf;
   if (n > 0) {
     hipLaunchKernelGGL(k_feat_fill, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a, P.start, P.feat, P.fslot,
                        P.fown);
-    hipLaunchKernelGGL(k_mirror, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, L, c->mirror);
+    hipLaunchKernelGGL(k_feat_range, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, L, P.start,
+                       c->frange);
     const bool have_old = c->nl_prev_valid && c->perm_valid && c->pl_valid;
     FeatCarryArgs fa{n, L, c->nl[1 - c->nl_cur], P, c->pl[c->pl_cur], c->perm, have_old ? 1 : 0};
     hipLaunchKernelGGL(k_feat_carry, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, fa);
@@ -638,7 +644,7 @@ struct PolyForceArgs {
   NbrList nl;
   PolyList pl;
   PolyWall pw;
-  const uint32_t* mirror;
+  const uint2* frange;
   double* contrib;    // [nfeat * 9]
   uint8_t* contact;   // [nfeat]
   const Surface* walls;
@@ -768,7 +774,7 @@ template <bool FUSED, bool KR, bool GUARD>
 __global__ __launch_bounds__(256) void k_poly_gather(PolyForceArgs a) {
   // the stall decision was taken by k_poly_feat of the same step
   if (GUARD && a.flags[1]) return;
-  // XCD-aware block order (partner contributions are read through the mirror slots)
+  // XCD-aware block order (partner contributions are read through the owner slots' feature ranges)
   const int64_t x = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (x >= a.n) return;
   const PState& s = a.s;
@@ -785,19 +791,26 @@ __global__ __launch_bounds__(256) void k_poly_gather(PolyForceArgs a) {
   }
   const uint32_t cnt = a.nl.cnt[x];
   const uint64_t b = a.nl.base[x >> 6] + (uint64_t)(x & 63);
-  // per entry, the pair subtotal in feature order (as the owner's pass forms it), then the sum
+  // per entry, the pair subtotal in feature order (as the owner's pass forms it), then the sum; the
+  // word and feature range of entry k+1 are loaded while entry k is summed
+  uint32_t wd_n = 0;
+  uint2 fr_n{0u, 0u};
+  if (cnt > 0) {
+    wd_n = a.nl.word[b];
+    fr_n = a.frange[b];
+  }
   for (uint32_t k = 0; k < cnt; k++) {
     const uint64_t sl = b + (uint64_t)k * kWave;
-    const uint32_t wd = a.nl.word[sl];
-    const bool own = (wd & kOwnerBit) != 0;
-    uint64_t ps = sl;
-    if (!own) {
-      const uint32_t m = a.mirror[sl];
-      if (m == 0xffffffffu) continue;
-      ps = m;
+    const uint32_t wd = wd_n;
+    const uint2 fr = fr_n;
+    if (k + 1 < cnt) {
+      wd_n = a.nl.word[sl + kWave];
+      fr_n = a.frange[sl + kWave];
     }
+    const bool own = (wd & kOwnerBit) != 0;
+    if (fr.x == 0xffffffffu) continue;
     V3 Fp{0, 0, 0}, Mp{0, 0, 0};
-    const uint64_t f0 = a.pl.start[ps], f1 = a.pl.start[ps + 1];
+    const uint64_t f0 = fr.x, f1 = (uint64_t)fr.x + fr.y;
     for (uint64_t f = f0; f < f1; f++) {
       if (!a.contact[f]) continue;
       const double* pr = a.contrib + 9 * f;
@@ -883,7 +896,7 @@ int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, d
   a.vtx = c->vtx;
   a.nl = c->nl[c->nl_cur];
   a.pl = c->pl[c->pl_cur];
-  a.mirror = c->mirror;
+  a.frange = c->frange;
   a.contrib = c->fcontrib;
   a.contact = c->fcontact;
   a.nw = (c->nwalls > 0 && c->pw_valid) ? c->nwalls : 0;

@@ -305,8 +305,9 @@ struct xdem_ctx {
   double* fcontrib = nullptr;       // [cap_contrib * 9] per feature: F_owner, M_owner, M_partner terms
   uint8_t* fcontact = nullptr;      // [cap_contrib] 1 when the feature is in contact
   int64_t cap_contrib = 0;
-  uint32_t* mirror = nullptr;       // [cap_mirror] owner slot of each partner entry
-  int64_t cap_mirror = 0;
+  uint2* frange = nullptr;          // [cap_frange] per list slot: first feature and count of its pair
+                                    // (through the owner entry for partner entries); x = ~0u: no owner entry
+  int64_t cap_frange = 0;
   xdem::PolyWall pw[2];             // vertex-wall candidates and history, current and previous
   int pw_cur = 0;
   bool pw_valid = false;

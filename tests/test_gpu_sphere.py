@@ -129,6 +129,38 @@ def test_operator_steps_match_oracle(gpu_ok, case):
     assert float(np.max(np.abs(fg - fo))) <= 1e-12 * scale
 
 
+@pytest.mark.parametrize("spacing", [1.2, 0.98])
+def test_deposition_config_1000_steps(gpu_ok, spacing):
+    """The CPU-runnable deposition input in full: 10,000 spheres of radius 0.5 on a jittered lattice in a
+    40 x 40 x 80 walled box under gravity, kn 1e5, kt 0.8 kn, mu 0.3, damping 0.9, dt 1e-5, 1000 steps,
+    through the engine (one kernel per step) against the oracle. At lattice spacing 1.2 the spheres fall
+    freely (the fall, 4.9e-4, is far below the gaps); spacing 0.98 is the contact-rich variant (initial
+    overlaps). Same rebuilds, interaction keys and history, forces and trajectories as the oracle."""
+    w = inputs.deposition(spacing=spacing)
+    assert w.n == 10000 and w.steps == 1000
+    sch = scheme(w)
+    o = oracle_for(w)
+    o.first_iteration(sch)
+    s = gpu_for(w)
+    s.first_iteration()
+    orb = o.run_steps(sch, w.steps)
+    st = s.run(w.steps, poll=64)
+    pg, pq = s.ctx.download(), o.particles()
+    floor = by_id(pq)["mass"] * 9.81 * 1e-3
+    compare_state(pg, pq, floor / by_id(pq)["mass"], f"deposition spacing {spacing}, {w.steps} steps")
+    assert st["rebuilds"] == orb
+    gi, gj, gt, gf = s.ctx.interactions()
+    oi, oj, ot, of, om = o.interactions()
+    kg, fg = interaction_table(gi, gj, gt, gf)
+    ko, fo = interaction_table(oi, oj, ot, of)
+    assert np.array_equal(kg, ko)
+    assert np.array_equal(fg, fo)
+    contacts = int(np.count_nonzero(np.any(fo != 0, axis=1)))
+    print(f"deposition spacing {spacing}: rebuilds {orb}, contacts {contacts}")
+    if spacing < 1.0:
+        assert contacts > 1000
+
+
 @pytest.mark.parametrize("case", ["dense", "periodic"])
 def test_fused_engine_equals_operator_path(gpu_ok, case):
     w = {"dense": dense_deposition, "periodic": periodic_pack}[case]()
