@@ -92,7 +92,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not provided "
-                                   "(the sharding design is described in DESIGN.md)"}))
+                                   "(spatial slabs with a per-step halo exchange of particle records would shard it)"}))
         sys.exit(2)
 
     if args.copy_probe:

@@ -316,8 +316,8 @@ int xdem_gnn_adam(xdem_gnn* gnn, double lr, double beta1, double beta2, double e
  * m ((v(t+K dt) - v(t)) / (K dt) - g) and I (w(t+K dt) - w(t)) / (K dt), backward, Adam. *used = 0 when
  * a neighbour rebuild fell inside the window (the DEM steps stand, no update is made). */
 int xdem_gnn_train_window(xdem_gnn* gnn, const xdem_scheme* scheme, double lr, double* loss, int32_t* used);
-/* Graph of the last forward pass in device order: per entry source and target index and the 8
- * edge inputs (16 bfloat16 bit patterns); per particle the 8 node inputs. */
+/* Graph of the last forward pass in device order: per entry source and target index and the 16
+ * edge inputs (bfloat16 bit patterns); per particle the 8 node inputs. */
 int xdem_gnn_dump_graph(xdem_gnn* gnn, int64_t cap, uint32_t* src, uint32_t* dst, uint16_t* edge_in,
                         uint16_t* node_in, int64_t* n_edges);
 /* Surrogate-accelerated run of nsteps surrogate steps (nsteps * K DEM steps of simulated time). Every
