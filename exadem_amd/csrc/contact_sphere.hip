@@ -10,6 +10,9 @@
 //
 // The fused variant also performs reset_force_moment + gravity_force before the contacts and
 // force_to_accel + the angular-acceleration part of combined_compute_epilog after them.
+#include <algorithm>
+#include <cmath>
+
 #include "contact_law.h"
 #include "step.h"
 
@@ -60,6 +63,10 @@ struct ForceArgs {
   double kdt_2;
   double thr2;
   int32_t trig_out;
+  // passes outside the integrating step, when recording: contact state per owner slot and per
+  // (wall, particle), kCStateLen doubles each (null: not recorded)
+  double* cstate;
+  double* wstate;
 };
 
 
@@ -178,6 +185,16 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
       V3 mint = e.mint;
       const V3 f = owner ? hooke_force<KR>(dn, nrm, a.dt, a.cp, meff, ft, mint, cpos, ri, vx, wx, rj, vy, wy)
                          : hooke_force<KR>(dn, nrm, a.dt, a.cp, meff, ft, mint, cpos, ri, vy, wy, rj, vx, wx);
+      if (!INTEG && a.cstate && owner) {
+        // the tuple the reference's functor returns for analyses: dn, contact point, f - ft, ft
+        // (contact_sphere.hpp:175-202), with the two positions the interaction dump writes
+        const V3 fnv = f - ft;
+        const double rec[kCStateLen] = {dn,   cpos.x, cpos.y, cpos.z, fnv.x, fnv.y, fnv.z, ft.x,
+                                        ft.y, ft.z,   ri.x,   ri.y,   ri.z,   rj.x,  rj.y,  rj.z};
+        double* r = a.cstate + (uint64_t)kCStateLen * sl;
+#pragma unroll
+        for (int q = 0; q < kCStateLen; q++) r[q] = rec[q];
+      }
       const bool act = !(ft.x == 0 && ft.y == 0 && ft.z == 0 && mint.x == 0 && mint.y == 0 && mint.z == 0);
       if (act || (wd & kActiveBit)) {
         a.nl.hx[sl] = ft.x;
@@ -199,16 +216,24 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
         M = M + (cross(cpos - rx, nf) + (-mint));
         F = F + nf;
       }
-    } else if (wd & kActiveBit) {
-      a.nl.hx[sl] = 0.0;
-      a.nl.hy[sl] = 0.0;
-      a.nl.hz[sl] = 0.0;
-      if (KR) {
-        a.nl.mhx[sl] = 0.0;
-        a.nl.mhy[sl] = 0.0;
-        a.nl.mhz[sl] = 0.0;
+    } else {
+      if (wd & kActiveBit) {
+        a.nl.hx[sl] = 0.0;
+        a.nl.hy[sl] = 0.0;
+        a.nl.hz[sl] = 0.0;
+        if (KR) {
+          a.nl.mhx[sl] = 0.0;
+          a.nl.mhy[sl] = 0.0;
+          a.nl.mhz[sl] = 0.0;
+        }
+        word[sl] = wd & ~kActiveBit;
       }
-      word[sl] = wd & ~kActiveBit;
+      if (!INTEG && a.cstate && owner) {
+        // no contact: dn reported 0, no forces (contact_sphere.hpp:197-200); the dump skips the entry
+        double* r = a.cstate + (uint64_t)kCStateLen * sl;
+#pragma unroll
+        for (int q = 0; q < kCStateLen; q++) r[q] = 0.0;
+      }
     }
   }
   if (WALLS) {
@@ -225,6 +250,11 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
       if (dn > 0) {
         a.wl.hx[j] = 0.0; a.wl.hy[j] = 0.0; a.wl.hz[j] = 0.0;
         a.wl.mhx[j] = 0.0; a.wl.mhy[j] = 0.0; a.wl.mhz[j] = 0.0;
+        if (!INTEG && a.wstate) {
+          double* r = a.wstate + (uint64_t)kCStateLen * j;
+#pragma unroll
+          for (int q = 0; q < kCStateLen; q++) r[q] = 0.0;
+        }
         continue;
       }
       const V3 nrm = s2p / d;
@@ -233,6 +263,14 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
       V3 mint{0.0, 0.0, 0.0};
       if (KR) mint = {a.wl.mhx[j], a.wl.mhy[j], a.wl.mhz[j]};
       const V3 f = hooke_force<KR>(dn, nrm, a.dt, a.cpd, mx, ft, mint, cpos, rx, vx, wx, sf.center, sf.vel, sf.vrot);
+      if (!INTEG && a.wstate) {
+        const V3 fnv = f - ft;
+        const double rec[kCStateLen] = {dn,   cpos.x, cpos.y, cpos.z, fnv.x, fnv.y, fnv.z, ft.x,
+                                        ft.y, ft.z,   rx.x,   rx.y,   rx.z,   0.0,   0.0,   0.0};
+        double* r = a.wstate + (uint64_t)kCStateLen * j;
+#pragma unroll
+        for (int q = 0; q < kCStateLen; q++) r[q] = rec[q];
+      }
       a.wl.hx[j] = ft.x; a.wl.hy[j] = ft.y; a.wl.hz[j] = ft.z;
       if (KR) {
         a.wl.mhx[j] = mint.x; a.wl.mhy[j] = mint.y; a.wl.mhz[j] = mint.z;
@@ -337,6 +375,7 @@ int launch_step(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, 
     else hipLaunchKernelGGL((k_force_fp64<true, false, false, true, true>), grid, block, 0, c->stream, a);
   }
   XDEM_HIP(c, hipGetLastError());
+  c->cs_valid = false;  // the integrating pass does not record
   *input = live(c).core;
   std::swap(live(c).core, c->core_alt);
   c->trig_slot = a.trig_out;
@@ -369,10 +408,134 @@ int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd,
   a.flags = c->d_flags;
   a.step_tag = step_tag;
   a.trig_in = c->trig_slot;
+  if (c->record_cs) {
+    const NbrList& L = c->nl[c->nl_cur];
+    if (L.slots > c->cap_cstate) {
+      const int64_t cap = (int64_t)(L.slots * 1.25) + 4096;
+      if ((rc = dev_realloc(c, &c->cstate, cap * kCStateLen))) return rc;
+      c->cap_cstate = cap;
+    }
+    a.cstate = c->cstate;
+    if (walls) {
+      const int64_t m = (int64_t)c->nwalls * c->cap;
+      if (m > c->cap_wstate) {
+        if ((rc = dev_realloc(c, &c->wstate, m * kCStateLen))) return rc;
+        c->cap_wstate = m;
+      }
+      a.wstate = c->wstate;
+    }
+  }
   if (fused) launch_f<true>(c, a, c->kr_used, walls, guarded);
   else launch_f<false>(c, a, c->kr_used, walls, guarded);
   XDEM_HIP(c, hipGetLastError());
+  c->cs_valid = c->record_cs;
+  c->cs_epoch = c->nl_epoch;
   return XDEM_OK;
 }
 
 }  // namespace xdem
+
+namespace xdem {
+// Rows of the recorded contact state that the reference's interaction dump writes
+// (src/interaction/include/exaDEM/itools/writer.hpp:35-75): owner-side interactions with a non-zero
+// force or a negative dn; particle pairs in device order, then walls.
+static int contact_state_rows(xdem_ctx* c, std::vector<uint64_t>& ii, std::vector<uint64_t>& jj,
+                              std::vector<uint16_t>& tt, std::vector<double>& vals) {
+  if (!c->record_cs) return set_err(c, XDEM_ERR_STATE, "contact state: recording is off (xdem_set_contact_record)");
+  if (!c->nl_valid || c->n == 0) return XDEM_OK;
+  if (!c->cs_valid || c->cs_epoch != c->nl_epoch)
+    return set_err(c, XDEM_ERR_STATE, "contact state: the last sphere force pass was not a recorded FP64 pass");
+  const int64_t n = c->n;
+  const NbrList& L = c->nl[c->nl_cur];
+  const int64_t nw = (n + kWave - 1) / kWave;
+  std::vector<uint32_t> cnt(n), word(L.slots);
+  std::vector<uint64_t> base(nw + 1), ids(n);
+  std::vector<double> cs((size_t)L.slots * kCStateLen);
+  XDEM_HIP(c, hipMemcpyAsync(cnt.data(), L.cnt, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipMemcpyAsync(base.data(), L.base, sizeof(uint64_t) * (nw + 1), hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipMemcpyAsync(ids.data(), live(c).id, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, c->stream));
+  if (L.slots > 0) {
+    XDEM_HIP(c, hipMemcpyAsync(word.data(), L.word, sizeof(uint32_t) * L.slots, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(cs.data(), c->cstate, sizeof(double) * kCStateLen * L.slots, hipMemcpyDeviceToHost,
+                               c->stream));
+  }
+  const bool walls = c->nwalls > 0 && c->wl_valid && c->wstate;
+  std::vector<uint8_t> mask;
+  std::vector<double> ws;
+  if (walls) {
+    mask.resize(n);
+    ws.resize((size_t)c->nwalls * c->cap * kCStateLen);
+    XDEM_HIP(c, hipMemcpyAsync(mask.data(), c->wl[c->wl_cur].mask, n, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(ws.data(), c->wstate, sizeof(double) * ws.size(), hipMemcpyDeviceToHost, c->stream));
+  }
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  auto keep = [](const double* v) {
+    const double f = std::sqrt(v[4] * v[4] + v[5] * v[5] + v[6] * v[6]) + std::sqrt(v[7] * v[7] + v[8] * v[8] + v[9] * v[9]);
+    return f != 0 || v[0] < 0;
+  };
+  for (int64_t x = 0; x < n; x++) {
+    const uint64_t b = base[x >> 6] + (uint64_t)(x & 63);
+    for (uint32_t e = 0; e < cnt[x]; e++) {
+      const uint64_t s = b + (uint64_t)e * kWave;
+      if (!(word[s] & kOwnerBit)) continue;
+      const double* v = cs.data() + (size_t)kCStateLen * s;
+      if (!keep(v)) continue;
+      ii.push_back(ids[x]);
+      jj.push_back(ids[word[s] & kIdxMask]);
+      tt.push_back(0);
+      vals.insert(vals.end(), v, v + kCStateLen);
+    }
+  }
+  if (walls)
+    for (int64_t x = 0; x < n; x++)
+      for (int w = 0; w < c->nwalls; w++) {
+        if (!((mask[x] >> w) & 1)) continue;
+        const double* v = ws.data() + (size_t)kCStateLen * ((int64_t)w * c->cap + x);
+        if (!keep(v)) continue;
+        ii.push_back(ids[x]);
+        jj.push_back((uint64_t)w);
+        tt.push_back(5);
+        vals.insert(vals.end(), v, v + kCStateLen);
+      }
+  return XDEM_OK;
+}
+}  // namespace xdem
+
+using namespace xdem;
+
+extern "C" int xdem_set_contact_record(xdem_ctx* c, int32_t on) {
+  if (!c || on < 0) return XDEM_ERR_ARG;
+  c->record_cs = on != 0;
+  if (!c->record_cs) c->cs_valid = false;
+  return XDEM_OK;
+}
+
+extern "C" int xdem_contact_state_count(xdem_ctx* c, int64_t* n) {
+  if (!c || !n) return XDEM_ERR_ARG;
+  *n = 0;
+  std::vector<uint64_t> ii, jj;
+  std::vector<uint16_t> tt;
+  std::vector<double> v;
+  const int rc = contact_state_rows(c, ii, jj, tt, v);
+  if (rc) return rc;
+  *n = (int64_t)ii.size();
+  return XDEM_OK;
+}
+
+extern "C" int xdem_contact_state_dump(xdem_ctx* c, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint16_t* type,
+                                       double* values, int64_t* written) {
+  if (!c || !id_i || !id_j || !type || !values || cap < 0) return XDEM_ERR_ARG;
+  if (written) *written = 0;
+  std::vector<uint64_t> ii, jj;
+  std::vector<uint16_t> tt;
+  std::vector<double> v;
+  const int rc = contact_state_rows(c, ii, jj, tt, v);
+  if (rc) return rc;
+  if ((int64_t)ii.size() > cap) return set_err(c, XDEM_ERR_CAPACITY, "contact_state_dump: cap too small");
+  std::copy(ii.begin(), ii.end(), id_i);
+  std::copy(jj.begin(), jj.end(), id_j);
+  std::copy(tt.begin(), tt.end(), type);
+  std::copy(v.begin(), v.end(), values);
+  if (written) *written = (int64_t)ii.size();
+  return XDEM_OK;
+}

@@ -418,6 +418,7 @@ int launch_force_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams&
     else walls ? launch32<false, false, true>(c, a, guarded) : launch32<false, false, false>(c, a, guarded);
   }
   XDEM_HIP(c, hipGetLastError());
+  c->cs_valid = false;  // the FP32 form does not record
   return XDEM_OK;
 }
 
@@ -463,6 +464,7 @@ int launch_step_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams& 
   *input = live(c).core;
   std::swap(live(c).core, c->core_alt);
   c->trig_slot = a.trig_out;
+  c->cs_valid = false;  // the FP32 form does not record
   return XDEM_OK;
 }
 

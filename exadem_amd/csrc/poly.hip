@@ -929,6 +929,7 @@ int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, d
 #undef PAIRS
 #undef GATHER
   XDEM_HIP(c, hipGetLastError());
+  c->cs_valid = false;  // polyhedron passes do not record
   return XDEM_OK;
 }
 

@@ -29,6 +29,10 @@ constexpr uint32_t kActiveBit = 1u << 30;
 constexpr uint32_t kOwnerBit = 1u << 31;
 constexpr int kNoShift = 13;
 constexpr int kWave = 64;
+// recorded contact state per interaction: dn, contact point (3), normal force f - ft (3), tangential
+// force ft (3), owner position (3), partner position (3; the periodic image across a boundary, zero
+// for walls)
+constexpr int kCStateLen = 16;
 
 struct alignas(32) d4 {
   double x, y, z, w;
@@ -273,6 +277,15 @@ struct xdem_ctx {
   int nl_max_rows = 0;          // largest per-wavefront row count of the current list
   xdem::Core* core_alt = nullptr;  // second record buffer of the integrating force pass (ping-pong)
   int32_t trig_slot = 0;        // d_flags slot of the pending rebuild trigger (0, or 3 after an integrating pass)
+  // contact state recorded by FP64 sphere force passes outside the integrating engine step (analysis
+  // dumps, xdem_set_contact_record): per list slot and per (wall, particle) kCStateLen doubles
+  bool record_cs = false;
+  bool cs_valid = false;        // cstate/wstate hold the last force pass of list epoch cs_epoch
+  uint64_t cs_epoch = 0;
+  double* cstate = nullptr;
+  int64_t cap_cstate = 0;       // slots
+  double* wstate = nullptr;
+  int64_t cap_wstate = 0;       // (wall, particle) entries
   double rcut_inc = 0;
   double nbh_dist = 0;
   xdem::WallList wl[2];
@@ -331,6 +344,15 @@ int scan_u64_from_u32(xdem_ctx* c, const uint32_t* in, uint64_t* out, int64_t n,
 int read_u64(xdem_ctx* c, const uint64_t* d, uint64_t* h);
 
 inline int grid_blocks(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// (Re)allocates a device array of count elements; the old contents are dropped.
+template <typename T>
+int dev_realloc(xdem_ctx* c, T** p, int64_t count) {
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  XDEM_HIP(c, hipMalloc((void**)p, sizeof(T) * (size_t)(count > 1 ? count : 1)));
+  return XDEM_OK;
+}
 
 // kernels launched from api code
 int launch_prolog(xdem_ctx* c, double dt, bool with_kick, double kick_dt, bool displ, double thr2, bool guarded);

@@ -89,7 +89,8 @@ HIP_SYMBOLS = [
     "xdem_combined_compute_prolog", "xdem_combined_compute_epilog", "xdem_reset_force_moment",
     "xdem_gravity_force", "xdem_force_to_accel", "xdem_particle_displ_over", "xdem_move_particles",
     "xdem_chunk_neighbors_contact", "xdem_nbh_sphere", "xdem_classify_interactions", "xdem_contact_sphere",
-    "xdem_run_steps", "xdem_interactions_count", "xdem_interactions_dump", "xdem_nbh_stats",
+    "xdem_run_steps", "xdem_interactions_count", "xdem_interactions_dump", "xdem_set_contact_record",
+    "xdem_contact_state_count", "xdem_contact_state_dump", "xdem_nbh_stats",
     "xdem_kernel_timing", "xdem_set_timing", "xdem_streams_dump",
     "xdem_set_shapes", "xdem_compute_vertices", "xdem_vertex_displ_over", "xdem_nbh_polyhedron",
     "xdem_contact_polyhedron", "xdem_poly_interactions_count", "xdem_poly_interactions_dump",
@@ -172,6 +173,10 @@ def load():
         "xdem_interactions_count": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "xdem_interactions_dump": (C.c_int, [vp, C.c_int64, _u64p, _u64p, C.POINTER(C.c_uint16), _dp,
                                              C.POINTER(C.c_int64)]),
+        "xdem_set_contact_record": (C.c_int, [vp, C.c_int32]),
+        "xdem_contact_state_count": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "xdem_contact_state_dump": (C.c_int, [vp, C.c_int64, _u64p, _u64p, C.POINTER(C.c_uint16), _dp,
+                                              C.POINTER(C.c_int64)]),
         "xdem_streams_dump": (C.c_int, [vp, C.c_int64, _u64p, _u64p, C.POINTER(C.c_int32),
                                         C.POINTER(C.c_uint8), C.POINTER(C.c_int64)]),
         "xdem_nbh_stats": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
@@ -359,6 +364,27 @@ class Context:
                   _ptr(tt, C.c_uint16), _ptr(fr, C.c_double), C.byref(wr))
         k = wr.value
         return ii[:k], jj[:k], tt[:k], fr[:3 * k].reshape(-1, 3)
+
+    def set_contact_record(self, on=True):
+        """Record the contact state of FP64 sphere force passes (outside the engine's integrating step)."""
+        self.call("xdem_set_contact_record", 1 if on else 0)
+
+    def contact_state(self):
+        """The recorded rows of the last force pass, as the reference's interaction dump selects them:
+        owner id, partner id (wall index), type (0 pair, 5 wall) and per row 16 values: dn, contact
+        point, f - ft, ft, owner position, partner position."""
+        cnt = C.c_int64()
+        self.call("xdem_contact_state_count", C.byref(cnt))
+        cap = max(1, cnt.value)
+        ii = np.zeros(cap, np.uint64)
+        jj = np.zeros(cap, np.uint64)
+        tt = np.zeros(cap, np.uint16)
+        vals = np.zeros(16 * cap, np.float64)
+        wr = C.c_int64()
+        self.call("xdem_contact_state_dump", cap, _ptr(ii, C.c_uint64), _ptr(jj, C.c_uint64), _ptr(tt, C.c_uint16),
+                  _ptr(vals, C.c_double), C.byref(wr))
+        k = wr.value
+        return ii[:k], jj[:k], tt[:k], vals[:16 * k].reshape(-1, 16)
 
     def streams(self):
         pairs, slots = C.c_int64(), C.c_int64()

@@ -408,3 +408,77 @@ def test_engine_variants_equal_operator_path(gpu_ok, form, case, kr):
     pa, pb = a.ctx.download(), b.ctx.download()
     for k in ("rx", "ry", "rz", "vx", "vy", "vz", "fx", "fy", "fz", "momx", "vrotx", "vroty", "qw", "qx", "arotx"):
         assert np.array_equal(pa[k], pb[k]), k
+
+
+@pytest.mark.parametrize("case", ["walls", "periodic"])
+def test_contact_state_dump(gpu_ok, case, tmp_path):
+    """Recorded contact state of a force pass (the reference's analysis tuple dn, contact point, f - ft,
+    ft, with the positions the interaction dump writes): dn and the contact point agree with the recorded
+    positions and radii, ft is the friction history of the interaction, the rows' forces add up to each
+    particle's force (owner rows, the partner side of interior pairs, walls, gravity), the engine records
+    what the operator path records, and the dump file reads back."""
+    from exadem_amd import itools
+    w = {"walls": wall_deposition, "periodic": periodic_pack}[case]()
+    s = gpu_for(w)
+    with pytest.raises(native.XdemError, match="STATE"):
+        s.ctx.contact_state()
+    s.ctx.set_contact_record(True)
+    s.first_iteration()
+    ii, jj, tt, v = s.ctx.contact_state()
+    p = by_id(s.ctx.download())
+    assert np.array_equal(p["id"], np.arange(len(p["id"])))
+    pp = tt == 0
+    assert np.count_nonzero(pp) > 0
+    if case == "walls":
+        assert np.count_nonzero(tt == 5) > 0
+    R = p["radius"]
+    ri, rj = v[:, 10:13], v[:, 13:16]
+    assert np.array_equal(ri, pos(p)[ii.astype(np.int64)])
+    d = ri[pp] - rj[pp]
+    dist = np.linalg.norm(d, axis=1)
+    dn = dist - (R[ii[pp].astype(np.int64)] + R[jj[pp].astype(np.int64)])
+    assert np.all(v[pp, 0] < 0)
+    assert float(np.max(np.abs(v[pp, 0] - dn))) <= 1e-12
+    nrm = d / dist[:, None]
+    cp = ri[pp] - nrm * (R[ii[pp].astype(np.int64)] + 0.5 * v[pp, 0])[:, None]
+    assert float(np.max(np.abs(v[pp, 1:4] - cp))) <= 1e-12
+    hi, hj, ht, hf = s.ctx.interactions()
+    hist = {(int(a), int(b), int(t)): f for a, b, t, f in zip(hi, hj, ht, hf)}
+    for k in range(len(ii)):
+        assert np.array_equal(v[k, 7:10], hist[(int(ii[k]), int(jj[k]), int(tt[k]))])
+    # per particle: gravity, + f of its own rows, - f of the interior pairs it is the partner of
+    # (a pair across a periodic boundary has a row on each side)
+    F = p["mass"][:, None] * np.asarray(w.gravity)[None, :]
+    f = v[:, 4:7] + v[:, 7:10]
+    keys = set(zip(ii.tolist(), jj.tolist(), tt.tolist()))
+    for k in range(len(ii)):
+        a = int(ii[k])
+        F[a] += f[k]
+        if tt[k] == 0 and (int(jj[k]), a, 0) not in keys:
+            F[int(jj[k])] -= f[k]
+    assert rel_err(F, force(p), p["mass"] * 9.81) <= 1e-12
+    # the engine's last step records the same rows as the operator path
+    a = gpu_for(w)
+    a.ctx.set_contact_record(True)
+    a.first_iteration()
+    for _ in range(20):
+        a.step()
+    b = gpu_for(w)
+    b.ctx.set_contact_record(True)
+    b.first_iteration()
+    b.run(20, poll=7)
+    for x, y in zip(a.ctx.contact_state(), b.ctx.contact_state()):
+        assert np.array_equal(x, y)
+    # the dump file in the reference's layout reads back to 6 significant digits
+    path, rows = itools.write_dump(b.ctx, str(tmp_path), "InteractionOutputDir-", 20)
+    keys2, vals2 = itools.read_dump(path)
+    bi, bj, bt, bv = b.ctx.contact_state()
+    assert rows == len(bi) > 0
+    assert np.array_equal(keys2[:, 0], bi.astype(np.int64)) and np.array_equal(keys2[:, 4], bt.astype(np.int64))
+    assert np.allclose(vals2, bv, rtol=1e-5, atol=1e-300)
+    # the FP32 form does not record
+    c = SphereSim(w, form=native.XDEM_FORM_FP32)
+    c.ctx.set_contact_record(True)
+    c.first_iteration()
+    with pytest.raises(native.XdemError, match="STATE"):
+        c.ctx.contact_state()
