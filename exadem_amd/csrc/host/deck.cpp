@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
 #include <functional>
 #include <map>
@@ -82,6 +83,7 @@ class Runner {
   void run(xdem_ctx* c) {
     ctx_ = c;
     stats = {};
+    timestep_ = (int64_t)gdouble("timestep", 0);
     const auto t0 = std::chrono::steady_clock::now();
     exec("simulation", nullptr, 0);
     stats.particles = (int64_t)P_.size();
@@ -99,6 +101,7 @@ class Runner {
   double bmin_[3] = {0, 0, 0}, bmax_[3] = {0, 0, 0};
   bool have_bounds_ = false;
   uint64_t seed_ = 0;
+  int64_t timestep_ = 0;  // the deck's timestep slot: 0 in first_iteration, k in the k-th loop iteration
 
   // ---------------------------------------------------------------- helpers
   [[noreturn]] void fail(int code, const std::string& m) { throw Fail(code, m); }
@@ -534,14 +537,59 @@ class Runner {
       if (f->s == "fp32") form = XDEM_FORM_FP32;
       else if (f->s != "fp64") fail(XDEM_ERR_ARG, "contact: precision is fp64 or fp32");
     }
+    // analysis (contact_force_sphere.cu.in:62-66, 190-199): every analysis_interaction_dump_frequency
+    // steps the interactions of this pass go to <dir_name>/ExaDEMAnalyses/<basename><timestep>/
+    const Node* fq = slot(s, "analysis_interaction_dump_frequency");
+    const int64_t freq =
+        fq ? (int64_t)xdem_host::to_double(*fq) : (int64_t)gdouble("analysis_interaction_dump_frequency", -1);
+    const bool dump = freq > 0 && timestep_ % freq == 0;
+    if (dump && poly) fail(XDEM_ERR_UNSUPPORTED, "contact_polyhedron: interaction dumps are provided for spheres");
+    if (dump) check(xdem_set_contact_record(ctx_, 1), "contact_sphere");
     if (poly) check(xdem_contact_polyhedron(ctx_, &cp, cfd ? &cpd : nullptr, dt()), "contact_polyhedron");
     else check(xdem_contact_sphere(ctx_, &cp, cfd ? &cpd : nullptr, dt(), form), "contact_sphere");
+    if (dump) {
+      write_interaction_dump(sstring(s, "dir_name", "ExaDEMOutputDir"),
+                             sstring(s, "interaction_basename", "InteractionOutputDir-"));
+      check(xdem_set_contact_record(ctx_, 0), "contact_sphere");
+    }
+  }
+
+  std::string sstring(const Node& s, const char* k, const char* dflt) {
+    if (const Node* v = slot(s, k)) return v->s;
+    if (const Node* v = global().get(k)) return v->s;
+    return dflt;
+  }
+
+  // itools::create_buffer + write_file (src/interaction/include/exaDEM/itools/writer.hpp:35-115): one
+  // row per interaction in contact, owner side, numbers as a default std::ostream writes them
+  void write_interaction_dump(const std::string& dir, const std::string& base) {
+    int64_t n = 0;
+    check(xdem_contact_state_count(ctx_, &n), "interaction dump");
+    std::vector<uint64_t> ii((size_t)std::max<int64_t>(n, 1)), jj(ii.size());
+    std::vector<uint16_t> tt(ii.size());
+    std::vector<double> v(16 * ii.size());
+    int64_t w = 0;
+    if (n > 0) check(xdem_contact_state_dump(ctx_, n, ii.data(), jj.data(), tt.data(), v.data(), &w), "interaction dump");
+    const std::string name = base + std::to_string(timestep_);
+    const std::string sub = dir + "/ExaDEMAnalyses/" + name;
+    std::error_code ec;
+    std::filesystem::create_directories(sub, ec);
+    if (ec) fail(XDEM_ERR_ARG, "interaction dump: cannot create " + sub);
+    std::ofstream out(sub + "/" + name + "_0.txt");
+    if (!out) fail(XDEM_ERR_ARG, "interaction dump: cannot write in " + sub);
+    for (int64_t k = 0; k < w; k++) {
+      out << ii[k] << "," << jj[k] << ",0,0," << tt[k];
+      for (int q = 0; q < 16; q++) out << "," << v[16 * k + q];
+      out << "\n";
+    }
+    stats.operator_calls++;
   }
 
   void compute_loop() {
     const int64_t end = (int64_t)gdouble("simulation_end_iteration", 0);
     int64_t t = (int64_t)gdouble("timestep", 0);
     for (; t < end; t++) {
+      timestep_ = t + 1;  // next_time_step ran after first_iteration and after each iteration
       exec("numerical_scheme", nullptr, 1);
       stats.iterations++;
     }

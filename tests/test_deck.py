@@ -139,3 +139,37 @@ def test_deck_unsupported_operator_fails_loudly(gpu_ok, tmp_path):
     deck = native.Deck(write(tmp_path / "cohesive.msp", text))
     with pytest.raises(native.XdemError, match="UNSUPPORTED"):
         deck.run(native.Context())
+
+
+@pytest.mark.gpu
+def test_deck_interaction_dumps(gpu_ok, tmp_path):
+    """analysis_interaction_dump_frequency (contact_force_sphere.cu.in:190-199): contact_sphere writes
+    the interactions of its pass every 10 steps, at timesteps 0 (first_iteration), 10 and 20, in the
+    reference's layout <dir_name>/ExaDEMAnalyses/<basename><timestep>/<basename><timestep>_0.txt. The file
+    at timestep 20 is, character for character, the operator path's recorded state after 20 steps as
+    exadem_amd.itools writes it."""
+    from exadem_amd import itools
+    w = wall_deposition()
+    path = wall_deck(tmp_path, w, 20)
+    out = tmp_path / "out"
+    text = open(path).read().replace(
+        "  simulation_end_iteration: 20\n",
+        f"  simulation_end_iteration: 20\n  analysis_interaction_dump_frequency: 10\n  dir_name: \"{out}\"\n")
+    deck = native.Deck(write(tmp_path / "dumps.msp", text))
+    ctx = native.Context()
+    st = deck.run(ctx)
+    assert st["iterations"] == 20
+    files = sorted(p.relative_to(out).as_posix() for p in out.rglob("*.txt"))
+    assert files == [f"ExaDEMAnalyses/InteractionOutputDir-{t}/InteractionOutputDir-{t}_0.txt" for t in (0, 10, 20)]
+    ref = SphereSim(w)
+    ref.ctx.set_contact_record(True)
+    ref.first_iteration()
+    for _ in range(20):
+        ref.step()
+    ii, jj, tt, vals = ref.ctx.contact_state()
+    zero = np.zeros(len(ii), np.int64)
+    lines = itools.format_rows(ii, jj, zero, zero, tt, vals)
+    got = open(itools.dump_path(str(out), "InteractionOutputDir-", 20)).read().splitlines()
+    assert len(got) > 0 and got == lines
+    keys, _ = itools.read_dump(itools.dump_path(str(out), "InteractionOutputDir-", 0))
+    assert np.count_nonzero(keys[:, 4] == 5) > 0  # wall contacts at timestep 0
