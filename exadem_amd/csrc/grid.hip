@@ -11,6 +11,7 @@
 // carry-over looks each new entry up among the owner's previous entries (owner id, partner id key).
 #include <algorithm>
 #include <cstring>
+#include <unordered_map>
 
 #include "xdem_internal.h"
 
@@ -661,5 +662,112 @@ extern "C" int xdem_streams_dump(xdem_ctx* c, int64_t cap, uint64_t* a, uint64_t
     }
   }
   if (written) *written = k;
+  return XDEM_OK;
+}
+
+// Restart with interaction history: the history of the current lists' interactions is set from a
+// saved interaction list, matched by key, as the reference's restart hands dumped histories to the
+// interaction update (extract_history / update, interaction_manager.hpp:88-112 and 205-217).
+namespace {
+struct IKey {
+  uint64_t i, j;
+  uint16_t t;
+  bool operator==(const IKey& o) const { return i == o.i && j == o.j && t == o.t; }
+};
+struct IKeyHash {
+  size_t operator()(const IKey& k) const {
+    uint64_t h = k.i * 0x9e3779b97f4a7c15ull;
+    h ^= k.j + 0x7f4a7c159e3779b9ull + (h << 6) + (h >> 2);
+    h ^= (uint64_t)k.t + (h << 6) + (h >> 2);
+    return (size_t)h;
+  }
+};
+}  // namespace
+
+extern "C" int xdem_interactions_load(xdem_ctx* c, int64_t m, const uint64_t* id_i, const uint64_t* id_j,
+                                      const uint16_t* type, const double* friction, const double* moment,
+                                      int64_t* matched) {
+  if (!c || m < 0 || (m > 0 && (!id_i || !id_j || !type || !friction))) return XDEM_ERR_ARG;
+  if (matched) *matched = 0;
+  if (c->nshapes > 0) return set_err(c, XDEM_ERR_UNSUPPORTED, "interactions_load: sphere interactions only");
+  if (!c->nl_valid || !c->hist_ready) return set_err(c, XDEM_ERR_STATE, "interactions_load: run nbh_sphere first");
+  int rc = set_history_form(c, false);
+  if (rc) return rc;
+  const int64_t n = c->n;
+  if (n == 0) return XDEM_OK;
+  std::unordered_map<IKey, int64_t, IKeyHash> saved;
+  saved.reserve((size_t)m * 2);
+  for (int64_t k = 0; k < m; k++)
+    if (!saved.emplace(IKey{id_i[k], id_j[k], type[k]}, k).second)
+      return set_err(c, XDEM_ERR_ARG, "interactions_load: duplicate interaction key");
+  NbrList& L = c->nl[c->nl_cur];
+  const int64_t nw = (n + kWave - 1) / kWave;
+  std::vector<uint32_t> cnt(n), word(L.slots);
+  std::vector<uint64_t> base(nw + 1), ids(n);
+  XDEM_HIP(c, hipMemcpyAsync(cnt.data(), L.cnt, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipMemcpyAsync(base.data(), L.base, sizeof(uint64_t) * (nw + 1), hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipMemcpyAsync(ids.data(), live(c).id, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, c->stream));
+  if (L.slots > 0)
+    XDEM_HIP(c, hipMemcpyAsync(word.data(), L.word, sizeof(uint32_t) * L.slots, hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  // both copies of a pair (the owner entry and the partner's entry) take the pair's history
+  std::vector<double> h[6];
+  for (auto& a : h) a.assign((size_t)L.slots, 0.0);
+  int64_t hit = 0;
+  for (int64_t x = 0; x < n; x++) {
+    const uint64_t b = base[x >> 6] + (uint64_t)(x & 63);
+    for (uint32_t e = 0; e < cnt[x]; e++) {
+      const uint64_t s = b + (uint64_t)e * kWave;
+      const uint32_t wd = word[s];
+      const uint64_t py = ids[wd & kIdxMask];
+      const bool own = (wd & kOwnerBit) != 0;
+      const auto it = saved.find(own ? IKey{ids[x], py, 0} : IKey{py, ids[x], 0});
+      bool act = false;
+      if (it != saved.end()) {
+        const int64_t k = it->second;
+        for (int q = 0; q < 3; q++) {
+          h[q][s] = friction[3 * k + q];
+          h[3 + q][s] = moment ? moment[3 * k + q] : 0.0;
+          act = act || h[q][s] != 0.0 || h[3 + q][s] != 0.0;
+        }
+        if (own) hit++;
+      }
+      word[s] = act ? (wd | kActiveBit) : (wd & ~kActiveBit);
+    }
+  }
+  if (L.slots > 0) {
+    double* dst[6] = {L.hx, L.hy, L.hz, L.mhx, L.mhy, L.mhz};
+    for (int q = 0; q < 6; q++)
+      XDEM_HIP(c, hipMemcpyAsync(dst[q], h[q].data(), sizeof(double) * L.slots, hipMemcpyHostToDevice, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(L.word, word.data(), sizeof(uint32_t) * L.slots, hipMemcpyHostToDevice, c->stream));
+  }
+  if (c->nwalls > 0 && c->wl_valid) {
+    WallList& W = c->wl[c->wl_cur];
+    const int64_t mw = (int64_t)c->nwalls * c->cap;
+    std::vector<uint8_t> mask(n);
+    XDEM_HIP(c, hipMemcpyAsync(mask.data(), W.mask, n, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<double> wh[6];
+    for (auto& a : wh) a.assign((size_t)mw, 0.0);
+    for (int64_t x = 0; x < n; x++)
+      for (int w = 0; w < c->nwalls; w++) {
+        if (!((mask[x] >> w) & 1)) continue;
+        const auto it = saved.find(IKey{ids[x], (uint64_t)w, 5});
+        if (it == saved.end()) continue;
+        const int64_t k = it->second, j = (int64_t)w * c->cap + x;
+        for (int q = 0; q < 3; q++) {
+          wh[q][j] = friction[3 * k + q];
+          wh[3 + q][j] = moment ? moment[3 * k + q] : 0.0;
+        }
+        hit++;
+      }
+    double* dst[6] = {W.hx, W.hy, W.hz, W.mhx, W.mhy, W.mhz};
+    for (int q = 0; q < 6; q++)
+      XDEM_HIP(c, hipMemcpyAsync(dst[q], wh[q].data(), sizeof(double) * mw, hipMemcpyHostToDevice, c->stream));
+  }
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  if (moment)
+    for (int64_t k = 0; k < 3 * m && !c->kr_used; k++) c->kr_used = moment[k] != 0.0;
+  if (matched) *matched = hit;
   return XDEM_OK;
 }

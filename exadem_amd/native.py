@@ -89,8 +89,8 @@ HIP_SYMBOLS = [
     "xdem_combined_compute_prolog", "xdem_combined_compute_epilog", "xdem_reset_force_moment",
     "xdem_gravity_force", "xdem_force_to_accel", "xdem_particle_displ_over", "xdem_move_particles",
     "xdem_chunk_neighbors_contact", "xdem_nbh_sphere", "xdem_classify_interactions", "xdem_contact_sphere",
-    "xdem_run_steps", "xdem_interactions_count", "xdem_interactions_dump", "xdem_set_contact_record",
-    "xdem_contact_state_count", "xdem_contact_state_dump", "xdem_nbh_stats",
+    "xdem_run_steps", "xdem_interactions_count", "xdem_interactions_dump", "xdem_interactions_load",
+    "xdem_set_contact_record", "xdem_contact_state_count", "xdem_contact_state_dump", "xdem_nbh_stats",
     "xdem_kernel_timing", "xdem_set_timing", "xdem_streams_dump",
     "xdem_set_shapes", "xdem_compute_vertices", "xdem_vertex_displ_over", "xdem_nbh_polyhedron",
     "xdem_contact_polyhedron", "xdem_poly_interactions_count", "xdem_poly_interactions_dump",
@@ -172,6 +172,8 @@ def load():
         "xdem_run_steps": (C.c_int, [vp, C.POINTER(Scheme), C.c_int64, C.POINTER(RunStats)]),
         "xdem_interactions_count": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "xdem_interactions_dump": (C.c_int, [vp, C.c_int64, _u64p, _u64p, C.POINTER(C.c_uint16), _dp,
+                                             C.POINTER(C.c_int64)]),
+        "xdem_interactions_load": (C.c_int, [vp, C.c_int64, _u64p, _u64p, C.POINTER(C.c_uint16), _dp, _dp,
                                              C.POINTER(C.c_int64)]),
         "xdem_set_contact_record": (C.c_int, [vp, C.c_int32]),
         "xdem_contact_state_count": (C.c_int, [vp, C.POINTER(C.c_int64)]),
@@ -364,6 +366,20 @@ class Context:
                   _ptr(tt, C.c_uint16), _ptr(fr, C.c_double), C.byref(wr))
         k = wr.value
         return ii[:k], jj[:k], tt[:k], fr[:3 * k].reshape(-1, 3)
+
+    def interactions_load(self, ii, jj, tt, friction, moment=None):
+        """Sets the history of the current interactions from saved rows (restart); returns the number
+        of rows applied."""
+        ii = np.ascontiguousarray(ii, np.uint64)
+        jj = np.ascontiguousarray(jj, np.uint64)
+        tt = np.ascontiguousarray(tt, np.uint16)
+        fr = np.ascontiguousarray(friction, np.float64).reshape(-1)
+        mo = None if moment is None else np.ascontiguousarray(moment, np.float64).reshape(-1)
+        assert fr.size == 3 * ii.size and (mo is None or mo.size == fr.size)
+        got = C.c_int64()
+        self.call("xdem_interactions_load", ii.size, _ptr(ii, C.c_uint64), _ptr(jj, C.c_uint64), _ptr(tt, C.c_uint16),
+                  _ptr(fr, C.c_double), None if mo is None else _ptr(mo, C.c_double), C.byref(got))
+        return got.value
 
     def set_contact_record(self, on=True):
         """Record the contact state of FP64 sphere force passes (outside the engine's integrating step)."""

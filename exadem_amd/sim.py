@@ -6,6 +6,10 @@ compute_force = [gravity_force, contact_sphere] and the rebuild branch of
 config_move_particles.msp:141-195). `run` hands whole blocks of steps to the fused engine
 (xdem_run_steps), which produces the same state with two kernels per step.
 """
+import copy
+
+import numpy as np
+
 from . import native
 
 
@@ -69,6 +73,29 @@ class SphereSim:
         st = self.ctx.run_steps(scheme(self.w, self.form, poll), nsteps)
         self.rebuilds += st["rebuilds"]
         return st
+
+    def checkpoint(self, path):
+        """Restart file (numpy .npz): every particle field, in device order, and the interactions with
+        their friction history (the layout of xdem_interactions_dump) - what the reference's restart
+        dump carries (sim_dump_writer_interaction.cpp:25-46). The rolling-moment history is not saved
+        (xdem_interactions_dump has no moment column)."""
+        p = self.ctx.download()
+        ii, jj, tt, fr = self.ctx.interactions()
+        np.savez(path, ii=ii, jj=jj, tt=tt, fr=fr, **{"p_" + k: v for k, v in p.items()})
+
+    @classmethod
+    def restart(cls, workload, path, device=0, form=native.XDEM_FORM_FP64, history=True):
+        """A simulation continued from a checkpoint: the saved particles (domain, walls and parameters
+        from the workload), lists rebuilt at the saved positions, the saved history handed to the
+        current interactions by key (xdem_interactions_load)."""
+        d = np.load(path)
+        w = copy.copy(workload)
+        w.particles = {k[2:]: d[k] for k in d.files if k.startswith("p_")}
+        sim = cls(w, device=device, form=form)
+        sim.rebuild()
+        if history:
+            sim.ctx.interactions_load(d["ii"], d["jj"], d["tt"], d["fr"])
+        return sim
 
 
 class PolySim(SphereSim):

@@ -233,6 +233,16 @@ int xdem_interactions_count(xdem_ctx* ctx, int64_t* n_pp, int64_t* n_pw);
  * friction history (3 doubles per interaction). cap is the array length. */
 int xdem_interactions_dump(xdem_ctx* ctx, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint16_t* type,
                            double* friction, int64_t* written);
+/* Restart with interaction history (replaces the history hand-over of the reference's restart: dumped
+ * interactions re-enter the interaction update by key, interaction_manager.hpp:88-112 and 205-217,
+ * sim_dump_writer_interaction.cpp:25-46). After nbh_sphere on the restored particles, sets the history
+ * of every current interaction whose key (owner id, partner id, type: 0 sphere pair, 5 wall with the
+ * wall index as partner id) is in the m saved rows (the layout of xdem_interactions_dump): friction
+ * (3 per row) and, when moment is not NULL, the rolling-moment history (3 per row). Both list entries
+ * of a pair take its history; current interactions without a saved row start from zero, saved rows
+ * without a current interaction are dropped. *matched = rows applied. */
+int xdem_interactions_load(xdem_ctx* ctx, int64_t m, const uint64_t* id_i, const uint64_t* id_j,
+                           const uint16_t* type, const double* friction, const double* moment, int64_t* matched);
 /* Contact-state recording for analyses (replaces the ContactState the reference's contact operators
  * fill for every interaction, contact_force_sphere.cu.in:190-199 with sphere/contact_sphere.hpp:
  * 175-202 and 282-296). With on != 0, every FP64 sphere force pass outside the engine's integrating

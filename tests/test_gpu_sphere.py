@@ -482,3 +482,81 @@ def test_contact_state_dump(gpu_ok, case, tmp_path):
     c.first_iteration()
     with pytest.raises(native.XdemError, match="STATE"):
         c.ctx.contact_state()
+
+
+def packing_stats(p, inter, dom):
+    """Coordination number 2 * (active particle pairs) / N, as the reference's interaction counter
+    forms it (itools/counter.hpp:145-151; active here: non-zero friction history, which every pair in
+    contact with a tangential velocity has), and solid fraction of the bed: sphere volume over the box
+    section times the height of the highest sphere top."""
+    ii, jj, tt, fr = inter[:4]
+    act = np.any(fr != 0, axis=1) & (tt == 0)
+    cn = 2.0 * np.count_nonzero(act) / len(p["id"])
+    top = float(np.max(p["rz"] + p["radius"]))
+    lo, hi = list(dom.lo), list(dom.hi)
+    vol = float(np.sum(4.0 / 3.0 * np.pi * p["radius"] ** 3))
+    return cn, vol / ((hi[0] - lo[0]) * (hi[1] - lo[1]) * (top - lo[2]))
+
+
+def test_packing_statistics_match_oracle(gpu_ok):
+    """Packing statistics of the contact-rich walled deposition after 100 steps: the FP64 engine's
+    coordination number and solid fraction equal the oracle's; the FP32-force form has the same
+    coordination number (bit-exact contact set) and its solid fraction within 1e-9."""
+    w = dense_deposition()
+    sch = scheme(w)
+    o = oracle_for(w)
+    o.first_iteration(sch)
+    o.run_steps(sch, 100)
+    s = gpu_for(w)
+    s.first_iteration()
+    s.run(100)
+    b = SphereSim(w, form=native.XDEM_FORM_FP32)
+    b.first_iteration()
+    b.run(100)
+    cn_o, phi_o = packing_stats(o.particles(), o.interactions(), w.domain)
+    cn_g, phi_g = packing_stats(s.ctx.download(), s.ctx.interactions(), w.domain)
+    cn_b, phi_b = packing_stats(b.ctx.download(), b.ctx.interactions(), w.domain)
+    print(f"coordination number {cn_g:.6f} (oracle {cn_o:.6f}, FP32 {cn_b:.6f}), solid fraction {phi_g:.9f} "
+          f"(oracle {phi_o:.9f}, FP32 {phi_b:.9f})")
+    assert cn_o > 1.0
+    assert cn_g == cn_o and phi_g == phi_o
+    assert cn_b == cn_g and abs(phi_b - phi_g) <= 1e-9
+
+
+@pytest.mark.parametrize("case", ["walls", "periodic"])
+def test_checkpoint_restart_with_history(gpu_ok, case, tmp_path):
+    """Restart with interaction history: a run checkpointed after 60 steps (particles and interaction
+    history) and restarted (particles uploaded, lists rebuilt, history handed over by key) continues as
+    the uninterrupted run does: after 60 more steps the same contacts (interactions with history),
+    friction history within 1e-12 and trajectories within 1e-10 (the lists are rebuilt at other positions, so the summation
+    order and the rebuild steps may differ). Restarting without the history gives another state."""
+    w = {"walls": wall_deposition, "periodic": periodic_pack}[case]()
+    a = gpu_for(w)
+    a.first_iteration()
+    a.run(60)
+    ck = str(tmp_path / "restart.npz")
+    a.checkpoint(ck)
+    nhist = int(np.count_nonzero(np.any(np.load(ck)["fr"] != 0, axis=1)))
+    assert nhist > 0
+    a.run(60)
+    b = SphereSim.restart(w, ck)
+    b.run(60)
+    c = SphereSim.restart(w, ck, history=False)
+    c.run(60)
+    pa, pb, pc = a.ctx.download(), b.ctx.download(), c.ctx.download()
+    floor = by_id(pa)["mass"] * 9.81 * 1e-3
+    compare_state(pb, pa, floor / by_id(pa)["mass"], f"{case} restart")
+    def active(sim):
+        k, f = interaction_table(*sim.ctx.interactions())
+        act = np.any(f != 0, axis=1)
+        return k[act], f[act]
+
+    # interactions with history (the contacts); the lists also hold non-contacting candidate pairs,
+    # which depend on the positions the lists were built at
+    ka, fa = active(a)
+    kb, fb = active(b)
+    kc, fc = active(c)
+    assert len(ka) > 0 and np.array_equal(ka, kb)
+    scale = float(np.max(np.abs(fa)))
+    assert float(np.max(np.abs(fb - fa))) <= 1e-12 * scale
+    assert not np.array_equal(kc, ka) or float(np.max(np.abs(fc - fa))) > 1e-6 * scale
