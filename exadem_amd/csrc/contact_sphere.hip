@@ -440,11 +440,13 @@ namespace xdem {
 // (src/interaction/include/exaDEM/itools/writer.hpp:35-75): owner-side interactions with a non-zero
 // force or a negative dn; particle pairs in device order, then walls.
 static int contact_state_rows(xdem_ctx* c, std::vector<uint64_t>& ii, std::vector<uint64_t>& jj,
-                              std::vector<uint16_t>& tt, std::vector<double>& vals) {
+                              std::vector<uint32_t>& si, std::vector<uint32_t>& sj, std::vector<uint16_t>& tt,
+                              std::vector<double>& vals) {
   if (!c->record_cs) return set_err(c, XDEM_ERR_STATE, "contact state: recording is off (xdem_set_contact_record)");
   if (!c->nl_valid || c->n == 0) return XDEM_OK;
   if (!c->cs_valid || c->cs_epoch != c->nl_epoch)
-    return set_err(c, XDEM_ERR_STATE, "contact state: the last sphere force pass was not a recorded FP64 pass");
+    return set_err(c, XDEM_ERR_STATE, "contact state: the last force pass was not a recorded FP64 pass");
+  if (c->nshapes > 0) return poly_contact_state_rows(c, ii, jj, si, sj, tt, vals);
   const int64_t n = c->n;
   const NbrList& L = c->nl[c->nl_cur];
   const int64_t nw = (n + kWave - 1) / kWave;
@@ -482,6 +484,8 @@ static int contact_state_rows(xdem_ctx* c, std::vector<uint64_t>& ii, std::vecto
       if (!keep(v)) continue;
       ii.push_back(ids[x]);
       jj.push_back(ids[word[s] & kIdxMask]);
+      si.push_back(0);
+      sj.push_back(0);
       tt.push_back(0);
       vals.insert(vals.end(), v, v + kCStateLen);
     }
@@ -494,6 +498,8 @@ static int contact_state_rows(xdem_ctx* c, std::vector<uint64_t>& ii, std::vecto
         if (!keep(v)) continue;
         ii.push_back(ids[x]);
         jj.push_back((uint64_t)w);
+        si.push_back(0);
+        sj.push_back(0);
         tt.push_back(5);
         vals.insert(vals.end(), v, v + kCStateLen);
       }
@@ -514,26 +520,30 @@ extern "C" int xdem_contact_state_count(xdem_ctx* c, int64_t* n) {
   if (!c || !n) return XDEM_ERR_ARG;
   *n = 0;
   std::vector<uint64_t> ii, jj;
+  std::vector<uint32_t> si, sj;
   std::vector<uint16_t> tt;
   std::vector<double> v;
-  const int rc = contact_state_rows(c, ii, jj, tt, v);
+  const int rc = contact_state_rows(c, ii, jj, si, sj, tt, v);
   if (rc) return rc;
   *n = (int64_t)ii.size();
   return XDEM_OK;
 }
 
-extern "C" int xdem_contact_state_dump(xdem_ctx* c, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint16_t* type,
-                                       double* values, int64_t* written) {
-  if (!c || !id_i || !id_j || !type || !values || cap < 0) return XDEM_ERR_ARG;
+extern "C" int xdem_contact_state_dump(xdem_ctx* c, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint32_t* sub_i,
+                                       uint32_t* sub_j, uint16_t* type, double* values, int64_t* written) {
+  if (!c || !id_i || !id_j || !sub_i || !sub_j || !type || !values || cap < 0) return XDEM_ERR_ARG;
   if (written) *written = 0;
   std::vector<uint64_t> ii, jj;
+  std::vector<uint32_t> si, sj;
   std::vector<uint16_t> tt;
   std::vector<double> v;
-  const int rc = contact_state_rows(c, ii, jj, tt, v);
+  const int rc = contact_state_rows(c, ii, jj, si, sj, tt, v);
   if (rc) return rc;
   if ((int64_t)ii.size() > cap) return set_err(c, XDEM_ERR_CAPACITY, "contact_state_dump: cap too small");
   std::copy(ii.begin(), ii.end(), id_i);
   std::copy(jj.begin(), jj.end(), id_j);
+  std::copy(si.begin(), si.end(), sub_i);
+  std::copy(sj.begin(), sj.end(), sub_j);
   std::copy(tt.begin(), tt.end(), type);
   std::copy(v.begin(), v.end(), values);
   if (written) *written = (int64_t)ii.size();

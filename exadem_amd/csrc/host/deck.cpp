@@ -537,13 +537,13 @@ class Runner {
       if (f->s == "fp32") form = XDEM_FORM_FP32;
       else if (f->s != "fp64") fail(XDEM_ERR_ARG, "contact: precision is fp64 or fp32");
     }
-    // analysis (contact_force_sphere.cu.in:62-66, 190-199): every analysis_interaction_dump_frequency
+    // analysis (contact_force_sphere.cu.in:62-66, 190-199, and the same slots of contact_polyhedron):
+    // every analysis_interaction_dump_frequency
     // steps the interactions of this pass go to <dir_name>/ExaDEMAnalyses/<basename><timestep>/
     const Node* fq = slot(s, "analysis_interaction_dump_frequency");
     const int64_t freq =
         fq ? (int64_t)xdem_host::to_double(*fq) : (int64_t)gdouble("analysis_interaction_dump_frequency", -1);
     const bool dump = freq > 0 && timestep_ % freq == 0;
-    if (dump && poly) fail(XDEM_ERR_UNSUPPORTED, "contact_polyhedron: interaction dumps are provided for spheres");
     if (dump) check(xdem_set_contact_record(ctx_, 1), "contact_sphere");
     if (poly) check(xdem_contact_polyhedron(ctx_, &cp, cfd ? &cpd : nullptr, dt()), "contact_polyhedron");
     else check(xdem_contact_sphere(ctx_, &cp, cfd ? &cpd : nullptr, dt(), form), "contact_sphere");
@@ -566,10 +566,13 @@ class Runner {
     int64_t n = 0;
     check(xdem_contact_state_count(ctx_, &n), "interaction dump");
     std::vector<uint64_t> ii((size_t)std::max<int64_t>(n, 1)), jj(ii.size());
+    std::vector<uint32_t> si(ii.size()), sj(ii.size());
     std::vector<uint16_t> tt(ii.size());
     std::vector<double> v(16 * ii.size());
     int64_t w = 0;
-    if (n > 0) check(xdem_contact_state_dump(ctx_, n, ii.data(), jj.data(), tt.data(), v.data(), &w), "interaction dump");
+    if (n > 0)
+      check(xdem_contact_state_dump(ctx_, n, ii.data(), jj.data(), si.data(), sj.data(), tt.data(), v.data(), &w),
+            "interaction dump");
     const std::string name = base + std::to_string(timestep_);
     const std::string sub = dir + "/ExaDEMAnalyses/" + name;
     std::error_code ec;
@@ -578,7 +581,7 @@ class Runner {
     std::ofstream out(sub + "/" + name + "_0.txt");
     if (!out) fail(XDEM_ERR_ARG, "interaction dump: cannot write in " + sub);
     for (int64_t k = 0; k < w; k++) {
-      out << ii[k] << "," << jj[k] << ",0,0," << tt[k];
+      out << ii[k] << "," << jj[k] << "," << si[k] << "," << sj[k] << "," << tt[k];
       for (int q = 0; q < 16; q++) out << "," << v[16 * k + q];
       out << "\n";
     }

@@ -645,6 +645,10 @@ struct PolyForceArgs {
   PolyList pl;
   PolyWall pw;
   const uint2* frange;
+  // when recording: contact state per feature and per (wall, vertex, particle), kCStateLen doubles
+  // each, in the evaluation orientation (null: not recorded)
+  double* cstate;
+  double* wstate;
   double* contrib;    // [nfeat * 9]
   uint8_t* contact;   // [nfeat]
   const Surface* walls;
@@ -732,6 +736,16 @@ __global__ __launch_bounds__(128) void k_poly_feat(PolyForceArgs a) {
     }
     const uint32_t nfw = act ? (fw | kFeatActive) : (fw & ~kFeatActive);
     if (nfw != fw) a.pl.feat[f] = nfw;
+    if (a.cstate) {
+      // the reference functor's analysis tuple (polyhedron/contact.hpp:160-210) with the positions:
+      // i is the vertex side, as the reference's swapped pair stores it
+      const V3 fnv = fc - ft;
+      const double rec[kCStateLen] = {d.dn,  d.cp.x, d.cp.y, d.cp.z, fnv.x, fnv.y, fnv.z, ft.x,
+                                      ft.y,  ft.z,   r_i.x,  r_i.y,  r_i.z,  r_j.x, r_j.y, r_j.z};
+      double* rc = a.cstate + (uint64_t)kCStateLen * f;
+#pragma unroll
+      for (int q = 0; q < kCStateLen; q++) rc[q] = rec[q];
+    }
     const V3 nf = -fc;
     V3 Fo, Mo, Mp;
     if (!swap) {  // owner is i
@@ -750,6 +764,11 @@ __global__ __launch_bounds__(128) void k_poly_feat(PolyForceArgs a) {
     a.contact[f] = 1;
   } else {
     a.contact[f] = 0;
+    if (a.cstate) {
+      double* rc = a.cstate + (uint64_t)kCStateLen * f;
+#pragma unroll
+      for (int q = 0; q < kCStateLen; q++) rc[q] = 0.0;
+    }
     if (fw & kFeatActive) {
       a.pl.hx[f] = 0.0; a.pl.hy[f] = 0.0; a.pl.hz[f] = 0.0;
       a.pl.mhx[f] = 0.0; a.pl.mhy[f] = 0.0; a.pl.mhz[f] = 0.0;
@@ -845,6 +864,11 @@ __global__ __launch_bounds__(256) void k_poly_gather(PolyForceArgs a) {
         if (dn > 0) {
           a.pw.hx[j] = 0.0; a.pw.hy[j] = 0.0; a.pw.hz[j] = 0.0;
           a.pw.mhx[j] = 0.0; a.pw.mhy[j] = 0.0; a.pw.mhz[j] = 0.0;
+          if (a.wstate) {
+            double* rc = a.wstate + (uint64_t)kCStateLen * j;
+#pragma unroll
+            for (int q = 0; q < kCStateLen; q++) rc[q] = 0.0;
+          }
           continue;
         }
         const V3 nrm = s2p / d;
@@ -853,6 +877,14 @@ __global__ __launch_bounds__(256) void k_poly_gather(PolyForceArgs a) {
         V3 mint{0.0, 0.0, 0.0};
         if (KR) mint = {a.pw.mhx[j], a.pw.mhy[j], a.pw.mhz[j]};
         const V3 f = hooke_force<KR>(dn, nrm, a.dt, a.cpd, mx, ft, mint, cpos, rx, vx, wx, sf.center, sf.vel, sf.vrot);
+        if (a.wstate) {
+          const V3 fnv = f - ft;
+          const double rec[kCStateLen] = {dn,   cpos.x, cpos.y, cpos.z, fnv.x, fnv.y, fnv.z, ft.x,
+                                          ft.y, ft.z,   rx.x,   rx.y,   rx.z,   0.0,   0.0,   0.0};
+          double* rc = a.wstate + (uint64_t)kCStateLen * j;
+#pragma unroll
+          for (int q = 0; q < kCStateLen; q++) rc[q] = rec[q];
+        }
         a.pw.hx[j] = ft.x; a.pw.hy[j] = ft.y; a.pw.hz[j] = ft.z;
         if (KR) {
           a.pw.mhx[j] = mint.x; a.pw.mhy[j] = mint.y; a.pw.mhz[j] = mint.z;
@@ -910,6 +942,23 @@ int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, d
   a.gravity_on = gravity_on ? 1 : 0;
   a.flags = c->d_flags;
   a.step_tag = step_tag;
+  int rc;
+  if (c->record_cs) {
+    if (a.pl.nfeat > c->cap_cstate) {
+      const int64_t cap = (int64_t)(a.pl.nfeat * 1.25) + 4096;
+      if ((rc = dev_realloc(c, &c->cstate, cap * kCStateLen))) return rc;
+      c->cap_cstate = cap;
+    }
+    a.cstate = c->cstate;
+    if (a.nw) {
+      const int64_t m = (int64_t)c->nwalls * XDEM_MAX_SHAPE_VERTICES * c->cap;
+      if (m > c->cap_wstate) {
+        if ((rc = dev_realloc(c, &c->wstate, m * kCStateLen))) return rc;
+        c->cap_wstate = m;
+      }
+      a.wstate = c->wstate;
+    }
+  }
   const dim3 gp(grid_blocks(std::max<int64_t>(a.pl.nfeat, 1), 128)), bp(128), gg(grid_blocks(a.n, 256)), bg(256);
   const bool kr = c->kr_used;
 #define PAIRS(KRV, G) hipLaunchKernelGGL((k_poly_feat<KRV, G>), gp, bp, 0, c->stream, a)
@@ -929,7 +978,8 @@ int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, d
 #undef PAIRS
 #undef GATHER
   XDEM_HIP(c, hipGetLastError());
-  c->cs_valid = false;  // polyhedron passes do not record
+  c->cs_valid = c->record_cs;
+  c->cs_epoch = c->nl_epoch;
   return XDEM_OK;
 }
 
@@ -1114,3 +1164,75 @@ extern "C" int xdem_poly_interactions_count(xdem_ctx* c, int64_t* n) {
   *n = c->pl[c->pl_cur].nfeat + walls;
   return XDEM_OK;
 }
+
+namespace xdem {
+// Rows of the recorded polyhedron contact state, as the reference's interaction dump writes them
+// (itools/writer.hpp:35-75): features in contact (non-zero force or dn < 0) in feature order, then
+// vertex-wall contacts. A feature's first particle is the vertex side (the partner for a swapped
+// VE/VF feature, as the reference's pair stores pi), with its sub-feature first.
+int poly_contact_state_rows(xdem_ctx* c, std::vector<uint64_t>& ii, std::vector<uint64_t>& jj,
+                            std::vector<uint32_t>& si, std::vector<uint32_t>& sj, std::vector<uint16_t>& tt,
+                            std::vector<double>& vals) {
+  const int64_t n = c->n;
+  const PolyList& P = c->pl[c->pl_cur];
+  const NbrList& L = c->nl[c->nl_cur];
+  const int64_t nf = P.nfeat;
+  std::vector<uint32_t> feat(nf), fslot(nf), fown(nf), word(L.slots);
+  std::vector<uint64_t> ids(n);
+  std::vector<double> cs((size_t)nf * kCStateLen);
+  XDEM_HIP(c, hipMemcpyAsync(ids.data(), live(c).id, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, c->stream));
+  if (nf > 0) {
+    XDEM_HIP(c, hipMemcpyAsync(feat.data(), P.feat, sizeof(uint32_t) * nf, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(fslot.data(), P.fslot, sizeof(uint32_t) * nf, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(fown.data(), P.fown, sizeof(uint32_t) * nf, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(cs.data(), c->cstate, sizeof(double) * kCStateLen * nf, hipMemcpyDeviceToHost,
+                               c->stream));
+  }
+  if (L.slots > 0)
+    XDEM_HIP(c, hipMemcpyAsync(word.data(), L.word, sizeof(uint32_t) * L.slots, hipMemcpyDeviceToHost, c->stream));
+  const bool walls = c->nwalls > 0 && c->pw_valid && c->wstate;
+  std::vector<uint64_t> mask;
+  std::vector<double> ws;
+  if (walls) {
+    mask.resize(n);
+    ws.resize((size_t)c->nwalls * XDEM_MAX_SHAPE_VERTICES * c->cap * kCStateLen);
+    XDEM_HIP(c, hipMemcpyAsync(mask.data(), c->pw[c->pw_cur].mask, sizeof(uint64_t) * n, hipMemcpyDeviceToHost,
+                               c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(ws.data(), c->wstate, sizeof(double) * ws.size(), hipMemcpyDeviceToHost, c->stream));
+  }
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  auto keep = [](const double* v) {
+    const double f =
+        std::sqrt(v[4] * v[4] + v[5] * v[5] + v[6] * v[6]) + std::sqrt(v[7] * v[7] + v[8] * v[8] + v[9] * v[9]);
+    return f != 0 || v[0] < 0;
+  };
+  for (int64_t f = 0; f < nf; f++) {
+    const double* v = cs.data() + (size_t)kCStateLen * f;
+    if (!keep(v)) continue;
+    const uint32_t fw = feat[f];
+    const bool swap = (fw & kFeatSwap) != 0;
+    const uint64_t own = ids[fown[f]], par = ids[word[fslot[f]] & kIdxMask];
+    const uint32_t osub = (fw >> 3) & 15u, psub = (fw >> 7) & 15u;
+    ii.push_back(swap ? par : own);
+    jj.push_back(swap ? own : par);
+    si.push_back(swap ? psub : osub);
+    sj.push_back(swap ? osub : psub);
+    tt.push_back((uint16_t)(fw & kFeatTypeMask));
+    vals.insert(vals.end(), v, v + kCStateLen);
+  }
+  if (walls)
+    for (int64_t x = 0; x < n; x++)
+      for (int bit = 0; bit < c->nwalls * XDEM_MAX_SHAPE_VERTICES; bit++) {
+        if (!((mask[x] >> bit) & 1ull)) continue;
+        const double* v = ws.data() + (size_t)kCStateLen * ((int64_t)bit * c->cap + x);
+        if (!keep(v)) continue;
+        ii.push_back(ids[x]);
+        jj.push_back((uint64_t)(bit / XDEM_MAX_SHAPE_VERTICES));
+        si.push_back((uint32_t)(bit % XDEM_MAX_SHAPE_VERTICES));
+        sj.push_back(0xffffffffu);  // the driver side's sub-feature is -1 (nbh_polyhedron_driver.hpp:220)
+        tt.push_back(5);
+        vals.insert(vals.end(), v, v + kCStateLen);
+      }
+  return XDEM_OK;
+}
+}  // namespace xdem

@@ -381,4 +381,7 @@ int poly_compute_vertices(xdem_ctx* c, bool displ, double thr2, bool guarded);
 int poly_nbh(xdem_ctx* c, double rcut_inc);
 int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool fused, bool gravity_on,
                V3 g, bool guarded, int32_t step_tag);
+int poly_contact_state_rows(xdem_ctx* c, std::vector<uint64_t>& ii, std::vector<uint64_t>& jj,
+                            std::vector<uint32_t>& si, std::vector<uint32_t>& sj, std::vector<uint16_t>& tt,
+                            std::vector<double>& vals);
 }  // namespace xdem

@@ -37,11 +37,11 @@ def dump_path(directory, basename, timestep, rank=0):
 
 
 def write_dump(ctx, directory, basename, timestep, rank=0):
-    """Writes the contact state recorded by the context's last sphere force pass (the context must
-    record: ctx.set_contact_record(True) before the pass). Returns the file path and the row count."""
-    ii, jj, tt, vals = ctx.contact_state()
-    zero = np.zeros(len(ii), np.int64)
-    lines = format_rows(ii, jj, zero, zero, tt, vals)
+    """Writes the contact state recorded by the context's last force pass (spheres or polyhedra; the
+    context must record: ctx.set_contact_record(True) before the pass). Returns the file path and the
+    row count."""
+    ii, jj, si, sj, tt, vals = ctx.contact_state(subs=True)
+    lines = format_rows(ii, jj, si, sj, tt, vals)
     path = dump_path(directory, basename, timestep, rank)
     os.makedirs(os.path.dirname(path), exist_ok=True)
     with open(path, "w") as f:

@@ -177,8 +177,8 @@ def load():
                                              C.POINTER(C.c_int64)]),
         "xdem_set_contact_record": (C.c_int, [vp, C.c_int32]),
         "xdem_contact_state_count": (C.c_int, [vp, C.POINTER(C.c_int64)]),
-        "xdem_contact_state_dump": (C.c_int, [vp, C.c_int64, _u64p, _u64p, C.POINTER(C.c_uint16), _dp,
-                                              C.POINTER(C.c_int64)]),
+        "xdem_contact_state_dump": (C.c_int, [vp, C.c_int64, _u64p, _u64p, _u32p, _u32p, C.POINTER(C.c_uint16),
+                                              _dp, C.POINTER(C.c_int64)]),
         "xdem_streams_dump": (C.c_int, [vp, C.c_int64, _u64p, _u64p, C.POINTER(C.c_int32),
                                         C.POINTER(C.c_uint8), C.POINTER(C.c_int64)]),
         "xdem_nbh_stats": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
@@ -385,21 +385,26 @@ class Context:
         """Record the contact state of FP64 sphere force passes (outside the engine's integrating step)."""
         self.call("xdem_set_contact_record", 1 if on else 0)
 
-    def contact_state(self):
+    def contact_state(self, subs=False):
         """The recorded rows of the last force pass, as the reference's interaction dump selects them:
-        owner id, partner id (wall index), type (0 pair, 5 wall) and per row 16 values: dn, contact
-        point, f - ft, ft, owner position, partner position."""
+        first id, second id (wall index), type (0 VV / sphere pair, 1 VE, 2 VF, 3 EE, 5 wall) and per
+        row 16 values: dn, contact point, f - ft, ft, first position, second position. subs=True
+        also returns the two sub-feature columns (after the ids)."""
         cnt = C.c_int64()
         self.call("xdem_contact_state_count", C.byref(cnt))
         cap = max(1, cnt.value)
         ii = np.zeros(cap, np.uint64)
         jj = np.zeros(cap, np.uint64)
+        si = np.zeros(cap, np.uint32)
+        sj = np.zeros(cap, np.uint32)
         tt = np.zeros(cap, np.uint16)
         vals = np.zeros(16 * cap, np.float64)
         wr = C.c_int64()
-        self.call("xdem_contact_state_dump", cap, _ptr(ii, C.c_uint64), _ptr(jj, C.c_uint64), _ptr(tt, C.c_uint16),
-                  _ptr(vals, C.c_double), C.byref(wr))
+        self.call("xdem_contact_state_dump", cap, _ptr(ii, C.c_uint64), _ptr(jj, C.c_uint64), _ptr(si, C.c_uint32),
+                  _ptr(sj, C.c_uint32), _ptr(tt, C.c_uint16), _ptr(vals, C.c_double), C.byref(wr))
         k = wr.value
+        if subs:
+            return ii[:k], jj[:k], si[:k], sj[:k], tt[:k], vals[:16 * k].reshape(-1, 16)
         return ii[:k], jj[:k], tt[:k], vals[:16 * k].reshape(-1, 16)
 
     def streams(self):

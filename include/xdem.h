@@ -244,21 +244,24 @@ int xdem_interactions_dump(xdem_ctx* ctx, int64_t cap, uint64_t* id_i, uint64_t*
 int xdem_interactions_load(xdem_ctx* ctx, int64_t m, const uint64_t* id_i, const uint64_t* id_j,
                            const uint16_t* type, const double* friction, const double* moment, int64_t* matched);
 /* Contact-state recording for analyses (replaces the ContactState the reference's contact operators
- * fill for every interaction, contact_force_sphere.cu.in:190-199 with sphere/contact_sphere.hpp:
- * 175-202 and 282-296). With on != 0, every FP64 sphere force pass outside the engine's integrating
- * step (the operator path, and the last step of each xdem_run_steps call) records per interaction
- * dn, the contact point, the normal force f - ft, the tangential force ft and the two positions. */
+ * fill for every interaction: contact_force_sphere.cu.in:190-199 with sphere/contact_sphere.hpp:
+ * 175-202 and 282-296; contact_force_polyhedron.cu.in with polyhedron/contact.hpp:160-210). With
+ * on != 0, every FP64 force pass outside the engine's integrating sphere step (the operator path,
+ * the last step of each xdem_run_steps call, every polyhedron pass) records per interaction dn, the
+ * contact point, the normal force f - ft, the tangential force ft and the two positions. */
 int xdem_set_contact_record(xdem_ctx* ctx, int32_t on);
 /* The recorded rows the reference's interaction dump writes (itools/writer.hpp:35-75 with the
- * dump of contact_force_sphere.cu.in:190-199): owner-side interactions with a non-zero force or
- * dn < 0, particle pairs then walls. Per row: owner id, partner id (wall index for walls), type
- * (0 sphere pair, 5 wall) and 16 doubles: dn, contact point (3), f - ft (3), ft (3), owner position
- * (3), partner position (3; the periodic image across a boundary, zeros for walls). Errors with
- * XDEM_ERR_STATE when recording is off or the last sphere force pass did not record (FP32 form,
+ * dump of contact_force_sphere.cu.in:190-199): interactions with a non-zero force or dn < 0,
+ * particle pairs (sphere pairs owner side; polyhedron features in feature order) then walls. Per
+ * row: first id, second id (wall index for walls), their sub-features, type (0 VV / sphere pair,
+ * 1 VE, 2 VF, 3 EE, 5 wall) and 16 doubles: dn, contact point (3), f - ft (3), ft (3), first
+ * position (3), second position (3; the periodic image across a boundary, zeros for walls). As in
+ * the reference's pairs, a swapped polyhedron feature lists the vertex side first. Errors with
+ * XDEM_ERR_STATE when recording is off or the last force pass did not record (FP32 form,
  * integrating engine step, a rebuild since). */
 int xdem_contact_state_count(xdem_ctx* ctx, int64_t* n);
-int xdem_contact_state_dump(xdem_ctx* ctx, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint16_t* type,
-                            double* values, int64_t* written);
+int xdem_contact_state_dump(xdem_ctx* ctx, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint32_t* sub_i,
+                            uint32_t* sub_j, uint16_t* type, double* values, int64_t* written);
 /* Neighbour-list statistics of the last build: half-list pairs P, list slots
  * allocated (ELL with per-wave padding) and the grid dimensions. */
 int xdem_nbh_stats(xdem_ctx* ctx, int64_t* pairs, int64_t* slots, int32_t dims[3]);

@@ -155,3 +155,45 @@ def test_vertex_displ_over(gpu_ok):
         s.ctx.combined_compute_prolog(0.02)
         o.prolog(0.02)
         assert s.ctx.vertex_displ_over(0.05) == o.vertex_displ_over(0.05), k
+
+
+def test_contact_state_dump(gpu_ok, tmp_path):
+    """Recorded contact state of a polyhedron force pass (the reference functor's analysis tuple with
+    the positions): one row per feature in contact and per vertex-wall contact; the vertex side comes
+    first, as the reference's swapped pairs store it; ft is the feature's friction history; the rows'
+    forces add up to each particle's force; the dump file reads back."""
+    from exadem_amd import itools
+    w = poly_case("walls")
+    s = PolySim(w)
+    s.ctx.set_contact_record(True)
+    s.first_iteration()
+    ii, jj, si, sj, tt, v = s.ctx.contact_state(subs=True)
+    feat, wall = tt < 4, tt == 5
+    assert np.count_nonzero(feat) > 0 and np.count_nonzero(wall) > 0
+    assert np.all(v[feat, 0] < 0)
+    assert np.all(sj[wall] == 0xffffffff)
+    fi, fj, ftp, fsw, fsi, fsj, ffr = s.ctx.poly_interactions()
+    hist = {(int(a), int(b), int(t), int(x), int(y), int(z)): f
+            for a, b, t, x, y, z, f in zip(fi, fj, ftp, fsw, fsi, fsj, ffr)}
+    p = by_id(s.ctx.download())
+    F = p["mass"][:, None] * np.asarray(w.gravity)[None, :]
+    for k in range(len(ii)):
+        a, b, t = int(ii[k]), int(jj[k]), int(tt[k])
+        if t == 5:
+            key = (a, b, 5, 0, int(si[k]), 0xffffffff)
+        else:
+            key = (a, b, t, 0, int(si[k]), int(sj[k]))
+            if key not in hist:  # a swapped feature: the owner is the second particle
+                key = (b, a, t, 1, int(sj[k]), int(si[k]))
+        assert np.array_equal(v[k, 7:10], hist[key]), key
+        assert np.array_equal(v[k, 10:13], pos(p)[a])
+        f = v[k, 4:7] + v[k, 7:10]
+        F[a] += f
+        if t != 5:
+            F[b] -= f
+    assert rel_err(F, force(p), p["mass"] * 9.81) <= 1e-12
+    path, rows = itools.write_dump(s.ctx, str(tmp_path), "InteractionOutputDir-", 0)
+    keys, vals = itools.read_dump(path)
+    assert rows == len(ii)
+    assert np.array_equal(keys[:, 2], si.astype(np.int64)) and np.array_equal(keys[:, 3], sj.astype(np.int64))
+    assert np.allclose(vals, v, rtol=1e-5, atol=1e-300)
