@@ -17,6 +17,7 @@
 // the owner slot's for a partner entry), forms each pair's subtotal over the features in list order,
 // adds it, then the vertex-wall contacts, then finishes force_to_accel and the angular acceleration.
 #include <algorithm>
+#include <unordered_map>
 
 #include "contact_law.h"
 
@@ -1236,3 +1237,110 @@ int poly_contact_state_rows(xdem_ctx* c, std::vector<uint64_t>& ii, std::vector<
   return XDEM_OK;
 }
 }  // namespace xdem
+
+// Restart of polyhedra with interaction history: saved rows (the layout of xdem_poly_interactions_dump)
+// set the history of the current feature and vertex-wall interactions with the same key.
+namespace {
+struct PKey {
+  uint64_t i, j;
+  uint32_t si, sj;
+  uint16_t t;
+  uint8_t sw;
+  bool operator==(const PKey& o) const {
+    return i == o.i && j == o.j && si == o.si && sj == o.sj && t == o.t && sw == o.sw;
+  }
+};
+struct PKeyHash {
+  size_t operator()(const PKey& k) const {
+    uint64_t h = k.i * 0x9e3779b97f4a7c15ull;
+    h ^= k.j + 0x7f4a7c159e3779b9ull + (h << 6) + (h >> 2);
+    h ^= ((uint64_t)k.si << 32 | k.sj) + (h << 6) + (h >> 2);
+    h ^= ((uint64_t)k.t << 8 | k.sw) + (h << 6) + (h >> 2);
+    return (size_t)h;
+  }
+};
+}  // namespace
+
+extern "C" int xdem_poly_interactions_load(xdem_ctx* c, int64_t m, const uint64_t* id_i, const uint64_t* id_j,
+                                           const uint16_t* type, const uint8_t* swap, const uint32_t* sub_i,
+                                           const uint32_t* sub_j, const double* friction, int64_t* matched) {
+  if (!c || m < 0 || (m > 0 && (!id_i || !id_j || !type || !swap || !sub_i || !sub_j || !friction)))
+    return XDEM_ERR_ARG;
+  if (matched) *matched = 0;
+  if (!c->pl_valid || c->pl_epoch != c->nl_epoch)
+    return set_err(c, XDEM_ERR_STATE, "poly_interactions_load: run nbh_polyhedron first");
+  const int64_t n = c->n;
+  if (n == 0) return XDEM_OK;
+  std::unordered_map<PKey, int64_t, PKeyHash> saved;
+  saved.reserve((size_t)m * 2);
+  for (int64_t k = 0; k < m; k++)
+    if (!saved.emplace(PKey{id_i[k], id_j[k], sub_i[k], sub_j[k], type[k], swap[k]}, k).second)
+      return set_err(c, XDEM_ERR_ARG, "poly_interactions_load: duplicate interaction key");
+  PolyList& P = c->pl[c->pl_cur];
+  const NbrList& L = c->nl[c->nl_cur];
+  const int64_t nf = P.nfeat;
+  std::vector<uint32_t> feat(nf), fslot(nf), fown(nf), word(L.slots);
+  std::vector<uint64_t> ids(n);
+  XDEM_HIP(c, hipMemcpyAsync(ids.data(), live(c).id, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, c->stream));
+  if (nf > 0) {
+    XDEM_HIP(c, hipMemcpyAsync(feat.data(), P.feat, sizeof(uint32_t) * nf, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(fslot.data(), P.fslot, sizeof(uint32_t) * nf, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(fown.data(), P.fown, sizeof(uint32_t) * nf, hipMemcpyDeviceToHost, c->stream));
+  }
+  if (L.slots > 0)
+    XDEM_HIP(c, hipMemcpyAsync(word.data(), L.word, sizeof(uint32_t) * L.slots, hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  std::vector<double> h[3];
+  for (auto& a : h) a.assign((size_t)nf, 0.0);
+  int64_t hit = 0;
+  for (int64_t f = 0; f < nf; f++) {
+    const uint32_t fw = feat[f];
+    const PKey key{ids[fown[f]], ids[word[fslot[f]] & kIdxMask], (fw >> 3) & 15u, (fw >> 7) & 15u,
+                   (uint16_t)(fw & kFeatTypeMask), (uint8_t)((fw & kFeatSwap) ? 1 : 0)};
+    const auto it = saved.find(key);
+    bool act = false;
+    if (it != saved.end()) {
+      for (int q = 0; q < 3; q++) {
+        h[q][f] = friction[3 * it->second + q];
+        act = act || h[q][f] != 0.0;
+      }
+      hit++;
+    }
+    feat[f] = act ? (fw | kFeatActive) : (fw & ~kFeatActive);
+  }
+  if (nf > 0) {
+    double* dst[3] = {P.hx, P.hy, P.hz};
+    for (int q = 0; q < 3; q++)
+      XDEM_HIP(c, hipMemcpyAsync(dst[q], h[q].data(), sizeof(double) * nf, hipMemcpyHostToDevice, c->stream));
+    double* mdst[3] = {P.mhx, P.mhy, P.mhz};
+    for (int q = 0; q < 3; q++) XDEM_HIP(c, hipMemsetAsync(mdst[q], 0, sizeof(double) * nf, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(P.feat, feat.data(), sizeof(uint32_t) * nf, hipMemcpyHostToDevice, c->stream));
+  }
+  if (c->nwalls > 0 && c->pw_valid) {
+    PolyWall& W = c->pw[c->pw_cur];
+    const int64_t mw = (int64_t)c->nwalls * XDEM_MAX_SHAPE_VERTICES * c->cap;
+    std::vector<uint64_t> mask(n);
+    XDEM_HIP(c, hipMemcpyAsync(mask.data(), W.mask, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<double> wh[3];
+    for (auto& a : wh) a.assign((size_t)mw, 0.0);
+    for (int64_t x = 0; x < n; x++)
+      for (int bit = 0; bit < c->nwalls * XDEM_MAX_SHAPE_VERTICES; bit++) {
+        if (!((mask[x] >> bit) & 1ull)) continue;
+        const auto it = saved.find(PKey{ids[x], (uint64_t)(bit / XDEM_MAX_SHAPE_VERTICES),
+                                        (uint32_t)(bit % XDEM_MAX_SHAPE_VERTICES), 0xffffffffu, 5, 0});
+        if (it == saved.end()) continue;
+        const int64_t j = (int64_t)bit * c->cap + x;
+        for (int q = 0; q < 3; q++) wh[q][j] = friction[3 * it->second + q];
+        hit++;
+      }
+    double* dst[3] = {W.hx, W.hy, W.hz};
+    for (int q = 0; q < 3; q++)
+      XDEM_HIP(c, hipMemcpyAsync(dst[q], wh[q].data(), sizeof(double) * mw, hipMemcpyHostToDevice, c->stream));
+    double* mdst[3] = {W.mhx, W.mhy, W.mhz};
+    for (int q = 0; q < 3; q++) XDEM_HIP(c, hipMemsetAsync(mdst[q], 0, sizeof(double) * mw, c->stream));
+  }
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  if (matched) *matched = hit;
+  return XDEM_OK;
+}

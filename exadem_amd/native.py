@@ -89,7 +89,7 @@ HIP_SYMBOLS = [
     "xdem_combined_compute_prolog", "xdem_combined_compute_epilog", "xdem_reset_force_moment",
     "xdem_gravity_force", "xdem_force_to_accel", "xdem_particle_displ_over", "xdem_move_particles",
     "xdem_chunk_neighbors_contact", "xdem_nbh_sphere", "xdem_classify_interactions", "xdem_contact_sphere",
-    "xdem_run_steps", "xdem_interactions_count", "xdem_interactions_dump", "xdem_interactions_load",
+    "xdem_run_steps", "xdem_interactions_count", "xdem_interactions_dump", "xdem_interactions_load", "xdem_poly_interactions_load",
     "xdem_set_contact_record", "xdem_contact_state_count", "xdem_contact_state_dump", "xdem_nbh_stats",
     "xdem_kernel_timing", "xdem_set_timing", "xdem_streams_dump",
     "xdem_set_shapes", "xdem_compute_vertices", "xdem_vertex_displ_over", "xdem_nbh_polyhedron",
@@ -175,6 +175,8 @@ def load():
                                              C.POINTER(C.c_int64)]),
         "xdem_interactions_load": (C.c_int, [vp, C.c_int64, _u64p, _u64p, C.POINTER(C.c_uint16), _dp, _dp,
                                              C.POINTER(C.c_int64)]),
+        "xdem_poly_interactions_load": (C.c_int, [vp, C.c_int64, _u64p, _u64p, C.POINTER(C.c_uint16),
+                                                  C.POINTER(C.c_uint8), _u32p, _u32p, _dp, C.POINTER(C.c_int64)]),
         "xdem_set_contact_record": (C.c_int, [vp, C.c_int32]),
         "xdem_contact_state_count": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "xdem_contact_state_dump": (C.c_int, [vp, C.c_int64, _u64p, _u64p, _u32p, _u32p, C.POINTER(C.c_uint16),
@@ -379,6 +381,23 @@ class Context:
         got = C.c_int64()
         self.call("xdem_interactions_load", ii.size, _ptr(ii, C.c_uint64), _ptr(jj, C.c_uint64), _ptr(tt, C.c_uint16),
                   _ptr(fr, C.c_double), None if mo is None else _ptr(mo, C.c_double), C.byref(got))
+        return got.value
+
+    def poly_interactions_load(self, ii, jj, tt, sw, si, sj, friction):
+        """Polyhedra: sets the history of the current feature and vertex-wall interactions from saved
+        rows (restart); returns the number of rows applied."""
+        ii = np.ascontiguousarray(ii, np.uint64)
+        jj = np.ascontiguousarray(jj, np.uint64)
+        tt = np.ascontiguousarray(tt, np.uint16)
+        sw = np.ascontiguousarray(sw, np.uint8)
+        si = np.ascontiguousarray(si, np.uint32)
+        sj = np.ascontiguousarray(sj, np.uint32)
+        fr = np.ascontiguousarray(friction, np.float64).reshape(-1)
+        assert fr.size == 3 * ii.size
+        got = C.c_int64()
+        self.call("xdem_poly_interactions_load", ii.size, _ptr(ii, C.c_uint64), _ptr(jj, C.c_uint64),
+                  _ptr(tt, C.c_uint16), _ptr(sw, C.c_uint8), _ptr(si, C.c_uint32), _ptr(sj, C.c_uint32),
+                  _ptr(fr, C.c_double), C.byref(got))
         return got.value
 
     def set_contact_record(self, on=True):

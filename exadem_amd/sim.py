@@ -106,6 +106,25 @@ class PolySim(SphereSim):
         super().__init__(workload, device, native.XDEM_FORM_FP64, ctx)
         self.ctx.set_shapes(workload.shapes)
 
+    def checkpoint(self, path):
+        """Restart file: every particle field and the feature and vertex-wall interactions with their
+        friction history (the layout of xdem_poly_interactions_dump)."""
+        p = self.ctx.download()
+        ii, jj, tt, sw, si, sj, fr = self.ctx.poly_interactions()
+        np.savez(path, ii=ii, jj=jj, tt=tt, sw=sw, si=si, sj=sj, fr=fr, **{"p_" + k: v for k, v in p.items()})
+
+    @classmethod
+    def restart(cls, workload, path, device=0, history=True):
+        """A polyhedra simulation continued from a checkpoint (see SphereSim.restart)."""
+        d = np.load(path)
+        w = copy.copy(workload)
+        w.particles = {k[2:]: d[k] for k in d.files if k.startswith("p_")}
+        sim = cls(w, device=device)
+        sim.rebuild()
+        if history:
+            sim.ctx.poly_interactions_load(d["ii"], d["jj"], d["tt"], d["sw"], d["si"], d["sj"], d["fr"])
+        return sim
+
     def rebuild(self):
         c = self.ctx
         c.move_particles()

@@ -243,6 +243,14 @@ int xdem_interactions_dump(xdem_ctx* ctx, int64_t cap, uint64_t* id_i, uint64_t*
  * without a current interaction are dropped. *matched = rows applied. */
 int xdem_interactions_load(xdem_ctx* ctx, int64_t m, const uint64_t* id_i, const uint64_t* id_j,
                            const uint16_t* type, const double* friction, const double* moment, int64_t* matched);
+/* The same for polyhedra: after nbh_polyhedron on the restored particles, saved rows in the layout of
+ * xdem_poly_interactions_dump (owner id, partner id or wall index, type, swap, owner and partner
+ * sub-features, friction) set the friction history of the current feature and vertex-wall
+ * interactions with the same key; the rolling-moment history starts from zero. *matched = rows
+ * applied. */
+int xdem_poly_interactions_load(xdem_ctx* ctx, int64_t m, const uint64_t* id_i, const uint64_t* id_j,
+                                const uint16_t* type, const uint8_t* swap, const uint32_t* sub_i,
+                                const uint32_t* sub_j, const double* friction, int64_t* matched);
 /* Contact-state recording for analyses (replaces the ContactState the reference's contact operators
  * fill for every interaction: contact_force_sphere.cu.in:190-199 with sphere/contact_sphere.hpp:
  * 175-202 and 282-296; contact_force_polyhedron.cu.in with polyhedron/contact.hpp:160-210). With

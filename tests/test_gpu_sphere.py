@@ -526,10 +526,12 @@ def test_packing_statistics_match_oracle(gpu_ok):
 @pytest.mark.parametrize("case", ["walls", "periodic"])
 def test_checkpoint_restart_with_history(gpu_ok, case, tmp_path):
     """Restart with interaction history: a run checkpointed after 60 steps (particles and interaction
-    history) and restarted (particles uploaded, lists rebuilt, history handed over by key) continues as
-    the uninterrupted run does: after 60 more steps the same contacts (interactions with history),
-    friction history within 1e-12 and trajectories within 1e-10 (the lists are rebuilt at other positions, so the summation
-    order and the rebuild steps may differ). Restarting without the history gives another state."""
+    history) and restarted (particles uploaded, lists rebuilt, history handed over by key) is, bit for
+    bit, the uninterrupted run with a neighbour rebuild at the checkpoint step; against the run without
+    that rebuild, after 60 more steps: the same contacts (interactions with history), friction history
+    within 1e-12 and trajectories within 1e-10 (the lists are built at other positions, so the
+    summation order and the rebuild steps may differ). Restarting without the history gives another
+    state."""
     w = {"walls": wall_deposition, "periodic": periodic_pack}[case]()
     a = gpu_for(w)
     a.first_iteration()
@@ -538,9 +540,18 @@ def test_checkpoint_restart_with_history(gpu_ok, case, tmp_path):
     a.checkpoint(ck)
     nhist = int(np.count_nonzero(np.any(np.load(ck)["fr"] != 0, axis=1)))
     assert nhist > 0
+    # what a restart is: the uninterrupted run with a neighbour rebuild at the checkpoint step
+    r = gpu_for(w)
+    r.first_iteration()
+    r.run(60)
+    r.rebuild()
+    r.run(60)
     a.run(60)
     b = SphereSim.restart(w, ck)
     b.run(60)
+    pr, pb = r.ctx.download(), b.ctx.download()
+    for k in ("rx", "ry", "rz", "vx", "vy", "vz", "vrotx", "fx", "momx"):
+        assert np.array_equal(by_id(pr)[k], by_id(pb)[k]), k
     c = SphereSim.restart(w, ck, history=False)
     c.run(60)
     pa, pb, pc = a.ctx.download(), b.ctx.download(), c.ctx.download()
