@@ -20,6 +20,7 @@ HIP_SOURCES = [
     "contact_sphere_fp32.hip",
     "poly.hip",
     "gnn.hip",
+    "analysis.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

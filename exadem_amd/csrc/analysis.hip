@@ -1,0 +1,130 @@
+// analysis.hip — the thermodynamic simulation state of spheres, reduced on the device
+// (simulation_state, src/io/simulation_state.cu:116-200, with ReduceSimulationStateFunctor of
+// src/io/include/exaDEM/simulation_state.hpp:48-57 and the interaction counter of
+// src/interaction/include/exaDEM/itools/counter.hpp:140-153).
+//
+// Per particle: kinetic energy 1/2 m v*v and rotation energy 1/2 m w*w per component (the reference
+// weights the spin by the mass), mass, count. Per interaction (owner entries of the neighbour list,
+// counted twice as the reference counts particle pairs under symetric: true; walls once): the total,
+// the active ones (overlap dn < 0 or non-zero friction history) and the smallest dn over the active
+// ones (0 when none overlaps). Blocks reduce in a fixed tree order and the host adds the block
+// partials in block order, so the result does not depend on scheduling.
+#include <algorithm>
+#include <cmath>
+
+#include "xdem_internal.h"
+
+namespace xdem {
+
+constexpr int kSimVals = 10;  // ke (3), re (3), mass, interactions, active interactions, min dn
+
+__global__ __launch_bounds__(256) void k_sim_state(int64_t n, PState s, NbrList nl, Grid g, WallList wl,
+                                                   const Surface* __restrict__ walls, int nw, int64_t wstride,
+                                                   double* __restrict__ part) {
+  __shared__ double red[kSimVals][256];
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[kSimVals];
+#pragma unroll
+  for (int q = 0; q < kSimVals; q++) v[q] = 0.0;
+  if (x < n) {
+    const Core me = s.core[x];
+    const double m = me.vm.w;
+    v[0] = 0.5 * me.vm.x * me.vm.x * m;
+    v[1] = 0.5 * me.vm.y * me.vm.y * m;
+    v[2] = 0.5 * me.vm.z * me.vm.z * m;
+    v[3] = 0.5 * me.w.x * me.w.x * m;
+    v[4] = 0.5 * me.w.y * me.w.y * m;
+    v[5] = 0.5 * me.w.z * me.w.z * m;
+    v[6] = m;
+    const V3 rx{me.xr.x, me.xr.y, me.xr.z};
+    const double Rx = me.xr.w;
+    const uint32_t cnt = nl.cnt ? nl.cnt[x] : 0u;
+    const uint64_t b = cnt ? nl.base[x >> 6] + (uint64_t)(x & 63) : 0;
+    for (uint32_t k = 0; k < cnt; k++) {
+      const uint32_t wd = nl.word[b + (uint64_t)k * kWave];
+      if (!(wd & kOwnerBit)) continue;
+      const Core py = s.core[wd & kIdxMask];
+      const V3 sh = shift_vec(g, (int)((wd >> kShiftShift) & 31u));
+      const V3 d{rx.x - (py.xr.x + sh.x), rx.y - (py.xr.y + sh.y), rx.z - (py.xr.z + sh.z)};
+      const double dn = sqrt(dot(d, d)) - (Rx + py.xr.w);
+      v[7] += 2.0;
+      if (dn < 0.0 || (wd & kActiveBit)) {
+        v[8] += 2.0;
+        v[9] = fmin(v[9], dn < 0.0 ? dn : 0.0);
+      }
+    }
+    if (nw > 0) {
+      const uint32_t mask = wl.mask[x];
+      for (int w = 0; w < nw; w++) {
+        if (!((mask >> w) & 1u)) continue;
+        const Surface& sf = walls[w];
+        const V3 proj = dot(rx, sf.normal) * sf.normal;
+        const V3 s2p = -(sf.center_proj - proj);
+        const double dn = norm(s2p) - Rx;
+        const int64_t j = (int64_t)w * wstride + x;
+        v[7] += 1.0;
+        if (dn < 0.0 || wl.hx[j] != 0.0 || wl.hy[j] != 0.0 || wl.hz[j] != 0.0) {
+          v[8] += 1.0;
+          v[9] = fmin(v[9], dn < 0.0 ? dn : 0.0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kSimVals; q++) red[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+#pragma unroll
+      for (int q = 0; q < kSimVals - 1; q++) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+      red[kSimVals - 1][threadIdx.x] = fmin(red[kSimVals - 1][threadIdx.x], red[kSimVals - 1][threadIdx.x + st]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < kSimVals) part[(int64_t)blockIdx.x * kSimVals + threadIdx.x] = red[threadIdx.x][0];
+}
+
+}  // namespace xdem
+
+using namespace xdem;
+
+extern "C" int xdem_simulation_state(xdem_ctx* c, xdem_sim_state* out) {
+  if (!c || !out) return XDEM_ERR_ARG;
+  *out = {};
+  if (c->nshapes > 0) return set_err(c, XDEM_ERR_UNSUPPORTED, "simulation_state: spheres only");
+  const int64_t n = c->n;
+  out->n_particles = n;
+  out->volume = c->grid.L[0] * c->grid.L[1] * c->grid.L[2];
+  if (n == 0) return XDEM_OK;
+  const int blocks = grid_blocks(n, 256);
+  if ((int64_t)blocks * kSimVals > c->cap_simpart) {
+    int rc = dev_realloc(c, &c->simpart, (int64_t)blocks * kSimVals);
+    if (rc) return rc;
+    c->cap_simpart = (int64_t)blocks * kSimVals;
+  }
+  NbrList nl{};
+  if (c->nl_valid && c->hist_ready) nl = c->nl[c->nl_cur];
+  const bool walls = c->nwalls > 0 && c->wl_valid && nl.cnt;
+  WallList wl{};
+  if (walls) wl = c->wl[c->wl_cur];
+  hipLaunchKernelGGL(k_sim_state, dim3(blocks), dim3(256), 0, c->stream, n, live(c), nl, c->grid, wl, c->d_walls,
+                     walls ? c->nwalls : 0, c->cap, c->simpart);
+  XDEM_HIP(c, hipGetLastError());
+  std::vector<double> h((size_t)blocks * kSimVals);
+  XDEM_HIP(c, hipMemcpyAsync(h.data(), c->simpart, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  double acc[kSimVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < blocks; k++) {
+    for (int q = 0; q < kSimVals - 1; q++) acc[q] += h[(size_t)k * kSimVals + q];
+    acc[kSimVals - 1] = std::min(acc[kSimVals - 1], h[(size_t)k * kSimVals + kSimVals - 1]);
+  }
+  for (int a = 0; a < 3; a++) {
+    out->kinetic_energy[a] = acc[a];
+    out->rotation_energy[a] = acc[3 + a];
+  }
+  out->mass = acc[6];
+  out->n_interactions = (int64_t)acc[7];
+  out->n_active = (int64_t)acc[8];
+  out->min_dn = acc[9];
+  return XDEM_OK;
+}

@@ -286,6 +286,8 @@ struct xdem_ctx {
   int64_t cap_cstate = 0;       // slots
   double* wstate = nullptr;
   int64_t cap_wstate = 0;       // (wall, particle) entries
+  double* simpart = nullptr;    // block partials of xdem_simulation_state
+  int64_t cap_simpart = 0;
   double rcut_inc = 0;
   double nbh_dist = 0;
   xdem::WallList wl[2];

@@ -73,6 +73,12 @@ class SurrogateStats(C.Structure):
                 ("max_err_free", C.c_double), ("rms_err", C.c_double), ("rms_err_free", C.c_double)]
 
 
+class SimState(C.Structure):
+    _fields_ = [("kinetic_energy", C.c_double * 3), ("rotation_energy", C.c_double * 3), ("mass", C.c_double),
+                ("volume", C.c_double), ("min_dn", C.c_double), ("n_particles", C.c_int64),
+                ("n_interactions", C.c_int64), ("n_active", C.c_int64)]
+
+
 GNN_NPARAMS = 205446
 
 
@@ -89,7 +95,7 @@ HIP_SYMBOLS = [
     "xdem_combined_compute_prolog", "xdem_combined_compute_epilog", "xdem_reset_force_moment",
     "xdem_gravity_force", "xdem_force_to_accel", "xdem_particle_displ_over", "xdem_move_particles",
     "xdem_chunk_neighbors_contact", "xdem_nbh_sphere", "xdem_classify_interactions", "xdem_contact_sphere",
-    "xdem_run_steps", "xdem_interactions_count", "xdem_interactions_dump", "xdem_interactions_load", "xdem_poly_interactions_load",
+    "xdem_run_steps", "xdem_interactions_count", "xdem_interactions_dump", "xdem_interactions_load", "xdem_poly_interactions_load", "xdem_simulation_state",
     "xdem_set_contact_record", "xdem_contact_state_count", "xdem_contact_state_dump", "xdem_nbh_stats",
     "xdem_kernel_timing", "xdem_set_timing", "xdem_streams_dump",
     "xdem_set_shapes", "xdem_compute_vertices", "xdem_vertex_displ_over", "xdem_nbh_polyhedron",
@@ -177,6 +183,7 @@ def load():
                                              C.POINTER(C.c_int64)]),
         "xdem_poly_interactions_load": (C.c_int, [vp, C.c_int64, _u64p, _u64p, C.POINTER(C.c_uint16),
                                                   C.POINTER(C.c_uint8), _u32p, _u32p, _dp, C.POINTER(C.c_int64)]),
+        "xdem_simulation_state": (C.c_int, [vp, C.POINTER(SimState)]),
         "xdem_set_contact_record": (C.c_int, [vp, C.c_int32]),
         "xdem_contact_state_count": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "xdem_contact_state_dump": (C.c_int, [vp, C.c_int64, _u64p, _u64p, _u32p, _u32p, C.POINTER(C.c_uint16),
@@ -399,6 +406,14 @@ class Context:
                   _ptr(tt, C.c_uint16), _ptr(sw, C.c_uint8), _ptr(si, C.c_uint32), _ptr(sj, C.c_uint32),
                   _ptr(fr, C.c_double), C.byref(got))
         return got.value
+
+    def simulation_state(self):
+        """Thermodynamic state (xdem_simulation_state) as a dict."""
+        st = SimState()
+        self.call("xdem_simulation_state", C.byref(st))
+        return {"kinetic_energy": list(st.kinetic_energy), "rotation_energy": list(st.rotation_energy),
+                "mass": st.mass, "volume": st.volume, "min_dn": st.min_dn, "n_particles": st.n_particles,
+                "n_interactions": st.n_interactions, "n_active": st.n_active}
 
     def set_contact_record(self, on=True):
         """Record the contact state of FP64 sphere force passes (outside the engine's integrating step)."""

@@ -270,6 +270,23 @@ int xdem_set_contact_record(xdem_ctx* ctx, int32_t on);
 int xdem_contact_state_count(xdem_ctx* ctx, int64_t* n);
 int xdem_contact_state_dump(xdem_ctx* ctx, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint32_t* sub_i,
                             uint32_t* sub_j, uint16_t* type, double* values, int64_t* written);
+/* Thermodynamic state of a sphere system (replaces simulation_state, src/io/simulation_state.cu:
+ * 116-200, with simulation_state.hpp:48-57 and itools/counter.hpp:140-153), reduced on the device in
+ * a fixed order: kinetic energy sum 1/2 m v_c^2 and rotation energy sum 1/2 m w_c^2 per component, mass,
+ * particle count, domain volume; interactions of the current lists (particle pairs counted twice as
+ * under symetric: true, walls once), the active ones (dn < 0 or non-zero friction history) and the
+ * smallest dn over them (0 when none overlaps), dn taken at the current positions. */
+typedef struct xdem_sim_state {
+  double kinetic_energy[3];
+  double rotation_energy[3];
+  double mass;
+  double volume;
+  double min_dn;
+  int64_t n_particles;
+  int64_t n_interactions;
+  int64_t n_active;
+} xdem_sim_state;
+int xdem_simulation_state(xdem_ctx* ctx, xdem_sim_state* out);
 /* Neighbour-list statistics of the last build: half-list pairs P, list slots
  * allocated (ELL with per-wave padding) and the grid dimensions. */
 int xdem_nbh_stats(xdem_ctx* ctx, int64_t* pairs, int64_t* slots, int32_t dims[3]);

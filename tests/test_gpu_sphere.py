@@ -571,3 +571,50 @@ def test_checkpoint_restart_with_history(gpu_ok, case, tmp_path):
     scale = float(np.max(np.abs(fa)))
     assert float(np.max(np.abs(fb - fa))) <= 1e-12 * scale
     assert not np.array_equal(kc, ka) or float(np.max(np.abs(fc - fa))) > 1e-6 * scale
+
+
+@pytest.mark.parametrize("case", ["walls", "periodic"])
+def test_simulation_state(gpu_ok, case):
+    """Thermodynamic state reduced on the device (simulation_state) against the same quantities in
+    float64 numpy from the downloaded fields and interactions: energies and mass within 1e-12
+    relative, particle and interaction counts exact (pairs twice, walls once; active = overlapping or
+    with history), the smallest dn from the positions, the domain volume."""
+    w = {"walls": wall_deposition, "periodic": periodic_pack}[case]()
+    s = gpu_for(w)
+    s.first_iteration()
+    s.run(20)
+    st = s.ctx.simulation_state()
+    p = by_id(s.ctx.download())
+    m = p["mass"]
+    for a, c in enumerate("xyz"):
+        ke = float(np.sum(0.5 * p["v" + c] * p["v" + c] * m))
+        re = float(np.sum(0.5 * p["vrot" + c] * p["vrot" + c] * m))
+        assert abs(st["kinetic_energy"][a] - ke) <= 1e-12 * max(ke, 1e-300), (c, st["kinetic_energy"][a], ke)
+        assert abs(st["rotation_energy"][a] - re) <= 1e-12 * max(re, 1e-300) or re == 0.0
+    assert abs(st["mass"] - float(np.sum(m))) <= 1e-12 * float(np.sum(m))
+    assert st["n_particles"] == len(m)
+    lo, hi = np.array(w.domain.lo), np.array(w.domain.hi)
+    assert st["volume"] == pytest.approx(float(np.prod(hi - lo)), rel=1e-15)
+    ii, jj, tt, fr = s.ctx.interactions()
+    r, R = pos(p), p["radius"]
+    per = np.array(list(w.domain.periodic), bool)
+    tot = act = 0
+    min_dn = 0.0
+    for k in range(len(ii)):
+        a = int(ii[k])
+        if tt[k] == 0:
+            d = r[a] - r[int(jj[k])]
+            d = np.where(per, d - (hi - lo) * np.round(d / (hi - lo)), d)
+            dn = float(np.sqrt(np.dot(d, d))) - (R[a] + R[int(jj[k])])
+            coef = 2
+        else:
+            wall = w.walls[int(jj[k])]
+            dn = abs(float(np.dot(r[a], wall["normal"])) - wall["offset"]) - R[a]
+            coef = 1
+        tot += coef
+        if dn < 0 or np.any(fr[k] != 0):
+            act += coef
+            min_dn = min(min_dn, dn if dn < 0 else 0.0)
+    print(f"{case}: {st['n_interactions']} interactions, {st['n_active']} active, min dn {st['min_dn']:.6g}")
+    assert st["n_interactions"] == tot and st["n_active"] == act and act > 0
+    assert abs(st["min_dn"] - min_dn) <= 1e-12 and min_dn < 0
