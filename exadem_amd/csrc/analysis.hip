@@ -57,10 +57,9 @@ __global__ __launch_bounds__(256) void k_sim_state(int64_t n, PState s, NbrList 
       const uint32_t mask = wl.mask[x];
       for (int w = 0; w < nw; w++) {
         if (!((mask >> w) & 1u)) continue;
-        const Surface& sf = walls[w];
-        const V3 proj = dot(rx, sf.normal) * sf.normal;
-        const V3 s2p = -(sf.center_proj - proj);
-        const double dn = norm(s2p) - Rx;
+        double dn;
+        V3 nrm, cpos;
+        driver_detect(walls[w], rx, Rx, dn, nrm, cpos);
         const int64_t j = (int64_t)w * wstride + x;
         v[7] += 1.0;
         if (dn < 0.0 || wl.hx[j] != 0.0 || wl.hy[j] != 0.0 || wl.hz[j] != 0.0) {

@@ -397,12 +397,23 @@ int xdem_set_surfaces(xdem_ctx* c, const xdem_surface* w, int32_t n) {
   if (n < 0 || n > XDEM_MAX_SURFACES) return set_err(c, XDEM_ERR_CAPACITY, "at most 8 surfaces");
   for (int k = 0; k < n; k++) {
     Surface s{};
+    s.kind = w[k].kind;
+    if (s.kind != XDEM_DRIVER_SURFACE && s.kind != XDEM_DRIVER_CYLINDER && s.kind != XDEM_DRIVER_BALL)
+      return set_err(c, XDEM_ERR_ARG, "driver: kind is XDEM_DRIVER_SURFACE, _CYLINDER or _BALL");
     s.offset = w[k].offset;
     s.normal = {w[k].normal[0], w[k].normal[1], w[k].normal[2]};
-    const double nn = norm(s.normal);
-    if (!(nn > 0)) return set_err(c, XDEM_ERR_ARG, "surface: normal must be non-zero");
+    s.radius = w[k].radius;
+    s.axis = {w[k].axis[0], w[k].axis[1], w[k].axis[2]};
+    if (s.kind == XDEM_DRIVER_SURFACE) {
+      if (!(norm(s.normal) > 0)) return set_err(c, XDEM_ERR_ARG, "surface: normal must be non-zero");
+    } else if (!(s.radius > 0)) {
+      return set_err(c, XDEM_ERR_ARG, "cylinder / ball: radius must be positive");
+    }
     s.center = {w[k].center[0], w[k].center[1], w[k].center[2]};
-    if (s.center.x == 0 && s.center.y == 0 && s.center.z == 0) s.center = s.offset * s.normal;
+    if (s.kind == XDEM_DRIVER_SURFACE && s.center.x == 0 && s.center.y == 0 && s.center.z == 0)
+      s.center = s.offset * s.normal;
+    if (s.kind == XDEM_DRIVER_CYLINDER)  // Cylinder::initialize (cylinder.hpp:104)
+      s.center = {s.axis.x * s.center.x, s.axis.y * s.center.y, s.axis.z * s.center.z};
     s.vel = {w[k].vel[0], w[k].vel[1], w[k].vel[2]};
     s.vrot = {w[k].vrot[0], w[k].vrot[1], w[k].vrot[2]};
     s.center_proj = s.offset * s.normal;

@@ -237,17 +237,16 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
     }
   }
   if (WALLS) {
-    // ContactLawDriverFunc<Surface> (contact_sphere.hpp:216-297) with Surface::detector (surface.hpp:337-350)
+    // ContactLawDriverFunc<Surface / Cylinder / Ball> (contact_sphere.hpp:216-297) with the driver's
+    // detector (driver_detect)
     const uint32_t mask = a.wl.mask[x];
     for (int w = 0; w < a.nw; w++) {
       if (!((mask >> w) & 1u)) continue;
       const Surface& sf = a.walls[w];
       const int64_t j = (int64_t)w * a.wstride + x;
-      const V3 proj = dot(rx, sf.normal) * sf.normal;
-      const V3 s2p = -(sf.center_proj - proj);
-      const double d = norm(s2p);
-      const double dn = d - Rx;
-      if (dn > 0) {
+      double dn;
+      V3 nrm, cpos;
+      if (!driver_detect(sf, rx, Rx, dn, nrm, cpos)) {
         a.wl.hx[j] = 0.0; a.wl.hy[j] = 0.0; a.wl.hz[j] = 0.0;
         a.wl.mhx[j] = 0.0; a.wl.mhy[j] = 0.0; a.wl.mhz[j] = 0.0;
         if (!INTEG && a.wstate) {
@@ -257,8 +256,6 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
         }
         continue;
       }
-      const V3 nrm = s2p / d;
-      const V3 cpos = rx - nrm * (Rx + 0.5 * dn);
       V3 ft{a.wl.hx[j], a.wl.hy[j], a.wl.hz[j]};
       V3 mint{0.0, 0.0, 0.0};
       if (KR) mint = {a.wl.mhx[j], a.wl.mhy[j], a.wl.mhz[j]};

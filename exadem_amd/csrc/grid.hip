@@ -451,7 +451,8 @@ __global__ __launch_bounds__(256) void k_hist_carry(CarryArgs a) {
   }
 }
 
-// Surface candidates (Surface::filter, surface.hpp:320-324, with rad + rcut_inc) and carry-over.
+// Driver candidates (the driver's filter with rad + rcut_inc, e.g. Surface::filter, surface.hpp:320-324)
+// and carry-over.
 __global__ __launch_bounds__(256) void k_wall_carry(int64_t n, int nw, const Surface* __restrict__ walls, PState s,
                                                     double rcut_inc, WallList nwl, WallList owl,
                                                     const uint32_t* __restrict__ perm, int have_old, int64_t stride) {
@@ -465,11 +466,9 @@ __global__ __launch_bounds__(256) void k_wall_carry(int64_t n, int nw, const Sur
   const uint8_t omask = have_old ? owl.mask[o] : (uint8_t)0;
   for (int w = 0; w < nw; w++) {
     const Surface& sf = walls[w];
-    const V3 proj = dot(p, sf.normal) * sf.normal;
-    const double d = norm(proj - sf.center_proj);
     const int64_t j = (int64_t)w * stride + x;
     double hx = 0, hy = 0, hz = 0, mx = 0, my = 0, mz = 0;
-    if (d <= rc) {
+    if (driver_filter(sf, p, rc)) {
       mask |= (uint8_t)(1u << w);
       if ((omask >> w) & 1u) {
         const int64_t jo = (int64_t)w * stride + o;

@@ -11,7 +11,8 @@
 //   input      particle_type, set_fields (src/field/set_fields.cu), init_rsa (src/rsa/init_rsa.cpp),
 //              read_xyz (example/spheres/movable-wall input format), polydisperse_pack (this library's
 //              seeded RSA + relaxation packing of the benchmark inputs)
-//   drivers    register_surface (src/driver/register_surface.cpp; STATIONARY motion only)
+//   drivers    register_surface, register_cylinder, register_ball (src/driver/register_*.cpp;
+//              STATIONARY motion only, a spinning cylinder or ball through vrot)
 //   scheme     combined_compute_prolog, combined_compute_epilog, reset_force_moment, gravity_force,
 //              force_to_accel, particle_displ_over, vertex_displ_over, compute_vertices
 //   lists      migrate_cell_particles_interaction / move_particles (with backup_r),
@@ -214,6 +215,8 @@ class Runner {
     N["set_fields"] = [this](const Node& s) { set_fields(s); };
     // ---- drivers
     N["register_surface"] = [this](const Node& s) { register_surface(s); };
+    N["register_cylinder"] = [this](const Node& s) { register_round(s, XDEM_DRIVER_CYLINDER, "register_cylinder"); };
+    N["register_ball"] = [this](const Node& s) { register_round(s, XDEM_DRIVER_BALL, "register_ball"); };
     // ---- scheme operators
     N["combined_compute_prolog"] = [this](const Node&) { check(xdem_combined_compute_prolog(ctx_, dt()), "prolog"); };
     N["combined_compute_epilog"] = [this](const Node&) { check(xdem_combined_compute_epilog(ctx_, dt()), "epilog"); };
@@ -449,6 +452,39 @@ class Runner {
     for (int a = 0; a < 3; a++) w.center[a] = w.normal[a] * w.offset;
     const size_t id = (size_t)sdouble(s, "id", (double)walls_.size());
     if (id >= XDEM_MAX_SURFACES) fail(XDEM_ERR_CAPACITY, "register_surface: at most 8 surfaces");
+    if (walls_.size() <= id) walls_.resize(id + 1);
+    walls_[id] = w;
+  }
+
+  // register_cylinder / register_ball (src/driver/register_cylinder.cpp, register_ball.cpp): state
+  // {center, radius[, axis (cylinder), vel, vrot]} as CylinderFields / BallFields decode it
+  // (cylinder.hpp:40-70, ball.hpp:44-80); STATIONARY motion only
+  void register_round(const Node& s, int kind, const char* what) {
+    const Node* st = slot(s, "state");
+    if (const Node* pr = slot(s, "params"))
+      if (const Node* m = slot(*pr, "motion_type"))
+        if (m->s != "STATIONARY") fail(XDEM_ERR_UNSUPPORTED, std::string(what) + ": only STATIONARY drivers are provided");
+    if (!st) fail(XDEM_ERR_ARG, std::string(what) + ": state required");
+    xdem_surface w{};
+    w.kind = kind;
+    auto vec3 = [&](const char* k, double* out, bool required) {
+      const Node* v = slot(*st, k);
+      if (!v) {
+        if (required) fail(XDEM_ERR_ARG, std::string(what) + ": " + k + " required");
+        return;
+      }
+      const auto x = xdem_host::to_doubles(*v);
+      if (x.size() != 3) fail(XDEM_ERR_ARG, std::string(what) + ": " + k + " needs 3 values");
+      for (int a = 0; a < 3; a++) out[a] = x[a];
+    };
+    vec3("center", w.center, true);
+    if (kind == XDEM_DRIVER_CYLINDER) vec3("axis", w.axis, true);
+    vec3("vel", w.vel, false);
+    vec3("vrot", w.vrot, false);
+    if (!slot(*st, "radius")) fail(XDEM_ERR_ARG, std::string(what) + ": radius required");
+    w.radius = sdouble(*st, "radius", 0.0);
+    const size_t id = (size_t)sdouble(s, "id", (double)walls_.size());
+    if (id >= XDEM_MAX_SURFACES) fail(XDEM_ERR_CAPACITY, std::string(what) + ": at most 8 drivers");
     if (walls_.size() <= id) walls_.resize(id + 1);
     walls_[id] = w;
   }

@@ -508,7 +508,7 @@ __global__ __launch_bounds__(256) void k_feat_carry(FeatCarryArgs a) {
   }
 }
 
-// vertex-wall candidates (Surface::filter with rVerlet + Minkowski radius) and their history
+// vertex-driver candidates (the driver's filter with rVerlet + Minkowski radius) and their history
 __global__ __launch_bounds__(256) void k_poly_wall(int64_t n, PState s, const xdem_shape* __restrict__ shapes,
                                                    const d4* __restrict__ vtx, int nw, const Surface* __restrict__ walls,
                                                    double rv, PolyWall nwl, PolyWall owl, const uint32_t* __restrict__ perm,
@@ -525,12 +525,10 @@ __global__ __launch_bounds__(256) void k_poly_wall(int64_t n, PState s, const xd
       const d4 v4 = vtx[x * XDEM_MAX_SHAPE_VERTICES + k];
       const V3 p{v4.x, v4.y, v4.z};
       const Surface& sf = walls[w];
-      const V3 proj = dot(p, sf.normal) * sf.normal;
-      const double d = norm(proj - sf.center_proj);
       const int bit = w * XDEM_MAX_SHAPE_VERTICES + k;
       const int64_t j = (int64_t)bit * stride + x;
       double hx = 0, hy = 0, hz = 0, mx = 0, my = 0, mz = 0;
-      if (d <= rv + r) {
+      if (driver_filter(sf, p, rv + r)) {
         mask |= 1ull << bit;
         if ((omask >> bit) & 1ull) {
           const int64_t jo = (int64_t)bit * stride + o;
@@ -858,11 +856,9 @@ __global__ __launch_bounds__(256) void k_poly_gather(PolyForceArgs a) {
         const d4 v4 = a.vtx[x * XDEM_MAX_SHAPE_VERTICES + k];
         const V3 p{v4.x, v4.y, v4.z};
         const Surface& sf = a.walls[w];
-        const V3 proj = dot(p, sf.normal) * sf.normal;
-        const V3 s2p = -(sf.center_proj - proj);
-        const double d = norm(s2p);
-        const double dn = d - r;
-        if (dn > 0) {
+        double dn;
+        V3 nrm, cpos;
+        if (!driver_detect(sf, p, r, dn, nrm, cpos)) {
           a.pw.hx[j] = 0.0; a.pw.hy[j] = 0.0; a.pw.hz[j] = 0.0;
           a.pw.mhx[j] = 0.0; a.pw.mhy[j] = 0.0; a.pw.mhz[j] = 0.0;
           if (a.wstate) {
@@ -872,8 +868,6 @@ __global__ __launch_bounds__(256) void k_poly_gather(PolyForceArgs a) {
           }
           continue;
         }
-        const V3 nrm = s2p / d;
-        const V3 cpos = p - nrm * (r + 0.5 * dn);
         V3 ft{a.pw.hx[j], a.pw.hy[j], a.pw.hz[j]};
         V3 mint{0.0, 0.0, 0.0};
         if (KR) mint = {a.pw.mhx[j], a.pw.mhy[j], a.pw.mhz[j]};

@@ -58,10 +58,73 @@ struct ContactParams {
   double kn, kt, kr, mu, damp_rate, fc, dncut, gamma;
 };
 
+// A stationary driver: plane (kind 0), cylinder (1) or ball (2), see xdem_surface.
 struct Surface {
   V3 normal, center, vel, vrot, center_proj;
   double offset;
+  int kind;
+  double radius;
+  V3 axis;  // cylinder: 0/1 mask of the coordinates the projection keeps
 };
+
+// Candidate test of the interaction update for a point p (a sphere centre or a polyhedron vertex)
+// with cut-off rc: Surface::filter (surface.hpp:320-324), Cylinder::filter (cylinder.hpp:225-234),
+// Ball::filter (ball.hpp:292-296).
+__host__ __device__ inline bool driver_filter(const Surface& s, V3 p, double rc) {
+  if (s.kind == XDEM_DRIVER_CYLINDER) {
+    const V3 proj{p.x * s.axis.x, p.y * s.axis.y, p.z * s.axis.z};
+    const V3 dir = proj - s.center;
+    const double d = norm(dir);
+    const double dn = s.radius - (d + rc);
+    return dn <= 0;
+  }
+  if (s.kind == XDEM_DRIVER_BALL) {
+    const V3 dist = s.center - p;
+    const double d = s.radius - norm(dist);
+    return fabs(d) <= rc;
+  }
+  const V3 proj = dot(p, s.normal) * s.normal;
+  const double d = norm(proj - s.center_proj);
+  return d <= rc;
+}
+
+// Detection of a point p with radius rc against the driver: Surface::detector (surface.hpp:337-350),
+// Cylinder::detector (cylinder.hpp:250-272), Ball::detector (ball.hpp:309-327). Returns the contact
+// flag; dn is set in every case, n and the contact point cp on contact.
+__host__ __device__ inline bool driver_detect(const Surface& s, V3 p, double rc, double& dn, V3& n, V3& cp) {
+  if (s.kind == XDEM_DRIVER_CYLINDER) {
+    const V3 proj{p.x * s.axis.x, p.y * s.axis.y, p.z * s.axis.z};
+    const V3 dir = s.center - proj;
+    const double d = norm(dir);
+    dn = s.radius - (rc + d);
+    if (dn > 0) return false;
+    n = dir / d;
+    cp = p - n * (rc + 0.5 * dn);
+    return true;
+  }
+  if (s.kind == XDEM_DRIVER_BALL) {
+    const V3 ptc = s.center - p;
+    const double d = norm(ptc);
+    n = ptc / d;
+    if (d > s.radius) {
+      dn = d - s.radius - rc;
+      n = -1.0 * n;
+    } else {
+      dn = s.radius - d - rc;
+    }
+    if (dn > 0) return false;
+    cp = p - n * (rc + 0.5 * dn);
+    return true;
+  }
+  const V3 proj = dot(p, s.normal) * s.normal;
+  const V3 s2p = -(s.center_proj - proj);
+  const double d = norm(s2p);
+  dn = d - rc;
+  if (dn > 0) return false;
+  n = s2p / d;
+  cp = p - n * (rc + 0.5 * dn);
+  return true;
+}
 
 // Domain and grid as seen by kernels.
 struct Grid {

@@ -40,9 +40,13 @@ class ContactParams(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("kn", "kt", "kr", "mu", "damp_rate", "fc", "dncut", "gamma")]
 
 
+DRIVER_SURFACE, DRIVER_CYLINDER, DRIVER_BALL = 0, 1, 2
+
+
 class Surface(C.Structure):
     _fields_ = [("offset", C.c_double), ("normal", C.c_double * 3), ("center", C.c_double * 3),
-                ("vel", C.c_double * 3), ("vrot", C.c_double * 3)]
+                ("vel", C.c_double * 3), ("vrot", C.c_double * 3), ("kind", C.c_int32), ("pad", C.c_int32),
+                ("radius", C.c_double), ("axis", C.c_double * 3)]
 
 
 class Shape(C.Structure):
@@ -240,15 +244,21 @@ def domain(lo, hi, periodic, cell_size):
 
 
 def surfaces(walls):
-    """walls: list of dicts {offset, normal[, center, vel, vrot]} -> ctypes array."""
+    """walls: list of driver dicts -> ctypes array. A plane: {offset, normal[, center, vel, vrot]}; a
+    cylinder: {"kind": "cylinder", radius, axis, center[, vel, vrot]}; a ball: {"kind": "ball", radius,
+    center[, vel, vrot]}."""
+    kinds = {"surface": DRIVER_SURFACE, "cylinder": DRIVER_CYLINDER, "ball": DRIVER_BALL}
     arr = (Surface * max(1, len(walls)))()
     for k, w in enumerate(walls):
-        arr[k].offset = w["offset"]
+        arr[k].kind = kinds[w.get("kind", "surface")]
+        arr[k].offset = w.get("offset", 0.0)
+        arr[k].radius = w.get("radius", 0.0)
         for a in range(3):
-            arr[k].normal[a] = w["normal"][a]
+            arr[k].normal[a] = w.get("normal", (0, 0, 0))[a]
             arr[k].center[a] = w.get("center", (0, 0, 0))[a]
             arr[k].vel[a] = w.get("vel", (0, 0, 0))[a]
             arr[k].vrot[a] = w.get("vrot", (0, 0, 0))[a]
+            arr[k].axis[a] = w.get("axis", (0, 0, 0))[a]
     return arr
 
 

@@ -85,14 +85,27 @@ typedef struct xdem_contact_params {
   double kn, kt, kr, mu, damp_rate, fc, dncut, gamma;
 } xdem_contact_params;
 
-/* Stationary planar wall: SurfaceFields (src/driver/include/exaDEM/surface.hpp:29-43).
- * center defaults to offset*normal when all three components are zero. */
+/* A stationary driver. kind XDEM_DRIVER_SURFACE: planar wall, SurfaceFields
+ * (src/driver/include/exaDEM/surface.hpp:29-43), center defaults to offset*normal when all three
+ * components are zero. XDEM_DRIVER_CYLINDER: CylinderFields (cylinder.hpp:28-36): radius, axis as
+ * the 0/1 mask of the coordinates the projection keeps ((1, 0, 1) for a cylinder along y), center
+ * (projected with the mask, Cylinder::initialize); particles inside. XDEM_DRIVER_BALL: BallFields
+ * (ball.hpp:27-40): center and radius, contact from inside or outside. vel and vrot enter the contact
+ * law as the driver's velocity and spin (a rotating drum); the driver does not move. offset and
+ * normal are used by planes only, radius and axis by cylinders and balls. */
+#define XDEM_DRIVER_SURFACE 0
+#define XDEM_DRIVER_CYLINDER 1
+#define XDEM_DRIVER_BALL 2
 typedef struct xdem_surface {
   double offset;
   double normal[3];
   double center[3];
   double vel[3];
   double vrot[3];
+  int32_t kind;
+  int32_t pad;
+  double radius;
+  double axis[3];
 } xdem_surface;
 
 /* Convex sphero-polyhedron shape (src/shape/include/exaDEM/shape.hpp:50-466): vertices, edges and

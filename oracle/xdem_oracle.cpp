@@ -149,17 +149,57 @@ static inline void contact_force_core(double dn, V3 n, double dt, const CP& p, d
 // compute_moments (common_kernels.hpp:134-142)
 static inline V3 compute_moments(V3 cp, V3 p, V3 f, V3 m) { return cross(cp - p, f) + m; }
 
-// Surface (surface.hpp:320-350)
+// Stationary drivers: Surface (surface.hpp:320-350), Cylinder (cylinder.hpp:225-272), Ball
+// (ball.hpp:292-327). kind: 0 surface, 1 cylinder, 2 ball.
 struct Surf {
   double offset;
   V3 normal, center, vel, vrot, center_proj;
+  int kind;
+  double radius;
+  V3 axis;
 };
+static inline V3 mask_mul(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
 static inline bool surf_filter(const Surf& s, double rcut, V3 p) {
+  if (s.kind == 1) {  // Cylinder::filter
+    const V3 proj = mask_mul(p, s.axis);
+    const V3 dir = proj - s.center;
+    const double d = norm(dir);
+    const double dn = s.radius - (d + rcut);
+    return dn <= 0;
+  }
+  if (s.kind == 2) {  // Ball::filter
+    const V3 dist = s.center - p;
+    const double d = s.radius - norm(dist);
+    return std::fabs(d) <= rcut;
+  }
   const V3 proj = dot(p, s.normal) * s.normal;
   const double d = norm(proj - s.center_proj);
   return d <= rcut;
 }
 static inline Contact surf_detect(const Surf& s, double rcut, V3 p) {
+  if (s.kind == 1) {  // Cylinder::detector
+    const V3 proj = mask_mul(p, s.axis);
+    const V3 dir = s.center - proj;
+    const double d = norm(dir);
+    const double dn = s.radius - (rcut + d);
+    if (dn > 0) return {false, dn, {0, 0, 0}, {0, 0, 0}};
+    const V3 n = dir / d;
+    return {true, dn, n, p - n * (rcut + 0.5 * dn)};
+  }
+  if (s.kind == 2) {  // Ball::detector
+    const V3 ptc = s.center - p;
+    const double d = norm(ptc);
+    V3 n = ptc / d;
+    double dn;
+    if (d > s.radius) {
+      dn = d - s.radius - rcut;
+      n = (-1.0) * n;
+    } else {
+      dn = s.radius - d - rcut;
+    }
+    if (dn > 0) return {false, dn, {0, 0, 0}, {0, 0, 0}};
+    return {true, dn, n, p - n * (rcut + 0.5 * dn)};
+  }
   const V3 proj = dot(p, s.normal) * s.normal;
   const V3 s2p = -(s.center_proj - proj);
   const double d = norm(s2p);
@@ -1150,10 +1190,14 @@ void xo_set_surfaces(void* h, const xdem_surface* w, int nw) {
   o->walls.clear();
   for (int k = 0; k < nw; k++) {
     Surf s{};
+    s.kind = w[k].kind;
+    s.radius = w[k].radius;
+    s.axis = {w[k].axis[0], w[k].axis[1], w[k].axis[2]};
     s.offset = w[k].offset;
     s.normal = {w[k].normal[0], w[k].normal[1], w[k].normal[2]};
     s.center = {w[k].center[0], w[k].center[1], w[k].center[2]};
-    if (s.center.x == 0 && s.center.y == 0 && s.center.z == 0) s.center = s.offset * s.normal;
+    if (s.kind == 0 && s.center.x == 0 && s.center.y == 0 && s.center.z == 0) s.center = s.offset * s.normal;
+    if (s.kind == 1) s.center = mask_mul(s.axis, s.center);  // Cylinder::initialize (cylinder.hpp:104)
     s.vel = {w[k].vel[0], w[k].vel[1], w[k].vel[2]};
     s.vrot = {w[k].vrot[0], w[k].vrot[1], w[k].vrot[2]};
     s.center_proj = s.offset * s.normal;  // Surface::initialize (surface.hpp:145-150)

@@ -173,3 +173,42 @@ def test_deck_interaction_dumps(gpu_ok, tmp_path):
     assert len(got) > 0 and got == lines
     keys, _ = itools.read_dump(itools.dump_path(str(out), "InteractionOutputDir-", 0))
     assert np.count_nonzero(keys[:, 4] == 5) > 0  # wall contacts at timestep 0
+
+
+@pytest.mark.gpu
+def test_deck_cylinder_drum_matches_operator_path(gpu_ok, tmp_path):
+    """register_cylinder (a spinning drum: state {center, axis, radius, vrot}) and register_surface in a
+    deck: the deck run equals the Python operator path on the same drivers bit for bit."""
+    import copy
+    from test_gpu_drivers import drum
+    w = copy.copy(drum())
+    p = dict(w.particles)
+    for k in ("vx", "vy", "vz"):
+        p[k] = np.zeros_like(p[k])  # set_fields gives every sphere velocity 0
+    w.particles = p
+    steps = 40
+    cyl, lo_end, hi_end = w.walls
+    planes = copy.copy(w)
+    planes.walls = [lo_end, hi_end]
+    path = wall_deck(tmp_path, planes, steps)
+    fl = lambda v: [float(x) for x in v]
+    drivers = (f"  - register_cylinder: {{ id: 0, state: {{ center: {fl(cyl['center'])}, axis: {fl(cyl['axis'])}, "
+               f"radius: {float(cyl['radius'])!r}, vrot: {fl(cyl['vrot'])} }}, params: {{ motion_type: STATIONARY }} }}\n"
+               f"  - register_surface: {{ id: 1, state: {{ normal: [0, 1, 0], offset: {float(lo_end['offset'])!r} }} }}\n"
+               f"  - register_surface: {{ id: 2, state: {{ normal: [0, 1, 0], offset: {float(hi_end['offset'])!r} }} }}\n")
+    text = open(path).read()
+    i, j = text.index("setup_drivers:\n"), text.index("compute_force:")
+    text = text[:i] + "setup_drivers:\n" + drivers + text[j:]
+    deck = native.Deck(write(tmp_path / "drum.msp", text))
+    ctx = native.Context()
+    st = deck.run(ctx)
+    assert st["iterations"] == steps
+    ref = SphereSim(w)
+    ref.first_iteration()
+    for _ in range(steps):
+        ref.step()
+    a, b = by_id(ctx.download()), by_id(ref.ctx.download())
+    for k in ("rx", "ry", "rz", "vx", "vy", "vz", "vrotx", "fx", "fz", "momx"):
+        assert np.array_equal(a[k], b[k]), k
+    ii, jj, tt, fr = ctx.interactions()
+    assert np.count_nonzero((tt == 5) & (jj == 0) & np.any(fr != 0, axis=1)) > 10  # contacts with the drum

@@ -296,6 +296,48 @@ def test_wall_contact_closed_form():
     assert sorted(jj[tt == 5].tolist()) == [4]  # only the floor (offset 0, normal +z) is a candidate
 
 
+@pytest.mark.parametrize("kind", ["cylinder", "ball_inside", "ball_outside"])
+def test_cylinder_and_ball_contact_closed_form(kind):
+    """Cylinder and ball drivers (cylinder.hpp:225-272, ball.hpp:292-327): a sphere at rest overlapping
+    the driver by 0.01 feels kn * 0.01 along the radial direction (towards the axis inside a cylinder
+    along y, towards the centre inside a ball, away from it outside), at any angle round the axis; a
+    sphere 0.05 away is a candidate only within rcut_inc and feels nothing."""
+    r, rc = 0.5, 0.1
+    c = np.array([5.0, 5.0, 5.0])
+    ang = 0.7
+    u = np.array([math.cos(ang), 0.0, math.sin(ang)]) if kind == "cylinder" else \
+        np.array([math.cos(ang) * 0.6, 0.8, math.sin(ang) * 0.6])
+    if kind == "cylinder":
+        wall = {"kind": "cylinder", "radius": 4.0, "axis": (1.0, 0.0, 1.0), "center": tuple(c)}
+        d_in, d_far = 4.0 - r + 0.01, 4.0 - r - 0.05      # distance of the centre from the axis
+        push = -u                                           # towards the axis
+    elif kind == "ball_inside":
+        wall = {"kind": "ball", "radius": 4.0, "center": tuple(c)}
+        d_in, d_far = 4.0 - r + 0.01, 4.0 - r - 0.05
+        push = -u
+    else:
+        wall = {"kind": "ball", "radius": 2.0, "center": tuple(c)}
+        d_in, d_far = 2.0 + r - 0.01, 2.0 + r + 0.05
+        push = u                                            # away from the centre
+    pos_in, pos_far = c + d_in * u, c + d_far * u + np.array([0.0, 0.0, 0.0])
+    pos_far = pos_far + (np.array([3.0, 0.0, 0.0]) if kind != "cylinder" else np.array([0.0, 2.5, 0.0]))
+    if kind != "cylinder":  # keep the far sphere at its distance from the centre, apart from the first
+        v = np.array([-u[2], 0.0, u[0]]) / math.hypot(u[0], u[2])
+        pos_far = c + d_far * v
+    p = inputs.sphere_set(np.array([pos_in[0], pos_far[0]]), np.array([pos_in[1], pos_far[1]]),
+                          np.array([pos_in[2], pos_far[2]]), np.array([r, r]), 2500.0)
+    dom = native.domain((0, 0, 0), (10, 10, 10), (False, False, False), 1.2)
+    o = po.Oracle(p, dom, [wall])
+    w = inputs.Workload("driver", p, dom, [], CP0, CP0, 1e-5, 1, rc, (0.0, 0.0, 0.0))
+    o.first_iteration(scheme(w))
+    q = by_id(o.particles())
+    f0 = np.array([q["fx"][0], q["fy"][0], q["fz"][0]])
+    assert np.allclose(f0, KN * 0.01 * push, rtol=1e-9, atol=1e-9)
+    assert np.all(np.array([q["fx"][1], q["fy"][1], q["fz"][1]]) == 0)
+    ii, jj, tt, fr, mo = o.interactions()
+    assert sorted(ii[tt == 5].tolist()) == [0, 1]  # both within rcut_inc of the driver
+
+
 # ---------------------------------------------------------------- the checks have teeth
 @pytest.mark.parametrize("mutation,check", [
     (1, check_damping),                       # damping sign flipped
