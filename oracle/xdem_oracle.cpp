@@ -20,7 +20,7 @@
 //   contact law      src/forcefield/include/exaDEM/forcefield/common_kernels.hpp:27-145,
 //                    contact_force.hpp:146-235, src/sphere/include/exaDEM/sphere/contact_sphere.hpp:121-297,
 //                    src/shape/include/exaDEM/shape_detection_core.hpp:57-81,
-//                    src/driver/include/exaDEM/surface.hpp:320-350
+//                    src/driver/include/exaDEM/surface.hpp:320-350, cylinder.hpp:225-272, ball.hpp:292-327
 //   operator order   data/config/main-config.msp:55-65 (first_iteration),
 //                    data/config/config_numerical_schemes.msp:6-28, config_move_particles.msp:141-195
 //
