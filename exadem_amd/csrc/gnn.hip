@@ -531,16 +531,19 @@ struct MsgStep {
 __device__ inline void msg_load(const MsgArgs& a, uint64_t sl, bool ok, uint32_t wd, int half, int ch0, uint32_t self,
                                 MsgStep& st) {
   st.ok = ok;
-  const uint32_t j = ok ? (wd & kIdxMask) : self;
+  (void)self;
   if (ok) {
     st.eb = *(const bf16x8*)(a.e + sl * FE + 8 * half);
+    const uint16_t* qrow = a.pq + (int64_t)(wd & kIdxMask) * H2 + H + ch0 + 8 * half;
+    st.q[0] = *(const uint4*)(qrow);
+    st.q[1] = *(const uint4*)(qrow + 16);
   } else {
+    // padding slot of the list group: nothing is gathered (its column of the MFMA is not summed)
 #pragma unroll
     for (int q = 0; q < 8; q++) st.eb[q] = (__bf16)0.0f;
+    st.q[0] = uint4{0u, 0u, 0u, 0u};
+    st.q[1] = uint4{0u, 0u, 0u, 0u};
   }
-  const uint16_t* qrow = a.pq + (int64_t)j * H2 + H + ch0 + 8 * half;
-  st.q[0] = *(const uint4*)(qrow);
-  st.q[1] = *(const uint4*)(qrow + 16);
 }
 __device__ inline float bf_lo(uint32_t v) { return bf2f((uint16_t)(v & 0xffffu)); }
 __device__ inline float bf_hi(uint32_t v) { return bf2f((uint16_t)(v >> 16)); }
