@@ -129,3 +129,20 @@ def test_drivers_fp32_form(gpu_ok, case):
     b.first_iteration()
     pa, pb = by_id(a.ctx.download()), by_id(b.ctx.download())
     assert rel_err(force(pb), force(pa), pa["mass"] * 9.81) <= 1e-5
+
+
+def test_driver_errors(gpu_ok):
+    """Invalid drivers fail with XDEM_ERR_ARG: an unknown kind, a cylinder or ball without a positive
+    radius, a plane without a normal; more than 8 drivers fail with XDEM_ERR_CAPACITY."""
+    ctx = native.Context()
+    for bad in ({"kind": "cylinder", "radius": 0.0, "axis": (1, 0, 1), "center": (1, 1, 1)},
+                {"kind": "ball", "radius": -1.0, "center": (1, 1, 1)},
+                {"offset": 1.0, "normal": (0.0, 0.0, 0.0)}):
+        with pytest.raises(native.XdemError, match="ARG"):
+            ctx.set_surfaces([bad])
+    arr = native.surfaces([{"offset": 0.0, "normal": (0, 0, 1)}])
+    arr[0].kind = 7
+    with pytest.raises(native.XdemError, match="ARG"):
+        ctx.call("xdem_set_surfaces", arr, 1)
+    with pytest.raises(native.XdemError, match="CAPACITY"):
+        ctx.set_surfaces([{"offset": float(k), "normal": (0, 0, 1)} for k in range(9)])
