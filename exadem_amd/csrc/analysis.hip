@@ -1,4 +1,5 @@
-// analysis.hip — the thermodynamic simulation state of spheres, reduced on the device
+// analysis.hip — the thermodynamic simulation state, reduced on the device (the interaction counts of
+// polyhedra come from poly_sim_counts in poly.hip)
 // (simulation_state, src/io/simulation_state.cu:116-200, with ReduceSimulationStateFunctor of
 // src/io/include/exaDEM/simulation_state.hpp:48-57 and the interaction counter of
 // src/interaction/include/exaDEM/itools/counter.hpp:140-153).
@@ -90,7 +91,7 @@ using namespace xdem;
 extern "C" int xdem_simulation_state(xdem_ctx* c, xdem_sim_state* out) {
   if (!c || !out) return XDEM_ERR_ARG;
   *out = {};
-  if (c->nshapes > 0) return set_err(c, XDEM_ERR_UNSUPPORTED, "simulation_state: spheres only");
+  const bool poly = c->nshapes > 0;
   const int64_t n = c->n;
   out->n_particles = n;
   out->volume = c->grid.L[0] * c->grid.L[1] * c->grid.L[2];
@@ -102,7 +103,7 @@ extern "C" int xdem_simulation_state(xdem_ctx* c, xdem_sim_state* out) {
     c->cap_simpart = (int64_t)blocks * kSimVals;
   }
   NbrList nl{};
-  if (c->nl_valid && c->hist_ready) nl = c->nl[c->nl_cur];
+  if (!poly && c->nl_valid && c->hist_ready) nl = c->nl[c->nl_cur];  // polyhedra: energies only here
   const bool walls = c->nwalls > 0 && c->wl_valid && nl.cnt;
   WallList wl{};
   if (walls) wl = c->wl[c->wl_cur];
@@ -125,5 +126,13 @@ extern "C" int xdem_simulation_state(xdem_ctx* c, xdem_sim_state* out) {
   out->n_interactions = (int64_t)acc[7];
   out->n_active = (int64_t)acc[8];
   out->min_dn = acc[9];
+  if (poly) {
+    double tot = 0, act = 0, mn = 0;
+    const int rc = poly_sim_counts(c, &tot, &act, &mn);
+    if (rc) return rc;
+    out->n_interactions = (int64_t)tot;
+    out->n_active = (int64_t)act;
+    out->min_dn = mn;
+  }
   return XDEM_OK;
 }

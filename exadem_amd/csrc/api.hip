@@ -335,7 +335,7 @@ void xdem_ctx_destroy(xdem_ctx* c) {
   dfree(&c->perm_tmp);
   dfree(&c->sort_key);
   dfree(&c->d_flags); dfree(&c->d_u32tmp); dfree(&c->d_walls);
-  dfree(&c->cstate); dfree(&c->wstate); dfree(&c->simpart); dfree(&c->d_shapes); dfree(&c->vtx); dfree(&c->frange); dfree(&c->fcontrib); dfree(&c->fcontact);
+  dfree(&c->cstate); dfree(&c->wstate); dfree(&c->simpart); dfree(&c->d_shapes); dfree(&c->vtx); dfree(&c->frange); dfree(&c->fcontrib); dfree(&c->fcontact); dfree(&c->fdn);
   for (int k = 0; k < 2; k++) {
     PolyList& P = c->pl[k];
     dfree(&P.start); dfree(&P.feat); dfree(&P.hx); dfree(&P.hy); dfree(&P.hz);

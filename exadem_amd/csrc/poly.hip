@@ -586,7 +586,9 @@ int poly_nbh(xdem_ctx* c, double rcut_inc) {
   }
   if ((int64_t)nf > c->cap_contrib) {
     const int64_t cap = (int64_t)(nf * 1.25) + 4096;
-    if ((rc = galloc(c, &c->fcontrib, cap * 9)) || (rc = galloc(c, &c->fcontact, cap))) return rc;
+    if ((rc = galloc(c, &c->fcontrib, cap * 9)) || (rc = galloc(c, &c->fcontact, cap)) ||
+        (rc = galloc(c, &c->fdn, cap)))
+      return rc;
     c->cap_contrib = cap;
   }
   if (nf >= 0xffffffffull) return set_err(c, XDEM_ERR_CAPACITY, "nbh_polyhedron: more than 2^32 - 2 features");
@@ -650,6 +652,7 @@ struct PolyForceArgs {
   double* wstate;
   double* contrib;    // [nfeat * 9]
   uint8_t* contact;   // [nfeat]
+  double* fdn;        // [nfeat] dn at this pass (0 without contact), for simulation_state
   const Surface* walls;
   int nw;
   int64_t wstride;
@@ -756,6 +759,7 @@ __global__ __launch_bounds__(128) void k_poly_feat(PolyForceArgs a) {
       Mo = cross(d.cp - r_j, nf) + (-mint);
       Mp = cross(d.cp - r_i, fc) + mint;
     }
+    a.fdn[f] = d.dn;
     double* pr = a.contrib + 9 * f;
     pr[0] = Fo.x; pr[1] = Fo.y; pr[2] = Fo.z;
     pr[3] = Mo.x; pr[4] = Mo.y; pr[5] = Mo.z;
@@ -763,6 +767,7 @@ __global__ __launch_bounds__(128) void k_poly_feat(PolyForceArgs a) {
     a.contact[f] = 1;
   } else {
     a.contact[f] = 0;
+    a.fdn[f] = 0.0;
     if (a.cstate) {
       double* rc = a.cstate + (uint64_t)kCStateLen * f;
 #pragma unroll
@@ -926,6 +931,7 @@ int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, d
   a.frange = c->frange;
   a.contrib = c->fcontrib;
   a.contact = c->fcontact;
+  a.fdn = c->fdn;
   a.nw = (c->nwalls > 0 && c->pw_valid) ? c->nwalls : 0;
   if (a.nw) a.pw = c->pw[c->pw_cur];
   a.walls = c->d_walls;
@@ -975,6 +981,7 @@ int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, d
   XDEM_HIP(c, hipGetLastError());
   c->cs_valid = c->record_cs;
   c->cs_epoch = c->nl_epoch;
+  c->poly_pass_epoch = c->nl_epoch;
   return XDEM_OK;
 }
 
@@ -1338,3 +1345,101 @@ extern "C" int xdem_poly_interactions_load(xdem_ctx* c, int64_t m, const uint64_
   if (matched) *matched = hit;
   return XDEM_OK;
 }
+
+namespace xdem {
+// simulation_state counts of polyhedra (itools/counter.hpp:140-153): features of the lists (particle
+// pairs, counted twice under symetric: true) and vertex-driver candidates (once); active: in contact at
+// the last pass or with history; the smallest dn over the active ones (0 when none overlaps).
+__device__ inline void block_reduce3(double tot, double act, double mn, double* part) {
+  __shared__ double r[3][256];
+  r[0][threadIdx.x] = tot;
+  r[1][threadIdx.x] = act;
+  r[2][threadIdx.x] = mn;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      r[0][threadIdx.x] += r[0][threadIdx.x + st];
+      r[1][threadIdx.x] += r[1][threadIdx.x + st];
+      r[2][threadIdx.x] = fmin(r[2][threadIdx.x], r[2][threadIdx.x + st]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) part[3 * (int64_t)blockIdx.x + threadIdx.x] = r[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_poly_sim_feat(int64_t nf, const uint32_t* __restrict__ feat,
+                                                       const uint8_t* __restrict__ contact,
+                                                       const double* __restrict__ fdn, double* __restrict__ part) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double tot = 0, act = 0, mn = 0;
+  if (f < nf) {
+    tot = 2.0;
+    const bool in = contact[f] != 0;
+    if (in || (feat[f] & kFeatActive)) {
+      act = 2.0;
+      mn = in && fdn[f] < 0.0 ? fdn[f] : 0.0;
+    }
+  }
+  block_reduce3(tot, act, mn, part);
+}
+
+__global__ __launch_bounds__(256) void k_poly_sim_wall(int64_t n, PState s, const xdem_shape* __restrict__ shapes,
+                                                       const d4* __restrict__ vtx, PolyWall pw,
+                                                       const Surface* __restrict__ walls, int nw, int64_t stride,
+                                                       double* __restrict__ part) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double tot = 0, act = 0, mn = 0;
+  if (x < n && nw > 0) {
+    const xdem_shape& sh = shapes[s.type[x]];
+    const double r = sh.minkowski * s.core[x].w.w;
+    const uint64_t mask = pw.mask[x];
+    for (int w = 0; w < nw; w++)
+      for (int k = 0; k < sh.nv; k++) {
+        const int bit = w * XDEM_MAX_SHAPE_VERTICES + k;
+        if (!((mask >> bit) & 1ull)) continue;
+        const d4 v4 = vtx[x * XDEM_MAX_SHAPE_VERTICES + k];
+        double dn;
+        V3 nrm, cpos;
+        const bool in = driver_detect(walls[w], V3{v4.x, v4.y, v4.z}, r, dn, nrm, cpos);
+        const int64_t j = (int64_t)bit * stride + x;
+        tot += 1.0;
+        if (in || pw.hx[j] != 0.0 || pw.hy[j] != 0.0 || pw.hz[j] != 0.0) {
+          act += 1.0;
+          mn = fmin(mn, in ? dn : 0.0);
+        }
+      }
+  }
+  block_reduce3(tot, act, mn, part);
+}
+
+int poly_sim_counts(xdem_ctx* c, double* tot, double* act, double* min_dn) {
+  *tot = *act = *min_dn = 0.0;
+  if (!c->pl_valid || c->pl_epoch != c->nl_epoch || c->poly_pass_epoch != c->nl_epoch)
+    return set_err(c, XDEM_ERR_STATE, "simulation_state: run contact_polyhedron after nbh_polyhedron first");
+  const PolyList& P = c->pl[c->pl_cur];
+  const int64_t nf = P.nfeat, n = c->n;
+  const int bf = grid_blocks(std::max<int64_t>(nf, 1), 256), bw = grid_blocks(std::max<int64_t>(n, 1), 256);
+  const int64_t need = 3 * (int64_t)(bf + bw);
+  if (need > c->cap_simpart) {
+    int rc = dev_realloc(c, &c->simpart, need);
+    if (rc) return rc;
+    c->cap_simpart = need;
+  }
+  const int nw = (c->nwalls > 0 && c->pw_valid) ? c->nwalls : 0;
+  hipLaunchKernelGGL(k_poly_sim_feat, dim3(bf), dim3(256), 0, c->stream, nf, P.feat, c->fcontact, c->fdn, c->simpart);
+  PolyWall pw{};
+  if (nw) pw = c->pw[c->pw_cur];
+  hipLaunchKernelGGL(k_poly_sim_wall, dim3(bw), dim3(256), 0, c->stream, n, live(c), c->d_shapes, c->vtx, pw,
+                     c->d_walls, nw, c->cap, c->simpart + 3 * (int64_t)bf);
+  XDEM_HIP(c, hipGetLastError());
+  std::vector<double> h((size_t)need);
+  XDEM_HIP(c, hipMemcpyAsync(h.data(), c->simpart, sizeof(double) * need, hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < bf + bw; k++) {
+    *tot += h[3 * (size_t)k];
+    *act += h[3 * (size_t)k + 1];
+    *min_dn = std::min(*min_dn, h[3 * (size_t)k + 2]);
+  }
+  return XDEM_OK;
+}
+}  // namespace xdem

@@ -382,6 +382,8 @@ struct xdem_ctx {
   bool pl_valid = false;
   double* fcontrib = nullptr;       // [cap_contrib * 9] per feature: F_owner, M_owner, M_partner terms
   uint8_t* fcontact = nullptr;      // [cap_contrib] 1 when the feature is in contact
+  double* fdn = nullptr;            // [cap_contrib] dn of the feature at the last pass (0 without contact)
+  uint64_t poly_pass_epoch = ~0ull; // neighbour build of the last polyhedron force pass
   int64_t cap_contrib = 0;
   uint2* frange = nullptr;          // [cap_frange] per list slot: first feature and count of its pair
                                     // (through the owner entry for partner entries); x = ~0u: no owner entry
@@ -446,6 +448,8 @@ int poly_compute_vertices(xdem_ctx* c, bool displ, double thr2, bool guarded);
 int poly_nbh(xdem_ctx* c, double rcut_inc);
 int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool fused, bool gravity_on,
                V3 g, bool guarded, int32_t step_tag);
+// interaction counts of the polyhedron lists for simulation_state: total, active, smallest dn
+int poly_sim_counts(xdem_ctx* c, double* tot, double* act, double* min_dn);
 int poly_contact_state_rows(xdem_ctx* c, std::vector<uint64_t>& ii, std::vector<uint64_t>& jj,
                             std::vector<uint32_t>& si, std::vector<uint32_t>& sj, std::vector<uint16_t>& tt,
                             std::vector<double>& vals);

@@ -283,12 +283,14 @@ int xdem_set_contact_record(xdem_ctx* ctx, int32_t on);
 int xdem_contact_state_count(xdem_ctx* ctx, int64_t* n);
 int xdem_contact_state_dump(xdem_ctx* ctx, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint32_t* sub_i,
                             uint32_t* sub_j, uint16_t* type, double* values, int64_t* written);
-/* Thermodynamic state of a sphere system (replaces simulation_state, src/io/simulation_state.cu:
- * 116-200, with simulation_state.hpp:48-57 and itools/counter.hpp:140-153), reduced on the device in
- * a fixed order: kinetic energy sum 1/2 m v_c^2 and rotation energy sum 1/2 m w_c^2 per component, mass,
- * particle count, domain volume; interactions of the current lists (particle pairs counted twice as
- * under symetric: true, walls once), the active ones (dn < 0 or non-zero friction history) and the
- * smallest dn over them (0 when none overlaps), dn taken at the current positions. */
+/* Thermodynamic state (replaces simulation_state, src/io/simulation_state.cu:116-200, with
+ * simulation_state.hpp:48-57 and itools/counter.hpp:140-153), reduced on the device in a fixed order:
+ * kinetic energy sum 1/2 m v_c^2 and rotation energy sum 1/2 m w_c^2 per component, mass, particle
+ * count, domain volume; interactions of the current lists (particle pairs - sphere pairs or polyhedron
+ * features - counted twice as under symetric: true, driver interactions once), the active ones (dn < 0
+ * or non-zero friction history) and the smallest dn over them (0 when none overlaps). Spheres: dn at the
+ * current positions. Polyhedra: dn and contact of the last contact_polyhedron pass (XDEM_ERR_STATE
+ * before one). */
 typedef struct xdem_sim_state {
   double kinetic_energy[3];
   double rotation_energy[3];

@@ -239,3 +239,28 @@ def test_checkpoint_restart_with_history(gpu_ok, tmp_path):
     for k in ("rx", "ry", "rz", "vx", "vy", "vz", "qw", "qx", "vrotx", "fx", "momx"):
         assert np.array_equal(pa[k], pb[k]), k
     assert not np.array_equal(kc, ka) or not np.array_equal(fc, fa)
+
+
+def test_simulation_state(gpu_ok):
+    """simulation_state of polyhedra: energies and mass as the downloaded fields give them, interactions
+    as the feature lists hold them (features twice, vertex-wall candidates once), the active ones (with
+    history), and the smallest dn equal to the smallest dn the recorded contact state of the same pass
+    holds."""
+    w = poly_case("walls")
+    s = PolySim(w)
+    s.ctx.set_contact_record(True)
+    s.first_iteration()
+    st = s.ctx.simulation_state()
+    p = by_id(s.ctx.download())
+    m = p["mass"]
+    ke = float(np.sum(0.5 * p["vx"] * p["vx"] * m))
+    assert abs(st["kinetic_energy"][0] - ke) <= 1e-12 * max(ke, 1e-300)
+    assert abs(st["mass"] - float(np.sum(m))) <= 1e-12 * float(np.sum(m))
+    ii, jj, tt, sw, si, sj, fr = s.ctx.poly_interactions()
+    coef = np.where(tt == 5, 1, 2)
+    assert st["n_interactions"] == int(np.sum(coef))
+    act = np.any(fr != 0, axis=1)
+    assert st["n_active"] == int(np.sum(coef[act])) and st["n_active"] > 0
+    _, _, _, v = s.ctx.contact_state()
+    assert st["min_dn"] == min(0.0, float(np.min(v[:, 0]))) and st["min_dn"] < 0
+    print(f"polyhedra: {st['n_interactions']} interactions, {st['n_active']} active, min dn {st['min_dn']:.6g}")
