@@ -547,9 +547,10 @@ extern "C" int xdem_interactions_count(xdem_ctx* c, int64_t* n_pp, int64_t* n_pw
   return XDEM_OK;
 }
 
-extern "C" int xdem_interactions_dump(xdem_ctx* c, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint16_t* type,
-                                      double* friction, int64_t* written) {
-  if (!c || !id_i || !id_j || !type || !friction) return XDEM_ERR_ARG;
+// The interaction rows of xdem_interactions_dump with, per row, the friction history or (moments) the
+// rolling-moment history.
+static int interactions_rows(xdem_ctx* c, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint16_t* type,
+                             double* friction, bool moments, int64_t* written) {
   if (written) *written = 0;
   if (!c->nl_valid || c->n == 0) return XDEM_OK;
   if (c->pending_kick) {
@@ -568,13 +569,19 @@ extern "C" int xdem_interactions_dump(xdem_ctx* c, int64_t cap, uint64_t* id_i, 
   if (L.slots > 0) {
     XDEM_HIP(c, hipMemcpyAsync(word.data(), L.word, sizeof(uint32_t) * L.slots, hipMemcpyDeviceToHost, c->stream));
     if (c->hist_fp32) {
-      XDEM_HIP(c, hipMemcpyAsync(fx.data(), L.fx32, sizeof(float) * L.slots, hipMemcpyDeviceToHost, c->stream));
-      XDEM_HIP(c, hipMemcpyAsync(fy.data(), L.fy32, sizeof(float) * L.slots, hipMemcpyDeviceToHost, c->stream));
-      XDEM_HIP(c, hipMemcpyAsync(fz.data(), L.fz32, sizeof(float) * L.slots, hipMemcpyDeviceToHost, c->stream));
+      XDEM_HIP(c, hipMemcpyAsync(fx.data(), moments ? L.mfx32 : L.fx32, sizeof(float) * L.slots, hipMemcpyDeviceToHost,
+                                 c->stream));
+      XDEM_HIP(c, hipMemcpyAsync(fy.data(), moments ? L.mfy32 : L.fy32, sizeof(float) * L.slots, hipMemcpyDeviceToHost,
+                                 c->stream));
+      XDEM_HIP(c, hipMemcpyAsync(fz.data(), moments ? L.mfz32 : L.fz32, sizeof(float) * L.slots, hipMemcpyDeviceToHost,
+                                 c->stream));
     } else {
-      XDEM_HIP(c, hipMemcpyAsync(hx.data(), L.hx, sizeof(double) * L.slots, hipMemcpyDeviceToHost, c->stream));
-      XDEM_HIP(c, hipMemcpyAsync(hy.data(), L.hy, sizeof(double) * L.slots, hipMemcpyDeviceToHost, c->stream));
-      XDEM_HIP(c, hipMemcpyAsync(hz.data(), L.hz, sizeof(double) * L.slots, hipMemcpyDeviceToHost, c->stream));
+      XDEM_HIP(c, hipMemcpyAsync(hx.data(), moments ? L.mhx : L.hx, sizeof(double) * L.slots, hipMemcpyDeviceToHost,
+                                 c->stream));
+      XDEM_HIP(c, hipMemcpyAsync(hy.data(), moments ? L.mhy : L.hy, sizeof(double) * L.slots, hipMemcpyDeviceToHost,
+                                 c->stream));
+      XDEM_HIP(c, hipMemcpyAsync(hz.data(), moments ? L.mhz : L.hz, sizeof(double) * L.slots, hipMemcpyDeviceToHost,
+                                 c->stream));
     }
   }
   std::vector<uint8_t> mask;
@@ -585,9 +592,9 @@ extern "C" int xdem_interactions_dump(xdem_ctx* c, int64_t cap, uint64_t* id_i, 
     mask.resize(n);
     whx.resize(m); why.resize(m); whz.resize(m);
     XDEM_HIP(c, hipMemcpyAsync(mask.data(), W.mask, n, hipMemcpyDeviceToHost, c->stream));
-    XDEM_HIP(c, hipMemcpyAsync(whx.data(), W.hx, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
-    XDEM_HIP(c, hipMemcpyAsync(why.data(), W.hy, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
-    XDEM_HIP(c, hipMemcpyAsync(whz.data(), W.hz, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(whx.data(), moments ? W.mhx : W.hx, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(why.data(), moments ? W.mhy : W.hy, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(whz.data(), moments ? W.mhz : W.hz, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
   }
   XDEM_HIP(c, hipStreamSynchronize(c->stream));
   if (c->hist_fp32)
@@ -628,6 +635,19 @@ extern "C" int xdem_interactions_dump(xdem_ctx* c, int64_t cap, uint64_t* id_i, 
   }
   if (written) *written = k;
   return XDEM_OK;
+}
+
+extern "C" int xdem_interactions_dump(xdem_ctx* c, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint16_t* type,
+                                      double* friction, int64_t* written) {
+  if (!c || !id_i || !id_j || !type || !friction) return XDEM_ERR_ARG;
+  return interactions_rows(c, cap, id_i, id_j, type, friction, false, written);
+}
+
+extern "C" int xdem_interactions_moments(xdem_ctx* c, int64_t cap, double* moment, int64_t* written) {
+  if (!c || !moment || cap < 0) return XDEM_ERR_ARG;
+  std::vector<uint64_t> a((size_t)std::max<int64_t>(cap, 1)), b(a.size());
+  std::vector<uint16_t> t(a.size());
+  return interactions_rows(c, cap, a.data(), b.data(), t.data(), moment, true, written);
 }
 
 // Full (both-direction) neighbour streams: particle id, neighbour id, image code, owner flag.

@@ -1069,10 +1069,11 @@ extern "C" int xdem_contact_polyhedron(xdem_ctx* c, const xdem_contact_params* c
   return poly_force(c, a, b, dt, false, false, V3{0, 0, 0}, false, 0);
 }
 
-extern "C" int xdem_poly_interactions_dump(xdem_ctx* c, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint16_t* type,
-                                           uint8_t* swap, uint32_t* sub_i, uint32_t* sub_j, double* friction,
-                                           int64_t* written) {
-  if (!c || !id_i || !id_j || !type || !swap || !sub_i || !sub_j || !friction) return XDEM_ERR_ARG;
+// The rows of xdem_poly_interactions_dump with, per row, the friction history or (moments) the
+// rolling-moment history.
+static int poly_interactions_rows(xdem_ctx* c, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint16_t* type,
+                                  uint8_t* swap, uint32_t* sub_i, uint32_t* sub_j, double* friction, bool moments,
+                                  int64_t* written) {
   if (written) *written = 0;
   if (!c->pl_valid || c->n == 0) return XDEM_OK;
   if (c->pl_epoch != c->nl_epoch) return set_err(c, XDEM_ERR_STATE, "poly dump: run nbh_polyhedron first");
@@ -1095,9 +1096,12 @@ extern "C" int xdem_poly_interactions_dump(xdem_ctx* c, int64_t cap, uint64_t* i
   }
   if (P.nfeat > 0) {
     XDEM_HIP(c, hipMemcpyAsync(feat.data(), P.feat, sizeof(uint32_t) * P.nfeat, hipMemcpyDeviceToHost, c->stream));
-    XDEM_HIP(c, hipMemcpyAsync(hx.data(), P.hx, sizeof(double) * P.nfeat, hipMemcpyDeviceToHost, c->stream));
-    XDEM_HIP(c, hipMemcpyAsync(hy.data(), P.hy, sizeof(double) * P.nfeat, hipMemcpyDeviceToHost, c->stream));
-    XDEM_HIP(c, hipMemcpyAsync(hz.data(), P.hz, sizeof(double) * P.nfeat, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(hx.data(), moments ? P.mhx : P.hx, sizeof(double) * P.nfeat, hipMemcpyDeviceToHost,
+                               c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(hy.data(), moments ? P.mhy : P.hy, sizeof(double) * P.nfeat, hipMemcpyDeviceToHost,
+                               c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(hz.data(), moments ? P.mhz : P.hz, sizeof(double) * P.nfeat, hipMemcpyDeviceToHost,
+                               c->stream));
   }
   std::vector<uint64_t> wmask;
   std::vector<double> whx, why, whz;
@@ -1108,9 +1112,9 @@ extern "C" int xdem_poly_interactions_dump(xdem_ctx* c, int64_t cap, uint64_t* i
     wmask.resize(n);
     whx.resize(m); why.resize(m); whz.resize(m);
     XDEM_HIP(c, hipMemcpyAsync(wmask.data(), W.mask, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, c->stream));
-    XDEM_HIP(c, hipMemcpyAsync(whx.data(), W.hx, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
-    XDEM_HIP(c, hipMemcpyAsync(why.data(), W.hy, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
-    XDEM_HIP(c, hipMemcpyAsync(whz.data(), W.hz, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(whx.data(), moments ? W.mhx : W.hx, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(why.data(), moments ? W.mhy : W.hy, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(whz.data(), moments ? W.mhz : W.hz, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
   }
   XDEM_HIP(c, hipStreamSynchronize(c->stream));
   int64_t k = 0;
@@ -1148,6 +1152,24 @@ extern "C" int xdem_poly_interactions_dump(xdem_ctx* c, int64_t cap, uint64_t* i
   }
   if (written) *written = k;
   return XDEM_OK;
+}
+
+extern "C" int xdem_poly_interactions_dump(xdem_ctx* c, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint16_t* type,
+                                           uint8_t* swap, uint32_t* sub_i, uint32_t* sub_j, double* friction,
+                                           int64_t* written) {
+  if (!c || !id_i || !id_j || !type || !swap || !sub_i || !sub_j || !friction) return XDEM_ERR_ARG;
+  return poly_interactions_rows(c, cap, id_i, id_j, type, swap, sub_i, sub_j, friction, false, written);
+}
+
+extern "C" int xdem_poly_interactions_moments(xdem_ctx* c, int64_t cap, double* moment, int64_t* written) {
+  if (!c || !moment || cap < 0) return XDEM_ERR_ARG;
+  const size_t m = (size_t)std::max<int64_t>(cap, 1);
+  std::vector<uint64_t> a(m), b(m);
+  std::vector<uint16_t> t(m);
+  std::vector<uint8_t> sw(m);
+  std::vector<uint32_t> si(m), sj(m);
+  return poly_interactions_rows(c, cap, a.data(), b.data(), t.data(), sw.data(), si.data(), sj.data(), moment, true,
+                                written);
 }
 
 extern "C" int xdem_poly_interactions_count(xdem_ctx* c, int64_t* n) {
@@ -1264,7 +1286,8 @@ struct PKeyHash {
 
 extern "C" int xdem_poly_interactions_load(xdem_ctx* c, int64_t m, const uint64_t* id_i, const uint64_t* id_j,
                                            const uint16_t* type, const uint8_t* swap, const uint32_t* sub_i,
-                                           const uint32_t* sub_j, const double* friction, int64_t* matched) {
+                                           const uint32_t* sub_j, const double* friction, const double* moment,
+                                           int64_t* matched) {
   if (!c || m < 0 || (m > 0 && (!id_i || !id_j || !type || !swap || !sub_i || !sub_j || !friction)))
     return XDEM_ERR_ARG;
   if (matched) *matched = 0;
@@ -1291,7 +1314,7 @@ extern "C" int xdem_poly_interactions_load(xdem_ctx* c, int64_t m, const uint64_
   if (L.slots > 0)
     XDEM_HIP(c, hipMemcpyAsync(word.data(), L.word, sizeof(uint32_t) * L.slots, hipMemcpyDeviceToHost, c->stream));
   XDEM_HIP(c, hipStreamSynchronize(c->stream));
-  std::vector<double> h[3];
+  std::vector<double> h[6];
   for (auto& a : h) a.assign((size_t)nf, 0.0);
   int64_t hit = 0;
   for (int64_t f = 0; f < nf; f++) {
@@ -1303,18 +1326,17 @@ extern "C" int xdem_poly_interactions_load(xdem_ctx* c, int64_t m, const uint64_
     if (it != saved.end()) {
       for (int q = 0; q < 3; q++) {
         h[q][f] = friction[3 * it->second + q];
-        act = act || h[q][f] != 0.0;
+        h[3 + q][f] = moment ? moment[3 * it->second + q] : 0.0;
+        act = act || h[q][f] != 0.0 || h[3 + q][f] != 0.0;
       }
       hit++;
     }
     feat[f] = act ? (fw | kFeatActive) : (fw & ~kFeatActive);
   }
   if (nf > 0) {
-    double* dst[3] = {P.hx, P.hy, P.hz};
-    for (int q = 0; q < 3; q++)
+    double* dst[6] = {P.hx, P.hy, P.hz, P.mhx, P.mhy, P.mhz};
+    for (int q = 0; q < 6; q++)
       XDEM_HIP(c, hipMemcpyAsync(dst[q], h[q].data(), sizeof(double) * nf, hipMemcpyHostToDevice, c->stream));
-    double* mdst[3] = {P.mhx, P.mhy, P.mhz};
-    for (int q = 0; q < 3; q++) XDEM_HIP(c, hipMemsetAsync(mdst[q], 0, sizeof(double) * nf, c->stream));
     XDEM_HIP(c, hipMemcpyAsync(P.feat, feat.data(), sizeof(uint32_t) * nf, hipMemcpyHostToDevice, c->stream));
   }
   if (c->nwalls > 0 && c->pw_valid) {
@@ -1323,7 +1345,7 @@ extern "C" int xdem_poly_interactions_load(xdem_ctx* c, int64_t m, const uint64_
     std::vector<uint64_t> mask(n);
     XDEM_HIP(c, hipMemcpyAsync(mask.data(), W.mask, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, c->stream));
     XDEM_HIP(c, hipStreamSynchronize(c->stream));
-    std::vector<double> wh[3];
+    std::vector<double> wh[6];
     for (auto& a : wh) a.assign((size_t)mw, 0.0);
     for (int64_t x = 0; x < n; x++)
       for (int bit = 0; bit < c->nwalls * XDEM_MAX_SHAPE_VERTICES; bit++) {
@@ -1332,16 +1354,19 @@ extern "C" int xdem_poly_interactions_load(xdem_ctx* c, int64_t m, const uint64_
                                         (uint32_t)(bit % XDEM_MAX_SHAPE_VERTICES), 0xffffffffu, 5, 0});
         if (it == saved.end()) continue;
         const int64_t j = (int64_t)bit * c->cap + x;
-        for (int q = 0; q < 3; q++) wh[q][j] = friction[3 * it->second + q];
+        for (int q = 0; q < 3; q++) {
+          wh[q][j] = friction[3 * it->second + q];
+          wh[3 + q][j] = moment ? moment[3 * it->second + q] : 0.0;
+        }
         hit++;
       }
-    double* dst[3] = {W.hx, W.hy, W.hz};
-    for (int q = 0; q < 3; q++)
+    double* dst[6] = {W.hx, W.hy, W.hz, W.mhx, W.mhy, W.mhz};
+    for (int q = 0; q < 6; q++)
       XDEM_HIP(c, hipMemcpyAsync(dst[q], wh[q].data(), sizeof(double) * mw, hipMemcpyHostToDevice, c->stream));
-    double* mdst[3] = {W.mhx, W.mhy, W.mhz};
-    for (int q = 0; q < 3; q++) XDEM_HIP(c, hipMemsetAsync(mdst[q], 0, sizeof(double) * mw, c->stream));
   }
   XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  if (moment)
+    for (int64_t k = 0; k < 3 * m && !c->kr_used; k++) c->kr_used = moment[k] != 0.0;
   if (matched) *matched = hit;
   return XDEM_OK;
 }

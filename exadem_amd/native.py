@@ -99,7 +99,8 @@ HIP_SYMBOLS = [
     "xdem_combined_compute_prolog", "xdem_combined_compute_epilog", "xdem_reset_force_moment",
     "xdem_gravity_force", "xdem_force_to_accel", "xdem_particle_displ_over", "xdem_move_particles",
     "xdem_chunk_neighbors_contact", "xdem_nbh_sphere", "xdem_classify_interactions", "xdem_contact_sphere",
-    "xdem_run_steps", "xdem_interactions_count", "xdem_interactions_dump", "xdem_interactions_load", "xdem_poly_interactions_load", "xdem_simulation_state",
+    "xdem_run_steps", "xdem_interactions_count", "xdem_interactions_dump", "xdem_interactions_load", "xdem_poly_interactions_load", "xdem_interactions_moments",
+    "xdem_poly_interactions_moments", "xdem_simulation_state",
     "xdem_set_contact_record", "xdem_contact_state_count", "xdem_contact_state_dump", "xdem_nbh_stats",
     "xdem_kernel_timing", "xdem_set_timing", "xdem_streams_dump",
     "xdem_set_shapes", "xdem_compute_vertices", "xdem_vertex_displ_over", "xdem_nbh_polyhedron",
@@ -186,7 +187,9 @@ def load():
         "xdem_interactions_load": (C.c_int, [vp, C.c_int64, _u64p, _u64p, C.POINTER(C.c_uint16), _dp, _dp,
                                              C.POINTER(C.c_int64)]),
         "xdem_poly_interactions_load": (C.c_int, [vp, C.c_int64, _u64p, _u64p, C.POINTER(C.c_uint16),
-                                                  C.POINTER(C.c_uint8), _u32p, _u32p, _dp, C.POINTER(C.c_int64)]),
+                                                  C.POINTER(C.c_uint8), _u32p, _u32p, _dp, _dp, C.POINTER(C.c_int64)]),
+        "xdem_interactions_moments": (C.c_int, [vp, C.c_int64, _dp, C.POINTER(C.c_int64)]),
+        "xdem_poly_interactions_moments": (C.c_int, [vp, C.c_int64, _dp, C.POINTER(C.c_int64)]),
         "xdem_simulation_state": (C.c_int, [vp, C.POINTER(SimState)]),
         "xdem_set_contact_record": (C.c_int, [vp, C.c_int32]),
         "xdem_contact_state_count": (C.c_int, [vp, C.POINTER(C.c_int64)]),
@@ -400,7 +403,27 @@ class Context:
                   _ptr(fr, C.c_double), None if mo is None else _ptr(mo, C.c_double), C.byref(got))
         return got.value
 
-    def poly_interactions_load(self, ii, jj, tt, sw, si, sj, friction):
+    def interaction_moments(self):
+        """Rolling-moment history [rows][3] of the rows interactions() returns, in the same order."""
+        npp, npw = C.c_int64(), C.c_int64()
+        self.call("xdem_interactions_count", C.byref(npp), C.byref(npw))
+        cap = max(1, npp.value + npw.value)
+        mo = np.zeros(3 * cap)
+        wr = C.c_int64()
+        self.call("xdem_interactions_moments", cap, _ptr(mo, C.c_double), C.byref(wr))
+        return mo[:3 * wr.value].reshape(-1, 3)
+
+    def poly_interaction_moments(self):
+        """Rolling-moment history [rows][3] of the rows poly_interactions() returns, in the same order."""
+        n = C.c_int64()
+        self.call("xdem_poly_interactions_count", C.byref(n))
+        cap = max(1, n.value)
+        mo = np.zeros(3 * cap)
+        wr = C.c_int64()
+        self.call("xdem_poly_interactions_moments", cap, _ptr(mo, C.c_double), C.byref(wr))
+        return mo[:3 * wr.value].reshape(-1, 3)
+
+    def poly_interactions_load(self, ii, jj, tt, sw, si, sj, friction, moment=None):
         """Polyhedra: sets the history of the current feature and vertex-wall interactions from saved
         rows (restart); returns the number of rows applied."""
         ii = np.ascontiguousarray(ii, np.uint64)
@@ -410,11 +433,12 @@ class Context:
         si = np.ascontiguousarray(si, np.uint32)
         sj = np.ascontiguousarray(sj, np.uint32)
         fr = np.ascontiguousarray(friction, np.float64).reshape(-1)
-        assert fr.size == 3 * ii.size
+        mo = None if moment is None else np.ascontiguousarray(moment, np.float64).reshape(-1)
+        assert fr.size == 3 * ii.size and (mo is None or mo.size == fr.size)
         got = C.c_int64()
         self.call("xdem_poly_interactions_load", ii.size, _ptr(ii, C.c_uint64), _ptr(jj, C.c_uint64),
                   _ptr(tt, C.c_uint16), _ptr(sw, C.c_uint8), _ptr(si, C.c_uint32), _ptr(sj, C.c_uint32),
-                  _ptr(fr, C.c_double), C.byref(got))
+                  _ptr(fr, C.c_double), None if mo is None else _ptr(mo, C.c_double), C.byref(got))
         return got.value
 
     def simulation_state(self):

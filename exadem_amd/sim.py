@@ -77,11 +77,11 @@ class SphereSim:
     def checkpoint(self, path):
         """Restart file (numpy .npz): every particle field, in device order, and the interactions with
         their friction history (the layout of xdem_interactions_dump) - what the reference's restart
-        dump carries (sim_dump_writer_interaction.cpp:25-46). The rolling-moment history is not saved
-        (xdem_interactions_dump has no moment column)."""
+        dump carries (sim_dump_writer_interaction.cpp:25-46), with the rolling-moment history."""
         p = self.ctx.download()
         ii, jj, tt, fr = self.ctx.interactions()
-        np.savez(path, ii=ii, jj=jj, tt=tt, fr=fr, **{"p_" + k: v for k, v in p.items()})
+        mo = self.ctx.interaction_moments()
+        np.savez(path, ii=ii, jj=jj, tt=tt, fr=fr, mo=mo, **{"p_" + k: v for k, v in p.items()})
 
     @classmethod
     def restart(cls, workload, path, device=0, form=native.XDEM_FORM_FP64, history=True):
@@ -94,7 +94,7 @@ class SphereSim:
         sim = cls(w, device=device, form=form)
         sim.rebuild()
         if history:
-            sim.ctx.interactions_load(d["ii"], d["jj"], d["tt"], d["fr"])
+            sim.ctx.interactions_load(d["ii"], d["jj"], d["tt"], d["fr"], d["mo"])
         return sim
 
 
@@ -108,10 +108,11 @@ class PolySim(SphereSim):
 
     def checkpoint(self, path):
         """Restart file: every particle field and the feature and vertex-wall interactions with their
-        friction history (the layout of xdem_poly_interactions_dump)."""
+        friction and rolling-moment history (the layout of xdem_poly_interactions_dump)."""
         p = self.ctx.download()
         ii, jj, tt, sw, si, sj, fr = self.ctx.poly_interactions()
-        np.savez(path, ii=ii, jj=jj, tt=tt, sw=sw, si=si, sj=sj, fr=fr, **{"p_" + k: v for k, v in p.items()})
+        mo = self.ctx.poly_interaction_moments()
+        np.savez(path, ii=ii, jj=jj, tt=tt, sw=sw, si=si, sj=sj, fr=fr, mo=mo, **{"p_" + k: v for k, v in p.items()})
 
     @classmethod
     def restart(cls, workload, path, device=0, history=True):
@@ -122,7 +123,7 @@ class PolySim(SphereSim):
         sim = cls(w, device=device)
         sim.rebuild()
         if history:
-            sim.ctx.poly_interactions_load(d["ii"], d["jj"], d["tt"], d["sw"], d["si"], d["sj"], d["fr"])
+            sim.ctx.poly_interactions_load(d["ii"], d["jj"], d["tt"], d["sw"], d["si"], d["sj"], d["fr"], d["mo"])
         return sim
 
     def rebuild(self):

@@ -236,6 +236,9 @@ int xdem_contact_polyhedron(xdem_ctx* ctx, const xdem_contact_params* cp, const 
 int xdem_poly_interactions_count(xdem_ctx* ctx, int64_t* n);
 int xdem_poly_interactions_dump(xdem_ctx* ctx, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint16_t* type,
                                 uint8_t* swap, uint32_t* sub_i, uint32_t* sub_j, double* friction, int64_t* written);
+/* The rolling-moment history (3 per row) of the rows xdem_poly_interactions_dump writes, in the same
+ * order (restart files of decks with kr != 0). */
+int xdem_poly_interactions_moments(xdem_ctx* ctx, int64_t cap, double* moment, int64_t* written);
 
 /* ------------------------------------------------------------ inspection */
 /* Number of particle-particle interactions held (owner side, half list) and
@@ -246,6 +249,9 @@ int xdem_interactions_count(xdem_ctx* ctx, int64_t* n_pp, int64_t* n_pw);
  * friction history (3 doubles per interaction). cap is the array length. */
 int xdem_interactions_dump(xdem_ctx* ctx, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint16_t* type,
                            double* friction, int64_t* written);
+/* The rolling-moment history (3 per row) of the rows xdem_interactions_dump writes, in the same order
+ * (interaction.hpp:33-35 keeps it beside the friction; restart files of decks with kr != 0). */
+int xdem_interactions_moments(xdem_ctx* ctx, int64_t cap, double* moment, int64_t* written);
 /* Restart with interaction history (replaces the history hand-over of the reference's restart: dumped
  * interactions re-enter the interaction update by key, interaction_manager.hpp:88-112 and 205-217,
  * sim_dump_writer_interaction.cpp:25-46). After nbh_sphere on the restored particles, sets the history
@@ -258,12 +264,13 @@ int xdem_interactions_load(xdem_ctx* ctx, int64_t m, const uint64_t* id_i, const
                            const uint16_t* type, const double* friction, const double* moment, int64_t* matched);
 /* The same for polyhedra: after nbh_polyhedron on the restored particles, saved rows in the layout of
  * xdem_poly_interactions_dump (owner id, partner id or wall index, type, swap, owner and partner
- * sub-features, friction) set the friction history of the current feature and vertex-wall
- * interactions with the same key; the rolling-moment history starts from zero. *matched = rows
+ * sub-features, friction) set the friction history and, when moment is not NULL, the rolling-moment
+ * history of the current feature and vertex-wall interactions with the same key. *matched = rows
  * applied. */
 int xdem_poly_interactions_load(xdem_ctx* ctx, int64_t m, const uint64_t* id_i, const uint64_t* id_j,
                                 const uint16_t* type, const uint8_t* swap, const uint32_t* sub_i,
-                                const uint32_t* sub_j, const double* friction, int64_t* matched);
+                                const uint32_t* sub_j, const double* friction, const double* moment,
+                                int64_t* matched);
 /* Contact-state recording for analyses (replaces the ContactState the reference's contact operators
  * fill for every interaction: contact_force_sphere.cu.in:190-199 with sphere/contact_sphere.hpp:
  * 175-202 and 282-296; contact_force_polyhedron.cu.in with polyhedron/contact.hpp:160-210). With

@@ -523,7 +523,7 @@ def test_packing_statistics_match_oracle(gpu_ok):
     assert cn_b == cn_g and abs(phi_b - phi_g) <= 1e-9
 
 
-@pytest.mark.parametrize("case", ["walls", "periodic"])
+@pytest.mark.parametrize("case", ["walls", "periodic", "walls_kr"])
 def test_checkpoint_restart_with_history(gpu_ok, case, tmp_path):
     """Restart with interaction history: a run checkpointed after 60 steps (particles and interaction
     history) and restarted (particles uploaded, lists rebuilt, history handed over by key) is, bit for
@@ -531,8 +531,12 @@ def test_checkpoint_restart_with_history(gpu_ok, case, tmp_path):
     that rebuild, after 60 more steps: the same contacts (interactions with history), friction history
     within 1e-12 and trajectories within 1e-10 (the lists are built at other positions, so the
     summation order and the rebuild steps may differ). Restarting without the history gives another
-    state."""
-    w = {"walls": wall_deposition, "periodic": periodic_pack}[case]()
+    state. walls_kr: with a rolling stiffness, so the rolling-moment history is saved and handed over
+    as well."""
+    w = {"walls": wall_deposition, "periodic": periodic_pack, "walls_kr": wall_deposition}[case]()
+    if case == "walls_kr":
+        w.cp = native.contact_params(kn=1e5, kt=8e4, mu=0.3, damp_rate=0.9, kr=2e3)
+        w.cpd = w.cp
     a = gpu_for(w)
     a.first_iteration()
     a.run(60)
@@ -540,6 +544,8 @@ def test_checkpoint_restart_with_history(gpu_ok, case, tmp_path):
     a.checkpoint(ck)
     nhist = int(np.count_nonzero(np.any(np.load(ck)["fr"] != 0, axis=1)))
     assert nhist > 0
+    if case == "walls_kr":
+        assert np.load(ck)["mo"].shape == np.load(ck)["fr"].shape
     # what a restart is: the uninterrupted run with a neighbour rebuild at the checkpoint step
     r = gpu_for(w)
     r.first_iteration()
