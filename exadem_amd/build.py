@@ -21,6 +21,7 @@ HIP_SOURCES = [
     "poly.hip",
     "gnn.hip",
     "analysis.hip",
+    "mesh.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -330,6 +330,7 @@ void xdem_ctx_destroy(xdem_ctx* c) {
     free_wl(c->wl[k]);
     for (auto& e : c->timers[k].ev) (void)hipEventDestroy(e);
   }
+  mesh_free(c);
   dfree(&c->core_alt);
   dfree(&c->cell_of); dfree(&c->cell_count); dfree(&c->cell_start); dfree(&c->cell_rank); dfree(&c->perm);
   dfree(&c->perm_tmp);

@@ -14,6 +14,7 @@
 #include <cmath>
 
 #include "contact_law.h"
+#include "mesh.h"
 #include "step.h"
 
 namespace xdem {
@@ -67,6 +68,9 @@ struct ForceArgs {
   // (wall, particle), kCStateLen doubles each (null: not recorded)
   double* cstate;
   double* wstate;
+  // triangle-mesh driver (mesh.cap = 0: none)
+  MeshView mesh;
+  MeshList ml;
 };
 
 
@@ -239,7 +243,7 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
   if (WALLS) {
     // ContactLawDriverFunc<Surface / Cylinder / Ball> (contact_sphere.hpp:216-297) with the driver's
     // detector (driver_detect)
-    const uint32_t mask = a.wl.mask[x];
+    const uint32_t mask = a.nw ? a.wl.mask[x] : 0u;
     for (int w = 0; w < a.nw; w++) {
       if (!((mask >> w) & 1u)) continue;
       const Surface& sf = a.walls[w];
@@ -274,6 +278,32 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
       }
       F = F + f;
       M = M + (cross(cpos - rx, f) + mint);
+    }
+    if (a.mesh.cap) {
+      // ContactLawRShapeDriverFunc (contact_sphere.hpp:298-396) over the sphere's mesh candidates in list
+      // order: driver at the mesh centre, stationary; no contact resets the history
+      const uint32_t mc = min(a.ml.cnt[x], (uint32_t)a.mesh.cap);
+      for (uint32_t k = 0; k < mc; k++) {
+        const int64_t j = (int64_t)k * a.mesh.stride + x;
+        double dn;
+        V3 nrm, cpos;
+        if (!mesh_detect(a.mesh, a.ml.word[j], rx, Rx, dn, nrm, cpos)) {
+          a.ml.hx[j] = 0.0; a.ml.hy[j] = 0.0; a.ml.hz[j] = 0.0;
+          a.ml.mhx[j] = 0.0; a.ml.mhy[j] = 0.0; a.ml.mhz[j] = 0.0;
+          continue;
+        }
+        V3 ft{a.ml.hx[j], a.ml.hy[j], a.ml.hz[j]};
+        V3 mint{0.0, 0.0, 0.0};
+        if (KR) mint = {a.ml.mhx[j], a.ml.mhy[j], a.ml.mhz[j]};
+        const V3 zero{0.0, 0.0, 0.0};
+        const V3 f = hooke_force<KR>(dn, nrm, a.dt, a.cpd, mx, ft, mint, cpos, rx, vx, wx, a.mesh.center, zero, zero);
+        a.ml.hx[j] = ft.x; a.ml.hy[j] = ft.y; a.ml.hz[j] = ft.z;
+        if (KR) {
+          a.ml.mhx[j] = mint.x; a.ml.mhy[j] = mint.y; a.ml.mhz[j] = mint.z;
+        }
+        F = F + f;
+        M = M + (cross(cpos - rx, f) + mint);
+      }
     }
   }
   if (FUSED && INTEG) {
@@ -341,14 +371,20 @@ int launch_step(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, 
   int rc = set_history_form(c, false);
   if (rc) return rc;
   if (cp.kr != 0.0 || cpd.kr != 0.0) c->kr_used = true;
-  const bool walls = c->nwalls > 0 && c->wl_valid;
+  const bool planes = c->nwalls > 0 && c->wl_valid;
+  const bool meshed = c->has_mesh && c->ml_valid;
+  const bool walls = planes || meshed;
   ForceArgs a{};
   a.n = c->n;
   a.s = live(c);
   a.nl = c->nl[c->nl_cur];
-  if (walls) a.wl = c->wl[c->wl_cur];
+  if (planes) a.wl = c->wl[c->wl_cur];
   a.walls = c->d_walls;
-  a.nw = c->nwalls;
+  a.nw = planes ? c->nwalls : 0;
+  if (meshed) {
+    a.mesh = c->mesh;
+    a.ml = c->ml[c->ml_cur];
+  }
   a.wstride = c->cap;
   a.g = c->grid;
   a.cp = cp;
@@ -387,14 +423,20 @@ int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd,
   int rc = set_history_form(c, false);
   if (rc) return rc;
   if (cp.kr != 0.0 || cpd.kr != 0.0) c->kr_used = true;
-  const bool walls = c->nwalls > 0 && c->wl_valid;
+  const bool planes = c->nwalls > 0 && c->wl_valid;
+  const bool meshed = c->has_mesh && c->ml_valid;
+  const bool walls = planes || meshed;
   ForceArgs a{};
   a.n = c->n;
   a.s = live(c);
   a.nl = c->nl[c->nl_cur];
-  if (walls) a.wl = c->wl[c->wl_cur];
+  if (planes) a.wl = c->wl[c->wl_cur];
   a.walls = c->d_walls;
-  a.nw = c->nwalls;
+  a.nw = planes ? c->nwalls : 0;
+  if (meshed) {
+    a.mesh = c->mesh;
+    a.ml = c->ml[c->ml_cur];
+  }
   a.wstride = c->cap;
   a.g = c->grid;
   a.cp = cp;
@@ -413,7 +455,7 @@ int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd,
       c->cap_cstate = cap;
     }
     a.cstate = c->cstate;
-    if (walls) {
+    if (planes) {
       const int64_t m = (int64_t)c->nwalls * c->cap;
       if (m > c->cap_wstate) {
         if ((rc = dev_realloc(c, &c->wstate, m * kCStateLen))) return rc;

@@ -385,6 +385,7 @@ static void launch32(xdem_ctx* c, const ForceArgs32& a, bool guarded) {
 
 int launch_force_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool fused,
                       bool gravity_on, V3 g, bool guarded, int32_t step_tag) {
+  if (c->has_mesh) return set_err(c, XDEM_ERR_ARG, "mesh drivers: the FP64 form only");
   int rc = set_history_form(c, true);
   if (rc) return rc;
   if (cp.kr != 0.0 || cpd.kr != 0.0) c->kr_used = true;
@@ -424,6 +425,7 @@ int launch_force_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams&
 // FP32-form engine step in one kernel (see launch_step)
 int launch_step_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool gravity_on, V3 g,
                      double thr2, int32_t step_tag, Core** input) {
+  if (c->has_mesh) return set_err(c, XDEM_ERR_ARG, "mesh drivers: the FP64 form only");
   int rc = set_history_form(c, true);
   if (rc) return rc;
   if (cp.kr != 0.0 || cpd.kr != 0.0) c->kr_used = true;

@@ -508,6 +508,7 @@ int do_nbh_sphere(xdem_ctx* c, double rcut_inc) {
     c->wl_cur = nxt;
     c->wl_valid = true;
   }
+  if ((rc = mesh_carry(c, rcut_inc, have_old))) return rc;
   c->nl_prev_valid = false;
   c->perm_valid = false;
   c->hist_ready = true;

@@ -1,0 +1,135 @@
+// mesh.h — sphere against a stationary triangle-mesh driver: candidate filters and contact detection
+// (the sphere is a vertex of radius R; the mesh features carry the Minkowski radius rm).
+//
+//   candidates  detection_sphere_driver (src/sphere/include/exaDEM/sphere/nbh_sphere.hpp:25-86): per driver
+//               vertex, edge and face passing its filter with rVerlet = rcut_inc
+//   filters     filter_vertex_vertex, filter_vertex_edge_core, filter_vertex_face
+//               (src/shape/include/exaDEM/shape_detection_core.hpp:43-53, 83-101;
+//               shape_detection_api.hpp:246-309)
+//   detection   detect<7>, detect<8>, detect<9> (shape_detection_api.hpp:357-386) over
+//               detection_vertex_vertex_core, detection_vertex_edge_core, detection_vertex_face_core
+//               (shape_detection_core.hpp:57-81, 122-148, 397-466)
+#pragma once
+
+#include "xdem_internal.h"
+
+namespace xdem {
+
+constexpr uint32_t kMeshKindShift = 30;
+constexpr uint32_t kMeshIdxMask = (1u << 30) - 1u;
+
+__device__ inline V3 mesh_unit(V3 v) { return v / norm(v); }
+
+// Vertex against a triangle: distance to its plane (normal oriented towards the vertex) and whether the
+// vertex projects inside (crossing count of the projected point over the triangle's edges). early: the
+// distance exceeds limit.
+__device__ inline bool mesh_vf_core(V3 vi, const V3 (&t)[3], double limit, double& dist_out, V3& n_out,
+                                    bool& early) {
+  const V3 va = t[0], vb = t[1], vc = t[2];
+  const V3 v = vi - va;
+  V3 v1 = vb - va;
+  V3 v2 = vc - va;
+  v1 = mesh_unit(v1);
+  V3 n = cross(v1, v2);
+  n = mesh_unit(n);
+  double dist = dot(n, v);
+  if (dist < 0) {
+    n = n * (-1);
+    dist = -dist;
+  }
+  early = false;
+  if (dist > limit) {
+    early = true;
+    return false;
+  }
+  const V3 P = vi - n * dist;
+  int odd = 0;
+  v2 = cross(n, v1);
+  const double ori1 = dot(P, v1), ori2 = dot(P, v2);
+#pragma unroll
+  for (int iva = 0; iva < 3; ++iva) {
+    const int ivb = iva == 2 ? 0 : iva + 1;
+    const V3 A = t[iva], B = t[ivb];
+    const double pa1 = dot(A, v1), pb1 = dot(B, v1), pa2 = dot(A, v2), pb2 = dot(B, v2);
+    if ((pa2 < ori2 && pb2 >= ori2) || (pb2 < ori2 && pa2 >= ori2)) {
+      if (pa1 + (ori2 - pa2) / (pb2 - pa2) * (pb1 - pa1) < ori1) odd = 1 - odd;
+    }
+  }
+  dist_out = dist;
+  n_out = n;
+  return odd == 1;
+}
+
+__device__ inline void mesh_tri(const MeshView& m, uint32_t k, V3 (&t)[3]) {
+  const uint4 f = m.f[k];
+  t[0] = m.v[f.x];
+  t[1] = m.v[f.y];
+  t[2] = m.v[f.z];
+}
+
+// Candidate filter of feature word wd for a sphere at p of radius R (rVerlet = rv).
+__device__ inline bool mesh_filter(const MeshView& m, uint32_t wd, V3 p, double R, double rv) {
+  const uint32_t kind = wd >> kMeshKindShift, k = wd & kMeshIdxMask;
+  if (kind == 0) {
+    const double Rm = R + m.rm + rv;
+    const V3 d = p - m.v[k];
+    return dot(d, d) <= Rm * Rm;
+  }
+  if (kind == 1) {
+    const uint2 e = m.e[k];
+    const V3 vf = m.v[e.x], vs = m.v[e.y];
+    const V3 fs = vs - vf;
+    const V3 fi = p - vf;
+    const double r = dot(fs, fi) / dot(fs, fs);
+    if (r < -rv || r > 1.0 + rv) return false;
+    const V3 d = fi - fs * r;
+    const double Rmax = R + m.rm + rv;
+    return dot(d, d) < Rmax * Rmax;
+  }
+  V3 t[3];
+  mesh_tri(m, k, t);
+  double dist;
+  V3 n;
+  bool early;
+  return mesh_vf_core(p, t, R + m.rm + rv, dist, n, early);
+}
+
+// Contact of feature word wd with the sphere (p, R): overlap dn, normal n (from the mesh to the sphere)
+// and contact point cp.
+__device__ inline bool mesh_detect(const MeshView& m, uint32_t wd, V3 p, double R, double& dn, V3& n, V3& cp) {
+  const uint32_t kind = wd >> kMeshKindShift, k = wd & kMeshIdxMask;
+  if (kind == 0) {
+    const double Rs = R + m.rm;
+    const V3 dist = p - m.v[k];
+    const double dist_norm = norm(dist);
+    const double inv = 1.0 / dist_norm;
+    dn = dist_norm - Rs;
+    n = dist * inv;
+    cp = p - n * (R + 0.5 * dn);
+    return dn < 0;
+  }
+  if (kind == 1) {
+    const uint2 e = m.e[k];
+    const V3 vf = m.v[e.x], vs = m.v[e.y];
+    const V3 fs = vs - vf, fi = p - vf;
+    const double r = dot(fs, fi) / dot(fs, fs);
+    if (r <= 0 || r >= 1.0) return false;
+    V3 nn = fi - fs * r;
+    dn = norm(nn) - (R + m.rm);
+    if (dn > 0) return false;
+    n = mesh_unit(nn);
+    cp = p - n * (R + 0.5 * dn);
+    return true;
+  }
+  V3 t[3];
+  mesh_tri(m, k, t);
+  double dist;
+  bool early;
+  const bool in = mesh_vf_core(p, t, R + m.rm, dist, n, early);
+  if (early) return false;
+  dn = dist - (R + m.rm);
+  cp = p - n * (R + 0.5 * dn);
+  return in;
+}
+
+}  // namespace xdem

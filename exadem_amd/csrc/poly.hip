@@ -552,6 +552,7 @@ static int galloc(xdem_ctx* c, T** p, int64_t count) {
 
 int poly_nbh(xdem_ctx* c, double rcut_inc) {
   if (!c->nl_valid) return set_err(c, XDEM_ERR_STATE, "nbh_polyhedron: run chunk_neighbors_contact first");
+  if (c->has_mesh) return set_err(c, XDEM_ERR_ARG, "nbh_polyhedron: mesh drivers act on spheres only");
   const int64_t n = c->n;
   int rc;
   if ((rc = poly_compute_vertices(c, false, 0.0, false))) return rc;

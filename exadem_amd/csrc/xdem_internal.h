@@ -295,6 +295,35 @@ struct KernelTimer {
 
 }  // namespace xdem
 
+namespace xdem {
+// Stationary triangle-mesh driver (register_stl_mesh, src/driver/register_stl_mesh.cpp:30-140): world
+// vertices, edges (two per entry, shared edges listed once per face as build_shape lists them), triangles,
+// and a uniform grid of bins over the mesh: bin b lists, as words sorted ascending, the features whose
+// bounding box enlarged by `reach` overlaps it. Word: kind << 30 | index, kind 0 vertex (interaction
+// type 7), 1 edge (8), 2 face (9).
+struct MeshView {
+  const V3* v = nullptr;
+  const uint2* e = nullptr;
+  const uint4* f = nullptr;
+  double rm = 0;  // Minkowski radius
+  V3 center{0, 0, 0};
+  V3 lo{0, 0, 0};
+  double inv_h = 0;
+  int dim[3] = {0, 0, 0};
+  const uint32_t* start = nullptr;
+  const uint32_t* feat = nullptr;
+  int cap = 0;  // candidate slots per particle (0: no mesh)
+  int64_t stride = 0;
+};
+// Per particle candidates: slot k of particle x at k * stride + x; cnt may exceed the capacity after a
+// failed build (readers clamp).
+struct MeshList {
+  uint32_t* cnt = nullptr;
+  uint32_t* word = nullptr;
+  double *hx = nullptr, *hy = nullptr, *hz = nullptr, *mhx = nullptr, *mhy = nullptr, *mhz = nullptr;
+};
+}  // namespace xdem
+
 struct xdem_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -391,6 +420,26 @@ struct xdem_ctx {
   xdem::PolyWall pw[2];             // vertex-wall candidates and history, current and previous
   int pw_cur = 0;
   bool pw_valid = false;
+
+  bool has_mesh = false;            // a triangle-mesh driver is registered
+  int32_t mesh_id = 0;              // its driver index (partner id of its interactions)
+  int mesh_cap = 0;                 // candidate slots per particle
+  xdem::MeshView mesh{};            // device view (bins filled at the neighbour build)
+  std::vector<xdem::V3> mesh_hv;    // host copies for binning
+  std::vector<uint2> mesh_he;
+  std::vector<uint4> mesh_hf;
+  double mesh_reach = -1;           // reach the bins were built for (-1: none)
+  double mesh_rv = -1;              // rcut_inc the bins were built for
+  xdem::V3* d_mesh_v = nullptr;
+  uint2* d_mesh_e = nullptr;
+  uint4* d_mesh_f = nullptr;
+  uint32_t* d_mesh_start = nullptr;
+  uint32_t* d_mesh_feat = nullptr;
+  xdem::MeshList ml[2];             // candidates and history, current and previous
+  int ml_cur = 0;
+  bool ml_valid = false;
+  int64_t ml_alloc = 0;             // particle capacity ml was allocated for
+  int32_t* d_mesh_over = nullptr;   // [2] largest candidate count of the last build, max radius bits
 };
 
 // helpers shared between translation units
@@ -439,6 +488,8 @@ int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd,
 int launch_step(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, int form, bool gravity_on,
                 V3 g, double thr2, int32_t step_tag, Core** input);
 int alloc_walls(xdem_ctx* c, int nw);
+int mesh_carry(xdem_ctx* c, double rcut_inc, bool have_old);  // mesh candidates + history carry-over
+void mesh_free(xdem_ctx* c);
 int set_history_form(xdem_ctx* c, bool fp32);
 int engine_rebuild(xdem_ctx* c, double rcut_inc);
 int engine_force_pass(xdem_ctx* c, const xdem_scheme* s);  // fused force pass at the current state

@@ -107,7 +107,7 @@ HIP_SYMBOLS = [
     "xdem_contact_polyhedron", "xdem_poly_interactions_count", "xdem_poly_interactions_dump",
     "xdem_gnn_create", "xdem_gnn_destroy", "xdem_gnn_set_params", "xdem_gnn_get_params", "xdem_gnn_get_grads",
     "xdem_gnn_forward", "xdem_gnn_set_target", "xdem_gnn_backward", "xdem_gnn_adam", "xdem_gnn_train_window",
-    "xdem_gnn_dump_graph", "xdem_surrogate_run",
+    "xdem_gnn_dump_graph", "xdem_surrogate_run", "xdem_set_mesh", "xdem_mesh_interactions",
 ]
 
 
@@ -191,6 +191,10 @@ def load():
         "xdem_interactions_moments": (C.c_int, [vp, C.c_int64, _dp, C.POINTER(C.c_int64)]),
         "xdem_poly_interactions_moments": (C.c_int, [vp, C.c_int64, _dp, C.POINTER(C.c_int64)]),
         "xdem_simulation_state": (C.c_int, [vp, C.POINTER(SimState)]),
+        "xdem_set_mesh": (C.c_int, [vp, C.c_int32, _dp, C.c_int64, _u32p, C.c_int64, _u32p, C.c_int64, C.c_double,
+                                    _dp, _dp, C.c_int32]),
+        "xdem_mesh_interactions": (C.c_int, [vp, C.c_int64, _u64p, _u32p, C.POINTER(C.c_uint16), _dp, _dp,
+                                             C.POINTER(C.c_int64)]),
         "xdem_set_contact_record": (C.c_int, [vp, C.c_int32]),
         "xdem_contact_state_count": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "xdem_contact_state_dump": (C.c_int, [vp, C.c_int64, _u64p, _u64p, _u32p, _u32p, C.POINTER(C.c_uint16),
@@ -412,6 +416,34 @@ class Context:
         wr = C.c_int64()
         self.call("xdem_interactions_moments", cap, _ptr(mo, C.c_double), C.byref(wr))
         return mo[:3 * wr.value].reshape(-1, 3)
+
+    def set_mesh(self, mesh_id, vertices, edges, faces, minkowski, center=(0.0, 0.0, 0.0),
+                 quat=(1.0, 0.0, 0.0, 0.0), max_candidates=32):
+        """Registers a stationary triangle-mesh driver acting on spheres (vertices [nv][3] in the mesh frame,
+        edges [ne][2], triangles [nf][3]); takes effect at the next neighbour build."""
+        v = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
+        e = np.ascontiguousarray(edges, np.uint32).reshape(-1, 2)
+        f = np.ascontiguousarray(faces, np.uint32).reshape(-1, 3)
+        c = np.ascontiguousarray(center, np.float64).reshape(3)
+        q = np.ascontiguousarray(quat, np.float64).reshape(4)
+        self.call("xdem_set_mesh", int(mesh_id), _ptr(v, C.c_double), len(v), _ptr(e, C.c_uint32), len(e),
+                  _ptr(f, C.c_uint32), len(f), float(minkowski), _ptr(c, C.c_double), _ptr(q, C.c_double),
+                  int(max_candidates))
+
+    def mesh_interactions(self):
+        """Mesh candidates with history: (owner id, feature, type 7/8/9, friction [k][3], moment [k][3]),
+        sorted by (owner id, type, feature)."""
+        wr = C.c_int64()
+        self.call("xdem_mesh_interactions", 1 << 62, None, None, None, None, None, C.byref(wr))
+        n = wr.value
+        ii = np.zeros(max(n, 1), np.uint64)
+        sj = np.zeros(max(n, 1), np.uint32)
+        tt = np.zeros(max(n, 1), np.uint16)
+        fr = np.zeros(3 * max(n, 1))
+        mo = np.zeros(3 * max(n, 1))
+        self.call("xdem_mesh_interactions", n, _ptr(ii, C.c_uint64), _ptr(sj, C.c_uint32), _ptr(tt, C.c_uint16),
+                  _ptr(fr, C.c_double), _ptr(mo, C.c_double), C.byref(wr))
+        return ii[:n], sj[:n], tt[:n], fr[:3 * n].reshape(-1, 3), mo[:3 * n].reshape(-1, 3)
 
     def poly_interaction_moments(self):
         """Rolling-moment history [rows][3] of the rows poly_interactions() returns, in the same order."""

@@ -28,12 +28,15 @@ def scheme(workload, form=native.XDEM_FORM_FP64, poll=16):
 
 
 class SphereSim:
-    def __init__(self, workload, device=0, form=native.XDEM_FORM_FP64, ctx=None):
+    def __init__(self, workload, device=0, form=native.XDEM_FORM_FP64, ctx=None, mesh=None):
+        """mesh: keyword arguments of Context.set_mesh (a triangle-mesh driver), or None."""
         self.w = workload
         self.form = form
         self.ctx = ctx or native.Context(device)
         self.ctx.set_domain(workload.domain)
         self.ctx.set_surfaces(workload.walls)
+        if mesh is not None:
+            self.ctx.set_mesh(**mesh)
         self.ctx.upload(workload.particles)
         self.rebuilds = 0
 

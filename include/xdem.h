@@ -404,6 +404,23 @@ int xdem_gnn_dump_graph(xdem_gnn* gnn, int64_t cap, uint32_t* src, uint32_t* dst
 int xdem_surrogate_run(xdem_gnn* gnn, const xdem_scheme* scheme, int64_t nsteps, int32_t check_every, double tol,
                        xdem_surrogate_stats* stats);
 
+/* Stationary triangle-mesh driver acting on spheres (register_stl_mesh, src/driver/register_stl_mesh.cpp:30-140;
+ * contact ContactLawRShapeDriverFunc, src/sphere/include/exaDEM/sphere/contact_sphere.hpp:298-396; candidates
+ * detection_sphere_driver, src/sphere/include/exaDEM/sphere/nbh_sphere.hpp:25-86). vertices [nv][3] in the
+ * mesh frame (world vertex = center + quat (w, x, y, z) applied to it), edges [ne][2] and triangles [nf][3]
+ * as vertex indices, Minkowski radius, driver index id (partner id of its interactions, types 7 vertex,
+ * 8 edge, 9 face). max_candidates: slots per sphere for candidate features (a neighbour build with more
+ * fails with XDEM_ERR_CAPACITY). nv = 0 removes the mesh. Takes effect at the next nbh_sphere; the FP64
+ * form only; spheres only. */
+int xdem_set_mesh(xdem_ctx* ctx, int32_t id, const double* vertices, int64_t nv, const uint32_t* edges, int64_t ne,
+                  const uint32_t* faces, int64_t nf, double minkowski, const double* center, const double* quat,
+                  int32_t max_candidates);
+/* Mesh candidates of the current neighbour list with their history, sorted by (owner id, type, feature):
+ * owner id, feature index, type (7, 8, 9), friction [3], moment [3] per row; *written = row count (rows
+ * beyond cap: XDEM_ERR_CAPACITY, nothing written). Null output arrays are skipped. */
+int xdem_mesh_interactions(xdem_ctx* ctx, int64_t cap, uint64_t* id_i, uint32_t* sub_j, uint16_t* type,
+                           double* friction, double* moment, int64_t* written);
+
 #ifdef __cplusplus
 }
 #endif

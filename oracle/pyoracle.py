@@ -36,6 +36,11 @@ def load():
     lib.xo_set_mutation.argtypes = [C.c_int]
     lib.xo_set_surfaces.argtypes = [vp, C.POINTER(Surface), C.c_int]
     lib.xo_rebuild.argtypes = [vp, C.c_double]
+    lib.xo_set_mesh.argtypes = [vp, C.c_uint64, dp, C.c_int64, C.POINTER(C.c_uint32), C.c_int64,
+                                C.POINTER(C.c_uint32), C.c_int64, C.c_double, dp, dp]
+    lib.xo_mesh_interactions.argtypes = [vp, C.c_int64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
+                                         C.POINTER(C.c_uint16), dp, dp]
+    lib.xo_mesh_interactions.restype = C.c_int64
     lib.xo_move_particles.argtypes = [vp]
     lib.xo_chunk_neighbors.argtypes = [vp, C.c_double, C.c_double]
     lib.xo_nbh_sphere.argtypes = [vp, C.c_double]
@@ -175,6 +180,31 @@ class Oracle:
                                           fr.ctypes.data_as(C.POINTER(C.c_double)),
                                           mo.ctypes.data_as(C.POINTER(C.c_double)))
         return ii[:k], jj[:k], tt[:k], fr[:3 * k].reshape(-1, 3), mo[:3 * k].reshape(-1, 3)
+
+    # ---- triangle-mesh driver
+    def set_mesh(self, mesh_id, vertices, edges, faces, minkowski, center=(0.0, 0.0, 0.0), quat=(1.0, 0.0, 0.0, 0.0)):
+        v = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
+        e = np.ascontiguousarray(edges, np.uint32).reshape(-1, 2)
+        f = np.ascontiguousarray(faces, np.uint32).reshape(-1, 3)
+        c = np.ascontiguousarray(center, np.float64)
+        q = np.ascontiguousarray(quat, np.float64)
+        u32 = C.POINTER(C.c_uint32)
+        self.lib.xo_set_mesh(self.h, mesh_id, v.ctypes.data_as(C.POINTER(C.c_double)), len(v), e.ctypes.data_as(u32),
+                             len(e), f.ctypes.data_as(u32), len(f), minkowski, c.ctypes.data_as(C.POINTER(C.c_double)),
+                             q.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def mesh_interactions(self):
+        """(owner id, feature, type, friction [k][3], moment [k][3]) sorted by (owner id, type, feature)."""
+        n = self.lib.xo_mesh_interactions(self.h, 0, None, None, None, None, None)
+        ii = np.zeros(n, np.uint64)
+        sj = np.zeros(n, np.uint32)
+        tt = np.zeros(n, np.uint16)
+        fr = np.zeros(3 * n)
+        mo = np.zeros(3 * n)
+        self.lib.xo_mesh_interactions(self.h, n, ii.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                      sj.ctypes.data_as(C.POINTER(C.c_uint32)), tt.ctypes.data_as(C.POINTER(C.c_uint16)),
+                                      fr.ctypes.data_as(C.POINTER(C.c_double)), mo.ctypes.data_as(C.POINTER(C.c_double)))
+        return ii, sj, tt, fr.reshape(-1, 3), mo.reshape(-1, 3)
 
     # ---- polyhedra (config_polyhedra.msp order)
     def set_shapes(self, shapes):

@@ -261,6 +261,16 @@ struct Oracle {
   int dims[3] = {1, 1, 1};
   std::vector<int64_t> cell_start;  // size ncells+1
   std::vector<Surf> walls;
+  // stationary triangle-mesh driver (register_stl_mesh.cpp:30-140), world-space features
+  struct Mesh {
+    bool on = false;
+    uint64_t id = 0;
+    double rm = 0;
+    V3 center{0, 0, 0};
+    std::vector<V3> v;
+    std::vector<std::array<uint32_t, 2>> e;
+    std::vector<std::array<uint32_t, 3>> f;
+  } mesh;
   // neighbour streams and interaction lists per particle
   std::vector<std::vector<Stream>> streams;
   std::vector<std::vector<Item>> items;  // per owner particle: walls first, then particle pairs
@@ -392,6 +402,95 @@ static void chunk_neighbors(Oracle& o, double rcut_inc, double nbh_dist) {
 }
 
 // nbh_sphere, symetric mode: rebuild per-particle item lists carrying history by key.
+// Sphere against the mesh (the sphere is a vertex of radius R; features carry the Minkowski radius rm):
+// filters and detection of detect<7>, detect<8>, detect<9> (shape_detection_api.hpp:246-309, 357-386;
+// shape_detection_core.hpp:43-53, 57-81, 83-101, 122-148, 397-466), restated for triangles.
+static inline V3 unit3(V3 v) { return v / norm(v); }
+static bool mesh_vf(V3 vi, const V3 (&t)[3], double limit, double& dist_out, V3& n_out, bool& early) {
+  const V3 va = t[0], vb = t[1], vc = t[2];
+  const V3 v = vi - va;
+  V3 v1 = vb - va;
+  V3 v2 = vc - va;
+  v1 = unit3(v1);
+  V3 n = unit3(cross(v1, v2));
+  double dist = dot(n, v);
+  if (dist < 0) {
+    n = n * (-1);
+    dist = -dist;
+  }
+  early = false;
+  if (dist > limit) {
+    early = true;
+    return false;
+  }
+  const V3 P = vi - n * dist;
+  int odd = 0;
+  v2 = cross(n, v1);
+  const double ori1 = dot(P, v1), ori2 = dot(P, v2);
+  for (int iva = 0; iva < 3; ++iva) {
+    const int ivb = iva == 2 ? 0 : iva + 1;
+    const V3 A = t[iva], B = t[ivb];
+    const double pa1 = dot(A, v1), pb1 = dot(B, v1), pa2 = dot(A, v2), pb2 = dot(B, v2);
+    if ((pa2 < ori2 && pb2 >= ori2) || (pb2 < ori2 && pa2 >= ori2)) {
+      if (pa1 + (ori2 - pa2) / (pb2 - pa2) * (pb1 - pa1) < ori1) odd = 1 - odd;
+    }
+  }
+  dist_out = dist;
+  n_out = n;
+  return odd == 1;
+}
+static void mesh_tri(const Oracle& o, uint32_t k, V3 (&t)[3]) {
+  for (int a = 0; a < 3; a++) t[a] = o.mesh.v[o.mesh.f[k][a]];
+}
+// kind 0 vertex, 1 edge, 2 face
+static bool mesh_filter(const Oracle& o, int kind, uint32_t k, V3 p, double R, double rv) {
+  const double rm = o.mesh.rm;
+  if (kind == 0) {
+    const double Rm = R + rm + rv;
+    const V3 d = p - o.mesh.v[k];
+    return dot(d, d) <= Rm * Rm;
+  }
+  if (kind == 1) {
+    const V3 vf = o.mesh.v[o.mesh.e[k][0]], vs = o.mesh.v[o.mesh.e[k][1]];
+    const V3 fs = vs - vf, fi = p - vf;
+    const double r = dot(fs, fi) / dot(fs, fs);
+    if (r < -rv || r > 1.0 + rv) return false;
+    const V3 d = fi - fs * r;
+    const double Rmax = R + rm + rv;
+    return dot(d, d) < Rmax * Rmax;
+  }
+  V3 t[3];
+  mesh_tri(o, k, t);
+  double dist;
+  V3 n;
+  bool early;
+  return mesh_vf(p, t, R + rm + rv, dist, n, early);
+}
+static Contact mesh_detect(const Oracle& o, int kind, uint32_t k, V3 p, double R) {
+  const double rm = o.mesh.rm;
+  if (kind == 0) return detect_vv(p, R, o.mesh.v[k], rm);
+  if (kind == 1) {
+    const V3 vf = o.mesh.v[o.mesh.e[k][0]], vs = o.mesh.v[o.mesh.e[k][1]];
+    const V3 fs = vs - vf, fi = p - vf;
+    const double r = dot(fs, fi) / dot(fs, fs);
+    if (r <= 0 || r >= 1.0) return {false, 0, {0, 0, 0}, {0, 0, 0}};
+    V3 n = fi - fs * r;
+    const double dn = norm(n) - (R + rm);
+    if (dn > 0) return {false, 0, {0, 0, 0}, {0, 0, 0}};
+    n = unit3(n);
+    return {true, dn, n, p - n * (R + 0.5 * dn)};
+  }
+  V3 t[3];
+  mesh_tri(o, k, t);
+  double dist;
+  V3 n;
+  bool early;
+  const bool in = mesh_vf(p, t, R + rm, dist, n, early);
+  if (early) return {false, 0, {0, 0, 0}, {0, 0, 0}};
+  const double dn = dist - (R + rm);
+  return {in, dn, n, p - n * (R + 0.5 * dn)};
+}
+
 static void nbh_sphere(Oracle& o, double rcut_inc) {
   const int64_t n = o.n;
   if ((int64_t)o.items.size() != n) o.items.assign(n, {});
@@ -420,6 +519,22 @@ static void nbh_sphere(Oracle& o, double rcut_inc) {
         it.shift = 13;
         lst.push_back(it);
       }
+    }
+    // mesh driver (detection_sphere_driver, nbh_sphere.hpp:25-86, rVerlet = rcut_inc): vertices, edges,
+    // faces, each by index
+    if (o.mesh.on) {
+      const size_t cnt[3] = {o.mesh.v.size(), o.mesh.e.size(), o.mesh.f.size()};
+      for (int kind = 0; kind < 3; kind++)
+        for (size_t k = 0; k < cnt[kind]; k++) {
+          if (!mesh_filter(o, kind, (uint32_t)k, o.r[x], o.radius[x], rcut_inc)) continue;
+          Item it = base;
+          it.type = (uint16_t)(7 + kind);
+          it.id_j = o.mesh.id;
+          it.sub_j = (uint32_t)k;
+          it.partner = -1;
+          it.shift = 13;
+          lst.push_back(it);
+        }
     }
     // particle pairs in stream order, owner = lower id unless the partner is a periodic image
     for (auto& s : o.streams[x]) {
@@ -481,6 +596,22 @@ static void contact(Oracle& o, const CP& cp, const CP& cpd, double dt) {
           it.friction = {0, 0, 0};
           it.moment = {0, 0, 0};
         }
+      } else if (it.type >= 7) {  // mesh (ContactLawRShapeDriverFunc, contact_sphere.hpp:298-396)
+        const V3 ri = o.r[x];
+        const Contact c = mesh_detect(o, it.type - 7, it.sub_j, ri, o.radius[x]);
+        if (c.contact) {
+          V3 f{0, 0, 0};
+          const V3 zero{0, 0, 0};
+          contact_force_core(c.dn, c.n, dt, cpd, o.mass[x], it.friction, c.cp, ri, o.v[x], f, it.moment, o.vrot[x],
+                             o.mesh.center, zero, zero);
+          it.in_contact = true;
+          it.f = f;
+          it.cp = c.cp;
+          it.mint = it.moment;
+        } else {
+          it.friction = {0, 0, 0};
+          it.moment = {0, 0, 0};
+        }
       } else {  // wall (ContactLawDriverFunc<Surface>, contact_sphere.hpp:216-297)
         const Surf& s = o.walls[it.id_j];
         const V3 ri = o.r[x];
@@ -500,7 +631,7 @@ static void contact(Oracle& o, const CP& cp, const CP& cpd, double dt) {
       }
     }
   }
-  // canonical accumulation: stream order, then walls
+  // canonical accumulation: stream order, then walls, then mesh features
 #pragma omp parallel for schedule(dynamic, 256)
   for (int64_t x = 0; x < n; x++) {
     V3 fx = o.f[x], mx = o.mom[x];
@@ -531,8 +662,8 @@ static void contact(Oracle& o, const CP& cp, const CP& cpd, double dt) {
         }
       }
     }
-    for (auto& it : own) {
-      if (it.type != 5 || !it.in_contact) continue;
+    for (auto& it : own) {  // drivers in list order: walls, then mesh features
+      if (it.type == 0 || !it.in_contact) continue;
       fx = fx + it.f;
       mx = mx + compute_moments(it.cp, o.r[x], it.f, it.mint);
     }
@@ -1203,6 +1334,54 @@ void xo_set_surfaces(void* h, const xdem_surface* w, int nw) {
     s.center_proj = s.offset * s.normal;  // Surface::initialize (surface.hpp:145-150)
     o->walls.push_back(s);
   }
+}
+
+// Mesh driver: world vertex = center + quat (w, x, y, z) applied to vertex k (shape::get_vertex with
+// homothety 1, shape.hpp:259-262). nv = 0 removes it.
+void xo_set_mesh(void* h, uint64_t id, const double* vertices, int64_t nv, const uint32_t* edges, int64_t ne,
+                 const uint32_t* faces, int64_t nf, double rm, const double* center, const double* quat) {
+  Oracle* o = (Oracle*)h;
+  o->mesh = Oracle::Mesh{};
+  if (nv == 0) return;
+  o->mesh.on = true;
+  o->mesh.id = id;
+  o->mesh.rm = rm;
+  o->mesh.center = {center[0], center[1], center[2]};
+  const Q4 q{quat[0], quat[1], quat[2], quat[3]};
+  for (int64_t k = 0; k < nv; k++)
+    o->mesh.v.push_back(o->mesh.center + qrot(q, V3{vertices[3 * k], vertices[3 * k + 1], vertices[3 * k + 2]}));
+  for (int64_t k = 0; k < ne; k++) o->mesh.e.push_back({edges[2 * k], edges[2 * k + 1]});
+  for (int64_t k = 0; k < nf; k++) o->mesh.f.push_back({faces[3 * k], faces[3 * k + 1], faces[3 * k + 2]});
+}
+
+// Mesh items (owner id, feature, type 7/8/9, friction, moment), sorted by (owner id, type, feature).
+int64_t xo_mesh_interactions(void* h, int64_t cap, uint64_t* id_i, uint32_t* sub_j, uint16_t* type, double* fr,
+                             double* mo) {
+  Oracle& o = *(Oracle*)h;
+  std::vector<const Item*> rows;
+  for (auto& l : o.items)
+    for (auto& it : l)
+      if (it.type >= 7) rows.push_back(&it);
+  std::sort(rows.begin(), rows.end(), [](const Item* a, const Item* b) {
+    if (a->id_i != b->id_i) return a->id_i < b->id_i;
+    if (a->type != b->type) return a->type < b->type;
+    return a->sub_j < b->sub_j;
+  });
+  int64_t k = 0;
+  for (const Item* it : rows) {
+    if (k >= cap) break;
+    id_i[k] = it->id_i;
+    sub_j[k] = it->sub_j;
+    type[k] = it->type;
+    fr[3 * k] = it->friction.x;
+    fr[3 * k + 1] = it->friction.y;
+    fr[3 * k + 2] = it->friction.z;
+    mo[3 * k] = it->moment.x;
+    mo[3 * k + 1] = it->moment.y;
+    mo[3 * k + 2] = it->moment.z;
+    k++;
+  }
+  return (int64_t)rows.size();
 }
 
 void xo_rebuild(void* h, double rcut_inc) { rebuild(*(Oracle*)h, rcut_inc); }

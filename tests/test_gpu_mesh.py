@@ -1,0 +1,120 @@
+"""Parity of the stationary triangle-mesh driver acting on spheres (register_stl_mesh; candidates
+detection_sphere_driver, nbh_sphere.hpp:25-86; contact ContactLawRShapeDriverFunc, contact_sphere.hpp:298-396)
+between the HIP path and the CPU oracle.
+
+Input: two jittered layers of spheres resting on a heightfield tray (flat faces, a convex apex vertex,
+convex ridge edges), plus one sphere on the apex and one on a ridge edge, with velocities N(0, 0.3) under
+gravity. Bars: mesh candidates (owner, feature, type) and friction history bit-exact, forces equal to the
+oracle's within 1e-12, trajectories of the operator path (with a neighbour rebuild half-way, which carries
+the mesh history over) and of the engine within 1e-10 after 60 steps.
+"""
+import numpy as np
+import pytest
+
+from exadem_amd import native
+from exadem_amd.sim import SphereSim, scheme
+import pyoracle as po
+
+import mesh_cases as mc
+from helpers import by_id, force, interaction_table, pos, rel_err, vel
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_for(w, m):
+    o = po.Oracle(w.particles, w.domain, w.walls)
+    o.set_mesh(m["mesh_id"], m["vertices"], m["edges"], m["faces"], m["minkowski"])
+    return o
+
+
+def mesh_table(ii, sj, tt, fr):
+    order = np.lexsort((sj, tt, ii))
+    return np.stack([ii[order].astype(np.int64), sj[order].astype(np.int64), tt[order].astype(np.int64)], 1), fr[order]
+
+
+def check_pairs(s, o):
+    kg, fg = interaction_table(*s.ctx.interactions())
+    oi, oj, ot, of, om = o.interactions()
+    keep = ot < 7
+    ko, fo = interaction_table(oi[keep], oj[keep], ot[keep], of[keep])
+    assert np.array_equal(kg, ko)
+    return fg, fo
+
+
+def check_state(g, pq):
+    L = max(1.0, float(np.max(np.abs(pos(pq)))))
+    assert float(np.max(np.abs(pos(g) - pos(pq)))) / L <= 1e-10
+    assert float(np.max(np.abs(vel(g) - vel(pq)))) / max(1e-3, float(np.max(np.abs(vel(pq))))) <= 1e-10
+
+
+@pytest.mark.parametrize("rm", [0.0, 0.02])
+def test_mesh_matches_oracle(gpu_ok, rm):
+    w = mc.workload("tray", mc.tray_bed())
+    m = mc.tray_mesh(rm)
+    assert w.n > 500
+    sch = scheme(w)
+    o = oracle_for(w, m)
+    o.first_iteration(sch)
+    s = SphereSim(w, mesh=m)
+    s.first_iteration()
+    pg, pq = by_id(s.ctx.download()), by_id(o.particles())
+    assert rel_err(force(pg), force(pq), pq["mass"] * 9.81) <= 1e-12
+    kg, fg = mesh_table(*s.ctx.mesh_interactions()[:4])
+    ko, fo = mesh_table(*o.mesh_interactions()[:4])
+    assert np.array_equal(kg, ko) and np.array_equal(fg, fo)
+    active = {t: int(np.count_nonzero((ko[:, 2] == t) & np.any(fo != 0, axis=1))) for t in (7, 8, 9)}
+    assert active[7] >= 1 and active[8] >= 1 and active[9] > 100, active
+    fg2, fo2 = check_pairs(s, o)
+    assert np.array_equal(fg2, fo2)
+    # operator path with a neighbour rebuild after 30 steps (mesh history carried by key)
+    half = w.steps // 2
+    o.run_steps(sch, half)
+    for _ in range(half):
+        s.step()
+    o.rebuild(w.rcut_inc)
+    s.rebuild()
+    o.run_steps(sch, w.steps - half)
+    for _ in range(w.steps - half):
+        s.step()
+    check_state(by_id(s.ctx.download()), by_id(o.particles()))
+    kg, fg = mesh_table(*s.ctx.mesh_interactions()[:4])
+    ko, fo = mesh_table(*o.mesh_interactions()[:4])
+    assert np.array_equal(kg, ko)
+    assert float(np.max(np.abs(fg - fo))) <= 1e-12 * max(1e-30, float(np.max(np.abs(fo))))
+    assert np.count_nonzero(np.any(fo != 0, axis=1)) > 100
+    # engine
+    o2 = oracle_for(w, m)
+    o2.first_iteration(sch)
+    o2.run_steps(sch, w.steps)
+    e = SphereSim(w, mesh=m)
+    e.first_iteration()
+    e.run(w.steps, poll=7)
+    check_state(by_id(e.ctx.download()), by_id(o2.particles()))
+    kg, fg = mesh_table(*e.ctx.mesh_interactions()[:4])
+    ko, fo = mesh_table(*o2.mesh_interactions()[:4])
+    assert np.array_equal(kg, ko)
+    assert float(np.max(np.abs(fg - fo))) <= 1e-12 * max(1e-30, float(np.max(np.abs(fo))))
+    print(f"rm={rm}: {w.n} spheres, active mesh contacts at the first pass {active}")
+
+
+def test_mesh_errors(gpu_ok):
+    """Bad meshes fail with XDEM_ERR_ARG; the FP32 form refuses a mesh; a sphere with more candidate
+    features than the slots fails the neighbour build with XDEM_ERR_CAPACITY."""
+    m = mc.tray_mesh()
+    ctx = native.Context()
+    bad = dict(m, edges=np.array([[0, 10**6]], np.uint32))
+    with pytest.raises(native.XdemError, match="ARG"):
+        ctx.set_mesh(**bad)
+    with pytest.raises(native.XdemError, match="ARG"):
+        ctx.set_mesh(**dict(m, minkowski=-1.0))
+    w = mc.workload("tray", mc.tray_bed())
+    s = SphereSim(w, mesh=m, form=native.XDEM_FORM_FP32)
+    with pytest.raises(native.XdemError, match="ARG"):
+        s.first_iteration()
+    s = SphereSim(w, mesh=dict(m, max_candidates=1))
+    with pytest.raises(native.XdemError, match="CAPACITY"):
+        s.first_iteration()
+    # the same context recovers with enough slots
+    s.ctx.set_mesh(**m)
+    s.first_iteration()
+    assert len(s.ctx.mesh_interactions()[0]) > w.n

@@ -1,0 +1,76 @@
+"""STL reading and mesh building (stl_mesh_reader.hpp:63-135, stl_mesh_to_driver.hpp:25-126) and the
+oracle's sphere-mesh contact against closed forms: a sphere resting on a face, on the apex vertex and on a
+ridge edge of the tray mesh feel kn * overlap along the face normal, the vertex direction and the edge
+bisector; candidates cover exactly the features within R + rm + rcut_inc."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from exadem_amd import stl
+from exadem_amd.sim import scheme
+
+import mesh_cases as mc
+
+
+def test_ascii_binary_same_mesh(tmp_path):
+    tri = mc.tray_triangles()
+    a, b = tmp_path / "t.stl", tmp_path / "tb.stl"
+    stl.write_stl(a, tri)
+    stl.write_stl(b, tri.astype(np.float32).astype(np.float64), binary=True)
+    va, ea, fa = stl.load(a)
+    vb, eb, fb = stl.load(b, binary=True)
+    assert np.array_equal(va, vb) and np.array_equal(ea, eb) and np.array_equal(fa, fb)
+    assert np.array_equal(stl.read_stl(a), tri)
+
+
+def test_build_shape_semantics():
+    tri = mc.tray_triangles()
+    v, e, f = stl.build_shape(tri)
+    # unique vertices in lexicographic order
+    assert len(v) == 11 * 11
+    assert np.all(np.diff(v[:, 0]) >= 0)
+    assert len(np.unique(v, axis=0)) == len(v)
+    # faces in file order, indices reproduce the triangles
+    assert f.shape == (200, 3) and np.array_equal(v[f], tri)
+    # three edges per face: interior edges appear once per adjacent face
+    assert e.shape == (600, 2)
+    und = np.sort(e, axis=1)
+    _, counts = np.unique(und, axis=0, return_counts=True)
+    assert set(counts.tolist()) == {1, 2}
+    assert int((counts == 1).sum()) == 40  # boundary edges of the 10 x 10 tray
+
+
+def test_ascii_rejects_polygons(tmp_path):
+    p = tmp_path / "q.stl"
+    p.write_text("solid q\nfacet normal 0 0 1\n outer loop\n vertex 0 0 0\n vertex 1 0 0\n vertex 1 1 0\n"
+                 " vertex 0 1 0\n endloop\nendfacet\nendsolid q\n")
+    with pytest.raises(ValueError, match="triangles only"):
+        stl.read_stl(p)
+
+
+@pytest.mark.parametrize("where,kind", [("face", 9), ("vertex", 7), ("edge", 8)])
+@pytest.mark.parametrize("rm", [0.0, 0.05])
+def test_oracle_mesh_contact_closed_form(where, kind, rm):
+    R, ov = 0.5, 0.01
+    c = mc.probes(R, ov, rm)[where]
+    w = mc.workload("probe", mc.single(c, R), gravity=(0.0, 0.0, 0.0))
+    o = po.Oracle(w.particles, w.domain, w.walls)
+    m = mc.tray_mesh(rm)
+    o.set_mesh(m["mesh_id"], m["vertices"], m["edges"], m["faces"], rm)
+    o.first_iteration(scheme(w))
+    p = o.particles()
+    F = np.array([p["fx"][0], p["fy"][0], p["fz"][0]])
+    ii, sj, tt, fr, mo = o.mesh_interactions()
+    # kn * overlap along the contact normal per feature in contact; the ridge edge is listed once per
+    # adjacent face (build_shape keeps both), so two vertex-edge interactions act there
+    mult = 2 if where == "edge" else 1
+    assert np.linalg.norm(F) == pytest.approx(mult * 1e5 * ov, rel=1e-9)
+    apex = np.array([10.0, 10.0, 1.0])
+    direction = {"vertex": c - apex, "face": np.array([0.0, 0.0, 1.0]),
+                 "edge": c - (apex + np.array([12.0, 10.0, 0.0])) / 2}[where]
+    assert F / np.linalg.norm(F) == pytest.approx(direction / np.linalg.norm(direction), abs=1e-12)
+    assert kind in tt.tolist()
+    assert np.all(ii == 0) and np.all(np.diff(tt.astype(int) * 10**6 + sj.astype(int)) > 0)
+    # candidates: exactly the features within R + rm + rcut_inc of the centre (vertices by distance)
+    near_v = np.nonzero(np.linalg.norm(m["vertices"] - c, axis=1) <= R + rm + w.rcut_inc)[0]
+    assert np.array_equal(np.sort(sj[tt == 7]), near_v)
