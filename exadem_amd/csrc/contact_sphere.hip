@@ -279,32 +279,7 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
       F = F + f;
       M = M + (cross(cpos - rx, f) + mint);
     }
-    if (a.mesh.cap) {
-      // ContactLawRShapeDriverFunc (contact_sphere.hpp:298-396) over the sphere's mesh candidates in list
-      // order: driver at the mesh centre, stationary; no contact resets the history
-      const uint32_t mc = min(a.ml.cnt[x], (uint32_t)a.mesh.cap);
-      for (uint32_t k = 0; k < mc; k++) {
-        const int64_t j = (int64_t)k * a.mesh.stride + x;
-        double dn;
-        V3 nrm, cpos;
-        if (!mesh_detect(a.mesh, a.ml.word[j], rx, Rx, dn, nrm, cpos)) {
-          a.ml.hx[j] = 0.0; a.ml.hy[j] = 0.0; a.ml.hz[j] = 0.0;
-          a.ml.mhx[j] = 0.0; a.ml.mhy[j] = 0.0; a.ml.mhz[j] = 0.0;
-          continue;
-        }
-        V3 ft{a.ml.hx[j], a.ml.hy[j], a.ml.hz[j]};
-        V3 mint{0.0, 0.0, 0.0};
-        if (KR) mint = {a.ml.mhx[j], a.ml.mhy[j], a.ml.mhz[j]};
-        const V3 zero{0.0, 0.0, 0.0};
-        const V3 f = hooke_force<KR>(dn, nrm, a.dt, a.cpd, mx, ft, mint, cpos, rx, vx, wx, a.mesh.center, zero, zero);
-        a.ml.hx[j] = ft.x; a.ml.hy[j] = ft.y; a.ml.hz[j] = ft.z;
-        if (KR) {
-          a.ml.mhx[j] = mint.x; a.ml.mhy[j] = mint.y; a.ml.mhz[j] = mint.z;
-        }
-        F = F + f;
-        M = M + (cross(cpos - rx, f) + mint);
-      }
-    }
+    if (a.mesh.cap) mesh_contacts<KR>(a.mesh, a.ml, x, rx, Rx, mx, vx, wx, a.dt, a.cpd, F, M);
   }
   if (FUSED && INTEG) {
     const V3 al = angular_accel_d(Q4f{me.q.x, me.q.y, me.q.z, me.q.w}, M, wx, V3{s.ix[x], s.iy[x], s.iz[x]});

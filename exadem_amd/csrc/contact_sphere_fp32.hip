@@ -9,6 +9,7 @@
 // per slot instead of 24). Per-particle sums of the contact forces and moments are accumulated in FP32
 // with TwoSum compensation and added to the FP64 gravity term at the end.
 #include "contact_law.h"
+#include "mesh.h"
 #include "step.h"
 
 namespace xdem {
@@ -97,6 +98,9 @@ struct ForceArgs32 {
   double kdt_2;
   double thr2;
   int32_t trig_out;
+  // triangle-mesh driver (mesh.cap = 0: none)
+  MeshView mesh;
+  MeshList ml;
 };
 
 
@@ -286,7 +290,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   M = M + V3{cval(Mx), cval(My), cval(Mz)};
   if (WALLS) {
     const V3 vx64{me.vm.x, me.vm.y, me.vm.z}, wx64{me.w.x, me.w.y, me.w.z};
-    const uint32_t mask = a.wl.mask[x];
+    const uint32_t mask = a.nw ? a.wl.mask[x] : 0u;
     for (int w = 0; w < a.nw; w++) {
       if (!((mask >> w) & 1u)) continue;
       const Surface& sf = a.walls[w];
@@ -310,6 +314,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       F = F + f;
       M = M + (cross(cpos - rx, f) + mint);
     }
+    if (a.mesh.cap) mesh_contacts<KR>(a.mesh, a.ml, x, rx, Rx, mx, vx64, wx64, a.dt, a.cpd, F, M);
   }
   if (FUSED && INTEG) {
     const V3 al = angular_accel_g(Q4g{me.q.x, me.q.y, me.q.z, me.q.w}, M, V3{me.w.x, me.w.y, me.w.z},
@@ -385,18 +390,23 @@ static void launch32(xdem_ctx* c, const ForceArgs32& a, bool guarded) {
 
 int launch_force_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool fused,
                       bool gravity_on, V3 g, bool guarded, int32_t step_tag) {
-  if (c->has_mesh) return set_err(c, XDEM_ERR_ARG, "mesh drivers: the FP64 form only");
   int rc = set_history_form(c, true);
   if (rc) return rc;
   if (cp.kr != 0.0 || cpd.kr != 0.0) c->kr_used = true;
-  const bool walls = c->nwalls > 0 && c->wl_valid;
+  const bool planes = c->nwalls > 0 && c->wl_valid;
+  const bool meshed = c->has_mesh && c->ml_valid;
+  const bool walls = planes || meshed;
   ForceArgs32 a{};
   a.n = c->n;
   a.s = live(c);
   a.nl = c->nl[c->nl_cur];
-  if (walls) a.wl = c->wl[c->wl_cur];
+  if (planes) a.wl = c->wl[c->wl_cur];
   a.walls = c->d_walls;
-  a.nw = c->nwalls;
+  a.nw = planes ? c->nwalls : 0;
+  if (meshed) {
+    a.mesh = c->mesh;
+    a.ml = c->ml[c->ml_cur];
+  }
   a.wstride = c->cap;
   a.g = c->grid;
   a.cpd = cpd;
@@ -425,18 +435,23 @@ int launch_force_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams&
 // FP32-form engine step in one kernel (see launch_step)
 int launch_step_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool gravity_on, V3 g,
                      double thr2, int32_t step_tag, Core** input) {
-  if (c->has_mesh) return set_err(c, XDEM_ERR_ARG, "mesh drivers: the FP64 form only");
   int rc = set_history_form(c, true);
   if (rc) return rc;
   if (cp.kr != 0.0 || cpd.kr != 0.0) c->kr_used = true;
-  const bool walls = c->nwalls > 0 && c->wl_valid;
+  const bool planes = c->nwalls > 0 && c->wl_valid;
+  const bool meshed = c->has_mesh && c->ml_valid;
+  const bool walls = planes || meshed;
   ForceArgs32 a{};
   a.n = c->n;
   a.s = live(c);
   a.nl = c->nl[c->nl_cur];
-  if (walls) a.wl = c->wl[c->wl_cur];
+  if (planes) a.wl = c->wl[c->wl_cur];
   a.walls = c->d_walls;
-  a.nw = c->nwalls;
+  a.nw = planes ? c->nwalls : 0;
+  if (meshed) {
+    a.mesh = c->mesh;
+    a.ml = c->ml[c->ml_cur];
+  }
   a.wstride = c->cap;
   a.g = c->grid;
   a.cpd = cpd;

@@ -11,7 +11,7 @@
 //               (shape_detection_core.hpp:57-81, 122-148, 397-466)
 #pragma once
 
-#include "xdem_internal.h"
+#include "contact_law.h"
 
 namespace xdem {
 
@@ -130,6 +130,36 @@ __device__ inline bool mesh_detect(const MeshView& m, uint32_t wd, V3 p, double 
   dn = dist - (R + m.rm);
   cp = p - n * (R + 0.5 * dn);
   return in;
+}
+
+// ContactLawRShapeDriverFunc (contact_sphere.hpp:298-396) over sphere x's mesh candidates in list order,
+// in FP64 (both force forms): the driver sits at the mesh centre, stationary; no contact resets the
+// history. Adds the forces and moments to F and M.
+template <bool KR>
+__device__ inline void mesh_contacts(const MeshView& m, const MeshList& l, int64_t x, V3 rx, double Rx, double mx,
+                                     V3 vx, V3 wx, double dt, const ContactParams& cpd, V3& F, V3& M) {
+  const uint32_t mc = min(l.cnt[x], (uint32_t)m.cap);
+  for (uint32_t k = 0; k < mc; k++) {
+    const int64_t j = (int64_t)k * m.stride + x;
+    double dn;
+    V3 nrm, cpos;
+    if (!mesh_detect(m, l.word[j], rx, Rx, dn, nrm, cpos)) {
+      l.hx[j] = 0.0; l.hy[j] = 0.0; l.hz[j] = 0.0;
+      l.mhx[j] = 0.0; l.mhy[j] = 0.0; l.mhz[j] = 0.0;
+      continue;
+    }
+    V3 ft{l.hx[j], l.hy[j], l.hz[j]};
+    V3 mint{0.0, 0.0, 0.0};
+    if (KR) mint = {l.mhx[j], l.mhy[j], l.mhz[j]};
+    const V3 zero{0.0, 0.0, 0.0};
+    const V3 f = hooke_force<KR>(dn, nrm, dt, cpd, mx, ft, mint, cpos, rx, vx, wx, m.center, zero, zero);
+    l.hx[j] = ft.x; l.hy[j] = ft.y; l.hz[j] = ft.z;
+    if (KR) {
+      l.mhx[j] = mint.x; l.mhy[j] = mint.y; l.mhz[j] = mint.z;
+    }
+    F = F + f;
+    M = M + (cross(cpos - rx, f) + mint);
+  }
 }
 
 }  // namespace xdem

@@ -410,8 +410,8 @@ int xdem_surrogate_run(xdem_gnn* gnn, const xdem_scheme* scheme, int64_t nsteps,
  * mesh frame (world vertex = center + quat (w, x, y, z) applied to it), edges [ne][2] and triangles [nf][3]
  * as vertex indices, Minkowski radius, driver index id (partner id of its interactions, types 7 vertex,
  * 8 edge, 9 face). max_candidates: slots per sphere for candidate features (a neighbour build with more
- * fails with XDEM_ERR_CAPACITY). nv = 0 removes the mesh. Takes effect at the next nbh_sphere; the FP64
- * form only; spheres only. */
+ * fails with XDEM_ERR_CAPACITY). nv = 0 removes the mesh. Takes effect at the next nbh_sphere; both
+ * force forms evaluate its contacts in FP64; spheres only. */
 int xdem_set_mesh(xdem_ctx* ctx, int32_t id, const double* vertices, int64_t nv, const uint32_t* edges, int64_t ne,
                   const uint32_t* faces, int64_t nf, double minkowski, const double* center, const double* quat,
                   int32_t max_candidates);

@@ -98,8 +98,8 @@ def test_mesh_matches_oracle(gpu_ok, rm):
 
 
 def test_mesh_errors(gpu_ok):
-    """Bad meshes fail with XDEM_ERR_ARG; the FP32 form refuses a mesh; a sphere with more candidate
-    features than the slots fails the neighbour build with XDEM_ERR_CAPACITY."""
+    """Bad meshes fail with XDEM_ERR_ARG; a sphere with more candidate features than the slots fails the
+    neighbour build with XDEM_ERR_CAPACITY."""
     m = mc.tray_mesh()
     ctx = native.Context()
     bad = dict(m, edges=np.array([[0, 10**6]], np.uint32))
@@ -108,9 +108,6 @@ def test_mesh_errors(gpu_ok):
     with pytest.raises(native.XdemError, match="ARG"):
         ctx.set_mesh(**dict(m, minkowski=-1.0))
     w = mc.workload("tray", mc.tray_bed())
-    s = SphereSim(w, mesh=m, form=native.XDEM_FORM_FP32)
-    with pytest.raises(native.XdemError, match="ARG"):
-        s.first_iteration()
     s = SphereSim(w, mesh=dict(m, max_candidates=1))
     with pytest.raises(native.XdemError, match="CAPACITY"):
         s.first_iteration()
@@ -118,3 +115,24 @@ def test_mesh_errors(gpu_ok):
     s.ctx.set_mesh(**m)
     s.first_iteration()
     assert len(s.ctx.mesh_interactions()[0]) > w.n
+
+
+def test_mesh_fp32_form(gpu_ok):
+    """The FP32-force form evaluates the mesh contacts in FP64, as the walls: same candidates, forces within
+    1e-5 of the FP64 form's at the first pass and after 60 engine steps."""
+    w = mc.workload("tray", mc.tray_bed())
+    m = mc.tray_mesh(0.02)
+    a = SphereSim(w, mesh=m)
+    a.first_iteration()
+    b = SphereSim(w, mesh=m, form=native.XDEM_FORM_FP32)
+    b.first_iteration()
+    pa, pb = by_id(a.ctx.download()), by_id(b.ctx.download())
+    assert rel_err(force(pb), force(pa), pa["mass"] * 9.81) <= 1e-5
+    ka, _ = mesh_table(*a.ctx.mesh_interactions()[:4])
+    kb, _ = mesh_table(*b.ctx.mesh_interactions()[:4])
+    assert np.array_equal(ka, kb)
+    a.run(w.steps, poll=7)
+    b.run(w.steps, poll=7)
+    pa, pb = by_id(a.ctx.download()), by_id(b.ctx.download())
+    L = max(1.0, float(np.max(np.abs(pos(pa)))))
+    assert float(np.max(np.abs(pos(pb) - pos(pa)))) / L <= 1e-9
