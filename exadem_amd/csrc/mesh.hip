@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 
 #include "mesh.h"
 
@@ -379,5 +380,74 @@ extern "C" int xdem_mesh_interactions(xdem_ctx* c, int64_t cap, uint64_t* id_i, 
       moment[3 * r + 2] = h[5][j];
     }
   }
+  return XDEM_OK;
+}
+
+// Restart: saved mesh interaction histories (rows as xdem_mesh_interactions writes them) handed to the
+// current candidate lists by key (owner id, type, feature); unmatched candidates keep a zero history.
+// *matched = rows that found their candidate.
+extern "C" int xdem_mesh_interactions_load(xdem_ctx* c, int64_t m, const uint64_t* id_i, const uint32_t* sub_j,
+                                           const uint16_t* type, const double* friction, const double* moment,
+                                           int64_t* matched) {
+  if (!c || (m > 0 && (!id_i || !sub_j || !type || !friction))) return XDEM_ERR_ARG;
+  if (matched) *matched = 0;
+  if (m == 0) return XDEM_OK;
+  if (!c->has_mesh || !c->ml_valid) return set_err(c, XDEM_ERR_STATE, "mesh_interactions_load: no mesh candidate lists (run nbh_sphere)");
+  XDEM_HIP(c, hipSetDevice(c->device));
+  struct Key {
+    uint64_t id;
+    uint32_t wd;
+    bool operator==(const Key& o) const { return id == o.id && wd == o.wd; }
+  };
+  struct KeyHash {
+    size_t operator()(const Key& k) const { return std::hash<uint64_t>()(k.id * 0x9E3779B97F4A7C15ull ^ k.wd); }
+  };
+  std::unordered_map<Key, int64_t, KeyHash> rows;
+  rows.reserve((size_t)m * 2);
+  for (int64_t r = 0; r < m; r++) {
+    if (type[r] < 7 || type[r] > 9 || sub_j[r] > kMeshIdxMask)
+      return set_err(c, XDEM_ERR_ARG, "mesh_interactions_load: type must be 7, 8 or 9");
+    rows[{id_i[r], ((uint32_t)(type[r] - 7) << kMeshKindShift) | sub_j[r]}] = r;
+  }
+  const int64_t n = c->n, st = c->cap;
+  const MeshList& L = c->ml[c->ml_cur];
+  std::vector<uint32_t> cnt(n);
+  std::vector<uint64_t> ids(n);
+  XDEM_HIP(c, hipMemcpyAsync(cnt.data(), L.cnt, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipMemcpyAsync(ids.data(), live(c).id, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  uint32_t kmax = 0;
+  for (auto v : cnt) kmax = std::max(kmax, std::min(v, (uint32_t)c->mesh_cap));
+  const int64_t tot = (int64_t)kmax * st;
+  if (tot == 0) return XDEM_OK;
+  std::vector<uint32_t> wd(tot);
+  std::vector<double> h[6];
+  double* dst[6] = {L.hx, L.hy, L.hz, L.mhx, L.mhy, L.mhz};
+  XDEM_HIP(c, hipMemcpyAsync(wd.data(), L.word, sizeof(uint32_t) * tot, hipMemcpyDeviceToHost, c->stream));
+  for (int q = 0; q < 6; q++) {
+    h[q].resize(tot);
+    XDEM_HIP(c, hipMemcpyAsync(h[q].data(), dst[q], sizeof(double) * tot, hipMemcpyDeviceToHost, c->stream));
+  }
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  int64_t hits = 0;
+  for (int64_t x = 0; x < n; x++) {
+    const uint32_t k1 = std::min(cnt[x], (uint32_t)c->mesh_cap);
+    for (uint32_t k = 0; k < k1; k++) {
+      const int64_t j = (int64_t)k * st + x;
+      auto it = rows.find({ids[x], wd[j]});
+      if (it == rows.end()) continue;
+      const int64_t r = it->second;
+      for (int a = 0; a < 3; a++) {
+        h[a][j] = friction[3 * r + a];
+        h[3 + a][j] = moment ? moment[3 * r + a] : 0.0;
+      }
+      hits++;
+    }
+  }
+  for (int q = 0; q < 6; q++)
+    XDEM_HIP(c, hipMemcpyAsync(dst[q], h[q].data(), sizeof(double) * tot, hipMemcpyHostToDevice, c->stream));
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  if (moment) c->kr_used = c->kr_used || std::any_of(moment, moment + 3 * m, [](double v) { return v != 0.0; });
+  if (matched) *matched = hits;
   return XDEM_OK;
 }

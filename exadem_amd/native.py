@@ -108,6 +108,7 @@ HIP_SYMBOLS = [
     "xdem_gnn_create", "xdem_gnn_destroy", "xdem_gnn_set_params", "xdem_gnn_get_params", "xdem_gnn_get_grads",
     "xdem_gnn_forward", "xdem_gnn_set_target", "xdem_gnn_backward", "xdem_gnn_adam", "xdem_gnn_train_window",
     "xdem_gnn_dump_graph", "xdem_surrogate_run", "xdem_set_mesh", "xdem_mesh_interactions",
+    "xdem_mesh_interactions_load",
 ]
 
 
@@ -195,6 +196,8 @@ def load():
                                     _dp, _dp, C.c_int32]),
         "xdem_mesh_interactions": (C.c_int, [vp, C.c_int64, _u64p, _u32p, C.POINTER(C.c_uint16), _dp, _dp,
                                              C.POINTER(C.c_int64)]),
+        "xdem_mesh_interactions_load": (C.c_int, [vp, C.c_int64, _u64p, _u32p, C.POINTER(C.c_uint16), _dp, _dp,
+                                                  C.POINTER(C.c_int64)]),
         "xdem_set_contact_record": (C.c_int, [vp, C.c_int32]),
         "xdem_contact_state_count": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "xdem_contact_state_dump": (C.c_int, [vp, C.c_int64, _u64p, _u64p, _u32p, _u32p, C.POINTER(C.c_uint16),
@@ -444,6 +447,19 @@ class Context:
         self.call("xdem_mesh_interactions", n, _ptr(ii, C.c_uint64), _ptr(sj, C.c_uint32), _ptr(tt, C.c_uint16),
                   _ptr(fr, C.c_double), _ptr(mo, C.c_double), C.byref(wr))
         return ii[:n], sj[:n], tt[:n], fr[:3 * n].reshape(-1, 3), mo[:3 * n].reshape(-1, 3)
+
+    def mesh_interactions_load(self, ii, sj, tt, fr, mo=None):
+        """Hands saved mesh histories (rows of mesh_interactions) to the current candidate lists by key;
+        returns the number of rows matched."""
+        ii = np.ascontiguousarray(ii, np.uint64)
+        sj = np.ascontiguousarray(sj, np.uint32)
+        tt = np.ascontiguousarray(tt, np.uint16)
+        fr = np.ascontiguousarray(fr, np.float64).reshape(-1)
+        mop = None if mo is None else _ptr(np.ascontiguousarray(mo, np.float64).reshape(-1), C.c_double)
+        hit = C.c_int64()
+        self.call("xdem_mesh_interactions_load", len(ii), _ptr(ii, C.c_uint64), _ptr(sj, C.c_uint32),
+                  _ptr(tt, C.c_uint16), _ptr(fr, C.c_double), mop, C.byref(hit))
+        return hit.value
 
     def poly_interaction_moments(self):
         """Rolling-moment history [rows][3] of the rows poly_interactions() returns, in the same order."""

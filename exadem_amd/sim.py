@@ -35,6 +35,7 @@ class SphereSim:
         self.ctx = ctx or native.Context(device)
         self.ctx.set_domain(workload.domain)
         self.ctx.set_surfaces(workload.walls)
+        self.mesh = mesh
         if mesh is not None:
             self.ctx.set_mesh(**mesh)
         self.ctx.upload(workload.particles)
@@ -84,20 +85,26 @@ class SphereSim:
         p = self.ctx.download()
         ii, jj, tt, fr = self.ctx.interactions()
         mo = self.ctx.interaction_moments()
-        np.savez(path, ii=ii, jj=jj, tt=tt, fr=fr, mo=mo, **{"p_" + k: v for k, v in p.items()})
+        mesh = {}
+        if self.mesh is not None:  # mesh-driver histories
+            mi, ms, mt, mf, mm = self.ctx.mesh_interactions()
+            mesh = {"m_ii": mi, "m_sj": ms, "m_tt": mt, "m_fr": mf, "m_mo": mm}
+        np.savez(path, ii=ii, jj=jj, tt=tt, fr=fr, mo=mo, **mesh, **{"p_" + k: v for k, v in p.items()})
 
     @classmethod
-    def restart(cls, workload, path, device=0, form=native.XDEM_FORM_FP64, history=True):
+    def restart(cls, workload, path, device=0, form=native.XDEM_FORM_FP64, history=True, mesh=None):
         """A simulation continued from a checkpoint: the saved particles (domain, walls and parameters
         from the workload), lists rebuilt at the saved positions, the saved history handed to the
         current interactions by key (xdem_interactions_load)."""
         d = np.load(path)
         w = copy.copy(workload)
         w.particles = {k[2:]: d[k] for k in d.files if k.startswith("p_")}
-        sim = cls(w, device=device, form=form)
+        sim = cls(w, device=device, form=form, mesh=mesh)
         sim.rebuild()
         if history:
             sim.ctx.interactions_load(d["ii"], d["jj"], d["tt"], d["fr"], d["mo"])
+            if mesh is not None and "m_ii" in d.files:
+                sim.ctx.mesh_interactions_load(d["m_ii"], d["m_sj"], d["m_tt"], d["m_fr"], d["m_mo"])
         return sim
 
 

@@ -420,6 +420,12 @@ int xdem_set_mesh(xdem_ctx* ctx, int32_t id, const double* vertices, int64_t nv,
  * beyond cap: XDEM_ERR_CAPACITY, nothing written). Null output arrays are skipped. */
 int xdem_mesh_interactions(xdem_ctx* ctx, int64_t cap, uint64_t* id_i, uint32_t* sub_j, uint16_t* type,
                            double* friction, double* moment, int64_t* written);
+/* Restart: m saved mesh rows (as xdem_mesh_interactions writes them; moment may be null) handed to the
+ * current candidate lists by key (owner id, type, feature) after a neighbour build; *matched = rows that
+ * found their candidate. No counterpart call in the reference, whose restart dump carries the history
+ * (sim_dump_writer_interaction.cpp:25-46). */
+int xdem_mesh_interactions_load(xdem_ctx* ctx, int64_t m, const uint64_t* id_i, const uint32_t* sub_j,
+                                const uint16_t* type, const double* friction, const double* moment, int64_t* matched);
 
 #ifdef __cplusplus
 }
