@@ -6,14 +6,15 @@
 //
 // Per particle: kinetic energy 1/2 m v*v and rotation energy 1/2 m w*w per component (the reference
 // weights the spin by the mass), mass, count. Per interaction (owner entries of the neighbour list,
-// counted twice as the reference counts particle pairs under symetric: true; walls once): the total,
+// counted twice as the reference counts particle pairs under symetric: true; walls and mesh features
+// once): the total,
 // the active ones (overlap dn < 0 or non-zero friction history) and the smallest dn over the active
 // ones (0 when none overlaps). Blocks reduce in a fixed tree order and the host adds the block
 // partials in block order, so the result does not depend on scheduling.
 #include <algorithm>
 #include <cmath>
 
-#include "xdem_internal.h"
+#include "mesh.h"
 
 namespace xdem {
 
@@ -21,7 +22,7 @@ constexpr int kSimVals = 10;  // ke (3), re (3), mass, interactions, active inte
 
 __global__ __launch_bounds__(256) void k_sim_state(int64_t n, PState s, NbrList nl, Grid g, WallList wl,
                                                    const Surface* __restrict__ walls, int nw, int64_t wstride,
-                                                   double* __restrict__ part) {
+                                                   MeshView mesh, MeshList ml, double* __restrict__ part) {
   __shared__ double red[kSimVals][256];
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double v[kSimVals];
@@ -69,6 +70,20 @@ __global__ __launch_bounds__(256) void k_sim_state(int64_t n, PState s, NbrList 
         }
       }
     }
+    if (mesh.cap) {  // mesh candidates, once each
+      const uint32_t mc = min(ml.cnt[x], (uint32_t)mesh.cap);
+      for (uint32_t k = 0; k < mc; k++) {
+        const int64_t j = (int64_t)k * mesh.stride + x;
+        double dn = 0.0;
+        V3 nrm, cpos;
+        const bool over = mesh_detect(mesh, ml.word[j], rx, Rx, dn, nrm, cpos) && dn < 0.0;
+        v[7] += 1.0;
+        if (over || ml.hx[j] != 0.0 || ml.hy[j] != 0.0 || ml.hz[j] != 0.0) {
+          v[8] += 1.0;
+          v[9] = fmin(v[9], over ? dn : 0.0);
+        }
+      }
+    }
   }
 #pragma unroll
   for (int q = 0; q < kSimVals; q++) red[q][threadIdx.x] = v[q];
@@ -107,8 +122,14 @@ extern "C" int xdem_simulation_state(xdem_ctx* c, xdem_sim_state* out) {
   const bool walls = c->nwalls > 0 && c->wl_valid && nl.cnt;
   WallList wl{};
   if (walls) wl = c->wl[c->wl_cur];
+  MeshView mesh{};
+  MeshList ml{};
+  if (!poly && nl.cnt && c->has_mesh && c->ml_valid) {
+    mesh = c->mesh;
+    ml = c->ml[c->ml_cur];
+  }
   hipLaunchKernelGGL(k_sim_state, dim3(blocks), dim3(256), 0, c->stream, n, live(c), nl, c->grid, wl, c->d_walls,
-                     walls ? c->nwalls : 0, c->cap, c->simpart);
+                     walls ? c->nwalls : 0, c->cap, mesh, ml, c->simpart);
   XDEM_HIP(c, hipGetLastError());
   std::vector<double> h((size_t)blocks * kSimVals);
   XDEM_HIP(c, hipMemcpyAsync(h.data(), c->simpart, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));

@@ -167,3 +167,66 @@ def test_mesh_checkpoint_restart(gpu_ok, tmp_path):
     c = SphereSim.restart(w, ck, mesh=m, history=False)
     c.run(40)
     assert not np.array_equal(by_id(c.ctx.download())["vx"], pb["vx"])
+
+
+def mesh_dn(m, rows_t, rows_s, p, R):
+    """dn of each mesh row (type 7/8/9, feature) for spheres p [k][3] of radius R, None without contact:
+    vertex distance, edge distance for projections strictly inside the edge, face distance for
+    projections inside the triangle; minus R + rm."""
+    v, e, f, rm = m["vertices"], m["edges"], m["faces"], m["minkowski"]
+    out = []
+    for t, k, x, r in zip(rows_t, rows_s, p, R):
+        if t == 7:
+            dn = float(np.linalg.norm(x - v[k])) - (r + rm)
+            out.append(dn if dn < 0 else None)
+        elif t == 8:
+            a, b = v[e[k][0]], v[e[k][1]]
+            s = float(np.dot(b - a, x - a) / np.dot(b - a, b - a))
+            dn = float(np.linalg.norm(x - a - (b - a) * s)) - (r + rm)
+            out.append(dn if 0 < s < 1 and dn <= 0 else None)
+        else:
+            a, b, c = v[f[k][0]], v[f[k][1]], v[f[k][2]]
+            n = np.cross(b - a, c - a)
+            n = n / np.linalg.norm(n)
+            h = float(np.dot(x - a, n))
+            q = x - h * n
+            T = np.stack([b - a, c - a], 1)
+            uv = np.linalg.lstsq(T, q - a, rcond=None)[0]
+            inside = uv[0] > 0 and uv[1] > 0 and uv.sum() < 1
+            dn = abs(h) - (r + rm)
+            out.append(dn if inside and dn <= 0 else None)
+    return out
+
+
+def test_mesh_simulation_state(gpu_ok):
+    """simulation_state counts each mesh candidate once (pairs twice): totals exact, active = overlapping
+    or with history, the smallest dn over pairs and mesh contacts from the positions."""
+    w = mc.workload("tray", mc.tray_bed())
+    m = mc.tray_mesh(0.02)
+    s = SphereSim(w, mesh=m)
+    s.first_iteration()
+    s.run(20)
+    st = s.ctx.simulation_state()
+    p = by_id(s.ctx.download())
+    r, R = pos(p), p["radius"]
+    ii, jj, tt, fr = s.ctx.interactions()
+    tot = act = 0
+    min_dn = 0.0
+    for k in range(len(ii)):
+        a, b = int(ii[k]), int(jj[k])
+        dn = float(np.linalg.norm(r[a] - r[b])) - (R[a] + R[b])
+        tot += 2
+        if dn < 0 or np.any(fr[k] != 0):
+            act += 2
+            min_dn = min(min_dn, dn if dn < 0 else 0.0)
+    mi, ms, mt, mf, _ = s.ctx.mesh_interactions()
+    dns = mesh_dn(m, mt, ms, r[mi.astype(np.int64)], R[mi.astype(np.int64)])
+    for k, dn in enumerate(dns):
+        tot += 1
+        if (dn is not None and dn < 0) or np.any(mf[k] != 0):
+            act += 1
+            min_dn = min(min_dn, dn if dn is not None and dn < 0 else 0.0)
+    print(f"mesh: {st['n_interactions']} interactions ({len(mi)} mesh), {st['n_active']} active, "
+          f"min dn {st['min_dn']:.6g}")
+    assert st["n_interactions"] == tot and st["n_active"] == act
+    assert abs(st["min_dn"] - min_dn) <= 1e-12 and min_dn < 0
