@@ -68,9 +68,10 @@ struct ForceArgs {
   // (wall, particle), kCStateLen doubles each (null: not recorded)
   double* cstate;
   double* wstate;
-  // triangle-mesh driver (mesh.cap = 0: none)
+  // triangle-mesh driver (mesh.cap = 0: none) and its recorded contact state (null: not recorded)
   MeshView mesh;
   MeshList ml;
+  double* mstate;
 };
 
 
@@ -279,7 +280,8 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
       F = F + f;
       M = M + (cross(cpos - rx, f) + mint);
     }
-    if (a.mesh.cap) mesh_contacts<KR>(a.mesh, a.ml, x, rx, Rx, mx, vx, wx, a.dt, a.cpd, F, M);
+    if (a.mesh.cap)
+      mesh_contacts<KR>(a.mesh, a.ml, x, rx, Rx, mx, vx, wx, a.dt, a.cpd, F, M, INTEG ? nullptr : a.mstate);
   }
   if (FUSED && INTEG) {
     const V3 al = angular_accel_d(Q4f{me.q.x, me.q.y, me.q.z, me.q.w}, M, wx, V3{s.ix[x], s.iy[x], s.iz[x]});
@@ -438,6 +440,14 @@ int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd,
       }
       a.wstate = c->wstate;
     }
+    if (meshed) {
+      const int64_t m = (int64_t)c->mesh_cap * c->cap;
+      if (m > c->cap_mstate) {
+        if ((rc = dev_realloc(c, &c->mstate, m * kCStateLen))) return rc;
+        c->cap_mstate = m;
+      }
+      a.mstate = c->mstate;
+    }
   }
   if (fused) launch_f<true>(c, a, c->kr_used, walls, guarded);
   else launch_f<false>(c, a, c->kr_used, walls, guarded);
@@ -514,9 +524,38 @@ static int contact_state_rows(xdem_ctx* c, std::vector<uint64_t>& ii, std::vecto
         jj.push_back((uint64_t)w);
         si.push_back(0);
         sj.push_back(0);
-        tt.push_back(5);
+        tt.push_back(driver_type(c->walls[w]));
         vals.insert(vals.end(), v, v + kCStateLen);
       }
+  if (c->has_mesh && c->ml_valid && c->mstate) {  // mesh features: type 7 + kind, partner = driver index
+    const MeshList& ML = c->ml[c->ml_cur];
+    const int64_t st = c->cap;
+    std::vector<uint32_t> mcnt(n);
+    XDEM_HIP(c, hipMemcpyAsync(mcnt.data(), ML.cnt, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipStreamSynchronize(c->stream));
+    uint32_t kmax = 0;
+    for (auto v : mcnt) kmax = std::max(kmax, std::min(v, (uint32_t)c->mesh_cap));
+    const int64_t tot = (int64_t)kmax * st;
+    std::vector<uint32_t> mw(tot);
+    std::vector<double> ms((size_t)tot * kCStateLen);
+    if (tot) {
+      XDEM_HIP(c, hipMemcpyAsync(mw.data(), ML.word, sizeof(uint32_t) * tot, hipMemcpyDeviceToHost, c->stream));
+      XDEM_HIP(c, hipMemcpyAsync(ms.data(), c->mstate, sizeof(double) * ms.size(), hipMemcpyDeviceToHost, c->stream));
+      XDEM_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    for (int64_t x = 0; x < n; x++)
+      for (uint32_t k = 0; k < std::min(mcnt[x], (uint32_t)c->mesh_cap); k++) {
+        const int64_t j = (int64_t)k * st + x;
+        const double* v = ms.data() + (size_t)kCStateLen * j;
+        if (!keep(v)) continue;
+        ii.push_back(ids[x]);
+        jj.push_back((uint64_t)c->mesh_id);
+        si.push_back(0);
+        sj.push_back(mw[j] & kMeshIdxMask);
+        tt.push_back((uint16_t)(7 + (mw[j] >> kMeshKindShift)));
+        vals.insert(vals.end(), v, v + kCStateLen);
+      }
+  }
   return XDEM_OK;
 }
 }  // namespace xdem

@@ -134,10 +134,12 @@ __device__ inline bool mesh_detect(const MeshView& m, uint32_t wd, V3 p, double 
 
 // ContactLawRShapeDriverFunc (contact_sphere.hpp:298-396) over sphere x's mesh candidates in list order,
 // in FP64 (both force forms): the driver sits at the mesh centre, stationary; no contact resets the
-// history. Adds the forces and moments to F and M.
+// history. Adds the forces and moments to F and M. cs (null: not recorded): contact state per slot,
+// kCStateLen doubles (dn, contact point, f - ft, ft, sphere centre, zeros; zeros without contact).
 template <bool KR>
 __device__ inline void mesh_contacts(const MeshView& m, const MeshList& l, int64_t x, V3 rx, double Rx, double mx,
-                                     V3 vx, V3 wx, double dt, const ContactParams& cpd, V3& F, V3& M) {
+                                     V3 vx, V3 wx, double dt, const ContactParams& cpd, V3& F, V3& M,
+                                     double* cs = nullptr) {
   const uint32_t mc = min(l.cnt[x], (uint32_t)m.cap);
   for (uint32_t k = 0; k < mc; k++) {
     const int64_t j = (int64_t)k * m.stride + x;
@@ -146,6 +148,11 @@ __device__ inline void mesh_contacts(const MeshView& m, const MeshList& l, int64
     if (!mesh_detect(m, l.word[j], rx, Rx, dn, nrm, cpos)) {
       l.hx[j] = 0.0; l.hy[j] = 0.0; l.hz[j] = 0.0;
       l.mhx[j] = 0.0; l.mhy[j] = 0.0; l.mhz[j] = 0.0;
+      if (cs) {
+        double* r = cs + (int64_t)kCStateLen * j;
+#pragma unroll
+        for (int q = 0; q < kCStateLen; q++) r[q] = 0.0;
+      }
       continue;
     }
     V3 ft{l.hx[j], l.hy[j], l.hz[j]};
@@ -156,6 +163,14 @@ __device__ inline void mesh_contacts(const MeshView& m, const MeshList& l, int64
     l.hx[j] = ft.x; l.hy[j] = ft.y; l.hz[j] = ft.z;
     if (KR) {
       l.mhx[j] = mint.x; l.mhy[j] = mint.y; l.mhz[j] = mint.z;
+    }
+    if (cs) {
+      const V3 fnv = f - ft;
+      const double rec[kCStateLen] = {dn,   cpos.x, cpos.y, cpos.z, fnv.x, fnv.y, fnv.z, ft.x,
+                                      ft.y, ft.z,   rx.x,   rx.y,   rx.z,   0.0,   0.0,   0.0};
+      double* r = cs + (int64_t)kCStateLen * j;
+#pragma unroll
+      for (int q = 0; q < kCStateLen; q++) r[q] = rec[q];
     }
     F = F + f;
     M = M + (cross(cpos - rx, f) + mint);

@@ -50,6 +50,9 @@ void mesh_free(xdem_ctx* c) {
   c->d_mesh_start = nullptr;
   c->d_mesh_feat = nullptr;
   c->d_mesh_over = nullptr;
+  if (c->mstate) (void)hipFree(c->mstate);
+  c->mstate = nullptr;
+  c->cap_mstate = 0;
   free_ml(c->ml[0]);
   free_ml(c->ml[1]);
   c->ml_alloc = 0;

@@ -1255,7 +1255,7 @@ int poly_contact_state_rows(xdem_ctx* c, std::vector<uint64_t>& ii, std::vector<
         jj.push_back((uint64_t)(bit / XDEM_MAX_SHAPE_VERTICES));
         si.push_back((uint32_t)(bit % XDEM_MAX_SHAPE_VERTICES));
         sj.push_back(0xffffffffu);  // the driver side's sub-feature is -1 (nbh_polyhedron_driver.hpp:220)
-        tt.push_back(5);
+        tt.push_back(driver_type(c->walls[bit / XDEM_MAX_SHAPE_VERTICES]));
         vals.insert(vals.end(), v, v + kCStateLen);
       }
   return XDEM_OK;

@@ -67,6 +67,8 @@ struct Surface {
   V3 axis;  // cylinder: 0/1 mask of the coordinates the projection keeps
 };
 
+// Interaction type of a driver contact (interaction_enum.hpp:36-38): 4 cylinder, 5 surface, 6 ball.
+inline uint16_t driver_type(const Surface& s) { return s.kind == 1 ? 4 : (s.kind == 2 ? 6 : 5); }
 // Candidate test of the interaction update for a point p (a sphere centre or a polyhedron vertex)
 // with cut-off rc: Surface::filter (surface.hpp:320-324), Cylinder::filter (cylinder.hpp:225-234),
 // Ball::filter (ball.hpp:292-296).
@@ -440,6 +442,8 @@ struct xdem_ctx {
   bool ml_valid = false;
   int64_t ml_alloc = 0;             // particle capacity ml was allocated for
   int32_t* d_mesh_over = nullptr;   // [2] largest candidate count of the last build, max radius bits
+  double* mstate = nullptr;         // recorded contact state per mesh slot (kCStateLen doubles each)
+  int64_t cap_mstate = 0;
 };
 
 // helpers shared between translation units

@@ -146,3 +146,16 @@ def test_driver_errors(gpu_ok):
         ctx.call("xdem_set_surfaces", arr, 1)
     with pytest.raises(native.XdemError, match="CAPACITY"):
         ctx.set_surfaces([{"offset": float(k), "normal": (0, 0, 1)} for k in range(9)])
+
+
+def test_driver_types_in_dump(gpu_ok):
+    """Interaction dumps type driver contacts by driver (interaction_enum.hpp:36-38): 4 cylinder,
+    5 surface, 6 ball."""
+    for case, want, must in (("drum", {4, 5}, 4), ("balls", {6}, 6)):
+        w = CASES[case]()
+        s = SphereSim(w)
+        s.ctx.set_contact_record(True)
+        s.first_iteration()
+        ii, jj, si, sj, tt, vals = s.ctx.contact_state(subs=True)
+        got = set(tt[tt != 0].tolist())
+        assert must in got and got <= want, (case, got)

@@ -230,3 +230,28 @@ def test_mesh_simulation_state(gpu_ok):
           f"min dn {st['min_dn']:.6g}")
     assert st["n_interactions"] == tot and st["n_active"] == act
     assert abs(st["min_dn"] - min_dn) <= 1e-12 and min_dn < 0
+
+
+def test_mesh_contact_dump(gpu_ok):
+    """The recorded contact state (the interaction dump) lists the mesh contacts: partner = the mesh's
+    driver index, sub_j = the feature, type 7/8/9; exactly the candidates in contact, with dn from the
+    positions and ft equal to the friction history."""
+    w = mc.workload("tray", mc.tray_bed())
+    m = mc.tray_mesh(0.02)
+    s = SphereSim(w, mesh=m)
+    s.ctx.set_contact_record(True)
+    s.first_iteration()
+    ii, jj, si, sj, tt, vals = s.ctx.contact_state(subs=True)
+    sel = tt >= 7
+    assert np.all(jj[sel] == m["mesh_id"]) and np.all(si[sel] == 0)
+    mi, ms, mt, mf, _ = s.ctx.mesh_interactions()
+    hist = {(int(a), int(b), int(c)): k for k, (a, b, c) in enumerate(zip(mi, mt, ms))}
+    p = by_id(s.ctx.download())
+    r, R = pos(p), p["radius"]
+    dns = mesh_dn(m, mt, ms, r[mi.astype(np.int64)], R[mi.astype(np.int64)])
+    assert int(sel.sum()) == sum(d is not None for d in dns) > 100
+    for a, t, f, v in zip(ii[sel], tt[sel], sj[sel], vals[sel]):
+        k = hist[(int(a), int(t), int(f))]
+        assert dns[k] is not None and abs(v[0] - dns[k]) <= 1e-12
+        assert np.array_equal(v[7:10], mf[k])
+        assert np.array_equal(v[10:13], r[int(a)]) and np.all(v[13:16] == 0)
