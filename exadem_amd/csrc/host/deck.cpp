@@ -11,7 +11,7 @@
 //   input      particle_type, set_fields (src/field/set_fields.cu), init_rsa (src/rsa/init_rsa.cpp),
 //              read_xyz (example/spheres/movable-wall input format), polydisperse_pack (this library's
 //              seeded RSA + relaxation packing of the benchmark inputs)
-//   drivers    register_surface, register_cylinder, register_ball (src/driver/register_*.cpp;
+//   drivers    register_surface, register_cylinder, register_ball, register_stl_mesh (src/driver/register_*.cpp;
 //              STATIONARY motion only, a spinning cylinder or ball through vrot)
 //   scheme     combined_compute_prolog, combined_compute_epilog, reset_force_moment, gravity_force,
 //              force_to_accel, particle_displ_over, vertex_displ_over, compute_vertices
@@ -28,9 +28,12 @@
 #include <filesystem>
 #include <fstream>
 #include <functional>
+#include <algorithm>
+#include <array>
 #include <map>
 #include <set>
 #include <sstream>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -217,6 +220,7 @@ class Runner {
     N["register_surface"] = [this](const Node& s) { register_surface(s); };
     N["register_cylinder"] = [this](const Node& s) { register_round(s, XDEM_DRIVER_CYLINDER, "register_cylinder"); };
     N["register_ball"] = [this](const Node& s) { register_round(s, XDEM_DRIVER_BALL, "register_ball"); };
+    N["register_stl_mesh"] = [this](const Node& s) { register_stl_mesh(s); };
     // ---- scheme operators
     N["combined_compute_prolog"] = [this](const Node&) { check(xdem_combined_compute_prolog(ctx_, dt()), "prolog"); };
     N["combined_compute_epilog"] = [this](const Node&) { check(xdem_combined_compute_epilog(ctx_, dt()), "epilog"); };
@@ -489,6 +493,132 @@ class Runner {
     walls_[id] = w;
   }
 
+  // register_stl_mesh (src/driver/register_stl_mesh.cpp:30-140) for a stationary .stl mesh acting on
+  // spheres: STLMeshReader (stl_mesh_reader.hpp:63-135) and build_shape (stl_mesh_to_driver.hpp:25-126) as it
+  // runs on one thread (vertices sorted and unique, faces in file order, three edges per face with
+  // consecutive repeats removed); scale multiplies the vertices. One mesh; STATIONARY with zero vel and
+  // vrot only; .shp meshes and deform are not provided.
+  struct MeshIn {
+    bool on = false;
+    int32_t id = 0;
+    std::vector<double> v;
+    std::vector<uint32_t> e, f;
+    double rm = 0, center[3] = {0, 0, 0}, quat[4] = {1, 0, 0, 0};
+  } mesh_;
+
+  static std::vector<std::array<double, 3>> read_stl(const std::string& path, bool binary, const std::string& what) {
+    std::vector<std::array<double, 3>> t;
+    if (binary) {
+      std::ifstream in(path, std::ios::binary);
+      char head[80];
+      uint32_t n = 0;
+      if (!in.read(head, 80) || !in.read((char*)&n, 4)) throw std::runtime_error(what + ": truncated binary STL " + path);
+      for (uint32_t k = 0; k < n; k++) {
+        float rec[12];
+        uint16_t attr;
+        if (!in.read((char*)rec, 48) || !in.read((char*)&attr, 2))
+          throw std::runtime_error(what + ": truncated binary STL " + path);
+        for (int q = 1; q < 4; q++) t.push_back({(double)rec[3 * q], (double)rec[3 * q + 1], (double)rec[3 * q + 2]});
+      }
+      return t;
+    }
+    std::ifstream in(path);
+    std::string tok;
+    bool inside = false;
+    std::vector<std::array<double, 3>> cur;
+    while (in >> tok) {
+      if (tok == "outer") {
+        inside = true;
+        cur.clear();
+      } else if (tok == "vertex" && inside) {
+        std::array<double, 3> x;
+        if (!(in >> x[0] >> x[1] >> x[2])) throw std::runtime_error(what + ": bad vertex line in " + path);
+        cur.push_back(x);
+      } else if (tok == "endloop" && inside) {
+        if (cur.size() != 3) throw std::runtime_error(what + ": a face with " + std::to_string(cur.size()) +
+                                                      " vertices in " + path + " (triangles only)");
+        t.insert(t.end(), cur.begin(), cur.end());
+        inside = false;
+      }
+    }
+    return t;
+  }
+
+  void register_stl_mesh(const Node& s) {
+    const char* what = "register_stl_mesh";
+    if (mesh_.on) fail(XDEM_ERR_CAPACITY, "register_stl_mesh: one mesh per run");
+    if (const Node* pr = slot(s, "params"))
+      if (const Node* m = slot(*pr, "motion_type"))
+        if (m->s != "STATIONARY") fail(XDEM_ERR_UNSUPPORTED, "register_stl_mesh: only STATIONARY meshes are provided");
+    if (slot(s, "deform")) fail(XDEM_ERR_UNSUPPORTED, "register_stl_mesh: deform is not provided");
+    const Node* f = slot(s, "filename");
+    if (!f) fail(XDEM_ERR_ARG, "register_stl_mesh: filename is required");
+    if (f->s.find(".stl") == std::string::npos)
+      fail(XDEM_ERR_UNSUPPORTED, "register_stl_mesh: only .stl meshes are provided (" + f->s + ")");
+    if (!slot(s, "minkowski")) fail(XDEM_ERR_ARG, "register_stl_mesh: minkowski is required");
+    mesh_.rm = sdouble(s, "minkowski", 0.0);
+    mesh_.id = (int32_t)sdouble(s, "id", (double)walls_.size());
+    if (const Node* st = slot(s, "state")) {
+      auto vec = [&](const char* k, double* out, size_t nval) {
+        if (const Node* v = slot(*st, k)) {
+          const auto x = xdem_host::to_doubles(*v);
+          if (x.size() != nval) fail(XDEM_ERR_ARG, std::string(what) + ": " + k + " needs " + std::to_string(nval) + " values");
+          for (size_t a = 0; a < nval; a++) out[a] = x[a];
+        }
+      };
+      vec("center", mesh_.center, 3);
+      vec("quat", mesh_.quat, 4);
+      double vel[3] = {0, 0, 0}, vrot[3] = {0, 0, 0};
+      vec("vel", vel, 3);
+      vec("vrot", vrot, 3);
+      for (int a = 0; a < 3; a++)
+        if (vel[a] != 0 || vrot[a] != 0) fail(XDEM_ERR_UNSUPPORTED, "register_stl_mesh: moving meshes are not provided");
+    }
+    std::string path = f->s;
+    if (!std::ifstream(path)) {
+      for (auto& d : search_dirs)
+        if (std::ifstream(d + "/" + path)) {
+          path = d + "/" + path;
+          break;
+        }
+    }
+    if (!std::ifstream(path)) fail(XDEM_ERR_ARG, "register_stl_mesh: cannot open " + f->s);
+    const Node* b = slot(s, "binary");
+    std::vector<std::array<double, 3>> t;
+    try {
+      t = read_stl(path, b && xdem_host::to_bool(*b), what);
+    } catch (const std::exception& ex) {
+      fail(XDEM_ERR_ARG, ex.what());
+    }
+    // build_shape
+    std::vector<std::array<double, 3>> v(t);
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    std::vector<uint32_t> fi(t.size());
+    for (size_t k = 0; k < t.size(); k++) fi[k] = (uint32_t)(std::lower_bound(v.begin(), v.end(), t[k]) - v.begin());
+    std::vector<std::array<uint32_t, 2>> e;
+    for (size_t k = 0; k < t.size(); k += 3)
+      for (int a = 0; a < 3; a++) {
+        const std::array<uint32_t, 2> ed{fi[k + a], fi[k + (a + 1) % 3]};
+        if (e.empty() || e.back() != ed) e.push_back(ed);
+      }
+    double scale = 1.0;
+    if (slot(s, "scale")) {
+      scale = sdouble(s, "scale", 1.0);
+      if (!(scale > 0)) fail(XDEM_ERR_ARG, "register_stl_mesh: scale <= 0");
+    }
+    mesh_.v.clear();
+    for (auto& x : v)
+      for (int a = 0; a < 3; a++) mesh_.v.push_back(scale == 1.0 ? x[a] : x[a] * scale);
+    mesh_.e.clear();
+    for (auto& ed : e) {
+      mesh_.e.push_back(ed[0]);
+      mesh_.e.push_back(ed[1]);
+    }
+    mesh_.f = fi;
+    mesh_.on = true;
+  }
+
   // ---------------------------------------------------------------- domain + upload
   void setup_domain() {
     const Node* d = deck.get("domain");
@@ -535,6 +665,11 @@ class Runner {
     p.inx = P_.inx.data(); p.iny = P_.iny.data(); p.inz = P_.inz.data();
     check(xdem_particles_upload(ctx_, &p), "upload");
     if (!walls_.empty()) check(xdem_set_surfaces(ctx_, walls_.data(), (int32_t)walls_.size()), "surfaces");
+    if (mesh_.on)
+      check(xdem_set_mesh(ctx_, mesh_.id, mesh_.v.data(), (int64_t)mesh_.v.size() / 3, mesh_.e.data(),
+                          (int64_t)mesh_.e.size() / 2, mesh_.f.data(), (int64_t)mesh_.f.size() / 3, mesh_.rm,
+                          mesh_.center, mesh_.quat, 32),
+            "mesh");
     uploaded_ = true;
   }
 

@@ -212,3 +212,40 @@ def test_deck_cylinder_drum_matches_operator_path(gpu_ok, tmp_path):
         assert np.array_equal(a[k], b[k]), k
     ii, jj, tt, fr = ctx.interactions()
     assert np.count_nonzero((tt == 5) & (jj == 0) & np.any(fr != 0, axis=1)) > 10  # contacts with the drum
+
+
+@pytest.mark.gpu
+def test_deck_stl_mesh_matches_operator_path(gpu_ok, tmp_path):
+    """register_stl_mesh (a stationary .stl tray, minkowski 0.02) in a deck: the deck run equals the
+    Python operator path with the same mesh bit for bit; a moving mesh is refused."""
+    import copy
+    import mesh_cases as mc
+    from exadem_amd import stl
+    w = copy.copy(mc.workload("tray", mc.tray_bed()))
+    p = dict(w.particles)
+    for k in ("vx", "vy", "vz"):
+        p[k] = np.zeros_like(p[k])  # set_fields gives every sphere velocity 0
+    w.particles = p
+    steps = 40
+    stl.write_stl(tmp_path / "tray.stl", mc.tray_triangles(), binary=True)
+    path = wall_deck(tmp_path, w, steps)
+    text = open(path).read()
+    i, j = text.index("setup_drivers:\n"), text.index("compute_force:")
+    mesh = f"  - register_stl_mesh: {{ id: 6, filename: {tmp_path / 'tray.stl'}, binary: true, minkowski: 0.02 }}\n"
+    deck = native.Deck(write(tmp_path / "tray.msp", text[:i] + "setup_drivers:\n" + mesh + text[j:]))
+    ctx = native.Context()
+    st = deck.run(ctx)
+    assert st["iterations"] == steps
+    ref = SphereSim(w, mesh=mc.tray_mesh(0.02))
+    ref.first_iteration()
+    for _ in range(steps):
+        ref.step()
+    a, b = by_id(ctx.download()), by_id(ref.ctx.download())
+    for k in ("rx", "ry", "rz", "vx", "vy", "vz", "vrotx", "fx", "fz", "momx"):
+        assert np.array_equal(a[k], b[k]), k
+    mi, ms, mt, mf, _ = ctx.mesh_interactions()
+    assert np.count_nonzero(np.any(mf != 0, axis=1)) > 100
+    moving = mesh.replace("minkowski: 0.02", "minkowski: 0.02, state: { vel: [0, 0, -1] }")
+    bad = native.Deck(write(tmp_path / "moving.msp", text[:i] + "setup_drivers:\n" + moving + text[j:]))
+    with pytest.raises(native.XdemError, match="UNSUPPORTED"):
+        bad.run(native.Context())
