@@ -67,6 +67,9 @@ struct Q4 {
 
 // Mutation switch used by the oracle's own self-tests: a non-zero value plants one known
 // mistake so that the closed-form checks can show they catch it. Always 0 in normal use.
+//   1 damping sign, 2 friction increment sign, 3 no Coulomb cap, 4 history dropped at rebuild,
+//   5 polyhedron swap side ignored in the force assembly, 6 the vertex side's moment arm taken from
+//   the vertex instead of the contact point, 7 polyhedron meff = m_i.
 static int g_mutation = 0;
 
 // ---------------------------------------------------------------- quaternion helpers
@@ -980,6 +983,7 @@ static void compute_vertices(Oracle& o) {
 struct PolyEval {
   bool in_contact;
   V3 f, cp, mint;
+  V3 arm_i;     // point whose arm gives the vertex side's moment (the contact point)
   bool x_is_i;  // the list holder is the evaluation's particle i (swap = 0)
   V3 ri, rj;    // centre of i and j in the owner's frame
 };
@@ -1148,7 +1152,7 @@ static PolyEval eval_pp(Oracle& o, int64_t x, Item& it, const CP& cp, double dt)
   e.rj = r_j;
   if (c.contact) {
     const double mi = o.mass[pi], mj = o.mass[pj];
-    const double meff = (mi * mj) / (mi + mj);
+    const double meff = g_mutation == 7 ? mi : (mi * mj) / (mi + mj);
     V3 f{0, 0, 0};
     contact_force_core(c.dn, c.n, dt, cp, meff, it.friction, c.cp, r_i, o.v[pi], f, it.moment, o.vrot[pi], r_j,
                        o.v[pj], o.vrot[pj]);
@@ -1156,6 +1160,8 @@ static PolyEval eval_pp(Oracle& o, int64_t x, Item& it, const CP& cp, double dt)
     e.f = f;
     e.cp = c.cp;
     e.mint = it.moment;
+    // planted mistake 6: the moment arm of the vertex side taken from the vertex, not the contact point
+    e.arm_i = (g_mutation == 6 && it.type != 3) ? vi[fi] : c.cp;
   } else {
     it.friction = {0, 0, 0};
     it.moment = {0, 0, 0};
@@ -1207,9 +1213,9 @@ static void contact_poly(Oracle& o, const CP& cp, const CP& cpd, double dt) {
           if (it.type == 5 || it.partner != y || it.shift != s.shift) continue;
           const PolyEval& e = ev[x][k];
           if (!e.in_contact) continue;
-          if (e.x_is_i) {
+          if (e.x_is_i || g_mutation == 5) {  // planted mistake 5: the swap side ignored
             Fp = Fp + e.f;
-            Mp = Mp + compute_moments(e.cp, e.ri, e.f, e.mint);
+            Mp = Mp + compute_moments(e.arm_i, e.ri, e.f, e.mint);
           } else {
             const V3 nf = -e.f;
             Fp = Fp + nf;
@@ -1223,13 +1229,13 @@ static void contact_poly(Oracle& o, const CP& cp, const CP& cpd, double dt) {
           if (it.type == 5 || it.partner != x || it.shift != mirror) continue;
           const PolyEval& e = ev[y][k];
           if (!e.in_contact) continue;
-          if (e.x_is_i) {  // y is i, x is j
+          if (e.x_is_i || g_mutation == 5) {  // y is i, x is j
             const V3 nf = -e.f;
             Fp = Fp + nf;
             Mp = Mp + compute_moments(e.cp, e.rj, nf, -e.mint);
           } else {  // x is i (vertex side of a swap item held by y)
             Fp = Fp + e.f;
-            Mp = Mp + compute_moments(e.cp, e.ri, e.f, e.mint);
+            Mp = Mp + compute_moments(e.arm_i, e.ri, e.f, e.mint);
           }
         }
       }

@@ -266,3 +266,214 @@ def test_feature_list_contains_every_detected_contact():
     assert found > 50
     # pairs are held once, by the smaller id
     assert np.all(ii[pp] < jj[pp])
+
+
+# ------------------------------------------------------------------------ contact assembly
+# The force and moment that polyhedron contacts put onto both bodies (polyhedron/contact.hpp:124-211 for
+# particle pairs: +f on the item's particle i, the vertex side, -f on j, moments (cp - r) x (+-f) from each
+# centre, meff = mi mj / (mi + mj); contact.hpp:219-297 for a vertex against a driver: f on the particle,
+# moment (cp - r) x f). Checked against rigid-body statics and kinematics worked out here, against the
+# conservation of momentum and angular momentum, and with planted mistakes that must be caught.
+KN = 1e5
+
+
+def _poly_set(specs, shapes_defs):
+    """Particle dictionary for polyhedra: specs = [(type, position, quaternion, velocity, spin)], ids in
+    list order, density 2500, homothety 1."""
+    n = len(specs)
+    pos = np.array([s[1] for s in specs], float)
+    rb = np.array([shapes.bounding_radius(shapes_defs[s[0]]) for s in specs])
+    p = inputs.sphere_set(pos[:, 0], pos[:, 1], pos[:, 2], rb, 2500.0)
+    typ = np.array([s[0] for s in specs], np.uint32)
+    p["type"] = typ
+    p["mass"] = 2500.0 * np.array([shapes_defs[t].volume for t in typ])
+    for a in ("inx", "iny", "inz"):
+        p[a] = p["mass"] * np.array([shapes_defs[t].inertia_on_mass[0] for t in typ])
+    for k, key in enumerate(("qw", "qx", "qy", "qz")):
+        p[key] = np.array([s[2][k] for s in specs], float)
+    for k, key in enumerate(("vx", "vy", "vz")):
+        p[key] = np.array([s[3][k] for s in specs], float)
+    for k, key in enumerate(("vrotx", "vroty", "vrotz")):
+        p[key] = np.array([s[4][k] for s in specs], float)
+    return p
+
+
+def _first_pass(p, shapes_defs, walls=(), box=((0, 0, 0), (12, 12, 12)), dt=1e-5, rcut_inc=0.1):
+    """nbh_polyhedron + reset_force_moment + one contact_polyhedron pass; returns the particle fields by
+    id and the interaction list."""
+    from exadem_amd.native import contact_params, domain
+    cell = 2 * float(np.max(p["radius"])) + rcut_inc
+    o = pyoracle.Oracle(p, domain(box[0], box[1], (False, False, False), cell), list(walls))
+    o.set_shapes([d.to_record() for d in shapes_defs])
+    o.rebuild_poly(rcut_inc)
+    o.reset_force_moment()
+    cp = contact_params(kn=KN, mu=0.3, damp_rate=0.9)
+    o.contact_poly(cp, cp, dt)
+    q = o.particles()
+    order = np.argsort(q["id"])
+    out = {k: v[order] for k, v in q.items()}
+    return out, o.poly_interactions()
+
+
+def _vec(q, key, i):
+    return np.array([q[key + a][i] for a in "xyz"])
+
+
+def test_cube_face_down_on_a_floor():
+    """A cube (edge 1, Minkowski radius 0.02) at rest on a floor wall at overlap d: its four bottom vertices
+    each feel kn d, so F = (0, 0, 4 kn d), and the symmetric contact points leave no moment."""
+    cube = shapes.cube(1.0, 0.02)
+    d = 2e-3
+    c = (5.0, 5.0, 0.5 + 0.02 - d)
+    p = _poly_set([(0, c, IDQ, (0, 0, 0), (0, 0, 0))], [cube])
+    q, inter = _first_pass(p, [cube], walls=[{"offset": 0.0, "normal": (0.0, 0.0, 1.0)}])
+    F, M = _vec(q, "f", 0), _vec(q, "mom", 0)
+    assert F[2] == pytest.approx(4 * KN * d, rel=1e-12)
+    assert abs(F[0]) + abs(F[1]) < 1e-9 * F[2]
+    assert np.max(np.abs(M)) < 1e-9 * F[2]
+    # the vertex-wall candidates are the four bottom vertices (type 5, partner = wall 0)
+    assert sorted(inter[4][inter[2] == 5].tolist()) == [0, 1, 2, 3]
+
+
+@pytest.mark.parametrize("moving", [False, True])
+def test_cube_on_one_vertex(moving):
+    """A tilted cube touching the floor with one vertex only. At rest: F = kn d along the floor normal
+    and M = (cp - r) x F with cp = vertex - n (r_m + dn / 2). Moving and spinning: the contact point's
+    velocity v + w x (cp - r) sets the damping and friction, and still M = (cp - r) x F."""
+    cube = shapes.cube(1.0, 0.02)
+    rm, d, dt = 0.02, 1e-3, 1e-5
+    qc = quat_axis((1.0, -2.0, 0.5), 0.9)
+    R = rotmat(qc)
+    body = np.array(cube.vertices)
+    z = (body @ R.T)[:, 2]
+    low = int(np.argmin(z))
+    assert np.sort(z)[1] - z[low] > 0.15  # one vertex well below the others
+    c = np.array([5.0, 5.0, 0.0])
+    c[2] = rm - d - z[low]
+    v = np.array([0.3, -0.2, -0.1]) if moving else np.zeros(3)
+    w = np.array([1.0, 2.0, -0.5]) if moving else np.zeros(3)
+    p = _poly_set([(0, c, qc, v, w)], [cube])
+    q, _ = _first_pass(p, [cube], walls=[{"offset": 0.0, "normal": (0.0, 0.0, 1.0)}], dt=dt)
+    F, M = _vec(q, "f", 0), _vec(q, "mom", 0)
+    vert = c + R @ body[low]
+    n = np.array([0.0, 0.0, 1.0])
+    dn = vert[2] - rm
+    assert dn == pytest.approx(-d, abs=1e-12)
+    cpt = vert - n * (rm + 0.5 * dn)
+    m = float(p["mass"][0])
+    if not moving:
+        assert F == pytest.approx([0.0, 0.0, KN * d], rel=1e-12, abs=1e-9)
+    else:
+        vc = v + np.cross(w, cpt - c)  # velocity of the body at the contact point; the floor is at rest
+        vrel = -vc
+        vn = vrel @ n
+        damp = 0.9 * 2.0 * math.sqrt(KN * m)
+        fn = max(0.0, KN * d + damp * vn) * n
+        ft = 0.8 * KN * dt * (vrel - vn * n)
+        assert np.linalg.norm(ft) < 0.3 * KN * d  # below the Coulomb cap
+        assert F == pytest.approx(fn + ft, rel=1e-9, abs=1e-9 * np.linalg.norm(fn))
+    assert M == pytest.approx(np.cross(cpt - c, F), rel=1e-9, abs=1e-9 * np.linalg.norm(F))
+
+
+@pytest.mark.parametrize("small_owns", [True, False])
+def test_cube_on_cube_face_contact(small_owns):
+    """A small cube (edge 0.5) turned about z and set off-centre on a big cube's top face (edge 1), pressing
+    by d and closing at speed u. Its four bottom vertices touch the big top face: four vertex-face items,
+    held by the lower id, so swap = 0 when the small cube owns the pair and 1 when the big one does. Each
+    contact gives f = (kn d + damp u) z with damp = 0.9 * 2 sqrt(kn meff), meff = ms mb / (ms + mb), +f on
+    the small cube and -f on the big one; the small cube's moments cancel, the big cube takes
+    -4 (cp_mean - c_big) x f. Both orders give the same forces and moments."""
+    small, big = shapes.cube(0.5, 0.02), shapes.cube(1.0, 0.02)
+    rm, d, u, dt = 0.02, 1e-3, 0.01, 1e-5
+    qs = quat_axis((0.0, 0.0, 1.0), 0.3)
+    cb = np.array([5.0, 5.0, 5.0])
+    cs = np.array([5.1, 4.95, 5.0 + 0.5 + 0.25 + 2 * rm - d])
+    spec_s = (0, cs, qs, (0.0, 0.0, -u), (0.0, 0.0, 0.0))
+    spec_b = (1, cb, IDQ, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))
+    specs = [spec_s, spec_b] if small_owns else [spec_b, spec_s]
+    p = _poly_set(specs, [small, big])
+    q, inter = _first_pass(p, [small, big], dt=dt)
+    i_s, i_b = (0, 1) if small_owns else (1, 0)
+    ii, jj, tt, sw, si, sj, _ = inter
+    pp = tt < 4
+    assert sorted(tt[pp].tolist()) == [2, 2, 2, 2] or np.count_nonzero((tt == 2) & pp) == 4
+    vf = pp & (tt == 2)
+    assert np.all(sw[vf] == (0 if small_owns else 1))
+    ms, mb = float(p["mass"][i_s]), float(p["mass"][i_b])
+    meff = ms * mb / (ms + mb)
+    fz = KN * d + 0.9 * 2.0 * math.sqrt(KN * meff) * u
+    Fs, Fb = _vec(q, "f", i_s), _vec(q, "f", i_b)
+    Ms, Mb = _vec(q, "mom", i_s), _vec(q, "mom", i_b)
+    assert Fs == pytest.approx([0.0, 0.0, 4 * fz], rel=1e-12, abs=1e-9 * fz)
+    assert Fb == pytest.approx(-Fs, rel=1e-14, abs=1e-12 * fz)
+    assert np.max(np.abs(Ms)) < 1e-9 * fz
+    verts = cs + np.array(small.vertices)[:4] @ rotmat(qs).T
+    cp_mean = verts.mean(axis=0) - np.array([0.0, 0.0, rm - 0.5 * d])
+    f = np.array([0.0, 0.0, fz])
+    assert Mb == pytest.approx(-4 * np.cross(cp_mean - cb, f), rel=1e-9, abs=1e-9 * fz)
+
+
+def _loose_packing(n=343, seed=21):
+    """Gravity-free, wall-free cubes and octahedra on a tight jittered lattice (many VV/VE/VF/EE contacts of
+    both swap sides) with random velocities and spins, non-periodic."""
+    w = inputs.polyhedra(n=n, spacing=1.42, jitter=0.08, seed=seed, periodic=(False, False, False), z0=1.5,
+                         sigma_v=0.3)
+    p = {k: v.copy() for k, v in w.particles.items()}
+    spin = inputs.normal(seed + 7, 3, 3 * w.n, 0.0, 2.0).reshape(3, w.n)
+    p["vrotx"], p["vroty"], p["vrotz"] = (np.ascontiguousarray(spin[a]) for a in range(3))
+    return w, p
+
+
+def _balance(w, p):
+    """Total force and total angular momentum source sum(r x F + M) of one pass, each relative to its
+    sum of magnitudes, and the (type, swap) pairs of the contacts met."""
+    from exadem_amd.native import contact_params
+    o = pyoracle.Oracle(p, w.domain, [])
+    o.set_shapes(w.shapes)
+    o.rebuild_poly(w.rcut_inc)
+    o.reset_force_moment()
+    cp = contact_params(kn=KN, mu=0.3, damp_rate=0.9)
+    o.contact_poly(cp, cp, 5e-5)  # a long step: friction forces of a few percent of the normal ones
+    q = o.particles()
+    r = np.stack([q["rx"], q["ry"], q["rz"]], 1)
+    F = np.stack([q["fx"], q["fy"], q["fz"]], 1)
+    M = np.stack([q["momx"], q["momy"], q["momz"]], 1)
+    L = np.cross(r, F) + M
+    fs = np.linalg.norm(F.sum(0)) / np.sum(np.linalg.norm(F, axis=1))
+    ls = np.linalg.norm(L.sum(0)) / np.sum(np.linalg.norm(np.cross(r, F), axis=1) + np.linalg.norm(M, axis=1))
+    _, _, tt, sw, _, _, fr = o.poly_interactions()
+    act = np.any(fr != 0, axis=1) & (tt < 4)
+    return fs, ls, set(zip(tt[act].tolist(), sw[act].tolist()))
+
+
+def test_packing_conserves_momentum_and_angular_momentum():
+    """Every contact puts +f and -f at one contact point into two bodies, so over a whole packing without
+    gravity, walls or periodic images sum F = 0 and sum (r x F + M) = 0 to round-off, whatever the shapes,
+    feature types, swap sides, velocities and spins."""
+    w, p = _loose_packing()
+    fs, ls, kinds = _balance(w, p)
+    assert {(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (3, 0)} <= kinds or len(kinds) >= 5, kinds
+    assert fs < 1e-13
+    assert ls < 1e-13
+
+
+@pytest.mark.parametrize("mutation", [5, 6, 7])
+def test_planted_assembly_mistakes_are_caught(mutation):
+    """Planted in the oracle: 5 the swap side ignored (the vertex side of a swap item pushed the wrong way,
+    moments from the wrong centres), 6 the vertex side's moment arm taken from the vertex instead of the
+    contact point, 7 meff = m_i. Each must fail the conservation test or the cube-on-cube statics."""
+    caught = []
+    pyoracle.set_mutation(mutation)
+    try:
+        w, p = _loose_packing()
+        fs, ls, _ = _balance(w, p)
+        caught.append(fs > 1e-12 or ls > 1e-12)  # round-off leaves ~1e-17
+        for small_owns in (True, False):
+            try:
+                test_cube_on_cube_face_contact(small_owns)
+                caught.append(False)
+            except AssertionError:
+                caught.append(True)
+    finally:
+        pyoracle.set_mutation(0)
+    assert any(caught), mutation
