@@ -239,3 +239,10 @@ def polyhedra(n=200_000, spacing=1.7, jitter=0.05, seed=4, density=2500.0, minko
                  {"spacing": spacing, "box": (L, L, top), "nx": nx})
     w.shapes = shapes
     return w
+
+
+def polyhedra_benchmark(n=200_000):
+    """The polyhedra benchmark input: n cubes and octahedra 50/50 (edge 1.0, circumradius 0.8, Minkowski
+    radius 0.02), uniform random orientations (seed 4), jittered lattice of spacing 1.55 (solid fraction
+    about 0.25), periodic in x and y with a floor and a lid, gravity -9.81 z, dt 2e-6."""
+    return polyhedra(n=n, spacing=1.55, jitter=0.05, seed=4, dt=2e-6)

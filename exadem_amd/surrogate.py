@@ -161,6 +161,56 @@ def window_errors(sim, gnn, substeps):
     return err, base
 
 
+def _min_image(d, L, periodic):
+    for a in range(3):
+        if periodic[a]:
+            d[:, a] -= L[a] * np.round(d[:, a] / L[a])
+    return d
+
+
+def trajectory_errors(w, params, scales, nsteps, check_every=20, tol=0.01):
+    """Accumulated position error of the drift-checked surrogate run against plain DEM over the same
+    simulated time from the same start.
+
+    Three runs of workload w start from the same state (first_iteration, then one settling window of K
+    DEM steps): plain DEM for nsteps * K steps; the surrogate-accelerated run (xdem_surrogate_run, a DEM
+    check every check_every steps with tolerance tol); and the same run with a network of zero
+    parameters, whose output is zero contact force, i.e. free flight under gravity between the same DEM
+    checks (the method without the network). Returns a dict with max and RMS |x - x_DEM| / R over the
+    particles at the end, for the network and for free flight, and both runs' statistics."""
+    K = scales["substeps"]
+    sch = scheme(w)
+    L = [w.domain.hi[a] - w.domain.lo[a] for a in range(3)]
+    per = [bool(w.domain.periodic[a]) for a in range(3)]
+
+    def final_positions(kind):
+        s = SphereSim(w)
+        s.first_iteration()
+        settle(s, K)
+        st = None
+        if kind == "dem":
+            s.run(nsteps * K)
+        else:
+            g = attach(s.ctx, params if kind == "net" else np.zeros_like(params), scales)
+            st = g.surrogate_run(sch, nsteps, check_every=check_every, tol=tol)
+            del g
+        p = s.ctx.download()
+        o = np.argsort(p["id"])
+        x = np.stack([p["rx"][o], p["ry"][o], p["rz"][o]], 1)
+        r = p["radius"][o]
+        del s
+        return x, r, st
+
+    xd, r, _ = final_positions("dem")
+    out = {"surrogate_steps": nsteps, "substeps": K, "check_every": check_every, "tol": tol}
+    for kind in ("net", "free"):
+        x, _, st = final_positions(kind)
+        e = np.linalg.norm(_min_image(x - xd, L, per), axis=1) / r
+        tag = "network" if kind == "net" else "free_flight"
+        out[tag] = {"max": float(np.max(e)), "rms": float(np.sqrt(np.mean(e * e))), "stats": st}
+    return out
+
+
 def main():
     import argparse
     ap = argparse.ArgumentParser(description="train the learned DEM surrogate on GPU-DEM trajectories")
