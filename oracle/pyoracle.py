@@ -62,6 +62,9 @@ def load():
     lib.xo_interactions_count.restype = C.c_int64
     lib.xo_interactions_dump.argtypes = [vp, C.c_int64, u64p, u64p, C.POINTER(C.c_uint16), dp, dp]
     lib.xo_interactions_dump.restype = C.c_int64
+    lib.xo_interactions_restore.argtypes = [vp, C.c_int64, u64p, u64p, C.POINTER(C.c_uint16), C.POINTER(C.c_uint8),
+                                            C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), dp, dp]
+    lib.xo_interactions_restore.restype = C.c_int64
     lib.xo_streams_count.argtypes = [vp]
     lib.xo_streams_count.restype = C.c_int64
     lib.xo_streams_dump.argtypes = [vp, C.c_int64, u64p, u64p, C.POINTER(C.c_int32)]
@@ -180,6 +183,25 @@ class Oracle:
                                           fr.ctypes.data_as(C.POINTER(C.c_double)),
                                           mo.ctypes.data_as(C.POINTER(C.c_double)))
         return ii[:k], jj[:k], tt[:k], fr[:3 * k].reshape(-1, 3), mo[:3 * k].reshape(-1, 3)
+
+    def restore_interactions(self, ii, jj, tt, fr, mo=None, sw=None, si=None, sj=None):
+        """Restart: saved interaction rows (a checkpoint's dump) become their owners' interaction storage;
+        the next rebuild hands the active ones to the new interactions by key. Returns the rows kept."""
+        m = len(ii)
+        ii = np.ascontiguousarray(ii, np.uint64)
+        jj = np.ascontiguousarray(jj, np.uint64)
+        tt = np.ascontiguousarray(tt, np.uint16)
+        fr = np.ascontiguousarray(fr, np.float64).reshape(-1)
+        arrs = {"mo": None if mo is None else np.ascontiguousarray(mo, np.float64).reshape(-1),
+                "sw": None if sw is None else np.ascontiguousarray(sw, np.uint8),
+                "si": None if si is None else np.ascontiguousarray(si, np.uint32),
+                "sj": None if sj is None else np.ascontiguousarray(sj, np.uint32)}
+        ptr = {"mo": C.c_double, "sw": C.c_uint8, "si": C.c_uint32, "sj": C.c_uint32}
+        p = {k: (None if v is None else v.ctypes.data_as(C.POINTER(ptr[k]))) for k, v in arrs.items()}
+        return int(self.lib.xo_interactions_restore(self.h, m, ii.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                    jj.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                    tt.ctypes.data_as(C.POINTER(C.c_uint16)), p["sw"], p["si"],
+                                                    p["sj"], fr.ctypes.data_as(C.POINTER(C.c_double)), p["mo"]))
 
     # ---- triangle-mesh driver
     def set_mesh(self, mesh_id, vertices, edges, faces, minkowski, center=(0.0, 0.0, 0.0), quat=(1.0, 0.0, 0.0, 0.0)):

@@ -42,6 +42,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../include/xdem.h"
@@ -1493,6 +1494,41 @@ int64_t xo_interactions_dump(void* h, int64_t cap, uint64_t* id_i, uint64_t* id_
       k++;
     }
   return k;
+}
+
+// Restart with interaction history (sim_dump_writer_interaction.cpp:25-46 writes, per particle, its
+// interaction storage with the friction and moment payload; the restarted run's first nbh_sphere /
+// nbh_polyhedron extracts the active ones and hands them to the new interactions by key,
+// interaction_manager.hpp:88-112,205-217). Saved rows become the storage of their owner particle (id_i)
+// before the caller's rebuild; rows whose owner is not a particle of this oracle are ignored. swap,
+// sub_i, sub_j and mo may be null (zero).
+int64_t xo_interactions_restore(void* h, int64_t m, const uint64_t* id_i, const uint64_t* id_j, const uint16_t* type,
+                                const uint8_t* swap, const uint32_t* sub_i, const uint32_t* sub_j, const double* fr,
+                                const double* mo) {
+  Oracle& o = *(Oracle*)h;
+  std::unordered_map<uint64_t, int64_t> at;
+  at.reserve((size_t)o.n * 2);
+  for (int64_t x = 0; x < o.n; x++) at[o.id[x]] = x;
+  o.items.assign(o.n, {});
+  int64_t used = 0;
+  for (int64_t k = 0; k < m; k++) {
+    const auto it = at.find(id_i[k]);
+    if (it == at.end()) continue;
+    Item t{};
+    t.id_i = id_i[k];
+    t.id_j = id_j[k];
+    t.type = type[k];
+    t.swap = swap ? swap[k] : 0;
+    t.sub_i = sub_i ? sub_i[k] : 0;
+    t.sub_j = sub_j ? sub_j[k] : 0;
+    t.partner = -1;
+    t.shift = 13;
+    t.friction = {fr[3 * k], fr[3 * k + 1], fr[3 * k + 2]};
+    t.moment = mo ? V3{mo[3 * k], mo[3 * k + 1], mo[3 * k + 2]} : V3{0, 0, 0};
+    o.items[it->second].push_back(t);
+    used++;
+  }
+  return used;
 }
 
 // Stream (neighbour list) dump: particle id, neighbour id, image code; full both-direction streams.

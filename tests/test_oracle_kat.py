@@ -14,7 +14,7 @@ from exadem_amd import inputs, native
 from exadem_amd.sim import scheme
 import pyoracle as po
 
-from helpers import by_id
+from helpers import by_id, wall_deposition
 
 KN = 1e5
 CP = native.contact_params(kn=KN, kt=0.8 * KN, mu=0.3, damp_rate=0.9)
@@ -280,6 +280,47 @@ def test_history_survives_forced_rebuild():
     assert all(not np.any(before[k] != 0) for k in set(before) - set(after))
     assert all(not np.any(after[k] != 0) for k in set(after) - set(before))
     assert len(set(before) ^ set(after)) < 0.01 * len(before)
+
+
+def test_restart_restores_history_by_key():
+    """A restart (particles and interaction rows of a checkpoint, the rows restored as their owners'
+    interaction storage, then a rebuild) is, bit for bit, the uninterrupted run with a rebuild at the
+    checkpoint step: the same interactions (pairs and walls) with the same friction and moment history
+    right after, the same state 40 steps later. Restoring no history gives another state."""
+    w = wall_deposition()
+    sch = scheme(w)
+    a = po.Oracle(w.particles, w.domain, w.walls)
+    a.first_iteration(sch)
+    a.run_steps(sch, 40)
+    ii, jj, tt, fr, mo = a.interactions()
+    act = np.any(fr != 0, axis=1)
+    assert np.count_nonzero(act & (tt == 0)) > 50 and np.count_nonzero(act & (tt == 5)) > 10
+    snap = a.particles()
+
+    def restarted(rows):
+        b = po.Oracle(snap, w.domain, w.walls)
+        if rows is not None:
+            assert b.restore_interactions(*rows) == len(ii)
+        b.rebuild(w.rcut_inc)
+        return b
+
+    def table(o):
+        i, j, t, f, m = o.interactions()
+        order = np.lexsort((t, j, i))
+        return np.stack([i, j, t], 1)[order], f[order], m[order]
+
+    b = restarted((ii, jj, tt, fr, mo))
+    c = restarted(None)
+    a.rebuild(w.rcut_inc)
+    ka, fa, ma = table(a)
+    kb, fb, mb = table(b)
+    assert np.array_equal(ka, kb) and np.array_equal(fa, fb) and np.array_equal(ma, mb)
+    for o in (a, b, c):
+        o.run_steps(sch, 40)
+    pa, pb, pc = (by_id(o.particles()) for o in (a, b, c))
+    for k in ("rx", "ry", "rz", "vx", "vy", "vz", "vrotx", "fx", "momx"):
+        assert np.array_equal(pa[k], pb[k]), k
+    assert not np.array_equal(pa["vx"], pc["vx"])
 
 
 def test_wall_contact_closed_form():
