@@ -13,10 +13,70 @@
 // partials in block order, so the result does not depend on scheduling.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "mesh.h"
 
 namespace xdem {
+
+// Block partials and results of xdem_simulation_state_noalloc, one copy per GPU, held until the process
+// ends. Host code never passes `scratch` itself to a kernel or a copy: it is a host-side stand-in; the
+// device address of each GPU's copy comes from hipGetSymbolAddress at context set-up
+// (noalloc_scratch_init) and is kept per device in g_scratch.
+constexpr int64_t kScratchDoubles = 196608;  // 1.5 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static double* g_scratch[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return XDEM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return XDEM_OK;
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess || cur != device) return XDEM_ERR_STATE;
+  double* ptr = nullptr;
+  if (hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return XDEM_ERR_HIP;
+  g_scratch[device] = ptr;
+  return XDEM_OK;
+}
+
+static double* scratch_of(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch[device];
+}
+
+// out[q] for q < nvals: the block partials part[k * nvals + q] folded in block order k = 0, 1, ..., as
+// the host does in xdem_simulation_state (sums from 0; the last value a minimum from 0 as std::min folds
+// it), so both entry points give the same bits.
+__global__ __launch_bounds__(64) void k_fold_partials(const double* __restrict__ part, int64_t nblocks, int nvals,
+                                                      double* __restrict__ out) {
+  const int q = (int)threadIdx.x;
+  if (q >= nvals) return;
+  const bool is_min = q == nvals - 1;
+  double acc = 0.0;
+  constexpr int kAhead = 16;
+  int64_t k = 0;
+  for (; k + kAhead <= nblocks; k += kAhead) {
+    double v[kAhead];
+#pragma unroll
+    for (int u = 0; u < kAhead; u++) v[u] = part[(k + u) * nvals + q];
+#pragma unroll
+    for (int u = 0; u < kAhead; u++) acc = is_min ? (v[u] < acc ? v[u] : acc) : acc + v[u];
+  }
+  for (; k < nblocks; k++) {
+    const double v = part[k * nvals + q];
+    acc = is_min ? (v < acc ? v : acc) : acc + v;
+  }
+  out[q] = acc;
+}
+
+int fold_partials(xdem_ctx* c, const double* part, int64_t nblocks, int nvals, double* out_dev) {
+  hipLaunchKernelGGL(k_fold_partials, dim3(1), dim3(64), 0, c->stream, part, nblocks, nvals, out_dev);
+  XDEM_HIP(c, hipGetLastError());
+  return XDEM_OK;
+}
 
 constexpr int kSimVals = 10;  // ke (3), re (3), mass, interactions, active interactions, min dn
 
@@ -103,8 +163,11 @@ __global__ __launch_bounds__(256) void k_sim_state(int64_t n, PState s, NbrList 
 
 using namespace xdem;
 
-extern "C" int xdem_simulation_state(xdem_ctx* c, xdem_sim_state* out) {
-  if (!c || !out) return XDEM_ERR_ARG;
+// The state of simulation_state with the block partials in `part` (the library's buffer, grown on demand,
+// or the caller's fixed buffer of part_cap doubles when fixed is set). Fixed buffers fold the partials on
+// the device after them and copy the results only; the library buffer copies the partials and folds them
+// on the host. Both fold in block order.
+static int sim_state_impl(xdem_ctx* c, xdem_sim_state* out, double* fixed, int64_t part_cap) {
   *out = {};
   const bool poly = c->nshapes > 0;
   const int64_t n = c->n;
@@ -112,10 +175,19 @@ extern "C" int xdem_simulation_state(xdem_ctx* c, xdem_sim_state* out) {
   out->volume = c->grid.L[0] * c->grid.L[1] * c->grid.L[2];
   if (n == 0) return XDEM_OK;
   const int blocks = grid_blocks(n, 256);
-  if ((int64_t)blocks * kSimVals > c->cap_simpart) {
-    int rc = dev_realloc(c, &c->simpart, (int64_t)blocks * kSimVals);
-    if (rc) return rc;
-    c->cap_simpart = (int64_t)blocks * kSimVals;
+  double* part = fixed;
+  if (fixed) {
+    if ((int64_t)blocks * kSimVals + kSimVals > part_cap)
+      return set_err(c, XDEM_ERR_CAPACITY, "simulation_state_noalloc: more than " +
+                                               std::to_string((part_cap - kSimVals) / kSimVals * 256) +
+                                               " particles");
+  } else {
+    if ((int64_t)blocks * kSimVals > c->cap_simpart) {
+      int rc = dev_realloc(c, &c->simpart, (int64_t)blocks * kSimVals);
+      if (rc) return rc;
+      c->cap_simpart = (int64_t)blocks * kSimVals;
+    }
+    part = c->simpart;
   }
   NbrList nl{};
   if (!poly && c->nl_valid && c->hist_ready) nl = c->nl[c->nl_cur];  // polyhedra: energies only here
@@ -129,15 +201,23 @@ extern "C" int xdem_simulation_state(xdem_ctx* c, xdem_sim_state* out) {
     ml = c->ml[c->ml_cur];
   }
   hipLaunchKernelGGL(k_sim_state, dim3(blocks), dim3(256), 0, c->stream, n, live(c), nl, c->grid, wl, c->d_walls,
-                     walls ? c->nwalls : 0, c->cap, mesh, ml, c->simpart);
+                     walls ? c->nwalls : 0, c->cap, mesh, ml, part);
   XDEM_HIP(c, hipGetLastError());
-  std::vector<double> h((size_t)blocks * kSimVals);
-  XDEM_HIP(c, hipMemcpyAsync(h.data(), c->simpart, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
-  XDEM_HIP(c, hipStreamSynchronize(c->stream));
   double acc[kSimVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int k = 0; k < blocks; k++) {
-    for (int q = 0; q < kSimVals - 1; q++) acc[q] += h[(size_t)k * kSimVals + q];
-    acc[kSimVals - 1] = std::min(acc[kSimVals - 1], h[(size_t)k * kSimVals + kSimVals - 1]);
+  if (fixed) {
+    double* res = part + (int64_t)blocks * kSimVals;
+    int rc = fold_partials(c, part, blocks, kSimVals, res);
+    if (rc) return rc;
+    XDEM_HIP(c, hipMemcpyAsync(acc, res, sizeof(acc), hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<double> h((size_t)blocks * kSimVals);
+    XDEM_HIP(c, hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < blocks; k++) {
+      for (int q = 0; q < kSimVals - 1; q++) acc[q] += h[(size_t)k * kSimVals + q];
+      acc[kSimVals - 1] = std::min(acc[kSimVals - 1], h[(size_t)k * kSimVals + kSimVals - 1]);
+    }
   }
   for (int a = 0; a < 3; a++) {
     out->kinetic_energy[a] = acc[a];
@@ -149,11 +229,25 @@ extern "C" int xdem_simulation_state(xdem_ctx* c, xdem_sim_state* out) {
   out->min_dn = acc[9];
   if (poly) {
     double tot = 0, act = 0, mn = 0;
-    const int rc = poly_sim_counts(c, &tot, &act, &mn);
+    const int rc = poly_sim_counts(c, &tot, &act, &mn, fixed, part_cap);
     if (rc) return rc;
     out->n_interactions = (int64_t)tot;
     out->n_active = (int64_t)act;
     out->min_dn = mn;
   }
   return XDEM_OK;
+}
+
+extern "C" int xdem_simulation_state(xdem_ctx* c, xdem_sim_state* out) {
+  if (!c || !out) return XDEM_ERR_ARG;
+  return sim_state_impl(c, out, nullptr, 0);
+}
+
+extern "C" int xdem_simulation_state_noalloc(xdem_ctx* c, xdem_sim_state* out) {
+  if (!c || !out) return XDEM_ERR_ARG;
+  *out = {};
+  double* ptr = scratch_of(c->device);
+  if (!ptr) return set_err(c, XDEM_ERR_STATE, "simulation_state_noalloc: no scratch address for this device");
+  XDEM_HIP(c, hipSetDevice(c->device));
+  return sim_state_impl(c, out, ptr, kScratchDoubles);
 }

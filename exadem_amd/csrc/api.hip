@@ -316,6 +316,11 @@ int xdem_ctx_create(int32_t device, xdem_ctx** out) {
   }
   (void)hipMemsetAsync(c->d_flags, 0, 4 * sizeof(int32_t), c->stream);
   (void)hipStreamSynchronize(c->stream);
+  // this GPU's no-allocation scratch address (xdem_simulation_state_noalloc), once per device
+  if (noalloc_scratch_init(device) != XDEM_OK) {
+    xdem_ctx_destroy(c);
+    return XDEM_ERR_HIP;
+  }
   *out = c;
   return XDEM_OK;
 }

@@ -1438,7 +1438,7 @@ __global__ __launch_bounds__(256) void k_poly_sim_wall(int64_t n, PState s, cons
   block_reduce3(tot, act, mn, part);
 }
 
-int poly_sim_counts(xdem_ctx* c, double* tot, double* act, double* min_dn) {
+int poly_sim_counts(xdem_ctx* c, double* tot, double* act, double* min_dn, double* fixed, int64_t fixed_cap) {
   *tot = *act = *min_dn = 0.0;
   if (!c->pl_valid || c->pl_epoch != c->nl_epoch || c->poly_pass_epoch != c->nl_epoch)
     return set_err(c, XDEM_ERR_STATE, "simulation_state: run contact_polyhedron after nbh_polyhedron first");
@@ -1446,20 +1446,39 @@ int poly_sim_counts(xdem_ctx* c, double* tot, double* act, double* min_dn) {
   const int64_t nf = P.nfeat, n = c->n;
   const int bf = grid_blocks(std::max<int64_t>(nf, 1), 256), bw = grid_blocks(std::max<int64_t>(n, 1), 256);
   const int64_t need = 3 * (int64_t)(bf + bw);
-  if (need > c->cap_simpart) {
-    int rc = dev_realloc(c, &c->simpart, need);
-    if (rc) return rc;
-    c->cap_simpart = need;
+  double* part = fixed;
+  if (fixed) {
+    if (need + 3 > fixed_cap)
+      return set_err(c, XDEM_ERR_CAPACITY, "simulation_state_noalloc: ceil(features / 256) + ceil(particles / 256) "
+                                           "exceeds " + std::to_string((fixed_cap - 3) / 3) + " blocks");
+  } else {
+    if (need > c->cap_simpart) {
+      int rc = dev_realloc(c, &c->simpart, need);
+      if (rc) return rc;
+      c->cap_simpart = need;
+    }
+    part = c->simpart;
   }
   const int nw = (c->nwalls > 0 && c->pw_valid) ? c->nwalls : 0;
-  hipLaunchKernelGGL(k_poly_sim_feat, dim3(bf), dim3(256), 0, c->stream, nf, P.feat, c->fcontact, c->fdn, c->simpart);
+  hipLaunchKernelGGL(k_poly_sim_feat, dim3(bf), dim3(256), 0, c->stream, nf, P.feat, c->fcontact, c->fdn, part);
   PolyWall pw{};
   if (nw) pw = c->pw[c->pw_cur];
   hipLaunchKernelGGL(k_poly_sim_wall, dim3(bw), dim3(256), 0, c->stream, n, live(c), c->d_shapes, c->vtx, pw,
-                     c->d_walls, nw, c->cap, c->simpart + 3 * (int64_t)bf);
+                     c->d_walls, nw, c->cap, part + 3 * (int64_t)bf);
   XDEM_HIP(c, hipGetLastError());
+  if (fixed) {
+    double res[3];
+    int rc = fold_partials(c, part, bf + bw, 3, part + need);
+    if (rc) return rc;
+    XDEM_HIP(c, hipMemcpyAsync(res, part + need, sizeof(res), hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipStreamSynchronize(c->stream));
+    *tot = res[0];
+    *act = res[1];
+    *min_dn = res[2];
+    return XDEM_OK;
+  }
   std::vector<double> h((size_t)need);
-  XDEM_HIP(c, hipMemcpyAsync(h.data(), c->simpart, sizeof(double) * need, hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipMemcpyAsync(h.data(), part, sizeof(double) * need, hipMemcpyDeviceToHost, c->stream));
   XDEM_HIP(c, hipStreamSynchronize(c->stream));
   for (int k = 0; k < bf + bw; k++) {
     *tot += h[3 * (size_t)k];

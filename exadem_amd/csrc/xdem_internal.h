@@ -503,8 +503,14 @@ int poly_compute_vertices(xdem_ctx* c, bool displ, double thr2, bool guarded);
 int poly_nbh(xdem_ctx* c, double rcut_inc);
 int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool fused, bool gravity_on,
                V3 g, bool guarded, int32_t step_tag);
-// interaction counts of the polyhedron lists for simulation_state: total, active, smallest dn
-int poly_sim_counts(xdem_ctx* c, double* tot, double* act, double* min_dn);
+// interaction counts of the polyhedron lists for simulation_state: total, active, smallest dn; block
+// partials in the library's buffer (fixed = null) or in the fixed buffer of fixed_cap doubles
+int poly_sim_counts(xdem_ctx* c, double* tot, double* act, double* min_dn, double* fixed, int64_t fixed_cap);
+// folds nblocks x nvals block partials in block order into out_dev[0..nvals) on the device (sums, the
+// last value a minimum), as xdem_simulation_state folds them on the host
+int fold_partials(xdem_ctx* c, const double* part, int64_t nblocks, int nvals, double* out_dev);
+// device address of this GPU's copy of the no-allocation scratch array (once per device, device current)
+int noalloc_scratch_init(int device);
 int poly_contact_state_rows(xdem_ctx* c, std::vector<uint64_t>& ii, std::vector<uint64_t>& jj,
                             std::vector<uint32_t>& si, std::vector<uint32_t>& sj, std::vector<uint16_t>& tt,
                             std::vector<double>& vals);

@@ -100,7 +100,7 @@ HIP_SYMBOLS = [
     "xdem_gravity_force", "xdem_force_to_accel", "xdem_particle_displ_over", "xdem_move_particles",
     "xdem_chunk_neighbors_contact", "xdem_nbh_sphere", "xdem_classify_interactions", "xdem_contact_sphere",
     "xdem_run_steps", "xdem_interactions_count", "xdem_interactions_dump", "xdem_interactions_load", "xdem_poly_interactions_load", "xdem_interactions_moments",
-    "xdem_poly_interactions_moments", "xdem_simulation_state",
+    "xdem_poly_interactions_moments", "xdem_simulation_state", "xdem_simulation_state_noalloc",
     "xdem_set_contact_record", "xdem_contact_state_count", "xdem_contact_state_dump", "xdem_nbh_stats",
     "xdem_kernel_timing", "xdem_set_timing", "xdem_streams_dump",
     "xdem_set_shapes", "xdem_compute_vertices", "xdem_vertex_displ_over", "xdem_nbh_polyhedron",
@@ -192,6 +192,7 @@ def load():
         "xdem_interactions_moments": (C.c_int, [vp, C.c_int64, _dp, C.POINTER(C.c_int64)]),
         "xdem_poly_interactions_moments": (C.c_int, [vp, C.c_int64, _dp, C.POINTER(C.c_int64)]),
         "xdem_simulation_state": (C.c_int, [vp, C.POINTER(SimState)]),
+        "xdem_simulation_state_noalloc": (C.c_int, [vp, C.POINTER(SimState)]),
         "xdem_set_mesh": (C.c_int, [vp, C.c_int32, _dp, C.c_int64, _u32p, C.c_int64, _u32p, C.c_int64, C.c_double,
                                     _dp, _dp, C.c_int32]),
         "xdem_mesh_interactions": (C.c_int, [vp, C.c_int64, _u64p, _u32p, C.POINTER(C.c_uint16), _dp, _dp,
@@ -489,10 +490,11 @@ class Context:
                   _ptr(fr, C.c_double), None if mo is None else _ptr(mo, C.c_double), C.byref(got))
         return got.value
 
-    def simulation_state(self):
-        """Thermodynamic state (xdem_simulation_state) as a dict."""
+    def simulation_state(self, noalloc=False):
+        """Thermodynamic state (xdem_simulation_state, or xdem_simulation_state_noalloc, which allocates no
+        device memory) as a dict."""
         st = SimState()
-        self.call("xdem_simulation_state", C.byref(st))
+        self.call("xdem_simulation_state_noalloc" if noalloc else "xdem_simulation_state", C.byref(st))
         return {"kinetic_energy": list(st.kinetic_energy), "rotation_energy": list(st.rotation_energy),
                 "mass": st.mass, "volume": st.volume, "min_dn": st.min_dn, "n_particles": st.n_particles,
                 "n_interactions": st.n_interactions, "n_active": st.n_active}

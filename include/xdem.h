@@ -309,6 +309,15 @@ typedef struct xdem_sim_state {
   int64_t n_active;
 } xdem_sim_state;
 int xdem_simulation_state(xdem_ctx* ctx, xdem_sim_state* out);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): xdem_simulation_state with
+ * the same result bit for bit, but its block partials and results live in a fixed 1.5 MiB device
+ * array of the library, one per GPU, held until the process ends, and it calls no hipMalloc or
+ * hipFree. Limits: at most 5,032,704 particles (19,659 blocks of 256); polyhedra, also
+ * ceil(features / 256) + ceil(particles / 256) <= 65,535. Larger inputs fail with
+ * XDEM_ERR_CAPACITY. Every call on one GPU shares that array: calls that overlap on one GPU (other
+ * host threads, other contexts on the same device) race and must be serialised by the caller. */
+int xdem_simulation_state_noalloc(xdem_ctx* ctx, xdem_sim_state* out);
 /* Neighbour-list statistics of the last build: half-list pairs P, list slots
  * allocated (ELL with per-wave padding) and the grid dimensions. */
 int xdem_nbh_stats(xdem_ctx* ctx, int64_t* pairs, int64_t* slots, int32_t dims[3]);
