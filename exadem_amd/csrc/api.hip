@@ -65,7 +65,7 @@ static void free_pstate(PState& s) {
 }
 
 static void free_nl(NbrList& l) {
-  dfree(&l.word); dfree(&l.hx); dfree(&l.hy); dfree(&l.hz); dfree(&l.mhx); dfree(&l.mhy); dfree(&l.mhz);
+  dfree(&l.word); dfree(&l.sx); dfree(&l.hx); dfree(&l.hy); dfree(&l.hz); dfree(&l.mhx); dfree(&l.mhy); dfree(&l.mhz);
   dfree(&l.fx32); dfree(&l.fy32); dfree(&l.fz32); dfree(&l.mfx32); dfree(&l.mfy32); dfree(&l.mfz32);
   dfree(&l.cnt); dfree(&l.kmax); dfree(&l.base);
   l.slots = l.cap_slots = l.pairs = 0;
@@ -107,6 +107,7 @@ int ensure_cells(xdem_ctx* c, int64_t ncells) {
     if ((rc = dalloc(c, &c->cell_count, ncells + 1))) return rc;
     if ((rc = dalloc(c, &c->cell_start, ncells + 1))) return rc;
     if ((rc = dalloc(c, &c->cell_rank, ncells))) return rc;
+    if ((rc = dalloc(c, &c->cell_lin, ncells))) return rc;
     c->cap_cells = ncells;
   }
   // rank of every cell along a Morton (Z-order) curve over the grid: consecutive particles then
@@ -125,9 +126,13 @@ int ensure_cells(xdem_ctx* c, int64_t ncells) {
         key[cid] = {spread(x) | (spread(y) << 1) | (spread(z) << 2), cid};
       }
   std::sort(key.begin(), key.end());
-  std::vector<uint32_t> rank(ncells);
-  for (int64_t r = 0; r < ncells; r++) rank[key[r].second] = (uint32_t)r;
+  std::vector<uint32_t> rank(ncells), lin(ncells);
+  for (int64_t r = 0; r < ncells; r++) {
+    rank[key[r].second] = (uint32_t)r;
+    lin[r] = key[r].second;
+  }
   XDEM_HIP(c, hipMemcpyAsync(c->cell_rank, rank.data(), sizeof(uint32_t) * ncells, hipMemcpyHostToDevice, c->stream));
+  XDEM_HIP(c, hipMemcpyAsync(c->cell_lin, lin.data(), sizeof(uint32_t) * ncells, hipMemcpyHostToDevice, c->stream));
   XDEM_HIP(c, hipStreamSynchronize(c->stream));
   return XDEM_OK;
 }
@@ -295,6 +300,81 @@ __global__ void k_unpack_out(int64_t n, PState s, double* rx, double* ry, double
 }  // namespace xdem
 
 // ================================================================== C ABI
+namespace xdem {
+static inline bool motion_on(const Motion& m, double time) { return time >= m.t_start && time <= m.t_end; }
+static inline double shaker_signal(const Motion& m, double time) {
+  time -= m.t_start;
+  return m.amplitude * std::sin(m.omega * time);
+}
+static inline V3 shaker_velocity(const Motion& m, double time) {
+  time -= m.t_start;
+  return m.amplitude * m.omega * std::cos(m.omega * time) * m.shaker_dir;
+}
+// push_f_v_r of a moving driver: Surface::push_f_v_r (surface.hpp:260-299, no acceleration for
+// kinematic motions), Cylinder (cylinder.hpp:160-173), Ball (ball.hpp:203-213)
+static void push_f_v_r_one(Surface& s, const Motion& m, double time, double dt) {
+  if (m.type == XDEM_MOTION_STATIONARY) return;
+  if (s.kind == XDEM_DRIVER_SURFACE) {
+    const double acc = 0.0;
+    double displ = dt * dot(s.vel, s.normal) + 0.5 * dt * dt * acc;
+    if (m.type == XDEM_MOTION_SHAKER) {
+      const double next = shaker_signal(m, time + dt), cur = shaker_signal(m, time);
+      displ = (next - cur) * dot(m.shaker_dir, s.normal);
+      s.vel = shaker_velocity(m, time + dt);
+    }
+    s.center = s.center + displ * s.normal;
+    s.offset += displ;
+    s.center_proj = s.center_proj + displ * s.normal;
+  } else if (s.kind == XDEM_DRIVER_CYLINDER) {
+    if (m.type == XDEM_MOTION_SHAKER) {
+      const double displ = shaker_signal(m, time + dt) - shaker_signal(m, time);
+      s.center = s.center + displ * m.shaker_dir;
+      s.vel = shaker_velocity(m, time + dt);
+    }
+  } else {
+    s.center = s.center + dt * s.vel;
+  }
+}
+// push_f_v (surface.hpp:234-252, ball.hpp:219-234; nothing for a cylinder, cylinder.hpp:149-151)
+static void push_f_v_one(Surface& s, const Motion& m) {
+  if (s.kind == XDEM_DRIVER_SURFACE) {
+    if (m.type == XDEM_MOTION_STATIONARY) s.vel = {0, 0, 0};
+    else if (m.type == XDEM_MOTION_LINEAR) s.vel = m.const_vel * m.mvec;
+  } else if (s.kind == XDEM_DRIVER_BALL) {
+    if (m.type == XDEM_MOTION_LINEAR) s.vel = m.mvec * m.const_vel;
+  }
+}
+void drivers_push_f_v(Surface* w, const Motion* m, int nw) {
+  for (int k = 0; k < nw; k++) push_f_v_one(w[k], m[k]);
+}
+// push_f_v_r_driver (gated by is_motion_triggered at time + dt, push_f_v_r_driver.cpp:28-37) then
+// push_f_v_driver (config_numerical_schemes.msp:13-14)
+void drivers_advance(Surface* w, const Motion* m, int nw, double time, double dt) {
+  for (int k = 0; k < nw; k++)
+    if (motion_on(m[k], time + dt)) push_f_v_r_one(w[k], m[k], time, dt);
+  drivers_push_f_v(w, m, nw);
+}
+// driver_displ_over (driver_displ_over.cu:59-130): projected centre of a surface, centre of a ball; a
+// cylinder never triggers
+bool drivers_displaced(const Surface* now, const Surface* bk, int nw, double thr) {
+  const double r2 = thr * thr;
+  for (int k = 0; k < nw; k++) {
+    V3 d{0, 0, 0};
+    if (now[k].kind == XDEM_DRIVER_SURFACE) d = now[k].center_proj - bk[k].center_proj;
+    else if (now[k].kind == XDEM_DRIVER_BALL) d = now[k].center - bk[k].center;
+    else continue;
+    if (dot(d, d) >= r2) return true;
+  }
+  return false;
+}
+static int upload_walls(xdem_ctx* c) {
+  XDEM_HIP(c, hipMemcpyAsync(c->d_walls, c->walls, sizeof(Surface) * XDEM_MAX_SURFACES, hipMemcpyHostToDevice,
+                             c->stream));
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  return XDEM_OK;
+}
+}  // namespace xdem
+
 extern "C" {
 
 int xdem_ctx_create(int32_t device, xdem_ctx** out) {
@@ -308,7 +388,7 @@ int xdem_ctx_create(int32_t device, xdem_ctx** out) {
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc((void**)&c->d_flags, 4 * sizeof(int32_t)) != hipSuccess ||
       hipHostMalloc((void**)&c->h_flags, 4 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
-      hipMalloc((void**)&c->d_u32tmp, 16) != hipSuccess ||
+      hipMalloc((void**)&c->d_u32tmp, 16) != hipSuccess || hipMalloc((void**)&c->d_rmax, 8) != hipSuccess ||
       hipHostMalloc((void**)&c->h_u32tmp, 16, hipHostMallocDefault) != hipSuccess ||
       hipMalloc((void**)&c->d_walls, sizeof(Surface) * XDEM_MAX_SURFACES) != hipSuccess) {
     delete c;
@@ -338,10 +418,13 @@ void xdem_ctx_destroy(xdem_ctx* c) {
   mesh_free(c);
   dfree(&c->core_alt);
   dfree(&c->cell_of); dfree(&c->cell_count); dfree(&c->cell_start); dfree(&c->cell_rank); dfree(&c->perm);
+  dfree(&c->cell_lin); dfree(&c->d_rmax); dfree(&c->nbh_tmp); dfree(&c->mxr); dfree(&c->mid); dfree(&c->blk_u32);
   dfree(&c->perm_tmp);
   dfree(&c->sort_key);
   dfree(&c->d_flags); dfree(&c->d_u32tmp); dfree(&c->d_walls);
   dfree(&c->cstate); dfree(&c->wstate); dfree(&c->simpart); dfree(&c->d_shapes); dfree(&c->vtx); dfree(&c->frange); dfree(&c->fcontrib); dfree(&c->fcontact); dfree(&c->fdn);
+  dfree(&c->forder); dfree(&c->fclass_blk); dfree(&c->fclass_info); dfree(&c->d_walls_ring);
+  if (c->h_walls_ring) (void)hipHostFree(c->h_walls_ring);
   for (int k = 0; k < 2; k++) {
     PolyList& P = c->pl[k];
     dfree(&P.start); dfree(&P.feat); dfree(&P.hx); dfree(&P.hy); dfree(&P.hz);
@@ -426,10 +509,136 @@ int xdem_set_surfaces(xdem_ctx* c, const xdem_surface* w, int32_t n) {
     c->walls[k] = s;
   }
   c->nwalls = n;
+  for (int k = 0; k < XDEM_MAX_SURFACES; k++) {
+    c->motion[k] = Motion{};
+    c->walls_bk[k] = c->walls[k];
+  }
+  c->drivers_move = false;
   XDEM_HIP(c, hipMemcpyAsync(c->d_walls, c->walls, sizeof(Surface) * XDEM_MAX_SURFACES, hipMemcpyHostToDevice,
                              c->stream));
   XDEM_HIP(c, hipStreamSynchronize(c->stream));
   c->nl_valid = false;  // wall candidates must be rebuilt
+  return XDEM_OK;
+}
+
+// ------------------------------------------------------------------ driver motion
+// Motion of each driver as register_surface / register_cylinder / register_ball set it up:
+// check_motion_coherence (driver_params.hpp:108-214) normalises a LINEAR motion vector or a shaker
+// direction when |v|^2 - 1 >= 1e-14, and a linear surface moves along its normal unless its motion vector
+// is +-normal (Surface::initialize, surface.hpp:183-195). Valid motion types per driver:
+// surface STATIONARY / LINEAR_MOTION / SHAKER, cylinder STATIONARY / SHAKER, ball STATIONARY / LINEAR_MOTION.
+int xdem_set_driver_motion(xdem_ctx* c, const xdem_driver_motion* m, int32_t n) {
+  if (!c || (n > 0 && !m)) return XDEM_ERR_ARG;
+  if (n != 0 && n != c->nwalls)
+    return set_err(c, XDEM_ERR_ARG, "set_driver_motion: one motion per registered driver (or none)");
+  Motion mo[XDEM_MAX_SURFACES];
+  for (int k = 0; k < n; k++) {
+    Motion& a = mo[k];
+    a.type = m[k].motion_type;
+    a.mvec = {m[k].motion_vector[0], m[k].motion_vector[1], m[k].motion_vector[2]};
+    a.const_vel = m[k].const_vel;
+    a.t_start = m[k].motion_start_threshold;
+    a.t_end = m[k].motion_end_threshold;
+    a.amplitude = m[k].amplitude;
+    a.omega = m[k].omega;
+    a.shaker_dir = {m[k].shaker_dir[0], m[k].shaker_dir[1], m[k].shaker_dir[2]};
+    const int kind = c->walls[k].kind;
+    const bool ok = a.type == XDEM_MOTION_STATIONARY ||
+                    (a.type == XDEM_MOTION_LINEAR && kind != XDEM_DRIVER_CYLINDER) ||
+                    (a.type == XDEM_MOTION_SHAKER && kind != XDEM_DRIVER_BALL);
+    if (!ok) {
+      if (a.type == XDEM_MOTION_LINEAR || a.type == XDEM_MOTION_SHAKER)
+        return set_err(c, XDEM_ERR_ARG, "set_driver_motion: motion type not valid for this driver kind");
+      return set_err(c, XDEM_ERR_UNSUPPORTED, "set_driver_motion: only STATIONARY, LINEAR_MOTION and SHAKER are provided");
+    }
+    if (a.type == XDEM_MOTION_SHAKER) {
+      if (!(a.amplitude > 0.0) || !(a.omega > 0.0))
+        return set_err(c, XDEM_ERR_ARG, "set_driver_motion: SHAKER needs amplitude > 0 and omega > 0");
+      if (dot(a.shaker_dir, a.shaker_dir) - 1 >= 1e-14) a.shaker_dir = a.shaker_dir / norm(a.shaker_dir);
+    }
+    if (a.type == XDEM_MOTION_LINEAR) {
+      if (a.mvec.x == 0 && a.mvec.y == 0 && a.mvec.z == 0)
+        return set_err(c, XDEM_ERR_ARG, "set_driver_motion: LINEAR_MOTION needs a motion vector");
+      if (dot(a.mvec, a.mvec) - 1 >= 1e-14) a.mvec = a.mvec / norm(a.mvec);
+      if (kind == XDEM_DRIVER_SURFACE) {
+        const V3 nn = c->walls[k].normal, mn = -1.0 * nn;
+        const bool same = a.mvec.x == nn.x && a.mvec.y == nn.y && a.mvec.z == nn.z;
+        const bool opp = a.mvec.x == mn.x && a.mvec.y == mn.y && a.mvec.z == mn.z;
+        if (!same && !opp) a.mvec = nn;
+      }
+    }
+  }
+  bool moving = false;
+  for (int k = 0; k < XDEM_MAX_SURFACES; k++) {
+    c->motion[k] = k < n ? mo[k] : Motion{};
+    moving = moving || c->motion[k].type != XDEM_MOTION_STATIONARY;
+  }
+  c->drivers_move = moving;
+  return XDEM_OK;
+}
+
+
+int xdem_push_f_v_r_driver(xdem_ctx* c, double time, double dt) {
+  if (!c) return XDEM_ERR_ARG;
+  if (c->nwalls == 0) return XDEM_OK;
+  for (int k = 0; k < c->nwalls; k++)
+    if (motion_on(c->motion[k], time + dt)) push_f_v_r_one(c->walls[k], c->motion[k], time, dt);
+  return upload_walls(c);
+}
+
+int xdem_push_f_v_driver(xdem_ctx* c, double dt) {
+  if (!c) return XDEM_ERR_ARG;
+  (void)dt;  // kinematic motions set the velocity outright
+  if (c->nwalls == 0) return XDEM_OK;
+  drivers_push_f_v(c->walls, c->motion, c->nwalls);
+  return upload_walls(c);
+}
+
+int xdem_driver_displ_over(xdem_ctx* c, double threshold, int32_t* result) {
+  if (!c || !result) return XDEM_ERR_ARG;
+  *result = drivers_displaced(c->walls, c->walls_bk, c->nwalls, threshold) ? 1 : 0;
+  return XDEM_OK;
+}
+
+int xdem_drivers_download(xdem_ctx* c, xdem_surface* out, int32_t n) {
+  if (!c || (n > 0 && !out)) return XDEM_ERR_ARG;
+  if (n > c->nwalls) return set_err(c, XDEM_ERR_ARG, "drivers_download: n exceeds the registered drivers");
+  for (int k = 0; k < n; k++) {
+    const Surface& s = c->walls[k];
+    xdem_surface r{};
+    r.offset = s.offset;
+    const V3* src[4] = {&s.normal, &s.center, &s.vel, &s.vrot};
+    double* dst[4] = {r.normal, r.center, r.vel, r.vrot};
+    for (int q = 0; q < 4; q++) {
+      dst[q][0] = src[q]->x;
+      dst[q][1] = src[q]->y;
+      dst[q][2] = src[q]->z;
+    }
+    r.kind = s.kind;
+    r.radius = s.radius;
+    r.axis[0] = s.axis.x;
+    r.axis[1] = s.axis.y;
+    r.axis[2] = s.axis.z;
+    out[k] = r;
+  }
+  return XDEM_OK;
+}
+
+int xdem_set_physical_time(xdem_ctx* c, double time) {
+  if (!c) return XDEM_ERR_ARG;
+  c->physical_time = time;
+  return XDEM_OK;
+}
+
+int xdem_physical_time(xdem_ctx* c, double* time) {
+  if (!c || !time) return XDEM_ERR_ARG;
+  *time = c->physical_time;
+  return XDEM_OK;
+}
+
+int xdem_next_time_step(xdem_ctx* c, double dt) {
+  if (!c) return XDEM_ERR_ARG;
+  c->physical_time += dt;
   return XDEM_OK;
 }
 
@@ -669,6 +878,10 @@ int xdem_nbh_sphere(xdem_ctx* c, double rcut_inc, int32_t symetric) {
 int xdem_classify_interactions(xdem_ctx* c) {
   if (!c) return XDEM_ERR_ARG;
   if (!c->nl_valid && c->n > 0) return set_err(c, XDEM_ERR_STATE, "classify_interactions: no interaction list");
+  // polyhedra: the feature interactions split into one wave per type (the contact pass runs a type per
+  // wavefront). Spheres hold one interaction type per structure already: particle pairs in the
+  // neighbour list, each driver's candidates in its own arrays, mesh candidates in the mesh lists.
+  if (c->nshapes > 0) return poly_classify(c);
   return XDEM_OK;
 }
 
@@ -707,7 +920,10 @@ static int rebuild(xdem_ctx* c, double rcut_inc) {
   int rc;
   if ((rc = do_move_particles(c))) return rc;
   if ((rc = do_chunk_neighbors(c, rcut_inc, -1.0))) return rc;
-  if (c->nshapes > 0) return poly_nbh(c, rcut_inc);
+  if (c->nshapes > 0) {
+    if ((rc = poly_nbh(c, rcut_inc))) return rc;
+    return poly_classify(c);
+  }
   return do_nbh_sphere(c, rcut_inc);
 }
 
@@ -763,6 +979,27 @@ int xdem_run_steps(xdem_ctx* c, const xdem_scheme* s, int64_t nsteps, xdem_run_s
   const V3 g{s->gravity[0], s->gravity[1], s->gravity[2]};
   const double thr = 0.5 * s->rcut_inc;
   const int poll = s->poll > 0 ? s->poll : 16;
+  const int nw = c->nwalls;
+  // Drivers. A step applies push_f_v_r_driver and push_f_v_driver after the particle prolog; with
+  // kinematic motions the drivers' states do not depend on the particles, so the host plans them for
+  // each step of a chunk (the same arithmetic as the operators), copies the chunk's states to the device
+  // once, and each force pass reads its step's state. A driver that moved by rcut_inc/2 since the last
+  // neighbour build raises the rebuild trigger of its step, as a particle does. Without motion, every
+  // step's push_f_v leaves the same state (a stationary surface has zero velocity), set here once.
+  const bool moving = nw > 0 && c->drivers_move;
+  if (nw > 0 && !moving) {
+    drivers_push_f_v(c->walls, c->motion, nw);
+    XDEM_HIP(c, hipMemcpyAsync(c->d_walls, c->walls, sizeof(Surface) * XDEM_MAX_SURFACES, hipMemcpyHostToDevice,
+                               c->stream));
+  }
+  if (moving && c->cap_ring < poll) {
+    if (c->h_walls_ring) (void)hipHostFree(c->h_walls_ring);
+    c->h_walls_ring = nullptr;
+    XDEM_HIP(c, hipHostMalloc((void**)&c->h_walls_ring, sizeof(Surface) * XDEM_MAX_SURFACES * poll,
+                              hipHostMallocDefault));
+    if ((rc = dev_realloc(c, &c->d_walls_ring, (int64_t)XDEM_MAX_SURFACES * poll))) return rc;
+    c->cap_ring = poll;
+  }
   XDEM_HIP(c, hipMemsetAsync(c->d_flags, 0, 4 * sizeof(int32_t), c->stream));
   c->trig_slot = 0;
   // spheres run a step as one kernel (force pass + kick + next prolog into the other record buffer,
@@ -771,14 +1008,30 @@ int xdem_run_steps(xdem_ctx* c, const xdem_scheme* s, int64_t nsteps, xdem_run_s
   // (Polyhedra keep separate kernels: an integrating gather that also recomputes the vertices measured
   // no faster than the separate vertex pass.)
   const bool fuse = c->nshapes == 0;
-  std::vector<Core*> input(poll);  // record buffer read by the integrating pass of each step of a chunk
-  bool chain = false;              // the live records hold the prolog of the next step
+  std::vector<Core*> input(poll);   // record buffer read by the integrating pass of each step of a chunk
+  std::vector<double> times(poll + 1);  // physical time at the start of each step of a chunk
+  bool chain = false;               // the live records hold the prolog of the next step
   int64_t done = 0, rebuilds = 0, polls = 0;
+  double t_done = c->physical_time;
   while (done < nsteps) {
     // issue up to `poll` steps. A step is either K1 (kicks of the previous pass + prolog +
     // displacement test) and K2 (force pass), or, inside a chain, one integrating pass. A force pass
     // turns a raised trigger into a stall that makes every later launch a no-op.
     const int64_t upto = std::min<int64_t>(nsteps, done + poll);
+    int64_t drv_trig = -1;  // first step of the chunk whose drivers moved by rcut_inc/2
+    times[0] = t_done;
+    for (int64_t step = done; step < upto; step++) times[step - done + 1] = times[step - done] + s->dt;
+    if (moving) {
+      Surface w[XDEM_MAX_SURFACES];
+      std::memcpy(w, c->walls, sizeof(w));
+      for (int64_t step = done; step < upto; step++) {
+        drivers_advance(w, c->motion, nw, times[step - done], s->dt);
+        std::memcpy(c->h_walls_ring + (step - done) * XDEM_MAX_SURFACES, w, sizeof(w));
+        if (drv_trig < 0 && drivers_displaced(w, c->walls_bk, nw, thr)) drv_trig = step;
+      }
+      XDEM_HIP(c, hipMemcpyAsync(c->d_walls_ring, c->h_walls_ring, sizeof(Surface) * XDEM_MAX_SURFACES * (upto - done),
+                                 hipMemcpyHostToDevice, c->stream));
+    }
     for (int64_t step = done; step < upto; step++) {
       const bool kick = c->pending_kick;
       const double kdt = c->pending_dt;
@@ -792,6 +1045,10 @@ int xdem_run_steps(xdem_ctx* c, const xdem_scheme* s, int64_t nsteps, xdem_run_s
         if (poly && (rc = poly_compute_vertices(c, true, thr * thr, true))) return rc;
         if (timed && (rc = record_timer(c, 0, false))) return rc;
         c->trig_slot = 0;
+      }
+      if (moving) {
+        c->d_walls_pass = c->d_walls_ring + (step - done) * XDEM_MAX_SURFACES;
+        if (step == drv_trig) XDEM_HIP(c, hipMemsetAsync(c->d_flags + c->trig_slot, 1, 1, c->stream));
       }
       if (timed && (rc = record_timer(c, 1, true))) return rc;
       if (fuse && step + 1 < nsteps) {
@@ -808,6 +1065,7 @@ int xdem_run_steps(xdem_ctx* c, const xdem_scheme* s, int64_t nsteps, xdem_run_s
       }
       if (timed && (rc = record_timer(c, 1, false))) return rc;
     }
+    c->d_walls_pass = nullptr;
     XDEM_HIP(c, hipMemcpyAsync(c->h_flags, c->d_flags, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     XDEM_HIP(c, hipStreamSynchronize(c->stream));
     polls++;
@@ -817,6 +1075,11 @@ int xdem_run_steps(xdem_ctx* c, const xdem_scheme* s, int64_t nsteps, xdem_run_s
       const int64_t stalled = (int64_t)c->h_flags[2];
       Core* state = stalled >= done && stalled < upto ? input[stalled - done] : nullptr;
       if (state && live(c).core != state) std::swap(live(c).core, c->core_alt);
+      if (moving && stalled >= done && stalled < upto) {
+        std::memcpy(c->walls, c->h_walls_ring + (stalled - done) * XDEM_MAX_SURFACES, sizeof(c->walls));
+        XDEM_HIP(c, hipMemcpyAsync(c->d_walls, c->walls, sizeof(Surface) * XDEM_MAX_SURFACES, hipMemcpyHostToDevice,
+                                   c->stream));
+      }
       XDEM_HIP(c, hipMemsetAsync(c->d_flags, 0, 4 * sizeof(int32_t), c->stream));
       c->trig_slot = 0;
       if ((rc = rebuild(c, s->rcut_inc))) return rc;
@@ -825,11 +1088,19 @@ int xdem_run_steps(xdem_ctx* c, const xdem_scheme* s, int64_t nsteps, xdem_run_s
       chain = false;
       c->pending_kick = true;
       c->pending_dt = s->dt;
+      t_done = times[stalled - done + 1];
       done = stalled + 1;
     } else {
+      if (moving) {
+        std::memcpy(c->walls, c->h_walls_ring + (upto - 1 - done) * XDEM_MAX_SURFACES, sizeof(c->walls));
+        XDEM_HIP(c, hipMemcpyAsync(c->d_walls, c->walls, sizeof(Surface) * XDEM_MAX_SURFACES, hipMemcpyHostToDevice,
+                                   c->stream));
+      }
+      t_done = times[upto - done];
       done = upto;
     }
   }
+  c->physical_time = t_done;
   c->trig_slot = 0;
   if ((rc = flush_kick(c))) return rc;
   if (c->timing && (rc = harvest_timers(c))) return rc;

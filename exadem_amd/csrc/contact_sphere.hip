@@ -347,7 +347,7 @@ int launch_step(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, 
   if (form == XDEM_FORM_FP32) return launch_step_fp32(c, cp, cpd, dt, gravity_on, g, thr2, step_tag, input);
   int rc = set_history_form(c, false);
   if (rc) return rc;
-  if (cp.kr != 0.0 || cpd.kr != 0.0) c->kr_used = true;
+  if ((cp.kr != 0.0 || cpd.kr != 0.0) && (rc = use_kr(c))) return rc;
   const bool planes = c->nwalls > 0 && c->wl_valid;
   const bool meshed = c->has_mesh && c->ml_valid;
   const bool walls = planes || meshed;
@@ -356,7 +356,7 @@ int launch_step(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, 
   a.s = live(c);
   a.nl = c->nl[c->nl_cur];
   if (planes) a.wl = c->wl[c->wl_cur];
-  a.walls = c->d_walls;
+  a.walls = walls_for_pass(c);
   a.nw = planes ? c->nwalls : 0;
   if (meshed) {
     a.mesh = c->mesh;
@@ -399,7 +399,7 @@ int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd,
   if (form == XDEM_FORM_FP32) return launch_force_fp32(c, cp, cpd, dt, fused, gravity_on, g, guarded, step_tag);
   int rc = set_history_form(c, false);
   if (rc) return rc;
-  if (cp.kr != 0.0 || cpd.kr != 0.0) c->kr_used = true;
+  if ((cp.kr != 0.0 || cpd.kr != 0.0) && (rc = use_kr(c))) return rc;
   const bool planes = c->nwalls > 0 && c->wl_valid;
   const bool meshed = c->has_mesh && c->ml_valid;
   const bool walls = planes || meshed;
@@ -408,7 +408,7 @@ int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd,
   a.s = live(c);
   a.nl = c->nl[c->nl_cur];
   if (planes) a.wl = c->wl[c->wl_cur];
-  a.walls = c->d_walls;
+  a.walls = walls_for_pass(c);
   a.nw = planes ? c->nwalls : 0;
   if (meshed) {
     a.mesh = c->mesh;

@@ -392,7 +392,7 @@ int launch_force_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams&
                       bool gravity_on, V3 g, bool guarded, int32_t step_tag) {
   int rc = set_history_form(c, true);
   if (rc) return rc;
-  if (cp.kr != 0.0 || cpd.kr != 0.0) c->kr_used = true;
+  if ((cp.kr != 0.0 || cpd.kr != 0.0) && (rc = use_kr(c))) return rc;
   const bool planes = c->nwalls > 0 && c->wl_valid;
   const bool meshed = c->has_mesh && c->ml_valid;
   const bool walls = planes || meshed;
@@ -401,7 +401,7 @@ int launch_force_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams&
   a.s = live(c);
   a.nl = c->nl[c->nl_cur];
   if (planes) a.wl = c->wl[c->wl_cur];
-  a.walls = c->d_walls;
+  a.walls = walls_for_pass(c);
   a.nw = planes ? c->nwalls : 0;
   if (meshed) {
     a.mesh = c->mesh;
@@ -437,7 +437,7 @@ int launch_step_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams& 
                      double thr2, int32_t step_tag, Core** input) {
   int rc = set_history_form(c, true);
   if (rc) return rc;
-  if (cp.kr != 0.0 || cpd.kr != 0.0) c->kr_used = true;
+  if ((cp.kr != 0.0 || cpd.kr != 0.0) && (rc = use_kr(c))) return rc;
   const bool planes = c->nwalls > 0 && c->wl_valid;
   const bool meshed = c->has_mesh && c->ml_valid;
   const bool walls = planes || meshed;
@@ -446,7 +446,7 @@ int launch_step_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams& 
   a.s = live(c);
   a.nl = c->nl[c->nl_cur];
   if (planes) a.wl = c->wl[c->wl_cur];
-  a.walls = c->d_walls;
+  a.walls = walls_for_pass(c);
   a.nw = planes ? c->nwalls : 0;
   if (meshed) {
     a.mesh = c->mesh;

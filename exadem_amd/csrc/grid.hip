@@ -24,28 +24,46 @@ __device__ inline int cell_coord(double x, double lo, double h, int dims) {
   return k;
 }
 
+// Periodic wrap and cell histogram; also the largest radius (dem_rcut_max, src/field/dem_rcut_max.cpp:55-65)
+// as IEEE bits (radii are positive, so the bit pattern orders like the value): one atomic per block.
 __global__ __launch_bounds__(256) void k_wrap_bin(int64_t n, PState s, Grid g, const uint32_t* __restrict__ rank,
-                                                  uint32_t* __restrict__ cell_of, uint32_t* __restrict__ count) {
+                                                  uint32_t* __restrict__ cell_of, uint32_t* __restrict__ count,
+                                                  unsigned long long* __restrict__ rmax_bits) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  d4 xr = s.core[i].xr;
-  double c[3] = {xr.x, xr.y, xr.z};
-  int ci[3];
+  unsigned long long bits = 0;
+  if (i < n) {
+    d4 xr = s.core[i].xr;
+    double c[3] = {xr.x, xr.y, xr.z};
+    int ci[3];
 #pragma unroll
-  for (int a = 0; a < 3; a++) {
-    if (g.per[a]) {
-      if (c[a] < g.lo[a]) c[a] += g.L[a];
-      else if (c[a] >= g.lo[a] + g.L[a]) c[a] -= g.L[a];
+    for (int a = 0; a < 3; a++) {
+      if (g.per[a]) {
+        if (c[a] < g.lo[a]) c[a] += g.L[a];
+        else if (c[a] >= g.lo[a] + g.L[a]) c[a] -= g.L[a];
+      }
+      ci[a] = cell_coord(c[a], g.lo[a], g.h[a], g.dims[a]);
     }
-    ci[a] = cell_coord(c[a], g.lo[a], g.h[a], g.dims[a]);
+    bits = (unsigned long long)__double_as_longlong(xr.w);
+    xr.x = c[0];
+    xr.y = c[1];
+    xr.z = c[2];
+    s.core[i].xr = xr;
+    const uint32_t cell = rank[((int64_t)ci[2] * g.dims[1] + ci[1]) * g.dims[0] + ci[0]];
+    cell_of[i] = cell;
+    atomicAdd(&count[cell], 1u);
   }
-  xr.x = c[0];
-  xr.y = c[1];
-  xr.z = c[2];
-  s.core[i].xr = xr;
-  const uint32_t cell = rank[((int64_t)ci[2] * g.dims[1] + ci[1]) * g.dims[0] + ci[0]];
-  cell_of[i] = cell;
-  atomicAdd(&count[cell], 1u);
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long other = __shfl_down(bits, o, 64);
+    bits = other > bits ? other : bits;
+  }
+  __shared__ unsigned long long wmax[4];
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = bits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = wmax[0];
+    for (int w = 1; w < 4; w++) m = wmax[w] > m ? wmax[w] : m;
+    if (m) atomicMax(rmax_bits, m);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_scatter(int64_t n, const uint32_t* __restrict__ cell_of,
@@ -94,20 +112,21 @@ __global__ __launch_bounds__(256) void k_gather(int64_t n, const uint32_t* __res
   dst.bq[i] = c.q;
 }
 
-__global__ __launch_bounds__(256) void k_max_radius(int64_t n, PState s, uint64_t* __restrict__ out_bits) {
+// Positions (with radius) and ids in Morton order, for the neighbour build; the full particle state is
+// reordered once, by the neighbour build, into its final (balanced) order.
+__global__ __launch_bounds__(256) void k_morton_pack(int64_t n, const uint32_t* __restrict__ perm, const Core* core,
+                                                     const uint64_t* __restrict__ id, d4* __restrict__ mxr,
+                                                     uint64_t* __restrict__ mid) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double r = 0.0;
-  if (i < n) r = s.core[i].xr.w;
-  // radii are positive: the IEEE bit pattern orders like the value
-  unsigned long long bits = (unsigned long long)__double_as_longlong(r);
-  for (int o = 32; o > 0; o >>= 1) {
-    unsigned long long other = __shfl_down(bits, o, 64);
-    bits = other > bits ? other : bits;
-  }
-  if ((threadIdx.x & 63) == 0) atomicMax((unsigned long long*)out_bits, bits);
+  if (i >= n) return;
+  const uint32_t o = perm[i];
+  mxr[i] = core[o].xr;
+  mid[i] = id[o];
 }
 
 int do_move_particles(xdem_ctx* c) {
+  // backup_drivers (config_move_particles.msp:149-153): the reference of driver_displ_over
+  for (int k = 0; k < XDEM_MAX_SURFACES; k++) c->walls_bk[k] = c->walls[k];
   const int64_t n = c->n;
   const Grid g = c->grid;
   if (n == 0) {
@@ -115,9 +134,10 @@ int do_move_particles(xdem_ctx* c) {
     return XDEM_OK;
   }
   XDEM_HIP(c, hipMemsetAsync(c->cell_count, 0, sizeof(uint32_t) * (g.ncells + 1), c->stream));
+  XDEM_HIP(c, hipMemsetAsync(c->d_rmax, 0, sizeof(uint64_t), c->stream));
   PState src = live(c);
   hipLaunchKernelGGL(k_wrap_bin, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, src, g, c->cell_rank,
-                     c->cell_of, c->cell_count);
+                     c->cell_of, c->cell_count, (unsigned long long*)c->d_rmax);
   int rc = scan_u32(c, c->cell_count, c->cell_start, g.ncells);
   if (rc) return rc;
   XDEM_HIP(c, hipMemsetAsync(c->cell_count, 0, sizeof(uint32_t) * (g.ncells + 1), c->stream));
@@ -125,11 +145,15 @@ int do_move_particles(xdem_ctx* c) {
                      c->cell_count, c->perm);
   hipLaunchKernelGGL(k_cell_sort, dim3(grid_blocks(g.ncells, 256)), dim3(256), 0, c->stream, g.ncells,
                      c->cell_start, src.id, c->perm);
-  PState dst = c->ps[1 - c->cur];
-  hipLaunchKernelGGL(k_gather, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, c->perm, src, dst);
+  if (c->cap_morton < c->cap) {
+    if ((rc = dev_realloc(c, &c->mxr, c->cap)) || (rc = dev_realloc(c, &c->mid, c->cap))) return rc;
+    c->cap_morton = c->cap;
+  }
+  hipLaunchKernelGGL(k_morton_pack, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, c->perm, src.core, src.id,
+                     c->mxr, c->mid);
   XDEM_HIP(c, hipGetLastError());
-  c->cur = 1 - c->cur;
-  // the list built for the previous order can seed the history carry-over through perm
+  // perm: Morton position -> current index. The particles keep their order until the neighbour build
+  // moves them into the final one; the list built for the current order seeds the history carry-over.
   c->nl_prev_valid = c->nl_valid;
   c->nl_valid = false;
   c->perm_valid = true;
@@ -138,83 +162,152 @@ int do_move_particles(xdem_ctx* c) {
 }
 
 // ---------------------------------------------------------------- neighbour lists
-struct NbhArgs {
+// Candidate lists in one pass, one wavefront per cell (cells in Morton rank order). All particles of a
+// cell share the 27-cell stencil: lanes 0..26 look up the stencil cells in stream order (dz, dy, dx in
+// -1..1, dx fastest; periodic images wrap with their image code) and a wavefront prefix sum numbers the
+// candidates t = 0 .. T-1 in that order (stencil cell, then cell order). For each particle x of the cell,
+// 64 candidates at a time are tested with the DEM filter of chunk_neighbors_contact.cpp:45-57
+// (0 < d^2 < (r_x + r_y + rcut_inc)^2 and d^2 < nbh_dist^2, strict), and a ballot + prefix count writes the
+// accepted ones to x's row of `tmp` in candidate order, which is the stream order. Rows hold at most
+// `cap` entries; the count is exact, so a longer row is detected and the pass repeated with a larger cap.
+// Entry: partner index (Morton order) | image code << 25 | owner << 31 (owner: lower id, or a periodic
+// image, as nbh_sphere's sym mode keeps it; polyhedra: lower id always, nbh_polyhedron.cpp:234-236).
+struct CellNbhArgs {
   Grid g;
   double rcut_inc;
   double rcut2;
   const uint32_t* cell_start;  // indexed by Morton rank
-  const uint32_t* rank;
-  const Core* core;
-  const uint64_t* id;
-  int64_t n;
-  int poly;  // polyhedra: owner = lower id always (nbh_polyhedron.cpp:234-236)
+  const uint32_t* rank;        // linear cell -> Morton rank
+  const uint32_t* lin;         // Morton rank -> linear cell
+  const d4* xr;                // positions and radii, Morton order
+  const uint64_t* id;          // ids, Morton order
+  int64_t ncells;
+  int poly;
+  uint32_t* tmp;  // [n * cap]
+  uint32_t cap;
+  uint32_t* cnt;  // [n] list lengths
 };
 
-// Visits the stencil of particle x in stream order: dz, dy, dx in -1..1 (dx fastest), then the
-// particles of each neighbour cell in cell order. fn(y, code) is called for each accepted partner.
-template <typename F>
-__device__ inline void for_each_neighbour(const NbhArgs& a, int64_t x, F&& fn) {
-  const d4 px = a.core[x].xr;
-  const int cx = cell_coord(px.x, a.g.lo[0], a.g.h[0], a.g.dims[0]);
-  const int cy = cell_coord(px.y, a.g.lo[1], a.g.h[1], a.g.dims[1]);
-  const int cz = cell_coord(px.z, a.g.lo[2], a.g.h[2], a.g.dims[2]);
-  for (int dz = -1; dz <= 1; dz++) {
-    int nz = cz + dz, sz = 0;
-    if (nz < 0) {
-      if (!a.g.per[2]) continue;
-      nz += a.g.dims[2];
-      sz = -1;
-    } else if (nz >= a.g.dims[2]) {
-      if (!a.g.per[2]) continue;
-      nz -= a.g.dims[2];
-      sz = 1;
-    }
-    for (int dy = -1; dy <= 1; dy++) {
-      int ny = cy + dy, sy = 0;
-      if (ny < 0) {
-        if (!a.g.per[1]) continue;
-        ny += a.g.dims[1];
-        sy = -1;
-      } else if (ny >= a.g.dims[1]) {
-        if (!a.g.per[1]) continue;
-        ny -= a.g.dims[1];
-        sy = 1;
-      }
-      for (int dx = -1; dx <= 1; dx++) {
-        int nx = cx + dx, sx = 0;
-        if (nx < 0) {
-          if (!a.g.per[0]) continue;
-          nx += a.g.dims[0];
-          sx = -1;
-        } else if (nx >= a.g.dims[0]) {
-          if (!a.g.per[0]) continue;
-          nx -= a.g.dims[0];
-          sx = 1;
-        }
-        const int code = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
-        const V3 sh = shift_vec(a.g, code);
-        const uint32_t cb = a.rank[((int64_t)nz * a.g.dims[1] + ny) * a.g.dims[0] + nx];
-        const uint32_t b = a.cell_start[cb], e = a.cell_start[cb + 1];
-        for (uint32_t y = b; y < e; y++) {
-          if ((int64_t)y == x && code == kNoShift) continue;
-          const d4 py = a.core[y].xr;
-          const V3 img{py.x + sh.x, py.y + sh.y, py.z + sh.z};
-          const V3 d{px.x - img.x, px.y - img.y, px.z - img.z};
-          const double d2 = dot(d, d);
-          const double rmax = px.w + py.w + a.rcut_inc;
-          if (d2 > 0.0 && d2 < rmax * rmax && d2 < a.rcut2) fn(y, code);
-        }
-      }
-    }
+__device__ inline bool stencil_cell(const Grid& g, int c, int d, int a, int& out, int& sh) {
+  int v = c + d;
+  sh = 0;
+  if (v < 0) {
+    if (!g.per[a]) return false;
+    v += g.dims[a];
+    sh = -1;
+  } else if (v >= g.dims[a]) {
+    if (!g.per[a]) return false;
+    v -= g.dims[a];
+    sh = 1;
   }
+  out = v;
+  return true;
 }
 
-__global__ __launch_bounds__(256) void k_nbh_count(NbhArgs a, uint32_t* __restrict__ cnt) {
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= a.n) return;
-  uint32_t k = 0;
-  for_each_neighbour(a, x, [&](uint32_t, int) { k++; });
-  cnt[x] = k;
+// value of lane j (wavefront-uniform j) in every lane
+__device__ inline uint32_t bcast_u32(uint32_t v, int j) { return (uint32_t)__builtin_amdgcn_readlane((int)v, j); }
+__device__ inline double bcast_f64(double v, int j) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = bcast_u32((uint32_t)b, j), hi = bcast_u32((uint32_t)(b >> 32), j);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ inline uint64_t bcast_u64(uint64_t v, int j) {
+  return ((uint64_t)bcast_u32((uint32_t)(v >> 32), j) << 32) | bcast_u32((uint32_t)v, j);
+}
+
+// Each chunk of 64 candidates is loaded once and tested against every particle of the cell (up to 64 at
+// a time, lane j holding particle j's position and running count), so a candidate record is read once
+// per cell rather than once per particle.
+__global__ __launch_bounds__(256) void k_nbh_cells(CellNbhArgs a) {
+  const int lane = (int)threadIdx.x & 63;
+  const int64_t c = (int64_t)blockIdx.x * 4 + ((int)threadIdx.x >> 6);
+  if (c >= a.ncells) return;
+  const uint32_t xb = a.cell_start[c], xe = a.cell_start[c + 1];
+  if (xb == xe) return;
+  const Grid& g = a.g;
+  const uint32_t li = a.lin[c];
+  const int cx = (int)(li % (uint32_t)g.dims[0]);
+  const int cy = (int)((li / (uint32_t)g.dims[0]) % (uint32_t)g.dims[1]);
+  const int cz = (int)(li / ((uint32_t)g.dims[0] * (uint32_t)g.dims[1]));
+  uint32_t sstart = 0, scnt = 0;
+  int scode = kNoShift;
+  if (lane < 27) {
+    int nx, ny, nz, sx, sy, sz;
+    if (stencil_cell(g, cz, lane / 9 - 1, 2, nz, sz) && stencil_cell(g, cy, (lane / 3) % 3 - 1, 1, ny, sy) &&
+        stencil_cell(g, cx, lane % 3 - 1, 0, nx, sx)) {
+      const uint32_t cb = a.rank[((int64_t)nz * g.dims[1] + ny) * g.dims[0] + nx];
+      sstart = a.cell_start[cb];
+      scnt = a.cell_start[cb + 1] - sstart;
+      scode = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+    }
+  }
+  uint32_t inc = scnt;  // inclusive prefix over the stencil lanes (lanes 27..31 then hold the total)
+  for (int o = 1; o < 32; o <<= 1) {
+    const uint32_t y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  const uint32_t total = __shfl(inc, 26, 64);
+  const uint64_t below = (1ull << lane) - 1ull;
+  for (uint32_t g0 = xb; g0 < xe; g0 += 64) {
+    const int ng = (int)min(64u, xe - g0);
+    d4 mine{0, 0, 0, 0};
+    uint64_t myid = 0;
+    if (lane < ng) {
+      mine = a.xr[g0 + lane];
+      myid = a.id[g0 + lane];
+    }
+    uint32_t run = 0;  // lane j: entries of particle g0 + j so far
+    for (uint32_t t0 = 0; t0 < total; t0 += 64) {
+      const uint32_t t = t0 + (uint32_t)lane;
+      // stencil cell holding candidate t: the first lane whose inclusive count exceeds t (branch-free
+      // binary search over lanes 0..31, every lane taking part)
+      int pos = 0;
+#pragma unroll
+      for (int step = 16; step >= 1; step >>= 1) {
+        const uint32_t v = __shfl(inc, pos + step - 1, 64);
+        pos += v <= t ? step : 0;
+      }
+      const int sc = pos < 27 ? pos : 26;
+      const uint32_t first = __shfl(inc, sc, 64) - __shfl(scnt, sc, 64);
+      const uint32_t ystart = __shfl(sstart, sc, 64);
+      const int code = __shfl(scode, sc, 64);
+      const bool valid = t < total;
+      const uint32_t y = ystart + (t - first);
+      d4 py{0, 0, 0, 0};
+      uint64_t idy = 0;
+      if (valid) {
+        py = a.xr[y];
+        idy = a.id[y];
+      }
+      const V3 sh = shift_vec(g, code);
+      const V3 img{py.x + sh.x, py.y + sh.y, py.z + sh.z};
+      for (int j = 0; j < ng; j++) {
+        const double pxx = bcast_f64(mine.x, j), pxy = bcast_f64(mine.y, j), pxz = bcast_f64(mine.z, j);
+        const double pxw = bcast_f64(mine.w, j);
+        const uint32_t x = g0 + (uint32_t)j;
+        bool acc = false;
+        if (valid && !(y == x && code == kNoShift)) {
+          const V3 d{pxx - img.x, pxy - img.y, pxz - img.z};
+          const double d2 = dot(d, d);
+          const double rmax = pxw + py.w + a.rcut_inc;
+          acc = d2 > 0.0 && d2 < rmax * rmax && d2 < a.rcut2;
+        }
+        const uint64_t m = __ballot(acc);
+        if (m) {
+          const uint32_t base = bcast_u32(run, j);
+          const uint64_t idx = bcast_u64(myid, j);
+          if (acc) {
+            const bool owner = a.poly ? (idx < idy) : !(idx >= idy && code == kNoShift);
+            const uint32_t p = base + (uint32_t)__popcll(m & below);
+            if (p < a.cap)
+              a.tmp[(uint64_t)x * a.cap + p] = y | ((uint32_t)code << kShiftShift) | (owner ? kOwnerBit : 0u);
+          }
+          run += lane == j ? (uint32_t)__popcll(m) : 0u;
+        }
+      }
+    }
+    if (lane < ng) a.cnt[g0 + lane] = run;
+  }
 }
 
 // Work balancing: inside each group of kBalance consecutive particles (a compact Morton region),
@@ -252,9 +345,11 @@ __global__ __launch_bounds__(64) void k_balance(int64_t n, const uint32_t* __res
   }
 }
 
+// List lengths in the balanced order, the per-wave row count of the ELL blocks, and the permutation from
+// the balanced order to the particles' current order (perm_new = perm_old o perm2).
 __global__ __launch_bounds__(256) void k_balance_apply(int64_t n, const uint32_t* __restrict__ perm2,
                                                        const uint32_t* __restrict__ cnt_m, uint32_t* __restrict__ cnt,
-                                                       uint32_t* __restrict__ kmax, uint32_t* __restrict__ max_rows,
+                                                       uint32_t* __restrict__ kmax,
                                                        const uint32_t* __restrict__ perm_old,
                                                        uint32_t* __restrict__ perm_new) {
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -263,7 +358,7 @@ __global__ __launch_bounds__(256) void k_balance_apply(int64_t n, const uint32_t
     const uint32_t m = perm2[x];
     k = cnt_m[m];
     cnt[x] = k;
-    if (perm_old) perm_new[x] = perm_old[m];
+    perm_new[x] = perm_old[m];
   }
   uint32_t mx = k;
   for (int o = 32; o > 0; o >>= 1) {
@@ -271,31 +366,87 @@ __global__ __launch_bounds__(256) void k_balance_apply(int64_t n, const uint32_t
     mx = t > mx ? t : mx;
   }
   const int64_t w = x >> 6;
-  if ((threadIdx.x & 63) == 0 && (w << 6) < n) {
-    kmax[w] = mx;
-    atomicMax(max_rows, mx);
+  if ((threadIdx.x & 63) == 0 && (w << 6) < n) kmax[w] = mx;
+}
+
+// out[0] = max of v[0..m) (one block)
+__global__ __launch_bounds__(1024) void k_max_u32(int64_t m, const uint32_t* __restrict__ v, uint32_t* __restrict__ out) {
+  __shared__ uint32_t red[16];
+  uint32_t mx = 0;
+  for (int64_t i = threadIdx.x; i < m; i += blockDim.x) mx = v[i] > mx ? v[i] : mx;
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t t = __shfl_down(mx, o, 64);
+    mx = t > mx ? t : mx;
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) mx = red[w] > mx ? red[w] : mx;
+    out[0] = mx;
   }
 }
 
-__global__ __launch_bounds__(256) void k_nbh_fill(NbhArgs a, const uint32_t* __restrict__ inv2,
-                                                  const uint64_t* __restrict__ base, uint32_t* __restrict__ word,
-                                                  unsigned long long* __restrict__ pairs) {
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t owned = 0;
-  if (x < a.n) {
-    const uint32_t xn = inv2[x];
-    const uint64_t b = base[xn >> 6] + (uint64_t)(xn & 63);
-    const uint64_t idx = a.id[x];
-    uint32_t k = 0;
-    for_each_neighbour(a, x, [&](uint32_t y, int code) {
-      const bool owner = a.poly ? (idx < a.id[y]) : !(idx >= a.id[y] && code == kNoShift);
-      word[b + (uint64_t)k * kWave] = inv2[y] | ((uint32_t)code << kShiftShift) | (owner ? kOwnerBit : 0u);
-      owned += owner ? 1u : 0u;
-      k++;
-    });
+// out[0] = sum of v[0..m) (one block)
+__global__ __launch_bounds__(1024) void k_sum_u32(int64_t m, const uint32_t* __restrict__ v,
+                                                  unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long red[16];
+  unsigned long long sm = 0;
+  for (int64_t i = threadIdx.x; i < m; i += blockDim.x) sm += v[i];
+  for (int o = 32; o > 0; o >>= 1) sm += __shfl_down(sm, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sm;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) sm += red[w];
+    out[0] = sm;
   }
+}
+
+// Rows of tmp (Morton order) into the per-wavefront ELL blocks of the balanced order: particle xn takes
+// the row of perm2[xn]; partner indices are mapped to the balanced order through inv2.
+__global__ __launch_bounds__(256) void k_nbh_emit(int64_t n, const uint32_t* __restrict__ perm2,
+                                                  const uint32_t* __restrict__ inv2, const uint32_t* __restrict__ tmp,
+                                                  uint32_t cap, const uint32_t* __restrict__ cnt,
+                                                  const uint64_t* __restrict__ base,
+                                                  const uint32_t* __restrict__ kmax, uint32_t* __restrict__ word,
+                                                  uint32_t* __restrict__ sx, uint32_t* __restrict__ blk_owned) {
+  const int64_t xn = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t owned = 0;
+  if (xn < n) {
+    const uint32_t* row = tmp + (uint64_t)perm2[xn] * cap;
+    const uint32_t k = cnt[xn];
+    const uint64_t b = base[xn >> 6] + (uint64_t)(xn & 63);
+    const uint32_t rows = kmax[xn >> 6];
+    for (uint32_t e = 0; e < rows; e++) sx[b + (uint64_t)e * kWave] = e < k ? (uint32_t)xn : 0xffffffffu;
+    uint32_t e = 0;
+    for (; e + 4 <= k; e += 4) {  // four rows' words and partner indices in flight
+      uint32_t w[4], q[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) w[u] = row[e + u];
+#pragma unroll
+      for (int u = 0; u < 4; u++) q[u] = inv2[w[u] & kIdxMask];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        word[b + (uint64_t)(e + u) * kWave] = q[u] | (w[u] & ~kIdxMask);
+        owned += w[u] >> 31;
+      }
+    }
+    for (; e < k; e++) {
+      const uint32_t w = row[e];
+      word[b + (uint64_t)e * kWave] = inv2[w & kIdxMask] | (w & ~kIdxMask);
+      owned += w >> 31;
+    }
+  } else if ((xn >> 6) == ((n - 1) >> 6)) {
+    // lanes past the last particle in the last wave: their slots of the ELL block are padding
+    const uint64_t b = base[xn >> 6] + (uint64_t)(xn & 63);
+    const uint32_t rows = kmax[xn >> 6];
+    for (uint32_t e = 0; e < rows; e++) sx[b + (uint64_t)e * kWave] = 0xffffffffu;
+  }
+  // owner entries of the block (pairs of the half list), summed by k_sum_u32
+  __shared__ uint32_t red[4];
   for (int o = 32; o > 0; o >>= 1) owned += __shfl_down(owned, o, 64);
-  if ((threadIdx.x & 63) == 0 && owned) atomicAdd(pairs, (unsigned long long)owned);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = owned;
+  __syncthreads();
+  if (threadIdx.x == 0) blk_owned[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
 template <typename T>
@@ -312,14 +463,12 @@ int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist) {
   if (!c->cells_valid) return set_err(c, XDEM_ERR_STATE, "chunk_neighbors_contact: run move_particles first");
   const int64_t n = c->n;
   const Grid g = c->grid;
+  int rc;
   if (n > 0 && nbh_dist <= 0) {
-    // dem_rcut_max (src/field/dem_rcut_max.cpp:55-65) + nbh_dist = rcut_max + rcut_inc
-    uint64_t* bits = (uint64_t*)c->d_u32tmp;
-    XDEM_HIP(c, hipMemsetAsync(bits, 0, sizeof(uint64_t), c->stream));
-    hipLaunchKernelGGL(k_max_radius, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, live(c), bits);
+    // dem_rcut_max (src/field/dem_rcut_max.cpp:55-65): the largest radius, reduced by move_particles'
+    // binning pass; nbh_dist = rcut_max + rcut_inc
     uint64_t hb = 0;
-    int rc = read_u64(c, bits, &hb);
-    if (rc) return rc;
+    if ((rc = read_u64(c, c->d_rmax, &hb))) return rc;
     double rmax;
     std::memcpy(&rmax, &hb, sizeof(double));
     nbh_dist = 2.0 * rmax + rcut_inc;
@@ -339,28 +488,46 @@ int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist) {
     c->nl_valid = true;
     return XDEM_OK;
   }
-  NbhArgs a{g, rcut_inc, nbh_dist * nbh_dist, c->cell_start, c->cell_rank, live(c).core, live(c).id, n,
-            c->nshapes > 0 ? 1 : 0};
   const int64_t nw = (n + kWave - 1) / kWave;
   uint32_t* max_rows = c->d_u32tmp + 2;
   uint32_t* cnt_m = c->cell_of;  // list lengths in Morton order (cell_of is free after the binning)
   uint32_t* perm2 = reinterpret_cast<uint32_t*>(c->sort_key);
   uint32_t* inv2 = perm2 + c->cap;
   uint32_t* perm_tmp = c->perm_tmp;
-  XDEM_HIP(c, hipMemsetAsync(max_rows, 0, sizeof(uint32_t), c->stream));
-  hipLaunchKernelGGL(k_nbh_count, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a, cnt_m);
+  if (c->nbh_tmp_cap == 0 || c->nbh_tmp_n < c->cap) {
+    const uint32_t rows = c->nbh_tmp_cap ? c->nbh_tmp_cap : 64u;
+    if ((rc = grow(c, &c->nbh_tmp, (int64_t)rows * c->cap))) return rc;
+    c->nbh_tmp_cap = rows;
+    c->nbh_tmp_n = c->cap;
+  }
+  CellNbhArgs a{g, rcut_inc, nbh_dist * nbh_dist, c->cell_start, c->cell_rank, c->cell_lin, c->mxr, c->mid,
+                g.ncells, c->nshapes > 0 ? 1 : 0, c->nbh_tmp, c->nbh_tmp_cap, cnt_m};
+  const dim3 cgrid((uint32_t)((g.ncells + 3) / 4));
+  hipLaunchKernelGGL(k_nbh_cells, cgrid, dim3(256), 0, c->stream, a);
   hipLaunchKernelGGL(k_balance, dim3(grid_blocks(n, kBalance)), dim3(64), 0, c->stream, n, cnt_m, perm2, inv2);
   hipLaunchKernelGGL(k_balance_apply, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, perm2, cnt_m, L.cnt,
-                     L.kmax, max_rows, c->perm_valid ? c->perm : nullptr, perm_tmp);
-  int rc = scan_u64_from_u32(c, L.kmax, L.base, nw, (uint32_t)kWave);
-  if (rc) return rc;
+                     L.kmax, c->perm, perm_tmp);
+  hipLaunchKernelGGL(k_max_u32, dim3(1), dim3(1024), 0, c->stream, nw, L.kmax, max_rows);
+  if ((rc = scan_u64_from_u32(c, L.kmax, L.base, nw, (uint32_t)kWave))) return rc;
   XDEM_HIP(c, hipMemcpyAsync(c->h_u32tmp + 2, max_rows, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
   uint64_t slots = 0;
   if ((rc = read_u64(c, L.base + nw, &slots))) return rc;
   c->nl_max_rows = (int)c->h_u32tmp[2];
+  if ((uint32_t)c->nl_max_rows > c->nbh_tmp_cap) {
+    // a row outgrew the candidate buffer: redo the pass with rows of the next power of two
+    uint32_t rows = 64;
+    while (rows < (uint32_t)c->nl_max_rows) rows <<= 1;
+    if ((rc = grow(c, &c->nbh_tmp, (int64_t)rows * c->cap))) return rc;
+    c->nbh_tmp_cap = rows;
+    c->nbh_tmp_n = c->cap;
+    a.tmp = c->nbh_tmp;
+    a.cap = rows;
+    hipLaunchKernelGGL(k_nbh_cells, cgrid, dim3(256), 0, c->stream, a);
+  }
   if ((int64_t)slots > L.cap_slots) {
     const int64_t cap = (int64_t)(slots * 1.25) + 4096;
-    if ((rc = grow(c, &L.word, cap)) || (rc = grow(c, &L.hx, cap)) || (rc = grow(c, &L.hy, cap)) ||
+    if ((rc = grow(c, &L.word, cap)) || (rc = grow(c, &L.sx, cap)) || (rc = grow(c, &L.hx, cap)) ||
+        (rc = grow(c, &L.hy, cap)) ||
         (rc = grow(c, &L.hz, cap)) || (rc = grow(c, &L.mhx, cap)) || (rc = grow(c, &L.mhy, cap)) ||
         (rc = grow(c, &L.mhz, cap)) || (rc = grow(c, &L.fx32, cap)) || (rc = grow(c, &L.fy32, cap)) ||
         (rc = grow(c, &L.fz32, cap)) || (rc = grow(c, &L.mfx32, cap)) || (rc = grow(c, &L.mfy32, cap)) ||
@@ -370,14 +537,19 @@ int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist) {
   }
   L.slots = (int64_t)slots;
   unsigned long long* pairs = (unsigned long long*)c->d_u32tmp;
-  XDEM_HIP(c, hipMemsetAsync(pairs, 0, sizeof(uint64_t), c->stream));
-  hipLaunchKernelGGL(k_nbh_fill, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a, inv2, L.base, L.word,
-                     pairs);
-  // move every field into the balanced order; compose the history permutation (new -> before binning)
+  const int64_t nblk = grid_blocks(n, 256);
+  if (nblk > c->cap_blk) {
+    if ((rc = grow(c, &c->blk_u32, nblk))) return rc;
+    c->cap_blk = nblk;
+  }
+  hipLaunchKernelGGL(k_nbh_emit, dim3((uint32_t)nblk), dim3(256), 0, c->stream, n, perm2, inv2, c->nbh_tmp,
+                     c->nbh_tmp_cap, L.cnt, L.base, L.kmax, L.word, L.sx, c->blk_u32);
+  hipLaunchKernelGGL(k_sum_u32, dim3(1), dim3(1024), 0, c->stream, nblk, c->blk_u32, pairs);
+  // every field moves once, from the current order into the balanced one (perm_tmp: balanced ->
+  // current); perm keeps that map for the history carry-over
   PState dst = c->ps[1 - c->cur];
-  hipLaunchKernelGGL(k_gather, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, perm2, live(c), dst);
-  if (c->perm_valid)
-    XDEM_HIP(c, hipMemcpyAsync(c->perm, perm_tmp, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_gather, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, perm_tmp, live(c), dst);
+  XDEM_HIP(c, hipMemcpyAsync(c->perm, perm_tmp, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, c->stream));
   XDEM_HIP(c, hipGetLastError());
   c->cur = 1 - c->cur;
   c->cells_valid = false;  // cell ranges refer to the Morton order
@@ -392,13 +564,6 @@ int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist) {
 }
 
 // ---------------------------------------------------------------- interaction update (nbh_sphere)
-struct CarryArgs {
-  int64_t n;
-  NbrList nl;        // new list
-  NbrList ol;        // previous list (valid when have_old)
-  const uint32_t* perm;
-  int have_old;
-};
 
 template <typename T>
 struct HistView {
@@ -411,43 +576,65 @@ __host__ __device__ inline HistView<float> hist_view(const NbrList& l, float*) {
   return {l.fx32, l.fy32, l.fz32, l.mfx32, l.mfy32, l.mfz32};
 }
 
-// One thread per (particle, entry). New entries start from zero history; an entry whose key
-// (owner id, partner id) matches an active owner entry of the previous list inherits its friction
-// and moment (interaction_manager.hpp:88-112 with placeholder_interaction.hpp:398-408).
-template <typename T>
-__global__ __launch_bounds__(256) void k_hist_carry(CarryArgs a) {
+// History carry-over by scatter from the previous list: thread per previous particle, for each of its
+// owner entries with history (interaction_manager.hpp:88-112 extracts the active ones; update() copies
+// the payload to the new interaction with the same key, placeholder_interaction.hpp:398-408). The key
+// is (owner id, partner id): the new owner entry of the same direction takes the history, and so does
+// the partner's entry of the pair when it is not an owner entry (its copy of the pair). A new entry has
+// at most one such source, so every target is written by one thread. Entries without a source keep
+// the active bit clear, which is what every reader of the history checks; their history is not
+// written.
+__global__ __launch_bounds__(256) void k_invert(int64_t n, const uint32_t* __restrict__ perm,
+                                                uint32_t* __restrict__ inv) {
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= a.n) return;
-  const HistView<T> nh = hist_view(a.nl, (T*)nullptr), oh = hist_view(a.ol, (T*)nullptr);
-  const uint32_t cnt = a.nl.cnt[x];
-  const uint64_t b = a.nl.base[x >> 6] + (uint64_t)(x & 63);
-  for (uint32_t k = 0; k < cnt; k++) {
-    const uint64_t s = b + (uint64_t)k * kWave;
-    uint32_t wd = a.nl.word[s] & ~kActiveBit;
-    T hx = 0, hy = 0, hz = 0, mx = 0, my = 0, mz = 0;
-    bool act = false;
-    if (a.have_old) {
-      const uint32_t y = wd & kIdxMask;
-      const bool owner = (wd & kOwnerBit) != 0;
-      const uint32_t o = owner ? (uint32_t)x : y;
-      const uint32_t p = owner ? y : (uint32_t)x;
-      const uint32_t oo = a.perm[o], op = a.perm[p];
-      const uint32_t ocnt = a.ol.cnt[oo];
-      const uint64_t ob = a.ol.base[oo >> 6] + (uint64_t)(oo & 63);
-      for (uint32_t k2 = 0; k2 < ocnt; k2++) {
-        const uint64_t s2 = ob + (uint64_t)k2 * kWave;
-        const uint32_t w2 = a.ol.word[s2];
-        if ((w2 & kOwnerBit) && (w2 & kActiveBit) && (w2 & kIdxMask) == op) {
-          hx = oh.x[s2]; hy = oh.y[s2]; hz = oh.z[s2];
-          mx = oh.mx[s2]; my = oh.my[s2]; mz = oh.mz[s2];
-          act = true;
-          break;
-        }
-      }
-    }
-    nh.x[s] = hx; nh.y[s] = hy; nh.z[s] = hz;
-    nh.mx[s] = mx; nh.my[s] = my; nh.mz[s] = mz;
-    a.nl.word[s] = wd | (act ? kActiveBit : 0u);
+  if (x < n) inv[perm[x]] = (uint32_t)x;
+}
+
+// slot of the entry of particle x with partner y in list l, or ~0ull
+__device__ inline uint64_t find_entry(const NbrList& l, uint32_t x, uint32_t y) {
+  const uint32_t cnt = l.cnt[x];
+  const uint64_t b = l.base[x >> 6] + (uint64_t)(x & 63);
+  for (uint32_t k = 0; k < cnt; k += 4) {  // four words in flight per step
+    uint32_t w[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) w[u] = k + u < cnt ? l.word[b + (uint64_t)(k + u) * kWave] : 0u;
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (k + u < cnt && (w[u] & kIdxMask) == y) return b + (uint64_t)(k + u) * kWave;
+  }
+  return ~0ull;
+}
+
+// One thread per slot of the previous list; only owner entries with history do work.
+template <typename T, bool KR>
+__global__ __launch_bounds__(256) void k_hist_scatter(int64_t oslots, NbrList nl, NbrList ol,
+                                                      const uint32_t* __restrict__ inv) {
+  const int64_t s2 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s2 >= oslots) return;
+  const uint32_t ox = ol.sx[s2];
+  if (ox == 0xffffffffu) return;
+  const uint32_t w2 = ol.word[s2];
+  if (!(w2 & kOwnerBit) || !(w2 & kActiveBit)) return;
+  const HistView<T> nh = hist_view(nl, (T*)nullptr), oh = hist_view(ol, (T*)nullptr);
+  const uint32_t nx = inv[ox], ny = inv[w2 & kIdxMask];
+  const T hx = oh.x[s2], hy = oh.y[s2], hz = oh.z[s2];
+  T mx = 0, my = 0, mz = 0;
+  if (KR) {
+    mx = oh.mx[s2];
+    my = oh.my[s2];
+    mz = oh.mz[s2];
+  }
+  const uint64_t t1 = find_entry(nl, nx, ny);
+  if (t1 != ~0ull && (nl.word[t1] & kOwnerBit)) {
+    nh.x[t1] = hx; nh.y[t1] = hy; nh.z[t1] = hz;
+    nh.mx[t1] = mx; nh.my[t1] = my; nh.mz[t1] = mz;
+    nl.word[t1] |= kActiveBit;
+  }
+  const uint64_t t2 = find_entry(nl, ny, nx);
+  if (t2 != ~0ull && !(nl.word[t2] & kOwnerBit)) {
+    nh.x[t2] = hx; nh.y[t2] = hy; nh.z[t2] = hz;
+    nh.mx[t2] = mx; nh.my[t2] = my; nh.mz[t2] = mz;
+    nl.word[t2] |= kActiveBit;
   }
 }
 
@@ -490,12 +677,25 @@ int do_nbh_sphere(xdem_ctx* c, double rcut_inc) {
     return XDEM_OK;
   }
   const bool have_old = c->nl_prev_valid && c->perm_valid;
-  CarryArgs a{n, c->nl[c->nl_cur], c->nl[1 - c->nl_cur], c->perm, have_old ? 1 : 0};
-  if (c->hist_fp32)
-    hipLaunchKernelGGL(k_hist_carry<float>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a);
-  else
-    hipLaunchKernelGGL(k_hist_carry<double>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a);
-  XDEM_HIP(c, hipGetLastError());
+  if (have_old) {
+    uint32_t* inv = c->perm_tmp;  // free after the neighbour build
+    hipLaunchKernelGGL(k_invert, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, c->perm, inv);
+    const NbrList& nl = c->nl[c->nl_cur];
+    const NbrList& ol = c->nl[1 - c->nl_cur];
+    const int64_t os = ol.slots;
+    if (os > 0) {
+      const dim3 gs(grid_blocks(os, 256)), bs(256);
+      // moment histories are zero (and not carried) until a rolling stiffness is used
+      if (c->hist_fp32) {
+        if (c->kr_used) hipLaunchKernelGGL((k_hist_scatter<float, true>), gs, bs, 0, c->stream, os, nl, ol, inv);
+        else hipLaunchKernelGGL((k_hist_scatter<float, false>), gs, bs, 0, c->stream, os, nl, ol, inv);
+      } else {
+        if (c->kr_used) hipLaunchKernelGGL((k_hist_scatter<double, true>), gs, bs, 0, c->stream, os, nl, ol, inv);
+        else hipLaunchKernelGGL((k_hist_scatter<double, false>), gs, bs, 0, c->stream, os, nl, ol, inv);
+      }
+      XDEM_HIP(c, hipGetLastError());
+    }
+  }
   // walls
   int rc = alloc_walls(c, c->nwalls);
   if (rc) return rc;
@@ -512,6 +712,20 @@ int do_nbh_sphere(xdem_ctx* c, double rcut_inc) {
   c->nl_prev_valid = false;
   c->perm_valid = false;
   c->hist_ready = true;
+  return XDEM_OK;
+}
+
+// A rolling stiffness comes into use: the moment histories of the current sphere list become live. Until
+// then no pass writes them (their value is zero by definition), so they are cleared here once.
+int use_kr(xdem_ctx* c) {
+  if (c->kr_used) return XDEM_OK;
+  const NbrList& L = c->nl[c->nl_cur];
+  if (c->nl_valid && L.slots > 0) {
+    const size_t b64 = sizeof(double) * (size_t)L.slots, b32 = sizeof(float) * (size_t)L.slots;
+    for (double* p : {L.mhx, L.mhy, L.mhz}) XDEM_HIP(c, hipMemsetAsync(p, 0, b64, c->stream));
+    for (float* p : {L.mfx32, L.mfy32, L.mfz32}) XDEM_HIP(c, hipMemsetAsync(p, 0, b32, c->stream));
+  }
+  c->kr_used = true;
   return XDEM_OK;
 }
 
@@ -628,9 +842,11 @@ static int interactions_rows(xdem_ctx* c, int64_t cap, uint64_t* id_i, uint64_t*
       id_i[k] = ids[x];
       id_j[k] = ids[wd & kIdxMask];
       type[k] = 0;
-      friction[3 * k] = hx[s];
-      friction[3 * k + 1] = hy[s];
-      friction[3 * k + 2] = hz[s];
+      // an entry without the active bit has no history (its memory is not kept at zero)
+      const bool act = (wd & kActiveBit) != 0 && (!moments || c->kr_used);
+      friction[3 * k] = act ? hx[s] : 0.0;
+      friction[3 * k + 1] = act ? hy[s] : 0.0;
+      friction[3 * k + 2] = act ? hz[s] : 0.0;
       k++;
     }
   }

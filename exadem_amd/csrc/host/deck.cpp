@@ -12,7 +12,8 @@
 //              read_xyz (example/spheres/movable-wall input format), polydisperse_pack (this library's
 //              seeded RSA + relaxation packing of the benchmark inputs)
 //   drivers    register_surface, register_cylinder, register_ball, register_stl_mesh (src/driver/register_*.cpp;
-//              STATIONARY motion only, a spinning cylinder or ball through vrot)
+//              STATIONARY, LINEAR_MOTION and SHAKER motions, a spinning cylinder or ball through vrot);
+//              push_f_v_r_driver, push_f_v_driver, driver_displ_over in check_and_update_particles
 //   scheme     combined_compute_prolog, combined_compute_epilog, reset_force_moment, gravity_force,
 //              force_to_accel, particle_displ_over, vertex_displ_over, compute_vertices
 //   lists      migrate_cell_particles_interaction / move_particles (with backup_r),
@@ -69,9 +70,9 @@ struct HostParticles {
 const std::set<std::string> kSkipped = {
     "message", "print_logo_banner", "hw_device_init", "hw_device_finalize", "mpi_comm_world", "init_cuda",
     "finalize_cuda", "io_tree", "io_config", "dump_data_paraview", "final_dump", "default_thermodynamic_state",
-    "default_print_thermodynamic_state", "print_simulation_state", "dt_critical", "next_time_step",
+    "default_print_thermodynamic_state", "print_simulation_state", "dt_critical",
     "start_iteration", "end_iteration", "update_stress_tensor", "freeze", "reine_des_neiges",
-    "push_f_v_r_driver", "push_f_v_driver", "push_av_to_quat_driver", "reset_force_driver", "force_to_accel_driver",
+    "push_av_to_quat_driver", "reset_force_driver", "force_to_accel_driver",
     "init_drivers", "backup_drivers", "driver_vertices", "init_particle_regions", "grid_flavor", "simulation_epilog",
     "compute_loop_prolog", "compute_loop_epilog", "update_interaction_ghost", "rebuild_amr", "backup_r",
     "dem_rcut_max", "nbh_dist", "check_rcut", "reduce_species_after_read", "cost_model", "save_shapes"};
@@ -101,6 +102,7 @@ class Runner {
   HostParticles P_;
   std::vector<std::string> type_names_;
   std::vector<xdem_surface> walls_;
+  std::vector<xdem_driver_motion> motions_;  // params slot of each register_* (parallel to walls_)
   bool uploaded_ = false, poly_ = false;
   double bmin_[3] = {0, 0, 0}, bmax_[3] = {0, 0, 0};
   bool have_bounds_ = false;
@@ -218,6 +220,17 @@ class Runner {
     N["set_fields"] = [this](const Node& s) { set_fields(s); };
     // ---- drivers
     N["register_surface"] = [this](const Node& s) { register_surface(s); };
+    // driver motion (config_numerical_schemes.msp:13-24): push_f_v_r_driver at the physical time of the
+    // step, push_f_v_driver; dt_scale multiplies dt as in the reference operators
+    N["push_f_v_r_driver"] = [this](const Node& s) {
+      double t = 0;
+      check(xdem_physical_time(ctx_, &t), "push_f_v_r_driver");
+      check(xdem_push_f_v_r_driver(ctx_, t, dt() * sdouble(s, "dt_scale", 1.0)), "push_f_v_r_driver");
+    };
+    N["push_f_v_driver"] = [this](const Node& s) {
+      check(xdem_push_f_v_driver(ctx_, dt() * sdouble(s, "dt_scale", 1.0)), "push_f_v_driver");
+    };
+    N["next_time_step"] = [this](const Node&) { check(xdem_next_time_step(ctx_, dt()), "next_time_step"); };
     N["register_cylinder"] = [this](const Node& s) { register_round(s, XDEM_DRIVER_CYLINDER, "register_cylinder"); };
     N["register_ball"] = [this](const Node& s) { register_round(s, XDEM_DRIVER_BALL, "register_ball"); };
     N["register_stl_mesh"] = [this](const Node& s) { register_stl_mesh(s); };
@@ -437,13 +450,62 @@ class Runner {
     }
   }
 
+  // params slot of register_* (Driver_params decode, driver_params.hpp:420-535): STATIONARY, LINEAR_MOTION
+  // {motion_vector, const_vel} and SHAKER {amplitude, omega, shaker_dir (default [0, 0, 1])}, with
+  // motion_start_threshold / motion_end_threshold; other motion types stop the run
+  xdem_driver_motion motion_params(const Node* pr, const char* what) {
+    xdem_driver_motion m{};
+    m.motion_end_threshold = 1e300;
+    m.shaker_dir[2] = 1.0;
+    if (!pr) return m;
+    const Node* t = slot(*pr, "motion_type");
+    const std::string type = t ? t->s : std::string("STATIONARY");
+    auto vec3 = [&](const char* k, double* out) {
+      if (const Node* v = slot(*pr, k)) {
+        const auto x = xdem_host::to_doubles(*v);
+        if (x.size() != 3) fail(XDEM_ERR_ARG, std::string(what) + ": " + k + " needs 3 values");
+        for (int a = 0; a < 3; a++) out[a] = x[a];
+        return true;
+      }
+      return false;
+    };
+    if (type == "STATIONARY") {
+      m.motion_type = XDEM_MOTION_STATIONARY;
+    } else if (type == "LINEAR_MOTION") {
+      m.motion_type = XDEM_MOTION_LINEAR;
+      if (!vec3("motion_vector", m.motion_vector)) fail(XDEM_ERR_ARG, std::string(what) + ": motion_vector required");
+      m.const_vel = sdouble(*pr, "const_vel", 0.0);
+    } else if (type == "SHAKER") {
+      m.motion_type = XDEM_MOTION_SHAKER;
+      if (!slot(*pr, "omega") || !slot(*pr, "amplitude"))
+        fail(XDEM_ERR_ARG, std::string(what) + ": SHAKER needs omega and amplitude");
+      m.omega = sdouble(*pr, "omega", 0.0);
+      m.amplitude = sdouble(*pr, "amplitude", 0.0);
+      vec3("shaker_dir", m.shaker_dir);
+    } else {
+      fail(XDEM_ERR_UNSUPPORTED, std::string(what) + ": motion type " + type +
+                                     " is not provided (STATIONARY, LINEAR_MOTION, SHAKER)");
+    }
+    m.motion_start_threshold = sdouble(*pr, "motion_start_threshold", 0.0);
+    m.motion_end_threshold = sdouble(*pr, "motion_end_threshold", 1e300);
+    return m;
+  }
+
+  void put_driver(size_t id, const xdem_surface& w, const xdem_driver_motion& m) {
+    if (walls_.size() <= id) {
+      walls_.resize(id + 1);
+      xdem_driver_motion still{};
+      still.motion_end_threshold = 1e300;
+      still.shaker_dir[2] = 1.0;
+      motions_.resize(id + 1, still);
+    }
+    walls_[id] = w;
+    motions_[id] = m;
+  }
+
   void register_surface(const Node& s) {
     const Node* st = slot(s, "state");
-    const Node* pr = slot(s, "params");
-    if (pr) {
-      const Node* m = slot(*pr, "motion_type");
-      if (m && m->s != "STATIONARY") fail(XDEM_ERR_UNSUPPORTED, "register_surface: only STATIONARY surfaces are provided");
-    }
+    const xdem_driver_motion mo = motion_params(slot(s, "params"), "register_surface");
     if (!st) fail(XDEM_ERR_ARG, "register_surface: state {normal, offset} required");
     xdem_surface w{};
     const Node* nrm = slot(*st, "normal");
@@ -454,10 +516,14 @@ class Runner {
     for (int a = 0; a < 3; a++) w.normal[a] = n[a] / len;
     w.offset = sdouble(*st, "offset", 0.0);
     for (int a = 0; a < 3; a++) w.center[a] = w.normal[a] * w.offset;
+    if (const Node* v = slot(*st, "vel")) {
+      const auto x = xdem_host::to_doubles(*v);
+      if (x.size() != 3) fail(XDEM_ERR_ARG, "register_surface: vel needs 3 values");
+      for (int a = 0; a < 3; a++) w.vel[a] = x[a];
+    }
     const size_t id = (size_t)sdouble(s, "id", (double)walls_.size());
     if (id >= XDEM_MAX_SURFACES) fail(XDEM_ERR_CAPACITY, "register_surface: at most 8 surfaces");
-    if (walls_.size() <= id) walls_.resize(id + 1);
-    walls_[id] = w;
+    put_driver(id, w, mo);
   }
 
   // register_cylinder / register_ball (src/driver/register_cylinder.cpp, register_ball.cpp): state
@@ -465,9 +531,7 @@ class Runner {
   // (cylinder.hpp:40-70, ball.hpp:44-80); STATIONARY motion only
   void register_round(const Node& s, int kind, const char* what) {
     const Node* st = slot(s, "state");
-    if (const Node* pr = slot(s, "params"))
-      if (const Node* m = slot(*pr, "motion_type"))
-        if (m->s != "STATIONARY") fail(XDEM_ERR_UNSUPPORTED, std::string(what) + ": only STATIONARY drivers are provided");
+    const xdem_driver_motion mo = motion_params(slot(s, "params"), what);
     if (!st) fail(XDEM_ERR_ARG, std::string(what) + ": state required");
     xdem_surface w{};
     w.kind = kind;
@@ -489,8 +553,7 @@ class Runner {
     w.radius = sdouble(*st, "radius", 0.0);
     const size_t id = (size_t)sdouble(s, "id", (double)walls_.size());
     if (id >= XDEM_MAX_SURFACES) fail(XDEM_ERR_CAPACITY, std::string(what) + ": at most 8 drivers");
-    if (walls_.size() <= id) walls_.resize(id + 1);
-    walls_[id] = w;
+    put_driver(id, w, mo);
   }
 
   // register_stl_mesh (src/driver/register_stl_mesh.cpp:30-140) for a stationary .stl mesh acting on
@@ -664,7 +727,10 @@ class Runner {
     p.vrotx = P_.wx.data(); p.vroty = P_.wy.data(); p.vrotz = P_.wz.data();
     p.inx = P_.inx.data(); p.iny = P_.iny.data(); p.inz = P_.inz.data();
     check(xdem_particles_upload(ctx_, &p), "upload");
-    if (!walls_.empty()) check(xdem_set_surfaces(ctx_, walls_.data(), (int32_t)walls_.size()), "surfaces");
+    if (!walls_.empty()) {
+      check(xdem_set_surfaces(ctx_, walls_.data(), (int32_t)walls_.size()), "surfaces");
+      check(xdem_set_driver_motion(ctx_, motions_.data(), (int32_t)motions_.size()), "driver motion");
+    }
     if (mesh_.on)
       check(xdem_set_mesh(ctx_, mesh_.id, mesh_.v.data(), (int64_t)mesh_.v.size() / 3, mesh_.e.data(),
                           (int64_t)mesh_.e.size() / 2, mesh_.f.data(), (int64_t)mesh_.f.size() / 3, mesh_.rm,
@@ -685,7 +751,10 @@ class Runner {
     const double thr = 0.5 * rcut_inc();
     if (poly_) check(xdem_vertex_displ_over(ctx_, thr, &over), "vertex_displ_over");
     else check(xdem_particle_displ_over(ctx_, thr, &over), "particle_displ_over");
-    if (over) {
+    // trigger_move_particles = particle_displ or driver_displ (config_move_particles.msp:22-27)
+    int32_t dover = 0;
+    check(xdem_driver_displ_over(ctx_, thr, &dover), "driver_displ_over");
+    if (over || dover) {
       stats.rebuilds++;
       exec("update_particles_full", nullptr, 1);
     } else if (poly_) {
@@ -765,6 +834,7 @@ class Runner {
     for (; t < end; t++) {
       timestep_ = t + 1;  // next_time_step ran after first_iteration and after each iteration
       exec("numerical_scheme", nullptr, 1);
+      check(xdem_next_time_step(ctx_, dt()), "next_time_step");
       stats.iterations++;
     }
   }

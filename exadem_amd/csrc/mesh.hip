@@ -450,7 +450,10 @@ extern "C" int xdem_mesh_interactions_load(xdem_ctx* c, int64_t m, const uint64_
   for (int q = 0; q < 6; q++)
     XDEM_HIP(c, hipMemcpyAsync(dst[q], h[q].data(), sizeof(double) * tot, hipMemcpyHostToDevice, c->stream));
   XDEM_HIP(c, hipStreamSynchronize(c->stream));
-  if (moment) c->kr_used = c->kr_used || std::any_of(moment, moment + 3 * m, [](double v) { return v != 0.0; });
+  if (moment && std::any_of(moment, moment + 3 * m, [](double v) { return v != 0.0; })) {
+    const int rk = use_kr(c);
+    if (rk) return rk;
+  }
   if (matched) *matched = hits;
   return XDEM_OK;
 }

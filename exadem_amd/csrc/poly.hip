@@ -20,6 +20,7 @@
 #include <unordered_map>
 
 #include "contact_law.h"
+#include "mesh.h"
 
 namespace xdem {
 
@@ -636,6 +637,156 @@ int poly_nbh(xdem_ctx* c, double rcut_inc) {
   return XDEM_OK;
 }
 
+// ------------------------------------------------------------------------------------------ classifier
+// classify_interactions (src/interaction/classify_interactions.cpp:52-136, classifier.hpp:33-271): the
+// feature interactions split by type (VV, VE, VF, EE), each type keeping the list order. A wave entry
+// carries what the contact pass needs of a feature, as the reference's per-type containers carry the
+// interaction's fields (classifier_container.hpp:87-300): the feature's index in the lists, its owner,
+// the owner's list word (partner and image code) and the feature word.
+// The split is made per window of 1024 consecutive features: each window's features of one type are
+// contiguous, padded (index ~0u) to whole wavefronts, so every wavefront of the contact pass evaluates one
+// type while all the writes of a window (history, contact terms) stay within the window's cache lines. The
+// wave of a type is its segments across the windows, in window order.
+constexpr int kClassPad = 64;         // wavefront: a padded segment is a whole number of them
+constexpr int kClassPerBlock = 1024;  // window of the partition (256 threads x 4 features)
+
+__device__ inline int feat_type_or_none(int64_t f, int64_t nf, const uint32_t* feat) {
+  return f < nf ? (int)(feat[f] & kFeatTypeMask) : -1;
+}
+
+__global__ __launch_bounds__(256) void k_class_count(int64_t nf, const uint32_t* __restrict__ feat,
+                                                     uint32_t* __restrict__ blk) {
+  __shared__ uint32_t cnt[4];
+  if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t f0 = (int64_t)blockIdx.x * kClassPerBlock + 4 * (int64_t)threadIdx.x;
+  uint32_t loc[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int t = feat_type_or_none(f0 + u, nf, feat);
+#pragma unroll
+    for (int q = 0; q < 4; q++) loc[q] += t == q ? 1u : 0u;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    if (loc[q]) atomicAdd(&cnt[q], loc[q]);
+  __syncthreads();
+  if (threadIdx.x < 4) blk[(int64_t)blockIdx.x * 4 + threadIdx.x] = cnt[threadIdx.x];
+}
+
+// blk: per window and type counts -> start of the window's segment of that type; info: [4] padded total,
+// [5..8] features per type
+__global__ __launch_bounds__(64) void k_class_scan(int64_t nblk, uint32_t* __restrict__ blk,
+                                                   int64_t* __restrict__ info) {
+  if (threadIdx.x != 0) return;
+  int64_t run = 0, size[4] = {0, 0, 0, 0};
+  for (int64_t b = 0; b < nblk; b++)
+    for (int t = 0; t < 4; t++) {
+      const uint32_t v = blk[b * 4 + t];
+      blk[b * 4 + t] = (uint32_t)run;
+      run += (v + kClassPad - 1) / kClassPad * kClassPad;
+      size[t] += v;
+    }
+  for (int t = 0; t < 4; t++) {
+    info[t] = 0;
+    info[5 + t] = size[t];
+  }
+  info[4] = run;
+}
+
+__global__ __launch_bounds__(256) void k_class_scatter(int64_t nf, const uint32_t* __restrict__ feat,
+                                                       const uint32_t* __restrict__ fslot,
+                                                       const uint32_t* __restrict__ fown,
+                                                       const uint32_t* __restrict__ nlword,
+                                                       const uint32_t* __restrict__ blk,
+                                                       const int64_t* __restrict__ info,
+                                                       uint4* __restrict__ wave) {
+  __shared__ uint32_t wsum[4][4];  // [type][wavefront of the block]
+  const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+  const int64_t f0 = (int64_t)blockIdx.x * kClassPerBlock + 4 * (int64_t)threadIdx.x;
+  int ty[4];
+  uint32_t loc[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    ty[u] = feat_type_or_none(f0 + u, nf, feat);
+#pragma unroll
+    for (int q = 0; q < 4; q++) loc[q] += ty[u] == q ? 1u : 0u;
+  }
+  // rank of this thread's first feature of each type among the block's features of that type
+  uint32_t excl[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    uint32_t inc = loc[q];
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += y;
+    }
+    excl[q] = inc - loc[q];
+    if (lane == 63) wsum[q][wv] = inc;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    for (int w = 0; w < wv; w++) excl[q] += wsum[q][w];
+  const int64_t b = blockIdx.x;
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    int64_t pos = -1;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (ty[u] == q) {
+        pos = (int64_t)blk[b * 4 + q] + (int64_t)excl[q];
+        excl[q]++;
+      }
+    if (pos >= 0) {
+      const int64_t f = f0 + u;
+      wave[pos] = uint4{(uint32_t)f, fown[f], nlword[fslot[f]] & ~kActiveBit, feat[f]};
+    }
+  }
+}
+
+int poly_classify(xdem_ctx* c) {
+  if (!c->pl_valid || c->pl_epoch != c->nl_epoch)
+    return set_err(c, XDEM_ERR_STATE, "classify_interactions: run nbh_polyhedron first");
+  const PolyList& P = c->pl[c->pl_cur];
+  const int64_t nf = P.nfeat;
+  const int64_t nblk = std::max<int64_t>(1, (nf + kClassPerBlock - 1) / kClassPerBlock);
+  int rc;
+  if (nblk * 4 > c->cap_fclass_blk) {
+    if ((rc = dev_realloc(c, &c->fclass_blk, nblk * 4))) return rc;
+    c->cap_fclass_blk = nblk * 4;
+  }
+  if (!c->fclass_info && (rc = dev_realloc(c, &c->fclass_info, 9))) return rc;
+  XDEM_HIP(c, hipMemsetAsync(c->fclass_blk, 0, sizeof(uint32_t) * nblk * 4, c->stream));
+  if (nf > 0)
+    hipLaunchKernelGGL(k_class_count, dim3((uint32_t)nblk), dim3(256), 0, c->stream, nf, P.feat, c->fclass_blk);
+  hipLaunchKernelGGL(k_class_scan, dim3(1), dim3(64), 0, c->stream, nblk, c->fclass_blk, c->fclass_info);
+  XDEM_HIP(c, hipGetLastError());
+  int64_t info[9];
+  XDEM_HIP(c, hipMemcpyAsync(info, c->fclass_info, sizeof(info), hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  const int64_t total = info[4];
+  if (total > c->cap_forder) {
+    const int64_t cap = (int64_t)(total * 1.25) + 4096;
+    if ((rc = dev_realloc(c, &c->forder, cap))) return rc;
+    c->cap_forder = cap;
+  }
+  if (total > 0) {
+    XDEM_HIP(c, hipMemsetAsync(c->forder, 0xff, sizeof(uint4) * total, c->stream));
+    if (nf > 0)
+      hipLaunchKernelGGL(k_class_scatter, dim3((uint32_t)nblk), dim3(256), 0, c->stream, nf, P.feat, P.fslot, P.fown,
+                         c->nl[c->nl_cur].word, c->fclass_blk, c->fclass_info, c->forder);
+    XDEM_HIP(c, hipGetLastError());
+  }
+  for (int t = 0; t < 4; t++) {
+    c->fwave_start[t] = info[t];
+    c->fwave_size[t] = info[5 + t];
+  }
+  c->fwave_start[4] = total;
+  c->pl_class_epoch = c->nl_epoch;
+  return XDEM_OK;
+}
+
 // ------------------------------------------------------------------------------------------ forces
 struct PolyForceArgs {
   int64_t n;
@@ -647,6 +798,8 @@ struct PolyForceArgs {
   PolyList pl;
   PolyWall pw;
   const uint2* frange;
+  uint4* wave;  // classifier waves: {feature, owner, owner list word, feature word}; feature ~0u pads
+  int64_t norder;
   // when recording: contact state per feature and per (wall, vertex, particle), kCStateLen doubles
   // each, in the evaluation orientation (null: not recorded)
   double* cstate;
@@ -683,13 +836,16 @@ __device__ inline bool poly_guard(const PolyForceArgs& a) {
 template <bool KR, bool GUARD>
 __global__ __launch_bounds__(128) void k_poly_feat(PolyForceArgs a) {
   if (GUARD && poly_guard(a)) return;
-  // XCD-aware block order: consecutive features (owners close in the order) share an XCD's L2
-  const int64_t f = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (f >= a.pl.nfeat) return;
+  // XCD-aware block order: consecutive wave entries (owners close in the order) share an XCD's L2. Every
+  // wavefront lies inside one type segment of the classifier, so it evaluates one feature type.
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= a.norder) return;
+  const uint4 we = a.wave[t];
+  if (we.x == 0xffffffffu) return;
+  const int64_t f = we.x;
   const PState& s = a.s;
-  const uint32_t sl = a.pl.fslot[f];
-  const uint32_t x = a.pl.fown[f];
-  const uint32_t wd = a.nl.word[sl];
+  const uint32_t x = we.y;
+  const uint32_t wd = we.z;
   const uint32_t y = wd & kIdxMask;
   const V3 sh = shift_vec(a.g, (int)((wd >> kShiftShift) & 31u));
   const Core cx = s.core[x], cy = s.core[y];
@@ -698,7 +854,7 @@ __global__ __launch_bounds__(128) void k_poly_feat(PolyForceArgs a) {
   const double ra = sa.minkowski * cx.w.w, rb = sb.minkowski * cy.w.w;
   const Verts va = verts_of(a.vtx, x, V3{0, 0, 0});
   const Verts vb = verts_of(a.vtx, y, sh);
-  const uint32_t fw = a.pl.feat[f];
+  const uint32_t fw = we.w;
   const int type = (int)(fw & kFeatTypeMask);
   const bool swap = (fw & kFeatSwap) != 0;
   const int osub = (int)((fw >> 3) & 15u), psub = (int)((fw >> 7) & 15u);
@@ -738,7 +894,10 @@ __global__ __launch_bounds__(128) void k_poly_feat(PolyForceArgs a) {
       }
     }
     const uint32_t nfw = act ? (fw | kFeatActive) : (fw & ~kFeatActive);
-    if (nfw != fw) a.pl.feat[f] = nfw;
+    if (nfw != fw) {
+      a.pl.feat[f] = nfw;
+      a.wave[t].w = nfw;
+    }
     if (a.cstate) {
       // the reference functor's analysis tuple (polyhedron/contact.hpp:160-210) with the positions:
       // i is the vertex side, as the reference's swapped pair stores it
@@ -778,6 +937,7 @@ __global__ __launch_bounds__(128) void k_poly_feat(PolyForceArgs a) {
       a.pl.hx[f] = 0.0; a.pl.hy[f] = 0.0; a.pl.hz[f] = 0.0;
       a.pl.mhx[f] = 0.0; a.pl.mhy[f] = 0.0; a.pl.mhz[f] = 0.0;
       a.pl.feat[f] = fw & ~kFeatActive;
+      a.wave[t].w = fw & ~kFeatActive;
     }
   }
 }
@@ -919,6 +1079,9 @@ int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, d
                V3 g, bool guarded, int32_t step_tag) {
   if (!c->pl_valid || !c->hist_ready || c->pl_epoch != c->nl_epoch)
     return set_err(c, XDEM_ERR_STATE, "contact_polyhedron: run nbh_polyhedron after chunk_neighbors_contact");
+  if (c->pl_class_epoch != c->nl_epoch)
+    return set_err(c, XDEM_ERR_STATE, "contact_polyhedron: run classify_interactions after nbh_polyhedron");
+  // feature and vertex-wall moment histories are written in full by every neighbour build
   if (cp.kr != 0.0 || cpd.kr != 0.0) c->kr_used = true;
   if (c->n == 0) return XDEM_OK;
   PolyForceArgs a{};
@@ -930,12 +1093,14 @@ int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, d
   a.nl = c->nl[c->nl_cur];
   a.pl = c->pl[c->pl_cur];
   a.frange = c->frange;
+  a.wave = c->forder;
+  a.norder = c->fwave_start[4];
   a.contrib = c->fcontrib;
   a.contact = c->fcontact;
   a.fdn = c->fdn;
   a.nw = (c->nwalls > 0 && c->pw_valid) ? c->nwalls : 0;
   if (a.nw) a.pw = c->pw[c->pw_cur];
-  a.walls = c->d_walls;
+  a.walls = walls_for_pass(c);
   a.wstride = c->cap;
   a.cp = cp;
   a.cpd = cpd;
@@ -961,7 +1126,8 @@ int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, d
       a.wstate = c->wstate;
     }
   }
-  const dim3 gp(grid_blocks(std::max<int64_t>(a.pl.nfeat, 1), 128)), bp(128), gg(grid_blocks(a.n, 256)), bg(256);
+  const dim3 gp(grid_blocks(std::max<int64_t>(a.norder, 1), 128)), bp(128), gg(grid_blocks(a.n, 256)),
+      bg(256);
   const bool kr = c->kr_used;
 #define PAIRS(KRV, G) hipLaunchKernelGGL((k_poly_feat<KRV, G>), gp, bp, 0, c->stream, a)
 #define GATHER(F, KRV, G) hipLaunchKernelGGL((k_poly_gather<F, KRV, G>), gg, bg, 0, c->stream, a)
@@ -1190,6 +1356,119 @@ extern "C" int xdem_poly_interactions_count(xdem_ctx* c, int64_t* n) {
   return XDEM_OK;
 }
 
+// Classifier wave sizes per interaction type (interaction_enum.hpp:24-57): polyhedra 0-3 from the
+// classified feature waves, 4-6 vertex-driver candidates by driver kind; spheres 0 the particle pairs
+// (owner entries), 4-6 driver candidates by kind, 7-9 mesh candidates by kind.
+extern "C" int xdem_classifier_waves(xdem_ctx* c, int64_t sizes[13]) {
+  if (!c || !sizes) return XDEM_ERR_ARG;
+  for (int t = 0; t < 13; t++) sizes[t] = 0;
+  const int64_t n = c->n;
+  if (n == 0) return XDEM_OK;
+  if (c->nshapes > 0) {
+    if (!c->pl_valid || c->pl_class_epoch != c->nl_epoch)
+      return set_err(c, XDEM_ERR_STATE, "classifier_waves: run classify_interactions after nbh_polyhedron");
+    for (int t = 0; t < 4; t++) sizes[t] = c->fwave_size[t];
+    if (c->nwalls > 0 && c->pw_valid) {
+      std::vector<uint64_t> m(n);
+      XDEM_HIP(c, hipMemcpyAsync(m.data(), c->pw[c->pw_cur].mask, sizeof(uint64_t) * n, hipMemcpyDeviceToHost,
+                                 c->stream));
+      XDEM_HIP(c, hipStreamSynchronize(c->stream));
+      for (int w = 0; w < c->nwalls; w++) {
+        const uint64_t sel = 0xffull << (w * XDEM_MAX_SHAPE_VERTICES);
+        int64_t k = 0;
+        for (auto v : m) k += __builtin_popcountll(v & sel);
+        sizes[driver_type(c->walls[w])] += k;
+      }
+    }
+    return XDEM_OK;
+  }
+  if (!c->nl_valid || !c->hist_ready)
+    return set_err(c, XDEM_ERR_STATE, "classifier_waves: run nbh_sphere and classify_interactions first");
+  sizes[0] = c->nl[c->nl_cur].pairs;
+  if (c->nwalls > 0 && c->wl_valid) {
+    std::vector<uint8_t> m(n);
+    XDEM_HIP(c, hipMemcpyAsync(m.data(), c->wl[c->wl_cur].mask, n, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipStreamSynchronize(c->stream));
+    for (int w = 0; w < c->nwalls; w++) {
+      int64_t k = 0;
+      for (auto v : m) k += (v >> w) & 1u;
+      sizes[driver_type(c->walls[w])] += k;
+    }
+  }
+  if (c->has_mesh && c->ml_valid && c->mesh.cap > 0) {
+    const MeshList& M = c->ml[c->ml_cur];
+    std::vector<uint32_t> cnt(n), word((size_t)c->mesh.cap * c->mesh.stride);
+    XDEM_HIP(c, hipMemcpyAsync(cnt.data(), M.cnt, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipMemcpyAsync(word.data(), M.word, sizeof(uint32_t) * word.size(), hipMemcpyDeviceToHost, c->stream));
+    XDEM_HIP(c, hipStreamSynchronize(c->stream));
+    for (int64_t x = 0; x < n; x++) {
+      const uint32_t mc = std::min<uint32_t>(cnt[x], (uint32_t)c->mesh.cap);
+      for (uint32_t k = 0; k < mc; k++) sizes[7 + (word[(size_t)k * c->mesh.stride + x] >> 30)]++;
+    }
+  }
+  return XDEM_OK;
+}
+
+// The rows of one polyhedron classifier wave (type 0-3) in wave order: owner id, partner id, swap,
+// owner and partner sub-features (the key fields of a ClassifierContainer, classifier_container.hpp:87-300)
+// and the feature's index in the interaction lists.
+extern "C" int xdem_classifier_wave_dump(xdem_ctx* c, int32_t type, int64_t cap, uint64_t* id_i, uint64_t* id_j,
+                                         uint8_t* swap, uint32_t* sub_i, uint32_t* sub_j, uint32_t* index,
+                                         int64_t* written) {
+  if (!c || !id_i || !id_j || !swap || !sub_i || !sub_j || !index) return XDEM_ERR_ARG;
+  if (written) *written = 0;
+  if (c->nshapes == 0) return set_err(c, XDEM_ERR_UNSUPPORTED, "classifier_wave_dump: polyhedron feature waves only");
+  if (type < 0 || type > 3) return set_err(c, XDEM_ERR_ARG, "classifier_wave_dump: type must be 0 (VV) to 3 (EE)");
+  if (c->n == 0) return XDEM_OK;
+  if (!c->pl_valid || c->pl_class_epoch != c->nl_epoch)
+    return set_err(c, XDEM_ERR_STATE, "classifier_wave_dump: run classify_interactions after nbh_polyhedron");
+  const int64_t m = c->fwave_size[type];
+  if (m > cap) return set_err(c, XDEM_ERR_CAPACITY, "classifier_wave_dump: cap too small");
+  if (m == 0) return XDEM_OK;
+  const PolyList& P = c->pl[c->pl_cur];
+  const NbrList& L = c->nl[c->nl_cur];
+  const int64_t total = c->fwave_start[4];
+  std::vector<uint4> all(total), ord;
+  std::vector<uint32_t> feat(P.nfeat), fslot(P.nfeat), fown(P.nfeat), word(L.slots);
+  std::vector<uint64_t> ids(c->n);
+  XDEM_HIP(c, hipMemcpyAsync(all.data(), c->forder, sizeof(uint4) * total, hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipMemcpyAsync(feat.data(), P.feat, sizeof(uint32_t) * P.nfeat, hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipMemcpyAsync(fslot.data(), P.fslot, sizeof(uint32_t) * P.nfeat, hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipMemcpyAsync(fown.data(), P.fown, sizeof(uint32_t) * P.nfeat, hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipMemcpyAsync(word.data(), L.word, sizeof(uint32_t) * L.slots, hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipMemcpyAsync(ids.data(), live(c).id, sizeof(uint64_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  // the wave of the type: its segments across the windows, in window order; padding only ends segments
+  for (int64_t q = 0; q < total; q++) {
+    const uint32_t f = all[q].x;
+    if (f == 0xffffffffu) {
+      if (q % kClassPad == 0) return set_err(c, XDEM_ERR_STATE, "classifier_wave_dump: a segment of padding only");
+      continue;
+    }
+    if (f >= (uint32_t)P.nfeat) return set_err(c, XDEM_ERR_STATE, "classifier_wave_dump: bad feature index");
+    if ((int)(feat[f] & kFeatTypeMask) == type) ord.push_back(all[q]);
+    const int64_t seg = q - q % kClassPad;  // a wavefront holds one type
+    if ((all[seg].w & kFeatTypeMask) != (feat[f] & kFeatTypeMask))
+      return set_err(c, XDEM_ERR_STATE, "classifier_wave_dump: a wavefront mixes types");
+  }
+  if ((int64_t)ord.size() != m) return set_err(c, XDEM_ERR_STATE, "classifier_wave_dump: wave size mismatch");
+  for (int64_t k = 0; k < m; k++) {
+    const uint32_t f = ord[k].x;
+    if (f >= (uint32_t)P.nfeat) return set_err(c, XDEM_ERR_STATE, "classifier_wave_dump: padding inside a wave");
+    if (ord[k].y != fown[f] || ord[k].z != (word[fslot[f]] & ~kActiveBit) || ord[k].w != feat[f])
+      return set_err(c, XDEM_ERR_STATE, "classifier_wave_dump: a wave entry differs from its feature");
+    const uint32_t fw = feat[f];
+    id_i[k] = ids[fown[f]];
+    id_j[k] = ids[word[fslot[f]] & kIdxMask];
+    swap[k] = (fw & kFeatSwap) ? 1 : 0;
+    sub_i[k] = (fw >> 3) & 15u;
+    sub_j[k] = (fw >> 7) & 15u;
+    index[k] = f;
+  }
+  if (written) *written = m;
+  return XDEM_OK;
+}
+
 namespace xdem {
 // Rows of the recorded polyhedron contact state, as the reference's interaction dump writes them
 // (itools/writer.hpp:35-75): features in contact (non-zero force or dn < 0) in feature order, then
@@ -1369,6 +1648,8 @@ extern "C" int xdem_poly_interactions_load(xdem_ctx* c, int64_t m, const uint64_
   if (moment)
     for (int64_t k = 0; k < 3 * m && !c->kr_used; k++) c->kr_used = moment[k] != 0.0;
   if (matched) *matched = hit;
+  // the classifier waves carry the feature words: refresh them with the loaded active bits
+  if (c->pl_class_epoch == c->nl_epoch) return poly_classify(c);
   return XDEM_OK;
 }
 

@@ -67,6 +67,15 @@ struct Surface {
   V3 axis;  // cylinder: 0/1 mask of the coordinates the projection keeps
 };
 
+// Kinematic motion of a driver (host side; the force passes read the driver's state of each step):
+// type 0 STATIONARY, 1 LINEAR_MOTION, 7 SHAKER (motion_type.hpp:23-35).
+struct Motion {
+  int type = 0;
+  V3 mvec{0, 0, 0};
+  double const_vel = 0, t_start = 0, t_end = 1e300, amplitude = 0, omega = 0;
+  V3 shaker_dir{0, 0, 1};
+};
+
 // Interaction type of a driver contact (interaction_enum.hpp:36-38): 4 cylinder, 5 surface, 6 ball.
 inline uint16_t driver_type(const Surface& s) { return s.kind == 1 ? 4 : (s.kind == 2 ? 6 : 5); }
 // Candidate test of the interaction update for a point p (a sphere centre or a polyhedron vertex)
@@ -234,6 +243,7 @@ struct PState {
 // Neighbour list + per-slot history.
 struct NbrList {
   uint32_t* word = nullptr;   // [slots]
+  uint32_t* sx = nullptr;     // [slots] particle of each slot (~0u for the padding of a wave's ELL block)
   double* hx = nullptr;       // friction history [slots]
   double* hy = nullptr;
   double* hz = nullptr;
@@ -340,12 +350,34 @@ struct xdem_ctx {
   int nwalls = 0;
   xdem::Surface walls[XDEM_MAX_SURFACES];
   xdem::Surface* d_walls = nullptr;
+  // driver motion: per driver its motion, the drivers at the last neighbour build (backup_drivers), and
+  // for the engine a ring of per-step driver states (pinned host + device, one set of
+  // XDEM_MAX_SURFACES per step of a poll chunk); d_walls_pass, when set, is the state the next force
+  // pass reads
+  xdem::Motion motion[XDEM_MAX_SURFACES];
+  bool drivers_move = false;  // some driver has a motion other than STATIONARY
+  xdem::Surface walls_bk[XDEM_MAX_SURFACES];
+  double physical_time = 0;
+  xdem::Surface* h_walls_ring = nullptr;
+  xdem::Surface* d_walls_ring = nullptr;
+  int64_t cap_ring = 0;  // steps
+  const xdem::Surface* d_walls_pass = nullptr;
 
   // binning scratch
   uint32_t* cell_of = nullptr;      // [cap]
   uint32_t* cell_count = nullptr;   // [ncells]
   uint32_t* cell_start = nullptr;   // [ncells+1], indexed by Morton rank of the cell
   uint32_t* cell_rank = nullptr;    // [ncells] rank of each cell along the Morton curve
+  uint32_t* cell_lin = nullptr;     // [ncells] linear cell index of each Morton rank
+  uint64_t* d_rmax = nullptr;       // largest radius as IEEE bits, reduced by the binning pass
+  xdem::d4* mxr = nullptr;          // [cap] positions and radii in Morton order (binning -> neighbour build)
+  uint64_t* mid = nullptr;          // [cap] ids in Morton order
+  int64_t cap_morton = 0;
+  uint32_t* blk_u32 = nullptr;      // per-block partial counts
+  int64_t cap_blk = 0;
+  uint32_t* nbh_tmp = nullptr;      // candidate rows of the neighbour build, nbh_tmp_cap per particle
+  uint32_t nbh_tmp_cap = 0;
+  int64_t nbh_tmp_n = 0;            // particles the rows were allocated for
   uint32_t* perm = nullptr;         // [cap] new index -> old index (last reorder)
   uint32_t* perm_tmp = nullptr;     // [cap] scratch for composing permutations
   uint64_t* sort_key = nullptr;     // [cap]
@@ -419,6 +451,15 @@ struct xdem_ctx {
   uint2* frange = nullptr;          // [cap_frange] per list slot: first feature and count of its pair
                                     // (through the owner entry for partner entries); x = ~0u: no owner entry
   int64_t cap_frange = 0;
+  uint4* forder = nullptr;          // [cap_forder] classifier waves by type (feature, owner, list word,
+                                    // feature word), padded
+  int64_t cap_forder = 0;
+  int64_t fwave_start[5] = {0, 0, 0, 0, 0};  // padded start of the VV, VE, VF, EE waves; [4] padded total
+  int64_t fwave_size[4] = {0, 0, 0, 0};      // features per wave
+  uint64_t pl_class_epoch = ~0ull;  // neighbour build the waves were classified from
+  uint32_t* fclass_blk = nullptr;   // per block of 1024 features and type: count, then offset
+  int64_t cap_fclass_blk = 0;
+  int64_t* fclass_info = nullptr;   // [9] wave starts, padded total, wave sizes
   xdem::PolyWall pw[2];             // vertex-wall candidates and history, current and previous
   int pw_cur = 0;
   bool pw_valid = false;
@@ -457,6 +498,12 @@ int hip_check(xdem_ctx* c, hipError_t e, const char* what);
   } while (0)
 
 inline PState& live(xdem_ctx* c) { return c->ps[c->cur]; }
+// drivers read by a force pass: the engine's state of that step, else the current drivers
+inline const Surface* walls_for_pass(const xdem_ctx* c) { return c->d_walls_pass ? c->d_walls_pass : c->d_walls; }
+// driver motion of one step on the host (push_f_v_r then push_f_v) and the driver trigger
+void drivers_advance(Surface* w, const Motion* m, int nw, double time, double dt);
+void drivers_push_f_v(Surface* w, const Motion* m, int nw);
+bool drivers_displaced(const Surface* now, const Surface* bk, int nw, double thr);
 int ensure_capacity(xdem_ctx* c, int64_t n);
 int ensure_cells(xdem_ctx* c, int64_t ncells);
 int scan_u32(xdem_ctx* c, const uint32_t* in, uint32_t* out, int64_t n);  // exclusive, out[n] = total
@@ -495,12 +542,16 @@ int alloc_walls(xdem_ctx* c, int nw);
 int mesh_carry(xdem_ctx* c, double rcut_inc, bool have_old);  // mesh candidates + history carry-over
 void mesh_free(xdem_ctx* c);
 int set_history_form(xdem_ctx* c, bool fp32);
+// a rolling stiffness comes into use (clears the sphere list's moment histories the first time)
+int use_kr(xdem_ctx* c);
 int engine_rebuild(xdem_ctx* c, double rcut_inc);
 int engine_force_pass(xdem_ctx* c, const xdem_scheme* s);  // fused force pass at the current state
 int engine_flush_kick(xdem_ctx* c);
 // polyhedra
 int poly_compute_vertices(xdem_ctx* c, bool displ, double thr2, bool guarded);
 int poly_nbh(xdem_ctx* c, double rcut_inc);
+// classify_interactions of polyhedra: the feature waves by type (forder, fwave_start, fwave_size)
+int poly_classify(xdem_ctx* c);
 int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool fused, bool gravity_on,
                V3 g, bool guarded, int32_t step_tag);
 // interaction counts of the polyhedron lists for simulation_state: total, active, smallest dn; block

@@ -49,6 +49,16 @@ class Surface(C.Structure):
                 ("radius", C.c_double), ("axis", C.c_double * 3)]
 
 
+class DriverMotion(C.Structure):
+    _fields_ = [("motion_type", C.c_int32), ("pad", C.c_int32), ("motion_vector", C.c_double * 3),
+                ("const_vel", C.c_double), ("motion_start_threshold", C.c_double),
+                ("motion_end_threshold", C.c_double), ("amplitude", C.c_double), ("omega", C.c_double),
+                ("shaker_dir", C.c_double * 3)]
+
+
+MOTION_TYPES = {"STATIONARY": 0, "LINEAR_MOTION": 1, "SHAKER": 7}
+
+
 class Shape(C.Structure):
     _fields_ = [("nv", C.c_int32), ("ne", C.c_int32), ("nf", C.c_int32), ("pad", C.c_int32),
                 ("minkowski", C.c_double), ("volume", C.c_double), ("inertia_on_mass", C.c_double * 3),
@@ -98,9 +108,12 @@ HIP_SYMBOLS = [
     "xdem_set_surfaces", "xdem_particles_upload", "xdem_particles_download", "xdem_num_particles",
     "xdem_combined_compute_prolog", "xdem_combined_compute_epilog", "xdem_reset_force_moment",
     "xdem_gravity_force", "xdem_force_to_accel", "xdem_particle_displ_over", "xdem_move_particles",
-    "xdem_chunk_neighbors_contact", "xdem_nbh_sphere", "xdem_classify_interactions", "xdem_contact_sphere",
+    "xdem_chunk_neighbors_contact", "xdem_nbh_sphere", "xdem_classify_interactions", "xdem_classifier_waves",
+    "xdem_classifier_wave_dump", "xdem_contact_sphere",
     "xdem_run_steps", "xdem_interactions_count", "xdem_interactions_dump", "xdem_interactions_load", "xdem_poly_interactions_load", "xdem_interactions_moments",
     "xdem_poly_interactions_moments", "xdem_simulation_state", "xdem_simulation_state_noalloc",
+    "xdem_set_driver_motion", "xdem_push_f_v_r_driver", "xdem_push_f_v_driver", "xdem_driver_displ_over",
+    "xdem_drivers_download", "xdem_set_physical_time", "xdem_physical_time", "xdem_next_time_step",
     "xdem_set_contact_record", "xdem_contact_state_count", "xdem_contact_state_dump", "xdem_nbh_stats",
     "xdem_kernel_timing", "xdem_set_timing", "xdem_streams_dump",
     "xdem_set_shapes", "xdem_compute_vertices", "xdem_vertex_displ_over", "xdem_nbh_polyhedron",
@@ -179,6 +192,9 @@ def load():
         "xdem_chunk_neighbors_contact": (C.c_int, [vp, C.c_double, C.c_double]),
         "xdem_nbh_sphere": (C.c_int, [vp, C.c_double, C.c_int32]),
         "xdem_classify_interactions": (C.c_int, [vp]),
+        "xdem_classifier_waves": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "xdem_classifier_wave_dump": (C.c_int, [vp, C.c_int32, C.c_int64, _u64p, _u64p, C.POINTER(C.c_uint8), _u32p,
+                                                _u32p, _u32p, C.POINTER(C.c_int64)]),
         "xdem_contact_sphere": (C.c_int, [vp, C.POINTER(ContactParams), C.POINTER(ContactParams), C.c_double,
                                           C.c_int32]),
         "xdem_run_steps": (C.c_int, [vp, C.POINTER(Scheme), C.c_int64, C.POINTER(RunStats)]),
@@ -193,6 +209,14 @@ def load():
         "xdem_poly_interactions_moments": (C.c_int, [vp, C.c_int64, _dp, C.POINTER(C.c_int64)]),
         "xdem_simulation_state": (C.c_int, [vp, C.POINTER(SimState)]),
         "xdem_simulation_state_noalloc": (C.c_int, [vp, C.POINTER(SimState)]),
+        "xdem_set_driver_motion": (C.c_int, [vp, C.POINTER(DriverMotion), C.c_int32]),
+        "xdem_push_f_v_r_driver": (C.c_int, [vp, C.c_double, C.c_double]),
+        "xdem_push_f_v_driver": (C.c_int, [vp, C.c_double]),
+        "xdem_driver_displ_over": (C.c_int, [vp, C.c_double, C.POINTER(C.c_int32)]),
+        "xdem_drivers_download": (C.c_int, [vp, C.POINTER(Surface), C.c_int32]),
+        "xdem_set_physical_time": (C.c_int, [vp, C.c_double]),
+        "xdem_physical_time": (C.c_int, [vp, C.POINTER(C.c_double)]),
+        "xdem_next_time_step": (C.c_int, [vp, C.c_double]),
         "xdem_set_mesh": (C.c_int, [vp, C.c_int32, _dp, C.c_int64, _u32p, C.c_int64, _u32p, C.c_int64, C.c_double,
                                     _dp, _dp, C.c_int32]),
         "xdem_mesh_interactions": (C.c_int, [vp, C.c_int64, _u64p, _u32p, C.POINTER(C.c_uint16), _dp, _dp,
@@ -273,6 +297,36 @@ def surfaces(walls):
     return arr
 
 
+def has_motion(walls):
+    return any(w.get("motion", {}).get("motion_type", "STATIONARY") != "STATIONARY" for w in walls)
+
+
+def motions(walls):
+    """walls: list of driver dicts -> ctypes array of DriverMotion from each dict's "motion" entry (the
+    params slot of register_*: motion_type name, motion_vector, const_vel, motion_start_threshold,
+    motion_end_threshold, amplitude, omega, shaker_dir); no entry = STATIONARY."""
+    arr = (DriverMotion * max(1, len(walls)))()
+    for k, w in enumerate(walls):
+        m = w.get("motion", {})
+        arr[k].motion_type = MOTION_TYPES[m.get("motion_type", "STATIONARY")]
+        arr[k].const_vel = m.get("const_vel", 0.0)
+        arr[k].motion_start_threshold = m.get("motion_start_threshold", 0.0)
+        arr[k].motion_end_threshold = m.get("motion_end_threshold", 1e300)
+        arr[k].amplitude = m.get("amplitude", 0.0)
+        arr[k].omega = m.get("omega", 0.0)
+        for a in range(3):
+            arr[k].motion_vector[a] = m.get("motion_vector", (0, 0, 0))[a]
+            arr[k].shaker_dir[a] = m.get("shaker_dir", (0, 0, 1))[a]
+    return arr
+
+
+def driver_rows(arr, n):
+    """xdem_surface rows -> list of dicts (offset, normal, center, vel, vrot, kind, radius, axis)."""
+    return [{"offset": arr[k].offset, "normal": np.array(arr[k].normal[:]), "center": np.array(arr[k].center[:]),
+             "vel": np.array(arr[k].vel[:]), "vrot": np.array(arr[k].vrot[:]), "kind": int(arr[k].kind),
+             "radius": arr[k].radius, "axis": np.array(arr[k].axis[:])} for k in range(n)]
+
+
 class Context:
     """One xdem_ctx on one GPU. Methods map 1:1 to the operator entry points of include/xdem.h."""
 
@@ -312,6 +366,40 @@ class Context:
     def set_surfaces(self, walls):
         arr = surfaces(walls)
         self.call("xdem_set_surfaces", arr, len(walls))
+        self.nwalls = len(walls)
+        if has_motion(walls):
+            self.call("xdem_set_driver_motion", motions(walls), len(walls))
+
+    def set_driver_motion(self, walls):
+        self.call("xdem_set_driver_motion", motions(walls), len(walls))
+
+    def push_f_v_r_driver(self, time, dt):
+        self.call("xdem_push_f_v_r_driver", time, dt)
+
+    def push_f_v_driver(self, dt):
+        self.call("xdem_push_f_v_driver", dt)
+
+    def driver_displ_over(self, threshold):
+        r = C.c_int32()
+        self.call("xdem_driver_displ_over", threshold, C.byref(r))
+        return bool(r.value)
+
+    def drivers(self):
+        n = getattr(self, "nwalls", 0)
+        arr = (Surface * max(1, n))()
+        self.call("xdem_drivers_download", arr, n)
+        return driver_rows(arr, n)
+
+    def physical_time(self):
+        t = C.c_double()
+        self.call("xdem_physical_time", C.byref(t))
+        return t.value
+
+    def set_physical_time(self, t):
+        self.call("xdem_set_physical_time", t)
+
+    def next_time_step(self, dt):
+        self.call("xdem_next_time_step", dt)
 
     def upload(self, arrays):
         n = int(arrays["rx"].shape[0])
@@ -363,6 +451,23 @@ class Context:
 
     def classify_interactions(self):
         self.call("xdem_classify_interactions")
+
+    def classifier_waves(self):
+        """Wave size per interaction type 0-12 (xdem_classifier_waves), as an int64 array."""
+        out = (C.c_int64 * 13)()
+        self.call("xdem_classifier_waves", out)
+        return np.array(list(out), np.int64)
+
+    def classifier_wave(self, t):
+        """Rows of polyhedron wave t (0-3) in wave order: id_i, id_j, swap, sub_i, sub_j, feature index."""
+        m = int(self.classifier_waves()[t])
+        a = [np.zeros(m, np.uint64), np.zeros(m, np.uint64), np.zeros(m, np.uint8), np.zeros(m, np.uint32),
+             np.zeros(m, np.uint32), np.zeros(m, np.uint32)]
+        got = C.c_int64()
+        self.call("xdem_classifier_wave_dump", t, m, _ptr(a[0], C.c_uint64), _ptr(a[1], C.c_uint64),
+                  _ptr(a[2], C.c_uint8), _ptr(a[3], C.c_uint32), _ptr(a[4], C.c_uint32), _ptr(a[5], C.c_uint32),
+                  C.byref(got))
+        return tuple(x[:got.value] for x in a)
 
     def contact_sphere(self, cp, cpd=None, dt=1e-5, form=XDEM_FORM_FP64):
         self.call("xdem_contact_sphere", C.byref(cp), C.byref(cpd) if cpd is not None else None, dt, form)

@@ -56,22 +56,33 @@ class SphereSim:
         c.contact_sphere(self.w.cp, self.w.cpd, self.w.dt, self.form)
 
     def first_iteration(self):
-        """init_particles (rebuild), reset_force_moment, compute_force; no force_to_accel."""
+        """init_particles (rebuild), reset_force_moment, compute_force, next_time_step; no force_to_accel."""
         self.rebuild()
         self.ctx.reset_force_moment()
         self.compute_force()
+        self.ctx.next_time_step(self.w.dt)
+
+    def drivers_prolog(self):
+        """push_f_v_r_driver at the step's physical time, then push_f_v_driver (dt_scale 0.5)."""
+        c, w = self.ctx, self.w
+        c.push_f_v_r_driver(c.physical_time(), w.dt)
+        c.push_f_v_driver(0.5 * w.dt)
 
     def step(self):
-        """One numerical_scheme_verlet_macro iteration, operator by operator."""
+        """One numerical_scheme_verlet_macro iteration, operator by operator, then next_time_step."""
         c, w = self.ctx, self.w
         c.combined_compute_prolog(w.dt)
-        if c.particle_displ_over(0.5 * w.rcut_inc):
+        self.drivers_prolog()
+        moved = c.particle_displ_over(0.5 * w.rcut_inc)
+        if c.driver_displ_over(0.5 * w.rcut_inc) or moved:
             self.rebuild()
             self.rebuilds += 1
         c.reset_force_moment()
         self.compute_force()
         c.force_to_accel()
+        c.push_f_v_driver(0.5 * w.dt)
         c.combined_compute_epilog(w.dt)
+        c.next_time_step(w.dt)
 
     def run(self, nsteps, poll=16):
         st = self.ctx.run_steps(scheme(self.w, self.form, poll), nsteps)
@@ -89,7 +100,29 @@ class SphereSim:
         if self.mesh is not None:  # mesh-driver histories
             mi, ms, mt, mf, mm = self.ctx.mesh_interactions()
             mesh = {"m_ii": mi, "m_sj": ms, "m_tt": mt, "m_fr": mf, "m_mo": mm}
-        np.savez(path, ii=ii, jj=jj, tt=tt, fr=fr, mo=mo, **mesh, **{"p_" + k: v for k, v in p.items()})
+        np.savez(path, ii=ii, jj=jj, tt=tt, fr=fr, mo=mo, **mesh, **self._driver_state(),
+                 **{"p_" + k: v for k, v in p.items()})
+
+    def _driver_state(self):
+        """physical time and the drivers' state (offset, centre, velocity), as dump_driver keeps them."""
+        d = self.ctx.drivers()
+        return {"time": np.array(self.ctx.physical_time()),
+                "d_offset": np.array([x["offset"] for x in d]), "d_center": np.array([x["center"] for x in d]),
+                "d_vel": np.array([x["vel"] for x in d])}
+
+    @staticmethod
+    def _restore_drivers(w, d):
+        """the workload's drivers moved to the saved state (offset, centre, velocity)."""
+        if "d_offset" not in d.files or len(w.walls) == 0:
+            return
+        walls = []
+        for k, x in enumerate(w.walls):
+            x = dict(x)
+            x["offset"] = float(d["d_offset"][k])
+            x["center"] = tuple(float(v) for v in d["d_center"][k])
+            x["vel"] = tuple(float(v) for v in d["d_vel"][k])
+            walls.append(x)
+        w.walls = walls
 
     @classmethod
     def restart(cls, workload, path, device=0, form=native.XDEM_FORM_FP64, history=True, mesh=None):
@@ -99,7 +132,10 @@ class SphereSim:
         d = np.load(path)
         w = copy.copy(workload)
         w.particles = {k[2:]: d[k] for k in d.files if k.startswith("p_")}
+        cls._restore_drivers(w, d)
         sim = cls(w, device=device, form=form, mesh=mesh)
+        if "time" in d.files:
+            sim.ctx.set_physical_time(float(d["time"]))
         sim.rebuild()
         if history:
             sim.ctx.interactions_load(d["ii"], d["jj"], d["tt"], d["fr"], d["mo"])
@@ -122,7 +158,8 @@ class PolySim(SphereSim):
         p = self.ctx.download()
         ii, jj, tt, sw, si, sj, fr = self.ctx.poly_interactions()
         mo = self.ctx.poly_interaction_moments()
-        np.savez(path, ii=ii, jj=jj, tt=tt, sw=sw, si=si, sj=sj, fr=fr, mo=mo, **{"p_" + k: v for k, v in p.items()})
+        np.savez(path, ii=ii, jj=jj, tt=tt, sw=sw, si=si, sj=sj, fr=fr, mo=mo, **self._driver_state(),
+                 **{"p_" + k: v for k, v in p.items()})
 
     @classmethod
     def restart(cls, workload, path, device=0, history=True):
@@ -130,7 +167,10 @@ class PolySim(SphereSim):
         d = np.load(path)
         w = copy.copy(workload)
         w.particles = {k[2:]: d[k] for k in d.files if k.startswith("p_")}
+        cls._restore_drivers(w, d)
         sim = cls(w, device=device)
+        if "time" in d.files:
+            sim.ctx.set_physical_time(float(d["time"]))
         sim.rebuild()
         if history:
             sim.ctx.poly_interactions_load(d["ii"], d["jj"], d["tt"], d["sw"], d["si"], d["sj"], d["fr"], d["mo"])
@@ -141,6 +181,7 @@ class PolySim(SphereSim):
         c.move_particles()
         c.chunk_neighbors_contact(self.w.rcut_inc)
         c.nbh_polyhedron(self.w.rcut_inc)
+        c.classify_interactions()
 
     def compute_force(self):
         c = self.ctx
@@ -151,7 +192,9 @@ class PolySim(SphereSim):
     def step(self):
         c, w = self.ctx, self.w
         c.combined_compute_prolog(w.dt)
-        if c.vertex_displ_over(0.5 * w.rcut_inc):
+        self.drivers_prolog()
+        moved = c.vertex_displ_over(0.5 * w.rcut_inc)
+        if c.driver_displ_over(0.5 * w.rcut_inc) or moved:
             self.rebuild()
             self.rebuilds += 1
         else:
@@ -159,4 +202,6 @@ class PolySim(SphereSim):
         c.reset_force_moment()
         self.compute_force()
         c.force_to_accel()
+        c.push_f_v_driver(0.5 * w.dt)
         c.combined_compute_epilog(w.dt)
+        c.next_time_step(w.dt)

@@ -162,6 +162,48 @@ int xdem_synchronize(xdem_ctx* ctx);
 int xdem_set_domain(xdem_ctx* ctx, const xdem_domain* dom);
 /* register_surface (src/driver/register_surface.cpp:27-58), whole list at once. */
 int xdem_set_surfaces(xdem_ctx* ctx, const xdem_surface* walls, int32_t n);
+
+/* Driver motion: the params slot of register_surface / register_cylinder / register_ball
+ * (Driver_params, src/driver/include/exaDEM/driver_params.hpp:33-60; MotionType values of
+ * motion_type.hpp:23-35). Kinematic motions are provided: STATIONARY (every driver), LINEAR_MOTION
+ * (surfaces and balls) and SHAKER (surfaces and cylinders); others fail with XDEM_ERR_UNSUPPORTED. */
+#define XDEM_MOTION_STATIONARY 0
+#define XDEM_MOTION_LINEAR 1
+#define XDEM_MOTION_SHAKER 7
+typedef struct xdem_driver_motion {
+  int32_t motion_type;
+  int32_t pad;
+  double motion_vector[3];
+  double const_vel;
+  double motion_start_threshold; /* motion acts while start <= time + dt <= end */
+  double motion_end_threshold;
+  double amplitude;              /* SHAKER: A sin(omega (t - start)) along shaker_dir */
+  double omega;
+  double shaker_dir[3];
+} xdem_driver_motion;
+/* Motion of each registered driver (n = the number registered by xdem_set_surfaces, in that order),
+ * checked and normalised as register_* does (check_motion_coherence, driver_params.hpp:108-214;
+ * Surface::initialize, surface.hpp:144-195: a linear surface moves along its normal). NULL with
+ * n = 0 makes every driver stationary. Drivers' positions at the last neighbour build are the
+ * reference of driver_displ_over. */
+int xdem_set_driver_motion(xdem_ctx* ctx, const xdem_driver_motion* motion, int32_t n);
+/* push_f_v_r_driver (src/driver/push_f_v_r_driver.cpp:28-65): every driver whose motion is active at
+ * time + dt moves by one step (Surface::push_f_v_r surface.hpp:260-299, Cylinder cylinder.hpp:160-173,
+ * Ball ball.hpp:203-213). */
+int xdem_push_f_v_r_driver(xdem_ctx* ctx, double time, double dt);
+/* push_f_v_driver (src/driver/push_f_v_driver.cpp): velocity update of the kinematic motions
+ * (LINEAR_MOTION: vel = const_vel * motion_vector; a STATIONARY surface: vel = 0). */
+int xdem_push_f_v_driver(xdem_ctx* ctx, double dt);
+/* driver_displ_over (src/driver/driver_displ_over.cu:59-130): *result = 1 when a surface's projected
+ * centre or a ball's centre moved by at least threshold since the last neighbour build. */
+int xdem_driver_displ_over(xdem_ctx* ctx, double threshold, int32_t* result);
+/* The drivers' current state (offset, centre, velocity, ...) in xdem_surface form. */
+int xdem_drivers_download(xdem_ctx* ctx, xdem_surface* out, int32_t n);
+/* physical_time of the time loop; next_time_step adds dt (main-config.msp:65,105), the fused engine
+ * (xdem_run_steps) adds dt per step. */
+int xdem_set_physical_time(xdem_ctx* ctx, double time);
+int xdem_physical_time(xdem_ctx* ctx, double* time);
+int xdem_next_time_step(xdem_ctx* ctx, double dt);
 /* Replaces the particle set (input_data); ids must be unique. */
 int xdem_particles_upload(xdem_ctx* ctx, const xdem_particles* p);
 /* Fills the caller's arrays in the current device (cell-sorted) order. */
@@ -194,9 +236,22 @@ int xdem_chunk_neighbors_contact(xdem_ctx* ctx, double rcut_inc, double nbh_dist
 /* nbh_sphere (src/sphere/nbh_sphere.cu:64-309) with symetric=true: owner
  * choice, wall candidates, friction-history carry-over by key. */
 int xdem_nbh_sphere(xdem_ctx* ctx, double rcut_inc, int32_t symetric);
-/* classify_interactions (src/interaction/classify_interactions.cpp:52-136):
- * interactions already live in per-type device arrays; validates state. */
+/* classify_interactions (src/interaction/classify_interactions.cpp:52-136, classifier.hpp:33-271):
+ * polyhedra: splits the feature interactions into one wave per type (VV, VE, VF, EE), each in list
+ * order (a stable partition on the device); contact_polyhedron evaluates one type per wavefront and
+ * requires a classification of the current lists (XDEM_ERR_STATE otherwise). Spheres: each
+ * interaction type already has its own device structure (pairs in the neighbour list, driver and
+ * mesh candidates in their arrays); validates state. */
 int xdem_classify_interactions(xdem_ctx* ctx);
+/* Classifier wave sizes, one per interaction type 0-12 (interaction_enum.hpp:24-57): polyhedra
+ * 0-3 feature waves and 4-6 vertex-driver candidates by driver kind; spheres 0 particle pairs (one
+ * per pair), 4-6 driver candidates by kind, 7-9 mesh candidates (vertex, edge, face). */
+int xdem_classifier_waves(xdem_ctx* ctx, int64_t sizes[13]);
+/* The rows of polyhedron wave `type` (0-3) in wave order: owner id, partner id, swap, owner and
+ * partner sub-features (the key fields of ClassifierContainer, classifier_container.hpp:87-300) and
+ * the feature's index in the interaction lists. XDEM_ERR_CAPACITY when cap is below the wave size. */
+int xdem_classifier_wave_dump(xdem_ctx* ctx, int32_t type, int64_t cap, uint64_t* id_i, uint64_t* id_j,
+                              uint8_t* swap, uint32_t* sub_i, uint32_t* sub_j, uint32_t* index, int64_t* written);
 /* contact_sphere / contact_singlemat_sphere[_hooke[_none]]
  * (src/sphere/contact_force_sphere.cu.in:44-333): adds contact forces and
  * moments of every particle-particle and particle-wall interaction to f and

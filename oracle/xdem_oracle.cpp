@@ -155,13 +155,83 @@ static inline V3 compute_moments(V3 cp, V3 p, V3 f, V3 m) { return cross(cp - p,
 
 // Stationary drivers: Surface (surface.hpp:320-350), Cylinder (cylinder.hpp:225-272), Ball
 // (ball.hpp:292-327). kind: 0 surface, 1 cylinder, 2 ball.
+// Kinematic motion of a driver (Driver_params, driver_params.hpp:33-60; MotionType values of
+// motion_type.hpp:23-35): 0 STATIONARY, 1 LINEAR_MOTION, 7 SHAKER.
+struct Motion {
+  int type = 0;
+  V3 mvec{0, 0, 0};
+  double const_vel = 0, t_start = 0, t_end = 1e300, amplitude = 0, omega = 0;
+  V3 shaker_dir{0, 0, 1};
+};
 struct Surf {
   double offset;
   V3 normal, center, vel, vrot, center_proj;
   int kind;
   double radius;
   V3 axis;
+  Motion mo;
 };
+
+// is_motion_triggered (driver_params.hpp:84-86)
+static inline bool motion_on(const Motion& m, double time) { return time >= m.t_start && time <= m.t_end; }
+// shaker_signal / shaker_velocity (driver_params.hpp:346-356)
+static inline double shaker_signal(const Motion& m, double time) {
+  time -= m.t_start;
+  return m.amplitude * std::sin(m.omega * time);
+}
+static inline V3 shaker_velocity(const Motion& m, double time) {
+  time -= m.t_start;
+  return m.amplitude * m.omega * std::cos(m.omega * time) * m.shaker_dir;
+}
+// push_f_v_r of one driver: Surface (surface.hpp:260-299; acc = 0 for kinematic motions), Cylinder
+// (cylinder.hpp:160-173), Ball (ball.hpp:203-213)
+static void driver_push_f_v_r(Surf& s, double time, double dt) {
+  const Motion& m = s.mo;
+  if (m.type == 0) return;
+  if (s.kind == 0) {
+    const double acc = 0.0;  // kinematic motions carry no acceleration
+    double displ = dt * dot(s.vel, s.normal) + 0.5 * dt * dt * acc;
+    if (m.type == 7) {
+      const double next = shaker_signal(m, time + dt), cur = shaker_signal(m, time);
+      displ = (next - cur) * dot(m.shaker_dir, s.normal);
+      s.vel = shaker_velocity(m, time + dt);
+    }
+    s.center = s.center + displ * s.normal;
+    s.offset += displ;
+    s.center_proj = s.center_proj + displ * s.normal;
+  } else if (s.kind == 1) {
+    if (m.type == 7) {
+      const double displ = shaker_signal(m, time + dt) - shaker_signal(m, time);
+      s.center = s.center + displ * m.shaker_dir;
+      s.vel = shaker_velocity(m, time + dt);
+    }
+  } else {
+    s.center = s.center + dt * s.vel;
+  }
+}
+// push_f_v of one driver (surface.hpp:234-252, ball.hpp:219-234; Cylinder: nothing, cylinder.hpp:149-151)
+static void driver_push_f_v(Surf& s) {
+  const Motion& m = s.mo;
+  if (s.kind == 0) {
+    if (m.type == 0) s.vel = {0, 0, 0};
+    else if (m.type == 1) s.vel = m.const_vel * m.mvec;
+  } else if (s.kind == 2) {
+    if (m.type == 1) s.vel = m.mvec * m.const_vel;
+  }
+}
+// driver_displ_over against the drivers at the last rebuild (driver_displ_over.cu:59-130): projected
+// centre of a surface, centre of a ball; a cylinder never triggers.
+static bool drivers_displ_over(const std::vector<Surf>& now, const std::vector<Surf>& bk, double thr) {
+  const double r2 = thr * thr;
+  for (size_t w = 0; w < now.size() && w < bk.size(); w++) {
+    V3 d{0, 0, 0};
+    if (now[w].kind == 0) d = now[w].center_proj - bk[w].center_proj;
+    else if (now[w].kind == 2) d = now[w].center - bk[w].center;
+    else continue;
+    if (dot(d, d) >= r2) return true;
+  }
+  return false;
+}
 static inline V3 mask_mul(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
 static inline bool surf_filter(const Surf& s, double rcut, V3 p) {
   if (s.kind == 1) {  // Cylinder::filter
@@ -265,6 +335,8 @@ struct Oracle {
   int dims[3] = {1, 1, 1};
   std::vector<int64_t> cell_start;  // size ncells+1
   std::vector<Surf> walls;
+  std::vector<Surf> walls_bk;  // drivers at the last rebuild (backup_drivers)
+  double time = 0;             // physical_time
   // stationary triangle-mesh driver (register_stl_mesh.cpp:30-140), world-space features
   struct Mesh {
     bool on = false;
@@ -307,6 +379,7 @@ static void set_domain(Oracle& o, const xdem_domain* d) {
 
 // migrate + backup_r: wrap, bin, reorder (cells in index order, particles by id)
 static void move_particles(Oracle& o) {
+  o.walls_bk = o.walls;  // backup_drivers (config_move_particles.msp:149-153)
   const int64_t n = o.n;
   std::vector<int64_t> cell(n);
   for (int64_t i = 0; i < n; i++) {
@@ -1405,23 +1478,36 @@ void xo_prolog(void* h, double dt) { prolog(*(Oracle*)h, dt); }
 void xo_epilog(void* h, double dt) { epilog(*(Oracle*)h, dt); }
 int xo_displ_over(void* h, double thr) { return displ_over(*(Oracle*)h, thr) ? 1 : 0; }
 
+// push_f_v_r_driver then push_f_v_driver (config_numerical_schemes.msp:13-14): every driver whose
+// motion is active at time + dt moves by one step, then the kinematic velocities are set.
+static void drivers_prolog(Oracle& o, double dt) {
+  for (auto& w : o.walls)
+    if (motion_on(w.mo, o.time + dt)) driver_push_f_v_r(w, g_mutation == 8 ? o.time + dt : o.time, dt);
+  for (auto& w : o.walls) driver_push_f_v(w);
+}
+
 // first_iteration (main-config.msp:55-65): init_particles (rebuild), reset_force_moment,
-// compute_force (gravity + contact); force_to_accel is not part of it.
+// compute_force (gravity + contact), next_time_step; force_to_accel is not part of it.
 void xo_first_iteration(void* h, const xdem_scheme* s) {
   Oracle& o = *(Oracle*)h;
   rebuild(o, s->rcut_inc);
   reset_force_moment(o);
   if (s->gravity_on) gravity(o, {s->gravity[0], s->gravity[1], s->gravity[2]});
   contact(o, to_cp(&s->cp), to_cp(&s->cpd), s->dt);
+  o.time += s->dt;
 }
 
-// numerical_scheme_verlet_macro, nsteps times. Returns the number of rebuilds.
+// numerical_scheme_verlet_macro then next_time_step, nsteps times (config_numerical_schemes.msp:6-28:
+// prolog, driver motion, rebuild when a particle or a driver moved by rcut_inc/2, forces, epilog).
+// Returns the number of rebuilds.
 int64_t xo_run_steps(void* h, const xdem_scheme* s, int64_t nsteps) {
   Oracle& o = *(Oracle*)h;
   int64_t rebuilds = 0;
   for (int64_t k = 0; k < nsteps; k++) {
     prolog(o, s->dt);
-    if (displ_over(o, 0.5 * s->rcut_inc)) {
+    drivers_prolog(o, s->dt);
+    const bool moved = displ_over(o, 0.5 * s->rcut_inc);
+    if (moved || drivers_displ_over(o.walls, o.walls_bk, 0.5 * s->rcut_inc)) {
       rebuild(o, s->rcut_inc);
       rebuilds++;
     }
@@ -1429,9 +1515,71 @@ int64_t xo_run_steps(void* h, const xdem_scheme* s, int64_t nsteps) {
     if (s->gravity_on) gravity(o, {s->gravity[0], s->gravity[1], s->gravity[2]});
     contact(o, to_cp(&s->cp), to_cp(&s->cpd), s->dt);
     force_to_accel(o);
+    for (auto& w : o.walls) driver_push_f_v(w);
     epilog(o, s->dt);
+    o.time += s->dt;
   }
   return rebuilds;
+}
+
+// Motion of each driver, as register_* checks it: check_motion_coherence (driver_params.hpp:108-214:
+// LINEAR motion vectors and shaker directions normalised when |v|^2 - 1 >= 1e-14) and, for a linear
+// surface, the motion vector replaced by the normal unless it is +-normal (surface.hpp:183-195).
+// Returns 0, or -1 for a motion the reference rejects (zero motion vector, amplitude or omega <= 0).
+int xo_set_driver_motion(void* h, const xdem_driver_motion* m, int n) {
+  Oracle& o = *(Oracle*)h;
+  for (int k = 0; k < n && k < (int)o.walls.size(); k++) {
+    Motion mo;
+    mo.type = m[k].motion_type;
+    mo.mvec = {m[k].motion_vector[0], m[k].motion_vector[1], m[k].motion_vector[2]};
+    mo.const_vel = m[k].const_vel;
+    mo.t_start = m[k].motion_start_threshold;
+    mo.t_end = m[k].motion_end_threshold;
+    mo.amplitude = m[k].amplitude;
+    mo.omega = m[k].omega;
+    mo.shaker_dir = {m[k].shaker_dir[0], m[k].shaker_dir[1], m[k].shaker_dir[2]};
+    if (mo.type == 7) {
+      if (mo.amplitude <= 0.0 || mo.omega <= 0.0) return -1;
+      if (dot(mo.shaker_dir, mo.shaker_dir) - 1 >= 1e-14) mo.shaker_dir = mo.shaker_dir / norm(mo.shaker_dir);
+    }
+    if (mo.type == 1) {
+      if (mo.mvec.x == 0 && mo.mvec.y == 0 && mo.mvec.z == 0) return -1;
+      if (dot(mo.mvec, mo.mvec) - 1 >= 1e-14) mo.mvec = mo.mvec / norm(mo.mvec);
+      Surf& w = o.walls[k];
+      if (w.kind == 0) {
+        const V3 nn = w.normal, mn = -1.0 * nn;
+        const bool same = mo.mvec.x == nn.x && mo.mvec.y == nn.y && mo.mvec.z == nn.z;
+        const bool opp = mo.mvec.x == mn.x && mo.mvec.y == mn.y && mo.mvec.z == mn.z;
+        if (!same && !opp) mo.mvec = nn;
+      }
+    }
+    o.walls[k].mo = mo;
+  }
+  return 0;
+}
+void xo_set_time(void* h, double t) { ((Oracle*)h)->time = t; }
+double xo_time(void* h) { return ((Oracle*)h)->time; }
+// drivers as xdem_surface rows (offset, normal, centre, velocity, spin, kind, radius, axis)
+void xo_get_drivers(void* h, xdem_surface* out, int n) {
+  Oracle& o = *(Oracle*)h;
+  for (int k = 0; k < n && k < (int)o.walls.size(); k++) {
+    const Surf& w = o.walls[k];
+    xdem_surface r{};
+    r.offset = w.offset;
+    const V3* src[4] = {&w.normal, &w.center, &w.vel, &w.vrot};
+    double* dst[4] = {r.normal, r.center, r.vel, r.vrot};
+    for (int q = 0; q < 4; q++) {
+      dst[q][0] = src[q]->x;
+      dst[q][1] = src[q]->y;
+      dst[q][2] = src[q]->z;
+    }
+    r.kind = w.kind;
+    r.radius = w.radius;
+    r.axis[0] = w.axis.x;
+    r.axis[1] = w.axis.y;
+    r.axis[2] = w.axis.z;
+    out[k] = r;
+  }
 }
 
 int64_t xo_num_particles(void* h) { return ((Oracle*)h)->n; }
@@ -1614,13 +1762,16 @@ void xo_first_iteration_poly(void* h, const xdem_scheme* s) {
   reset_force_moment(o);
   if (s->gravity_on) gravity(o, {s->gravity[0], s->gravity[1], s->gravity[2]});
   contact_poly(o, to_cp(&s->cp), to_cp(&s->cpd), s->dt);
+  o.time += s->dt;
 }
 int64_t xo_run_steps_poly(void* h, const xdem_scheme* s, int64_t nsteps) {
   Oracle& o = *(Oracle*)h;
   int64_t rebuilds = 0;
   for (int64_t k = 0; k < nsteps; k++) {
     prolog(o, s->dt);
-    if (vertex_displ_over(o, 0.5 * s->rcut_inc)) {
+    drivers_prolog(o, s->dt);
+    const bool moved = vertex_displ_over(o, 0.5 * s->rcut_inc);
+    if (moved || drivers_displ_over(o.walls, o.walls_bk, 0.5 * s->rcut_inc)) {
       rebuild_poly(o, s->rcut_inc);
       rebuilds++;
     } else {
@@ -1630,7 +1781,9 @@ int64_t xo_run_steps_poly(void* h, const xdem_scheme* s, int64_t nsteps) {
     if (s->gravity_on) gravity(o, {s->gravity[0], s->gravity[1], s->gravity[2]});
     contact_poly(o, to_cp(&s->cp), to_cp(&s->cpd), s->dt);
     force_to_accel(o);
+    for (auto& w : o.walls) driver_push_f_v(w);
     epilog(o, s->dt);
+    o.time += s->dt;
   }
   return rebuilds;
 }

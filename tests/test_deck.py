@@ -70,9 +70,16 @@ def wall_deck(tmp_path, w, steps):
         for k in range(w.n):
             f.write(f"Sphere1 {float(p['rx'][k])!r} {float(p['ry'][k])!r} {float(p['rz'][k])!r}\n")
     lo, hi = list(w.domain.lo), list(w.domain.hi)
+    def params(wl):
+        m = wl.get("motion", {"motion_type": "STATIONARY"})
+        items = [f"motion_type: {m['motion_type']}"]
+        for key, v in m.items():
+            if key != "motion_type":
+                items.append(f"{key}: [{v[0]!r}, {v[1]!r}, {v[2]!r}]" if isinstance(v, tuple) else f"{key}: {v!r}")
+        return "{ " + ", ".join(items) + " }"
     walls = "\n".join(
         f"  - register_surface: {{ id: {k}, state: {{ normal: [{wl['normal'][0]}, {wl['normal'][1]}, "
-        f"{wl['normal'][2]}], offset: {wl['offset']!r} }}, params: {{ motion_type: STATIONARY }} }}"
+        f"{wl['normal'][2]}], offset: {wl['offset']!r} }}, params: {params(wl)} }}"
         for k, wl in enumerate(w.walls))
     cp = w.cp
     cfg = f"{{ kn: {cp.kn!r}, kt: {cp.kt!r}, kr: {cp.kr!r}, mu: {cp.mu!r}, damp_rate: {cp.damp_rate!r} }}"

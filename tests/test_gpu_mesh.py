@@ -255,3 +255,28 @@ def test_mesh_contact_dump(gpu_ok):
         assert dns[k] is not None and abs(v[0] - dns[k]) <= 1e-12
         assert np.array_equal(v[7:10], mf[k])
         assert np.array_equal(v[10:13], r[int(a)]) and np.all(v[13:16] == 0)
+
+
+def test_classifier_waves_spheres(gpu_ok):
+    """Classifier wave sizes of a sphere context: particle pairs (one per pair), and the mesh candidates by
+    kind (vertex 7, edge 8, face 9) equal to the oracle's items of those types; walls by driver kind in a
+    walled box."""
+    w = mc.workload("tray", mc.tray_bed())
+    m = mc.tray_mesh(0.02)
+    s = SphereSim(w, mesh=m)
+    s.first_iteration()
+    o = oracle_for(w, m)
+    o.first_iteration(scheme(w))
+    sizes = s.ctx.classifier_waves()
+    oi, oj, ot, of, om = o.interactions()
+    assert sizes[0] == int(np.count_nonzero(ot == 0)) > 0
+    for t in (7, 8, 9):
+        assert sizes[t] == int(np.count_nonzero(ot == t)) > 0, t
+    assert np.all(sizes[1:7] == 0) and np.all(sizes[10:] == 0)
+    from helpers import wall_deposition
+    ww = wall_deposition()
+    b = SphereSim(ww)
+    b.first_iteration()
+    bs = b.ctx.classifier_waves()
+    _, _, bt, _ = b.ctx.interactions()
+    assert bs[0] == int(np.count_nonzero(bt == 0)) and bs[5] == int(np.count_nonzero(bt == 5)) > 0

@@ -1,0 +1,139 @@
+"""Moving drivers (push_f_v_r_driver, push_f_v_driver, driver_displ_over; surface.hpp:234-299,
+cylinder.hpp:149-173, ball.hpp:203-234, driver_displ_over.cu:59-130), HIP path against the CPU oracle.
+
+Cases (tests/driver_cases.py): a piston wall moving into a packing (LINEAR_MOTION, rebuilds triggered by
+the wall), the same with a start threshold, a shaking floor (SHAKER), a ball obstacle moving through a
+ball container (LINEAR_MOTION), a spinning drum shaking along x (SHAKER). Bars: the drivers' states and
+the physical time bit-exact, rebuild counts equal, interaction keys and friction history bit-exact,
+positions, velocities and forces within 1e-12 relative, for the fused engine (polled every 16 steps, so
+driver-triggered rebuilds fall inside step chains) and the operator path; polyhedra under a descending
+lid; a deck with a moving wall through the deck runner.
+"""
+import numpy as np
+import pytest
+
+from exadem_amd import inputs, native
+from exadem_amd.sim import PolySim, SphereSim, scheme
+import pyoracle as po
+
+import driver_cases as dcs
+from helpers import by_id, force, interaction_table, pos, rel_err, vel
+
+pytestmark = pytest.mark.gpu
+
+STEPS = dcs.STEPS
+
+
+def same_drivers(a, b, what):
+    assert len(a) == len(b)
+    for k, (x, y) in enumerate(zip(a, b)):
+        for key in ("offset", "center", "vel"):
+            assert np.array_equal(np.asarray(x[key]), np.asarray(y[key])), (what, k, key, x[key], y[key])
+
+
+def same_state(g, q, what, tol=1e-12):
+    g, q = by_id(g), by_id(q)
+    assert np.array_equal(g["id"], q["id"])
+    L = max(1.0, float(np.max(np.abs(pos(q)))))
+    dpos = float(np.max(np.abs(pos(g) - pos(q)))) / L
+    dvel = float(np.max(np.abs(vel(g) - vel(q)))) / max(1e-3, float(np.max(np.abs(vel(q)))))
+    ef = rel_err(force(g), force(q), q["mass"] * 9.81)
+    print(f"{what}: positions {dpos:.2e}, velocities {dvel:.2e}, forces {ef:.2e}", flush=True)
+    assert dpos <= tol and dvel <= tol and ef <= tol, what
+
+
+def same_interactions(s, o, what):
+    kg, fg = interaction_table(*s.ctx.interactions())
+    oi, oj, ot, of, _ = o.interactions()
+    ko, fo = interaction_table(oi, oj, ot, of)
+    assert np.array_equal(kg, ko), what
+    assert np.array_equal(fg, fo), what
+
+
+@pytest.mark.parametrize("case", list(dcs.CASES))
+def test_moving_drivers_match_oracle(gpu_ok, case):
+    w = dcs.CASES[case]()
+    sch = scheme(w)
+    o = po.Oracle(w.particles, w.domain, w.walls)
+    o.first_iteration(sch)
+    orb = o.run_steps(sch, STEPS)
+    s = SphereSim(w)
+    s.first_iteration()
+    st = s.run(STEPS, poll=16)
+    same_drivers(s.ctx.drivers(), o.drivers(), f"{case} engine drivers")
+    assert s.ctx.physical_time() == o.time()
+    assert st["rebuilds"] == orb, (st["rebuilds"], orb)
+    same_interactions(s, o, f"{case} engine")
+    same_state(s.ctx.download(), o.particles(), f"{case} engine")
+    a = SphereSim(w)
+    a.first_iteration()
+    for _ in range(STEPS):
+        a.step()
+    assert a.rebuilds == orb
+    same_drivers(a.ctx.drivers(), o.drivers(), f"{case} operator path drivers")
+    same_state(a.ctx.download(), o.particles(), f"{case} operator path")
+    print(f"{case}: {w.n} spheres, {orb} rebuilds in {STEPS} steps", flush=True)
+    if case == "piston":
+        assert orb >= 1
+
+
+def test_moving_drivers_fp32_form(gpu_ok):
+    """The FP32-force form with the piston: drivers as in the FP64 form (they are FP64 in both), positions
+    within 1e-7 relative of the oracle after the run."""
+    w = dcs.piston()
+    sch = scheme(w)
+    o = po.Oracle(w.particles, w.domain, w.walls)
+    o.first_iteration(sch)
+    orb = o.run_steps(sch, STEPS)
+    s = SphereSim(w, form=native.XDEM_FORM_FP32)
+    s.first_iteration()
+    st = s.run(STEPS, poll=16)
+    same_drivers(s.ctx.drivers(), o.drivers(), "fp32 drivers")
+    assert st["rebuilds"] == orb
+    g, q = by_id(s.ctx.download()), by_id(o.particles())
+    assert float(np.max(np.abs(pos(g) - pos(q)))) / float(np.max(np.abs(pos(q)))) <= 1e-7
+
+
+def test_polyhedra_under_descending_lid(gpu_ok):
+    w = inputs.polyhedra(n=1500, spacing=1.5, jitter=0.05, seed=5, dt=2e-4, sigma_v=0.5, z0=0.6,
+                         periodic=(False, False, False))
+    lid = max(range(len(w.walls)), key=lambda k: (w.walls[k]["normal"][2], w.walls[k]["offset"]))
+    w.walls = [dict(x) for x in w.walls]
+    w.walls[lid]["motion"] = {"motion_type": "LINEAR_MOTION", "motion_vector": (0.0, 0.0, 1.0), "const_vel": -2.0}
+    sch = scheme(w)
+    o = po.Oracle(w.particles, w.domain, w.walls)
+    o.set_shapes(w.shapes)
+    o.first_iteration_poly(sch)
+    orb = o.run_steps_poly(sch, 120)
+    s = PolySim(w)
+    s.first_iteration()
+    st = s.run(120, poll=16)
+    same_drivers(s.ctx.drivers(), o.drivers(), "polyhedra drivers")
+    assert st["rebuilds"] == orb and orb >= 1
+    same_state(s.ctx.download(), o.particles(), "polyhedra", tol=1e-10)
+
+
+def test_deck_with_moving_walls(gpu_ok, tmp_path):
+    """A deck whose walls move (register_surface params LINEAR_MOTION and SHAKER) through the deck runner
+    equals the operator path and the oracle."""
+    from test_deck import wall_deck
+    w = dcs.piston()
+    w.walls[4]["motion"] = {"motion_type": "SHAKER", "amplitude": 0.01, "omega": 300.0, "shaker_dir": (0.0, 0.0, 1.0)}
+    steps = 100
+    deck = native.Deck(wall_deck(tmp_path, w, steps))
+    ctx = native.Context()
+    ctx.nwalls = len(w.walls)
+    stats = deck.run(ctx)
+    ref = SphereSim(w)
+    ref.first_iteration()
+    for _ in range(steps):
+        ref.step()
+    assert stats["rebuilds"] == ref.rebuilds and ref.rebuilds >= 1
+    same_drivers(ctx.drivers(), ref.ctx.drivers(), "deck drivers")
+    a, b = by_id(ctx.download()), by_id(ref.ctx.download())
+    for k in ("rx", "ry", "rz", "vx", "vy", "vz", "vrotx", "fx", "fz", "momx"):
+        assert np.array_equal(a[k], b[k]), k
+    o = po.Oracle(w.particles, w.domain, w.walls)
+    o.first_iteration(scheme(w))
+    o.run_steps(scheme(w), steps)
+    same_drivers(ctx.drivers(), o.drivers(), "deck drivers against the oracle")

@@ -8,7 +8,7 @@ trajectories within 1e-10 relative. The fused engine must equal the operator-by-
 import numpy as np
 import pytest
 
-from exadem_amd import inputs
+from exadem_amd import inputs, native
 from exadem_amd.sim import PolySim, scheme
 import pyoracle as po
 
@@ -264,3 +264,45 @@ def test_simulation_state(gpu_ok):
     _, _, _, v = s.ctx.contact_state()
     assert st["min_dn"] == min(0.0, float(np.min(v[:, 0]))) and st["min_dn"] < 0
     print(f"polyhedra: {st['n_interactions']} interactions, {st['n_active']} active, min dn {st['min_dn']:.6g}")
+
+
+def test_classifier_waves(gpu_ok):
+    """classify_interactions on the GPU (classify_interactions.cpp:52-136): the feature interactions split
+    into one wave per type (VV, VE, VF, EE). Wave sizes equal the oracle's feature counts by type and the
+    vertex-wall candidates its type-5 items; each wave holds exactly the list's rows of its type (owner,
+    partner, swap, sub-features), in increasing list index; the waves together hold every feature once.
+    The contact pass refuses lists that were not classified."""
+    w = poly_case("walls")
+    s = PolySim(w)
+    s.first_iteration()
+    o = oracle_for(w)
+    o.first_iteration_poly(scheme(w))
+    sizes = s.ctx.classifier_waves()
+    ii, jj, tt, sw, si, sj, fr = s.ctx.poly_interactions()
+    ot = o.poly_interactions()[2]
+    assert np.array_equal(sizes[:4], np.bincount(ot[ot < 4], minlength=4))
+    assert sizes[5] == int(np.count_nonzero(ot == 5)) and sizes[5] > 0
+    assert np.all(sizes[:4] > 0) and np.all(sizes[6:] == 0) and sizes[4] == 0
+    every = []
+    for t in range(4):
+        a, b, wsw, wsi, wsj, idx = s.ctx.classifier_wave(t)
+        assert len(a) == sizes[t]
+        assert np.all(np.diff(idx.astype(np.int64)) > 0)
+        sel = tt == t
+        got = sorted(zip(a.tolist(), b.tolist(), wsw.tolist(), wsi.tolist(), wsj.tolist()))
+        ref = sorted(zip(ii[sel].tolist(), jj[sel].tolist(), sw[sel].tolist(), si[sel].tolist(), sj[sel].tolist()))
+        assert got == ref, t
+        every.append(idx)
+    every = np.sort(np.concatenate(every).astype(np.int64))
+    assert np.array_equal(every, np.arange(int(np.sum(sizes[:4]))))
+    print(f"waves VV {sizes[0]}, VE {sizes[1]}, VF {sizes[2]}, EE {sizes[3]}, vertex-wall {sizes[5]}")
+    # operator path: nbh_polyhedron without classify_interactions is refused by the contact pass
+    c = s.ctx
+    c.move_particles()
+    c.chunk_neighbors_contact(w.rcut_inc)
+    c.nbh_polyhedron(w.rcut_inc)
+    c.reset_force_moment()
+    with pytest.raises(native.XdemError, match="classify_interactions"):
+        c.contact_polyhedron(w.cp, w.cpd, w.dt)
+    c.classify_interactions()
+    c.contact_polyhedron(w.cp, w.cpd, w.dt)
