@@ -40,6 +40,25 @@ def bytes_model(form, n, pairs):
     return 412 * n + 52 * pairs, 300 * n + 52 * pairs
 
 
+def bytes_model_poly(n, entries, nfeat, nvert=7.0):
+    """Algorithmic HBM bytes per step of the polyhedra path (FP64), per kernel.
+
+    k_poly_vertices, per particle: the record's position, spin/homothety and orientation (96 B), type (4 B),
+    the world vertices written (32 B each, nvert per particle on average: 8 for a cube, 6 for an
+    octahedron), the rebuild reference read (position 24 B + orientation 32 B).
+    k_poly_feat, per particle: its record (128 B) and vertices (32 B each) read once; per feature: the
+    classifier wave entry (16 B), friction history read and written (48 B), its contact terms written
+    (9 doubles), the contact flag (1 B) and dn (8 B).
+    k_poly_gather, per particle: record (128 B), inertia (24 B), list length (4 B), the accelerations,
+    moments and angular accelerations written (72 B); per list entry: word and feature range (12 B); per
+    feature: contact flag and contact terms read (73 B)."""
+    verts = (96 + 4 + 32 * nvert + 56) * n
+    feat = (128 + 32 * nvert) * n + (16 + 48 + 72 + 1 + 8) * nfeat
+    gather = (128 + 24 + 4 + 72) * n + 12 * entries + 73 * nfeat
+    return {"k_poly_vertices": verts, "k_poly_feat": feat, "k_poly_gather": gather,
+            "force_pass": feat + gather, "step": verts + feat + gather}
+
+
 def profiled_traffic(form):
     """HBM bytes per force launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE, scaled by a copy
     probe), read from the committed summary for this form; None when there is none."""
@@ -89,6 +108,9 @@ def main():
     ap.add_argument("--sur-steps", type=int, default=100, help="timed surrogate steps")
     ap.add_argument("--poly-n", type=int, default=200_000)
     ap.add_argument("--poly-steps", type=int, default=1000)
+    ap.add_argument("--no-trajectory", action="store_true", help="skip the surrogate's accumulated-error runs")
+    ap.add_argument("--only", default="", help="comma-separated forms to run after the headline line "
+                                                "(fp32, surrogate, polyhedra, mesh); default: all")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not provided "
@@ -149,10 +171,23 @@ def main():
     ms_step = 1e3 * elapsed / args.steps
     achieved = force_bytes / (f_ms * 1e-3) / 1e9 if f_ms > 0 else 0.0
     traffic, traffic_src = profiled_traffic(args.form)
+    # one neighbour rebuild (move_particles, chunk_neighbors_contact, nbh_sphere, classify_interactions),
+    # forced three times on the state the run left; wall time with the stream synchronised around it
+    reb = []
+    for _ in range(3):
+        sim64.ctx.synchronize()
+        tr = time.perf_counter()
+        sim64.rebuild()
+        sim64.ctx.synchronize()
+        reb.append(1e3 * (time.perf_counter() - tr))
+    rebuild_ms = float(np.median(reb))
+    log(f"{args.form}: rebuild {rebuild_ms:.3f} ms (median of {[round(r, 3) for r in reb]})")
     del sim64
 
     forms = {}
-    if args.form == "fp64" and not args.no_forms:
+    only = [k for k in args.only.split(",") if k]
+    want = lambda k: not only or k in only
+    if args.form == "fp64" and not args.no_forms and want("fp32"):
         # FP32-force form: throughput, and the error of its first force pass against the FP64 pass
         # from the same state (same contacts: the sign test is FP64)
         a = SphereSim(w, form=native.XDEM_FORM_FP64)
@@ -198,6 +233,7 @@ def main():
                          "contact_error_stated_as": "|F32 - F64| / |F64 - m g| and |M32 - M64| / |M64| over the "
                                                     "particles whose contact force (moment) is at least 1% of "
                                                     "the largest, first force pass"}
+    if args.form == "fp64" and not args.no_forms and want("surrogate"):
         # surrogate-accelerated forms: the trained network with K = 200, 100 and 10 DEM steps per step
         from exadem_amd import surrogate
         for form_key, wpath in (("surrogate", surrogate.WEIGHTS_K200), ("surrogate_k100", surrogate.WEIGHTS_K100),
@@ -236,16 +272,38 @@ def main():
                                                   "(one surrogate step against K DEM steps from the same state); "
                                                   "RMS error of the predicted mean contact force over a window, "
                                                   "units m0 a0, against zero contact force (free flight)"}
+            if form_key == "surrogate_k10" and not args.no_trajectory:
+                # accumulated error at the end of a drift-checked run against plain DEM over the same
+                # simulated time from the same start, and the same for free flight between the same checks
+                tr = surrogate.trajectory_errors(w, params, scales, nsteps=args.sur_steps, check_every=20, tol=0.01)
+                forms[form_key]["accumulated_error_over_radius"] = {
+                    "network": {"max": tr["network"]["max"], "rms": tr["network"]["rms"]},
+                    "free_flight": {"max": tr["free_flight"]["max"], "rms": tr["free_flight"]["rms"]},
+                    "dem_steps": args.sur_steps * K,
+                    "stated_as": "|x - x_DEM| / R over all particles after the run, against plain FP64 DEM over "
+                                 "the same simulated time from the same state"}
+                log(f"{form_key}: accumulated error rms {tr['network']['rms']:.3g} R (free flight "
+                    f"{tr['free_flight']['rms']:.3g} R), max {tr['network']['max']:.3g} R "
+                    f"(free flight {tr['free_flight']['max']:.3g} R)")
+            if K == 10:
+                forms[form_key]["target_5x_at_k10"] = ("met" if eff / value >= 5.0 else
+                                                       f"unmet: {eff / value:.2f}x; a K = 10 network pass costs "
+                                                       f"about {1e3 * el / args.sur_steps / ms_step:.0f} DEM steps")
             log(f"{form_key}: {args.sur_steps} steps of K={K} in {el:.3f} s, checks {sst['checks']}, "
                 f"max err {sst['max_err']:.3g} R (free flight {sst['max_err_free']:.3g} R), "
                 f"rms {sst['rms_err']:.3g} R (free flight {sst['rms_err_free']:.3g} R), "
                 f"force err {werr[-1][0] if werr else float('nan'):.3g} vs {werr[-1][1] if werr else float('nan'):.3g}")
             del g, s
+    if args.form == "fp64" and not args.no_forms and want("polyhedra"):
         # polyhedra: cubes (edge 1) and octahedra (circumradius 0.8), Minkowski radius 0.02, random
-        # orientations, jittered lattice (spacing 1.55, solid fraction 0.25) periodic in x and y with
-        # a floor and a lid, gravity, dt 2e-6
+        # orientations, grown on the GPU from a jittered lattice to solid fraction 0.45 of the box and
+        # settled under gravity, periodic in x and y with a floor and a lid, dt 2e-6
         from exadem_amd.sim import PolySim
-        wp = inputs.polyhedra(n=args.poly_n, spacing=1.55, jitter=0.05, seed=4, dt=2e-6)
+        tg = time.perf_counter()
+        wp = inputs.polyhedra_benchmark(n=args.poly_n)
+        gen_s = time.perf_counter() - tg
+        log(f"polyhedra: grew {wp.n} to solid fraction {wp.meta['solid_fraction']:.3f} (bed "
+            f"{wp.meta['bed_solid_fraction']:.3f}) in {gen_s:.1f} s")
         ps = PolySim(wp)
         ps.first_iteration()
         ps.run(20, poll=args.poll)
@@ -260,17 +318,64 @@ def main():
         ps.ctx.set_timing(False)
         pf_ms, _ = ps.ctx.kernel_timing(1)
         pi_ms, _ = ps.ctx.kernel_timing(0)
-        nfeat = ps.ctx.poly_interactions_count()
-        vol = float(np.sum(wp.particles["mass"] / 2500.0))
+        waves = ps.ctx.classifier_waves()
+        nfeat = int(np.sum(waves[:4]))
+        pnb = ps.ctx.nbh_stats()
+        nvert = float(np.mean([wp.shapes[t].nv for t in wp.particles["type"]]))
+        pbytes = bytes_model_poly(wp.n, 2 * pnb["pairs"], nfeat, nvert)  # both directions of each pair
+        pach = pbytes["force_pass"] / (pf_ms * 1e-3) / 1e9 if pf_ms > 0 else 0.0
+        reb = []
+        for _ in range(3):
+            ps.ctx.synchronize()
+            tr = time.perf_counter()
+            ps.rebuild()
+            ps.ctx.synchronize()
+            reb.append(1e3 * (time.perf_counter() - tr))
         forms["polyhedra"] = {"value": wp.n * args.poly_steps / el, "unit": "particle-steps/s",
                               "n": wp.n, "steps": args.poly_steps, "ms_per_step": 1e3 * el / args.poly_steps,
                               "force_avg_ms": pf_ms, "integrate_vertices_avg_ms": pi_ms,
-                              "feature_interactions": nfeat, "rebuilds": pst["rebuilds"],
-                              "solid_fraction": vol / float(np.prod(wp.meta["box"])),
-                              "workload": "cubes + octahedra 50/50, random orientations, jittered lattice "
-                                          "spacing 1.55, periodic x/y, floor and lid, dt 2e-6"}
-        log(f"polyhedra: {args.poly_steps} steps of {wp.n} in {el:.3f} s (force {pf_ms * 1e3:.1f} us)")
+                              "feature_interactions": nfeat,
+                              "classifier_waves": {"VV": int(waves[0]), "VE": int(waves[1]), "VF": int(waves[2]),
+                                                   "EE": int(waves[3]), "vertex_wall": int(waves[5])},
+                              "rebuilds": pst["rebuilds"], "rebuild_ms": float(np.median(reb)),
+                              "solid_fraction": wp.meta["solid_fraction"],
+                              "bed_solid_fraction": wp.meta["bed_solid_fraction"],
+                              "generation_s": gen_s,
+                              "roofline": {"bound": "hbm", "achieved": pach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                           "frac": pach / PEAK_HBM_GBS,
+                                           "kernel": "force pass: k_poly_feat + k_poly_gather",
+                                           "bytes_per_launch": pbytes["force_pass"], "avg_ms": pf_ms,
+                                           "step_bytes": pbytes["step"],
+                                           "step_frac": pbytes["step"] / (el / args.poly_steps) / 1e9 / PEAK_HBM_GBS},
+                              "workload": wp.meta.get("workload", "cubes + octahedra 50/50, random orientations, "
+                                                                  "jittered lattice spacing 1.55, periodic x/y, "
+                                                                  "floor and lid, dt 2e-6")}
+        log(f"polyhedra: {args.poly_steps} steps of {wp.n} in {el:.3f} s (force {pf_ms * 1e3:.1f} us, "
+            f"rebuild {np.median(reb):.2f} ms)")
+        if not args.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import pyoracle as po
+            op = po.Oracle(wp.particles, wp.domain, wp.walls)
+            op.set_shapes(wp.shapes)
+            schp = scheme(wp)
+            op.first_iteration_poly(schp)
+            steps_p = 0
+            tc = time.perf_counter()
+            while True:
+                op.run_steps_poly(schp, 1)
+                steps_p += 1
+                tcp = time.perf_counter() - tc
+                if tcp >= args.cpu_seconds:
+                    break
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            forms["polyhedra"]["cpu_baseline"] = {
+                "value": wp.n * steps_p / tcp, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+                "sample": f"CPU oracle (FP64 restatement of the reference polyhedra path, OpenMP) on the same "
+                          f"{wp.n}-polyhedron input: {steps_p} steps after first_iteration, {tcp:.1f} s"}
+            log(f"polyhedra cpu baseline: {wp.n * steps_p / tcp:.3g} particle-steps/s")
+            del op
         del ps
+    if args.form == "fp64" and not args.no_forms and want("mesh"):
         # STL mesh driver: a stationary triangulated plane through the box at mid-height (grid spacing
         # about 2, two triangles per cell, Minkowski radius 0.01) in the same sphere input, FP64 form
         from exadem_amd import stl
@@ -295,8 +400,9 @@ def main():
                          "mesh_contacts": int(np.count_nonzero(np.any(mfr != 0, axis=1))),
                          "spheres_with_candidates": int(len(np.unique(mi))),
                          "workload": "the FP64 input with a stationary triangulated plane at mid-height "
-                                     "(spacing ~2, Minkowski radius 0.01); mesh contacts evaluated in the "
-                                     "force pass after the pairs"}
+                                     "(spacing ~2, Minkowski radius 0.01); mesh contacts evaluated one "
+                                     "candidate per thread in a pass of their own, whose terms the force "
+                                     "pass adds in candidate order after the walls"}
         log(f"mesh: {len(mf)} triangles, {len(mi)} candidates on {len(np.unique(mi))} spheres: timed "
             f"{args.steps} steps in {rmesh['elapsed']:.3f} s (force {rmesh['f_ms'] * 1e3:.1f} us)")
         del msim
@@ -339,6 +445,8 @@ def main():
         "config": {"workload": f"polydisperse spheres {n}, phi 0.55, periodic {w.meta['box'][0]:.1f}x"
                                f"{w.meta['box'][1]:.1f}x{w.meta['box'][2]:.2f}, dt 5e-6, form {args.form}",
                    "n": n, "pairs": pairs, "rebuilds": st["rebuilds"], "polls": st["polls"],
+                   "rebuild_ms": rebuild_ms,
+                   "rebuild_ms_per_step_if_every_100_steps": rebuild_ms / 100.0,
                    "list_slots": st_nb2["slots"], "max_initial_overlap": w.meta["max_overlap"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "traffic_source": traffic_src,

@@ -726,6 +726,7 @@ int xdem_particles_upload(xdem_ctx* c, const xdem_particles* p) {
   for (auto* x : dtmp) (void)hipFree(x);
   XDEM_HIP(c, hipGetLastError());
   c->nl_valid = c->nl_prev_valid = c->perm_valid = false;
+  c->ml_valid = false;  // mesh candidates (and their sphere index) refer to the previous particles
   c->pending_kick = false;
   c->wl_nw = -1;
   return XDEM_OK;

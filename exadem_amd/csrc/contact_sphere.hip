@@ -68,10 +68,8 @@ struct ForceArgs {
   // (wall, particle), kCStateLen doubles each (null: not recorded)
   double* cstate;
   double* wstate;
-  // triangle-mesh driver (mesh.cap = 0: none) and its recorded contact state (null: not recorded)
-  MeshView mesh;
-  MeshList ml;
-  double* mstate;
+  // terms of the triangle-mesh contacts, evaluated by the mesh pass just before (nsph = 0: none)
+  MeshTerms mt;
 };
 
 
@@ -280,8 +278,7 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
       F = F + f;
       M = M + (cross(cpos - rx, f) + mint);
     }
-    if (a.mesh.cap)
-      mesh_contacts<KR>(a.mesh, a.ml, x, rx, Rx, mx, vx, wx, a.dt, a.cpd, F, M, INTEG ? nullptr : a.mstate);
+    if (a.mt.nsph) mesh_add(a.mt, x, F, M);
   }
   if (FUSED && INTEG) {
     const V3 al = angular_accel_d(Q4f{me.q.x, me.q.y, me.q.z, me.q.w}, M, wx, V3{s.ix[x], s.iy[x], s.iz[x]});
@@ -358,10 +355,7 @@ int launch_step(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, 
   if (planes) a.wl = c->wl[c->wl_cur];
   a.walls = walls_for_pass(c);
   a.nw = planes ? c->nwalls : 0;
-  if (meshed) {
-    a.mesh = c->mesh;
-    a.ml = c->ml[c->ml_cur];
-  }
+  if (meshed && (rc = launch_mesh_pass(c, cpd, dt, true, c->trig_slot, nullptr, &a.mt))) return rc;
   a.wstride = c->cap;
   a.g = c->grid;
   a.cp = cp;
@@ -410,10 +404,7 @@ int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd,
   if (planes) a.wl = c->wl[c->wl_cur];
   a.walls = walls_for_pass(c);
   a.nw = planes ? c->nwalls : 0;
-  if (meshed) {
-    a.mesh = c->mesh;
-    a.ml = c->ml[c->ml_cur];
-  }
+  double* mstate = nullptr;
   a.wstride = c->cap;
   a.g = c->grid;
   a.cp = cp;
@@ -446,9 +437,10 @@ int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd,
         if ((rc = dev_realloc(c, &c->mstate, m * kCStateLen))) return rc;
         c->cap_mstate = m;
       }
-      a.mstate = c->mstate;
+      mstate = c->mstate;
     }
   }
+  if (meshed && (rc = launch_mesh_pass(c, cpd, dt, guarded, c->trig_slot, mstate, &a.mt))) return rc;
   if (fused) launch_f<true>(c, a, c->kr_used, walls, guarded);
   else launch_f<false>(c, a, c->kr_used, walls, guarded);
   XDEM_HIP(c, hipGetLastError());

@@ -98,9 +98,8 @@ struct ForceArgs32 {
   double kdt_2;
   double thr2;
   int32_t trig_out;
-  // triangle-mesh driver (mesh.cap = 0: none)
-  MeshView mesh;
-  MeshList ml;
+  // terms of the triangle-mesh contacts, evaluated by the mesh pass just before (nsph = 0: none)
+  MeshTerms mt;
 };
 
 
@@ -314,7 +313,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       F = F + f;
       M = M + (cross(cpos - rx, f) + mint);
     }
-    if (a.mesh.cap) mesh_contacts<KR>(a.mesh, a.ml, x, rx, Rx, mx, vx64, wx64, a.dt, a.cpd, F, M);
+    if (a.mt.nsph) mesh_add(a.mt, x, F, M);
   }
   if (FUSED && INTEG) {
     const V3 al = angular_accel_g(Q4g{me.q.x, me.q.y, me.q.z, me.q.w}, M, V3{me.w.x, me.w.y, me.w.z},
@@ -403,10 +402,7 @@ int launch_force_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams&
   if (planes) a.wl = c->wl[c->wl_cur];
   a.walls = walls_for_pass(c);
   a.nw = planes ? c->nwalls : 0;
-  if (meshed) {
-    a.mesh = c->mesh;
-    a.ml = c->ml[c->ml_cur];
-  }
+  if (meshed && (rc = launch_mesh_pass(c, cpd, dt, guarded, c->trig_slot, nullptr, &a.mt))) return rc;
   a.wstride = c->cap;
   a.g = c->grid;
   a.cpd = cpd;
@@ -448,10 +444,7 @@ int launch_step_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams& 
   if (planes) a.wl = c->wl[c->wl_cur];
   a.walls = walls_for_pass(c);
   a.nw = planes ? c->nwalls : 0;
-  if (meshed) {
-    a.mesh = c->mesh;
-    a.ml = c->ml[c->ml_cur];
-  }
+  if (meshed && (rc = launch_mesh_pass(c, cpd, dt, true, c->trig_slot, nullptr, &a.mt))) return rc;
   a.wstride = c->cap;
   a.g = c->grid;
   a.cpd = cpd;

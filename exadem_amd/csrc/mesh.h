@@ -136,44 +136,64 @@ __device__ inline bool mesh_detect(const MeshView& m, uint32_t wd, V3 p, double 
 // in FP64 (both force forms): the driver sits at the mesh centre, stationary; no contact resets the
 // history. Adds the forces and moments to F and M. cs (null: not recorded): contact state per slot,
 // kCStateLen doubles (dn, contact point, f - ft, ft, sphere centre, zeros; zeros without contact).
+// Candidate slot j (kind and index in l.word[j]) of a sphere at rx (radius Rx, mass mx, velocity vx,
+// spin wx): the contact law against the mesh (ContactLawDriverFunc, contact_sphere.hpp:298-396) with its
+// friction and rolling history in slot j, and the terms it adds to the sphere's force and moment written
+// to out[0..5]; out of contact the history is cleared and the terms are -0.0, which adds nothing to any
+// value. cs: the recorded contact state of slot j (null: not recorded).
 template <bool KR>
-__device__ inline void mesh_contacts(const MeshView& m, const MeshList& l, int64_t x, V3 rx, double Rx, double mx,
-                                     V3 vx, V3 wx, double dt, const ContactParams& cpd, V3& F, V3& M,
-                                     double* cs = nullptr) {
-  const uint32_t mc = min(l.cnt[x], (uint32_t)m.cap);
-  for (uint32_t k = 0; k < mc; k++) {
-    const int64_t j = (int64_t)k * m.stride + x;
-    double dn;
-    V3 nrm, cpos;
-    if (!mesh_detect(m, l.word[j], rx, Rx, dn, nrm, cpos)) {
-      l.hx[j] = 0.0; l.hy[j] = 0.0; l.hz[j] = 0.0;
-      l.mhx[j] = 0.0; l.mhy[j] = 0.0; l.mhz[j] = 0.0;
-      if (cs) {
-        double* r = cs + (int64_t)kCStateLen * j;
-#pragma unroll
-        for (int q = 0; q < kCStateLen; q++) r[q] = 0.0;
-      }
-      continue;
-    }
-    V3 ft{l.hx[j], l.hy[j], l.hz[j]};
-    V3 mint{0.0, 0.0, 0.0};
-    if (KR) mint = {l.mhx[j], l.mhy[j], l.mhz[j]};
-    const V3 zero{0.0, 0.0, 0.0};
-    const V3 f = hooke_force<KR>(dn, nrm, dt, cpd, mx, ft, mint, cpos, rx, vx, wx, m.center, zero, zero);
-    l.hx[j] = ft.x; l.hy[j] = ft.y; l.hz[j] = ft.z;
-    if (KR) {
-      l.mhx[j] = mint.x; l.mhy[j] = mint.y; l.mhz[j] = mint.z;
-    }
+__device__ inline void mesh_candidate(const MeshView& m, const MeshList& l, int64_t j, V3 rx, double Rx, double mx,
+                                      V3 vx, V3 wx, double dt, const ContactParams& cpd, double* __restrict__ out,
+                                      double* cs) {
+  double dn;
+  V3 nrm, cpos;
+  if (!mesh_detect(m, l.word[j], rx, Rx, dn, nrm, cpos)) {
+    l.hx[j] = 0.0; l.hy[j] = 0.0; l.hz[j] = 0.0;
+    l.mhx[j] = 0.0; l.mhy[j] = 0.0; l.mhz[j] = 0.0;
     if (cs) {
-      const V3 fnv = f - ft;
-      const double rec[kCStateLen] = {dn,   cpos.x, cpos.y, cpos.z, fnv.x, fnv.y, fnv.z, ft.x,
-                                      ft.y, ft.z,   rx.x,   rx.y,   rx.z,   0.0,   0.0,   0.0};
       double* r = cs + (int64_t)kCStateLen * j;
 #pragma unroll
-      for (int q = 0; q < kCStateLen; q++) r[q] = rec[q];
+      for (int q = 0; q < kCStateLen; q++) r[q] = 0.0;
     }
-    F = F + f;
-    M = M + (cross(cpos - rx, f) + mint);
+    reinterpret_cast<double2*>(out)[0] = double2{-0.0, -0.0};
+    reinterpret_cast<double2*>(out)[1] = double2{-0.0, -0.0};
+    reinterpret_cast<double2*>(out)[2] = double2{-0.0, -0.0};
+    return;
+  }
+  V3 ft{l.hx[j], l.hy[j], l.hz[j]};
+  V3 mint{0.0, 0.0, 0.0};
+  if (KR) mint = {l.mhx[j], l.mhy[j], l.mhz[j]};
+  const V3 zero{0.0, 0.0, 0.0};
+  const V3 f = hooke_force<KR>(dn, nrm, dt, cpd, mx, ft, mint, cpos, rx, vx, wx, m.center, zero, zero);
+  l.hx[j] = ft.x; l.hy[j] = ft.y; l.hz[j] = ft.z;
+  if (KR) {
+    l.mhx[j] = mint.x; l.mhy[j] = mint.y; l.mhz[j] = mint.z;
+  }
+  if (cs) {
+    const V3 fnv = f - ft;
+    const double rec[kCStateLen] = {dn,   cpos.x, cpos.y, cpos.z, fnv.x, fnv.y, fnv.z, ft.x,
+                                    ft.y, ft.z,   rx.x,   rx.y,   rx.z,   0.0,   0.0,   0.0};
+    double* r = cs + (int64_t)kCStateLen * j;
+#pragma unroll
+    for (int q = 0; q < kCStateLen; q++) r[q] = rec[q];
+  }
+  const V3 mt = cross(cpos - rx, f) + mint;
+  reinterpret_cast<double2*>(out)[0] = double2{f.x, f.y};
+  reinterpret_cast<double2*>(out)[1] = double2{f.z, mt.x};
+  reinterpret_cast<double2*>(out)[2] = double2{mt.y, mt.z};
+}
+
+// The mesh terms of particle x added to its force and moment in candidate order, after the walls
+// (the same sums, in the same order, as evaluating each candidate here).
+__device__ inline void mesh_add(const MeshTerms& t, int64_t x, V3& F, V3& M) {
+  const int32_t i = t.idx[x];
+  if (i < 0) return;
+  const uint32_t c = t.scnt[i];
+  for (uint32_t k = 0; k < c; k++) {
+    const double2* r = reinterpret_cast<const double2*>(t.terms + ((int64_t)k * t.nsph + i) * 6);
+    const double2 a = r[0], b = r[1], d = r[2];
+    F = F + V3{a.x, a.y, b.x};
+    M = M + V3{b.y, d.x, d.y};
   }
 }
 

@@ -53,6 +53,15 @@ void mesh_free(xdem_ctx* c) {
   if (c->mstate) (void)hipFree(c->mstate);
   c->mstate = nullptr;
   c->cap_mstate = 0;
+  for (void* p : {(void*)c->mesh_sph, (void*)c->mesh_scnt, (void*)c->mesh_idx, (void*)c->mesh_terms})
+    if (p) (void)hipFree(p);
+  c->mesh_sph = nullptr;
+  c->mesh_scnt = nullptr;
+  c->mesh_idx = nullptr;
+  c->mesh_terms = nullptr;
+  c->cap_mesh_terms = 0;
+  c->mesh_nsph = 0;
+  c->mesh_kmax = 0;
   free_ml(c->ml[0]);
   free_ml(c->ml[1]);
   c->ml_alloc = 0;
@@ -82,6 +91,9 @@ static int alloc_ml(xdem_ctx* c) {
     if ((rc = dev_realloc(c, &l.mhy, m))) return rc;
     if ((rc = dev_realloc(c, &l.mhz, m))) return rc;
   }
+  if ((rc = dev_realloc(c, &c->mesh_sph, c->cap))) return rc;
+  if ((rc = dev_realloc(c, &c->mesh_scnt, c->cap))) return rc;
+  if ((rc = dev_realloc(c, &c->mesh_idx, c->cap))) return rc;
   c->ml_alloc = c->cap;
   c->mesh.stride = c->cap;
   c->ml_valid = false;  // the lists no longer match the particle capacity
@@ -234,6 +246,89 @@ __global__ __launch_bounds__(256) void k_mesh_carry(int64_t n, PState s, MeshVie
   if (cnt > (uint32_t)m.cap) atomicMax(over, (int32_t)cnt);
 }
 
+// The spheres that have mesh candidates, compacted per wavefront (ballot; one atomic per wavefront): their
+// index (idx, -1 for the others), counts, the number of them (ctr[0]) and the largest count (ctr[1]).
+// Their order only places the terms of the mesh pass; no sum depends on it.
+__global__ __launch_bounds__(256) void k_mesh_index(int64_t n, const uint32_t* __restrict__ cnt, uint32_t cap,
+                                                    uint32_t* __restrict__ sph, uint32_t* __restrict__ scnt,
+                                                    int32_t* __restrict__ idx, int32_t* __restrict__ ctr) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t c = x < n ? min(cnt[x], cap) : 0u;
+  const unsigned long long bal = __ballot(c > 0);
+  const int lane = threadIdx.x & 63;
+  int32_t base = 0;
+  if (bal) {
+    if (lane == 0) base = atomicAdd(ctr, (int32_t)__popcll(bal));
+    base = __shfl(base, 0, 64);
+  }
+  uint32_t cm = c;
+  for (int o = 32; o > 0; o >>= 1) cm = max(cm, (uint32_t)__shfl_down(cm, o, 64));
+  if (lane == 0 && cm) atomicMax(ctr + 1, (int32_t)cm);
+  if (x >= n) return;
+  if (c > 0) {
+    const int32_t i = base + (int32_t)__popcll(bal & ((1ull << lane) - 1ull));
+    sph[i] = (uint32_t)x;
+    scnt[i] = c;
+    idx[x] = i;
+  } else {
+    idx[x] = -1;
+  }
+}
+
+// One thread per (candidate k, sphere i): blockIdx.y = k, the sphere from blockIdx.x and the lane.
+template <bool KR, bool GUARD>
+__global__ __launch_bounds__(256) void k_mesh_pass(MeshView m, MeshList l, MeshTerms t, const Core* __restrict__ core,
+                                                   double dt, ContactParams cpd, const int32_t* __restrict__ flags,
+                                                   int32_t trig_in, double* cs) {
+  if (GUARD && (flags[1] || flags[trig_in])) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k = blockIdx.y;
+  if (i >= t.nsph || k >= t.scnt[i]) return;
+  const int64_t x = t.sph[i];
+  const Core me = core[x];
+  const V3 rx{me.xr.x, me.xr.y, me.xr.z}, vx{me.vm.x, me.vm.y, me.vm.z}, wx{me.w.x, me.w.y, me.w.z};
+  mesh_candidate<KR>(m, l, (int64_t)k * m.stride + x, rx, me.xr.w, me.vm.w, vx, wx, dt, cpd,
+                     t.terms + ((int64_t)k * t.nsph + i) * 6, cs);
+}
+
+static MeshTerms terms_view(xdem_ctx* c) {
+  MeshTerms t;
+  t.sph = c->mesh_sph;
+  t.scnt = c->mesh_scnt;
+  t.idx = c->mesh_idx;
+  t.terms = c->mesh_terms;
+  t.nsph = c->mesh_nsph;
+  t.kmax = c->mesh_kmax;
+  return t;
+}
+
+int launch_mesh_pass(xdem_ctx* c, const ContactParams& cpd, double dt, bool guarded, int32_t trig_in, double* cs,
+                     MeshTerms* t) {
+  *t = MeshTerms{};
+  if (!c->has_mesh || !c->ml_valid || c->mesh_nsph == 0) return XDEM_OK;
+  *t = terms_view(c);
+  const dim3 grid(grid_blocks(c->mesh_nsph, 256), (unsigned)c->mesh_kmax), block(256);
+  const MeshList& l = c->ml[c->ml_cur];
+  const Core* core = live(c).core;
+  if (c->kr_used) {
+    if (guarded)
+      hipLaunchKernelGGL((k_mesh_pass<true, true>), grid, block, 0, c->stream, c->mesh, l, *t, core, dt, cpd,
+                         c->d_flags, trig_in, cs);
+    else
+      hipLaunchKernelGGL((k_mesh_pass<true, false>), grid, block, 0, c->stream, c->mesh, l, *t, core, dt, cpd,
+                         c->d_flags, trig_in, cs);
+  } else {
+    if (guarded)
+      hipLaunchKernelGGL((k_mesh_pass<false, true>), grid, block, 0, c->stream, c->mesh, l, *t, core, dt, cpd,
+                         c->d_flags, trig_in, cs);
+    else
+      hipLaunchKernelGGL((k_mesh_pass<false, false>), grid, block, 0, c->stream, c->mesh, l, *t, core, dt, cpd,
+                         c->d_flags, trig_in, cs);
+  }
+  XDEM_HIP(c, hipGetLastError());
+  return XDEM_OK;
+}
+
 int mesh_carry(xdem_ctx* c, double rcut_inc, bool have_old) {
   if (!c->has_mesh || c->n == 0) return XDEM_OK;
   int rc = alloc_ml(c);
@@ -241,7 +336,7 @@ int mesh_carry(xdem_ctx* c, double rcut_inc, bool have_old) {
   const int64_t n = c->n;
   // reach of the bins
   unsigned long long* d_rmax = (unsigned long long*)c->d_mesh_over;
-  XDEM_HIP(c, hipMemsetAsync(c->d_mesh_over, 0, 4 * sizeof(int32_t), c->stream));
+  XDEM_HIP(c, hipMemsetAsync(c->d_mesh_over, 0, 8 * sizeof(int32_t), c->stream));
   hipLaunchKernelGGL(k_max_radius, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, live(c).core, d_rmax);
   XDEM_HIP(c, hipGetLastError());
   unsigned long long rb = 0;
@@ -259,14 +354,25 @@ int mesh_carry(xdem_ctx* c, double rcut_inc, bool have_old) {
   hipLaunchKernelGGL(k_mesh_carry, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, live(c), c->mesh, rcut_inc,
                      c->ml[nxt], c->ml[c->ml_cur], c->perm, old ? 1 : 0, d_over);
   XDEM_HIP(c, hipGetLastError());
-  int32_t over = 0;
-  XDEM_HIP(c, hipMemcpyAsync(&over, d_over, sizeof(over), hipMemcpyDeviceToHost, c->stream));
+  hipLaunchKernelGGL(k_mesh_index, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, c->ml[nxt].cnt,
+                     (uint32_t)c->mesh_cap, c->mesh_sph, c->mesh_scnt, c->mesh_idx, d_over + 1);
+  XDEM_HIP(c, hipGetLastError());
+  int32_t over[3] = {0, 0, 0};  // overflowing count, spheres with candidates, largest count
+  XDEM_HIP(c, hipMemcpyAsync(over, d_over, sizeof(over), hipMemcpyDeviceToHost, c->stream));
   XDEM_HIP(c, hipStreamSynchronize(c->stream));
   c->ml_cur = nxt;
-  if (over > 0) {
+  c->mesh_nsph = over[1];
+  c->mesh_kmax = over[2];
+  const int64_t nt = (int64_t)over[1] * over[2] * 6;
+  if (nt > c->cap_mesh_terms) {
+    const int64_t cap = nt + nt / 4 + 4096;
+    if ((rc = dev_realloc(c, &c->mesh_terms, cap))) return rc;
+    c->cap_mesh_terms = cap;
+  }
+  if (over[0] > 0) {
     c->ml_valid = false;
     return set_err(c, XDEM_ERR_CAPACITY,
-                   "mesh: a sphere has " + std::to_string(over) + " candidate mesh features, more than the " +
+                   "mesh: a sphere has " + std::to_string(over[0]) + " candidate mesh features, more than the " +
                        std::to_string(c->mesh_cap) + " slots given to xdem_set_mesh");
   }
   c->ml_valid = true;
@@ -308,7 +414,7 @@ extern "C" int xdem_set_mesh(xdem_ctx* c, int32_t id, const double* vertices, in
   if ((rc = dev_realloc(c, &c->d_mesh_v, nv))) return rc;
   if ((rc = dev_realloc(c, &c->d_mesh_e, std::max<int64_t>(ne, 1)))) return rc;
   if ((rc = dev_realloc(c, &c->d_mesh_f, std::max<int64_t>(nf, 1)))) return rc;
-  if ((rc = dev_realloc(c, &c->d_mesh_over, 4))) return rc;
+  if ((rc = dev_realloc(c, &c->d_mesh_over, 8))) return rc;
   XDEM_HIP(c, hipMemcpy(c->d_mesh_v, c->mesh_hv.data(), sizeof(V3) * nv, hipMemcpyHostToDevice));
   if (ne) XDEM_HIP(c, hipMemcpy(c->d_mesh_e, c->mesh_he.data(), sizeof(uint2) * ne, hipMemcpyHostToDevice));
   if (nf) XDEM_HIP(c, hipMemcpy(c->d_mesh_f, c->mesh_hf.data(), sizeof(uint4) * nf, hipMemcpyHostToDevice));

@@ -381,76 +381,70 @@ __device__ inline void for_each_feature(const PolyNbhArgs& a, int64_t x, uint32_
   }
 }
 
+// One thread per list slot (slot -> particle through nl.sx; padding slots hold ~0u): the feature count of
+// an owner entry, 0 for the others.
 __global__ __launch_bounds__(256) void k_feat_count(PolyNbhArgs a, uint32_t* __restrict__ fcount) {
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= a.n) return;
-  const uint32_t cnt = a.nl.cnt[x];
-  const uint64_t b = a.nl.base[x >> 6] + (uint64_t)(x & 63);
-  for (uint32_t k = 0; k < cnt; k++) {
-    const uint64_t sl = b + (uint64_t)k * kWave;
+  const int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl >= a.nl.slots) return;
+  const uint32_t x = a.nl.sx[sl];
+  uint32_t nf = 0;
+  if (x != 0xffffffffu) {
     const uint32_t wd = a.nl.word[sl];
-    uint32_t nf = 0;
     if (wd & kOwnerBit)
       for_each_feature(a, x, wd & kIdxMask, (int)((wd >> kShiftShift) & 31u), [&](int, int, int, int) { nf++; });
-    fcount[sl] = nf;
   }
+  fcount[sl] = nf;
 }
 
 __global__ __launch_bounds__(256) void k_feat_fill(PolyNbhArgs a, const uint64_t* __restrict__ start,
                                                    uint32_t* __restrict__ feat, uint32_t* __restrict__ fslot,
                                                    uint32_t* __restrict__ fown) {
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= a.n) return;
-  const uint32_t cnt = a.nl.cnt[x];
-  const uint64_t b = a.nl.base[x >> 6] + (uint64_t)(x & 63);
-  for (uint32_t k = 0; k < cnt; k++) {
-    const uint64_t sl = b + (uint64_t)k * kWave;
-    const uint32_t wd = a.nl.word[sl];
-    if (!(wd & kOwnerBit)) continue;
-    uint64_t o = start[sl];
-    for_each_feature(a, x, wd & kIdxMask, (int)((wd >> kShiftShift) & 31u), [&](int t, int sw, int si, int sj) {
-      // keys use owner().sub_, partner().sub_: the swap pass stores the vertex of the partner as partner sub
-      feat[o] = sw ? feat_word(t, 1, si, sj) : feat_word(t, 0, si, sj);
-      fslot[o] = (uint32_t)sl;
-      fown[o] = (uint32_t)x;
-      o++;
-    });
-  }
+  const int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl >= a.nl.slots) return;
+  const uint32_t x = a.nl.sx[sl];
+  if (x == 0xffffffffu) return;
+  const uint32_t wd = a.nl.word[sl];
+  if (!(wd & kOwnerBit)) return;
+  uint64_t o = start[sl];
+  for_each_feature(a, x, wd & kIdxMask, (int)((wd >> kShiftShift) & 31u), [&](int t, int sw, int si, int sj) {
+    // keys use owner().sub_, partner().sub_: the swap pass stores the vertex of the partner as partner sub
+    feat[o] = sw ? feat_word(t, 1, si, sj) : feat_word(t, 0, si, sj);
+    fslot[o] = (uint32_t)sl;
+    fown[o] = x;
+    o++;
+  });
 }
 
 // Feature range of every list slot: an owner entry's own range; a partner entry gets the range of the
 // owner's entry pointing back with the mirrored image code, or x = ~0u when there is none. The force
 // gather then reads one 8-byte record per entry instead of following slot -> owner slot -> range.
-__global__ __launch_bounds__(256) void k_feat_range(int64_t n, NbrList nl, const uint64_t* __restrict__ start,
+__global__ __launch_bounds__(256) void k_feat_range(NbrList nl, const uint64_t* __restrict__ start,
                                                     uint2* __restrict__ frange) {
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= n) return;
-  const uint32_t cnt = nl.cnt[x];
-  const uint64_t b = nl.base[x >> 6] + (uint64_t)(x & 63);
-  for (uint32_t k = 0; k < cnt; k++) {
-    const uint64_t sl = b + (uint64_t)k * kWave;
-    const uint32_t wd = nl.word[sl];
-    if (wd & kOwnerBit) {
-      frange[sl] = uint2{(uint32_t)start[sl], (uint32_t)(start[sl + 1] - start[sl])};
-      continue;
-    }
-    const uint32_t y = wd & kIdxMask;
-    const uint32_t code = (wd >> kShiftShift) & 31u;
-    const uint32_t want = 26u - code;
-    const uint32_t cy = nl.cnt[y];
-    const uint64_t by = nl.base[y >> 6] + (uint64_t)(y & 63);
-    uint32_t found = 0xffffffffu;
-    for (uint32_t k2 = 0; k2 < cy; k2++) {
-      const uint64_t s2 = by + (uint64_t)k2 * kWave;
-      const uint32_t w2 = nl.word[s2];
-      if ((w2 & kOwnerBit) && (w2 & kIdxMask) == (uint32_t)x && ((w2 >> kShiftShift) & 31u) == want) {
-        found = (uint32_t)s2;
-        break;
-      }
-    }
-    frange[sl] = found == 0xffffffffu ? uint2{0xffffffffu, 0u}
-                                      : uint2{(uint32_t)start[found], (uint32_t)(start[found + 1] - start[found])};
+  const int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl >= nl.slots) return;
+  const uint32_t x = nl.sx[sl];
+  if (x == 0xffffffffu) return;
+  const uint32_t wd = nl.word[sl];
+  if (wd & kOwnerBit) {
+    frange[sl] = uint2{(uint32_t)start[sl], (uint32_t)(start[sl + 1] - start[sl])};
+    return;
   }
+  const uint32_t y = wd & kIdxMask;
+  const uint32_t code = (wd >> kShiftShift) & 31u;
+  const uint32_t want = 26u - code;
+  const uint32_t cy = nl.cnt[y];
+  const uint64_t by = nl.base[y >> 6] + (uint64_t)(y & 63);
+  uint32_t found = 0xffffffffu;
+  for (uint32_t k2 = 0; k2 < cy; k2++) {
+    const uint64_t s2 = by + (uint64_t)k2 * kWave;
+    const uint32_t w2 = nl.word[s2];
+    if ((w2 & kOwnerBit) && (w2 & kIdxMask) == x && ((w2 >> kShiftShift) & 31u) == want) {
+      found = (uint32_t)s2;
+      break;
+    }
+  }
+  frange[sl] = found == 0xffffffffu ? uint2{0xffffffffu, 0u}
+                                    : uint2{(uint32_t)start[found], (uint32_t)(start[found + 1] - start[found])};
 }
 
 struct FeatCarryArgs {
@@ -463,14 +457,13 @@ struct FeatCarryArgs {
 
 // history of each new feature: the old owner pair (old owner, old partner) and the same feature key
 __global__ __launch_bounds__(256) void k_feat_carry(FeatCarryArgs a) {
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= a.n) return;
-  const uint32_t cnt = a.nl.cnt[x];
-  const uint64_t b = a.nl.base[x >> 6] + (uint64_t)(x & 63);
-  for (uint32_t k = 0; k < cnt; k++) {
-    const uint64_t sl = b + (uint64_t)k * kWave;
+  const int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl >= a.nl.slots) return;
+  const uint32_t x = a.nl.sx[sl];
+  if (x == 0xffffffffu) return;
+  {
     const uint32_t wd = a.nl.word[sl];
-    if (!(wd & kOwnerBit)) continue;
+    if (!(wd & kOwnerBit)) return;
     const uint64_t f0 = a.pl.start[sl], f1 = a.pl.start[sl + 1];
     uint64_t o0 = 0, o1 = 0;
     if (a.have_old) {
@@ -573,7 +566,7 @@ int poly_nbh(xdem_ctx* c, double rcut_inc) {
   XDEM_HIP(c, hipMemsetAsync(P.fcount, 0, sizeof(uint32_t) * std::max<int64_t>(L.slots, 1), c->stream));
   PolyNbhArgs a{n, live(c), c->grid, c->d_shapes, c->vtx, L, rcut_inc};
   if (n > 0) {
-    hipLaunchKernelGGL(k_feat_count, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a, P.fcount);
+    hipLaunchKernelGGL(k_feat_count, dim3(grid_blocks(L.slots, 256)), dim3(256), 0, c->stream, a, P.fcount);
     if ((rc = scan_u64_from_u32(c, P.fcount, P.start, L.slots, 1u))) return rc;
   }
   uint64_t nf = 0;
@@ -596,13 +589,13 @@ int poly_nbh(xdem_ctx* c, double rcut_inc) {
   if (nf >= 0xffffffffull) return set_err(c, XDEM_ERR_CAPACITY, "nbh_polyhedron: more than 2^32 - 2 features");
   P.nfeat = (int64_t)nf;
   if (n > 0) {
-    hipLaunchKernelGGL(k_feat_fill, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, a, P.start, P.feat, P.fslot,
-                       P.fown);
-    hipLaunchKernelGGL(k_feat_range, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, L, P.start,
+    hipLaunchKernelGGL(k_feat_fill, dim3(grid_blocks(L.slots, 256)), dim3(256), 0, c->stream, a, P.start, P.feat,
+                       P.fslot, P.fown);
+    hipLaunchKernelGGL(k_feat_range, dim3(grid_blocks(L.slots, 256)), dim3(256), 0, c->stream, L, P.start,
                        c->frange);
     const bool have_old = c->nl_prev_valid && c->perm_valid && c->pl_valid;
     FeatCarryArgs fa{n, L, c->nl[1 - c->nl_cur], P, c->pl[c->pl_cur], c->perm, have_old ? 1 : 0};
-    hipLaunchKernelGGL(k_feat_carry, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, fa);
+    hipLaunchKernelGGL(k_feat_carry, dim3(grid_blocks(L.slots, 256)), dim3(256), 0, c->stream, fa);
     XDEM_HIP(c, hipGetLastError());
     // vertex-wall candidates
     if (c->nwalls > 0) {

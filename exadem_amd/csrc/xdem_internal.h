@@ -334,6 +334,18 @@ struct MeshList {
   uint32_t* word = nullptr;
   double *hx = nullptr, *hy = nullptr, *hz = nullptr, *mhx = nullptr, *mhy = nullptr, *mhz = nullptr;
 };
+// Mesh contacts of a force pass, evaluated one candidate per thread just before it (k_mesh_pass): the
+// spheres that have candidates (sph), their candidate counts (scnt), each sphere's index among them (idx,
+// -1: none) and the force and moment terms of candidate k of sphere i at (k * nsph + i) * 6 (-0.0 when
+// not in contact), which the force pass adds in candidate order.
+struct MeshTerms {
+  const uint32_t* sph = nullptr;
+  const uint32_t* scnt = nullptr;
+  const int32_t* idx = nullptr;
+  double* terms = nullptr;
+  int64_t nsph = 0;
+  int kmax = 0;
+};
 }  // namespace xdem
 
 struct xdem_ctx {
@@ -482,7 +494,15 @@ struct xdem_ctx {
   int ml_cur = 0;
   bool ml_valid = false;
   int64_t ml_alloc = 0;             // particle capacity ml was allocated for
-  int32_t* d_mesh_over = nullptr;   // [2] largest candidate count of the last build, max radius bits
+  int32_t* d_mesh_over = nullptr;   // [0..1] max radius bits, [2] largest overflowing count, [3] spheres
+                                    // with candidates, [4] largest candidate count
+  uint32_t* mesh_sph = nullptr;     // MeshTerms arrays (sph, scnt, idx: particle capacity each)
+  uint32_t* mesh_scnt = nullptr;
+  int32_t* mesh_idx = nullptr;
+  double* mesh_terms = nullptr;
+  int64_t cap_mesh_terms = 0;
+  int64_t mesh_nsph = 0;
+  int mesh_kmax = 0;
   double* mstate = nullptr;         // recorded contact state per mesh slot (kCStateLen doubles each)
   int64_t cap_mstate = 0;
 };
@@ -541,6 +561,12 @@ int launch_step(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, 
 int alloc_walls(xdem_ctx* c, int nw);
 int mesh_carry(xdem_ctx* c, double rcut_inc, bool have_old);  // mesh candidates + history carry-over
 void mesh_free(xdem_ctx* c);
+// Mesh contacts of the coming force pass (k_mesh_pass over the live records): histories updated and the
+// terms written for the pass to add; guarded: a no-op when the pass would be (stall or trigger in
+// flags[trig_in]); cs: recorded contact state per mesh slot (null: not recorded). *t receives the view
+// the force pass reads (t->nsph = 0: no mesh contacts to add).
+int launch_mesh_pass(xdem_ctx* c, const ContactParams& cpd, double dt, bool guarded, int32_t trig_in, double* cs,
+                     MeshTerms* t);
 int set_history_form(xdem_ctx* c, bool fp32);
 // a rolling stiffness comes into use (clears the sphere list's moment histories the first time)
 int use_kr(xdem_ctx* c);

@@ -241,8 +241,99 @@ def polyhedra(n=200_000, spacing=1.7, jitter=0.05, seed=4, density=2500.0, minko
     return w
 
 
-def polyhedra_benchmark(n=200_000):
+def _start_with_accelerations(sim):
+    """Rebuild and one force pass ending with force_to_accel, so that the next step kicks with accelerations
+    (the first_iteration of the reference decks leaves forces in place of accelerations, which at the
+    overlaps of a growing packing would kick the particles by F dt / 2 instead of F dt / 2m)."""
+    sim.rebuild()
+    sim.ctx.reset_force_moment()
+    sim.compute_force()
+    sim.ctx.force_to_accel()
+
+
+def polyhedra_dense(n=200_000, phi=0.45, seed=4, dt=2e-6, growth=1.01, relax_steps=200, settle_steps=3000,
+                    dt_grow=1e-3, log=None):
+    """Polyhedra at solid fraction phi of the box, grown with the GPU DEM and settled under gravity.
+
+    Starts from the jittered lattice of `polyhedra` (spacing 1.55, solid fraction about 0.25, periodic in
+    x and y, floor and lid) and grows every particle's homothety by `growth` per stage (mass x h^3,
+    inertia x h^5, bounding radius x h, cells re-sized to 2 max radius + rcut_inc) until the particles'
+    volume fills phi of the box; each stage relaxes for relax_steps DEM steps of dt_grow without gravity.
+    The grown particles then settle onto the floor under gravity for settle_steps steps of dt_grow, and
+    their velocities are zeroed; dt is set to `dt`. meta gives the achieved solid fraction of the box,
+    that of the settled bed (particle volume over the box area times the bed's top), the homothety and the
+    smallest overlap (dn) of the last pass."""
+    import copy as _copy
+    from .sim import PolySim
+    w = polyhedra(n=n, spacing=1.55, jitter=0.05, seed=seed, dt=dt)
+    L, _, top = w.meta["box"]
+    vbox = L * L * top
+    p = {k: np.array(v, copy=True) for k, v in w.particles.items()}
+    vol_of = lambda q: np.array([w.shapes[t].volume for t in q["type"]]) * q["homothety"] ** 3
+    phi0 = float(np.sum(vol_of(p))) / vbox
+    h_target = (phi / phi0) ** (1.0 / 3.0)
+    rb_max = float(np.max(p["radius"]))
+    g = _copy.copy(w)
+    g.gravity = (0.0, 0.0, 0.0)
+    g.dt = dt_grow
+    sim = PolySim(g)
+    h, stages = 1.0, 0
+    while h < h_target:
+        h_new = min(h * growth, h_target)
+        f = h_new / h
+        p["homothety"] = p["homothety"] * f
+        p["radius"] = p["radius"] * f
+        p["mass"] = p["mass"] * f ** 3
+        for a in ("inx", "iny", "inz"):
+            p[a] = p[a] * f ** 5
+        g.domain = domain((0.0, 0.0, 0.0), (L, L, top), (True, True, False), 2 * rb_max * h_new + w.rcut_inc)
+        for k in ("vx", "vy", "vz", "vrotx", "vroty", "vrotz"):  # each stage starts at rest
+            p[k] = np.zeros_like(p[k])
+        sim.w = g
+        sim.ctx.set_domain(g.domain)
+        sim.ctx.upload(p)
+        _start_with_accelerations(sim)
+        st = sim.run(relax_steps, poll=64)
+        p = sim.ctx.download()
+        h = h_new
+        stages += 1
+        if log:
+            vmax = float(np.max(np.abs(np.stack([p["vx"], p["vy"], p["vz"]]))))
+            log(f"polyhedra_dense: stage {stages}, homothety {h:.4f}, rebuilds {st['rebuilds']}, max |v| {vmax:.3g}")
+    g.gravity = w.gravity
+    for k in ("vx", "vy", "vz", "vrotx", "vroty", "vrotz"):
+        p[k] = np.zeros_like(p[k])
+    sim.w = g
+    sim.ctx.upload(p)
+    _start_with_accelerations(sim)
+    rs = sim.run(settle_steps, poll=64)
+    st = sim.ctx.simulation_state()
+    if log:
+        log(f"polyhedra_dense: settled, {rs['rebuilds']} rebuilds, {st['n_active']} active interactions")
+    p = sim.ctx.download()
+    del sim
+    for k in ("vx", "vy", "vz", "vrotx", "vroty", "vrotz", "fx", "fy", "fz", "momx", "momy", "momz", "arotx",
+              "aroty", "arotz"):
+        p[k] = np.zeros_like(p[k])
+    out = _copy.copy(w)
+    out.particles = p
+    out.domain = g.domain
+    vp = float(np.sum(vol_of(p)))
+    bed = float(np.max(p["rz"] + p["radius"]))
+    achieved = vp / vbox
+    out.meta = dict(w.meta, solid_fraction=achieved, bed_solid_fraction=vp / (L * L * bed), bed_top=bed,
+                    homothety=h, growth_stages=stages, min_dn=st["min_dn"], interactions=st["n_interactions"],
+                    active_interactions=st["n_active"],
+                    workload=f"cubes + octahedra 50/50, random orientations, grown from a jittered lattice to "
+                             f"solid fraction {achieved:.3f} of the box (homothety {h:.4f}) and settled under "
+                             f"gravity (bed solid fraction {vp / (L * L * bed):.3f}), periodic x/y, floor and lid, "
+                             f"dt {dt:g}")
+    return out
+
+
+def polyhedra_benchmark(n=200_000, log=None):
     """The polyhedra benchmark input: n cubes and octahedra 50/50 (edge 1.0, circumradius 0.8, Minkowski
-    radius 0.02), uniform random orientations (seed 4), jittered lattice of spacing 1.55 (solid fraction
-    about 0.25), periodic in x and y with a floor and a lid, gravity -9.81 z, dt 2e-6."""
-    return polyhedra(n=n, spacing=1.55, jitter=0.05, seed=4, dt=2e-6)
+    radius 0.02 before growth), uniform random orientations (seed 4), grown from a jittered lattice of
+    spacing 1.55 to solid fraction 0.45 of the box and settled under gravity (polyhedra_dense), periodic
+    in x and y with a floor and a lid, gravity -9.81 z, dt 2e-6."""
+    return polyhedra_dense(n=n, phi=0.45, seed=4, dt=2e-6, log=log)
