@@ -40,29 +40,30 @@ def bytes_model(form, n, pairs):
     return 412 * n + 52 * pairs, 300 * n + 52 * pairs
 
 
-def bytes_model_poly(n, entries, nfeat, nvert=7.0):
-    """Algorithmic HBM bytes per step of the polyhedra path (FP64), per kernel.
+def bytes_model_poly(n, entries, nfeat, contacts, nvert=7.0):
+    """Algorithmic HBM bytes per step of the polyhedra path (FP64), per kernel: what each kernel must move
+    at least once.
 
     k_poly_vertices, per particle: the record's position, spin/homothety and orientation (96 B), type (4 B),
-    the world vertices written (32 B each, nvert per particle on average: 8 for a cube, 6 for an
-    octahedron), the rebuild reference read (position 24 B + orientation 32 B).
+    the rebuild reference read (position 24 B + orientation 32 B), the world vertices written (32 B each,
+    nvert per particle on average: 8 for a cube, 6 for an octahedron).
     k_poly_feat, per particle: its record (128 B) and vertices (32 B each) read once; per feature: the
-    classifier wave entry (16 B), friction history read and written (48 B), its contact terms written
-    (9 doubles), the contact flag (1 B) and dn (8 B).
+    classifier entry (16 B), the contact flag (1 B) and dn (8 B) written; per feature in contact: the
+    friction history read and written (48 B) and its force and moment terms written (72 B).
     k_poly_gather, per particle: record (128 B), inertia (24 B), list length (4 B), the accelerations,
     moments and angular accelerations written (72 B); per list entry: word and feature range (12 B); per
-    feature: contact flag and contact terms read (73 B)."""
-    verts = (96 + 4 + 32 * nvert + 56) * n
-    feat = (128 + 32 * nvert) * n + (16 + 48 + 72 + 1 + 8) * nfeat
-    gather = (128 + 24 + 4 + 72) * n + 12 * entries + 73 * nfeat
+    feature: the contact flag (1 B); per feature in contact: its terms read (72 B)."""
+    verts = (96 + 4 + 56 + 32 * nvert) * n
+    feat = (128 + 32 * nvert) * n + 25 * nfeat + 120 * contacts
+    gather = (128 + 24 + 4 + 72) * n + 12 * entries + nfeat + 72 * contacts
     return {"k_poly_vertices": verts, "k_poly_feat": feat, "k_poly_gather": gather,
             "force_pass": feat + gather, "step": verts + feat + gather}
 
 
-def profiled_traffic(form):
+def profiled_traffic(stem):
     """HBM bytes per force launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE, scaled by a copy
-    probe), read from the committed summary for this form; None when there is none."""
-    path = os.path.join(ROOT, "profiles", f"k_force_{form}_1M_spheres.md")
+    probe), read from the committed summary profiles/<stem>.md; None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"{stem}.md")
     try:
         with open(path) as fh:
             for line in fh:
@@ -109,6 +110,8 @@ def main():
     ap.add_argument("--poly-n", type=int, default=200_000)
     ap.add_argument("--poly-steps", type=int, default=1000)
     ap.add_argument("--no-trajectory", action="store_true", help="skip the surrogate's accumulated-error runs")
+    ap.add_argument("--poly-cache", default=None, help="npz file for the grown polyhedra input (read if it "
+                                                         "holds the same input, else written)")
     ap.add_argument("--only", default="", help="comma-separated forms to run after the headline line "
                                                 "(fp32, surrogate, polyhedra, mesh); default: all")
     args = ap.parse_args()
@@ -170,7 +173,7 @@ def main():
     value = n * args.steps / elapsed
     ms_step = 1e3 * elapsed / args.steps
     achieved = force_bytes / (f_ms * 1e-3) / 1e9 if f_ms > 0 else 0.0
-    traffic, traffic_src = profiled_traffic(args.form)
+    traffic, traffic_src = profiled_traffic(f"k_force_{args.form}_1M_spheres")
     # one neighbour rebuild (move_particles, chunk_neighbors_contact, nbh_sphere, classify_interactions),
     # forced three times on the state the run left; wall time with the stream synchronised around it
     reb = []
@@ -295,12 +298,12 @@ def main():
                 f"force err {werr[-1][0] if werr else float('nan'):.3g} vs {werr[-1][1] if werr else float('nan'):.3g}")
             del g, s
     if args.form == "fp64" and not args.no_forms and want("polyhedra"):
-        # polyhedra: cubes (edge 1) and octahedra (circumradius 0.8), Minkowski radius 0.02, random
-        # orientations, grown on the GPU from a jittered lattice to solid fraction 0.45 of the box and
-        # settled under gravity, periodic in x and y with a floor and a lid, dt 2e-6
+        # polyhedra: cubes (edge 1) and octahedra (circumradius 0.8), Minkowski radius 0.02, density 1,
+        # random orientations, grown on the GPU from a jittered lattice to solid fraction 0.45 of the box,
+        # periodic in x and y with a floor and a lid, gravity, dt 2e-6
         from exadem_amd.sim import PolySim
         tg = time.perf_counter()
-        wp = inputs.polyhedra_benchmark(n=args.poly_n)
+        wp = inputs.polyhedra_benchmark(n=args.poly_n, cache=args.poly_cache, log=log)
         gen_s = time.perf_counter() - tg
         log(f"polyhedra: grew {wp.n} to solid fraction {wp.meta['solid_fraction']:.3f} (bed "
             f"{wp.meta['bed_solid_fraction']:.3f}) in {gen_s:.1f} s")
@@ -322,7 +325,9 @@ def main():
         nfeat = int(np.sum(waves[:4]))
         pnb = ps.ctx.nbh_stats()
         nvert = float(np.mean([wp.shapes[t].nv for t in wp.particles["type"]]))
-        pbytes = bytes_model_poly(wp.n, 2 * pnb["pairs"], nfeat, nvert)  # both directions of each pair
+        _, _, ptt, _, _, _, pfr = ps.ctx.poly_interactions()
+        contacts = int(np.count_nonzero(np.any(pfr[ptt < 4] != 0, axis=1)))  # features with friction
+        pbytes = bytes_model_poly(wp.n, 2 * pnb["pairs"], nfeat, contacts, nvert)  # both directions
         pach = pbytes["force_pass"] / (pf_ms * 1e-3) / 1e9 if pf_ms > 0 else 0.0
         reb = []
         for _ in range(3):
@@ -334,7 +339,7 @@ def main():
         forms["polyhedra"] = {"value": wp.n * args.poly_steps / el, "unit": "particle-steps/s",
                               "n": wp.n, "steps": args.poly_steps, "ms_per_step": 1e3 * el / args.poly_steps,
                               "force_avg_ms": pf_ms, "integrate_vertices_avg_ms": pi_ms,
-                              "feature_interactions": nfeat,
+                              "feature_interactions": nfeat, "features_in_contact": contacts,
                               "classifier_waves": {"VV": int(waves[0]), "VE": int(waves[1]), "VF": int(waves[2]),
                                                    "EE": int(waves[3]), "vertex_wall": int(waves[5])},
                               "rebuilds": pst["rebuilds"], "rebuild_ms": float(np.median(reb)),
@@ -343,6 +348,8 @@ def main():
                               "generation_s": gen_s,
                               "roofline": {"bound": "hbm", "achieved": pach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                                            "frac": pach / PEAK_HBM_GBS,
+                                           "traffic": profiled_traffic("poly_force_200k_polyhedra")[0],
+                                           "traffic_source": profiled_traffic("poly_force_200k_polyhedra")[1],
                                            "kernel": "force pass: k_poly_feat + k_poly_gather",
                                            "bytes_per_launch": pbytes["force_pass"], "avg_ms": pf_ms,
                                            "step_bytes": pbytes["step"],

@@ -104,7 +104,8 @@ __device__ inline void fetch_entry(const ForceArgs& a, uint64_t sl, uint32_t wd,
 // are applied and the next record is written to core_out, with the displacement test of the rebuild
 // trigger; accelerations, moments and angular accelerations are then not stored (the next pass
 // recomputes them).
-template <bool FUSED, bool KR, bool WALLS, bool GUARD, bool INTEG = false>
+// MESH: add the mesh terms of the pass (the particle's index among those with candidates is read ahead)
+template <bool FUSED, bool KR, bool WALLS, bool GUARD, bool INTEG = false, bool MESH = false>
 __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
   if (GUARD) {
     if (a.flags[1]) return;
@@ -125,6 +126,7 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
   __syncthreads();
   const int64_t x = b0 + threadIdx.x;
   if (x >= a.n) return;
+  const int32_t midx = MESH && a.mt.nsph ? a.mt.idx[x] : -1;
   const PState& s = a.s;
   uint32_t* __restrict__ word = a.nl.word;
   const uint32_t cnt = a.nl.cnt[x];
@@ -278,8 +280,8 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
       F = F + f;
       M = M + (cross(cpos - rx, f) + mint);
     }
-    if (a.mt.nsph) mesh_add(a.mt, x, F, M);
   }
+  if (MESH) mesh_add_at(a.mt, midx, F, M);
   if (FUSED && INTEG) {
     const V3 al = angular_accel_d(Q4f{me.q.x, me.q.y, me.q.z, me.q.w}, M, wx, V3{s.ix[x], s.iy[x], s.iz[x]});
     Core nx = me;
@@ -309,11 +311,12 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
   }
 }
 
+// operator path: the driver variant (WALLS) also adds the mesh terms
 template <bool FUSED, bool KR, bool WALLS>
 static void launch_g(xdem_ctx* c, const ForceArgs& a, bool guarded) {
   const dim3 grid(grid_blocks(a.n, 256)), block(256);
-  if (guarded) hipLaunchKernelGGL((k_force_fp64<FUSED, KR, WALLS, true>), grid, block, 0, c->stream, a);
-  else hipLaunchKernelGGL((k_force_fp64<FUSED, KR, WALLS, false>), grid, block, 0, c->stream, a);
+  if (guarded) hipLaunchKernelGGL((k_force_fp64<FUSED, KR, WALLS, true, false, WALLS>), grid, block, 0, c->stream, a);
+  else hipLaunchKernelGGL((k_force_fp64<FUSED, KR, WALLS, false, false, WALLS>), grid, block, 0, c->stream, a);
 }
 
 template <bool FUSED>
@@ -347,7 +350,6 @@ int launch_step(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, 
   if ((cp.kr != 0.0 || cpd.kr != 0.0) && (rc = use_kr(c))) return rc;
   const bool planes = c->nwalls > 0 && c->wl_valid;
   const bool meshed = c->has_mesh && c->ml_valid;
-  const bool walls = planes || meshed;
   ForceArgs a{};
   a.n = c->n;
   a.s = live(c);
@@ -371,13 +373,16 @@ int launch_step(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, 
   a.kdt_2 = dt * 0.5;
   a.thr2 = thr2;
   const dim3 grid(grid_blocks(a.n, 256)), block(256);
+  // engine step: drivers and mesh terms each only when present
+#define STEP(KRV, W, M) hipLaunchKernelGGL((k_force_fp64<true, KRV, W, true, true, M>), grid, block, 0, c->stream, a)
   if (c->kr_used) {
-    if (walls) hipLaunchKernelGGL((k_force_fp64<true, true, true, true, true>), grid, block, 0, c->stream, a);
-    else hipLaunchKernelGGL((k_force_fp64<true, true, false, true, true>), grid, block, 0, c->stream, a);
+    if (planes) { if (meshed) STEP(true, true, true); else STEP(true, true, false); }
+    else { if (meshed) STEP(true, false, true); else STEP(true, false, false); }
   } else {
-    if (walls) hipLaunchKernelGGL((k_force_fp64<true, false, true, true, true>), grid, block, 0, c->stream, a);
-    else hipLaunchKernelGGL((k_force_fp64<true, false, false, true, true>), grid, block, 0, c->stream, a);
+    if (planes) { if (meshed) STEP(false, true, true); else STEP(false, true, false); }
+    else { if (meshed) STEP(false, false, true); else STEP(false, false, false); }
   }
+#undef STEP
   XDEM_HIP(c, hipGetLastError());
   c->cs_valid = false;  // the integrating pass does not record
   *input = live(c).core;

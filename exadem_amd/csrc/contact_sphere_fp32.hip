@@ -154,7 +154,8 @@ __device__ inline void fetch32(const ForceArgs32& a, uint64_t sl, uint32_t wd, E
 // Four wavefronts per SIMD: the pass needs 130 VGPRs unconstrained (three wavefronts); capped at 128
 // it spills nothing and hides more of the partner-gather latency (integrating pass 164.9 -> 154.9 us
 // at 1M spheres).
-template <bool FUSED, bool KR, bool WALLS, bool GUARD, bool INTEG = false>
+// MESH: add the mesh terms of the pass (the particle's index among those with candidates is read ahead)
+template <bool FUSED, bool KR, bool WALLS, bool GUARD, bool INTEG = false, bool MESH = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_force_fp32(ForceArgs32 a) {
   if (GUARD) {
     if (a.flags[1]) return;
@@ -174,6 +175,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   __syncthreads();
   const int64_t x = b0 + threadIdx.x;
   if (x >= a.n) return;
+  const int32_t midx = MESH && a.mt.nsph ? a.mt.idx[x] : -1;
   const PState& s = a.s;
   uint32_t* __restrict__ word = a.nl.word;
   const uint32_t cnt = a.nl.cnt[x];
@@ -313,8 +315,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       F = F + f;
       M = M + (cross(cpos - rx, f) + mint);
     }
-    if (a.mt.nsph) mesh_add(a.mt, x, F, M);
   }
+  if (MESH) mesh_add_at(a.mt, midx, F, M);
   if (FUSED && INTEG) {
     const V3 al = angular_accel_g(Q4g{me.q.x, me.q.y, me.q.z, me.q.w}, M, V3{me.w.x, me.w.y, me.w.z},
                                   V3{s.ix[x], s.iy[x], s.iz[x]});
@@ -383,8 +385,8 @@ int set_history_form(xdem_ctx* c, bool fp32) {
 template <bool FUSED, bool KR, bool WALLS>
 static void launch32(xdem_ctx* c, const ForceArgs32& a, bool guarded) {
   const dim3 grid(grid_blocks(a.n, 256)), block(256);
-  if (guarded) hipLaunchKernelGGL((k_force_fp32<FUSED, KR, WALLS, true>), grid, block, 0, c->stream, a);
-  else hipLaunchKernelGGL((k_force_fp32<FUSED, KR, WALLS, false>), grid, block, 0, c->stream, a);
+  if (guarded) hipLaunchKernelGGL((k_force_fp32<FUSED, KR, WALLS, true, false, WALLS>), grid, block, 0, c->stream, a);
+  else hipLaunchKernelGGL((k_force_fp32<FUSED, KR, WALLS, false, false, WALLS>), grid, block, 0, c->stream, a);
 }
 
 int launch_force_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, bool fused,
@@ -436,7 +438,6 @@ int launch_step_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams& 
   if ((cp.kr != 0.0 || cpd.kr != 0.0) && (rc = use_kr(c))) return rc;
   const bool planes = c->nwalls > 0 && c->wl_valid;
   const bool meshed = c->has_mesh && c->ml_valid;
-  const bool walls = planes || meshed;
   ForceArgs32 a{};
   a.n = c->n;
   a.s = live(c);
@@ -462,13 +463,15 @@ int launch_step_fp32(xdem_ctx* c, const ContactParams& cp, const ContactParams& 
   a.kdt_2 = dt * 0.5;
   a.thr2 = thr2;
   const dim3 grid(grid_blocks(a.n, 256)), block(256);
+#define STEP(KRV, W, M) hipLaunchKernelGGL((k_force_fp32<true, KRV, W, true, true, M>), grid, block, 0, c->stream, a)
   if (c->kr_used) {
-    if (walls) hipLaunchKernelGGL((k_force_fp32<true, true, true, true, true>), grid, block, 0, c->stream, a);
-    else hipLaunchKernelGGL((k_force_fp32<true, true, false, true, true>), grid, block, 0, c->stream, a);
+    if (planes) { if (meshed) STEP(true, true, true); else STEP(true, true, false); }
+    else { if (meshed) STEP(true, false, true); else STEP(true, false, false); }
   } else {
-    if (walls) hipLaunchKernelGGL((k_force_fp32<true, false, true, true, true>), grid, block, 0, c->stream, a);
-    else hipLaunchKernelGGL((k_force_fp32<true, false, false, true, true>), grid, block, 0, c->stream, a);
+    if (planes) { if (meshed) STEP(false, true, true); else STEP(false, true, false); }
+    else { if (meshed) STEP(false, false, true); else STEP(false, false, false); }
   }
+#undef STEP
   XDEM_HIP(c, hipGetLastError());
   *input = live(c).core;
   std::swap(live(c).core, c->core_alt);

@@ -183,18 +183,31 @@ __device__ inline void mesh_candidate(const MeshView& m, const MeshList& l, int6
   reinterpret_cast<double2*>(out)[2] = double2{mt.y, mt.z};
 }
 
-// The mesh terms of particle x added to its force and moment in candidate order, after the walls
-// (the same sums, in the same order, as evaluating each candidate here).
-__device__ inline void mesh_add(const MeshTerms& t, int64_t x, V3& F, V3& M) {
-  const int32_t i = t.idx[x];
+// The mesh terms of particle x (index i among the particles with candidates, -1: none) added to its
+// force and moment in candidate order, after the walls (the same sums, in the same order, as evaluating
+// each candidate here).
+__device__ inline void mesh_add_at(const MeshTerms& t, int32_t i, V3& F, V3& M) {
   if (i < 0) return;
   const uint32_t c = t.scnt[i];
-  for (uint32_t k = 0; k < c; k++) {
-    const double2* r = reinterpret_cast<const double2*>(t.terms + ((int64_t)k * t.nsph + i) * 6);
-    const double2 a = r[0], b = r[1], d = r[2];
-    F = F + V3{a.x, a.y, b.x};
-    M = M + V3{b.y, d.x, d.y};
+  // loads of four candidates in flight together, then their sums in order
+  for (uint32_t k0 = 0; k0 < c; k0 += 4) {
+    double2 r[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (k0 + q < c) {
+        const double2* p = reinterpret_cast<const double2*>(t.terms + ((int64_t)(k0 + q) * t.nsph + i) * 6);
+        r[q][0] = p[0];
+        r[q][1] = p[1];
+        r[q][2] = p[2];
+      }
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (k0 + q < c) {
+        F = F + V3{r[q][0].x, r[q][0].y, r[q][1].x};
+        M = M + V3{r[q][1].y, r[q][2].x, r[q][2].y};
+      }
   }
 }
+__device__ inline void mesh_add(const MeshTerms& t, int64_t x, V3& F, V3& M) { mesh_add_at(t, t.idx[x], F, M); }
 
 }  // namespace xdem

@@ -329,7 +329,7 @@ int launch_mesh_pass(xdem_ctx* c, const ContactParams& cpd, double dt, bool guar
   return XDEM_OK;
 }
 
-int mesh_carry(xdem_ctx* c, double rcut_inc, bool have_old) {
+int mesh_carry(xdem_ctx* c, double rcut_inc, bool have_old, bool poly) {
   if (!c->has_mesh || c->n == 0) return XDEM_OK;
   int rc = alloc_ml(c);
   if (rc) return rc;
@@ -344,16 +344,23 @@ int mesh_carry(xdem_ctx* c, double rcut_inc, bool have_old) {
   XDEM_HIP(c, hipStreamSynchronize(c->stream));
   double rmax;
   memcpy(&rmax, &rb, sizeof(rmax));
-  const double reach = rmax + rcut_inc + c->mesh.rm;
+  // a sphere's candidates lie within R + rcut_inc + r_m of its centre; a polyhedron's (radius: its
+  // bounding sphere) also through its own edges, which the vertex-edge filter extends by rcut_inc of their
+  // length (at most 2 R) at both ends
+  const double reach = (poly ? rmax * (1.0 + 2.0 * rcut_inc) : rmax) + rcut_inc + c->mesh.rm;
   if (!(reach <= c->mesh_reach) || rcut_inc != c->mesh_rv) {
     if ((rc = build_bins(c, reach * 1.05, rcut_inc))) return rc;
   }
   const bool old = have_old && c->ml_valid;
   const int nxt = 1 - c->ml_cur;
   int32_t* d_over = c->d_mesh_over + 2;
-  hipLaunchKernelGGL(k_mesh_carry, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, live(c), c->mesh, rcut_inc,
-                     c->ml[nxt], c->ml[c->ml_cur], c->perm, old ? 1 : 0, d_over);
-  XDEM_HIP(c, hipGetLastError());
+  if (poly) {
+    if ((rc = launch_poly_mesh_carry(c, rcut_inc, c->ml[nxt], c->ml[c->ml_cur], old, d_over))) return rc;
+  } else {
+    hipLaunchKernelGGL(k_mesh_carry, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, live(c), c->mesh,
+                       rcut_inc, c->ml[nxt], c->ml[c->ml_cur], c->perm, old ? 1 : 0, d_over);
+    XDEM_HIP(c, hipGetLastError());
+  }
   hipLaunchKernelGGL(k_mesh_index, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, c->ml[nxt].cnt,
                      (uint32_t)c->mesh_cap, c->mesh_sph, c->mesh_scnt, c->mesh_idx, d_over + 1);
   XDEM_HIP(c, hipGetLastError());
@@ -372,7 +379,7 @@ int mesh_carry(xdem_ctx* c, double rcut_inc, bool have_old) {
   if (over[0] > 0) {
     c->ml_valid = false;
     return set_err(c, XDEM_ERR_CAPACITY,
-                   "mesh: a sphere has " + std::to_string(over[0]) + " candidate mesh features, more than the " +
+                   "mesh: a particle has " + std::to_string(over[0]) + " candidate mesh features, more than the " +
                        std::to_string(c->mesh_cap) + " slots given to xdem_set_mesh");
   }
   c->ml_valid = true;

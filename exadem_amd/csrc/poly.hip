@@ -381,19 +381,46 @@ __device__ inline void for_each_feature(const PolyNbhArgs& a, int64_t x, uint32_
   }
 }
 
-// One thread per list slot (slot -> particle through nl.sx; padding slots hold ~0u): the feature count of
-// an owner entry, 0 for the others.
-__global__ __launch_bounds__(256) void k_feat_count(PolyNbhArgs a, uint32_t* __restrict__ fcount) {
+// One thread per list slot (slot -> particle through nl.sx; padding slots hold ~0u): an owner slot
+// enumerates its pair's features once, writing its count and up to kFeatPerSlot feature words into its
+// own region of tmp; a slot with more raises *over, and k_feat_fill then enumerates again into place.
+constexpr int kFeatPerSlot = 24;
+__global__ __launch_bounds__(256) void k_feat_enum(PolyNbhArgs a, uint32_t* __restrict__ fcount,
+                                                   uint32_t* __restrict__ tmp, int32_t* __restrict__ over) {
   const int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (sl >= a.nl.slots) return;
   const uint32_t x = a.nl.sx[sl];
   uint32_t nf = 0;
   if (x != 0xffffffffu) {
     const uint32_t wd = a.nl.word[sl];
-    if (wd & kOwnerBit)
-      for_each_feature(a, x, wd & kIdxMask, (int)((wd >> kShiftShift) & 31u), [&](int, int, int, int) { nf++; });
+    if (wd & kOwnerBit) {
+      uint32_t* out = tmp + sl * kFeatPerSlot;
+      for_each_feature(a, x, wd & kIdxMask, (int)((wd >> kShiftShift) & 31u), [&](int t, int sw, int si, int sj) {
+        if (nf < (uint32_t)kFeatPerSlot) out[nf] = sw ? feat_word(t, 1, si, sj) : feat_word(t, 0, si, sj);
+        nf++;
+      });
+      if (nf > (uint32_t)kFeatPerSlot) *over = 1;
+    }
   }
   fcount[sl] = nf;
+}
+
+// Copies each owner slot's enumerated words to its place in the feature list.
+__global__ __launch_bounds__(256) void k_feat_place(NbrList nl, const uint32_t* __restrict__ fcount,
+                                                    const uint32_t* __restrict__ tmp, const uint64_t* __restrict__ start,
+                                                    uint32_t* __restrict__ feat, uint32_t* __restrict__ fslot,
+                                                    uint32_t* __restrict__ fown) {
+  const int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl >= nl.slots) return;
+  const uint32_t nf = fcount[sl];
+  if (nf == 0) return;
+  const uint32_t x = nl.sx[sl];
+  const uint64_t o = start[sl];
+  for (uint32_t q = 0; q < nf; q++) {
+    feat[o + q] = tmp[sl * kFeatPerSlot + q];
+    fslot[o + q] = (uint32_t)sl;
+    fown[o + q] = x;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_feat_fill(PolyNbhArgs a, const uint64_t* __restrict__ start,
@@ -536,6 +563,264 @@ __global__ __launch_bounds__(256) void k_poly_wall(int64_t n, PState s, const xd
   nwl.mask[x] = mask;
 }
 
+// ------------------------------------------------------------------------------------------ mesh driver
+// Polyhedron against the stationary triangle mesh (RShape driver). A candidate is one 32-bit word:
+//   bits 31..29  type - 7: 7 particle vertex / mesh vertex, 8 particle vertex / mesh edge, 9 particle
+//                vertex / mesh face, 10 particle edge / mesh edge, 11 particle edge / mesh vertex,
+//                12 particle face / mesh vertex (interaction_enum.hpp:40-46)
+//   bits 28..25  particle sub-feature
+//   bits 24..0   mesh feature index
+constexpr uint32_t kPmTypeShift = 29, kPmSubShift = 25, kPmIdxMask = (1u << 25) - 1u;
+__host__ __device__ inline uint32_t pm_word(int type, int sub, uint32_t k) {
+  return ((uint32_t)(type - 7) << kPmTypeShift) | ((uint32_t)sub << kPmSubShift) | k;
+}
+__host__ __device__ inline int pm_type(uint32_t wd) { return 7 + (int)(wd >> kPmTypeShift); }
+__host__ __device__ inline int pm_sub(uint32_t wd) { return (int)((wd >> kPmSubShift) & 15u); }
+
+// Candidates of one particle in the loop order of add_driver_interaction for RShapeDriver
+// (nbh_polyhedron_driver.hpp:96-200), over the mesh features of the particle's bin [q0, q1), which hold
+// in ascending word order (vertices, edges, faces, each by index) every feature that can pass a filter:
+// vertices against mesh vertices, edges, faces; edges against mesh edges; then the mesh vertices whose box
+// (the vertex enlarged by r_m) meets obb_i against the particle's edges and faces. The face boxes that
+// gate the vertex-face pass contain every point that passes its filter, so that gate is implied.
+template <typename F>
+__device__ inline void for_each_mesh_feature(const MeshView& m, const xdem_shape& sa, const Verts& va, double ra,
+                                             const OBBd& obb_i, uint32_t q0, uint32_t q1, double rv, F&& fn) {
+  uint32_t qe = q0;
+  while (qe < q1 && (m.feat[qe] >> kMeshKindShift) == 0u) qe++;
+  uint32_t qf = qe;
+  while (qf < q1 && (m.feat[qf] >> kMeshKindShift) == 1u) qf++;
+  const double rm = m.rm;
+  for (int i = 0; i < sa.nv; i++) {
+    const V3 v = va[i];
+    for (uint32_t q = q0; q < qe; q++) {
+      const uint32_t k = m.feat[q] & kMeshIdxMask;
+      if (filter_vv(rv, v, ra, m.v[k], rm)) fn(7, i, k);
+    }
+    for (uint32_t q = qe; q < qf; q++) {
+      const uint32_t k = m.feat[q] & kMeshIdxMask;
+      const uint2 e = m.e[k];
+      if (filter_ve(rv, v, ra, m.v[e.x], m.v[e.y], rm)) fn(8, i, k);
+    }
+    for (uint32_t q = qf; q < q1; q++) {
+      const uint32_t k = m.feat[q] & kMeshIdxMask;
+      const uint4 f = m.f[k];
+      const V3 t[3] = {m.v[f.x], m.v[f.y], m.v[f.z]};
+      double dist;
+      V3 nn;
+      bool early;
+      if (mesh_vf_core(v, t, ra + rm + rv, dist, nn, early)) fn(9, i, k);
+    }
+  }
+  for (int i = 0; i < sa.ne; i++) {
+    const V3 a = va[sa.edges[i][0]], b = va[sa.edges[i][1]];
+    for (uint32_t q = qe; q < qf; q++) {
+      const uint32_t k = m.feat[q] & kMeshIdxMask;
+      const uint2 e = m.e[k];
+      if (filter_ee(rv, a, b, ra, m.v[e.x], m.v[e.y], rm)) fn(10, i, k);
+    }
+  }
+  for (uint32_t q = q0; q < qe; q++) {
+    const uint32_t k = m.feat[q] & kMeshIdxMask;
+    const V3 mv = m.v[k];
+    OBBd ov;
+    ov.c = mv;
+    ov.e1 = {1, 0, 0};
+    ov.e2 = {0, 1, 0};
+    ov.e3 = {0, 0, 1};
+    ov.ext = {0, 0, 0};
+    obb_enlarge(ov, rm);
+    if (!obb_intersect(ov, obb_i)) continue;
+    for (int i = 0; i < sa.ne; i++)
+      if (filter_ve(rv, mv, rm, va[sa.edges[i][0]], va[sa.edges[i][1]], ra)) fn(11, i, k);
+    for (int i = 0; i < sa.nf; i++) {
+      double dist;
+      V3 nn;
+      bool early;
+      if (vf_core(mv, va, sa, i, rm + ra + rv, dist, nn, early)) fn(12, i, k);
+    }
+  }
+}
+
+// Candidate lists of every particle against the mesh (slot k of particle x at k * stride + x), the
+// history of each carried from the previous lists by word (the previous particle through perm). The
+// particle's box for the mesh-vertex gate is the shape box rotated and moved without the homothety and
+// enlarged by rVerlet, as add_driver_interaction builds it.
+__global__ __launch_bounds__(256) void k_poly_mesh_carry(int64_t n, PState s, const xdem_shape* __restrict__ shapes,
+                                                         const d4* __restrict__ vtx, MeshView m, double rv,
+                                                         MeshList nl, MeshList ol, const uint32_t* __restrict__ perm,
+                                                         int have_old, int32_t* __restrict__ over) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  const Core cx = s.core[x];
+  const V3 p{cx.xr.x, cx.xr.y, cx.xr.z};
+  const double fx = (p.x - m.lo.x) * m.inv_h, fy = (p.y - m.lo.y) * m.inv_h, fz = (p.z - m.lo.z) * m.inv_h;
+  uint32_t cnt = 0;
+  if (fx >= 0.0 && fy >= 0.0 && fz >= 0.0 && fx < (double)m.dim[0] && fy < (double)m.dim[1] &&
+      fz < (double)m.dim[2]) {
+    const int64_t b = ((int64_t)(int)fz * m.dim[1] + (int)fy) * m.dim[0] + (int)fx;
+    const uint32_t q0 = m.start[b], q1 = m.start[b + 1];
+    if (q1 > q0) {
+      const xdem_shape& sa = shapes[s.type[x]];
+      const double ra = sa.minkowski * cx.w.w;
+      const Verts va = verts_of(vtx, x, V3{0, 0, 0});
+      OBBd obb_i = compute_obb(sa, p, cx.q, 1.0);
+      obb_enlarge(obb_i, rv);
+      const uint32_t o = have_old ? perm[x] : 0u;
+      const uint32_t oc = have_old ? min(ol.cnt[o], (uint32_t)m.cap) : 0u;
+      for_each_mesh_feature(m, sa, va, ra, obb_i, q0, q1, rv, [&](int t, int i, uint32_t k) {
+        const uint32_t wd = pm_word(t, i, k);
+        if (cnt < (uint32_t)m.cap) {
+          const int64_t j = (int64_t)cnt * m.stride + x;
+          double hx = 0, hy = 0, hz = 0, mx = 0, my = 0, mz = 0;
+          for (uint32_t kk = 0; kk < oc; kk++) {
+            const int64_t jo = (int64_t)kk * m.stride + o;
+            if (ol.word[jo] == wd) {
+              hx = ol.hx[jo]; hy = ol.hy[jo]; hz = ol.hz[jo];
+              mx = ol.mhx[jo]; my = ol.mhy[jo]; mz = ol.mhz[jo];
+              break;
+            }
+          }
+          nl.word[j] = wd;
+          nl.hx[j] = hx; nl.hy[j] = hy; nl.hz[j] = hz;
+          nl.mhx[j] = mx; nl.mhy[j] = my; nl.mhz[j] = mz;
+        }
+        cnt++;
+      });
+    }
+  }
+  nl.cnt[x] = cnt;
+  if (cnt > (uint32_t)m.cap) atomicMax(over, (int32_t)cnt);
+}
+
+int launch_poly_mesh_carry(xdem_ctx* c, double rv, const MeshList& nl, const MeshList& ol, bool have_old,
+                           int32_t* over) {
+  hipLaunchKernelGGL(k_poly_mesh_carry, dim3(grid_blocks(c->n, 256)), dim3(256), 0, c->stream, c->n, live(c),
+                     c->d_shapes, c->vtx, c->mesh, rv, nl, ol, c->perm, have_old ? 1 : 0, over);
+  XDEM_HIP(c, hipGetLastError());
+  return XDEM_OK;
+}
+
+// Detection of one candidate (detect<7..12>, shape_detection_api.hpp:357-406): 7-10 with the particle
+// feature as i, 11-12 with the mesh vertex as i.
+__device__ inline Det poly_mesh_detect(const MeshView& m, uint32_t wd, const xdem_shape& sa, const Verts& va,
+                                       double ra) {
+  const int t = pm_type(wd), sub = pm_sub(wd);
+  const uint32_t k = wd & kPmIdxMask;
+  const double rm = m.rm;
+  switch (t) {
+    case 7:
+      return detect_vv(va[sub], ra, m.v[k], rm);
+    case 8: {
+      const uint2 e = m.e[k];
+      return detect_ve(va[sub], ra, m.v[e.x], m.v[e.y], rm);
+    }
+    case 9: {
+      const uint4 f = m.f[k];
+      const V3 tr[3] = {m.v[f.x], m.v[f.y], m.v[f.z]};
+      const V3 vi = va[sub];
+      double dist;
+      V3 nn;
+      bool early;
+      const bool in = mesh_vf_core(vi, tr, ra + rm, dist, nn, early);
+      if (early) return det_none();
+      const double dn = dist - (ra + rm);
+      return {in, dn, nn, vi - nn * (ra + 0.5 * dn)};
+    }
+    case 10: {
+      const uint2 e = m.e[k];
+      return detect_ee(va[sa.edges[sub][0]], va[sa.edges[sub][1]], ra, m.v[e.x], m.v[e.y], rm);
+    }
+    case 11:
+      return detect_ve(m.v[k], rm, va[sa.edges[sub][0]], va[sa.edges[sub][1]], ra);
+    default:
+      return detect_vf(m.v[k], rm, va, sa, sub, ra);
+  }
+}
+
+struct PolyMeshArgs {
+  MeshView m;
+  MeshList l;
+  MeshTerms t;
+  PState s;
+  const xdem_shape* shapes;
+  const d4* vtx;
+  double dt;
+  ContactParams cpd;
+  const int32_t* flags;
+  double* cs;  // recorded contact state per candidate slot (null: not recorded)
+};
+
+// One thread per (candidate k = blockIdx.y, particle i): ContactLawRShapeDriverFunc
+// (polyhedron/contact.hpp:300-432) against the stationary mesh (its centre, no velocity, no spin). 7-10:
+// the law with the particle as i, its force f and moment (cp - r) x f + m on the particle. 11-12: the law
+// with the mesh vertex as i; the particle takes -f and (cp - r) x (-f) - m, and the friction is stored
+// negated, as the reference does. The terms go to t.terms for the gather; -0.0 without contact.
+template <bool KR, bool GUARD>
+__global__ __launch_bounds__(256) void k_poly_mesh_pass(PolyMeshArgs a) {
+  if (GUARD && (a.flags[1] || a.flags[0])) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k = blockIdx.y;
+  if (i >= a.t.nsph || k >= a.t.scnt[i]) return;
+  const int64_t x = a.t.sph[i];
+  const int64_t j = (int64_t)k * a.m.stride + x;
+  const MeshList& l = a.l;
+  double* out = a.t.terms + ((int64_t)k * a.t.nsph + i) * 6;
+  const uint32_t wd = l.word[j];
+  const Core cx = a.s.core[x];
+  const xdem_shape& sa = a.shapes[a.s.type[x]];
+  const double ra = sa.minkowski * cx.w.w;
+  const Verts va = verts_of(a.vtx, x, V3{0, 0, 0});
+  const Det d = poly_mesh_detect(a.m, wd, sa, va, ra);
+  if (!d.contact) {
+    l.hx[j] = 0.0; l.hy[j] = 0.0; l.hz[j] = 0.0;
+    l.mhx[j] = 0.0; l.mhy[j] = 0.0; l.mhz[j] = 0.0;
+    if (a.cs) {
+      double* r = a.cs + (int64_t)kCStateLen * j;
+#pragma unroll
+      for (int q = 0; q < kCStateLen; q++) r[q] = 0.0;
+    }
+    reinterpret_cast<double2*>(out)[0] = double2{-0.0, -0.0};
+    reinterpret_cast<double2*>(out)[1] = double2{-0.0, -0.0};
+    reinterpret_cast<double2*>(out)[2] = double2{-0.0, -0.0};
+    return;
+  }
+  const V3 rx{cx.xr.x, cx.xr.y, cx.xr.z}, vx{cx.vm.x, cx.vm.y, cx.vm.z}, wx{cx.w.x, cx.w.y, cx.w.z};
+  const double mx = cx.vm.w;
+  const V3 zero{0.0, 0.0, 0.0};
+  V3 ft{l.hx[j], l.hy[j], l.hz[j]};
+  V3 mint{0.0, 0.0, 0.0};
+  if (KR) mint = {l.mhx[j], l.mhy[j], l.mhz[j]};
+  V3 fp, mp, fnv, fst;
+  if (pm_type(wd) <= 10) {
+    const V3 f = hooke_force<KR>(d.dn, d.n, a.dt, a.cpd, mx, ft, mint, d.cp, rx, vx, wx, a.m.center, zero, zero);
+    fp = f;
+    mp = cross(d.cp - rx, f) + mint;
+    fnv = f - ft;
+    fst = ft;
+  } else {
+    const V3 f = hooke_force<KR>(d.dn, d.n, a.dt, a.cpd, mx, ft, mint, d.cp, a.m.center, zero, zero, rx, vx, wx);
+    fp = -f;
+    mp = cross(d.cp - rx, fp) + (-mint);
+    fnv = ft - f;
+    fst = -ft;
+  }
+  l.hx[j] = fst.x; l.hy[j] = fst.y; l.hz[j] = fst.z;
+  if (KR) {
+    l.mhx[j] = mint.x; l.mhy[j] = mint.y; l.mhz[j] = mint.z;
+  }
+  if (a.cs) {
+    const double rec[kCStateLen] = {d.dn, d.cp.x, d.cp.y, d.cp.z, fnv.x, fnv.y, fnv.z, fst.x,
+                                    fst.y, fst.z, rx.x,   rx.y,   rx.z,   0.0,   0.0,   0.0};
+    double* r = a.cs + (int64_t)kCStateLen * j;
+#pragma unroll
+    for (int q = 0; q < kCStateLen; q++) r[q] = rec[q];
+  }
+  reinterpret_cast<double2*>(out)[0] = double2{fp.x, fp.y};
+  reinterpret_cast<double2*>(out)[1] = double2{fp.z, mp.x};
+  reinterpret_cast<double2*>(out)[2] = double2{mp.y, mp.z};
+}
+
 template <typename T>
 static int galloc(xdem_ctx* c, T** p, int64_t count) {
   if (*p) (void)hipFree(*p);
@@ -546,7 +831,6 @@ static int galloc(xdem_ctx* c, T** p, int64_t count) {
 
 int poly_nbh(xdem_ctx* c, double rcut_inc) {
   if (!c->nl_valid) return set_err(c, XDEM_ERR_STATE, "nbh_polyhedron: run chunk_neighbors_contact first");
-  if (c->has_mesh) return set_err(c, XDEM_ERR_ARG, "nbh_polyhedron: mesh drivers act on spheres only");
   const int64_t n = c->n;
   int rc;
   if ((rc = poly_compute_vertices(c, false, 0.0, false))) return rc;
@@ -563,14 +847,25 @@ int poly_nbh(xdem_ctx* c, double rcut_inc) {
     if ((rc = galloc(c, &c->frange, cap))) return rc;
     c->cap_frange = cap;
   }
-  XDEM_HIP(c, hipMemsetAsync(P.fcount, 0, sizeof(uint32_t) * std::max<int64_t>(L.slots, 1), c->stream));
+  if (L.slots > c->cap_feat_tmp) {
+    const int64_t cap = (int64_t)(L.slots * 1.25) + 4096;
+    if ((rc = galloc(c, &c->feat_tmp, cap * kFeatPerSlot))) return rc;
+    c->cap_feat_tmp = cap;
+  }
+  if (!c->d_feat_over && (rc = galloc(c, &c->d_feat_over, 2))) return rc;
+  XDEM_HIP(c, hipMemsetAsync(c->d_feat_over, 0, sizeof(int32_t), c->stream));
   PolyNbhArgs a{n, live(c), c->grid, c->d_shapes, c->vtx, L, rcut_inc};
   if (n > 0) {
-    hipLaunchKernelGGL(k_feat_count, dim3(grid_blocks(L.slots, 256)), dim3(256), 0, c->stream, a, P.fcount);
+    hipLaunchKernelGGL(k_feat_enum, dim3(grid_blocks(L.slots, 256)), dim3(256), 0, c->stream, a, P.fcount,
+                       c->feat_tmp, c->d_feat_over);
     if ((rc = scan_u64_from_u32(c, P.fcount, P.start, L.slots, 1u))) return rc;
   }
   uint64_t nf = 0;
-  if (n > 0 && (rc = read_u64(c, P.start + L.slots, &nf))) return rc;
+  int32_t over = 0;
+  if (n > 0) {
+    XDEM_HIP(c, hipMemcpyAsync(&over, c->d_feat_over, sizeof(over), hipMemcpyDeviceToHost, c->stream));
+    if ((rc = read_u64(c, P.start + L.slots, &nf))) return rc;
+  }
   if ((int64_t)nf > P.cap_feat) {
     const int64_t cap = (int64_t)(nf * 1.25) + 4096;
     if ((rc = galloc(c, &P.feat, cap)) || (rc = galloc(c, &P.hx, cap)) || (rc = galloc(c, &P.hy, cap)) ||
@@ -589,8 +884,12 @@ int poly_nbh(xdem_ctx* c, double rcut_inc) {
   if (nf >= 0xffffffffull) return set_err(c, XDEM_ERR_CAPACITY, "nbh_polyhedron: more than 2^32 - 2 features");
   P.nfeat = (int64_t)nf;
   if (n > 0) {
-    hipLaunchKernelGGL(k_feat_fill, dim3(grid_blocks(L.slots, 256)), dim3(256), 0, c->stream, a, P.start, P.feat,
-                       P.fslot, P.fown);
+    if (over)
+      hipLaunchKernelGGL(k_feat_fill, dim3(grid_blocks(L.slots, 256)), dim3(256), 0, c->stream, a, P.start, P.feat,
+                         P.fslot, P.fown);
+    else
+      hipLaunchKernelGGL(k_feat_place, dim3(grid_blocks(L.slots, 256)), dim3(256), 0, c->stream, L, P.fcount,
+                         c->feat_tmp, P.start, P.feat, P.fslot, P.fown);
     hipLaunchKernelGGL(k_feat_range, dim3(grid_blocks(L.slots, 256)), dim3(256), 0, c->stream, L, P.start,
                        c->frange);
     const bool have_old = c->nl_prev_valid && c->perm_valid && c->pl_valid;
@@ -620,6 +919,7 @@ int poly_nbh(xdem_ctx* c, double rcut_inc) {
       c->pw_cur = pn;
       c->pw_valid = true;
     }
+    if ((rc = mesh_carry(c, rcut_inc, have_old, true))) return rc;
   }
   c->pl_cur = nxt;
   c->pl_valid = true;
@@ -809,6 +1109,7 @@ struct PolyForceArgs {
   int gravity_on;
   int32_t* flags;
   int32_t step_tag;
+  MeshTerms mt;  // terms of the mesh contacts, evaluated by k_poly_mesh_pass just before (nsph = 0: none)
 };
 
 __device__ inline bool poly_guard(const PolyForceArgs& a) {
@@ -1047,6 +1348,7 @@ __global__ __launch_bounds__(256) void k_poly_gather(PolyForceArgs a) {
         M = M + (cross(cpos - rx, f) + mint);
       }
   }
+  if (a.mt.nsph) mesh_add(a.mt, x, F, M);
   if (FUSED) {
     s.ax[x] = F.x / mx;
     s.ay[x] = F.y / mx;
@@ -1118,6 +1420,34 @@ int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, d
       }
       a.wstate = c->wstate;
     }
+  }
+  if (c->has_mesh && c->ml_valid && c->mesh_nsph > 0) {
+    double* mcs = nullptr;
+    if (c->record_cs) {
+      const int64_t m = (int64_t)c->mesh_cap * c->cap;
+      if (m > c->cap_mstate) {
+        if ((rc = dev_realloc(c, &c->mstate, m * kCStateLen))) return rc;
+        c->cap_mstate = m;
+      }
+      mcs = c->mstate;
+    }
+    PolyMeshArgs pm{c->mesh, c->ml[c->ml_cur], MeshTerms{}, a.s, c->d_shapes, c->vtx, dt, cpd, c->d_flags, mcs};
+    pm.t.sph = c->mesh_sph;
+    pm.t.scnt = c->mesh_scnt;
+    pm.t.idx = c->mesh_idx;
+    pm.t.terms = c->mesh_terms;
+    pm.t.nsph = c->mesh_nsph;
+    pm.t.kmax = c->mesh_kmax;
+    a.mt = pm.t;
+    const dim3 gm(grid_blocks(c->mesh_nsph, 256), (unsigned)c->mesh_kmax), bm(256);
+    if (c->kr_used) {
+      if (guarded) hipLaunchKernelGGL((k_poly_mesh_pass<true, true>), gm, bm, 0, c->stream, pm);
+      else hipLaunchKernelGGL((k_poly_mesh_pass<true, false>), gm, bm, 0, c->stream, pm);
+    } else {
+      if (guarded) hipLaunchKernelGGL((k_poly_mesh_pass<false, true>), gm, bm, 0, c->stream, pm);
+      else hipLaunchKernelGGL((k_poly_mesh_pass<false, false>), gm, bm, 0, c->stream, pm);
+    }
+    XDEM_HIP(c, hipGetLastError());
   }
   const dim3 gp(grid_blocks(std::max<int64_t>(a.norder, 1), 128)), bp(128), gg(grid_blocks(a.n, 256)),
       bg(256);
@@ -1231,6 +1561,41 @@ extern "C" int xdem_contact_polyhedron(xdem_ctx* c, const xdem_contact_params* c
 
 // The rows of xdem_poly_interactions_dump with, per row, the friction history or (moments) the
 // rolling-moment history.
+// Host copy of the current mesh candidate lists (slot k of particle x at k * stride + x); h: friction then
+// moment history (6 arrays), when asked for. Empty when there is no valid mesh list.
+struct HostMeshLists {
+  int64_t stride = 0;
+  uint32_t cap = 0;
+  std::vector<uint32_t> cnt, word;
+  std::vector<double> h[6];
+  uint32_t count(int64_t x) const { return std::min(cnt[x], cap); }
+};
+static int read_mesh_lists(xdem_ctx* c, HostMeshLists& m, bool hist) {
+  if (!c->has_mesh || !c->ml_valid || c->n == 0) return XDEM_OK;
+  const MeshList& L = c->ml[c->ml_cur];
+  const int64_t n = c->n;
+  m.stride = c->mesh.stride;
+  m.cap = (uint32_t)c->mesh.cap;
+  m.cnt.resize(n);
+  XDEM_HIP(c, hipMemcpyAsync(m.cnt.data(), L.cnt, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  uint32_t kmax = 0;
+  for (int64_t x = 0; x < n; x++) kmax = std::max(kmax, m.count(x));
+  const int64_t tot = (int64_t)kmax * m.stride;
+  m.word.resize(tot);
+  if (tot == 0) return XDEM_OK;
+  XDEM_HIP(c, hipMemcpyAsync(m.word.data(), L.word, sizeof(uint32_t) * tot, hipMemcpyDeviceToHost, c->stream));
+  if (hist) {
+    double* src[6] = {L.hx, L.hy, L.hz, L.mhx, L.mhy, L.mhz};
+    for (int q = 0; q < 6; q++) {
+      m.h[q].resize(tot);
+      XDEM_HIP(c, hipMemcpyAsync(m.h[q].data(), src[q], sizeof(double) * tot, hipMemcpyDeviceToHost, c->stream));
+    }
+  }
+  XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  return XDEM_OK;
+}
+
 static int poly_interactions_rows(xdem_ctx* c, int64_t cap, uint64_t* id_i, uint64_t* id_j, uint16_t* type,
                                   uint8_t* swap, uint32_t* sub_i, uint32_t* sub_j, double* friction, bool moments,
                                   int64_t* written) {
@@ -1277,6 +1642,9 @@ static int poly_interactions_rows(xdem_ctx* c, int64_t cap, uint64_t* id_i, uint
     XDEM_HIP(c, hipMemcpyAsync(whz.data(), moments ? W.mhz : W.hz, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
   }
   XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  HostMeshLists ml;
+  int rc = read_mesh_lists(c, ml, true);
+  if (rc) return rc;
   int64_t k = 0;
   for (int64_t x = 0; x < n; x++) {
     if (walls)
@@ -1291,6 +1659,17 @@ static int poly_interactions_rows(xdem_ctx* c, int64_t cap, uint64_t* id_i, uint
           friction[3 * k] = whx[j]; friction[3 * k + 1] = why[j]; friction[3 * k + 2] = whz[j];
           k++;
         }
+    if (!ml.cnt.empty())
+      for (uint32_t e = 0; e < ml.count(x); e++) {  // mesh candidates (type 7-12, partner = the mesh id)
+        if (k >= cap) return set_err(c, XDEM_ERR_CAPACITY, "poly dump: cap too small");
+        const int64_t j = (int64_t)e * ml.stride + x;
+        const uint32_t wd = ml.word[j];
+        id_i[k] = ids[x]; id_j[k] = (uint64_t)c->mesh_id; type[k] = (uint16_t)pm_type(wd); swap[k] = 0;
+        sub_i[k] = (uint32_t)pm_sub(wd); sub_j[k] = wd & kPmIdxMask;
+        const int q0 = moments ? 3 : 0;
+        for (int q = 0; q < 3; q++) friction[3 * k + q] = ml.h[q0 + q][j];
+        k++;
+      }
     const uint64_t b = base[x >> 6] + (uint64_t)(x & 63);
     for (uint32_t e = 0; e < cnt[x]; e++) {
       const uint64_t s = b + (uint64_t)e * kWave;
@@ -1345,7 +1724,12 @@ extern "C" int xdem_poly_interactions_count(xdem_ctx* c, int64_t* n) {
     XDEM_HIP(c, hipStreamSynchronize(c->stream));
     for (auto v : m) walls += __builtin_popcountll(v);
   }
-  *n = c->pl[c->pl_cur].nfeat + walls;
+  HostMeshLists ml;
+  const int rc = read_mesh_lists(c, ml, false);
+  if (rc) return rc;
+  int64_t mesh = 0;
+  for (int64_t x = 0; x < (int64_t)ml.cnt.size(); x++) mesh += ml.count(x);
+  *n = c->pl[c->pl_cur].nfeat + walls + mesh;
   return XDEM_OK;
 }
 
@@ -1373,6 +1757,11 @@ extern "C" int xdem_classifier_waves(xdem_ctx* c, int64_t sizes[13]) {
         sizes[driver_type(c->walls[w])] += k;
       }
     }
+    HostMeshLists ml;
+    const int rc = read_mesh_lists(c, ml, false);
+    if (rc) return rc;
+    for (int64_t x = 0; x < (int64_t)ml.cnt.size(); x++)
+      for (uint32_t e = 0; e < ml.count(x); e++) sizes[pm_type(ml.word[(int64_t)e * ml.stride + x])]++;
     return XDEM_OK;
   }
   if (!c->nl_valid || !c->hist_ready)
@@ -1530,6 +1919,29 @@ int poly_contact_state_rows(xdem_ctx* c, std::vector<uint64_t>& ii, std::vector<
         tt.push_back(driver_type(c->walls[bit / XDEM_MAX_SHAPE_VERTICES]));
         vals.insert(vals.end(), v, v + kCStateLen);
       }
+  if (c->mstate) {  // mesh candidates: partner = the mesh id, sub_j = the mesh feature
+    HostMeshLists ml;
+    const int rc = read_mesh_lists(c, ml, false);
+    if (rc) return rc;
+    if (!ml.word.empty()) {
+      std::vector<double> ms(ml.word.size() * kCStateLen);
+      XDEM_HIP(c, hipMemcpyAsync(ms.data(), c->mstate, sizeof(double) * ms.size(), hipMemcpyDeviceToHost, c->stream));
+      XDEM_HIP(c, hipStreamSynchronize(c->stream));
+      for (int64_t x = 0; x < n; x++)
+        for (uint32_t e = 0; e < ml.count(x); e++) {
+          const int64_t j = (int64_t)e * ml.stride + x;
+          const double* v = ms.data() + (size_t)kCStateLen * j;
+          if (!keep(v)) continue;
+          const uint32_t wd = ml.word[j];
+          ii.push_back(ids[x]);
+          jj.push_back((uint64_t)c->mesh_id);
+          si.push_back((uint32_t)pm_sub(wd));
+          sj.push_back(wd & kPmIdxMask);
+          tt.push_back((uint16_t)pm_type(wd));
+          vals.insert(vals.end(), v, v + kCStateLen);
+        }
+    }
+  }
   return XDEM_OK;
 }
 }  // namespace xdem
@@ -1638,6 +2050,33 @@ extern "C" int xdem_poly_interactions_load(xdem_ctx* c, int64_t m, const uint64_
       XDEM_HIP(c, hipMemcpyAsync(dst[q], wh[q].data(), sizeof(double) * mw, hipMemcpyHostToDevice, c->stream));
   }
   XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  {
+    HostMeshLists ml;
+    const int rc = read_mesh_lists(c, ml, true);
+    if (rc) return rc;
+    if (!ml.word.empty()) {
+      for (int64_t x = 0; x < n; x++)
+        for (uint32_t e = 0; e < ml.count(x); e++) {
+          const int64_t j = (int64_t)e * ml.stride + x;
+          const uint32_t wd = ml.word[j];
+          const auto it = saved.find(PKey{ids[x], (uint64_t)c->mesh_id, (uint32_t)pm_sub(wd), wd & kPmIdxMask,
+                                          (uint16_t)pm_type(wd), 0});
+          for (int q = 0; q < 6; q++) ml.h[q][j] = 0.0;
+          if (it == saved.end()) continue;
+          for (int q = 0; q < 3; q++) {
+            ml.h[q][j] = friction[3 * it->second + q];
+            ml.h[3 + q][j] = moment ? moment[3 * it->second + q] : 0.0;
+          }
+          hit++;
+        }
+      const MeshList& L = c->ml[c->ml_cur];
+      double* dst[6] = {L.hx, L.hy, L.hz, L.mhx, L.mhy, L.mhz};
+      for (int q = 0; q < 6; q++)
+        XDEM_HIP(c, hipMemcpyAsync(dst[q], ml.h[q].data(), sizeof(double) * ml.h[q].size(), hipMemcpyHostToDevice,
+                                   c->stream));
+      XDEM_HIP(c, hipStreamSynchronize(c->stream));
+    }
+  }
   if (moment)
     for (int64_t k = 0; k < 3 * m && !c->kr_used; k++) c->kr_used = moment[k] != 0.0;
   if (matched) *matched = hit;
@@ -1683,12 +2122,28 @@ __global__ __launch_bounds__(256) void k_poly_sim_feat(int64_t nf, const uint32_
   block_reduce3(tot, act, mn, part);
 }
 
+// vertex-driver candidates, then mesh candidates (once each, the driver rule of counter.hpp:145-151)
 __global__ __launch_bounds__(256) void k_poly_sim_wall(int64_t n, PState s, const xdem_shape* __restrict__ shapes,
                                                        const d4* __restrict__ vtx, PolyWall pw,
                                                        const Surface* __restrict__ walls, int nw, int64_t stride,
-                                                       double* __restrict__ part) {
+                                                       MeshView m, MeshList ml, double* __restrict__ part) {
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double tot = 0, act = 0, mn = 0;
+  if (x < n && m.cap > 0) {
+    const xdem_shape& sa = shapes[s.type[x]];
+    const double ra = sa.minkowski * s.core[x].w.w;
+    const Verts va = verts_of(vtx, x, V3{0, 0, 0});
+    const uint32_t mc = min(ml.cnt[x], (uint32_t)m.cap);
+    for (uint32_t k = 0; k < mc; k++) {
+      const int64_t j = (int64_t)k * m.stride + x;
+      const Det d = poly_mesh_detect(m, ml.word[j], sa, va, ra);
+      tot += 1.0;
+      if (d.contact || ml.hx[j] != 0.0 || ml.hy[j] != 0.0 || ml.hz[j] != 0.0) {
+        act += 1.0;
+        mn = fmin(mn, d.contact ? d.dn : 0.0);
+      }
+    }
+  }
   if (x < n && nw > 0) {
     const xdem_shape& sh = shapes[s.type[x]];
     const double r = sh.minkowski * s.core[x].w.w;
@@ -1737,8 +2192,14 @@ int poly_sim_counts(xdem_ctx* c, double* tot, double* act, double* min_dn, doubl
   hipLaunchKernelGGL(k_poly_sim_feat, dim3(bf), dim3(256), 0, c->stream, nf, P.feat, c->fcontact, c->fdn, part);
   PolyWall pw{};
   if (nw) pw = c->pw[c->pw_cur];
+  MeshView mv{};
+  MeshList ml{};
+  if (c->has_mesh && c->ml_valid) {
+    mv = c->mesh;
+    ml = c->ml[c->ml_cur];
+  }
   hipLaunchKernelGGL(k_poly_sim_wall, dim3(bw), dim3(256), 0, c->stream, n, live(c), c->d_shapes, c->vtx, pw,
-                     c->d_walls, nw, c->cap, part + 3 * (int64_t)bf);
+                     c->d_walls, nw, c->cap, mv, ml, part + 3 * (int64_t)bf);
   XDEM_HIP(c, hipGetLastError());
   if (fixed) {
     double res[3];

@@ -460,6 +460,9 @@ struct xdem_ctx {
   double* fdn = nullptr;            // [cap_contrib] dn of the feature at the last pass (0 without contact)
   uint64_t poly_pass_epoch = ~0ull; // neighbour build of the last polyhedron force pass
   int64_t cap_contrib = 0;
+  uint32_t* feat_tmp = nullptr;     // [cap_feat_tmp] per list slot, up to kFeatPerSlot feature words of the
+  int64_t cap_feat_tmp = 0;         // one-pass enumeration (nbh_polyhedron)
+  int32_t* d_feat_over = nullptr;   // a slot enumerated more than kFeatPerSlot features
   uint2* frange = nullptr;          // [cap_frange] per list slot: first feature and count of its pair
                                     // (through the owner entry for partner entries); x = ~0u: no owner entry
   int64_t cap_frange = 0;
@@ -559,7 +562,11 @@ int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd,
 int launch_step(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, int form, bool gravity_on,
                 V3 g, double thr2, int32_t step_tag, Core** input);
 int alloc_walls(xdem_ctx* c, int nw);
-int mesh_carry(xdem_ctx* c, double rcut_inc, bool have_old);  // mesh candidates + history carry-over
+// mesh candidates + history carry-over (poly: the polyhedron candidates of the particles' vertices, edges
+// and faces, built by launch_poly_mesh_carry)
+int mesh_carry(xdem_ctx* c, double rcut_inc, bool have_old, bool poly = false);
+int launch_poly_mesh_carry(xdem_ctx* c, double rv, const MeshList& nl, const MeshList& ol, bool have_old,
+                           int32_t* over);
 void mesh_free(xdem_ctx* c);
 // Mesh contacts of the coming force pass (k_mesh_pass over the live records): histories updated and the
 // terms written for the pass to add; guarded: a no-op when the pass would be (stall or trigger in

@@ -197,10 +197,14 @@ def random_quaternions(seed, n):
                      b * np.sin(2 * np.pi * u3)])
 
 
-def polyhedra(n=200_000, spacing=1.7, jitter=0.05, seed=4, density=2500.0, minkowski=0.02, dt=2e-6, steps=1000,
+def polyhedra(n=200_000, spacing=1.7, jitter=0.05, seed=4, density=1.0, minkowski=0.02, dt=2e-6, steps=1000,
               rcut_inc=0.1, nx=None, gravity=(0.0, 0.0, -9.81), sigma_v=0.0, z0=None, periodic=(True, True, False)):
     """Cubes (edge 1.0) and octahedra (circumradius 0.8), 50/50 by id parity, random orientations, on a
-    jittered simple-cubic lattice; periodic in x and y, floor wall at z = 0 and lid above."""
+    jittered simple-cubic lattice; periodic in x and y, floor wall at z = 0 and lid above.
+
+    Density 1 is the reference's default for polyhedra (set_fields.cu:84-85, "If not defined, density is
+    1"): with kn = 1e5 the weight of a particle then presses a contact by about 1e-4, well inside the
+    Minkowski shells (0.02 each), where vertex-face detection holds."""
     from .native import contact_params
     from . import shapes as shp
     defs = [shp.cube(1.0, minkowski), shp.octahedron(0.8, minkowski)]
@@ -251,21 +255,39 @@ def _start_with_accelerations(sim):
     sim.ctx.force_to_accel()
 
 
-def polyhedra_dense(n=200_000, phi=0.45, seed=4, dt=2e-6, growth=1.01, relax_steps=200, settle_steps=3000,
-                    dt_grow=1e-3, log=None):
-    """Polyhedra at solid fraction phi of the box, grown with the GPU DEM and settled under gravity.
+def polyhedra_dense(n=200_000, phi=0.45, seed=4, dt=2e-6, growth=1.01, relax_steps=200, final_relax=3,
+                    settle_steps=0, dt_grow=5e-4, log=None, cache=None):
+    """Polyhedra at solid fraction phi of the box, grown with the GPU DEM.
 
     Starts from the jittered lattice of `polyhedra` (spacing 1.55, solid fraction about 0.25, periodic in
     x and y, floor and lid) and grows every particle's homothety by `growth` per stage (mass x h^3,
     inertia x h^5, bounding radius x h, cells re-sized to 2 max radius + rcut_inc) until the particles'
-    volume fills phi of the box; each stage relaxes for relax_steps DEM steps of dt_grow without gravity.
-    The grown particles then settle onto the floor under gravity for settle_steps steps of dt_grow, and
-    their velocities are zeroed; dt is set to `dt`. meta gives the achieved solid fraction of the box,
-    that of the settled bed (particle volume over the box area times the bed's top), the homothety and the
-    smallest overlap (dn) of the last pass."""
+    volume fills phi of the box; each stage starts at rest and relaxes for relax_steps DEM steps of
+    dt_grow without gravity, and final_relax more such stages follow at the final size. The result stands
+    for the configuration's random packing at phi, with gravity acting from the run's first step;
+    settle_steps > 0 lets it settle onto the floor under gravity first. Velocities are zeroed; dt is set
+    to `dt`. meta gives the achieved solid fraction of the box, that of the bed (particle volume over the
+    box area times the bed's top), the homothety and the smallest overlap (dn) of the last pass.
+
+    cache: an .npz path; when it holds the result for the same arguments it is loaded instead of grown,
+    otherwise the grown particles are written there (so that a profiled run can skip the growth)."""
     import copy as _copy
+    import json as _json
+    import os as _os
     from .sim import PolySim
     w = polyhedra(n=n, spacing=1.55, jitter=0.05, seed=seed, dt=dt)
+    key = _json.dumps([n, phi, seed, dt, growth, relax_steps, final_relax, settle_steps, dt_grow,
+                       float(np.sum(w.particles["mass"]))])
+    if cache and _os.path.exists(cache):
+        with np.load(cache) as z:
+            meta = _json.loads(str(z["meta_json"]))
+            if str(z["key"]) == key:
+                out = _copy.copy(w)
+                out.particles = {k[2:]: np.array(z[k]) for k in z.files if k.startswith("p_")}
+                L, _, top = w.meta["box"]
+                out.domain = domain((0.0, 0.0, 0.0), (L, L, top), (True, True, False), float(z["cell"]))
+                out.meta = meta
+                return out
     L, _, top = w.meta["box"]
     vbox = L * L * top
     p = {k: np.array(v, copy=True) for k, v in w.particles.items()}
@@ -300,17 +322,30 @@ def polyhedra_dense(n=200_000, phi=0.45, seed=4, dt=2e-6, growth=1.01, relax_ste
         if log:
             vmax = float(np.max(np.abs(np.stack([p["vx"], p["vy"], p["vz"]]))))
             log(f"polyhedra_dense: stage {stages}, homothety {h:.4f}, rebuilds {st['rebuilds']}, max |v| {vmax:.3g}")
-    g.gravity = w.gravity
-    for k in ("vx", "vy", "vz", "vrotx", "vroty", "vrotz"):
-        p[k] = np.zeros_like(p[k])
-    sim.w = g
-    sim.ctx.upload(p)
-    _start_with_accelerations(sim)
-    rs = sim.run(settle_steps, poll=64)
+    for r in range(final_relax):
+        for k in ("vx", "vy", "vz", "vrotx", "vroty", "vrotz"):
+            p[k] = np.zeros_like(p[k])
+        sim.ctx.upload(p)
+        _start_with_accelerations(sim)
+        st = sim.run(relax_steps, poll=64)
+        p = sim.ctx.download()
+        if log:
+            vmax = float(np.max(np.abs(np.stack([p["vx"], p["vy"], p["vz"]]))))
+            log(f"polyhedra_dense: relaxation {r + 1}, rebuilds {st['rebuilds']}, max |v| {vmax:.3g}")
+    if settle_steps > 0:
+        g.gravity = w.gravity
+        for k in ("vx", "vy", "vz", "vrotx", "vroty", "vrotz"):
+            p[k] = np.zeros_like(p[k])
+        sim.w = g
+        sim.ctx.upload(p)
+        _start_with_accelerations(sim)
+        rs = sim.run(settle_steps, poll=64)
+        if log:
+            log(f"polyhedra_dense: settled, {rs['rebuilds']} rebuilds")
+        p = sim.ctx.download()
     st = sim.ctx.simulation_state()
     if log:
-        log(f"polyhedra_dense: settled, {rs['rebuilds']} rebuilds, {st['n_active']} active interactions")
-    p = sim.ctx.download()
+        log(f"polyhedra_dense: {st['n_interactions']} interactions, {st['n_active']} active, min dn {st['min_dn']:.3g}")
     del sim
     for k in ("vx", "vy", "vz", "vrotx", "vroty", "vrotz", "fx", "fy", "fz", "momx", "momy", "momz", "arotx",
               "aroty", "arotz"):
@@ -324,16 +359,20 @@ def polyhedra_dense(n=200_000, phi=0.45, seed=4, dt=2e-6, growth=1.01, relax_ste
     out.meta = dict(w.meta, solid_fraction=achieved, bed_solid_fraction=vp / (L * L * bed), bed_top=bed,
                     homothety=h, growth_stages=stages, min_dn=st["min_dn"], interactions=st["n_interactions"],
                     active_interactions=st["n_active"],
-                    workload=f"cubes + octahedra 50/50, random orientations, grown from a jittered lattice to "
-                             f"solid fraction {achieved:.3f} of the box (homothety {h:.4f}) and settled under "
-                             f"gravity (bed solid fraction {vp / (L * L * bed):.3f}), periodic x/y, floor and lid, "
-                             f"dt {dt:g}")
+                    workload=f"cubes + octahedra 50/50, density 1, random orientations, grown from a jittered "
+                             f"lattice to solid fraction {achieved:.3f} of the box (homothety {h:.4f})"
+                             + (f" and settled under gravity (bed solid fraction {vp / (L * L * bed):.3f})"
+                                if settle_steps > 0 else ", relaxed at rest, gravity from the first step")
+                             + f", periodic x/y, floor and lid, dt {dt:g}")
+    if cache:
+        np.savez(cache, key=np.array(key), meta_json=np.array(_json.dumps(out.meta)),
+                 cell=np.array(out.domain.cell_size), **{"p_" + k: v for k, v in p.items()})
     return out
 
 
-def polyhedra_benchmark(n=200_000, log=None):
+def polyhedra_benchmark(n=200_000, log=None, cache=None):
     """The polyhedra benchmark input: n cubes and octahedra 50/50 (edge 1.0, circumradius 0.8, Minkowski
-    radius 0.02 before growth), uniform random orientations (seed 4), grown from a jittered lattice of
-    spacing 1.55 to solid fraction 0.45 of the box and settled under gravity (polyhedra_dense), periodic
-    in x and y with a floor and a lid, gravity -9.81 z, dt 2e-6."""
-    return polyhedra_dense(n=n, phi=0.45, seed=4, dt=2e-6, log=log)
+    radius 0.02 before growth, density 1), uniform random orientations (seed 4), grown from a jittered
+    lattice of spacing 1.55 to solid fraction 0.45 of the box (polyhedra_dense), periodic in x and y with a
+    floor and a lid, gravity -9.81 z, dt 2e-6."""
+    return polyhedra_dense(n=n, phi=0.45, seed=4, dt=2e-6, log=log, cache=cache)

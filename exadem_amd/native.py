@@ -528,8 +528,9 @@ class Context:
 
     def set_mesh(self, mesh_id, vertices, edges, faces, minkowski, center=(0.0, 0.0, 0.0),
                  quat=(1.0, 0.0, 0.0, 0.0), max_candidates=32):
-        """Registers a stationary triangle-mesh driver acting on spheres (vertices [nv][3] in the mesh frame,
-        edges [ne][2], triangles [nf][3]); takes effect at the next neighbour build."""
+        """Registers a stationary triangle-mesh driver acting on spheres or polyhedra (vertices [nv][3] in
+        the mesh frame, edges [ne][2], triangles [nf][3]); takes effect at the next neighbour build.
+        max_candidates: slots per particle (polyhedra meet more mesh features than spheres)."""
         v = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
         e = np.ascontiguousarray(edges, np.uint32).reshape(-1, 2)
         f = np.ascontiguousarray(faces, np.uint32).reshape(-1, 3)

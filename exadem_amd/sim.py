@@ -148,8 +148,8 @@ class PolySim(SphereSim):
     """The polyhedra scheme (data/config/config_polyhedra.msp): compute_vertices on the fast path,
     vertex_displ_over as the rebuild trigger, nbh_polyhedron and contact_polyhedron."""
 
-    def __init__(self, workload, device=0, ctx=None):
-        super().__init__(workload, device, native.XDEM_FORM_FP64, ctx)
+    def __init__(self, workload, device=0, ctx=None, mesh=None):
+        super().__init__(workload, device, native.XDEM_FORM_FP64, ctx, mesh=mesh)
         self.ctx.set_shapes(workload.shapes)
 
     def checkpoint(self, path):
@@ -162,13 +162,14 @@ class PolySim(SphereSim):
                  **{"p_" + k: v for k, v in p.items()})
 
     @classmethod
-    def restart(cls, workload, path, device=0, history=True):
-        """A polyhedra simulation continued from a checkpoint (see SphereSim.restart)."""
+    def restart(cls, workload, path, device=0, history=True, mesh=None):
+        """A polyhedra simulation continued from a checkpoint (see SphereSim.restart); the mesh driver's
+        interactions are rows of the same table (types 7-12, partner = the mesh id)."""
         d = np.load(path)
         w = copy.copy(workload)
         w.particles = {k[2:]: d[k] for k in d.files if k.startswith("p_")}
         cls._restore_drivers(w, d)
-        sim = cls(w, device=device)
+        sim = cls(w, device=device, mesh=mesh)
         if "time" in d.files:
             sim.ctx.set_physical_time(float(d["time"]))
         sim.rebuild()

@@ -21,6 +21,9 @@
 //                    contact_force.hpp:146-235, src/sphere/include/exaDEM/sphere/contact_sphere.hpp:121-297,
 //                    src/shape/include/exaDEM/shape_detection_core.hpp:57-81,
 //                    src/driver/include/exaDEM/surface.hpp:320-350, cylinder.hpp:225-272, ball.hpp:292-327
+//   polyhedra        src/polyhedron/nbh_polyhedron.cpp:75-384, polyhedron/contact.hpp:124-432,
+//                    polyhedron/include/exaDEM/polyhedron/nbh_polyhedron_driver.hpp:60-200 (mesh driver),
+//                    src/shape/include/exaDEM/shape_detection_core.hpp:43-566, shape_detection_api.hpp:38-406
 //   operator order   data/config/main-config.msp:55-65 (first_iteration),
 //                    data/config/config_numerical_schemes.msp:6-28, config_move_particles.msp:141-195
 //
@@ -70,7 +73,9 @@ struct Q4 {
 // mistake so that the closed-form checks can show they catch it. Always 0 in normal use.
 //   1 damping sign, 2 friction increment sign, 3 no Coulomb cap, 4 history dropped at rebuild,
 //   5 polyhedron swap side ignored in the force assembly, 6 the vertex side's moment arm taken from
-//   the vertex instead of the contact point, 7 polyhedron meff = m_i.
+//   the vertex instead of the contact point, 7 polyhedron meff = m_i, 8 shaker evaluated at the end of
+//   the step, 9 the friction negation of mesh-vertex items (11, 12) dropped, 10 the mesh-vertex box gate
+//   of those items dropped.
 static int g_mutation = 0;
 
 // ---------------------------------------------------------------- quaternion helpers
@@ -1068,6 +1073,69 @@ static std::array<V3, 8> shifted(const std::array<V3, 8>& v, int nv, V3 s) {
   return out;
 }
 
+// Polyhedron against the triangle-mesh driver (RShape): add_driver_interaction for RShapeDriver
+// (nbh_polyhedron_driver.hpp:60-200, rVerlet = rcut_inc), in its loop order with the mesh features by
+// index: particle vertices against mesh vertices (7), edges (8) and faces (9); particle edges against mesh
+// edges (10); then the mesh vertices whose box (the vertex enlarged by the mesh's Minkowski radius) meets
+// the particle's box -- the shape box rotated and moved but not scaled by the homothety, enlarged by
+// rVerlet -- against the particle's edges (11) and faces (12). sub_i is the particle's feature, sub_j the
+// mesh's. The face boxes that gate (9) contain every point that passes the face filter, so the gate is
+// not restated.
+static void poly_mesh_items(const Oracle& o, int64_t x, const Item& base, double rv, std::vector<Item>& lst) {
+  const xdem_shape& sa = o.shapes[o.type[x]];
+  const double ra = sa.minkowski * o.homothety[x], rm = o.mesh.rm;
+  const std::array<V3, 8>& va = o.verts[x];
+  const auto& mv = o.mesh.v;
+  const auto& me = o.mesh.e;
+  auto add = [&](int type, int sub_i, size_t k) {
+    Item it = base;
+    it.type = (uint16_t)type;
+    it.id_j = o.mesh.id;
+    it.sub_i = (uint32_t)sub_i;
+    it.sub_j = (uint32_t)k;
+    it.partner = -1;
+    it.shift = 13;
+    lst.push_back(it);
+  };
+  for (int i = 0; i < sa.nv; i++) {
+    for (size_t k = 0; k < mv.size(); k++)
+      if (filter_vv(rv, va[i], ra, mv[k], rm)) add(7, i, k);
+    for (size_t k = 0; k < me.size(); k++)
+      if (filter_ve(rv, va[i], ra, mv[me[k][0]], mv[me[k][1]], rm)) add(8, i, k);
+    for (size_t k = 0; k < o.mesh.f.size(); k++) {
+      V3 t[3];
+      mesh_tri(o, (uint32_t)k, t);
+      double dist;
+      V3 nn;
+      bool early;
+      if (mesh_vf(va[i], t, ra + rm + rv, dist, nn, early)) add(9, i, k);
+    }
+  }
+  for (int i = 0; i < sa.ne; i++)
+    for (size_t k = 0; k < me.size(); k++)
+      if (filter_ee(rv, va[sa.edges[i][0]], va[sa.edges[i][1]], ra, mv[me[k][0]], mv[me[k][1]], rm)) add(10, i, k);
+  OBBo obb_i = compute_obb(sa, o.r[x], o.q[x], 1.0);
+  obb_enlarge(obb_i, rv);
+  for (size_t k = 0; k < mv.size(); k++) {
+    OBBo ov;
+    ov.c = mv[k];
+    ov.e1 = {1, 0, 0};
+    ov.e2 = {0, 1, 0};
+    ov.e3 = {0, 0, 1};
+    ov.ext = {0, 0, 0};
+    obb_enlarge(ov, rm);
+    if (g_mutation != 10 && !obb_intersect(ov, obb_i)) continue;  // planted mistake 10: the gate dropped
+    for (int i = 0; i < sa.ne; i++)
+      if (filter_ve(rv, mv[k], rm, va[sa.edges[i][0]], va[sa.edges[i][1]], ra)) add(11, i, k);
+    for (int i = 0; i < sa.nf; i++) {
+      double dist;
+      V3 nn;
+      bool early;
+      if (vf_core(mv[k], va, sa, i, rm + ra + rv, dist, nn, early)) add(12, i, k);
+    }
+  }
+}
+
 // nbh_polyhedron: rebuild the per-particle item lists (owner = lower id), carrying history by key.
 static void nbh_polyhedron(Oracle& o, double rv) {
   const int64_t n = o.n;
@@ -1097,6 +1165,7 @@ static void nbh_polyhedron(Oracle& o, double rv) {
           it.sub_j = (uint32_t)-1;
           lst.push_back(it);
         }
+    if (o.mesh.on) poly_mesh_items(o, x, base, rv, lst);
     OBBo obb_i0 = compute_obb(sa, o.r[x], o.q[x], ha);
     for (auto& s : o.streams[x]) {
       const int64_t y = s.j;
@@ -1243,6 +1312,62 @@ static PolyEval eval_pp(Oracle& o, int64_t x, Item& it, const CP& cp, double dt)
   return e;
 }
 
+// One mesh item of x (ContactLawRShapeDriverFunc, polyhedron/contact.hpp:300-432; detect<7..12>,
+// shape_detection_api.hpp:357-406): 7-10 with the particle as i and the mesh as j, 11-12 with the mesh
+// vertex as i (the particle then takes -f and -moment, and the stored friction is negated after the law,
+// as the reference does). The mesh is stationary: position its centre, no velocity, no spin.
+static PolyEval eval_mesh(Oracle& o, int64_t x, Item& it, const CP& cpd, double dt) {
+  PolyEval e{};
+  const xdem_shape& sa = o.shapes[o.type[x]];
+  const double ra = sa.minkowski * o.homothety[x], rm = o.mesh.rm;
+  const std::array<V3, 8>& va = o.verts[x];
+  const auto& mv = o.mesh.v;
+  const uint32_t i = it.sub_i, k = it.sub_j;
+  Contact c{false, 0, {0, 0, 0}, {0, 0, 0}};
+  switch (it.type) {
+    case 7:
+      c = detect_vv(va[i], ra, mv[k], rm);
+      break;
+    case 8:
+      c = detect_ve(va[i], ra, mv[o.mesh.e[k][0]], mv[o.mesh.e[k][1]], rm);
+      break;
+    case 9:
+      c = mesh_detect(o, 2, k, va[i], ra);
+      break;
+    case 10:
+      c = detect_ee(va[sa.edges[i][0]], va[sa.edges[i][1]], ra, mv[o.mesh.e[k][0]], mv[o.mesh.e[k][1]], rm);
+      break;
+    case 11:
+      c = detect_ve(mv[k], rm, va[sa.edges[i][0]], va[sa.edges[i][1]], ra);
+      break;
+    case 12:
+      c = detect_vf(mv[k], rm, va, sa, (int)i, ra);
+      break;
+  }
+  const V3 zero{0, 0, 0};
+  if (c.contact) {
+    V3 f{0, 0, 0};
+    e.in_contact = true;
+    e.cp = c.cp;
+    if (it.type <= 10) {
+      contact_force_core(c.dn, c.n, dt, cpd, o.mass[x], it.friction, c.cp, o.r[x], o.v[x], f, it.moment, o.vrot[x],
+                         o.mesh.center, zero, zero);
+      e.f = f;
+      e.mint = it.moment;
+    } else {
+      contact_force_core(c.dn, c.n, dt, cpd, o.mass[x], it.friction, c.cp, o.mesh.center, zero, f, it.moment, zero,
+                         o.r[x], o.v[x], o.vrot[x]);
+      e.f = -f;
+      e.mint = -it.moment;
+      if (g_mutation != 9) it.friction = -it.friction;  // planted mistake 9: the negation dropped
+    }
+  } else {
+    it.friction = {0, 0, 0};
+    it.moment = {0, 0, 0};
+  }
+  return e;
+}
+
 static void contact_poly(Oracle& o, const CP& cp, const CP& cpd, double dt) {
   const int64_t n = o.n;
   std::vector<std::vector<PolyEval>> ev(n);
@@ -1270,6 +1395,8 @@ static void contact_poly(Oracle& o, const CP& cp, const CP& cpd, double dt) {
           it.moment = {0, 0, 0};
         }
         ev[x][k] = e;
+      } else if (it.type >= 7) {
+        ev[x][k] = eval_mesh(o, x, it, cpd, dt);
       } else {
         ev[x][k] = eval_pp(o, x, it, cp, dt);
       }
@@ -1319,6 +1446,14 @@ static void contact_poly(Oracle& o, const CP& cp, const CP& cpd, double dt) {
     for (size_t k = 0; k < o.items[x].size(); k++) {
       const Item& it = o.items[x][k];
       if (it.type != 5 || !ev[x][k].in_contact) continue;
+      const PolyEval& e = ev[x][k];
+      F = F + e.f;
+      M = M + compute_moments(e.cp, o.r[x], e.f, e.mint);
+    }
+    // mesh contacts after the walls, in list order (f and mint already on the particle's side)
+    for (size_t k = 0; k < o.items[x].size(); k++) {
+      const Item& it = o.items[x][k];
+      if (it.type < 7 || !ev[x][k].in_contact) continue;
       const PolyEval& e = ev[x][k];
       F = F + e.f;
       M = M + compute_moments(e.cp, o.r[x], e.f, e.mint);

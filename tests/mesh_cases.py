@@ -3,7 +3,7 @@
 z = 1, so the mesh has flat faces, a convex apex vertex and convex ridge edges."""
 import numpy as np
 
-from exadem_amd import inputs, native, stl
+from exadem_amd import inputs, native, shapes, stl
 
 
 def tray_triangles(n=10, h=2.0, apex=1.0):
@@ -72,3 +72,39 @@ def tray_bed(seed=21, R=0.5):
     jit[:, -2:] = 0.0
     v = inputs.normal(seed + 1, 2, 3 * n, 0.0, 0.3).reshape(3, n)
     return inputs.sphere_set(xs + jit[0], ys + jit[1], zs, np.full(n, R), 2500.0, v[0], v[1], v[2])
+
+
+def tray_polyhedra(seed=3, vz=-3.0):
+    """Cubes (edge 1) and octahedra (circumradius 0.8), Minkowski radius 0.02, density 1, alternating, in
+    random orientations on a 7 x 7 grid of spacing 2.2 over the tray (x, y in 3.4 .. 16.6), two layers:
+    the lower one 0.03 to 0.1 above the tray under it (the bump included), the upper one 1.8 higher;
+    falling at vz plus N(0, 0.3) per component, spins N(0, 1). Gravity, no walls, dt 1e-4 (the contact
+    time is about 0.01 s at kn 1e5 for these masses)."""
+    defs = [shapes.cube(1.0, 0.02), shapes.octahedron(0.8, 0.02)]
+    recs = [d.to_record() for d in defs]
+    g = 3.4 + 2.2 * np.arange(7)
+    x, y = np.meshgrid(g, g, indexing="ij")
+    x, y = x.ravel(), y.ravel()
+    # highest tray point under a footprint of radius 0.9: the bump falls by 1/2 per unit of max(|dx|, |dy|)
+    top = np.clip(1.0 - (np.maximum(np.abs(x - 10.0), np.abs(y - 10.0)) - 0.9) / 2.0, 0.0, 1.0)
+    x, y, z = np.concatenate([x, x]), np.concatenate([y, y]), np.concatenate([top + 0.9, top + 2.7])
+    n = x.size
+    t = (np.arange(n) % 2).astype(np.uint32)
+    v = inputs.normal(seed, 2, 3 * n, 0.0, 0.3).reshape(3, n)
+    rb = np.array([shapes.bounding_radius(defs[k]) for k in t])
+    p = inputs.sphere_set(x, y, z, rb, 1.0, v[0], v[1], v[2] + vz)
+    p["type"] = t
+    p["mass"] = np.array([recs[k].volume for k in t])
+    for a in ("inx", "iny", "inz"):
+        p[a] = p["mass"] * np.array([recs[k].inertia_on_mass[0] for k in t])
+    q = inputs.normal(seed + 1, 2, 4 * n, 0.0, 1.0).reshape(4, n)
+    q = q / np.linalg.norm(q, axis=0)
+    for k, key in enumerate(("qw", "qx", "qy", "qz")):
+        p[key] = np.ascontiguousarray(q[k])
+    w = inputs.normal(seed + 2, 2, 3 * n, 0.0, 1.0).reshape(3, n)
+    for k, key in enumerate(("vrotx", "vroty", "vrotz")):
+        p[key] = np.ascontiguousarray(w[k])
+    wl = workload("tray polyhedra", p, dt=1e-4, steps=300)
+    wl.domain = native.domain((0.0, 0.0, -1.0), (20.0, 20.0, 7.0), (False, False, False), 2 * float(rb.max()) + 0.1)
+    wl.shapes = recs
+    return wl

@@ -95,7 +95,7 @@ def test_moving_drivers_fp32_form(gpu_ok):
 
 
 def test_polyhedra_under_descending_lid(gpu_ok):
-    w = inputs.polyhedra(n=1500, spacing=1.5, jitter=0.05, seed=5, dt=2e-4, sigma_v=0.5, z0=0.6,
+    w = inputs.polyhedra(density=2500.0, n=1500, spacing=1.5, jitter=0.05, seed=5, dt=2e-4, sigma_v=0.5, z0=0.6,
                          periodic=(False, False, False))
     lid = max(range(len(w.walls)), key=lambda k: (w.walls[k]["normal"][2], w.walls[k]["offset"]))
     w.walls = [dict(x) for x in w.walls]

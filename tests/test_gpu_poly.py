@@ -22,9 +22,9 @@ TRAJ_TOL = 1e-10
 
 def poly_case(name):
     if name == "periodic":
-        return inputs.polyhedra(n=2000, spacing=1.45, jitter=0.08, seed=21, dt=2e-4, sigma_v=1.0, z0=0.75)
+        return inputs.polyhedra(density=2500.0, n=2000, spacing=1.45, jitter=0.08, seed=21, dt=2e-4, sigma_v=1.0, z0=0.75)
     if name == "walls":
-        return inputs.polyhedra(n=1500, spacing=1.5, jitter=0.05, seed=5, dt=2e-4, sigma_v=0.5, z0=0.6,
+        return inputs.polyhedra(density=2500.0, n=1500, spacing=1.5, jitter=0.05, seed=5, dt=2e-4, sigma_v=0.5, z0=0.6,
                                 periodic=(False, False, False))
     raise KeyError(name)
 

@@ -69,7 +69,7 @@ def test_sphere_restart_matches_oracle(gpu_ok, case, tmp_path):
 
 
 def test_poly_restart_matches_oracle(gpu_ok, tmp_path):
-    w = inputs.polyhedra(n=1500, spacing=1.5, jitter=0.05, seed=5, dt=2e-4, sigma_v=0.5, z0=0.6,
+    w = inputs.polyhedra(density=2500.0, n=1500, spacing=1.5, jitter=0.05, seed=5, dt=2e-4, sigma_v=0.5, z0=0.6,
                          periodic=(False, False, False))
     sch = scheme(w)
     a = PolySim(w)

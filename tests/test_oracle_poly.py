@@ -233,7 +233,7 @@ def _keys(ii, jj, tt, sw, si, sj):
 def test_feature_list_contains_every_detected_contact():
     """nbh_polyhedron's filters are supersets of detection: every feature pair in contact (found by
     running the detection over all feature pairs of all particle pairs) is in the interaction list."""
-    w = inputs.polyhedra(n=600, spacing=1.5, jitter=0.1, seed=9, periodic=(False, False, False), z0=1.2)
+    w = inputs.polyhedra(density=2500.0, n=600, spacing=1.5, jitter=0.1, seed=9, periodic=(False, False, False), z0=1.2)
     o = pyoracle.Oracle(w.particles, w.domain, w.walls)
     o.set_shapes(w.shapes)
     o.rebuild_poly(w.rcut_inc)
@@ -416,7 +416,7 @@ def test_cube_on_cube_face_contact(small_owns):
 def _loose_packing(n=343, seed=21):
     """Gravity-free, wall-free cubes and octahedra on a tight jittered lattice (many VV/VE/VF/EE contacts of
     both swap sides) with random velocities and spins, non-periodic."""
-    w = inputs.polyhedra(n=n, spacing=1.42, jitter=0.08, seed=seed, periodic=(False, False, False), z0=1.5,
+    w = inputs.polyhedra(density=2500.0, n=n, spacing=1.42, jitter=0.08, seed=seed, periodic=(False, False, False), z0=1.5,
                          sigma_v=0.3)
     p = {k: v.copy() for k, v in w.particles.items()}
     spin = inputs.normal(seed + 7, 3, 3 * w.n, 0.0, 2.0).reshape(3, w.n)

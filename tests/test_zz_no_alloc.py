@@ -43,7 +43,7 @@ def test_noalloc_equals_simulation_state_spheres(gpu_ok):
 
 def test_noalloc_equals_simulation_state_polyhedra(gpu_ok):
     for n, seed in ((1500, 5), (300, 6)):
-        w = inputs.polyhedra(n=n, spacing=1.5, jitter=0.05, seed=seed, dt=2e-4, sigma_v=0.5, z0=0.6,
+        w = inputs.polyhedra(density=2500.0, n=n, spacing=1.5, jitter=0.05, seed=seed, dt=2e-4, sigma_v=0.5, z0=0.6,
                              periodic=(False, False, False))
         s = PolySim(w)
         s.first_iteration()
