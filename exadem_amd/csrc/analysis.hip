@@ -197,6 +197,8 @@ static int sim_state_impl(xdem_ctx* c, xdem_sim_state* out, double* fixed, int64
   MeshView mesh{};
   MeshList ml{};
   if (!poly && nl.cnt && c->has_mesh && c->ml_valid) {
+    int rc = mesh_sync(c);
+    if (rc) return rc;
     mesh = c->mesh;
     ml = c->ml[c->ml_cur];
   }

@@ -590,12 +590,20 @@ extern "C" int xdem_contact_state_dump(xdem_ctx* c, int64_t cap, uint64_t* id_i,
   const int rc = contact_state_rows(c, ii, jj, si, sj, tt, v);
   if (rc) return rc;
   if ((int64_t)ii.size() > cap) return set_err(c, XDEM_ERR_CAPACITY, "contact_state_dump: cap too small");
-  std::copy(ii.begin(), ii.end(), id_i);
-  std::copy(jj.begin(), jj.end(), id_j);
-  std::copy(si.begin(), si.end(), sub_i);
-  std::copy(sj.begin(), sj.end(), sub_j);
-  std::copy(tt.begin(), tt.end(), type);
-  std::copy(v.begin(), v.end(), values);
+  // one block per interaction type in ascending type, device order inside each, as the reference's
+  // writer walks the classifier's waves (itools/writer.hpp:35-75)
+  std::vector<int64_t> order(ii.size());
+  for (size_t k = 0; k < order.size(); k++) order[k] = (int64_t)k;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return tt[a] < tt[b]; });
+  for (size_t k = 0; k < order.size(); k++) {
+    const int64_t r = order[k];
+    id_i[k] = ii[r];
+    id_j[k] = jj[r];
+    sub_i[k] = si[r];
+    sub_j[k] = sj[r];
+    type[k] = tt[r];
+    std::copy(v.begin() + r * kCStateLen, v.begin() + (r + 1) * kCStateLen, values + k * kCStateLen);
+  }
   if (written) *written = (int64_t)ii.size();
   return XDEM_OK;
 }

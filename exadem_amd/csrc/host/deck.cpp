@@ -491,7 +491,17 @@ class Runner {
     return m;
   }
 
+  // driver ids in use (surfaces, cylinders, balls and the mesh share one numbering: the partner id of
+  // their interactions)
+  std::vector<size_t> driver_ids_;
+  void claim_driver_id(size_t id, const std::string& what) {
+    if (std::find(driver_ids_.begin(), driver_ids_.end(), id) != driver_ids_.end())
+      fail(XDEM_ERR_ARG, what + ": driver id " + std::to_string(id) + " is already registered");
+    driver_ids_.push_back(id);
+  }
+
   void put_driver(size_t id, const xdem_surface& w, const xdem_driver_motion& m) {
+    claim_driver_id(id, "register driver");
     if (walls_.size() <= id) {
       walls_.resize(id + 1);
       xdem_driver_motion still{};
@@ -567,6 +577,7 @@ class Runner {
     std::vector<double> v;
     std::vector<uint32_t> e, f;
     double rm = 0, center[3] = {0, 0, 0}, quat[4] = {1, 0, 0, 0};
+    int cap = 32;
   } mesh_;
 
   static std::vector<std::array<double, 3>> read_stl(const std::string& path, bool binary, const std::string& what) {
@@ -620,7 +631,20 @@ class Runner {
       fail(XDEM_ERR_UNSUPPORTED, "register_stl_mesh: only .stl meshes are provided (" + f->s + ")");
     if (!slot(s, "minkowski")) fail(XDEM_ERR_ARG, "register_stl_mesh: minkowski is required");
     mesh_.rm = sdouble(s, "minkowski", 0.0);
-    mesh_.id = (int32_t)sdouble(s, "id", (double)walls_.size());
+    // id is required (register_stl_mesh.cpp:37) and must differ from the other drivers' ids
+    if (!slot(s, "id")) fail(XDEM_ERR_ARG, "register_stl_mesh: id is required");
+    const double idv = sdouble(s, "id", 0.0);
+    if (!(idv >= 0) || idv != (double)(int32_t)idv) fail(XDEM_ERR_ARG, "register_stl_mesh: id must be a non-negative integer");
+    mesh_.id = (int32_t)idv;
+    claim_driver_id((size_t)mesh_.id, what);
+    // candidate slots per sphere (no counterpart in the reference, whose lists grow): max_candidates
+    mesh_.cap = 32;
+    if (slot(s, "max_candidates")) {
+      const double mc = sdouble(s, "max_candidates", 32.0);
+      if (!(mc >= 1) || mc > 1024 || mc != (double)(int)mc)
+        fail(XDEM_ERR_ARG, "register_stl_mesh: max_candidates must be an integer in 1..1024");
+      mesh_.cap = (int)mc;
+    }
     if (const Node* st = slot(s, "state")) {
       auto vec = [&](const char* k, double* out, size_t nval) {
         if (const Node* v = slot(*st, k)) {
@@ -734,7 +758,7 @@ class Runner {
     if (mesh_.on)
       check(xdem_set_mesh(ctx_, mesh_.id, mesh_.v.data(), (int64_t)mesh_.v.size() / 3, mesh_.e.data(),
                           (int64_t)mesh_.e.size() / 2, mesh_.f.data(), (int64_t)mesh_.f.size() / 3, mesh_.rm,
-                          mesh_.center, mesh_.quat, 32),
+                          mesh_.center, mesh_.quat, mesh_.cap),
             "mesh");
     uploaded_ = true;
   }

@@ -136,20 +136,21 @@ __device__ inline bool mesh_detect(const MeshView& m, uint32_t wd, V3 p, double 
 // in FP64 (both force forms): the driver sits at the mesh centre, stationary; no contact resets the
 // history. Adds the forces and moments to F and M. cs (null: not recorded): contact state per slot,
 // kCStateLen doubles (dn, contact point, f - ft, ft, sphere centre, zeros; zeros without contact).
-// Candidate slot j (kind and index in l.word[j]) of a sphere at rx (radius Rx, mass mx, velocity vx,
-// spin wx): the contact law against the mesh (ContactLawDriverFunc, contact_sphere.hpp:298-396) with its
-// friction and rolling history in slot j, and the terms it adds to the sphere's force and moment written
-// to out[0..5]; out of contact the history is cleared and the terms are -0.0, which adds nothing to any
-// value. cs: the recorded contact state of slot j (null: not recorded).
+// Candidate jc of the compact lists l (kind and index in l.word[jc]) of a sphere at rx (radius Rx, mass
+// mx, velocity vx, spin wx): the contact law against the mesh (ContactLawDriverFunc,
+// contact_sphere.hpp:298-396) with its friction and rolling history at jc, and the terms it adds to the
+// sphere's force and moment written to out[0..5]; out of contact the history is cleared and the terms
+// are -0.0, which adds nothing to any value. cs: the recorded contact state of the candidate's list slot
+// j (null: not recorded).
 template <bool KR>
-__device__ inline void mesh_candidate(const MeshView& m, const MeshList& l, int64_t j, V3 rx, double Rx, double mx,
-                                      V3 vx, V3 wx, double dt, const ContactParams& cpd, double* __restrict__ out,
-                                      double* cs) {
+__device__ inline void mesh_candidate(const MeshView& m, const MeshList& l, int64_t jc, int64_t j, V3 rx, double Rx,
+                                      double mx, V3 vx, V3 wx, double dt, const ContactParams& cpd,
+                                      double* __restrict__ out, double* cs) {
   double dn;
   V3 nrm, cpos;
-  if (!mesh_detect(m, l.word[j], rx, Rx, dn, nrm, cpos)) {
-    l.hx[j] = 0.0; l.hy[j] = 0.0; l.hz[j] = 0.0;
-    l.mhx[j] = 0.0; l.mhy[j] = 0.0; l.mhz[j] = 0.0;
+  if (!mesh_detect(m, l.word[jc], rx, Rx, dn, nrm, cpos)) {
+    l.hx[jc] = 0.0; l.hy[jc] = 0.0; l.hz[jc] = 0.0;
+    l.mhx[jc] = 0.0; l.mhy[jc] = 0.0; l.mhz[jc] = 0.0;
     if (cs) {
       double* r = cs + (int64_t)kCStateLen * j;
 #pragma unroll
@@ -160,14 +161,14 @@ __device__ inline void mesh_candidate(const MeshView& m, const MeshList& l, int6
     reinterpret_cast<double2*>(out)[2] = double2{-0.0, -0.0};
     return;
   }
-  V3 ft{l.hx[j], l.hy[j], l.hz[j]};
+  V3 ft{l.hx[jc], l.hy[jc], l.hz[jc]};
   V3 mint{0.0, 0.0, 0.0};
-  if (KR) mint = {l.mhx[j], l.mhy[j], l.mhz[j]};
+  if (KR) mint = {l.mhx[jc], l.mhy[jc], l.mhz[jc]};
   const V3 zero{0.0, 0.0, 0.0};
   const V3 f = hooke_force<KR>(dn, nrm, dt, cpd, mx, ft, mint, cpos, rx, vx, wx, m.center, zero, zero);
-  l.hx[j] = ft.x; l.hy[j] = ft.y; l.hz[j] = ft.z;
+  l.hx[jc] = ft.x; l.hy[jc] = ft.y; l.hz[jc] = ft.z;
   if (KR) {
-    l.mhx[j] = mint.x; l.mhy[j] = mint.y; l.mhz[j] = mint.z;
+    l.mhx[jc] = mint.x; l.mhy[jc] = mint.y; l.mhz[jc] = mint.z;
   }
   if (cs) {
     const V3 fnv = f - ft;

@@ -64,6 +64,9 @@ void mesh_free(xdem_ctx* c) {
   c->mesh_kmax = 0;
   free_ml(c->ml[0]);
   free_ml(c->ml[1]);
+  free_ml(c->mc);
+  c->cap_mc = 0;
+  c->mc_valid = c->mc_dirty = false;
   c->ml_alloc = 0;
   c->ml_valid = false;
   c->has_mesh = false;
@@ -287,8 +290,53 @@ __global__ __launch_bounds__(256) void k_mesh_pass(MeshView m, MeshList l, MeshT
   const int64_t x = t.sph[i];
   const Core me = core[x];
   const V3 rx{me.xr.x, me.xr.y, me.xr.z}, vx{me.vm.x, me.vm.y, me.vm.z}, wx{me.w.x, me.w.y, me.w.z};
-  mesh_candidate<KR>(m, l, (int64_t)k * m.stride + x, rx, me.xr.w, me.vm.w, vx, wx, dt, cpd,
-                     t.terms + ((int64_t)k * t.nsph + i) * 6, cs);
+  const int64_t jc = (int64_t)k * t.nsph + i;
+  mesh_candidate<KR>(m, l, jc, (int64_t)k * m.stride + x, rx, me.xr.w, me.vm.w, vx, wx, dt, cpd, t.terms + jc * 6,
+                     cs);
+}
+
+// Compact copy <-> lists, one thread per (candidate k = blockIdx.y, particle i): the gather copies the
+// words and histories, the write-back the histories.
+__global__ __launch_bounds__(256) void k_mesh_gather(MeshTerms t, MeshList l, MeshList lc, int64_t stride) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k = blockIdx.y;
+  if (i >= t.nsph || k >= t.scnt[i]) return;
+  const int64_t j = (int64_t)k * stride + t.sph[i], jc = (int64_t)k * t.nsph + i;
+  lc.word[jc] = l.word[j];
+  lc.hx[jc] = l.hx[j]; lc.hy[jc] = l.hy[j]; lc.hz[jc] = l.hz[j];
+  lc.mhx[jc] = l.mhx[j]; lc.mhy[jc] = l.mhy[j]; lc.mhz[jc] = l.mhz[j];
+}
+__global__ __launch_bounds__(256) void k_mesh_writeback(MeshTerms t, MeshList l, MeshList lc, int64_t stride) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k = blockIdx.y;
+  if (i >= t.nsph || k >= t.scnt[i]) return;
+  const int64_t j = (int64_t)k * stride + t.sph[i], jc = (int64_t)k * t.nsph + i;
+  l.hx[j] = lc.hx[jc]; l.hy[j] = lc.hy[jc]; l.hz[j] = lc.hz[jc];
+  l.mhx[j] = lc.mhx[jc]; l.mhy[j] = lc.mhy[jc]; l.mhz[j] = lc.mhz[jc];
+}
+
+static MeshTerms terms_view(xdem_ctx* c);
+
+int mesh_compact(xdem_ctx* c) {
+  if (c->mc_valid || !c->has_mesh || !c->ml_valid || c->mesh_nsph == 0) return XDEM_OK;
+  const dim3 grid(grid_blocks(c->mesh_nsph, 256), (unsigned)c->mesh_kmax), block(256);
+  hipLaunchKernelGGL(k_mesh_gather, grid, block, 0, c->stream, terms_view(c), c->ml[c->ml_cur], c->mc,
+                     c->mesh.stride);
+  XDEM_HIP(c, hipGetLastError());
+  c->mc_valid = true;
+  c->mc_dirty = false;
+  return XDEM_OK;
+}
+
+int mesh_sync(xdem_ctx* c) {
+  if (!c->mc_dirty) return XDEM_OK;
+  c->mc_dirty = false;
+  if (!c->has_mesh || !c->ml_valid || c->mesh_nsph == 0) return XDEM_OK;
+  const dim3 grid(grid_blocks(c->mesh_nsph, 256), (unsigned)c->mesh_kmax), block(256);
+  hipLaunchKernelGGL(k_mesh_writeback, grid, block, 0, c->stream, terms_view(c), c->ml[c->ml_cur], c->mc,
+                     c->mesh.stride);
+  XDEM_HIP(c, hipGetLastError());
+  return XDEM_OK;
 }
 
 static MeshTerms terms_view(xdem_ctx* c) {
@@ -306,9 +354,12 @@ int launch_mesh_pass(xdem_ctx* c, const ContactParams& cpd, double dt, bool guar
                      MeshTerms* t) {
   *t = MeshTerms{};
   if (!c->has_mesh || !c->ml_valid || c->mesh_nsph == 0) return XDEM_OK;
+  int rc = mesh_compact(c);
+  if (rc) return rc;
   *t = terms_view(c);
   const dim3 grid(grid_blocks(c->mesh_nsph, 256), (unsigned)c->mesh_kmax), block(256);
-  const MeshList& l = c->ml[c->ml_cur];
+  const MeshList& l = c->mc;
+  c->mc_dirty = true;
   const Core* core = live(c).core;
   if (c->kr_used) {
     if (guarded)
@@ -331,8 +382,10 @@ int launch_mesh_pass(xdem_ctx* c, const ContactParams& cpd, double dt, bool guar
 
 int mesh_carry(xdem_ctx* c, double rcut_inc, bool have_old, bool poly) {
   if (!c->has_mesh || c->n == 0) return XDEM_OK;
-  int rc = alloc_ml(c);
+  int rc = mesh_sync(c);  // the previous lists' histories, as the last pass left them
   if (rc) return rc;
+  c->mc_valid = false;
+  if ((rc = alloc_ml(c))) return rc;
   const int64_t n = c->n;
   // reach of the bins
   unsigned long long* d_rmax = (unsigned long long*)c->d_mesh_over;
@@ -375,6 +428,17 @@ int mesh_carry(xdem_ctx* c, double rcut_inc, bool have_old, bool poly) {
     const int64_t cap = nt + nt / 4 + 4096;
     if ((rc = dev_realloc(c, &c->mesh_terms, cap))) return rc;
     c->cap_mesh_terms = cap;
+  }
+  const int64_t ncomp = (int64_t)over[1] * over[2];
+  if (ncomp > c->cap_mc) {
+    const int64_t cap = ncomp + ncomp / 4 + 1024;
+    MeshList& l = c->mc;
+    free_ml(l);
+    if ((rc = dev_realloc(c, &l.word, cap)) || (rc = dev_realloc(c, &l.hx, cap)) || (rc = dev_realloc(c, &l.hy, cap)) ||
+        (rc = dev_realloc(c, &l.hz, cap)) || (rc = dev_realloc(c, &l.mhx, cap)) || (rc = dev_realloc(c, &l.mhy, cap)) ||
+        (rc = dev_realloc(c, &l.mhz, cap)))
+      return rc;
+    c->cap_mc = cap;
   }
   if (over[0] > 0) {
     c->ml_valid = false;
@@ -446,6 +510,8 @@ extern "C" int xdem_mesh_interactions(xdem_ctx* c, int64_t cap, uint64_t* id_i, 
   *written = 0;
   if (!c->has_mesh || !c->ml_valid || c->n == 0) return XDEM_OK;
   XDEM_HIP(c, hipSetDevice(c->device));
+  int rc = mesh_sync(c);
+  if (rc) return rc;
   const int64_t n = c->n, st = c->cap;
   const MeshList& L = c->ml[c->ml_cur];
   std::vector<uint32_t> cnt(n);
@@ -510,6 +576,10 @@ extern "C" int xdem_mesh_interactions_load(xdem_ctx* c, int64_t m, const uint64_
   if (m == 0) return XDEM_OK;
   if (!c->has_mesh || !c->ml_valid) return set_err(c, XDEM_ERR_STATE, "mesh_interactions_load: no mesh candidate lists (run nbh_sphere)");
   XDEM_HIP(c, hipSetDevice(c->device));
+  {
+    const int rs = mesh_sync(c);  // unmatched candidates keep the history the last pass left
+    if (rs) return rs;
+  }
   struct Key {
     uint64_t id;
     uint32_t wd;
@@ -563,6 +633,7 @@ extern "C" int xdem_mesh_interactions_load(xdem_ctx* c, int64_t m, const uint64_
   for (int q = 0; q < 6; q++)
     XDEM_HIP(c, hipMemcpyAsync(dst[q], h[q].data(), sizeof(double) * tot, hipMemcpyHostToDevice, c->stream));
   XDEM_HIP(c, hipStreamSynchronize(c->stream));
+  c->mc_valid = false;  // the passes gather the loaded histories again
   if (moment && std::any_of(moment, moment + 3 * m, [](double v) { return v != 0.0; })) {
     const int rk = use_kr(c);
     if (rk) return rk;

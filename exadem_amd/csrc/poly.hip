@@ -763,18 +763,19 @@ __global__ __launch_bounds__(256) void k_poly_mesh_pass(PolyMeshArgs a) {
   const uint32_t k = blockIdx.y;
   if (i >= a.t.nsph || k >= a.t.scnt[i]) return;
   const int64_t x = a.t.sph[i];
-  const int64_t j = (int64_t)k * a.m.stride + x;
+  const int64_t j = (int64_t)k * a.m.stride + x;  // list slot (recorded state)
+  const int64_t jc = (int64_t)k * a.t.nsph + i;   // compact copy (word, history, terms)
   const MeshList& l = a.l;
-  double* out = a.t.terms + ((int64_t)k * a.t.nsph + i) * 6;
-  const uint32_t wd = l.word[j];
+  double* out = a.t.terms + jc * 6;
+  const uint32_t wd = l.word[jc];
   const Core cx = a.s.core[x];
   const xdem_shape& sa = a.shapes[a.s.type[x]];
   const double ra = sa.minkowski * cx.w.w;
   const Verts va = verts_of(a.vtx, x, V3{0, 0, 0});
   const Det d = poly_mesh_detect(a.m, wd, sa, va, ra);
   if (!d.contact) {
-    l.hx[j] = 0.0; l.hy[j] = 0.0; l.hz[j] = 0.0;
-    l.mhx[j] = 0.0; l.mhy[j] = 0.0; l.mhz[j] = 0.0;
+    l.hx[jc] = 0.0; l.hy[jc] = 0.0; l.hz[jc] = 0.0;
+    l.mhx[jc] = 0.0; l.mhy[jc] = 0.0; l.mhz[jc] = 0.0;
     if (a.cs) {
       double* r = a.cs + (int64_t)kCStateLen * j;
 #pragma unroll
@@ -788,9 +789,9 @@ __global__ __launch_bounds__(256) void k_poly_mesh_pass(PolyMeshArgs a) {
   const V3 rx{cx.xr.x, cx.xr.y, cx.xr.z}, vx{cx.vm.x, cx.vm.y, cx.vm.z}, wx{cx.w.x, cx.w.y, cx.w.z};
   const double mx = cx.vm.w;
   const V3 zero{0.0, 0.0, 0.0};
-  V3 ft{l.hx[j], l.hy[j], l.hz[j]};
+  V3 ft{l.hx[jc], l.hy[jc], l.hz[jc]};
   V3 mint{0.0, 0.0, 0.0};
-  if (KR) mint = {l.mhx[j], l.mhy[j], l.mhz[j]};
+  if (KR) mint = {l.mhx[jc], l.mhy[jc], l.mhz[jc]};
   V3 fp, mp, fnv, fst;
   if (pm_type(wd) <= 10) {
     const V3 f = hooke_force<KR>(d.dn, d.n, a.dt, a.cpd, mx, ft, mint, d.cp, rx, vx, wx, a.m.center, zero, zero);
@@ -805,9 +806,9 @@ __global__ __launch_bounds__(256) void k_poly_mesh_pass(PolyMeshArgs a) {
     fnv = ft - f;
     fst = -ft;
   }
-  l.hx[j] = fst.x; l.hy[j] = fst.y; l.hz[j] = fst.z;
+  l.hx[jc] = fst.x; l.hy[jc] = fst.y; l.hz[jc] = fst.z;
   if (KR) {
-    l.mhx[j] = mint.x; l.mhy[j] = mint.y; l.mhz[j] = mint.z;
+    l.mhx[jc] = mint.x; l.mhy[jc] = mint.y; l.mhz[jc] = mint.z;
   }
   if (a.cs) {
     const double rec[kCStateLen] = {d.dn, d.cp.x, d.cp.y, d.cp.z, fnv.x, fnv.y, fnv.z, fst.x,
@@ -1431,7 +1432,9 @@ int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, d
       }
       mcs = c->mstate;
     }
-    PolyMeshArgs pm{c->mesh, c->ml[c->ml_cur], MeshTerms{}, a.s, c->d_shapes, c->vtx, dt, cpd, c->d_flags, mcs};
+    if ((rc = mesh_compact(c))) return rc;
+    c->mc_dirty = true;
+    PolyMeshArgs pm{c->mesh, c->mc, MeshTerms{}, a.s, c->d_shapes, c->vtx, dt, cpd, c->d_flags, mcs};
     pm.t.sph = c->mesh_sph;
     pm.t.scnt = c->mesh_scnt;
     pm.t.idx = c->mesh_idx;
@@ -1572,6 +1575,10 @@ struct HostMeshLists {
 };
 static int read_mesh_lists(xdem_ctx* c, HostMeshLists& m, bool hist) {
   if (!c->has_mesh || !c->ml_valid || c->n == 0) return XDEM_OK;
+  if (hist) {
+    const int rc = mesh_sync(c);
+    if (rc) return rc;
+  }
   const MeshList& L = c->ml[c->ml_cur];
   const int64_t n = c->n;
   m.stride = c->mesh.stride;
@@ -2075,6 +2082,7 @@ extern "C" int xdem_poly_interactions_load(xdem_ctx* c, int64_t m, const uint64_
         XDEM_HIP(c, hipMemcpyAsync(dst[q], ml.h[q].data(), sizeof(double) * ml.h[q].size(), hipMemcpyHostToDevice,
                                    c->stream));
       XDEM_HIP(c, hipStreamSynchronize(c->stream));
+      c->mc_valid = false;  // the passes gather the loaded histories again
     }
   }
   if (moment)
@@ -2195,6 +2203,8 @@ int poly_sim_counts(xdem_ctx* c, double* tot, double* act, double* min_dn, doubl
   MeshView mv{};
   MeshList ml{};
   if (c->has_mesh && c->ml_valid) {
+    int rc = mesh_sync(c);
+    if (rc) return rc;
     mv = c->mesh;
     ml = c->ml[c->ml_cur];
   }

@@ -504,6 +504,13 @@ struct xdem_ctx {
   int32_t* mesh_idx = nullptr;
   double* mesh_terms = nullptr;
   int64_t cap_mesh_terms = 0;
+  // compact copy of the current candidate words and histories, candidate k of particle i (among those
+  // with candidates) at k * mesh_nsph + i, which the mesh passes read and write; mc_valid: gathered from
+  // the lists since their last change; mc_dirty: newer than the lists (mesh_sync writes them back)
+  xdem::MeshList mc;
+  int64_t cap_mc = 0;
+  bool mc_valid = false;
+  bool mc_dirty = false;
   int64_t mesh_nsph = 0;
   int mesh_kmax = 0;
   double* mstate = nullptr;         // recorded contact state per mesh slot (kCStateLen doubles each)
@@ -574,6 +581,10 @@ void mesh_free(xdem_ctx* c);
 // the force pass reads (t->nsph = 0: no mesh contacts to add).
 int launch_mesh_pass(xdem_ctx* c, const ContactParams& cpd, double dt, bool guarded, int32_t trig_in, double* cs,
                      MeshTerms* t);
+// The compact copy the passes use (gathered from the lists when they changed), and the write-back of
+// its histories into the lists before anything else reads or replaces them.
+int mesh_compact(xdem_ctx* c);
+int mesh_sync(xdem_ctx* c);
 int set_history_form(xdem_ctx* c, bool fp32);
 // a rolling stiffness comes into use (clears the sphere list's moment histories the first time)
 int use_kr(xdem_ctx* c);

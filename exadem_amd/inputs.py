@@ -255,12 +255,13 @@ def _start_with_accelerations(sim):
     sim.ctx.force_to_accel()
 
 
-def polyhedra_dense(n=200_000, phi=0.45, seed=4, dt=2e-6, growth=1.01, relax_steps=200, final_relax=3,
-                    settle_steps=0, dt_grow=5e-4, log=None, cache=None):
+def polyhedra_dense(n=200_000, phi=0.45, seed=4, dt=2e-6, growth=1.015, relax_steps=200, final_relax=3,
+                    settle_steps=0, dt_grow=5e-4, h0=0.8, log=None, cache=None):
     """Polyhedra at solid fraction phi of the box, grown with the GPU DEM.
 
-    Starts from the jittered lattice of `polyhedra` (spacing 1.55, solid fraction about 0.25, periodic in
-    x and y, floor and lid) and grows every particle's homothety by `growth` per stage (mass x h^3,
+    Starts from the jittered lattice of `polyhedra` (spacing 1.55, periodic in x and y, floor and lid)
+    with every particle scaled by h0 (0.8: bounding spheres 1.42 across, so that no two particles
+    overlap whatever their orientations) and grows every particle's homothety by `growth` per stage (mass x h^3,
     inertia x h^5, bounding radius x h, cells re-sized to 2 max radius + rcut_inc) until the particles'
     volume fills phi of the box; each stage starts at rest and relaxes for relax_steps DEM steps of
     dt_grow without gravity, and final_relax more such stages follow at the final size. The result stands
@@ -276,7 +277,7 @@ def polyhedra_dense(n=200_000, phi=0.45, seed=4, dt=2e-6, growth=1.01, relax_ste
     import os as _os
     from .sim import PolySim
     w = polyhedra(n=n, spacing=1.55, jitter=0.05, seed=seed, dt=dt)
-    key = _json.dumps([n, phi, seed, dt, growth, relax_steps, final_relax, settle_steps, dt_grow,
+    key = _json.dumps([n, phi, seed, dt, growth, relax_steps, final_relax, settle_steps, dt_grow, h0,
                        float(np.sum(w.particles["mass"]))])
     if cache and _os.path.exists(cache):
         with np.load(cache) as z:
@@ -295,11 +296,16 @@ def polyhedra_dense(n=200_000, phi=0.45, seed=4, dt=2e-6, growth=1.01, relax_ste
     phi0 = float(np.sum(vol_of(p))) / vbox
     h_target = (phi / phi0) ** (1.0 / 3.0)
     rb_max = float(np.max(p["radius"]))
+    p["homothety"] = p["homothety"] * h0
+    p["radius"] = p["radius"] * h0
+    p["mass"] = p["mass"] * h0 ** 3
+    for a in ("inx", "iny", "inz"):
+        p[a] = p[a] * h0 ** 5
     g = _copy.copy(w)
     g.gravity = (0.0, 0.0, 0.0)
     g.dt = dt_grow
     sim = PolySim(g)
-    h, stages = 1.0, 0
+    h, stages = h0, 0
     while h < h_target:
         h_new = min(h * growth, h_target)
         f = h_new / h

@@ -224,7 +224,8 @@ def test_deck_cylinder_drum_matches_operator_path(gpu_ok, tmp_path):
 @pytest.mark.gpu
 def test_deck_stl_mesh_matches_operator_path(gpu_ok, tmp_path):
     """register_stl_mesh (a stationary .stl tray, minkowski 0.02) in a deck: the deck run equals the
-    Python operator path with the same mesh bit for bit; a moving mesh is refused."""
+    Python operator path with the same mesh bit for bit; a moving mesh, a missing or duplicate id, and
+    more candidates than max_candidates are refused."""
     import copy
     import mesh_cases as mc
     from exadem_amd import stl
@@ -256,3 +257,12 @@ def test_deck_stl_mesh_matches_operator_path(gpu_ok, tmp_path):
     bad = native.Deck(write(tmp_path / "moving.msp", text[:i] + "setup_drivers:\n" + moving + text[j:]))
     with pytest.raises(native.XdemError, match="UNSUPPORTED"):
         bad.run(native.Context())
+    # the id is required and unique among the drivers; max_candidates sets the slots per sphere
+    no_id = mesh.replace("id: 6, ", "")
+    wall6 = "  - register_surface: { id: 6, state: { normal: [0, 0, 1], offset: -5.0 } }\n"
+    few = mesh.replace("minkowski: 0.02", "minkowski: 0.02, max_candidates: 1")
+    for name, body, err in (("noid", no_id, "id is required"), ("dup", wall6 + mesh, "already registered"),
+                            ("few", few, "CAPACITY")):
+        bad = native.Deck(write(tmp_path / f"{name}.msp", text[:i] + "setup_drivers:\n" + body + text[j:]))
+        with pytest.raises(native.XdemError, match=err):
+            bad.run(native.Context())

@@ -10,6 +10,8 @@
   its accumulated position error against plain DEM below half that of free flight between the same
   checks.
 """
+import copy
+
 import numpy as np
 import pytest
 
@@ -93,9 +95,13 @@ def _feature_table(ii, jj, tt, sw, si, sj, fr):
 
 
 def test_200k_polyhedra_match_oracle(gpu_ok):
+    """The polyhedra benchmark input (grown to solid fraction 0.45 and relaxed: no overlaps, every pair
+    within rVerlet a candidate) bit-exact with the oracle after first_iteration and 20 benchmark steps;
+    then, to bring many contacts of every type, the same particles with velocities N(0, 2) per component
+    and dt 4e-4 (about 40 steps per contact time), 50 more engine steps with rebuilds, also bit-exact."""
     w = inputs.polyhedra_benchmark(n=200_000)
-    assert abs(w.meta["solid_fraction"] - 0.45) < 1e-9 and w.meta["active_interactions"] > 50_000
-    assert w.meta["min_dn"] > -0.02  # no vertex deeper than the Minkowski shells
+    assert abs(w.meta["solid_fraction"] - 0.45) < 1e-9 and w.meta["interactions"] > 1_000_000
+    assert w.meta["min_dn"] == 0.0  # relaxed: no overlap left
     sch = scheme(w)
     s = PolySim(w)
     s.first_iteration()
@@ -104,11 +110,11 @@ def test_200k_polyhedra_match_oracle(gpu_ok):
     o.first_iteration_poly(sch)
     gk, gf = _feature_table(*s.ctx.poly_interactions())
     ok, of = _feature_table(*o.poly_interactions())
-    assert len(gk) == len(ok) and len(gk) > 100_000
+    assert len(gk) == len(ok) and len(gk) > 500_000
     assert np.array_equal(gk, ok) and np.array_equal(gf, of)
     types = np.bincount(gk[:, 2], minlength=6)
-    print(f"200k polyhedra: feature interactions by type {types.tolist()}", flush=True)
-    assert types[0] + types[1] > 0 and types[2] > 0 and types[3] > 0 and types[5] > 0
+    print(f"200k polyhedra: feature candidates by type {types.tolist()}", flush=True)
+    assert np.all(types[:4] > 0)
     floor = by_id(o.particles())["mass"] * 9.81
     _state_close(s.ctx.download(), o.particles(), floor, "200k polyhedra, first_iteration", ptol=0.0)
     s.run(20)
@@ -117,6 +123,33 @@ def test_200k_polyhedra_match_oracle(gpu_ok):
     ok, of = _feature_table(*o.poly_interactions())
     assert np.array_equal(gk, ok) and np.array_equal(gf, of)
     _state_close(s.ctx.download(), o.particles(), floor, "200k polyhedra, 20 engine steps")
+    # shaken: contacts of every type
+    p = s.ctx.download()
+    v = inputs.normal(77, 2, 3 * w.n, 0.0, 2.0).reshape(3, w.n)
+    order = np.argsort(p["id"])
+    for a, k in enumerate(("vx", "vy", "vz")):
+        p[k] = np.empty(w.n)
+        p[k][order] = v[a]
+    w2 = copy.copy(w)
+    w2.particles = p
+    w2.dt = 4e-4
+    sch2 = scheme(w2)
+    s2 = PolySim(w2)
+    s2.first_iteration()
+    o2 = po.Oracle(p, w2.domain, w2.walls)
+    o2.set_shapes(w2.shapes)
+    o2.first_iteration_poly(sch2)
+    st = s2.run(50)
+    o2.run_steps_poly(sch2, 50)
+    gk, gf = _feature_table(*s2.ctx.poly_interactions())
+    ok, of = _feature_table(*o2.poly_interactions())
+    assert np.array_equal(gk, ok) and np.array_equal(gf, of)
+    act = np.any(gf != 0, axis=1)
+    contacts = np.bincount(gk[act, 2], minlength=6)
+    print(f"200k polyhedra shaken, 50 steps of 4e-4: {st['rebuilds']} rebuilds, contacts by type "
+          f"{contacts.tolist()}", flush=True)
+    assert np.all(contacts[1:4] > 0) and int(np.sum(contacts)) > 10_000
+    _state_close(s2.ctx.download(), o2.particles(), floor, "200k polyhedra shaken, 50 engine steps")
 
 
 def test_1m_surrogate_k10_accumulated_error(gpu_ok, spheres_1m):

@@ -242,6 +242,7 @@ def test_mesh_contact_dump(gpu_ok):
     s.ctx.set_contact_record(True)
     s.first_iteration()
     ii, jj, si, sj, tt, vals = s.ctx.contact_state(subs=True)
+    assert np.all(np.diff(tt.astype(np.int64)) >= 0)  # one block per type, as the reference writes
     sel = tt >= 7
     assert np.all(jj[sel] == m["mesh_id"]) and np.all(si[sel] == 0)
     mi, ms, mt, mf, _ = s.ctx.mesh_interactions()
@@ -280,3 +281,27 @@ def test_classifier_waves_spheres(gpu_ok):
     bs = b.ctx.classifier_waves()
     _, _, bt, _ = b.ctx.interactions()
     assert bs[0] == int(np.count_nonzero(bt == 0)) and bs[5] == int(np.count_nonzero(bt == 5)) > 0
+
+
+def test_placed_mesh_matches_oracle(gpu_ok):
+    """The tray registered with a pose (quaternion of 90 degrees about z, centre (20, 0, 0), which maps the
+    tray onto itself with its vertices renumbered): the first pass and 30 engine steps bit-exact with
+    the oracle given the same pose."""
+    import math
+    w = mc.workload("tray", mc.tray_bed())
+    m = dict(mc.tray_mesh(0.02), center=(20.0, 0.0, 0.0), quat=(math.cos(math.pi / 4), 0.0, 0.0, math.sin(math.pi / 4)))
+    sch = scheme(w)
+    o = po.Oracle(w.particles, w.domain, w.walls)
+    o.set_mesh(m["mesh_id"], m["vertices"], m["edges"], m["faces"], m["minkowski"], center=m["center"], quat=m["quat"])
+    o.first_iteration(sch)
+    s = SphereSim(w, mesh=m)
+    s.first_iteration()
+    kg, fg = mesh_table(*s.ctx.mesh_interactions()[:4])
+    ko, fo = mesh_table(*o.mesh_interactions()[:4])
+    assert np.array_equal(kg, ko) and np.array_equal(fg, fo) and np.count_nonzero(np.any(fo != 0, axis=1)) > 100
+    s.run(30)
+    o.run_steps(sch, 30)
+    check_state(by_id(s.ctx.download()), by_id(o.particles()))
+    kg, fg = mesh_table(*s.ctx.mesh_interactions()[:4])
+    ko, fo = mesh_table(*o.mesh_interactions()[:4])
+    assert np.array_equal(kg, ko) and np.array_equal(fg, fo)

@@ -168,6 +168,7 @@ def test_contact_state_dump(gpu_ok, tmp_path):
     s.ctx.set_contact_record(True)
     s.first_iteration()
     ii, jj, si, sj, tt, v = s.ctx.contact_state(subs=True)
+    assert np.all(np.diff(tt.astype(np.int64)) >= 0)  # one block per type, as the reference writes
     feat, wall = tt < 4, tt == 5
     assert np.count_nonzero(feat) > 0 and np.count_nonzero(wall) > 0
     assert np.all(v[feat, 0] < 0)

@@ -122,6 +122,7 @@ def test_tray_polyhedra_waves_state_and_dump(gpu_ok):
     assert st["n_interactions"] == int(np.sum(coef))
     assert st["n_active"] == int(np.sum(coef[np.any(fr != 0, axis=1)]))
     ci, cj, csi, csj, ctt, v = s.ctx.contact_state(subs=True)
+    assert np.all(np.diff(ctt.astype(np.int64)) >= 0)  # one block per type, as the reference writes
     hist = {(int(a), int(b), int(t), int(x), int(y), int(z)): f for a, b, t, x, y, z, f in zip(ii, jj, tt, sw, si, sj, fr)}
     p = by_id(s.ctx.download())
     F = p["mass"][:, None] * np.asarray(w.gravity)[None, :]

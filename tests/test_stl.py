@@ -74,3 +74,28 @@ def test_oracle_mesh_contact_closed_form(where, kind, rm):
     # candidates: exactly the features within R + rm + rcut_inc of the centre (vertices by distance)
     near_v = np.nonzero(np.linalg.norm(m["vertices"] - c, axis=1) <= R + rm + w.rcut_inc)[0]
     assert np.array_equal(np.sort(sj[tt == 7]), near_v)
+
+
+@pytest.mark.parametrize("where", ["vertex", "face"])
+def test_oracle_mesh_pose(where):
+    """A posed mesh (register_stl_mesh state {center, quat}): world vertex = centre + R(q) v with R the
+    rotation matrix of q. The tray turned by 90 degrees about z and moved by (20, 0, 0) lies where it
+    lay before (its cells split along the other diagonal), so a sphere pressing the apex or a flat face
+    by the overlap feels kn * overlap along the same direction as on the unposed tray; the inverse
+    rotation would put the bump at (30, -10) and leave them untouched."""
+    import math
+    R, ov, rm = 0.5, 0.01, 0.02
+    # the turned tray splits its cells along the other diagonal: the face probe sits 0.35 from both
+    c = mc.probes(R, ov, rm)["vertex"] if where == "vertex" else np.array([2.5, 5.0, R + rm - ov])
+    w = mc.workload("probe", mc.single(c, R), gravity=(0.0, 0.0, 0.0))
+    o = po.Oracle(w.particles, w.domain, w.walls)
+    m = mc.tray_mesh(rm)
+    q = (math.cos(math.pi / 4), 0.0, 0.0, math.sin(math.pi / 4))
+    o.set_mesh(m["mesh_id"], m["vertices"], m["edges"], m["faces"], rm, center=(20.0, 0.0, 0.0), quat=q)
+    o.first_iteration(scheme(w))
+    p = o.particles()
+    F = np.array([p["fx"][0], p["fy"][0], p["fz"][0]])
+    apex = np.array([10.0, 10.0, 1.0])
+    direction = c - apex if where == "vertex" else np.array([0.0, 0.0, 1.0])
+    assert np.linalg.norm(F) == pytest.approx(1e5 * ov, rel=1e-9)
+    assert F / np.linalg.norm(F) == pytest.approx(direction / np.linalg.norm(direction), abs=1e-12)
