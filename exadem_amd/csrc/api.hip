@@ -423,7 +423,7 @@ void xdem_ctx_destroy(xdem_ctx* c) {
   dfree(&c->sort_key);
   dfree(&c->d_flags); dfree(&c->d_u32tmp); dfree(&c->d_walls);
   dfree(&c->cstate); dfree(&c->wstate); dfree(&c->simpart); dfree(&c->d_shapes); dfree(&c->vtx); dfree(&c->frange); dfree(&c->fcontrib); dfree(&c->fcontact); dfree(&c->fdn);
-  dfree(&c->feat_tmp); dfree(&c->d_feat_over);
+  dfree(&c->feat_tmp); dfree(&c->oslot); dfree(&c->d_nown);
   dfree(&c->forder); dfree(&c->fclass_blk); dfree(&c->fclass_info); dfree(&c->d_walls_ring);
   if (c->h_walls_ring) (void)hipHostFree(c->h_walls_ring);
   for (int k = 0; k < 2; k++) {

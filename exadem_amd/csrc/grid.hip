@@ -624,16 +624,21 @@ __global__ __launch_bounds__(256) void k_hist_scatter(int64_t oslots, NbrList nl
     my = oh.my[s2];
     mz = oh.mz[s2];
   }
+  // moment histories are meaningful only once a rolling stiffness is used (use_kr clears them then)
   const uint64_t t1 = find_entry(nl, nx, ny);
   if (t1 != ~0ull && (nl.word[t1] & kOwnerBit)) {
     nh.x[t1] = hx; nh.y[t1] = hy; nh.z[t1] = hz;
-    nh.mx[t1] = mx; nh.my[t1] = my; nh.mz[t1] = mz;
+    if (KR) {
+      nh.mx[t1] = mx; nh.my[t1] = my; nh.mz[t1] = mz;
+    }
     nl.word[t1] |= kActiveBit;
   }
   const uint64_t t2 = find_entry(nl, ny, nx);
   if (t2 != ~0ull && !(nl.word[t2] & kOwnerBit)) {
     nh.x[t2] = hx; nh.y[t2] = hy; nh.z[t2] = hz;
-    nh.mx[t2] = mx; nh.my[t2] = my; nh.mz[t2] = mz;
+    if (KR) {
+      nh.mx[t2] = mx; nh.my[t2] = my; nh.mz[t2] = mz;
+    }
     nl.word[t2] |= kActiveBit;
   }
 }

@@ -381,62 +381,68 @@ __device__ inline void for_each_feature(const PolyNbhArgs& a, int64_t x, uint32_
   }
 }
 
-// One thread per list slot (slot -> particle through nl.sx; padding slots hold ~0u): an owner slot
-// enumerates its pair's features once, writing its count and up to kFeatPerSlot feature words into its
-// own region of tmp; a slot with more raises *over, and k_feat_fill then enumerates again into place.
-constexpr int kFeatPerSlot = 24;
-__global__ __launch_bounds__(256) void k_feat_enum(PolyNbhArgs a, uint32_t* __restrict__ fcount,
-                                                   uint32_t* __restrict__ tmp, int32_t* __restrict__ over) {
+// The owner slots of the list (one per pair, the lower id's entry; both entries of a pair seen through a
+// periodic image), compacted per wavefront (ballot, one atomic per wavefront): the feature enumeration
+// then runs one owner pair per thread, without the partner entries' idle lanes.
+__global__ __launch_bounds__(256) void k_owner_slots(NbrList nl, uint32_t* __restrict__ oslot,
+                                                     uint32_t* __restrict__ count) {
   const int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (sl >= a.nl.slots) return;
-  const uint32_t x = a.nl.sx[sl];
+  const bool own = sl < nl.slots && nl.sx[sl] != 0xffffffffu && (nl.word[sl] & kOwnerBit);
+  const unsigned long long bal = __ballot(own);
+  if (!bal) return;
+  const int lane = threadIdx.x & 63;
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(count, (uint32_t)__popcll(bal));
+  base = __shfl(base, 0, 64);
+  if (own) oslot[base + __popcll(bal & ((1ull << lane) - 1ull))] = (uint32_t)sl;
+}
+
+// One thread per owner slot: the pair's features enumerated once, the count and up to kFeatPerSlot
+// feature words kept in the slot's region of tmp.
+constexpr int kFeatPerSlot = 24;
+__global__ __launch_bounds__(256) void k_feat_enum(PolyNbhArgs a, const uint32_t* __restrict__ oslot,
+                                                   const uint32_t* __restrict__ nown, uint32_t* __restrict__ fcount,
+                                                   uint32_t* __restrict__ tmp) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)*nown) return;
+  const uint32_t sl = oslot[t];
+  const uint32_t x = a.nl.sx[sl], wd = a.nl.word[sl];
   uint32_t nf = 0;
-  if (x != 0xffffffffu) {
-    const uint32_t wd = a.nl.word[sl];
-    if (wd & kOwnerBit) {
-      uint32_t* out = tmp + sl * kFeatPerSlot;
-      for_each_feature(a, x, wd & kIdxMask, (int)((wd >> kShiftShift) & 31u), [&](int t, int sw, int si, int sj) {
-        if (nf < (uint32_t)kFeatPerSlot) out[nf] = sw ? feat_word(t, 1, si, sj) : feat_word(t, 0, si, sj);
-        nf++;
-      });
-      if (nf > (uint32_t)kFeatPerSlot) *over = 1;
-    }
-  }
+  uint32_t* out = tmp + (int64_t)sl * kFeatPerSlot;
+  for_each_feature(a, x, wd & kIdxMask, (int)((wd >> kShiftShift) & 31u), [&](int ty, int sw, int si, int sj) {
+    if (nf < (uint32_t)kFeatPerSlot) out[nf] = sw ? feat_word(ty, 1, si, sj) : feat_word(ty, 0, si, sj);
+    nf++;
+  });
   fcount[sl] = nf;
 }
 
-// Copies each owner slot's enumerated words to its place in the feature list.
-__global__ __launch_bounds__(256) void k_feat_place(NbrList nl, const uint32_t* __restrict__ fcount,
+// One thread per owner slot: its feature words copied to their place in the list (slot order, from the
+// scan of the counts); a pair with more than kFeatPerSlot features is enumerated again, into place.
+__global__ __launch_bounds__(256) void k_feat_place(PolyNbhArgs a, const uint32_t* __restrict__ oslot,
+                                                    const uint32_t* __restrict__ nown,
+                                                    const uint32_t* __restrict__ fcount,
                                                     const uint32_t* __restrict__ tmp, const uint64_t* __restrict__ start,
                                                     uint32_t* __restrict__ feat, uint32_t* __restrict__ fslot,
                                                     uint32_t* __restrict__ fown) {
-  const int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (sl >= nl.slots) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)*nown) return;
+  const uint32_t sl = oslot[t];
   const uint32_t nf = fcount[sl];
   if (nf == 0) return;
-  const uint32_t x = nl.sx[sl];
-  const uint64_t o = start[sl];
-  for (uint32_t q = 0; q < nf; q++) {
-    feat[o + q] = tmp[sl * kFeatPerSlot + q];
-    fslot[o + q] = (uint32_t)sl;
-    fown[o + q] = x;
-  }
-}
-
-__global__ __launch_bounds__(256) void k_feat_fill(PolyNbhArgs a, const uint64_t* __restrict__ start,
-                                                   uint32_t* __restrict__ feat, uint32_t* __restrict__ fslot,
-                                                   uint32_t* __restrict__ fown) {
-  const int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (sl >= a.nl.slots) return;
   const uint32_t x = a.nl.sx[sl];
-  if (x == 0xffffffffu) return;
-  const uint32_t wd = a.nl.word[sl];
-  if (!(wd & kOwnerBit)) return;
   uint64_t o = start[sl];
-  for_each_feature(a, x, wd & kIdxMask, (int)((wd >> kShiftShift) & 31u), [&](int t, int sw, int si, int sj) {
-    // keys use owner().sub_, partner().sub_: the swap pass stores the vertex of the partner as partner sub
-    feat[o] = sw ? feat_word(t, 1, si, sj) : feat_word(t, 0, si, sj);
-    fslot[o] = (uint32_t)sl;
+  if (nf <= (uint32_t)kFeatPerSlot) {
+    for (uint32_t q = 0; q < nf; q++) {
+      feat[o + q] = tmp[(int64_t)sl * kFeatPerSlot + q];
+      fslot[o + q] = sl;
+      fown[o + q] = x;
+    }
+    return;
+  }
+  const uint32_t wd = a.nl.word[sl];
+  for_each_feature(a, x, wd & kIdxMask, (int)((wd >> kShiftShift) & 31u), [&](int ty, int sw, int si, int sj) {
+    feat[o] = sw ? feat_word(ty, 1, si, sj) : feat_word(ty, 0, si, sj);
+    fslot[o] = sl;
     fown[o] = x;
     o++;
   });
@@ -853,20 +859,23 @@ int poly_nbh(xdem_ctx* c, double rcut_inc) {
     if ((rc = galloc(c, &c->feat_tmp, cap * kFeatPerSlot))) return rc;
     c->cap_feat_tmp = cap;
   }
-  if (!c->d_feat_over && (rc = galloc(c, &c->d_feat_over, 2))) return rc;
-  XDEM_HIP(c, hipMemsetAsync(c->d_feat_over, 0, sizeof(int32_t), c->stream));
+  if (L.slots > c->cap_oslot) {
+    const int64_t cap = (int64_t)(L.slots * 1.25) + 4096;
+    if ((rc = galloc(c, &c->oslot, cap))) return rc;
+    c->cap_oslot = cap;
+  }
+  if (!c->d_nown && (rc = galloc(c, &c->d_nown, 2))) return rc;
+  XDEM_HIP(c, hipMemsetAsync(c->d_nown, 0, sizeof(uint32_t), c->stream));
+  XDEM_HIP(c, hipMemsetAsync(P.fcount, 0, sizeof(uint32_t) * std::max<int64_t>(L.slots, 1), c->stream));
   PolyNbhArgs a{n, live(c), c->grid, c->d_shapes, c->vtx, L, rcut_inc};
+  const dim3 gsl(grid_blocks(std::max<int64_t>(L.slots, 1), 256)), bsl(256);
   if (n > 0) {
-    hipLaunchKernelGGL(k_feat_enum, dim3(grid_blocks(L.slots, 256)), dim3(256), 0, c->stream, a, P.fcount,
-                       c->feat_tmp, c->d_feat_over);
+    hipLaunchKernelGGL(k_owner_slots, gsl, bsl, 0, c->stream, L, c->oslot, c->d_nown);
+    hipLaunchKernelGGL(k_feat_enum, gsl, bsl, 0, c->stream, a, c->oslot, c->d_nown, P.fcount, c->feat_tmp);
     if ((rc = scan_u64_from_u32(c, P.fcount, P.start, L.slots, 1u))) return rc;
   }
   uint64_t nf = 0;
-  int32_t over = 0;
-  if (n > 0) {
-    XDEM_HIP(c, hipMemcpyAsync(&over, c->d_feat_over, sizeof(over), hipMemcpyDeviceToHost, c->stream));
-    if ((rc = read_u64(c, P.start + L.slots, &nf))) return rc;
-  }
+  if (n > 0 && (rc = read_u64(c, P.start + L.slots, &nf))) return rc;
   if ((int64_t)nf > P.cap_feat) {
     const int64_t cap = (int64_t)(nf * 1.25) + 4096;
     if ((rc = galloc(c, &P.feat, cap)) || (rc = galloc(c, &P.hx, cap)) || (rc = galloc(c, &P.hy, cap)) ||
@@ -885,12 +894,8 @@ int poly_nbh(xdem_ctx* c, double rcut_inc) {
   if (nf >= 0xffffffffull) return set_err(c, XDEM_ERR_CAPACITY, "nbh_polyhedron: more than 2^32 - 2 features");
   P.nfeat = (int64_t)nf;
   if (n > 0) {
-    if (over)
-      hipLaunchKernelGGL(k_feat_fill, dim3(grid_blocks(L.slots, 256)), dim3(256), 0, c->stream, a, P.start, P.feat,
-                         P.fslot, P.fown);
-    else
-      hipLaunchKernelGGL(k_feat_place, dim3(grid_blocks(L.slots, 256)), dim3(256), 0, c->stream, L, P.fcount,
-                         c->feat_tmp, P.start, P.feat, P.fslot, P.fown);
+    hipLaunchKernelGGL(k_feat_place, gsl, bsl, 0, c->stream, a, c->oslot, c->d_nown, P.fcount, c->feat_tmp, P.start,
+                       P.feat, P.fslot, P.fown);
     hipLaunchKernelGGL(k_feat_range, dim3(grid_blocks(L.slots, 256)), dim3(256), 0, c->stream, L, P.start,
                        c->frange);
     const bool have_old = c->nl_prev_valid && c->perm_valid && c->pl_valid;

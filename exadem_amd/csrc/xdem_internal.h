@@ -462,7 +462,9 @@ struct xdem_ctx {
   int64_t cap_contrib = 0;
   uint32_t* feat_tmp = nullptr;     // [cap_feat_tmp] per list slot, up to kFeatPerSlot feature words of the
   int64_t cap_feat_tmp = 0;         // one-pass enumeration (nbh_polyhedron)
-  int32_t* d_feat_over = nullptr;   // a slot enumerated more than kFeatPerSlot features
+  uint32_t* oslot = nullptr;        // [cap_oslot] owner slots of the list (nbh_polyhedron)
+  int64_t cap_oslot = 0;
+  uint32_t* d_nown = nullptr;       // their number
   uint2* frange = nullptr;          // [cap_frange] per list slot: first feature and count of its pair
                                     // (through the owner entry for partner entries); x = ~0u: no owner entry
   int64_t cap_frange = 0;
