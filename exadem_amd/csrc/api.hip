@@ -389,6 +389,7 @@ int xdem_ctx_create(int32_t device, xdem_ctx** out) {
       hipMalloc((void**)&c->d_flags, 4 * sizeof(int32_t)) != hipSuccess ||
       hipHostMalloc((void**)&c->h_flags, 4 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
       hipMalloc((void**)&c->d_u32tmp, 16) != hipSuccess || hipMalloc((void**)&c->d_rmax, 8) != hipSuccess ||
+      hipMalloc((void**)&c->d_pairs, 16) != hipSuccess ||
       hipHostMalloc((void**)&c->h_u32tmp, 16, hipHostMallocDefault) != hipSuccess ||
       hipMalloc((void**)&c->d_walls, sizeof(Surface) * XDEM_MAX_SURFACES) != hipSuccess) {
     delete c;
@@ -421,9 +422,9 @@ void xdem_ctx_destroy(xdem_ctx* c) {
   dfree(&c->cell_lin); dfree(&c->d_rmax); dfree(&c->nbh_tmp); dfree(&c->mxr); dfree(&c->mid); dfree(&c->blk_u32);
   dfree(&c->perm_tmp);
   dfree(&c->sort_key);
-  dfree(&c->d_flags); dfree(&c->d_u32tmp); dfree(&c->d_walls);
+  dfree(&c->d_flags); dfree(&c->d_u32tmp); dfree(&c->d_pairs); dfree(&c->d_walls);
   dfree(&c->cstate); dfree(&c->wstate); dfree(&c->simpart); dfree(&c->d_shapes); dfree(&c->vtx); dfree(&c->frange); dfree(&c->fcontrib); dfree(&c->fcontact); dfree(&c->fdn);
-  dfree(&c->feat_tmp); dfree(&c->oslot); dfree(&c->d_nown);
+  dfree(&c->feat_tmp); dfree(&c->oslot); dfree(&c->feat_ovf); dfree(&c->feat_fpool); dfree(&c->feat_pool); dfree(&c->d_nown);
   dfree(&c->forder); dfree(&c->fclass_blk); dfree(&c->fclass_info); dfree(&c->d_walls_ring);
   if (c->h_walls_ring) (void)hipHostFree(c->h_walls_ring);
   for (int k = 0; k < 2; k++) {
@@ -650,6 +651,7 @@ int xdem_particles_upload(xdem_ctx* c, const xdem_particles* p) {
   if (n > XDEM_MAX_PARTICLES) return set_err(c, XDEM_ERR_CAPACITY, "too many particles");
   if (n > 0 && (!p->id || !p->rx || !p->ry || !p->rz || !p->mass || !p->radius))
     return set_err(c, XDEM_ERR_ARG, "upload: id, rx, ry, rz, mass and radius are required");
+  double rmax = 0.0;
   {
     std::unordered_set<uint64_t> seen;
     seen.reserve((size_t)n * 2);
@@ -657,12 +659,14 @@ int xdem_particles_upload(xdem_ctx* c, const xdem_particles* p) {
       if (!seen.insert(p->id[i]).second) return set_err(c, XDEM_ERR_ARG, "upload: particle ids must be unique");
       if (!(p->mass[i] > 0) || !(p->radius[i] > 0))
         return set_err(c, XDEM_ERR_ARG, "upload: mass and radius must be > 0");
+      rmax = std::max(rmax, p->radius[i]);
     }
   }
   XDEM_HIP(c, hipSetDevice(c->device));
   int rc = ensure_capacity(c, std::max<int64_t>(n, 1));
   if (rc) return rc;
   c->n = n;
+  c->rmax_host = rmax;  // radii change only here
   c->cur = 0;
   PState& s = c->ps[0];
   // staging: host vectors with defaults for absent fields
@@ -1117,7 +1121,11 @@ int xdem_interactions_dump(xdem_ctx* c, int64_t cap, uint64_t* id_i, uint64_t* i
 int xdem_nbh_stats(xdem_ctx* c, int64_t* pairs, int64_t* slots, int32_t dims[3]) {
   if (!c) return XDEM_ERR_ARG;
   const NbrList& l = c->nl[c->nl_cur];
-  if (pairs) *pairs = c->nl_valid ? l.pairs : 0;
+  if (pairs) *pairs = 0;
+  if (pairs && c->nl_valid) {
+    const int rc = nl_pairs(c, pairs);
+    if (rc) return rc;
+  }
   if (slots) *slots = c->nl_valid ? l.slots : 0;
   if (dims)
     for (int a = 0; a < 3; a++) dims[a] = c->grid.dims[a];

@@ -68,7 +68,7 @@ struct ForceArgs {
   // (wall, particle), kCStateLen doubles each (null: not recorded)
   double* cstate;
   double* wstate;
-  // terms of the triangle-mesh contacts, evaluated by the mesh pass just before (nsph = 0: none)
+  // terms of the triangle-mesh contacts, evaluated by the mesh pass just before (ncand = 0: none)
   MeshTerms mt;
 };
 
@@ -126,7 +126,10 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
   __syncthreads();
   const int64_t x = b0 + threadIdx.x;
   if (x >= a.n) return;
-  const int32_t midx = MESH && a.mt.nsph ? a.mt.idx[x] : -1;
+  // mesh terms: only waves of 64 particles that have candidates read their particles' index (one
+  // scalar load per wave)
+  const bool mwave = MESH && a.mt.ncand && a.mt.wcnt[__builtin_amdgcn_readfirstlane((int)(x >> 6))] != 0u;
+  const uint2 midx = mwave ? a.mt.idx[x] : uint2{0u, 0u};
   const PState& s = a.s;
   uint32_t* __restrict__ word = a.nl.word;
   const uint32_t cnt = a.nl.cnt[x];
@@ -241,6 +244,13 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
       }
     }
   }
+  // the integration's own inputs (inertia, rebuild reference), loaded before the walls and the mesh terms
+  // so that their latency overlaps with those
+  V3 inert{0.0, 0.0, 0.0}, bref{0.0, 0.0, 0.0};
+  if (FUSED && INTEG) {
+    inert = {s.ix[x], s.iy[x], s.iz[x]};
+    bref = {s.bx[x], s.by[x], s.bz[x]};
+  }
   if (WALLS) {
     // ContactLawDriverFunc<Surface / Cylinder / Ball> (contact_sphere.hpp:216-297) with the driver's
     // detector (driver_detect)
@@ -281,13 +291,13 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
       M = M + (cross(cpos - rx, f) + mint);
     }
   }
-  if (MESH) mesh_add_at(a.mt, midx, F, M);
+  if (MESH && mwave) mesh_add_at(a.mt, midx, F, M);
   if (FUSED && INTEG) {
-    const V3 al = angular_accel_d(Q4f{me.q.x, me.q.y, me.q.z, me.q.w}, M, wx, V3{s.ix[x], s.iy[x], s.iz[x]});
+    const V3 al = angular_accel_d(Q4f{me.q.x, me.q.y, me.q.z, me.q.w}, M, wx, inert);
     Core nx = me;
     kick_prolog<true>(nx.xr, nx.vm, nx.w, nx.q, F.x / mx, F.y / mx, F.z / mx, al.x, al.y, al.z, a.dt, a.kdt_2);
     a.core_out[x] = nx;
-    const double dx = nx.xr.x - s.bx[x], dy = nx.xr.y - s.by[x], dz = nx.xr.z - s.bz[x];
+    const double dx = nx.xr.x - bref.x, dy = nx.xr.y - bref.y, dz = nx.xr.z - bref.z;
     if (dx * dx + dy * dy + dz * dz >= a.thr2) a.flags[a.trig_out] = 1;
   } else if (FUSED) {
     s.ax[x] = F.x / mx;

@@ -98,7 +98,7 @@ struct ForceArgs32 {
   double kdt_2;
   double thr2;
   int32_t trig_out;
-  // terms of the triangle-mesh contacts, evaluated by the mesh pass just before (nsph = 0: none)
+  // terms of the triangle-mesh contacts, evaluated by the mesh pass just before (ncand = 0: none)
   MeshTerms mt;
 };
 
@@ -175,7 +175,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   __syncthreads();
   const int64_t x = b0 + threadIdx.x;
   if (x >= a.n) return;
-  const int32_t midx = MESH && a.mt.nsph ? a.mt.idx[x] : -1;
+  // mesh terms: only waves of 64 particles that have candidates read their particles' index (one
+  // scalar load per wave)
+  const bool mwave = MESH && a.mt.ncand && a.mt.wcnt[__builtin_amdgcn_readfirstlane((int)(x >> 6))] != 0u;
+  const uint2 midx = mwave ? a.mt.idx[x] : uint2{0u, 0u};
   const PState& s = a.s;
   uint32_t* __restrict__ word = a.nl.word;
   const uint32_t cnt = a.nl.cnt[x];
@@ -289,6 +292,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   }
   F = F + V3{cval(Fx), cval(Fy), cval(Fz)};
   M = M + V3{cval(Mx), cval(My), cval(Mz)};
+  // the integration's own inputs (inertia, rebuild reference), loaded before the walls and the mesh terms
+  // so that their latency overlaps with those
+  V3 inert{0.0, 0.0, 0.0}, bref{0.0, 0.0, 0.0};
+  if (FUSED && INTEG) {
+    inert = {s.ix[x], s.iy[x], s.iz[x]};
+    bref = {s.bx[x], s.by[x], s.bz[x]};
+  }
   if (WALLS) {
     const V3 vx64{me.vm.x, me.vm.y, me.vm.z}, wx64{me.w.x, me.w.y, me.w.z};
     const uint32_t mask = a.nw ? a.wl.mask[x] : 0u;
@@ -316,14 +326,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       M = M + (cross(cpos - rx, f) + mint);
     }
   }
-  if (MESH) mesh_add_at(a.mt, midx, F, M);
+  if (MESH && mwave) mesh_add_at(a.mt, midx, F, M);
   if (FUSED && INTEG) {
-    const V3 al = angular_accel_g(Q4g{me.q.x, me.q.y, me.q.z, me.q.w}, M, V3{me.w.x, me.w.y, me.w.z},
-                                  V3{s.ix[x], s.iy[x], s.iz[x]});
+    const V3 al = angular_accel_g(Q4g{me.q.x, me.q.y, me.q.z, me.q.w}, M, V3{me.w.x, me.w.y, me.w.z}, inert);
     Core nx = me;
     kick_prolog<true>(nx.xr, nx.vm, nx.w, nx.q, F.x / mx, F.y / mx, F.z / mx, al.x, al.y, al.z, a.dt, a.kdt_2);
     a.core_out[x] = nx;
-    const double dx = nx.xr.x - s.bx[x], dy = nx.xr.y - s.by[x], dz = nx.xr.z - s.bz[x];
+    const double dx = nx.xr.x - bref.x, dy = nx.xr.y - bref.y, dz = nx.xr.z - bref.z;
     if (dx * dx + dy * dy + dz * dz >= a.thr2) a.flags[a.trig_out] = 1;
   } else if (FUSED) {
     s.ax[x] = F.x / mx;

@@ -162,16 +162,21 @@ int do_move_particles(xdem_ctx* c) {
 }
 
 // ---------------------------------------------------------------- neighbour lists
-// Candidate lists in one pass, one wavefront per cell (cells in Morton rank order). All particles of a
-// cell share the 27-cell stencil: lanes 0..26 look up the stencil cells in stream order (dz, dy, dx in
-// -1..1, dx fastest; periodic images wrap with their image code) and a wavefront prefix sum numbers the
-// candidates t = 0 .. T-1 in that order (stencil cell, then cell order). For each particle x of the cell,
-// 64 candidates at a time are tested with the DEM filter of chunk_neighbors_contact.cpp:45-57
-// (0 < d^2 < (r_x + r_y + rcut_inc)^2 and d^2 < nbh_dist^2, strict), and a ballot + prefix count writes the
-// accepted ones to x's row of `tmp` in candidate order, which is the stream order. Rows hold at most
-// `cap` entries; the count is exact, so a longer row is detected and the pass repeated with a larger cap.
+// Candidate lists in one pass, one wavefront per row segment of kRowCells cells along x (same y and z).
+// Each particle's candidates come from its 27-cell stencil, in stream order: stencil cell (dz, dy, dx in
+// -1..1, dx fastest; periodic images wrap with their image code), then cell order. The segment's cells
+// share one stencil of 9 (kRowCells + 2) cells: lane l looks up cell (dz, dy, dxx) with l = ((dz + 1) 3
+// + dy + 1) (kRowCells + 2) + dxx + 1, and a wavefront prefix sum numbers the candidates t = 0 .. T-1 in
+// lane order. For a particle in column q of the segment, the candidates of lanes with dxx in q-1..q+1 are
+// exactly its stencil, and in its stream order. Its particles are tested 64 candidates at a time with
+// the DEM filter of chunk_neighbors_contact.cpp:45-57 (0 < d^2 < (r_x + r_y + rcut_inc)^2 and
+// d^2 < nbh_dist^2, strict), and a ballot + prefix count writes the accepted ones to x's row of `tmp` in
+// candidate order. Sharing the stencil lookups and the dependent loads over a segment cuts the
+// per-wavefront latency chains per cell by kRowCells. Rows hold at most `cap` entries; the count is exact,
+// so a longer row is detected and the pass repeated with a larger cap.
 // Entry: partner index (Morton order) | image code << 25 | owner << 31 (owner: lower id, or a periodic
 // image, as nbh_sphere's sym mode keeps it; polyhedra: lower id always, nbh_polyhedron.cpp:234-236).
+constexpr int kRowCells = 3;
 struct CellNbhArgs {
   Grid g;
   double rcut_inc;
@@ -215,62 +220,93 @@ __device__ inline uint64_t bcast_u64(uint64_t v, int j) {
   return ((uint64_t)bcast_u32((uint32_t)(v >> 32), j) << 32) | bcast_u32((uint32_t)v, j);
 }
 
-// Each chunk of 64 candidates is loaded once and tested against every particle of the cell (up to 64 at
-// a time, lane j holding particle j's position and running count), so a candidate record is read once
-// per cell rather than once per particle.
-__global__ __launch_bounds__(256) void k_nbh_cells(CellNbhArgs a) {
+// Each chunk of 64 candidates is loaded once and tested against up to 64 particles of the segment (lane j
+// holding particle j's position and running count), so a candidate record is read once per segment.
+__global__ __launch_bounds__(256) void k_nbh_rows(CellNbhArgs a) {
+  constexpr int NC = kRowCells + 2;  // stencil columns dxx = -1 .. kRowCells
+  constexpr int NS = 9 * NC;         // stencil lanes
+  static_assert(NS <= 64, "one stencil cell per lane");
   const int lane = (int)threadIdx.x & 63;
-  const int64_t c = (int64_t)blockIdx.x * 4 + ((int)threadIdx.x >> 6);
-  if (c >= a.ncells) return;
-  const uint32_t xb = a.cell_start[c], xe = a.cell_start[c + 1];
-  if (xb == xe) return;
   const Grid& g = a.g;
-  const uint32_t li = a.lin[c];
-  const int cx = (int)(li % (uint32_t)g.dims[0]);
-  const int cy = (int)((li / (uint32_t)g.dims[0]) % (uint32_t)g.dims[1]);
-  const int cz = (int)(li / ((uint32_t)g.dims[0] * (uint32_t)g.dims[1]));
+  const int nxb = (g.dims[0] + kRowCells - 1) / kRowCells;
+  const int64_t w = (int64_t)blockIdx.x * 4 + ((int)threadIdx.x >> 6);
+  if (w >= (int64_t)nxb * g.dims[1] * g.dims[2]) return;
+  const int cy = (int)((w / nxb) % g.dims[1]);
+  const int cz = (int)(w / ((int64_t)nxb * g.dims[1]));
+  const int x0 = (int)(w % nxb) * kRowCells;
+  const int nq = min(kRowCells, g.dims[0] - x0);  // columns of the segment
+  // the segment's own cells: lane q < nq holds column q's particle range
+  uint32_t ob = 0, on = 0;
+  if (lane < nq) {
+    const uint32_t r = a.rank[((int64_t)cz * g.dims[1] + cy) * g.dims[0] + x0 + lane];
+    ob = a.cell_start[r];
+    on = a.cell_start[r + 1] - ob;
+  }
+  uint32_t cbeg[kRowCells], cend[kRowCells];  // wavefront-uniform: first particle, end of the prefix
+  uint32_t own = 0;
+#pragma unroll
+  for (int q = 0; q < kRowCells; q++) {
+    cbeg[q] = bcast_u32(ob, q);
+    own += bcast_u32(on, q);
+    cend[q] = own;
+  }
+  if (own == 0) return;
   uint32_t sstart = 0, scnt = 0;
-  int scode = kNoShift;
-  if (lane < 27) {
+  int scode = kNoShift, scol = -2;
+  if (lane < NS) {
+    const int dz = lane / (3 * NC) - 1, dy = (lane / NC) % 3 - 1, dxx = lane % NC - 1;
     int nx, ny, nz, sx, sy, sz;
-    if (stencil_cell(g, cz, lane / 9 - 1, 2, nz, sz) && stencil_cell(g, cy, (lane / 3) % 3 - 1, 1, ny, sy) &&
-        stencil_cell(g, cx, lane % 3 - 1, 0, nx, sx)) {
+    // columns past the segment's last neighbour column stay empty (x0 + dxx <= dims: one wrap at most)
+    if (dxx <= nq && stencil_cell(g, cz, dz, 2, nz, sz) && stencil_cell(g, cy, dy, 1, ny, sy) &&
+        stencil_cell(g, x0, dxx, 0, nx, sx)) {
       const uint32_t cb = a.rank[((int64_t)nz * g.dims[1] + ny) * g.dims[0] + nx];
       sstart = a.cell_start[cb];
       scnt = a.cell_start[cb + 1] - sstart;
       scode = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+      scol = dxx;
     }
   }
-  uint32_t inc = scnt;  // inclusive prefix over the stencil lanes (lanes 27..31 then hold the total)
-  for (int o = 1; o < 32; o <<= 1) {
+  uint32_t inc = scnt;  // inclusive prefix over the stencil lanes (lanes NS..63 then hold the total)
+  for (int o = 1; o < 64; o <<= 1) {
     const uint32_t y = __shfl_up(inc, o, 64);
     if (lane >= o) inc += y;
   }
-  const uint32_t total = __shfl(inc, 26, 64);
+  const uint32_t total = __shfl(inc, NS - 1, 64);
   const uint64_t below = (1ull << lane) - 1ull;
-  for (uint32_t g0 = xb; g0 < xe; g0 += 64) {
-    const int ng = (int)min(64u, xe - g0);
+  for (uint32_t g0 = 0; g0 < own; g0 += 64) {
+    const int ng = (int)min(64u, own - g0);
+    // lane j: particle g0 + j of the segment, its column and index
+    const uint32_t tj = g0 + (uint32_t)lane;
+    int myq = 0;
+    uint32_t myx = cbeg[0] + tj;
+#pragma unroll
+    for (int q = 1; q < kRowCells; q++)
+      if (tj >= cend[q - 1]) {
+        myq = q;
+        myx = cbeg[q] + (tj - cend[q - 1]);
+      }
     d4 mine{0, 0, 0, 0};
     uint64_t myid = 0;
     if (lane < ng) {
-      mine = a.xr[g0 + lane];
-      myid = a.id[g0 + lane];
+      mine = a.xr[myx];
+      myid = a.id[myx];
     }
     uint32_t run = 0;  // lane j: entries of particle g0 + j so far
     for (uint32_t t0 = 0; t0 < total; t0 += 64) {
       const uint32_t t = t0 + (uint32_t)lane;
-      // stencil cell holding candidate t: the first lane whose inclusive count exceeds t (branch-free
-      // binary search over lanes 0..31, every lane taking part)
+      // stencil lane holding candidate t: the first lane whose inclusive count exceeds t (branch-free
+      // binary search over lanes 0..63, every lane taking part)
       int pos = 0;
 #pragma unroll
-      for (int step = 16; step >= 1; step >>= 1) {
+      for (int step = 32; step >= 1; step >>= 1) {
         const uint32_t v = __shfl(inc, pos + step - 1, 64);
         pos += v <= t ? step : 0;
       }
-      const int sc = pos < 27 ? pos : 26;
+      const int sc = pos < NS ? pos : NS - 1;
       const uint32_t first = __shfl(inc, sc, 64) - __shfl(scnt, sc, 64);
       const uint32_t ystart = __shfl(sstart, sc, 64);
       const int code = __shfl(scode, sc, 64);
+      const int col = __shfl(scol, sc, 64);
       const bool valid = t < total;
       const uint32_t y = ystart + (t - first);
       d4 py{0, 0, 0, 0};
@@ -284,9 +320,10 @@ __global__ __launch_bounds__(256) void k_nbh_cells(CellNbhArgs a) {
       for (int j = 0; j < ng; j++) {
         const double pxx = bcast_f64(mine.x, j), pxy = bcast_f64(mine.y, j), pxz = bcast_f64(mine.z, j);
         const double pxw = bcast_f64(mine.w, j);
-        const uint32_t x = g0 + (uint32_t)j;
+        const uint32_t x = bcast_u32(myx, j);
+        const int qj = (int)bcast_u32((uint32_t)myq, j);
         bool acc = false;
-        if (valid && !(y == x && code == kNoShift)) {
+        if (valid && col >= qj - 1 && col <= qj + 1 && !(y == x && code == kNoShift)) {
           const V3 d{pxx - img.x, pxy - img.y, pxz - img.z};
           const double d2 = dot(d, d);
           const double rmax = pxw + py.w + a.rcut_inc;
@@ -306,7 +343,7 @@ __global__ __launch_bounds__(256) void k_nbh_cells(CellNbhArgs a) {
         }
       }
     }
-    if (lane < ng) a.cnt[g0 + lane] = run;
+    if (lane < ng) a.cnt[myx] = run;
   }
 }
 
@@ -467,10 +504,14 @@ int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist) {
   if (n > 0 && nbh_dist <= 0) {
     // dem_rcut_max (src/field/dem_rcut_max.cpp:55-65): the largest radius, reduced by move_particles'
     // binning pass; nbh_dist = rcut_max + rcut_inc
-    uint64_t hb = 0;
-    if ((rc = read_u64(c, c->d_rmax, &hb))) return rc;
-    double rmax;
-    std::memcpy(&rmax, &hb, sizeof(double));
+    // binning pass; nbh_dist = rcut_max + rcut_inc. Radii change only at upload, which keeps their
+    // maximum on the host, so the reduction is read back only without it.
+    double rmax = c->rmax_host;
+    if (rmax < 0) {
+      uint64_t hb = 0;
+      if ((rc = read_u64(c, c->d_rmax, &hb))) return rc;
+      std::memcpy(&rmax, &hb, sizeof(double));
+    }
     nbh_dist = 2.0 * rmax + rcut_inc;
   }
   // check_rcut (src/field/check_rcut.cpp:67-118): the one-layer stencil needs cells >= nbh_dist
@@ -502,8 +543,9 @@ int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist) {
   }
   CellNbhArgs a{g, rcut_inc, nbh_dist * nbh_dist, c->cell_start, c->cell_rank, c->cell_lin, c->mxr, c->mid,
                 g.ncells, c->nshapes > 0 ? 1 : 0, c->nbh_tmp, c->nbh_tmp_cap, cnt_m};
-  const dim3 cgrid((uint32_t)((g.ncells + 3) / 4));
-  hipLaunchKernelGGL(k_nbh_cells, cgrid, dim3(256), 0, c->stream, a);
+  const int64_t nseg = (int64_t)((g.dims[0] + kRowCells - 1) / kRowCells) * g.dims[1] * g.dims[2];
+  const dim3 cgrid((uint32_t)((nseg + 3) / 4));
+  hipLaunchKernelGGL(k_nbh_rows, cgrid, dim3(256), 0, c->stream, a);
   hipLaunchKernelGGL(k_balance, dim3(grid_blocks(n, kBalance)), dim3(64), 0, c->stream, n, cnt_m, perm2, inv2);
   hipLaunchKernelGGL(k_balance_apply, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, perm2, cnt_m, L.cnt,
                      L.kmax, c->perm, perm_tmp);
@@ -522,7 +564,7 @@ int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist) {
     c->nbh_tmp_n = c->cap;
     a.tmp = c->nbh_tmp;
     a.cap = rows;
-    hipLaunchKernelGGL(k_nbh_cells, cgrid, dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_nbh_rows, cgrid, dim3(256), 0, c->stream, a);
   }
   if ((int64_t)slots > L.cap_slots) {
     const int64_t cap = (int64_t)(slots * 1.25) + 4096;
@@ -536,7 +578,7 @@ int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist) {
     L.cap_slots = cap;
   }
   L.slots = (int64_t)slots;
-  unsigned long long* pairs = (unsigned long long*)c->d_u32tmp;
+  unsigned long long* pairs = (unsigned long long*)(c->d_pairs + nxt);
   const int64_t nblk = grid_blocks(n, 256);
   if (nblk > c->cap_blk) {
     if ((rc = grow(c, &c->blk_u32, nblk))) return rc;
@@ -553,9 +595,7 @@ int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist) {
   XDEM_HIP(c, hipGetLastError());
   c->cur = 1 - c->cur;
   c->cells_valid = false;  // cell ranges refer to the Morton order
-  uint64_t hp = 0;
-  if ((rc = read_u64(c, (const uint64_t*)pairs, &hp))) return rc;
-  L.pairs = (int64_t)hp;
+  L.pairs = -1;  // on the device until asked for (nl_pairs)
   c->nl_cur = nxt;
   c->nl_valid = true;
   c->hist_ready = false;
@@ -590,10 +630,13 @@ __global__ __launch_bounds__(256) void k_invert(int64_t n, const uint32_t* __res
   if (x < n) inv[perm[x]] = (uint32_t)x;
 }
 
-// slot of the entry of particle x with partner y in list l, or ~0ull
-__device__ inline uint64_t find_entry(const NbrList& l, uint32_t x, uint32_t y) {
+// slot of the entry of particle x with partner y in list l, or ~0ull. kh: a rank to try first (the
+// entry's rank in the previous list, right when nothing moved in front of it); only when a partner
+// appears at most once per list (hint_ok), since the first match is the one wanted.
+__device__ inline uint64_t find_entry(const NbrList& l, uint32_t x, uint32_t y, uint32_t kh = ~0u) {
   const uint32_t cnt = l.cnt[x];
   const uint64_t b = l.base[x >> 6] + (uint64_t)(x & 63);
+  if (kh < cnt && (l.word[b + (uint64_t)kh * kWave] & kIdxMask) == y) return b + (uint64_t)kh * kWave;
   for (uint32_t k = 0; k < cnt; k += 4) {  // four words in flight per step
     uint32_t w[4];
 #pragma unroll
@@ -608,7 +651,7 @@ __device__ inline uint64_t find_entry(const NbrList& l, uint32_t x, uint32_t y) 
 // One thread per slot of the previous list; only owner entries with history do work.
 template <typename T, bool KR>
 __global__ __launch_bounds__(256) void k_hist_scatter(int64_t oslots, NbrList nl, NbrList ol,
-                                                      const uint32_t* __restrict__ inv) {
+                                                      const uint32_t* __restrict__ inv, int hint_ok) {
   const int64_t s2 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s2 >= oslots) return;
   const uint32_t ox = ol.sx[s2];
@@ -625,7 +668,9 @@ __global__ __launch_bounds__(256) void k_hist_scatter(int64_t oslots, NbrList nl
     mz = oh.mz[s2];
   }
   // moment histories are meaningful only once a rolling stiffness is used (use_kr clears them then)
-  const uint64_t t1 = find_entry(nl, nx, ny);
+  // rank of this entry in ox's previous list (slot = base + lane + rank * 64)
+  const uint32_t kh = hint_ok ? (uint32_t)((uint64_t)(s2 - (int64_t)ol.base[ox >> 6] - (int64_t)(ox & 63)) / kWave) : ~0u;
+  const uint64_t t1 = find_entry(nl, nx, ny, kh);
   if (t1 != ~0ull && (nl.word[t1] & kOwnerBit)) {
     nh.x[t1] = hx; nh.y[t1] = hy; nh.z[t1] = hz;
     if (KR) {
@@ -674,6 +719,18 @@ __global__ __launch_bounds__(256) void k_wall_carry(int64_t n, int nw, const Sur
   nwl.mask[x] = mask;
 }
 
+int nl_pairs(xdem_ctx* c, int64_t* out) {
+  NbrList& L = c->nl[c->nl_cur];
+  if (L.pairs < 0) {
+    uint64_t hp = 0;
+    const int rc = read_u64(c, c->d_pairs + c->nl_cur, &hp);
+    if (rc) return rc;
+    L.pairs = (int64_t)hp;
+  }
+  *out = L.pairs;
+  return XDEM_OK;
+}
+
 int do_nbh_sphere(xdem_ctx* c, double rcut_inc) {
   if (!c->nl_valid) return set_err(c, XDEM_ERR_STATE, "nbh_sphere: run chunk_neighbors_contact first");
   const int64_t n = c->n;
@@ -690,13 +747,17 @@ int do_nbh_sphere(xdem_ctx* c, double rcut_inc) {
     const int64_t os = ol.slots;
     if (os > 0) {
       const dim3 gs(grid_blocks(os, 256)), bs(256);
+      // a partner appears at most once per list unless a periodic direction has fewer than 3 cells
+      int hint_ok = 1;
+      for (int d = 0; d < 3; d++)
+        if (c->grid.per[d] && c->grid.dims[d] < 3) hint_ok = 0;
       // moment histories are zero (and not carried) until a rolling stiffness is used
       if (c->hist_fp32) {
-        if (c->kr_used) hipLaunchKernelGGL((k_hist_scatter<float, true>), gs, bs, 0, c->stream, os, nl, ol, inv);
-        else hipLaunchKernelGGL((k_hist_scatter<float, false>), gs, bs, 0, c->stream, os, nl, ol, inv);
+        if (c->kr_used) hipLaunchKernelGGL((k_hist_scatter<float, true>), gs, bs, 0, c->stream, os, nl, ol, inv, hint_ok);
+        else hipLaunchKernelGGL((k_hist_scatter<float, false>), gs, bs, 0, c->stream, os, nl, ol, inv, hint_ok);
       } else {
-        if (c->kr_used) hipLaunchKernelGGL((k_hist_scatter<double, true>), gs, bs, 0, c->stream, os, nl, ol, inv);
-        else hipLaunchKernelGGL((k_hist_scatter<double, false>), gs, bs, 0, c->stream, os, nl, ol, inv);
+        if (c->kr_used) hipLaunchKernelGGL((k_hist_scatter<double, true>), gs, bs, 0, c->stream, os, nl, ol, inv, hint_ok);
+        else hipLaunchKernelGGL((k_hist_scatter<double, false>), gs, bs, 0, c->stream, os, nl, ol, inv, hint_ok);
       }
       XDEM_HIP(c, hipGetLastError());
     }
@@ -755,7 +816,10 @@ extern "C" int xdem_interactions_count(xdem_ctx* c, int64_t* n_pp, int64_t* n_pw
   if (n_pp) *n_pp = 0;
   if (n_pw) *n_pw = 0;
   if (!c->nl_valid || c->n == 0) return XDEM_OK;
-  if (n_pp) *n_pp = c->nl[c->nl_cur].pairs;
+  if (n_pp) {
+    const int rc = nl_pairs(c, n_pp);
+    if (rc) return rc;
+  }
   if (n_pw && c->nwalls > 0 && c->wl_valid) {
     std::vector<uint8_t> m(c->n);
     XDEM_HIP(c, hipMemcpyAsync(m.data(), c->wl[c->wl_cur].mask, c->n, hipMemcpyDeviceToHost, c->stream));

@@ -94,13 +94,31 @@ __device__ inline bool mesh_filter(const MeshView& m, uint32_t wd, V3 p, double 
   return mesh_vf_core(p, t, R + m.rm + rv, dist, n, early);
 }
 
-// Contact of feature word wd with the sphere (p, R): overlap dn, normal n (from the mesh to the sphere)
-// and contact point cp.
-__device__ inline bool mesh_detect(const MeshView& m, uint32_t wd, V3 p, double R, double& dn, V3& n, V3& cp) {
+// Geometry of feature word wd: kind 0 a vertex t[0], 1 an edge t[0] -> t[1], 2 a triangle t[0..2]
+// (unused points zero). Returns the kind.
+__device__ inline uint32_t mesh_feature(const MeshView& m, uint32_t wd, V3 (&t)[3]) {
   const uint32_t kind = wd >> kMeshKindShift, k = wd & kMeshIdxMask;
+  t[1] = V3{0.0, 0.0, 0.0};
+  t[2] = V3{0.0, 0.0, 0.0};
   if (kind == 0) {
-    const double Rs = R + m.rm;
-    const V3 dist = p - m.v[k];
+    t[0] = m.v[k];
+  } else if (kind == 1) {
+    const uint2 e = m.e[k];
+    t[0] = m.v[e.x];
+    t[1] = m.v[e.y];
+  } else {
+    mesh_tri(m, k, t);
+  }
+  return kind;
+}
+
+// Contact of a mesh feature (kind, geometry t as mesh_feature gives it, Minkowski radius rm) with the
+// sphere (p, R): overlap dn, normal n (from the mesh to the sphere) and contact point cp.
+__device__ inline bool mesh_detect_t(uint32_t kind, const V3 (&t)[3], double rm, V3 p, double R, double& dn, V3& n,
+                                     V3& cp) {
+  if (kind == 0) {
+    const double Rs = R + rm;
+    const V3 dist = p - t[0];
     const double dist_norm = norm(dist);
     const double inv = 1.0 / dist_norm;
     dn = dist_norm - Rs;
@@ -109,46 +127,50 @@ __device__ inline bool mesh_detect(const MeshView& m, uint32_t wd, V3 p, double 
     return dn < 0;
   }
   if (kind == 1) {
-    const uint2 e = m.e[k];
-    const V3 vf = m.v[e.x], vs = m.v[e.y];
+    const V3 vf = t[0], vs = t[1];
     const V3 fs = vs - vf, fi = p - vf;
     const double r = dot(fs, fi) / dot(fs, fs);
     if (r <= 0 || r >= 1.0) return false;
     V3 nn = fi - fs * r;
-    dn = norm(nn) - (R + m.rm);
+    dn = norm(nn) - (R + rm);
     if (dn > 0) return false;
     n = mesh_unit(nn);
     cp = p - n * (R + 0.5 * dn);
     return true;
   }
-  V3 t[3];
-  mesh_tri(m, k, t);
   double dist;
   bool early;
-  const bool in = mesh_vf_core(p, t, R + m.rm, dist, n, early);
+  const bool in = mesh_vf_core(p, t, R + rm, dist, n, early);
   if (early) return false;
-  dn = dist - (R + m.rm);
+  dn = dist - (R + rm);
   cp = p - n * (R + 0.5 * dn);
   return in;
+}
+
+// Contact of feature word wd with the sphere (p, R).
+__device__ inline bool mesh_detect(const MeshView& m, uint32_t wd, V3 p, double R, double& dn, V3& n, V3& cp) {
+  V3 t[3];
+  const uint32_t kind = mesh_feature(m, wd, t);
+  return mesh_detect_t(kind, t, m.rm, p, R, dn, n, cp);
 }
 
 // ContactLawRShapeDriverFunc (contact_sphere.hpp:298-396) over sphere x's mesh candidates in list order,
 // in FP64 (both force forms): the driver sits at the mesh centre, stationary; no contact resets the
 // history. Adds the forces and moments to F and M. cs (null: not recorded): contact state per slot,
 // kCStateLen doubles (dn, contact point, f - ft, ft, sphere centre, zeros; zeros without contact).
-// Candidate jc of the compact lists l (kind and index in l.word[jc]) of a sphere at rx (radius Rx, mass
+// Candidate jc of the compact lists l (its feature: kind and geometry tg) of a sphere at rx (radius Rx, mass
 // mx, velocity vx, spin wx): the contact law against the mesh (ContactLawDriverFunc,
 // contact_sphere.hpp:298-396) with its friction and rolling history at jc, and the terms it adds to the
 // sphere's force and moment written to out[0..5]; out of contact the history is cleared and the terms
 // are -0.0, which adds nothing to any value. cs: the recorded contact state of the candidate's list slot
 // j (null: not recorded).
 template <bool KR>
-__device__ inline void mesh_candidate(const MeshView& m, const MeshList& l, int64_t jc, int64_t j, V3 rx, double Rx,
-                                      double mx, V3 vx, V3 wx, double dt, const ContactParams& cpd,
-                                      double* __restrict__ out, double* cs) {
+__device__ inline void mesh_candidate(const MeshView& m, const MeshList& l, int64_t jc, int64_t j, uint32_t kind,
+                                      const V3 (&tg)[3], V3 rx, double Rx, double mx, V3 vx, V3 wx, double dt,
+                                      const ContactParams& cpd, double* __restrict__ out, double* cs) {
   double dn;
   V3 nrm, cpos;
-  if (!mesh_detect(m, l.word[jc], rx, Rx, dn, nrm, cpos)) {
+  if (!mesh_detect_t(kind, tg, m.rm, rx, Rx, dn, nrm, cpos)) {
     l.hx[jc] = 0.0; l.hy[jc] = 0.0; l.hz[jc] = 0.0;
     l.mhx[jc] = 0.0; l.mhy[jc] = 0.0; l.mhz[jc] = 0.0;
     if (cs) {
@@ -184,22 +206,58 @@ __device__ inline void mesh_candidate(const MeshView& m, const MeshList& l, int6
   reinterpret_cast<double2*>(out)[2] = double2{mt.y, mt.z};
 }
 
-// The mesh terms of particle x (index i among the particles with candidates, -1: none) added to its
-// force and moment in candidate order, after the walls (the same sums, in the same order, as evaluating
-// each candidate here).
-__device__ inline void mesh_add_at(const MeshTerms& t, int32_t i, V3& F, V3& M) {
-  if (i < 0) return;
-  const uint32_t c = t.scnt[i];
-  // loads of four candidates in flight together, then their sums in order
+// The mesh terms of a particle (mi = its first flat candidate and count) added to its force and moment
+// in candidate order, after the walls (the same sums, in the same order, as evaluating each candidate
+// here). A particle's terms are contiguous, 48 bytes per candidate.
+__device__ inline void mesh_add_at(const MeshTerms& t, uint2 mi, V3& F, V3& M) {
+  const uint32_t c = mi.y;
+  const double2* p = reinterpret_cast<const double2*>(t.terms) + (int64_t)mi.x * 3;
+  // batches of four candidates, the loads of the next batch in flight while the current one is summed
+  // (in order)
+  double2 r[4][3] = {}, nx[4][3] = {};
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    if (q < (int)c) {
+      r[q][0] = p[q * 3];
+      r[q][1] = p[q * 3 + 1];
+      r[q][2] = p[q * 3 + 2];
+    }
   for (uint32_t k0 = 0; k0 < c; k0 += 4) {
-    double2 r[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (k0 + 4 + q < c) {
+        nx[q][0] = p[(k0 + 4 + q) * 3];
+        nx[q][1] = p[(k0 + 4 + q) * 3 + 1];
+        nx[q][2] = p[(k0 + 4 + q) * 3 + 2];
+      }
 #pragma unroll
     for (int q = 0; q < 4; q++)
       if (k0 + q < c) {
-        const double2* p = reinterpret_cast<const double2*>(t.terms + ((int64_t)(k0 + q) * t.nsph + i) * 6);
-        r[q][0] = p[0];
-        r[q][1] = p[1];
-        r[q][2] = p[2];
+        F = F + V3{r[q][0].x, r[q][0].y, r[q][1].x};
+        M = M + V3{r[q][1].y, r[q][2].x, r[q][2].y};
+      }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      r[q][0] = nx[q][0];
+      r[q][1] = nx[q][1];
+      r[q][2] = nx[q][2];
+    }
+  }
+}
+// The same sums with four loads in flight per batch and no pipelining across batches (fewer registers:
+// the polyhedra gather, where the pipelined form lowers occupancy).
+__device__ inline void mesh_add(const MeshTerms& t, int64_t x, V3& F, V3& M) {
+  const uint2 mi = t.idx[x];
+  const uint32_t c = mi.y;
+  const double2* p = reinterpret_cast<const double2*>(t.terms) + (int64_t)mi.x * 3;
+  for (uint32_t k0 = 0; k0 < c; k0 += 4) {
+    double2 r[4][3] = {};
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (k0 + q < c) {
+        r[q][0] = p[(k0 + q) * 3];
+        r[q][1] = p[(k0 + q) * 3 + 1];
+        r[q][2] = p[(k0 + q) * 3 + 2];
       }
 #pragma unroll
     for (int q = 0; q < 4; q++)
@@ -209,6 +267,5 @@ __device__ inline void mesh_add_at(const MeshTerms& t, int32_t i, V3& F, V3& M) 
       }
   }
 }
-__device__ inline void mesh_add(const MeshTerms& t, int64_t x, V3& F, V3& M) { mesh_add_at(t, t.idx[x], F, M); }
 
 }  // namespace xdem

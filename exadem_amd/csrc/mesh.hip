@@ -53,19 +53,19 @@ void mesh_free(xdem_ctx* c) {
   if (c->mstate) (void)hipFree(c->mstate);
   c->mstate = nullptr;
   c->cap_mstate = 0;
-  for (void* p : {(void*)c->mesh_sph, (void*)c->mesh_scnt, (void*)c->mesh_idx, (void*)c->mesh_terms})
+  for (void* p : {(void*)c->mesh_cand, (void*)c->mesh_idx, (void*)c->mesh_terms, (void*)c->mesh_geo,
+                  (void*)c->mesh_wcnt})
     if (p) (void)hipFree(p);
-  c->mesh_sph = nullptr;
-  c->mesh_scnt = nullptr;
+  c->mesh_wcnt = nullptr;
+  c->mesh_geo = nullptr;
+  c->mesh_cand = nullptr;
   c->mesh_idx = nullptr;
   c->mesh_terms = nullptr;
-  c->cap_mesh_terms = 0;
-  c->mesh_nsph = 0;
-  c->mesh_kmax = 0;
+  c->cap_mesh_cand = 0;
+  c->mesh_ncand = 0;
   free_ml(c->ml[0]);
   free_ml(c->ml[1]);
   free_ml(c->mc);
-  c->cap_mc = 0;
   c->mc_valid = c->mc_dirty = false;
   c->ml_alloc = 0;
   c->ml_valid = false;
@@ -94,9 +94,8 @@ static int alloc_ml(xdem_ctx* c) {
     if ((rc = dev_realloc(c, &l.mhy, m))) return rc;
     if ((rc = dev_realloc(c, &l.mhz, m))) return rc;
   }
-  if ((rc = dev_realloc(c, &c->mesh_sph, c->cap))) return rc;
-  if ((rc = dev_realloc(c, &c->mesh_scnt, c->cap))) return rc;
   if ((rc = dev_realloc(c, &c->mesh_idx, c->cap))) return rc;
+  if ((rc = dev_realloc(c, &c->mesh_wcnt, (c->cap + 63) / 64))) return rc;
   c->ml_alloc = c->cap;
   c->mesh.stride = c->cap;
   c->ml_valid = false;  // the lists no longer match the particle capacity
@@ -205,7 +204,15 @@ __global__ __launch_bounds__(256) void k_max_radius(int64_t n, const Core* __res
     const unsigned long long t = __shfl_down(b, o, 64);
     b = t > b ? t : b;
   }
-  if ((threadIdx.x & 63) == 0 && b) atomicMax(out, b);
+  // one atomic per block: per-wavefront atomics on the one address serialise (190 us at 1M)
+  __shared__ unsigned long long wmax[4];
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = b;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = wmax[0];
+    for (int w = 1; w < 4; w++) m = wmax[w] > m ? wmax[w] : m;
+    if (m) atomicMax(out, m);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_mesh_carry(int64_t n, PState s, MeshView m, double rcut_inc, MeshList nl,
@@ -249,68 +256,87 @@ __global__ __launch_bounds__(256) void k_mesh_carry(int64_t n, PState s, MeshVie
   if (cnt > (uint32_t)m.cap) atomicMax(over, (int32_t)cnt);
 }
 
-// The spheres that have mesh candidates, compacted per wavefront (ballot; one atomic per wavefront): their
-// index (idx, -1 for the others), counts, the number of them (ctr[0]) and the largest count (ctr[1]).
-// Their order only places the terms of the mesh pass; no sum depends on it.
+// Flat candidate index: each particle's candidates get a contiguous run of flat indices, runs placed per
+// wavefront (a wavefront prefix sum of the counts, one atomic per wavefront on ctr[0], the total). idx[x]
+// = (first flat index, count), (0, 0) without candidates; wcnt[x >> 6]: the wave's candidates. The
+// placement of the runs only places the terms of the mesh pass; no sum depends on it.
 __global__ __launch_bounds__(256) void k_mesh_index(int64_t n, const uint32_t* __restrict__ cnt, uint32_t cap,
-                                                    uint32_t* __restrict__ sph, uint32_t* __restrict__ scnt,
-                                                    int32_t* __restrict__ idx, int32_t* __restrict__ ctr) {
+                                                    uint2* __restrict__ idx, uint32_t* __restrict__ wcnt,
+                                                    int32_t* __restrict__ ctr) {
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t c = x < n ? min(cnt[x], cap) : 0u;
-  const unsigned long long bal = __ballot(c > 0);
   const int lane = threadIdx.x & 63;
-  int32_t base = 0;
-  if (bal) {
-    if (lane == 0) base = atomicAdd(ctr, (int32_t)__popcll(bal));
+  uint32_t incl = c;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t v = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += v;
+  }
+  const uint32_t total = __shfl(incl, 63, 64);
+  if (lane == 0 && x < n) wcnt[x >> 6] = total;
+  uint32_t base = 0;
+  if (total) {
+    if (lane == 0) base = (uint32_t)atomicAdd(ctr, (int32_t)total);
     base = __shfl(base, 0, 64);
   }
-  uint32_t cm = c;
-  for (int o = 32; o > 0; o >>= 1) cm = max(cm, (uint32_t)__shfl_down(cm, o, 64));
-  if (lane == 0 && cm) atomicMax(ctr + 1, (int32_t)cm);
   if (x >= n) return;
-  if (c > 0) {
-    const int32_t i = base + (int32_t)__popcll(bal & ((1ull << lane) - 1ull));
-    sph[i] = (uint32_t)x;
-    scnt[i] = c;
-    idx[x] = i;
-  } else {
-    idx[x] = -1;
-  }
+  idx[x] = c ? uint2{base + incl - c, c} : uint2{0u, 0u};
 }
 
-// One thread per (candidate k, sphere i): blockIdx.y = k, the sphere from blockIdx.x and the lane.
+// The flat candidates: cand[t] = (particle, rank), one thread per particle over its run.
+__global__ __launch_bounds__(256) void k_mesh_cand(int64_t n, const uint2* __restrict__ idx,
+                                                   uint2* __restrict__ cand) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  const uint2 r = idx[x];
+  for (uint32_t k = 0; k < r.y; k++) cand[r.x + k] = uint2{(uint32_t)x, k};
+}
+
+// One thread per flat candidate t.
 template <bool KR, bool GUARD>
 __global__ __launch_bounds__(256) void k_mesh_pass(MeshView m, MeshList l, MeshTerms t, const Core* __restrict__ core,
                                                    double dt, ContactParams cpd, const int32_t* __restrict__ flags,
                                                    int32_t trig_in, double* cs) {
   if (GUARD && (flags[1] || flags[trig_in])) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t k = blockIdx.y;
-  if (i >= t.nsph || k >= t.scnt[i]) return;
-  const int64_t x = t.sph[i];
+  const int64_t jc = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (jc >= t.ncand) return;
+  const uint2 ck = t.cand[jc];
+  const double2* gp = reinterpret_cast<const double2*>(t.geo + jc * 10);
+  const double2 g0 = gp[0], g1 = gp[1], g2 = gp[2], g3 = gp[3], g4 = gp[4];
+  const V3 tg[3] = {V3{g0.x, g0.y, g1.x}, V3{g1.y, g2.x, g2.y}, V3{g3.x, g3.y, g4.x}};
+  const int64_t x = ck.x;
   const Core me = core[x];
   const V3 rx{me.xr.x, me.xr.y, me.xr.z}, vx{me.vm.x, me.vm.y, me.vm.z}, wx{me.w.x, me.w.y, me.w.z};
-  const int64_t jc = (int64_t)k * t.nsph + i;
-  mesh_candidate<KR>(m, l, jc, (int64_t)k * m.stride + x, rx, me.xr.w, me.vm.w, vx, wx, dt, cpd, t.terms + jc * 6,
-                     cs);
+  mesh_candidate<KR>(m, l, jc, (int64_t)ck.y * m.stride + x, (uint32_t)g4.y, tg, rx, me.xr.w, me.vm.w, vx, wx, dt,
+                     cpd, t.terms + jc * 6, cs);
 }
 
-// Compact copy <-> lists, one thread per (candidate k = blockIdx.y, particle i): the gather copies the
-// words and histories, the write-back the histories.
-__global__ __launch_bounds__(256) void k_mesh_gather(MeshTerms t, MeshList l, MeshList lc, int64_t stride) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t k = blockIdx.y;
-  if (i >= t.nsph || k >= t.scnt[i]) return;
-  const int64_t j = (int64_t)k * stride + t.sph[i], jc = (int64_t)k * t.nsph + i;
-  lc.word[jc] = l.word[j];
+// Compact copy <-> lists, one thread per flat candidate: the gather copies the words and histories (and,
+// for spheres, geo non-null, caches the feature's geometry and kind), the write-back the histories.
+__global__ __launch_bounds__(256) void k_mesh_gather(MeshTerms t, MeshList l, MeshList lc, MeshView m, double* geo) {
+  const int64_t jc = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (jc >= t.ncand) return;
+  const uint2 ck = t.cand[jc];
+  const int64_t j = (int64_t)ck.y * m.stride + ck.x;
+  const uint32_t wd = l.word[j];
+  lc.word[jc] = wd;
+  if (geo) {
+    V3 tg[3];
+    const uint32_t kind = mesh_feature(m, wd, tg);
+    double2* gp = reinterpret_cast<double2*>(geo + jc * 10);
+    gp[0] = double2{tg[0].x, tg[0].y};
+    gp[1] = double2{tg[0].z, tg[1].x};
+    gp[2] = double2{tg[1].y, tg[1].z};
+    gp[3] = double2{tg[2].x, tg[2].y};
+    gp[4] = double2{tg[2].z, (double)kind};
+  }
   lc.hx[jc] = l.hx[j]; lc.hy[jc] = l.hy[j]; lc.hz[jc] = l.hz[j];
   lc.mhx[jc] = l.mhx[j]; lc.mhy[jc] = l.mhy[j]; lc.mhz[jc] = l.mhz[j];
 }
 __global__ __launch_bounds__(256) void k_mesh_writeback(MeshTerms t, MeshList l, MeshList lc, int64_t stride) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t k = blockIdx.y;
-  if (i >= t.nsph || k >= t.scnt[i]) return;
-  const int64_t j = (int64_t)k * stride + t.sph[i], jc = (int64_t)k * t.nsph + i;
+  const int64_t jc = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (jc >= t.ncand) return;
+  const uint2 ck = t.cand[jc];
+  const int64_t j = (int64_t)ck.y * stride + ck.x;
   l.hx[j] = lc.hx[jc]; l.hy[j] = lc.hy[jc]; l.hz[j] = lc.hz[jc];
   l.mhx[j] = lc.mhx[jc]; l.mhy[j] = lc.mhy[jc]; l.mhz[j] = lc.mhz[jc];
 }
@@ -318,10 +344,11 @@ __global__ __launch_bounds__(256) void k_mesh_writeback(MeshTerms t, MeshList l,
 static MeshTerms terms_view(xdem_ctx* c);
 
 int mesh_compact(xdem_ctx* c) {
-  if (c->mc_valid || !c->has_mesh || !c->ml_valid || c->mesh_nsph == 0) return XDEM_OK;
-  const dim3 grid(grid_blocks(c->mesh_nsph, 256), (unsigned)c->mesh_kmax), block(256);
-  hipLaunchKernelGGL(k_mesh_gather, grid, block, 0, c->stream, terms_view(c), c->ml[c->ml_cur], c->mc,
-                     c->mesh.stride);
+  if (c->mc_valid || !c->has_mesh || !c->ml_valid || c->mesh_ncand == 0) return XDEM_OK;
+  const dim3 grid(grid_blocks(c->mesh_ncand, 256)), block(256);
+  // the feature geometry cache serves the sphere pass (polyhedra words have another layout)
+  hipLaunchKernelGGL(k_mesh_gather, grid, block, 0, c->stream, terms_view(c), c->ml[c->ml_cur], c->mc, c->mesh,
+                     c->mesh_poly ? nullptr : c->mesh_geo);
   XDEM_HIP(c, hipGetLastError());
   c->mc_valid = true;
   c->mc_dirty = false;
@@ -331,8 +358,8 @@ int mesh_compact(xdem_ctx* c) {
 int mesh_sync(xdem_ctx* c) {
   if (!c->mc_dirty) return XDEM_OK;
   c->mc_dirty = false;
-  if (!c->has_mesh || !c->ml_valid || c->mesh_nsph == 0) return XDEM_OK;
-  const dim3 grid(grid_blocks(c->mesh_nsph, 256), (unsigned)c->mesh_kmax), block(256);
+  if (!c->has_mesh || !c->ml_valid || c->mesh_ncand == 0) return XDEM_OK;
+  const dim3 grid(grid_blocks(c->mesh_ncand, 256)), block(256);
   hipLaunchKernelGGL(k_mesh_writeback, grid, block, 0, c->stream, terms_view(c), c->ml[c->ml_cur], c->mc,
                      c->mesh.stride);
   XDEM_HIP(c, hipGetLastError());
@@ -341,23 +368,25 @@ int mesh_sync(xdem_ctx* c) {
 
 static MeshTerms terms_view(xdem_ctx* c) {
   MeshTerms t;
-  t.sph = c->mesh_sph;
-  t.scnt = c->mesh_scnt;
+  t.cand = c->mesh_cand;
   t.idx = c->mesh_idx;
+  t.wcnt = c->mesh_wcnt;
   t.terms = c->mesh_terms;
-  t.nsph = c->mesh_nsph;
-  t.kmax = c->mesh_kmax;
+  t.geo = c->mesh_geo;
+  t.ncand = c->mesh_ncand;
   return t;
 }
 
 int launch_mesh_pass(xdem_ctx* c, const ContactParams& cpd, double dt, bool guarded, int32_t trig_in, double* cs,
                      MeshTerms* t) {
   *t = MeshTerms{};
-  if (!c->has_mesh || !c->ml_valid || c->mesh_nsph == 0) return XDEM_OK;
+  if (!c->has_mesh || !c->ml_valid || c->mesh_ncand == 0) return XDEM_OK;
+  if (c->mesh_poly)
+    return set_err(c, XDEM_ERR_STATE, "mesh contacts of spheres: the mesh lists were built by nbh_polyhedron");
   int rc = mesh_compact(c);
   if (rc) return rc;
   *t = terms_view(c);
-  const dim3 grid(grid_blocks(c->mesh_nsph, 256), (unsigned)c->mesh_kmax), block(256);
+  const dim3 grid(grid_blocks(c->mesh_ncand, 256)), block(256);
   const MeshList& l = c->mc;
   c->mc_dirty = true;
   const Core* core = live(c).core;
@@ -415,30 +444,30 @@ int mesh_carry(xdem_ctx* c, double rcut_inc, bool have_old, bool poly) {
     XDEM_HIP(c, hipGetLastError());
   }
   hipLaunchKernelGGL(k_mesh_index, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, c->ml[nxt].cnt,
-                     (uint32_t)c->mesh_cap, c->mesh_sph, c->mesh_scnt, c->mesh_idx, d_over + 1);
+                     (uint32_t)c->mesh_cap, c->mesh_idx, c->mesh_wcnt, d_over + 1);
   XDEM_HIP(c, hipGetLastError());
-  int32_t over[3] = {0, 0, 0};  // overflowing count, spheres with candidates, largest count
+  int32_t over[2] = {0, 0};  // largest overflowing count, candidates
   XDEM_HIP(c, hipMemcpyAsync(over, d_over, sizeof(over), hipMemcpyDeviceToHost, c->stream));
   XDEM_HIP(c, hipStreamSynchronize(c->stream));
   c->ml_cur = nxt;
-  c->mesh_nsph = over[1];
-  c->mesh_kmax = over[2];
-  const int64_t nt = (int64_t)over[1] * over[2] * 6;
-  if (nt > c->cap_mesh_terms) {
-    const int64_t cap = nt + nt / 4 + 4096;
-    if ((rc = dev_realloc(c, &c->mesh_terms, cap))) return rc;
-    c->cap_mesh_terms = cap;
-  }
-  const int64_t ncomp = (int64_t)over[1] * over[2];
-  if (ncomp > c->cap_mc) {
-    const int64_t cap = ncomp + ncomp / 4 + 1024;
+  c->mesh_poly = poly;
+  c->mesh_ncand = over[1];
+  if (c->mesh_ncand > c->cap_mesh_cand) {
+    const int64_t cap = c->mesh_ncand + c->mesh_ncand / 4 + 1024;
     MeshList& l = c->mc;
     free_ml(l);
-    if ((rc = dev_realloc(c, &l.word, cap)) || (rc = dev_realloc(c, &l.hx, cap)) || (rc = dev_realloc(c, &l.hy, cap)) ||
+    if ((rc = dev_realloc(c, &c->mesh_cand, cap)) || (rc = dev_realloc(c, &c->mesh_terms, cap * 6)) ||
+        (rc = dev_realloc(c, &c->mesh_geo, cap * 10)) ||
+        (rc = dev_realloc(c, &l.word, cap)) || (rc = dev_realloc(c, &l.hx, cap)) || (rc = dev_realloc(c, &l.hy, cap)) ||
         (rc = dev_realloc(c, &l.hz, cap)) || (rc = dev_realloc(c, &l.mhx, cap)) || (rc = dev_realloc(c, &l.mhy, cap)) ||
         (rc = dev_realloc(c, &l.mhz, cap)))
       return rc;
-    c->cap_mc = cap;
+    c->cap_mesh_cand = cap;
+  }
+  if (c->mesh_ncand > 0) {
+    hipLaunchKernelGGL(k_mesh_cand, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, c->mesh_idx,
+                       c->mesh_cand);
+    XDEM_HIP(c, hipGetLastError());
   }
   if (over[0] > 0) {
     c->ml_valid = false;

@@ -336,6 +336,9 @@ __device__ inline void for_each_feature(const PolyNbhArgs& a, int64_t x, uint32_
   const Verts va = verts_of(a.vtx, x, V3{0, 0, 0});
   const Verts vb = verts_of(a.vtx, y, sh);
   const double rv = a.rv;
+  // the per-vertex box gates first (each vertex's box against the other particle's enlarged box), so
+  // that the two boxes are dead during the feature loops
+  uint32_t gate_a = 0, gate_b = 0;
   for (int i = 0; i < sa.nv; i++) {
     OBBd ov;
     ov.c = va[i];
@@ -344,7 +347,20 @@ __device__ inline void for_each_feature(const PolyNbhArgs& a, int64_t x, uint32_
     ov.e3 = {0, 0, 1};
     ov.ext = {0, 0, 0};
     obb_enlarge(ov, ra);
-    if (!obb_intersect(obb_j, ov)) continue;
+    if (obb_intersect(obb_j, ov)) gate_a |= 1u << i;
+  }
+  for (int j = 0; j < sb.nv; j++) {
+    OBBd ov;
+    ov.c = vb[j];
+    ov.e1 = {1, 0, 0};
+    ov.e2 = {0, 1, 0};
+    ov.e3 = {0, 0, 1};
+    ov.ext = {0, 0, 0};
+    obb_enlarge(ov, rb);
+    if (obb_intersect(obb_i, ov)) gate_b |= 1u << j;
+  }
+  for (int i = 0; i < sa.nv; i++) {
+    if (!((gate_a >> i) & 1u)) continue;
     for (int j = 0; j < sb.nv; j++)
       if (filter_vv(rv, va[i], ra, vb[j], rb)) fn(0, 0, i, j);
     for (int j = 0; j < sb.ne; j++)
@@ -362,14 +378,7 @@ __device__ inline void for_each_feature(const PolyNbhArgs& a, int64_t x, uint32_
                     rb))
         fn(3, 0, i, j);
   for (int j = 0; j < sb.nv; j++) {
-    OBBd ov;
-    ov.c = vb[j];
-    ov.e1 = {1, 0, 0};
-    ov.e2 = {0, 1, 0};
-    ov.e3 = {0, 0, 1};
-    ov.ext = {0, 0, 0};
-    obb_enlarge(ov, rb);
-    if (!obb_intersect(obb_i, ov)) continue;
+    if (!((gate_b >> j) & 1u)) continue;
     for (int i = 0; i < sa.ne; i++)
       if (filter_ve(rv, vb[j], rb, va[sa.edges[i][0]], va[sa.edges[i][1]], ra)) fn(1, 1, i, j);
     for (int i = 0; i < sa.nf; i++) {
@@ -382,70 +391,128 @@ __device__ inline void for_each_feature(const PolyNbhArgs& a, int64_t x, uint32_
 }
 
 // The owner slots of the list (one per pair, the lower id's entry; both entries of a pair seen through a
-// periodic image), compacted per wavefront (ballot, one atomic per wavefront): the feature enumeration
-// then runs one owner pair per thread, without the partner entries' idle lanes.
+// periodic image), compacted so that the feature enumeration runs one owner pair per thread, without the
+// partner entries' idle lanes. A block takes 16 x 256 slots and places them with one atomic (one atomic
+// per wavefront on the one counter serialised to 0.38 ms at 200k polyhedra); their order within the
+// compacted list places nothing that depends on it.
+constexpr int kOwnChunk = 16;
 __global__ __launch_bounds__(256) void k_owner_slots(NbrList nl, uint32_t* __restrict__ oslot,
                                                      uint32_t* __restrict__ count) {
-  const int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool own = sl < nl.slots && nl.sx[sl] != 0xffffffffu && (nl.word[sl] & kOwnerBit);
-  const unsigned long long bal = __ballot(own);
-  if (!bal) return;
-  const int lane = threadIdx.x & 63;
-  uint32_t base = 0;
-  if (lane == 0) base = atomicAdd(count, (uint32_t)__popcll(bal));
-  base = __shfl(base, 0, 64);
-  if (own) oslot[base + __popcll(bal & ((1ull << lane) - 1ull))] = (uint32_t)sl;
+  __shared__ uint32_t wtot[4];
+  __shared__ uint32_t bbase;
+  const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+  const int64_t c0 = (int64_t)blockIdx.x * 256 * kOwnChunk + threadIdx.x;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  uint32_t ownbits = 0, wcount = 0;
+#pragma unroll
+  for (int k = 0; k < kOwnChunk; k++) {
+    const int64_t sl = c0 + (int64_t)k * 256;
+    const bool own = sl < nl.slots && nl.sx[sl] != 0xffffffffu && (nl.word[sl] & kOwnerBit);
+    ownbits |= own ? (1u << k) : 0u;
+    wcount += (uint32_t)__popcll(__ballot(own));
+  }
+  if (lane == 0) wtot[wv] = wcount;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t tot = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+    bbase = tot ? atomicAdd(count, tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t pos = bbase;
+  for (int w = 0; w < wv; w++) pos += wtot[w];
+#pragma unroll
+  for (int k = 0; k < kOwnChunk; k++) {
+    const bool own = (ownbits >> k) & 1u;
+    const unsigned long long bal = __ballot(own);
+    if (own) oslot[pos + (uint32_t)__popcll(bal & below)] = (uint32_t)(c0 + (int64_t)k * 256);
+    pos += (uint32_t)__popcll(bal);
+  }
 }
 
 // One thread per owner slot: the pair's features enumerated once, the count and up to kFeatPerSlot
 // feature words kept in the slot's region of tmp.
 constexpr int kFeatPerSlot = 24;
-__global__ __launch_bounds__(256) void k_feat_enum(PolyNbhArgs a, const uint32_t* __restrict__ oslot,
-                                                   const uint32_t* __restrict__ nown, uint32_t* __restrict__ fcount,
-                                                   uint32_t* __restrict__ tmp) {
+// Features past kFeatPerSlot go to a chunk of kPoolChunk words claimed from a shared pool (counter
+// nown[2], capacity pool_cap) at the first one; fpool[sl] = the chunk, or ~0u when the pool is exhausted
+// or the pair has more than kFeatPerSlot + kPoolChunk features (then k_feat_overflow enumerates it again).
+constexpr int kEnumBlock = 128;
+constexpr uint32_t kPoolChunk = 232;
+__global__ __launch_bounds__(kEnumBlock) void k_feat_enum(PolyNbhArgs a, const uint32_t* __restrict__ oslot,
+                                                          uint32_t* __restrict__ nown, uint32_t* __restrict__ fcount,
+                                                          uint32_t* __restrict__ tmp, uint32_t* __restrict__ pool,
+                                                          uint32_t pool_cap, uint32_t* __restrict__ fpool) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)*nown) return;
+  if (t >= (int64_t)nown[0]) return;
   const uint32_t sl = oslot[t];
   const uint32_t x = a.nl.sx[sl], wd = a.nl.word[sl];
-  uint32_t nf = 0;
+  uint32_t nf = 0, pb = ~0u;
   uint32_t* out = tmp + (int64_t)sl * kFeatPerSlot;
   for_each_feature(a, x, wd & kIdxMask, (int)((wd >> kShiftShift) & 31u), [&](int ty, int sw, int si, int sj) {
-    if (nf < (uint32_t)kFeatPerSlot) out[nf] = sw ? feat_word(ty, 1, si, sj) : feat_word(ty, 0, si, sj);
+    const uint32_t w = sw ? feat_word(ty, 1, si, sj) : feat_word(ty, 0, si, sj);
+    if (nf < (uint32_t)kFeatPerSlot) {
+      out[nf] = w;
+    } else {
+      if (nf == (uint32_t)kFeatPerSlot) {
+        const uint32_t b = atomicAdd(nown + 2, kPoolChunk);
+        pb = b <= pool_cap - kPoolChunk ? b : ~0u;
+      }
+      if (pb != ~0u && nf < (uint32_t)kFeatPerSlot + kPoolChunk) pool[pb + (nf - kFeatPerSlot)] = w;
+    }
     nf++;
   });
   fcount[sl] = nf;
+  if (nf > (uint32_t)kFeatPerSlot) fpool[sl] = nf <= (uint32_t)kFeatPerSlot + kPoolChunk ? pb : ~0u;
 }
 
 // One thread per owner slot: its feature words copied to their place in the list (slot order, from the
-// scan of the counts); a pair with more than kFeatPerSlot features is enumerated again, into place.
-__global__ __launch_bounds__(256) void k_feat_place(PolyNbhArgs a, const uint32_t* __restrict__ oslot,
-                                                    const uint32_t* __restrict__ nown,
-                                                    const uint32_t* __restrict__ fcount,
-                                                    const uint32_t* __restrict__ tmp, const uint64_t* __restrict__ start,
-                                                    uint32_t* __restrict__ feat, uint32_t* __restrict__ fslot,
-                                                    uint32_t* __restrict__ fown) {
+// scan of the counts), the first kFeatPerSlot from tmp and the rest from its pool chunk; a pair without a
+// chunk goes to the overflow list (count in nown[1]) for k_feat_overflow.
+__global__ __launch_bounds__(256) void k_feat_place(NbrList nl, const uint32_t* __restrict__ oslot,
+                                                    uint32_t* __restrict__ nown, const uint32_t* __restrict__ fcount,
+                                                    const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ pool,
+                                                    const uint32_t* __restrict__ fpool,
+                                                    const uint64_t* __restrict__ start, uint32_t* __restrict__ feat,
+                                                    uint32_t* __restrict__ fslot, uint32_t* __restrict__ fown,
+                                                    uint32_t* __restrict__ ovf) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)*nown) return;
+  if (t >= (int64_t)nown[0]) return;
   const uint32_t sl = oslot[t];
   const uint32_t nf = fcount[sl];
   if (nf == 0) return;
-  const uint32_t x = a.nl.sx[sl];
-  uint64_t o = start[sl];
-  if (nf <= (uint32_t)kFeatPerSlot) {
-    for (uint32_t q = 0; q < nf; q++) {
-      feat[o + q] = tmp[(int64_t)sl * kFeatPerSlot + q];
-      fslot[o + q] = sl;
-      fown[o + q] = x;
-    }
+  const uint32_t pb = nf > (uint32_t)kFeatPerSlot ? fpool[sl] : 0u;
+  if (pb == ~0u) {
+    ovf[atomicAdd(nown + 1, 1u)] = sl;
     return;
   }
-  const uint32_t wd = a.nl.word[sl];
-  for_each_feature(a, x, wd & kIdxMask, (int)((wd >> kShiftShift) & 31u), [&](int ty, int sw, int si, int sj) {
-    feat[o] = sw ? feat_word(ty, 1, si, sj) : feat_word(ty, 0, si, sj);
-    fslot[o] = sl;
-    fown[o] = x;
-    o++;
-  });
+  const uint32_t x = nl.sx[sl];
+  const uint64_t o = start[sl];
+  for (uint32_t q = 0; q < nf; q++) {
+    feat[o + q] = q < (uint32_t)kFeatPerSlot ? tmp[(int64_t)sl * kFeatPerSlot + q] : pool[pb + (q - kFeatPerSlot)];
+    fslot[o + q] = sl;
+    fown[o + q] = x;
+  }
+}
+
+// The overflowing pairs (rare) enumerated again, straight into place; a fixed grid walks the list.
+constexpr int kOvfBlocks = 1024;
+__global__ __launch_bounds__(kEnumBlock) void k_feat_overflow(PolyNbhArgs a, const uint32_t* __restrict__ nown,
+                                                              const uint32_t* __restrict__ ovf,
+                                                              const uint64_t* __restrict__ start,
+                                                              uint32_t* __restrict__ feat, uint32_t* __restrict__ fslot,
+                                                              uint32_t* __restrict__ fown) {
+  const uint32_t novf = nown[1];
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (int64_t)novf;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t sl = ovf[t];
+    const uint32_t x = a.nl.sx[sl], wd = a.nl.word[sl];
+    uint64_t o = start[sl];
+    for_each_feature(a, x, wd & kIdxMask, (int)((wd >> kShiftShift) & 31u), [&](int ty, int sw, int si, int sj) {
+      feat[o] = sw ? feat_word(ty, 1, si, sj) : feat_word(ty, 0, si, sj);
+      fslot[o] = sl;
+      fown[o] = x;
+      o++;
+    });
+  }
 }
 
 // Feature range of every list slot: an owner entry's own range; a partner entry gets the range of the
@@ -757,7 +824,7 @@ struct PolyMeshArgs {
   double* cs;  // recorded contact state per candidate slot (null: not recorded)
 };
 
-// One thread per (candidate k = blockIdx.y, particle i): ContactLawRShapeDriverFunc
+// One thread per flat candidate: ContactLawRShapeDriverFunc
 // (polyhedron/contact.hpp:300-432) against the stationary mesh (its centre, no velocity, no spin). 7-10:
 // the law with the particle as i, its force f and moment (cp - r) x f + m on the particle. 11-12: the law
 // with the mesh vertex as i; the particle takes -f and (cp - r) x (-f) - m, and the friction is stored
@@ -765,12 +832,11 @@ struct PolyMeshArgs {
 template <bool KR, bool GUARD>
 __global__ __launch_bounds__(256) void k_poly_mesh_pass(PolyMeshArgs a) {
   if (GUARD && (a.flags[1] || a.flags[0])) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t k = blockIdx.y;
-  if (i >= a.t.nsph || k >= a.t.scnt[i]) return;
-  const int64_t x = a.t.sph[i];
-  const int64_t j = (int64_t)k * a.m.stride + x;  // list slot (recorded state)
-  const int64_t jc = (int64_t)k * a.t.nsph + i;   // compact copy (word, history, terms)
+  const int64_t jc = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // compact copy (word, history, terms)
+  if (jc >= a.t.ncand) return;
+  const uint2 ck = a.t.cand[jc];
+  const int64_t x = ck.x;
+  const int64_t j = (int64_t)ck.y * a.m.stride + x;  // list slot (recorded state)
   const MeshList& l = a.l;
   double* out = a.t.terms + jc * 6;
   const uint32_t wd = l.word[jc];
@@ -861,17 +927,28 @@ int poly_nbh(xdem_ctx* c, double rcut_inc) {
   }
   if (L.slots > c->cap_oslot) {
     const int64_t cap = (int64_t)(L.slots * 1.25) + 4096;
-    if ((rc = galloc(c, &c->oslot, cap))) return rc;
+    // the pool holds about one chunk per 128 slots (pairs past it are enumerated again);
+    // XDEM_FEAT_POOL_WORDS sets another capacity (the tests shrink it to take the re-enumeration path)
+    int64_t pcap = std::min<int64_t>(cap / 128 * kPoolChunk + 64 * kPoolChunk, 0x7fffffffLL);
+    if (const char* e = std::getenv("XDEM_FEAT_POOL_WORDS"))
+      pcap = std::min<int64_t>(std::max<int64_t>(std::atoll(e), kPoolChunk), 0x7fffffffLL);
+    if ((rc = galloc(c, &c->oslot, cap)) || (rc = galloc(c, &c->feat_ovf, cap)) ||
+        (rc = galloc(c, &c->feat_fpool, cap)) || (rc = galloc(c, &c->feat_pool, pcap)))
+      return rc;
     c->cap_oslot = cap;
+    c->cap_feat_pool = pcap;
   }
-  if (!c->d_nown && (rc = galloc(c, &c->d_nown, 2))) return rc;
-  XDEM_HIP(c, hipMemsetAsync(c->d_nown, 0, sizeof(uint32_t), c->stream));
+  if (!c->d_nown && (rc = galloc(c, &c->d_nown, 4))) return rc;
+  XDEM_HIP(c, hipMemsetAsync(c->d_nown, 0, 4 * sizeof(uint32_t), c->stream));
   XDEM_HIP(c, hipMemsetAsync(P.fcount, 0, sizeof(uint32_t) * std::max<int64_t>(L.slots, 1), c->stream));
   PolyNbhArgs a{n, live(c), c->grid, c->d_shapes, c->vtx, L, rcut_inc};
   const dim3 gsl(grid_blocks(std::max<int64_t>(L.slots, 1), 256)), bsl(256);
   if (n > 0) {
-    hipLaunchKernelGGL(k_owner_slots, gsl, bsl, 0, c->stream, L, c->oslot, c->d_nown);
-    hipLaunchKernelGGL(k_feat_enum, gsl, bsl, 0, c->stream, a, c->oslot, c->d_nown, P.fcount, c->feat_tmp);
+    hipLaunchKernelGGL(k_owner_slots, dim3(grid_blocks(std::max<int64_t>(L.slots, 1), 256 * kOwnChunk)), dim3(256), 0,
+                       c->stream, L, c->oslot, c->d_nown);
+    hipLaunchKernelGGL(k_feat_enum, dim3(grid_blocks(std::max<int64_t>(L.slots, 1), kEnumBlock)), dim3(kEnumBlock), 0,
+                       c->stream, a, c->oslot, c->d_nown, P.fcount, c->feat_tmp, c->feat_pool,
+                       (uint32_t)c->cap_feat_pool, c->feat_fpool);
     if ((rc = scan_u64_from_u32(c, P.fcount, P.start, L.slots, 1u))) return rc;
   }
   uint64_t nf = 0;
@@ -894,8 +971,10 @@ int poly_nbh(xdem_ctx* c, double rcut_inc) {
   if (nf >= 0xffffffffull) return set_err(c, XDEM_ERR_CAPACITY, "nbh_polyhedron: more than 2^32 - 2 features");
   P.nfeat = (int64_t)nf;
   if (n > 0) {
-    hipLaunchKernelGGL(k_feat_place, gsl, bsl, 0, c->stream, a, c->oslot, c->d_nown, P.fcount, c->feat_tmp, P.start,
-                       P.feat, P.fslot, P.fown);
+    hipLaunchKernelGGL(k_feat_place, gsl, bsl, 0, c->stream, L, c->oslot, c->d_nown, P.fcount, c->feat_tmp,
+                       c->feat_pool, c->feat_fpool, P.start, P.feat, P.fslot, P.fown, c->feat_ovf);
+    hipLaunchKernelGGL(k_feat_overflow, dim3(kOvfBlocks), dim3(kEnumBlock), 0, c->stream, a, c->d_nown, c->feat_ovf,
+                       P.start, P.feat, P.fslot, P.fown);
     hipLaunchKernelGGL(k_feat_range, dim3(grid_blocks(L.slots, 256)), dim3(256), 0, c->stream, L, P.start,
                        c->frange);
     const bool have_old = c->nl_prev_valid && c->perm_valid && c->pl_valid;
@@ -973,24 +1052,53 @@ __global__ __launch_bounds__(256) void k_class_count(int64_t nf, const uint32_t*
   if (threadIdx.x < 4) blk[(int64_t)blockIdx.x * 4 + threadIdx.x] = cnt[threadIdx.x];
 }
 
-// blk: per window and type counts -> start of the window's segment of that type; info: [4] padded total,
-// [5..8] features per type
-__global__ __launch_bounds__(64) void k_class_scan(int64_t nblk, uint32_t* __restrict__ blk,
-                                                   int64_t* __restrict__ info) {
-  if (threadIdx.x != 0) return;
-  int64_t run = 0, size[4] = {0, 0, 0, 0};
-  for (int64_t b = 0; b < nblk; b++)
-    for (int t = 0; t < 4; t++) {
-      const uint32_t v = blk[b * 4 + t];
-      blk[b * 4 + t] = (uint32_t)run;
-      run += (v + kClassPad - 1) / kClassPad * kClassPad;
-      size[t] += v;
-    }
-  for (int t = 0; t < 4; t++) {
-    info[t] = 0;
-    info[5 + t] = size[t];
+// blk: per window and type counts -> start of the window's segment of that type, in (window, type)
+// order; info: [4] padded total, [5..8] features per type. One block of 1024 threads: each sums a
+// contiguous run of the (window, type) counts, a block scan of the run sums, then each writes its
+// run's starts.
+__global__ __launch_bounds__(1024) void k_class_scan(int64_t nblk, uint32_t* __restrict__ blk,
+                                                     int64_t* __restrict__ info) {
+  __shared__ int64_t part[1024];
+  __shared__ int64_t tsz[4][1024];
+  const int t = (int)threadIdx.x;
+  const int64_t m = nblk * 4;
+  const int64_t chunk = (m + 1023) / 1024;
+  const int64_t lo = min(m, (int64_t)t * chunk), hi = min(m, lo + chunk);
+  int64_t sum = 0, sz[4] = {0, 0, 0, 0};
+  for (int64_t e = lo; e < hi; e++) {
+    const uint32_t v = blk[e];
+    sum += (v + kClassPad - 1) / kClassPad * kClassPad;
+    sz[e & 3] += v;
   }
-  info[4] = run;
+  part[t] = sum;
+  for (int q = 0; q < 4; q++) tsz[q][t] = sz[q];
+  __syncthreads();
+  // inclusive scan of part (Hillis-Steele)
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int64_t y = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += y;
+    __syncthreads();
+  }
+  int64_t run = part[t] - sum;
+  for (int64_t e = lo; e < hi; e++) {
+    const uint32_t v = blk[e];
+    blk[e] = (uint32_t)run;
+    run += (v + kClassPad - 1) / kClassPad * kClassPad;
+  }
+  // per-type totals (tree reduction)
+  for (int o = 512; o > 0; o >>= 1) {
+    if (t < o)
+      for (int q = 0; q < 4; q++) tsz[q][t] += tsz[q][t + o];
+    __syncthreads();
+  }
+  if (t == 0) {
+    for (int q = 0; q < 4; q++) {
+      info[q] = 0;
+      info[5 + q] = tsz[q][0];
+    }
+    info[4] = part[1023];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_class_scatter(int64_t nf, const uint32_t* __restrict__ feat,
@@ -1059,7 +1167,7 @@ int poly_classify(xdem_ctx* c) {
   XDEM_HIP(c, hipMemsetAsync(c->fclass_blk, 0, sizeof(uint32_t) * nblk * 4, c->stream));
   if (nf > 0)
     hipLaunchKernelGGL(k_class_count, dim3((uint32_t)nblk), dim3(256), 0, c->stream, nf, P.feat, c->fclass_blk);
-  hipLaunchKernelGGL(k_class_scan, dim3(1), dim3(64), 0, c->stream, nblk, c->fclass_blk, c->fclass_info);
+  hipLaunchKernelGGL(k_class_scan, dim3(1), dim3(1024), 0, c->stream, nblk, c->fclass_blk, c->fclass_info);
   XDEM_HIP(c, hipGetLastError());
   int64_t info[9];
   XDEM_HIP(c, hipMemcpyAsync(info, c->fclass_info, sizeof(info), hipMemcpyDeviceToHost, c->stream));
@@ -1115,7 +1223,7 @@ struct PolyForceArgs {
   int gravity_on;
   int32_t* flags;
   int32_t step_tag;
-  MeshTerms mt;  // terms of the mesh contacts, evaluated by k_poly_mesh_pass just before (nsph = 0: none)
+  MeshTerms mt;  // terms of the mesh contacts, evaluated by k_poly_mesh_pass just before (ncand = 0: none)
 };
 
 __device__ inline bool poly_guard(const PolyForceArgs& a) {
@@ -1148,10 +1256,11 @@ __global__ __launch_bounds__(128) void k_poly_feat(PolyForceArgs a) {
   const uint32_t wd = we.z;
   const uint32_t y = wd & kIdxMask;
   const V3 sh = shift_vec(a.g, (int)((wd >> kShiftShift) & 31u));
-  const Core cx = s.core[x], cy = s.core[y];
+  // detection needs only the homotheties; the rest of the two records is read for contacts only
+  const double hx = s.core[x].w.w, hy = s.core[y].w.w;
   const xdem_shape& sa = a.shapes[s.type[x]];
   const xdem_shape& sb = a.shapes[s.type[y]];
-  const double ra = sa.minkowski * cx.w.w, rb = sb.minkowski * cy.w.w;
+  const double ra = sa.minkowski * hx, rb = sb.minkowski * hy;
   const Verts va = verts_of(a.vtx, x, V3{0, 0, 0});
   const Verts vb = verts_of(a.vtx, y, sh);
   const uint32_t fw = we.w;
@@ -1171,6 +1280,7 @@ __global__ __launch_bounds__(128) void k_poly_feat(PolyForceArgs a) {
   else if (type == 2) d = detect_vf(vi[fi], ri, vj, sj, fj, rj);
   else d = detect_ee(vi[si.edges[fi][0]], vi[si.edges[fi][1]], ri, vj[sj.edges[fj][0]], vj[sj.edges[fj][1]], rj);
   if (d.contact) {
+    const Core cx = s.core[x], cy = s.core[y];
     const V3 rx{cx.xr.x, cx.xr.y, cx.xr.z}, vx{cx.vm.x, cx.vm.y, cx.vm.z}, wx{cx.w.x, cx.w.y, cx.w.z};
     const V3 ry{cy.xr.x + sh.x, cy.xr.y + sh.y, cy.xr.z + sh.z}, vy{cy.vm.x, cy.vm.y, cy.vm.z},
         wy{cy.w.x, cy.w.y, cy.w.z};
@@ -1295,15 +1405,28 @@ __global__ __launch_bounds__(256) void k_poly_gather(PolyForceArgs a) {
     if (fr.x == 0xffffffffu) continue;
     V3 Fp{0, 0, 0}, Mp{0, 0, 0};
     const uint64_t f0 = fr.x, f1 = (uint64_t)fr.x + fr.y;
-    for (uint64_t f = f0; f < f1; f++) {
-      if (!a.contact[f]) continue;
-      const double* pr = a.contrib + 9 * f;
-      if (own) {
-        Fp = Fp + V3{pr[0], pr[1], pr[2]};
-        Mp = Mp + V3{pr[3], pr[4], pr[5]};
-      } else {
-        Fp = Fp + V3{-pr[0], -pr[1], -pr[2]};
-        Mp = Mp + V3{pr[6], pr[7], pr[8]};
+    // the pair's contact flags as aligned 32-bit words, four words (16 flags) in flight at a time (the
+    // flag array has at least 4096 bytes of padding), then its features in contact in feature order
+    const uint32_t* __restrict__ cw = reinterpret_cast<const uint32_t*>(a.contact);
+    const uint64_t wend = (f1 + 3) >> 2;
+    for (uint64_t w0 = f0 >> 2; w0 < wend; w0 += 4) {
+      uint32_t v[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) v[q] = w0 + q < wend ? cw[w0 + q] : 0u;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        for (int byte = 0; byte < 4; byte++) {
+          const uint64_t f = ((w0 + q) << 2) + byte;
+          if (((v[q] >> (8 * byte)) & 0xffu) == 0u || f < f0 || f >= f1) continue;
+          const double* pr = a.contrib + 9 * f;
+          if (own) {
+            Fp = Fp + V3{pr[0], pr[1], pr[2]};
+            Mp = Mp + V3{pr[3], pr[4], pr[5]};
+          } else {
+            Fp = Fp + V3{-pr[0], -pr[1], -pr[2]};
+            Mp = Mp + V3{pr[6], pr[7], pr[8]};
+          }
+        }
       }
     }
     F = F + Fp;
@@ -1354,7 +1477,7 @@ __global__ __launch_bounds__(256) void k_poly_gather(PolyForceArgs a) {
         M = M + (cross(cpos - rx, f) + mint);
       }
   }
-  if (a.mt.nsph) mesh_add(a.mt, x, F, M);
+  if (a.mt.ncand) mesh_add(a.mt, x, F, M);
   if (FUSED) {
     s.ax[x] = F.x / mx;
     s.ay[x] = F.y / mx;
@@ -1427,7 +1550,9 @@ int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, d
       a.wstate = c->wstate;
     }
   }
-  if (c->has_mesh && c->ml_valid && c->mesh_nsph > 0) {
+  if (c->has_mesh && c->ml_valid && c->mesh_ncand > 0) {
+    if (!c->mesh_poly)
+      return set_err(c, XDEM_ERR_STATE, "mesh contacts of polyhedra: the mesh lists were built by nbh_sphere");
     double* mcs = nullptr;
     if (c->record_cs) {
       const int64_t m = (int64_t)c->mesh_cap * c->cap;
@@ -1440,14 +1565,13 @@ int poly_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, d
     if ((rc = mesh_compact(c))) return rc;
     c->mc_dirty = true;
     PolyMeshArgs pm{c->mesh, c->mc, MeshTerms{}, a.s, c->d_shapes, c->vtx, dt, cpd, c->d_flags, mcs};
-    pm.t.sph = c->mesh_sph;
-    pm.t.scnt = c->mesh_scnt;
+    pm.t.cand = c->mesh_cand;
     pm.t.idx = c->mesh_idx;
+    pm.t.wcnt = c->mesh_wcnt;
     pm.t.terms = c->mesh_terms;
-    pm.t.nsph = c->mesh_nsph;
-    pm.t.kmax = c->mesh_kmax;
+    pm.t.ncand = c->mesh_ncand;
     a.mt = pm.t;
-    const dim3 gm(grid_blocks(c->mesh_nsph, 256), (unsigned)c->mesh_kmax), bm(256);
+    const dim3 gm(grid_blocks(c->mesh_ncand, 256)), bm(256);
     if (c->kr_used) {
       if (guarded) hipLaunchKernelGGL((k_poly_mesh_pass<true, true>), gm, bm, 0, c->stream, pm);
       else hipLaunchKernelGGL((k_poly_mesh_pass<true, false>), gm, bm, 0, c->stream, pm);
@@ -1778,7 +1902,8 @@ extern "C" int xdem_classifier_waves(xdem_ctx* c, int64_t sizes[13]) {
   }
   if (!c->nl_valid || !c->hist_ready)
     return set_err(c, XDEM_ERR_STATE, "classifier_waves: run nbh_sphere and classify_interactions first");
-  sizes[0] = c->nl[c->nl_cur].pairs;
+  int rc = nl_pairs(c, &sizes[0]);
+  if (rc) return rc;
   if (c->nwalls > 0 && c->wl_valid) {
     std::vector<uint8_t> m(n);
     XDEM_HIP(c, hipMemcpyAsync(m.data(), c->wl[c->wl_cur].mask, n, hipMemcpyDeviceToHost, c->stream));

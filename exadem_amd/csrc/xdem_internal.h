@@ -335,16 +335,17 @@ struct MeshList {
   double *hx = nullptr, *hy = nullptr, *hz = nullptr, *mhx = nullptr, *mhy = nullptr, *mhz = nullptr;
 };
 // Mesh contacts of a force pass, evaluated one candidate per thread just before it (k_mesh_pass): the
-// spheres that have candidates (sph), their candidate counts (scnt), each sphere's index among them (idx,
-// -1: none) and the force and moment terms of candidate k of sphere i at (k * nsph + i) * 6 (-0.0 when
-// not in contact), which the force pass adds in candidate order.
+// candidates of all particles in one flat array, each particle's contiguous and in rank order; the
+// force and moment terms of candidate t at t * 6 (-0.0 when not in contact), which the force pass adds in
+// candidate order. Spheres: each candidate's feature geometry and kind cached at geo + t * 10 (three
+// points, then the kind), since the mesh is stationary.
 struct MeshTerms {
-  const uint32_t* sph = nullptr;
-  const uint32_t* scnt = nullptr;
-  const int32_t* idx = nullptr;
-  double* terms = nullptr;
-  int64_t nsph = 0;
-  int kmax = 0;
+  const uint2* cand = nullptr;  // per candidate t: (particle x, rank k among x's candidates)
+  const uint2* idx = nullptr;   // per particle: (first candidate t, count); count 0: none
+  const uint32_t* wcnt = nullptr;  // per wave of 64 particles (x >> 6): its candidates
+  double* terms = nullptr;      // 6 per candidate: force, then moment
+  const double* geo = nullptr;  // spheres: 10 per candidate (null for polyhedra)
+  int64_t ncand = 0;
 };
 }  // namespace xdem
 
@@ -428,6 +429,8 @@ struct xdem_ctx {
   int64_t cap_simpart = 0;
   double rcut_inc = 0;
   double nbh_dist = 0;
+  double rmax_host = -1;            // largest radius of the uploaded particles (-1: none yet)
+  uint64_t* d_pairs = nullptr;      // [2] half-list pair count of each neighbour list (read on demand)
   xdem::WallList wl[2];
   int wl_cur = 0;
   int wl_nw = 0;  // wall count used to size wl
@@ -463,6 +466,10 @@ struct xdem_ctx {
   uint32_t* feat_tmp = nullptr;     // [cap_feat_tmp] per list slot, up to kFeatPerSlot feature words of the
   int64_t cap_feat_tmp = 0;         // one-pass enumeration (nbh_polyhedron)
   uint32_t* oslot = nullptr;        // [cap_oslot] owner slots of the list (nbh_polyhedron)
+  uint32_t* feat_ovf = nullptr;     // [cap_oslot] owner slots whose features are enumerated again
+  uint32_t* feat_fpool = nullptr;   // [cap_oslot] per slot: its chunk of feat_pool (features past the tmp region)
+  uint32_t* feat_pool = nullptr;    // [cap_feat_pool]
+  int64_t cap_feat_pool = 0;
   int64_t cap_oslot = 0;
   uint32_t* d_nown = nullptr;       // their number
   uint2* frange = nullptr;          // [cap_frange] per list slot: first feature and count of its pair
@@ -499,22 +506,24 @@ struct xdem_ctx {
   int ml_cur = 0;
   bool ml_valid = false;
   int64_t ml_alloc = 0;             // particle capacity ml was allocated for
-  int32_t* d_mesh_over = nullptr;   // [0..1] max radius bits, [2] largest overflowing count, [3] spheres
-                                    // with candidates, [4] largest candidate count
-  uint32_t* mesh_sph = nullptr;     // MeshTerms arrays (sph, scnt, idx: particle capacity each)
-  uint32_t* mesh_scnt = nullptr;
-  int32_t* mesh_idx = nullptr;
+  int32_t* d_mesh_over = nullptr;   // [0..1] max radius bits, [2] largest overflowing count, [3] candidates
+  // The candidates of the current lists in one flat array, each particle's candidates contiguous and in
+  // rank order (MeshTerms): cand and terms have room for cap_mesh_cand candidates, idx for the particle
+  // capacity; mesh_ncand candidates in all
+  uint2* mesh_cand = nullptr;
+  uint2* mesh_idx = nullptr;
+  uint32_t* mesh_wcnt = nullptr;    // candidates per wave of 64 particles (MeshTerms::wcnt)
   double* mesh_terms = nullptr;
-  int64_t cap_mesh_terms = 0;
-  // compact copy of the current candidate words and histories, candidate k of particle i (among those
-  // with candidates) at k * mesh_nsph + i, which the mesh passes read and write; mc_valid: gathered from
-  // the lists since their last change; mc_dirty: newer than the lists (mesh_sync writes them back)
+  double* mesh_geo = nullptr;       // spheres: feature geometry per candidate (MeshTerms::geo)
+  bool mesh_poly = false;           // the lists hold polyhedron words (built by nbh_polyhedron)
+  int64_t cap_mesh_cand = 0;
+  int64_t mesh_ncand = 0;
+  // compact copy of the current candidate words and histories at the flat index t, which the mesh passes
+  // read and write; mc_valid: gathered from the lists since their last change; mc_dirty: newer than the
+  // lists (mesh_sync writes them back)
   xdem::MeshList mc;
-  int64_t cap_mc = 0;
   bool mc_valid = false;
   bool mc_dirty = false;
-  int64_t mesh_nsph = 0;
-  int mesh_kmax = 0;
   double* mstate = nullptr;         // recorded contact state per mesh slot (kCStateLen doubles each)
   int64_t cap_mstate = 0;
 };
@@ -563,6 +572,8 @@ int launch_to_accel(xdem_ctx* c);
 int launch_displ(xdem_ctx* c, double thr2);
 int do_move_particles(xdem_ctx* c);
 int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist);
+// Half-list pairs of the current neighbour list (the device count is read the first time it is asked for).
+int nl_pairs(xdem_ctx* c, int64_t* out);
 int do_nbh_sphere(xdem_ctx* c, double rcut_inc);
 int launch_force(xdem_ctx* c, const ContactParams& cp, const ContactParams& cpd, double dt, int form, bool fused,
                  bool gravity_on, V3 g, bool guarded, int32_t step_tag);
@@ -580,7 +591,7 @@ void mesh_free(xdem_ctx* c);
 // Mesh contacts of the coming force pass (k_mesh_pass over the live records): histories updated and the
 // terms written for the pass to add; guarded: a no-op when the pass would be (stall or trigger in
 // flags[trig_in]); cs: recorded contact state per mesh slot (null: not recorded). *t receives the view
-// the force pass reads (t->nsph = 0: no mesh contacts to add).
+// the force pass reads (t->ncand = 0: no mesh contacts to add).
 int launch_mesh_pass(xdem_ctx* c, const ContactParams& cpd, double dt, bool guarded, int32_t trig_in, double* cs,
                      MeshTerms* t);
 // The compact copy the passes use (gathered from the lists when they changed), and the write-back of

@@ -85,6 +85,48 @@ def test_feature_lists_bit_exact(gpu_ok, case):
         assert types[5] > 0
 
 
+def aligned_cubes(n=1000, gap=0.01, jitter=0.002):
+    """Axis-aligned cubes on a simple-cubic lattice, faces `gap` apart between the Minkowski shells (well
+    inside rVerlet): face-to-face neighbours share about 28 features (4 VV, 16 VE, 8 VF)."""
+    w = inputs.polyhedra(density=2500.0, n=n, spacing=1.0 + 2 * 0.02 + gap, jitter=jitter, seed=9, dt=2e-4)
+    cube = w.shapes[0]
+    assert cube.nv == 8
+    p = w.particles
+    p["type"] = np.zeros(w.n, dtype=np.uint32)
+    p["radius"] = np.full(w.n, float(np.max(p["radius"])))
+    p["mass"] = np.full(w.n, 2500.0 * cube.volume)
+    for a in ("inx", "iny", "inz"):
+        p[a] = p["mass"] * cube.inertia_on_mass[0]
+    p["qw"], p["qx"], p["qy"], p["qz"] = np.ones(w.n), np.zeros(w.n), np.zeros(w.n), np.zeros(w.n)
+    return w
+
+
+@pytest.mark.parametrize("pool_words", [None, 2 * 232])
+def test_feature_lists_through_overflow_paths(gpu_ok, monkeypatch, pool_words):
+    """Aligned cubes face to face, whose pairs have more than the 24 features a list slot keeps during the
+    enumeration: with the default pool their extra features come from pool chunks; with a pool of two
+    chunks most of them are enumerated again, straight into place. The lists stay bit-exact with the
+    oracle either way."""
+    if pool_words is None:
+        monkeypatch.delenv("XDEM_FEAT_POOL_WORDS", raising=False)
+    else:
+        monkeypatch.setenv("XDEM_FEAT_POOL_WORDS", str(pool_words))
+    w = aligned_cubes()
+    o = oracle_for(w)
+    o.rebuild_poly(w.rcut_inc)
+    s = PolySim(w)
+    s.rebuild()
+    gk, gf = feature_table(*s.ctx.poly_interactions())
+    ok, of = feature_table(*o.poly_interactions())
+    assert len(gk) == len(ok) and np.array_equal(gk, ok) and np.array_equal(gf, of)
+    feat = gk[gk[:, 2] < 4]
+    _, per_pair = np.unique(feat[:, :2], axis=0, return_counts=True)
+    big = int(np.sum(per_pair > 24))
+    print(f"pool {pool_words}: {len(per_pair)} pairs, {big} with more than 24 features, max {per_pair.max()}",
+          flush=True)
+    assert big >= 100
+
+
 @pytest.mark.parametrize("case", ["periodic", "walls"])
 def test_first_force_pass(gpu_ok, case):
     w = poly_case(case)
