@@ -54,8 +54,10 @@ void mesh_free(xdem_ctx* c) {
   c->mstate = nullptr;
   c->cap_mstate = 0;
   for (void* p : {(void*)c->mesh_cand, (void*)c->mesh_idx, (void*)c->mesh_terms, (void*)c->mesh_geo,
-                  (void*)c->mesh_wcnt})
+                  (void*)c->mesh_wcnt, (void*)c->mesh_cc, (void*)c->mesh_off})
     if (p) (void)hipFree(p);
+  c->mesh_cc = nullptr;
+  c->mesh_off = nullptr;
   c->mesh_wcnt = nullptr;
   c->mesh_geo = nullptr;
   c->mesh_cand = nullptr;
@@ -95,6 +97,8 @@ static int alloc_ml(xdem_ctx* c) {
     if ((rc = dev_realloc(c, &l.mhz, m))) return rc;
   }
   if ((rc = dev_realloc(c, &c->mesh_idx, c->cap))) return rc;
+  if ((rc = dev_realloc(c, &c->mesh_cc, c->cap))) return rc;
+  if ((rc = dev_realloc(c, &c->mesh_off, c->cap + 1))) return rc;
   if ((rc = dev_realloc(c, &c->mesh_wcnt, (c->cap + 63) / 64))) return rc;
   c->ml_alloc = c->cap;
   c->mesh.stride = c->cap;
@@ -256,32 +260,22 @@ __global__ __launch_bounds__(256) void k_mesh_carry(int64_t n, PState s, MeshVie
   if (cnt > (uint32_t)m.cap) atomicMax(over, (int32_t)cnt);
 }
 
-// Flat candidate index: each particle's candidates get a contiguous run of flat indices, runs placed per
-// wavefront (a wavefront prefix sum of the counts, one atomic per wavefront on ctr[0], the total). idx[x]
-// = (first flat index, count), (0, 0) without candidates; wcnt[x >> 6]: the wave's candidates. The
-// placement of the runs only places the terms of the mesh pass; no sum depends on it.
-__global__ __launch_bounds__(256) void k_mesh_index(int64_t n, const uint32_t* __restrict__ cnt, uint32_t cap,
-                                                    uint2* __restrict__ idx, uint32_t* __restrict__ wcnt,
-                                                    int32_t* __restrict__ ctr) {
+// Flat candidate index in particle order: the counts (clamped to the capacity) are scanned into off[0..n]
+// (k_mesh_count, then a scan); idx[x] = (off[x], count), (0, 0) without candidates; wcnt[x >> 6]: the
+// wave's candidates.
+__global__ __launch_bounds__(256) void k_mesh_count(int64_t n, const uint32_t* __restrict__ cnt, uint32_t cap,
+                                                    uint32_t* __restrict__ cc) {
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t c = x < n ? min(cnt[x], cap) : 0u;
-  const int lane = threadIdx.x & 63;
-  uint32_t incl = c;
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t v = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += v;
-  }
-  const uint32_t total = __shfl(incl, 63, 64);
-  if (lane == 0 && x < n) wcnt[x >> 6] = total;
-  uint32_t base = 0;
-  if (total) {
-    if (lane == 0) base = (uint32_t)atomicAdd(ctr, (int32_t)total);
-    base = __shfl(base, 0, 64);
-  }
-  if (x >= n) return;
-  idx[x] = c ? uint2{base + incl - c, c} : uint2{0u, 0u};
+  if (x < n) cc[x] = min(cnt[x], cap);
 }
-
+__global__ __launch_bounds__(256) void k_mesh_index(int64_t n, const uint32_t* __restrict__ off,
+                                                    uint2* __restrict__ idx, uint32_t* __restrict__ wcnt) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  const uint32_t o = off[x], c = off[x + 1] - o;
+  idx[x] = c ? uint2{o, c} : uint2{0u, 0u};
+  if ((x & 63) == 0) wcnt[x >> 6] = off[x + 64 < n ? x + 64 : n] - o;
+}
 // The flat candidates: cand[t] = (particle, rank), one thread per particle over its run.
 __global__ __launch_bounds__(256) void k_mesh_cand(int64_t n, const uint2* __restrict__ idx,
                                                    uint2* __restrict__ cand) {
@@ -291,7 +285,8 @@ __global__ __launch_bounds__(256) void k_mesh_cand(int64_t n, const uint2* __res
   for (uint32_t k = 0; k < r.y; k++) cand[r.x + k] = uint2{(uint32_t)x, k};
 }
 
-// One thread per flat candidate t.
+// One thread per flat candidate. (Running each XCD's candidates on that XCD, for the L2, measured slower:
+// the candidates of a plane cluster in a few XCDs' particle ranges, which then do all the work.)
 template <bool KR, bool GUARD>
 __global__ __launch_bounds__(256) void k_mesh_pass(MeshView m, MeshList l, MeshTerms t, const Core* __restrict__ core,
                                                    double dt, ContactParams cpd, const int32_t* __restrict__ flags,
@@ -418,7 +413,7 @@ int mesh_carry(xdem_ctx* c, double rcut_inc, bool have_old, bool poly) {
   const int64_t n = c->n;
   // reach of the bins
   unsigned long long* d_rmax = (unsigned long long*)c->d_mesh_over;
-  XDEM_HIP(c, hipMemsetAsync(c->d_mesh_over, 0, 8 * sizeof(int32_t), c->stream));
+  XDEM_HIP(c, hipMemsetAsync(c->d_mesh_over, 0, 16 * sizeof(int32_t), c->stream));
   hipLaunchKernelGGL(k_max_radius, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, live(c).core, d_rmax);
   XDEM_HIP(c, hipGetLastError());
   unsigned long long rb = 0;
@@ -443,11 +438,15 @@ int mesh_carry(xdem_ctx* c, double rcut_inc, bool have_old, bool poly) {
                        rcut_inc, c->ml[nxt], c->ml[c->ml_cur], c->perm, old ? 1 : 0, d_over);
     XDEM_HIP(c, hipGetLastError());
   }
-  hipLaunchKernelGGL(k_mesh_index, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, c->ml[nxt].cnt,
-                     (uint32_t)c->mesh_cap, c->mesh_idx, c->mesh_wcnt, d_over + 1);
+  hipLaunchKernelGGL(k_mesh_count, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, c->ml[nxt].cnt,
+                     (uint32_t)c->mesh_cap, c->mesh_cc);
+  if ((rc = scan_u32(c, c->mesh_cc, c->mesh_off, n))) return rc;
+  hipLaunchKernelGGL(k_mesh_index, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, c->mesh_off, c->mesh_idx,
+                     c->mesh_wcnt);
   XDEM_HIP(c, hipGetLastError());
   int32_t over[2] = {0, 0};  // largest overflowing count, candidates
-  XDEM_HIP(c, hipMemcpyAsync(over, d_over, sizeof(over), hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipMemcpyAsync(over, d_over, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  XDEM_HIP(c, hipMemcpyAsync(over + 1, c->mesh_off + n, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
   XDEM_HIP(c, hipStreamSynchronize(c->stream));
   c->ml_cur = nxt;
   c->mesh_poly = poly;
@@ -514,7 +513,7 @@ extern "C" int xdem_set_mesh(xdem_ctx* c, int32_t id, const double* vertices, in
   if ((rc = dev_realloc(c, &c->d_mesh_v, nv))) return rc;
   if ((rc = dev_realloc(c, &c->d_mesh_e, std::max<int64_t>(ne, 1)))) return rc;
   if ((rc = dev_realloc(c, &c->d_mesh_f, std::max<int64_t>(nf, 1)))) return rc;
-  if ((rc = dev_realloc(c, &c->d_mesh_over, 8))) return rc;
+  if ((rc = dev_realloc(c, &c->d_mesh_over, 16))) return rc;
   XDEM_HIP(c, hipMemcpy(c->d_mesh_v, c->mesh_hv.data(), sizeof(V3) * nv, hipMemcpyHostToDevice));
   if (ne) XDEM_HIP(c, hipMemcpy(c->d_mesh_e, c->mesh_he.data(), sizeof(uint2) * ne, hipMemcpyHostToDevice));
   if (nf) XDEM_HIP(c, hipMemcpy(c->d_mesh_f, c->mesh_hf.data(), sizeof(uint4) * nf, hipMemcpyHostToDevice));

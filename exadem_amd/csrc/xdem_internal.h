@@ -506,13 +506,15 @@ struct xdem_ctx {
   int ml_cur = 0;
   bool ml_valid = false;
   int64_t ml_alloc = 0;             // particle capacity ml was allocated for
-  int32_t* d_mesh_over = nullptr;   // [0..1] max radius bits, [2] largest overflowing count, [3] candidates
+  int32_t* d_mesh_over = nullptr;   // [0..1] max radius bits, [2] largest overflowing count
   // The candidates of the current lists in one flat array, each particle's candidates contiguous and in
   // rank order (MeshTerms): cand and terms have room for cap_mesh_cand candidates, idx for the particle
   // capacity; mesh_ncand candidates in all
   uint2* mesh_cand = nullptr;
   uint2* mesh_idx = nullptr;
   uint32_t* mesh_wcnt = nullptr;    // candidates per wave of 64 particles (MeshTerms::wcnt)
+  uint32_t* mesh_cc = nullptr;      // [cap] clamped candidate counts, scanned into mesh_off [cap + 1]
+  uint32_t* mesh_off = nullptr;
   double* mesh_terms = nullptr;
   double* mesh_geo = nullptr;       // spheres: feature geometry per candidate (MeshTerms::geo)
   bool mesh_poly = false;           // the lists hold polyhedron words (built by nbh_polyhedron)
