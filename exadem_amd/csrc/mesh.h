@@ -164,10 +164,12 @@ __device__ inline bool mesh_detect(const MeshView& m, uint32_t wd, V3 p, double 
 // sphere's force and moment written to out[0..5]; out of contact the history is cleared and the terms
 // are -0.0, which adds nothing to any value. cs: the recorded contact state of the candidate's list slot
 // j (null: not recorded).
+// ft0 / mint0: the history at jc, read by the caller ahead of the particle's record.
 template <bool KR>
 __device__ inline void mesh_candidate(const MeshView& m, const MeshList& l, int64_t jc, int64_t j, uint32_t kind,
                                       const V3 (&tg)[3], V3 rx, double Rx, double mx, V3 vx, V3 wx, double dt,
-                                      const ContactParams& cpd, double* __restrict__ out, double* cs) {
+                                      const ContactParams& cpd, double* __restrict__ out, double* cs, V3 ft0,
+                                      V3 mint0) {
   double dn;
   V3 nrm, cpos;
   if (!mesh_detect_t(kind, tg, m.rm, rx, Rx, dn, nrm, cpos)) {
@@ -183,9 +185,9 @@ __device__ inline void mesh_candidate(const MeshView& m, const MeshList& l, int6
     reinterpret_cast<double2*>(out)[2] = double2{-0.0, -0.0};
     return;
   }
-  V3 ft{l.hx[jc], l.hy[jc], l.hz[jc]};
+  V3 ft = ft0;
   V3 mint{0.0, 0.0, 0.0};
-  if (KR) mint = {l.mhx[jc], l.mhy[jc], l.mhz[jc]};
+  if (KR) mint = mint0;
   const V3 zero{0.0, 0.0, 0.0};
   const V3 f = hooke_force<KR>(dn, nrm, dt, cpd, mx, ft, mint, cpos, rx, vx, wx, m.center, zero, zero);
   l.hx[jc] = ft.x; l.hy[jc] = ft.y; l.hz[jc] = ft.z;

@@ -291,18 +291,23 @@ template <bool KR, bool GUARD>
 __global__ __launch_bounds__(256) void k_mesh_pass(MeshView m, MeshList l, MeshTerms t, const Core* __restrict__ core,
                                                    double dt, ContactParams cpd, const int32_t* __restrict__ flags,
                                                    int32_t trig_in, double* cs) {
-  if (GUARD && (flags[1] || flags[trig_in])) return;
   const int64_t jc = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (jc >= t.ncand) return;
+  // the candidate's own loads (entry, cached feature, history) go out before the guard is tested, which
+  // then waits on them (the particle's record needs the entry)
   const uint2 ck = t.cand[jc];
   const double2* gp = reinterpret_cast<const double2*>(t.geo + jc * 10);
   const double2 g0 = gp[0], g1 = gp[1], g2 = gp[2], g3 = gp[3], g4 = gp[4];
+  const V3 ft0{l.hx[jc], l.hy[jc], l.hz[jc]};
+  V3 mint0{0.0, 0.0, 0.0};
+  if (KR) mint0 = {l.mhx[jc], l.mhy[jc], l.mhz[jc]};
+  if (GUARD && (flags[1] || flags[trig_in]) && ck.x != 0xffffffffu) return;
   const V3 tg[3] = {V3{g0.x, g0.y, g1.x}, V3{g1.y, g2.x, g2.y}, V3{g3.x, g3.y, g4.x}};
   const int64_t x = ck.x;
   const Core me = core[x];
   const V3 rx{me.xr.x, me.xr.y, me.xr.z}, vx{me.vm.x, me.vm.y, me.vm.z}, wx{me.w.x, me.w.y, me.w.z};
   mesh_candidate<KR>(m, l, jc, (int64_t)ck.y * m.stride + x, (uint32_t)g4.y, tg, rx, me.xr.w, me.vm.w, vx, wx, dt,
-                     cpd, t.terms + jc * 6, cs);
+                     cpd, t.terms + jc * 6, cs, ft0, mint0);
 }
 
 // Compact copy <-> lists, one thread per flat candidate: the gather copies the words and histories (and,
