@@ -83,6 +83,62 @@ def test_streams_bit_exact(gpu_ok, case):
     assert np.array_equal(pg["rx"], pq["rx"]) and np.array_equal(pg["rz"], pq["rz"])
 
 
+def small_periodic_box(dims, seed):
+    """Spheres (radius U[0.4, 0.55]) on a jittered lattice of spacing about 1.1 in a periodic box of dims
+    cells of 1.25 (the neighbour distance is at most 1.2): three cells along an axis make every stencil
+    wrap, and dims not a multiple of 3 end the neighbour build's row segments early."""
+    rng = np.random.default_rng(seed)
+    cell, sp = 1.25, 1.1
+    L = [d * cell for d in dims]
+    g = [max(1, int(round(Lk / sp))) for Lk in L]
+    ix, iy, iz = np.meshgrid(*[np.arange(k) for k in g], indexing="ij")
+    pts = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], axis=1).astype(float)
+    pts = (pts + 0.5) * np.array([Lk / k for Lk, k in zip(L, g)]) + rng.uniform(-0.02, 0.02, pts.shape)
+    n = len(pts)
+    r = rng.uniform(0.4, 0.55, n)
+    v = rng.normal(0.0, 0.5, (n, 3))
+    p = inputs.sphere_set(pts[:, 0].copy(), pts[:, 1].copy(), pts[:, 2].copy(), r, 2500.0,
+                          v[:, 0].copy(), v[:, 1].copy(), v[:, 2].copy())
+    dom = native.domain((0.0, 0.0, 0.0), tuple(L), (True, True, True), cell)
+    cp = native.contact_params(kn=1e5, mu=0.3, damp_rate=0.9)
+    return inputs.Workload("small periodic", p, dom, [], cp, cp, 1e-3, 200, 0.1, (0.0, 0.0, 0.0))
+
+
+@pytest.mark.parametrize("dims", [(3, 4, 5), (4, 3, 4), (5, 7, 3), (7, 4, 3), (3, 3, 4), (8, 5, 6)])
+def test_small_periodic_boxes_match_oracle(gpu_ok, dims):
+    """Streams bit-exact with the oracle after the first build, and 200 engine steps with rebuilds
+    (velocities N(0, 0.5), dt 1e-3) equal to the oracle's, in periodic boxes of 3 to 8 cells per axis
+    (fewer than 3 on a periodic axis is refused by set_domain)."""
+    w = small_periodic_box(dims, seed=sum(dims))
+    o = oracle_for(w)
+    o.rebuild(w.rcut_inc)
+    s = gpu_for(w)
+    s.rebuild()
+    a, b, code, own = s.ctx.streams()
+    oa, ob, ocode = o.streams()
+    assert len(a) == len(oa) and len(a) > 0
+    g_ord = np.argsort(a, kind="stable")
+    o_ord = np.argsort(oa, kind="stable")
+    assert np.array_equal(a[g_ord], oa[o_ord])
+    assert np.array_equal(b[g_ord], ob[o_ord]) and np.array_equal(code[g_ord], ocode[o_ord])
+    sch = scheme(w)
+    s2 = gpu_for(w)
+    s2.first_iteration()
+    st = s2.run(200, poll=8)
+    o2 = oracle_for(w)
+    o2.first_iteration(sch)
+    o2.run_steps(sch, 200)
+    floor = by_id(w.particles)["mass"] * 9.81
+    compare_state(s2.ctx.download(), o2.particles(), floor, f"dims {dims}, 200 steps")
+    print(f"dims {dims}: {w.n} spheres, {len(a)} stream entries, {st['rebuilds']} rebuilds", flush=True)
+
+
+def test_periodic_axis_needs_three_cells(gpu_ok):
+    w = small_periodic_box((2, 4, 4), seed=1)
+    with pytest.raises(native.XdemError, match="at least 3 cells"):
+        gpu_for(w)
+
+
 @pytest.mark.parametrize("case", ["dense", "walls", "periodic"])
 def test_first_iteration_forces(gpu_ok, case):
     w = {"dense": dense_deposition, "walls": wall_deposition, "periodic": periodic_pack}[case]()
