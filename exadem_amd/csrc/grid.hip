@@ -688,6 +688,53 @@ __global__ __launch_bounds__(256) void k_hist_scatter(int64_t oslots, NbrList nl
   }
 }
 
+// The same carry-over as a gather, one thread per slot of the new list (its writes coalesce), when a
+// partner appears at most once per list (hint_ok). For new entry s = (nx -> ny) with nx, ny at old
+// indices ox, oy: an owner entry takes the history of the old entry (ox -> oy) when that was an owner
+// entry with history (the scatter's first target); a non-owner entry takes that of the old owner entry
+// (oy -> ox) (the scatter's second target). When the old (ox -> oy) was itself a non-owner entry, it is
+// the copy of (oy -> ox), written by every pass with the same values and active bit, and is read in
+// its place; only a pair whose image changed at this rebuild (old (ox -> oy) an owner) looks up
+// (oy -> ox) in oy's old list.
+template <typename T, bool KR>
+__global__ __launch_bounds__(256) void k_hist_gather(int64_t nslots, NbrList nl, NbrList ol,
+                                                     const uint32_t* __restrict__ perm) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  const uint32_t nx = nl.sx[s];
+  if (nx == 0xffffffffu) return;
+  const uint32_t w = nl.word[s];
+  const uint32_t ox = perm[nx], oy = perm[w & kIdxMask];
+  // rank of this entry in nx's new list: the first place to look in ox's old list
+  const uint32_t kh = (uint32_t)((uint64_t)(s - (int64_t)nl.base[nx >> 6] - (int64_t)(nx & 63)) / kWave);
+  const uint64_t e = find_entry(ol, ox, oy, kh);
+  if (e == ~0ull) return;
+  const uint32_t we = ol.word[e];
+  uint64_t src = ~0ull;
+  if (w & kOwnerBit) {
+    if ((we & kOwnerBit) && (we & kActiveBit)) src = e;
+  } else if (!(we & kOwnerBit)) {
+    if (we & kActiveBit) src = e;
+  } else {
+    const uint64_t e2 = find_entry(ol, oy, ox);
+    if (e2 != ~0ull) {
+      const uint32_t w2 = ol.word[e2];
+      if ((w2 & kOwnerBit) && (w2 & kActiveBit)) src = e2;
+    }
+  }
+  if (src == ~0ull) return;
+  const HistView<T> nh = hist_view(nl, (T*)nullptr), oh = hist_view(ol, (T*)nullptr);
+  nh.x[s] = oh.x[src];
+  nh.y[s] = oh.y[src];
+  nh.z[s] = oh.z[src];
+  if (KR) {
+    nh.mx[s] = oh.mx[src];
+    nh.my[s] = oh.my[src];
+    nh.mz[s] = oh.mz[src];
+  }
+  nl.word[s] = w | kActiveBit;
+}
+
 // Driver candidates (the driver's filter with rad + rcut_inc, e.g. Surface::filter, surface.hpp:320-324)
 // and carry-over.
 __global__ __launch_bounds__(256) void k_wall_carry(int64_t n, int nw, const Surface* __restrict__ walls, PState s,
@@ -740,18 +787,29 @@ int do_nbh_sphere(xdem_ctx* c, double rcut_inc) {
   }
   const bool have_old = c->nl_prev_valid && c->perm_valid;
   if (have_old) {
-    uint32_t* inv = c->perm_tmp;  // free after the neighbour build
-    hipLaunchKernelGGL(k_invert, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, c->perm, inv);
     const NbrList& nl = c->nl[c->nl_cur];
     const NbrList& ol = c->nl[1 - c->nl_cur];
-    const int64_t os = ol.slots;
-    if (os > 0) {
+    const int64_t os = ol.slots, ns = nl.slots;
+    // a partner appears at most once per list unless a periodic direction has fewer than 3 cells (which
+    // set_domain refuses); the scatter form stays for that case
+    int hint_ok = 1;
+    for (int d = 0; d < 3; d++)
+      if (c->grid.per[d] && c->grid.dims[d] < 3) hint_ok = 0;
+    // moment histories are zero (and not carried) until a rolling stiffness is used
+    if (hint_ok && os > 0 && ns > 0) {
+      const dim3 gs(grid_blocks(ns, 256)), bs(256);
+      if (c->hist_fp32) {
+        if (c->kr_used) hipLaunchKernelGGL((k_hist_gather<float, true>), gs, bs, 0, c->stream, ns, nl, ol, c->perm);
+        else hipLaunchKernelGGL((k_hist_gather<float, false>), gs, bs, 0, c->stream, ns, nl, ol, c->perm);
+      } else {
+        if (c->kr_used) hipLaunchKernelGGL((k_hist_gather<double, true>), gs, bs, 0, c->stream, ns, nl, ol, c->perm);
+        else hipLaunchKernelGGL((k_hist_gather<double, false>), gs, bs, 0, c->stream, ns, nl, ol, c->perm);
+      }
+      XDEM_HIP(c, hipGetLastError());
+    } else if (os > 0) {
+      uint32_t* inv = c->perm_tmp;  // free after the neighbour build
+      hipLaunchKernelGGL(k_invert, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, c->perm, inv);
       const dim3 gs(grid_blocks(os, 256)), bs(256);
-      // a partner appears at most once per list unless a periodic direction has fewer than 3 cells
-      int hint_ok = 1;
-      for (int d = 0; d < 3; d++)
-        if (c->grid.per[d] && c->grid.dims[d] < 3) hint_ok = 0;
-      // moment histories are zero (and not carried) until a rolling stiffness is used
       if (c->hist_fp32) {
         if (c->kr_used) hipLaunchKernelGGL((k_hist_scatter<float, true>), gs, bs, 0, c->stream, os, nl, ol, inv, hint_ok);
         else hipLaunchKernelGGL((k_hist_scatter<float, false>), gs, bs, 0, c->stream, os, nl, ol, inv, hint_ok);
