@@ -194,6 +194,74 @@ __device__ inline bool vf_core(V3 vi, const Verts& vj, const xdem_shape& shj, in
   n_out = n;
   return odd == 1;
 }
+// The vertex-independent part of vf_core for one face (the same operations): its first vertex, the unit
+// edge v1, the unit normal n0, v2 = n0 x v1 and the projections of the face's vertices on v1 and v2.
+struct FaceFrame {
+  V3 va, v1, n0, v2;
+  double q1[XDEM_MAX_FACE_VERTICES], q2[XDEM_MAX_FACE_VERTICES];
+  int nf;
+};
+__device__ inline FaceFrame face_frame(const Verts& vj, const xdem_shape& shj, int face) {
+  FaceFrame fr;
+  const int nf = shj.face_size[face];
+  const int* f = shj.faces[face];
+  const V3 va = vj[f[0]];
+  const V3 vb = vj[f[1]];
+  const V3 vc = vj[f[nf - 1]];
+  V3 v1 = vb - va;
+  const V3 v2 = vc - va;
+  v1 = vnormalize(v1);
+  V3 n = cross(v1, v2);
+  n = vnormalize(n);
+  fr.va = va;
+  fr.v1 = v1;
+  fr.n0 = n;
+  fr.v2 = cross(n, v1);
+  fr.nf = nf;
+#pragma unroll
+  for (int k = 0; k < XDEM_MAX_FACE_VERTICES; k++) {
+    fr.q1[k] = 0.0;
+    fr.q2[k] = 0.0;
+    if (k < nf) {
+      const V3 A = vj[f[k]];
+      fr.q1[k] = dot(A, v1);
+      fr.q2[k] = dot(A, fr.v2);
+    }
+  }
+  return fr;
+}
+// vf_core's verdict from a face frame. Where the normal flips (n = n0 (-1)), v2 = n x v1 and the
+// projections on it are the frame's negated, which is exact (negating a factor negates each product, and
+// rounding to nearest is symmetric): every comparison sees the values vf_core computes.
+__device__ inline bool vf_frame(V3 vi, const FaceFrame& fr, double limit) {
+  const V3 v = vi - fr.va;
+  V3 n = fr.n0;
+  double dist = dot(n, v);
+  bool flip = false;
+  if (dist < 0) {
+    n = n * (-1);
+    dist = -dist;
+    flip = true;
+  }
+  if (dist > limit) return false;
+  const V3 P = vi - n * dist;
+  const double ori1 = dot(P, fr.v1), o2 = dot(P, fr.v2), ori2 = flip ? -o2 : o2;
+  int odd = 0;
+#pragma unroll
+  for (int iva = 0; iva < XDEM_MAX_FACE_VERTICES; ++iva) {
+    if (iva < fr.nf) {
+      const bool last = iva + 1 == fr.nf;
+      const double pa1 = fr.q1[iva], pb1 = last ? fr.q1[0] : fr.q1[(iva + 1) % XDEM_MAX_FACE_VERTICES];
+      const double qa2 = fr.q2[iva], qb2 = last ? fr.q2[0] : fr.q2[(iva + 1) % XDEM_MAX_FACE_VERTICES];
+      const double pa2 = flip ? -qa2 : qa2, pb2 = flip ? -qb2 : qb2;
+      if ((pa2 < ori2 && pb2 >= ori2) || (pb2 < ori2 && pa2 >= ori2)) {
+        if (pa1 + (ori2 - pa2) / (pb2 - pa2) * (pb1 - pa1) < ori1) odd = 1 - odd;
+      }
+    }
+  }
+  return odd == 1;
+}
+
 __device__ inline bool filter_ee(double rv, V3 vfi, V3 vsi, double ri, V3 vfj, V3 vsj, double rj) {
   const double R = ri + rj + rv;
   const V3 Ei = vsi - vfi, Ej = vsj - vfj, v = vfi - vfj;
@@ -359,18 +427,33 @@ __device__ inline void for_each_feature(const PolyNbhArgs& a, int64_t x, uint32_
     obb_enlarge(ov, rb);
     if (obb_intersect(obb_i, ov)) gate_b |= 1u << j;
   }
+  // vertex-face filters face by face (each face's frame once, its gated vertices against it), as bits
+  // 8 (vertex) + face read back in the reference's loop order below
+  uint64_t hit_a = 0, hit_b = 0;
+  if (gate_a) {
+    const double lim = ra + rb + rv;
+    for (int j = 0; j < sb.nf; j++) {
+      const FaceFrame fr = face_frame(vb, sb, j);
+      for (int i = 0; i < sa.nv; i++)
+        if (((gate_a >> i) & 1u) && vf_frame(va[i], fr, lim)) hit_a |= 1ull << (8 * i + j);
+    }
+  }
+  if (gate_b) {
+    const double lim = rb + ra + rv;
+    for (int i = 0; i < sa.nf; i++) {
+      const FaceFrame fr = face_frame(va, sa, i);
+      for (int j = 0; j < sb.nv; j++)
+        if (((gate_b >> j) & 1u) && vf_frame(vb[j], fr, lim)) hit_b |= 1ull << (8 * j + i);
+    }
+  }
   for (int i = 0; i < sa.nv; i++) {
     if (!((gate_a >> i) & 1u)) continue;
     for (int j = 0; j < sb.nv; j++)
       if (filter_vv(rv, va[i], ra, vb[j], rb)) fn(0, 0, i, j);
     for (int j = 0; j < sb.ne; j++)
       if (filter_ve(rv, va[i], ra, vb[sb.edges[j][0]], vb[sb.edges[j][1]], rb)) fn(1, 0, i, j);
-    for (int j = 0; j < sb.nf; j++) {
-      double dist;
-      V3 nn;
-      bool early;
-      if (vf_core(va[i], vb, sb, j, ra + rb + rv, dist, nn, early)) fn(2, 0, i, j);
-    }
+    for (int j = 0; j < sb.nf; j++)
+      if ((hit_a >> (8 * i + j)) & 1ull) fn(2, 0, i, j);
   }
   for (int i = 0; i < sa.ne; i++)
     for (int j = 0; j < sb.ne; j++)
@@ -381,12 +464,8 @@ __device__ inline void for_each_feature(const PolyNbhArgs& a, int64_t x, uint32_
     if (!((gate_b >> j) & 1u)) continue;
     for (int i = 0; i < sa.ne; i++)
       if (filter_ve(rv, vb[j], rb, va[sa.edges[i][0]], va[sa.edges[i][1]], ra)) fn(1, 1, i, j);
-    for (int i = 0; i < sa.nf; i++) {
-      double dist;
-      V3 nn;
-      bool early;
-      if (vf_core(vb[j], va, sa, i, rb + ra + rv, dist, nn, early)) fn(2, 1, i, j);
-    }
+    for (int i = 0; i < sa.nf; i++)
+      if ((hit_b >> (8 * j + i)) & 1ull) fn(2, 1, i, j);
   }
 }
 
