@@ -1047,8 +1047,11 @@ int xdem_run_steps(xdem_ctx* c, const xdem_scheme* s, int64_t nsteps, xdem_run_s
       if (!chain) {
         if (timed && (rc = record_timer(c, 0, true))) return rc;
         // spheres test the centre displacement in the prolog; polyhedra test vertices (vertex_displ_over)
-        if ((rc = launch_prolog(c, s->dt, kick, kdt, !poly, thr * thr, true))) return rc;
-        if (poly && (rc = poly_compute_vertices(c, true, thr * thr, true))) return rc;
+        if (poly) {
+          if ((rc = launch_prolog_verts(c, s->dt, kick, kdt, thr * thr, true))) return rc;
+        } else if ((rc = launch_prolog(c, s->dt, kick, kdt, true, thr * thr, true))) {
+          return rc;
+        }
         if (timed && (rc = record_timer(c, 0, false))) return rc;
         c->trig_slot = 0;
       }

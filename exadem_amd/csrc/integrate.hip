@@ -85,6 +85,52 @@ __global__ __launch_bounds__(kStageBS) void k_prolog(int64_t n, PState s, double
   stage_out(s.core, first, cnt, lds);
 }
 
+// The prolog of polyhedra (k_prolog without the centre test) fused with compute_vertices and
+// vertex_displ_over: the world vertices of the new pose go to vtx, and the trigger (flags[0]) is raised
+// when a vertex moved by sqrt(thr2) or more from its place at the last rebuild. The same values as
+// k_prolog followed by k_poly_vertices, which reads the pose back from the record.
+template <bool KICK, bool GUARD>
+__global__ __launch_bounds__(kStageBS) void k_prolog_verts(int64_t n, PState s, double dt, double kdt_2, double thr2,
+                                                           int32_t* __restrict__ flags,
+                                                           const xdem_shape* __restrict__ shapes, d4* __restrict__ vtx) {
+  if (GUARD && flags[1]) return;
+  __shared__ uint4 lds[kStageBS * kStageRow];
+  const int64_t first = (int64_t)blockIdx.x * kStageBS;
+  const int cnt = (int)((n - first) < kStageBS ? (n - first) : kStageBS);
+  stage_in(s.core, first, cnt, lds);
+  __syncthreads();
+  const int t = threadIdx.x;
+  const int64_t i = first + t;
+  if (t < cnt) {
+    uint4* row = lds + t * kStageRow;
+    d4 xr = lds_get(row);
+    d4 vm = lds_get(row + 2);
+    d4 w = lds_get(row + 4);
+    d4 qq = lds_get(row + 6);
+    kick_prolog<KICK>(xr, vm, w, qq, s.ax[i], s.ay[i], s.az[i], s.alx[i], s.aly[i], s.alz[i], dt, kdt_2);
+    lds_put(row, xr);
+    lds_put(row + 2, vm);
+    lds_put(row + 4, w);
+    lds_put(row + 6, qq);
+    const xdem_shape& sh = shapes[s.type[i]];
+    const V3 p{xr.x, xr.y, xr.z};
+    const double h = w.w;
+    const V3 pb{s.bx[i], s.by[i], s.bz[i]};
+    const d4 qb = s.bq[i];
+    bool moved = false;
+    for (int k = 0; k < sh.nv; k++) {
+      const V3 v = get_vertex(sh, k, p, h, qq);
+      vtx[i * XDEM_MAX_SHAPE_VERTICES + k] = {v.x, v.y, v.z, 0.0};
+      const V3 a = get_vertex(sh, k, pb, h, qb);
+      const V3 d = v - a;
+      if (dot(d, d) >= thr2) moved = true;
+    }
+    if (moved) flags[0] = 1;
+  }
+  __syncthreads();
+  stage_out(s.core, first, cnt, lds);
+}
+
 __global__ __launch_bounds__(256) void k_epilog(int64_t n, PState s, double dt) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -183,6 +229,24 @@ int launch_prolog(xdem_ctx* c, double dt, bool with_kick, double kick_dt, bool d
     else LAUNCH1((k_prolog<false, false, false>), n, s, dt, kdt_2, thr2, c->d_flags);
   }
   c->cells_valid = false;  // positions moved: the binning no longer matches them
+  return XDEM_OK;
+}
+
+int launch_prolog_verts(xdem_ctx* c, double dt, bool with_kick, double kick_dt, double thr2, bool guarded) {
+  if (c->n == 0) return XDEM_OK;
+  int rc = poly_vtx_alloc(c);
+  if (rc) return rc;
+  PState s = live(c);
+  const double kdt_2 = kick_dt * 0.5;
+  const int64_t n = c->n;
+  if (with_kick) {
+    if (guarded) LAUNCH1((k_prolog_verts<true, true>), n, s, dt, kdt_2, thr2, c->d_flags, c->d_shapes, c->vtx);
+    else LAUNCH1((k_prolog_verts<true, false>), n, s, dt, kdt_2, thr2, c->d_flags, c->d_shapes, c->vtx);
+  } else {
+    if (guarded) LAUNCH1((k_prolog_verts<false, true>), n, s, dt, kdt_2, thr2, c->d_flags, c->d_shapes, c->vtx);
+    else LAUNCH1((k_prolog_verts<false, false>), n, s, dt, kdt_2, thr2, c->d_flags, c->d_shapes, c->vtx);
+  }
+  c->cells_valid = false;
   return XDEM_OK;
 }
 

@@ -118,10 +118,6 @@ __device__ inline OBBd compute_obb(const xdem_shape& sh, V3 p, const d4& q, doub
   return b;
 }
 
-__device__ inline V3 get_vertex(const xdem_shape& sh, int k, V3 p, double h, const d4& q) {
-  const V3 v{sh.vertices[k][0], sh.vertices[k][1], sh.vertices[k][2]};
-  return p + qrot4(q, h * v);
-}
 __device__ inline V3 vnormalize(V3 v) { return v / norm(v); }
 
 // Vertices of one particle read in place from the world-vertex array (plus a periodic image shift),
@@ -359,14 +355,20 @@ __global__ __launch_bounds__(256) void k_poly_vertices(int64_t n, PState s, cons
   if (moved) flags[0] = 1;
 }
 
-int poly_compute_vertices(xdem_ctx* c, bool displ, double thr2, bool guarded) {
-  if (c->n == 0) return XDEM_OK;
+int poly_vtx_alloc(xdem_ctx* c) {
   if (c->cap_vtx < c->cap) {
     if (c->vtx) (void)hipFree(c->vtx);
     c->vtx = nullptr;
     XDEM_HIP(c, hipMalloc((void**)&c->vtx, sizeof(d4) * XDEM_MAX_SHAPE_VERTICES * (size_t)c->cap));
     c->cap_vtx = c->cap;
   }
+  return XDEM_OK;
+}
+
+int poly_compute_vertices(xdem_ctx* c, bool displ, double thr2, bool guarded) {
+  if (c->n == 0) return XDEM_OK;
+  int rc = poly_vtx_alloc(c);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_poly_vertices, dim3(grid_blocks(c->n, 256)), dim3(256), 0, c->stream, c->n, live(c),
                      c->d_shapes, c->vtx, displ ? 1 : 0, thr2, c->d_flags, guarded ? 1 : 0);
   XDEM_HIP(c, hipGetLastError());

@@ -158,6 +158,12 @@ __host__ __device__ inline V3 qrot4(const d4& q, V3 v) {
   return v + uv + uuv;
 }
 
+// shape::get_vertex (src/shape/include/exaDEM/shape.hpp:244-262): p + Q (h v_k)
+__host__ __device__ inline V3 get_vertex(const xdem_shape& sh, int k, V3 p, double h, const d4& q) {
+  const V3 v{sh.vertices[k][0], sh.vertices[k][1], sh.vertices[k][2]};
+  return p + qrot4(q, h * v);
+}
+
 __host__ __device__ inline V3 shift_vec(const Grid& g, int code) {
   const int sx = code % 3 - 1, sy = (code / 3) % 3 - 1, sz = code / 9 - 1;
   return {sx * g.L[0], sy * g.L[1], sz * g.L[2]};
@@ -608,6 +614,9 @@ int engine_force_pass(xdem_ctx* c, const xdem_scheme* s);  // fused force pass a
 int engine_flush_kick(xdem_ctx* c);
 // polyhedra
 int poly_compute_vertices(xdem_ctx* c, bool displ, double thr2, bool guarded);
+int poly_vtx_alloc(xdem_ctx* c);
+// The prolog of polyhedra fused with compute_vertices and the vertex displacement test (k_prolog_verts).
+int launch_prolog_verts(xdem_ctx* c, double dt, bool with_kick, double kick_dt, double thr2, bool guarded);
 int poly_nbh(xdem_ctx* c, double rcut_inc);
 // classify_interactions of polyhedra: the feature waves by type (forder, fwave_start, fwave_size)
 int poly_classify(xdem_ctx* c);
