@@ -107,7 +107,6 @@ int ensure_cells(xdem_ctx* c, int64_t ncells) {
     if ((rc = dalloc(c, &c->cell_count, ncells + 1))) return rc;
     if ((rc = dalloc(c, &c->cell_start, ncells + 1))) return rc;
     if ((rc = dalloc(c, &c->cell_rank, ncells))) return rc;
-    if ((rc = dalloc(c, &c->cell_lin, ncells))) return rc;
     c->cap_cells = ncells;
   }
   // rank of every cell along a Morton (Z-order) curve over the grid: consecutive particles then
@@ -126,13 +125,9 @@ int ensure_cells(xdem_ctx* c, int64_t ncells) {
         key[cid] = {spread(x) | (spread(y) << 1) | (spread(z) << 2), cid};
       }
   std::sort(key.begin(), key.end());
-  std::vector<uint32_t> rank(ncells), lin(ncells);
-  for (int64_t r = 0; r < ncells; r++) {
-    rank[key[r].second] = (uint32_t)r;
-    lin[r] = key[r].second;
-  }
+  std::vector<uint32_t> rank(ncells);
+  for (int64_t r = 0; r < ncells; r++) rank[key[r].second] = (uint32_t)r;
   XDEM_HIP(c, hipMemcpyAsync(c->cell_rank, rank.data(), sizeof(uint32_t) * ncells, hipMemcpyHostToDevice, c->stream));
-  XDEM_HIP(c, hipMemcpyAsync(c->cell_lin, lin.data(), sizeof(uint32_t) * ncells, hipMemcpyHostToDevice, c->stream));
   XDEM_HIP(c, hipStreamSynchronize(c->stream));
   return XDEM_OK;
 }
@@ -419,7 +414,7 @@ void xdem_ctx_destroy(xdem_ctx* c) {
   mesh_free(c);
   dfree(&c->core_alt);
   dfree(&c->cell_of); dfree(&c->cell_count); dfree(&c->cell_start); dfree(&c->cell_rank); dfree(&c->perm);
-  dfree(&c->cell_lin); dfree(&c->d_rmax); dfree(&c->nbh_tmp); dfree(&c->mxr); dfree(&c->mid); dfree(&c->blk_u32);
+  dfree(&c->d_rmax); dfree(&c->nbh_tmp); dfree(&c->mxr); dfree(&c->mid); dfree(&c->blk_u32);
   dfree(&c->perm_tmp);
   dfree(&c->sort_key);
   dfree(&c->d_flags); dfree(&c->d_u32tmp); dfree(&c->d_pairs); dfree(&c->d_walls);

@@ -183,7 +183,6 @@ struct CellNbhArgs {
   double rcut2;
   const uint32_t* cell_start;  // indexed by Morton rank
   const uint32_t* rank;        // linear cell -> Morton rank
-  const uint32_t* lin;         // Morton rank -> linear cell
   const d4* xr;                // positions and radii, Morton order
   const uint64_t* id;          // ids, Morton order
   int64_t ncells;
@@ -541,7 +540,7 @@ int do_chunk_neighbors(xdem_ctx* c, double rcut_inc, double nbh_dist) {
     c->nbh_tmp_cap = rows;
     c->nbh_tmp_n = c->cap;
   }
-  CellNbhArgs a{g, rcut_inc, nbh_dist * nbh_dist, c->cell_start, c->cell_rank, c->cell_lin, c->mxr, c->mid,
+  CellNbhArgs a{g, rcut_inc, nbh_dist * nbh_dist, c->cell_start, c->cell_rank, c->mxr, c->mid,
                 g.ncells, c->nshapes > 0 ? 1 : 0, c->nbh_tmp, c->nbh_tmp_cap, cnt_m};
   const int64_t nseg = (int64_t)((g.dims[0] + kRowCells - 1) / kRowCells) * g.dims[1] * g.dims[2];
   const dim3 cgrid((uint32_t)((nseg + 3) / 4));

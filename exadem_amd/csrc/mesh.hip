@@ -418,7 +418,7 @@ int mesh_carry(xdem_ctx* c, double rcut_inc, bool have_old, bool poly) {
   const int64_t n = c->n;
   // reach of the bins
   unsigned long long* d_rmax = (unsigned long long*)c->d_mesh_over;
-  XDEM_HIP(c, hipMemsetAsync(c->d_mesh_over, 0, 16 * sizeof(int32_t), c->stream));
+  XDEM_HIP(c, hipMemsetAsync(c->d_mesh_over, 0, 8 * sizeof(int32_t), c->stream));
   hipLaunchKernelGGL(k_max_radius, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream, n, live(c).core, d_rmax);
   XDEM_HIP(c, hipGetLastError());
   unsigned long long rb = 0;
@@ -518,7 +518,7 @@ extern "C" int xdem_set_mesh(xdem_ctx* c, int32_t id, const double* vertices, in
   if ((rc = dev_realloc(c, &c->d_mesh_v, nv))) return rc;
   if ((rc = dev_realloc(c, &c->d_mesh_e, std::max<int64_t>(ne, 1)))) return rc;
   if ((rc = dev_realloc(c, &c->d_mesh_f, std::max<int64_t>(nf, 1)))) return rc;
-  if ((rc = dev_realloc(c, &c->d_mesh_over, 16))) return rc;
+  if ((rc = dev_realloc(c, &c->d_mesh_over, 8))) return rc;
   XDEM_HIP(c, hipMemcpy(c->d_mesh_v, c->mesh_hv.data(), sizeof(V3) * nv, hipMemcpyHostToDevice));
   if (ne) XDEM_HIP(c, hipMemcpy(c->d_mesh_e, c->mesh_he.data(), sizeof(uint2) * ne, hipMemcpyHostToDevice));
   if (nf) XDEM_HIP(c, hipMemcpy(c->d_mesh_f, c->mesh_hf.data(), sizeof(uint4) * nf, hipMemcpyHostToDevice));

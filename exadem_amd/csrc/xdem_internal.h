@@ -387,7 +387,6 @@ struct xdem_ctx {
   uint32_t* cell_count = nullptr;   // [ncells]
   uint32_t* cell_start = nullptr;   // [ncells+1], indexed by Morton rank of the cell
   uint32_t* cell_rank = nullptr;    // [ncells] rank of each cell along the Morton curve
-  uint32_t* cell_lin = nullptr;     // [ncells] linear cell index of each Morton rank
   uint64_t* d_rmax = nullptr;       // largest radius as IEEE bits, reduced by the binning pass
   xdem::d4* mxr = nullptr;          // [cap] positions and radii in Morton order (binning -> neighbour build)
   uint64_t* mid = nullptr;          // [cap] ids in Morton order
