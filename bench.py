@@ -359,6 +359,56 @@ def main():
                                                                   "floor and lid, dt 2e-6")}
         log(f"polyhedra: {args.poly_steps} steps of {wp.n} in {el:.3f} s (force {pf_ms * 1e3:.1f} us, "
             f"rebuild {np.median(reb):.2f} ms)")
+        # The relaxed bed has no contact within the configured 1000 steps (gravity moves a body by ~2e-5 in
+        # 2 ms), so the contact law is measured on the same particles shaken into contact: velocities
+        # N(0, 2) per component, 50 steps of dt 4e-4 (rebuilds included), then the configured dt again.
+        import copy
+        p = ps.ctx.download()
+        del ps
+        vsh = inputs.normal(77, 2, 3 * wp.n, 0.0, 2.0).reshape(3, wp.n)
+        order = np.argsort(p["id"])
+        for a_, k_ in enumerate(("vx", "vy", "vz")):
+            p[k_] = np.empty(wp.n)
+            p[k_][order] = vsh[a_]
+        wsh = copy.copy(wp)
+        wsh.particles = p
+        wsh.dt = 4e-4
+        ss = PolySim(wsh)
+        ss.first_iteration()
+        ss.run(50, poll=args.poll)
+        wsh.dt = wp.dt
+        ss.run(20, poll=args.poll)
+        ss.ctx.synchronize()
+        ss.ctx.set_timing(True, every=args.timing_every)
+        ss.ctx.kernel_timing(1, reset=True)
+        t1 = time.perf_counter()
+        sst_ = ss.run(args.poly_steps, poll=args.poll)
+        ss.ctx.synchronize()
+        el_s = time.perf_counter() - t1
+        ss.ctx.set_timing(False)
+        sf_ms, _ = ss.ctx.kernel_timing(1)
+        swaves = ss.ctx.classifier_waves()
+        snfeat = int(np.sum(swaves[:4]))
+        snb = ss.ctx.nbh_stats()
+        _, _, stt, _, _, _, sfr = ss.ctx.poly_interactions()
+        sact = np.any(sfr != 0, axis=1)
+        scont = int(np.count_nonzero(sact & (stt < 4)))
+        sbytes = bytes_model_poly(wp.n, 2 * snb["pairs"], snfeat, scont, nvert)
+        sach = sbytes["force_pass"] / (sf_ms * 1e-3) / 1e9 if sf_ms > 0 else 0.0
+        forms["polyhedra"]["with_contacts"] = {
+            "value": wp.n * args.poly_steps / el_s, "unit": "particle-steps/s",
+            "ms_per_step": 1e3 * el_s / args.poly_steps, "force_avg_ms": sf_ms, "rebuilds": sst_["rebuilds"],
+            "feature_interactions": snfeat,
+            "features_in_contact": {n_: int(np.count_nonzero(sact & (stt == t_)))
+                                    for t_, n_ in enumerate(("VV", "VE", "VF", "EE"))},
+            "vertex_wall_in_contact": int(np.count_nonzero(sact & (stt == 5))),
+            "roofline": {"bound": "hbm", "achieved": sach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": sach / PEAK_HBM_GBS, "bytes_per_launch": sbytes["force_pass"]},
+            "workload": "the polyhedra input above with velocities N(0, 2) per component, 50 steps of dt 4e-4 "
+                        f"to bring contacts, then {args.poly_steps} timed steps of dt {wp.dt:g}"}
+        log(f"polyhedra with contacts: {scont} features in contact, {args.poly_steps} steps in {el_s:.3f} s "
+            f"(force {sf_ms * 1e3:.1f} us, {sst_['rebuilds']} rebuilds)")
+        del ss
         if not args.no_cpu:
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
             import pyoracle as po
@@ -381,7 +431,6 @@ def main():
                           f"{wp.n}-polyhedron input: {steps_p} steps after first_iteration, {tcp:.1f} s"}
             log(f"polyhedra cpu baseline: {wp.n * steps_p / tcp:.3g} particle-steps/s")
             del op
-        del ps
     if args.form == "fp64" and not args.no_forms and want("mesh"):
         # STL mesh driver: a stationary triangulated plane through the box at mid-height (grid spacing
         # about 2, two triangles per cell, Minkowski radius 0.01) in the same sphere input, FP64 form
