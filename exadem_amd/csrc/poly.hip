@@ -1453,8 +1453,8 @@ __global__ __launch_bounds__(256) void k_poly_gather(PolyForceArgs a) {
   const int64_t x = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (x >= a.n) return;
   const PState& s = a.s;
-  const Core cx = s.core[x];
-  const double mx = cx.vm.w;
+  // only the mass is held across the list walk; the rest of the record is read after it
+  const double mx = s.core[x].vm.w;
   V3 F, M;
   if (FUSED) {
     F = {0.0, 0.0, 0.0};
@@ -1496,16 +1496,33 @@ __global__ __launch_bounds__(256) void k_poly_gather(PolyForceArgs a) {
       for (int q = 0; q < 4; q++) v[q] = w0 + q < wend ? cw[w0 + q] : 0u;
 #pragma unroll
       for (int q = 0; q < 4; q++) {
+        // the word's features in contact; for each half word in which any lane of the wavefront has one,
+        // the two features' terms are loaded together (a position without a hit reads row 0, which exists
+        // whenever some feature is in contact, and discards it), then added in feature order
+        uint32_t hits = 0;
+#pragma unroll
         for (int byte = 0; byte < 4; byte++) {
           const uint64_t f = ((w0 + q) << 2) + byte;
-          if (((v[q] >> (8 * byte)) & 0xffu) == 0u || f < f0 || f >= f1) continue;
-          const double* pr = a.contrib + 9 * f;
-          if (own) {
-            Fp = Fp + V3{pr[0], pr[1], pr[2]};
-            Mp = Mp + V3{pr[3], pr[4], pr[5]};
-          } else {
-            Fp = Fp + V3{-pr[0], -pr[1], -pr[2]};
-            Mp = Mp + V3{pr[6], pr[7], pr[8]};
+          if (((v[q] >> (8 * byte)) & 0xffu) != 0u && f >= f0 && f < f1) hits |= 1u << byte;
+        }
+#pragma unroll
+        for (int h = 0; h < 4; h += 2) {
+          const uint32_t hh = (hits >> h) & 3u;
+          if (__builtin_amdgcn_ballot_w64(hh != 0u) == 0ull) continue;
+          V3 tf[2], tm[2];
+#pragma unroll
+          for (int e = 0; e < 2; e++) {
+            const uint64_t f = ((hh >> e) & 1u) ? ((w0 + q) << 2) + h + e : 0u;
+            const double* pr = a.contrib + 9 * f;
+            tf[e] = V3{pr[0], pr[1], pr[2]};
+            tm[e] = own ? V3{pr[3], pr[4], pr[5]} : V3{pr[6], pr[7], pr[8]};
+          }
+#pragma unroll
+          for (int e = 0; e < 2; e++) {
+            if (!((hh >> e) & 1u)) continue;
+            if (own) Fp = Fp + tf[e];
+            else Fp = Fp + V3{-tf[e].x, -tf[e].y, -tf[e].z};
+            Mp = Mp + tm[e];
           }
         }
       }
@@ -1513,6 +1530,7 @@ __global__ __launch_bounds__(256) void k_poly_gather(PolyForceArgs a) {
     F = F + Fp;
     M = M + Mp;
   }
+  const Core cx = s.core[x];
   if (a.nw > 0) {
     const xdem_shape& sh = a.shapes[s.type[x]];
     const double r = sh.minkowski * cx.w.w;
