@@ -110,6 +110,9 @@ def main():
     ap.add_argument("--poly-n", type=int, default=200_000)
     ap.add_argument("--poly-steps", type=int, default=1000)
     ap.add_argument("--no-trajectory", action="store_true", help="skip the surrogate's accumulated-error runs")
+    ap.add_argument("--poly-contacts", choices=["yes", "no", "only"], default="yes",
+                    help="polyhedra: also time the same particles shaken into contact (yes), skip that (no), or "
+                         "time only that (only; the relaxed line then carries no timing), e.g. for separate profiles")
     ap.add_argument("--poly-cache", default=None, help="npz file for the grown polyhedra input (read if it "
                                                          "holds the same input, else written)")
     ap.add_argument("--only", default="", help="comma-separated forms to run after the headline line "
@@ -310,106 +313,115 @@ def main():
         ps = PolySim(wp)
         ps.first_iteration()
         ps.run(20, poll=args.poll)
-        ps.ctx.synchronize()
-        ps.ctx.set_timing(True, every=args.timing_every)
-        ps.ctx.kernel_timing(0, reset=True)
-        ps.ctx.kernel_timing(1, reset=True)
-        t1 = time.perf_counter()
-        pst = ps.run(args.poly_steps, poll=args.poll)
-        ps.ctx.synchronize()
-        el = time.perf_counter() - t1
-        ps.ctx.set_timing(False)
-        pf_ms, _ = ps.ctx.kernel_timing(1)
-        pi_ms, _ = ps.ctx.kernel_timing(0)
-        waves = ps.ctx.classifier_waves()
-        nfeat = int(np.sum(waves[:4]))
-        pnb = ps.ctx.nbh_stats()
         nvert = float(np.mean([wp.shapes[t].nv for t in wp.particles["type"]]))
-        _, _, ptt, _, _, _, pfr = ps.ctx.poly_interactions()
-        contacts = int(np.count_nonzero(np.any(pfr[ptt < 4] != 0, axis=1)))  # features with friction
-        pbytes = bytes_model_poly(wp.n, 2 * pnb["pairs"], nfeat, contacts, nvert)  # both directions
-        pach = pbytes["force_pass"] / (pf_ms * 1e-3) / 1e9 if pf_ms > 0 else 0.0
-        reb = []
-        for _ in range(3):
+        if args.poly_contacts != "only":
             ps.ctx.synchronize()
-            tr = time.perf_counter()
-            ps.rebuild()
+            ps.ctx.set_timing(True, every=args.timing_every)
+            ps.ctx.kernel_timing(0, reset=True)
+            ps.ctx.kernel_timing(1, reset=True)
+            t1 = time.perf_counter()
+            pst = ps.run(args.poly_steps, poll=args.poll)
             ps.ctx.synchronize()
-            reb.append(1e3 * (time.perf_counter() - tr))
-        forms["polyhedra"] = {"value": wp.n * args.poly_steps / el, "unit": "particle-steps/s",
-                              "n": wp.n, "steps": args.poly_steps, "ms_per_step": 1e3 * el / args.poly_steps,
-                              "force_avg_ms": pf_ms, "integrate_vertices_avg_ms": pi_ms,
-                              "feature_interactions": nfeat, "features_in_contact": contacts,
-                              "classifier_waves": {"VV": int(waves[0]), "VE": int(waves[1]), "VF": int(waves[2]),
-                                                   "EE": int(waves[3]), "vertex_wall": int(waves[5])},
-                              "rebuilds": pst["rebuilds"], "rebuild_ms": float(np.median(reb)),
-                              "solid_fraction": wp.meta["solid_fraction"],
-                              "bed_solid_fraction": wp.meta["bed_solid_fraction"],
-                              "generation_s": gen_s,
-                              "roofline": {"bound": "hbm", "achieved": pach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                                           "frac": pach / PEAK_HBM_GBS,
-                                           "traffic": profiled_traffic("poly_force_200k_polyhedra")[0],
-                                           "traffic_source": profiled_traffic("poly_force_200k_polyhedra")[1],
-                                           "kernel": "force pass: k_poly_feat + k_poly_gather",
-                                           "bytes_per_launch": pbytes["force_pass"], "avg_ms": pf_ms,
-                                           "step_bytes": pbytes["step"],
-                                           "step_frac": pbytes["step"] / (el / args.poly_steps) / 1e9 / PEAK_HBM_GBS},
-                              "workload": wp.meta.get("workload", "cubes + octahedra 50/50, random orientations, "
-                                                                  "jittered lattice spacing 1.55, periodic x/y, "
-                                                                  "floor and lid, dt 2e-6")}
-        log(f"polyhedra: {args.poly_steps} steps of {wp.n} in {el:.3f} s (force {pf_ms * 1e3:.1f} us, "
-            f"rebuild {np.median(reb):.2f} ms)")
-        # The relaxed bed has no contact within the configured 1000 steps (gravity moves a body by ~2e-5 in
-        # 2 ms), so the contact law is measured on the same particles shaken into contact: velocities
-        # N(0, 2) per component, 50 steps of dt 4e-4 (rebuilds included), then the configured dt again.
-        import copy
-        p = ps.ctx.download()
-        del ps
-        vsh = inputs.normal(77, 2, 3 * wp.n, 0.0, 2.0).reshape(3, wp.n)
-        order = np.argsort(p["id"])
-        for a_, k_ in enumerate(("vx", "vy", "vz")):
-            p[k_] = np.empty(wp.n)
-            p[k_][order] = vsh[a_]
-        wsh = copy.copy(wp)
-        wsh.particles = p
-        wsh.dt = 4e-4
-        ss = PolySim(wsh)
-        ss.first_iteration()
-        ss.run(50, poll=args.poll)
-        wsh.dt = wp.dt
-        ss.run(20, poll=args.poll)
-        ss.ctx.synchronize()
-        ss.ctx.set_timing(True, every=args.timing_every)
-        ss.ctx.kernel_timing(1, reset=True)
-        t1 = time.perf_counter()
-        sst_ = ss.run(args.poly_steps, poll=args.poll)
-        ss.ctx.synchronize()
-        el_s = time.perf_counter() - t1
-        ss.ctx.set_timing(False)
-        sf_ms, _ = ss.ctx.kernel_timing(1)
-        swaves = ss.ctx.classifier_waves()
-        snfeat = int(np.sum(swaves[:4]))
-        snb = ss.ctx.nbh_stats()
-        _, _, stt, _, _, _, sfr = ss.ctx.poly_interactions()
-        sact = np.any(sfr != 0, axis=1)
-        scont = int(np.count_nonzero(sact & (stt < 4)))
-        sbytes = bytes_model_poly(wp.n, 2 * snb["pairs"], snfeat, scont, nvert)
-        sach = sbytes["force_pass"] / (sf_ms * 1e-3) / 1e9 if sf_ms > 0 else 0.0
-        forms["polyhedra"]["with_contacts"] = {
-            "value": wp.n * args.poly_steps / el_s, "unit": "particle-steps/s",
-            "ms_per_step": 1e3 * el_s / args.poly_steps, "force_avg_ms": sf_ms, "rebuilds": sst_["rebuilds"],
-            "feature_interactions": snfeat,
-            "features_in_contact": {n_: int(np.count_nonzero(sact & (stt == t_)))
-                                    for t_, n_ in enumerate(("VV", "VE", "VF", "EE"))},
-            "vertex_wall_in_contact": int(np.count_nonzero(sact & (stt == 5))),
-            "roofline": {"bound": "hbm", "achieved": sach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                         "frac": sach / PEAK_HBM_GBS, "bytes_per_launch": sbytes["force_pass"]},
-            "workload": "the polyhedra input above with velocities N(0, 2) per component, 50 steps of dt 4e-4 "
-                        f"to bring contacts, then {args.poly_steps} timed steps of dt {wp.dt:g}"}
-        log(f"polyhedra with contacts: {scont} features in contact, {args.poly_steps} steps in {el_s:.3f} s "
-            f"(force {sf_ms * 1e3:.1f} us, {sst_['rebuilds']} rebuilds)")
-        del ss
-        if not args.no_cpu:
+            el = time.perf_counter() - t1
+            ps.ctx.set_timing(False)
+            pf_ms, _ = ps.ctx.kernel_timing(1)
+            pi_ms, _ = ps.ctx.kernel_timing(0)
+            waves = ps.ctx.classifier_waves()
+            nfeat = int(np.sum(waves[:4]))
+            pnb = ps.ctx.nbh_stats()
+            _, _, ptt, _, _, _, pfr = ps.ctx.poly_interactions()
+            contacts = int(np.count_nonzero(np.any(pfr[ptt < 4] != 0, axis=1)))  # features with friction
+            pbytes = bytes_model_poly(wp.n, 2 * pnb["pairs"], nfeat, contacts, nvert)  # both directions
+            pach = pbytes["force_pass"] / (pf_ms * 1e-3) / 1e9 if pf_ms > 0 else 0.0
+            reb = []
+            for _ in range(3):
+                ps.ctx.synchronize()
+                tr = time.perf_counter()
+                ps.rebuild()
+                ps.ctx.synchronize()
+                reb.append(1e3 * (time.perf_counter() - tr))
+            forms["polyhedra"] = {"value": wp.n * args.poly_steps / el, "unit": "particle-steps/s",
+                                  "n": wp.n, "steps": args.poly_steps, "ms_per_step": 1e3 * el / args.poly_steps,
+                                  "force_avg_ms": pf_ms, "integrate_vertices_avg_ms": pi_ms,
+                                  "feature_interactions": nfeat, "features_in_contact": contacts,
+                                  "classifier_waves": {"VV": int(waves[0]), "VE": int(waves[1]), "VF": int(waves[2]),
+                                                       "EE": int(waves[3]), "vertex_wall": int(waves[5])},
+                                  "rebuilds": pst["rebuilds"], "rebuild_ms": float(np.median(reb)),
+                                  "solid_fraction": wp.meta["solid_fraction"],
+                                  "bed_solid_fraction": wp.meta["bed_solid_fraction"],
+                                  "generation_s": gen_s,
+                                  "roofline": {"bound": "hbm", "achieved": pach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                               "frac": pach / PEAK_HBM_GBS,
+                                               "traffic": profiled_traffic("poly_force_200k_polyhedra")[0],
+                                               "traffic_source": profiled_traffic("poly_force_200k_polyhedra")[1],
+                                               "kernel": "force pass: k_poly_feat + k_poly_gather",
+                                               "bytes_per_launch": pbytes["force_pass"], "avg_ms": pf_ms,
+                                               "step_bytes": pbytes["step"],
+                                               "step_frac": pbytes["step"] / (el / args.poly_steps) / 1e9 / PEAK_HBM_GBS},
+                                  "workload": wp.meta.get("workload", "cubes + octahedra 50/50, random orientations, "
+                                                                      "jittered lattice spacing 1.55, periodic x/y, "
+                                                                      "floor and lid, dt 2e-6")}
+            log(f"polyhedra: {args.poly_steps} steps of {wp.n} in {el:.3f} s (force {pf_ms * 1e3:.1f} us, "
+                f"rebuild {np.median(reb):.2f} ms)")
+        else:
+            forms["polyhedra"] = {"n": wp.n, "solid_fraction": wp.meta["solid_fraction"]}
+        if args.poly_contacts != "no":
+            # The relaxed bed has no contact within the configured 1000 steps (gravity moves a body by ~2e-5 in
+            # 2 ms), so the contact law is measured on the same particles shaken into contact: velocities
+            # N(0, 2) per component, 50 steps of dt 4e-4 (rebuilds included), then the configured dt again.
+            import copy
+            p = ps.ctx.download()
+            del ps
+            vsh = inputs.normal(77, 2, 3 * wp.n, 0.0, 2.0).reshape(3, wp.n)
+            order = np.argsort(p["id"])
+            for a_, k_ in enumerate(("vx", "vy", "vz")):
+                p[k_] = np.empty(wp.n)
+                p[k_][order] = vsh[a_]
+            wsh = copy.copy(wp)
+            wsh.particles = p
+            wsh.dt = 4e-4
+            ss = PolySim(wsh)
+            ss.first_iteration()
+            ss.run(50, poll=args.poll)
+            wsh.dt = wp.dt
+            ss.run(20, poll=args.poll)
+            ss.ctx.synchronize()
+            ss.ctx.set_timing(True, every=args.timing_every)
+            ss.ctx.kernel_timing(1, reset=True)
+            t1 = time.perf_counter()
+            sst_ = ss.run(args.poly_steps, poll=args.poll)
+            ss.ctx.synchronize()
+            el_s = time.perf_counter() - t1
+            ss.ctx.set_timing(False)
+            sf_ms, _ = ss.ctx.kernel_timing(1)
+            swaves = ss.ctx.classifier_waves()
+            snfeat = int(np.sum(swaves[:4]))
+            snb = ss.ctx.nbh_stats()
+            _, _, stt, _, _, _, sfr = ss.ctx.poly_interactions()
+            sact = np.any(sfr != 0, axis=1)
+            scont = int(np.count_nonzero(sact & (stt < 4)))
+            sbytes = bytes_model_poly(wp.n, 2 * snb["pairs"], snfeat, scont, nvert)
+            sach = sbytes["force_pass"] / (sf_ms * 1e-3) / 1e9 if sf_ms > 0 else 0.0
+            forms["polyhedra"]["with_contacts"] = {
+                "value": wp.n * args.poly_steps / el_s, "unit": "particle-steps/s",
+                "ms_per_step": 1e3 * el_s / args.poly_steps, "force_avg_ms": sf_ms, "rebuilds": sst_["rebuilds"],
+                "feature_interactions": snfeat,
+                "features_in_contact": {n_: int(np.count_nonzero(sact & (stt == t_)))
+                                        for t_, n_ in enumerate(("VV", "VE", "VF", "EE"))},
+                "vertex_wall_in_contact": int(np.count_nonzero(sact & (stt == 5))),
+                "roofline": {"bound": "hbm", "achieved": sach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                             "frac": sach / PEAK_HBM_GBS, "bytes_per_launch": sbytes["force_pass"],
+                             "traffic": profiled_traffic("poly_force_200k_polyhedra_contacts")[0],
+                             "traffic_source": profiled_traffic("poly_force_200k_polyhedra_contacts")[1],
+                             "kernel": "force pass: k_poly_feat + k_poly_gather", "avg_ms": sf_ms},
+                "workload": "the polyhedra input above with velocities N(0, 2) per component, 50 steps of dt 4e-4 "
+                            f"to bring contacts, then {args.poly_steps} timed steps of dt {wp.dt:g}"}
+            log(f"polyhedra with contacts: {scont} features in contact, {args.poly_steps} steps in {el_s:.3f} s "
+                f"(force {sf_ms * 1e3:.1f} us, {sst_['rebuilds']} rebuilds)")
+            del ss
+        else:
+            del ps
+        if not args.no_cpu and args.poly_contacts != "only":
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
             import pyoracle as po
             op = po.Oracle(wp.particles, wp.domain, wp.walls)

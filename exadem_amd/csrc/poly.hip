@@ -1450,11 +1450,16 @@ __global__ __launch_bounds__(256) void k_poly_gather(PolyForceArgs a) {
   // the stall decision was taken by k_poly_feat of the same step
   if (GUARD && a.flags[1]) return;
   // XCD-aware block order (partner contributions are read through the owner slots' feature ranges)
-  const int64_t x = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int64_t b0 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x;
+  // the block's 256 records, staged in LDS with coalesced 16-byte copies: the mass is read from there
+  // before the list walk and the rest of the record after it, without holding it in registers
+  __shared__ Core tile[256];
+  stage_tile(tile, a.s.core, b0, a.n);
+  __syncthreads();
+  const int64_t x = b0 + threadIdx.x;
   if (x >= a.n) return;
   const PState& s = a.s;
-  // only the mass is held across the list walk; the rest of the record is read after it
-  const double mx = s.core[x].vm.w;
+  const double mx = tile[threadIdx.x].vm.w;
   V3 F, M;
   if (FUSED) {
     F = {0.0, 0.0, 0.0};
@@ -1530,7 +1535,8 @@ __global__ __launch_bounds__(256) void k_poly_gather(PolyForceArgs a) {
     F = F + Fp;
     M = M + Mp;
   }
-  const Core cx = s.core[x];
+  // the record's fields are read from the tile where they are used below
+  const Core& cx = tile[threadIdx.x];
   if (a.nw > 0) {
     const xdem_shape& sh = a.shapes[s.type[x]];
     const double r = sh.minkowski * cx.w.w;
